@@ -1,0 +1,219 @@
+/*
+ * geon.h — C ABI of the gfx950 physics backend for Geon (a9lim/physsim).
+ *
+ * One handle owns one particle system resident in one GPU's HBM: SoA particle
+ * state, the Barnes-Hut quadtree, force accumulators and the scalar-field
+ * grids.  Every entry point is plain C: pointers, sizes, int return codes.
+ * No exception, no torch type and no HIP type crosses this boundary.
+ *
+ * The entry points replace the de-facto GPU backend contract the reference app
+ * calls on `GPUPhysics` (src/gpu/gpu-physics.js) and the CPU `Physics.update`
+ * call site (main.js:697); each declaration cites the method it stands in for.
+ *
+ * Return codes: 0 = ok, negative = error (GEON_E*).  An entry point that fails
+ * writes nothing to its outputs.  A handle is not thread-safe; each handle owns
+ * one HIP stream.
+ */
+#ifndef GEON_H
+#define GEON_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GEON_ABI_VERSION 1
+
+/* ---- return codes ---- */
+#define GEON_OK           0
+#define GEON_EINVAL      -1   /* bad argument */
+#define GEON_EFULL       -2   /* no free particle slot (gpu-physics.js:1864 returns -1) */
+#define GEON_EDEVICE     -3   /* HIP runtime error or no device */
+#define GEON_ENOMEM      -4   /* device allocation failed */
+#define GEON_ESTATE      -5   /* call not valid in the current state */
+#define GEON_EUNSUPPORTED -6  /* toggle/mode combination this backend does not run */
+#define GEON_EOVERFLOW   -7   /* tree node pool or stack capacity exceeded */
+
+/* ---- toggles0 bits: same values as src/gpu/gpu-constants.js:196-213 ---- */
+#define GEON_GRAVITY        (1u << 0)
+#define GEON_COULOMB        (1u << 1)
+#define GEON_MAGNETIC       (1u << 2)
+#define GEON_GRAVITOMAG     (1u << 3)
+#define GEON_ONE_PN         (1u << 4)
+#define GEON_RELATIVITY     (1u << 5)
+#define GEON_SPIN_ORBIT     (1u << 6)
+#define GEON_RADIATION      (1u << 7)   /* not run by this backend: rejected */
+#define GEON_BLACK_HOLE     (1u << 8)
+#define GEON_DISINTEGRATION (1u << 9)   /* not run by this backend: rejected */
+#define GEON_EXPANSION      (1u << 10)
+#define GEON_YUKAWA         (1u << 11)
+#define GEON_HIGGS          (1u << 12)
+#define GEON_AXION          (1u << 13)
+#define GEON_BARNES_HUT     (1u << 14)
+#define GEON_BOSON_INTER    (1u << 15)  /* not run by this backend: rejected */
+
+/* ---- mode enums: src/config.js:10-26 ---- */
+#define GEON_COL_PASS   0
+#define GEON_COL_MERGE  1
+#define GEON_COL_BOUNCE 2
+#define GEON_BOUND_DESPAWN 0
+#define GEON_BOUND_BOUNCE  1
+#define GEON_BOUND_LOOP    2
+#define GEON_TORUS 0
+#define GEON_KLEIN 1
+#define GEON_RP2   2
+
+/* Periodic-image semantics (the reference's BH walk counts a source once per
+ * inserted ghost image, src/integrator.js:247-258 with forces.js:633-635).
+ * MIN_IMAGE: no ghosts, every displacement minimum-imaged, each source counted
+ * once (what the reference's own direct sum does, forces.js:93-126). */
+#define GEON_PERIODIC_MIN_IMAGE 0
+#define GEON_PERIODIC_GHOSTS    1
+
+typedef struct GeonContext* GeonHandle;
+
+/* Every constant the reference hard-codes (src/config.js) as a field, so that
+ * configurations that need other values can set them.  geon_default_params
+ * fills the reference defaults. */
+typedef struct GeonParams {
+    uint32_t toggles0;          /* GEON_GRAVITY | ... */
+    int32_t  boundary_mode;     /* GEON_BOUND_* */
+    int32_t  topology;          /* GEON_TORUS only on this backend */
+    int32_t  collision_mode;    /* GEON_COL_PASS only on this backend */
+    int32_t  periodic_mode;     /* GEON_PERIODIC_* */
+    int32_t  max_substeps;      /* MAX_SUBSTEPS = 32 (config.js:36) */
+    double   domain_w, domain_h;/* width/height passed to Physics.update */
+    double   softening;         /* SOFTENING = 8 (config.js:44) */
+    double   bh_softening;      /* BH_SOFTENING = 4 (config.js:46) */
+    double   theta;             /* BH_THETA = 0.5 (config.js:57) */
+    double   yukawa_coupling;   /* YUKAWA_COUPLING = 14 (config.js:78) */
+    double   yukawa_mu;         /* DEFAULT_PION_MASS = 0.15 (config.js:79) */
+    double   higgs_mass;        /* DEFAULT_HIGGS_MASS = 0.5 (config.js:88) */
+    double   axion_mass;        /* DEFAULT_AXION_MASS = 0.05 (config.js:95) */
+    double   hubble;            /* DEFAULT_HUBBLE = 0.001 (config.js:131) */
+    double   ext_gravity, ext_gravity_angle;   /* integrator.js:83-87 */
+    double   ext_electric, ext_electric_angle;
+    double   ext_bz;
+    int32_t  field_grid;        /* scalar-field grid side: 64 CPU, 128 GPU (config.js:84,125) */
+    int32_t  kahan;             /* 1: compensated FP32 force accumulation */
+} GeonParams;
+
+/* addParticle argument (gpu-physics.js:1857): vx/vy are proper velocity w. */
+typedef struct GeonParticleInit {
+    double x, y, wx, wy, mass, charge, angw;
+    int32_t antimatter;
+} GeonParticleInit;
+
+/* Caller-owned host arrays, one entry per slot.  Any pointer may be NULL on
+ * download (that field is skipped); on upload x,y,wx,wy,mass,charge must be
+ * given, the rest default (base_mass = mass, angw = 0). */
+typedef struct GeonSoAView {
+    double *x, *y, *wx, *wy, *mass, *base_mass, *charge, *angw;
+    int32_t *alive;             /* download only: 1 alive, 0 despawned/removed */
+} GeonSoAView;
+
+/* Per-particle results of the last force pass, slot order (any may be NULL).
+ * Totals and per-type display vectors as on `Particle` (particle.js:24-36). */
+typedef struct GeonForceView {
+    double *fx, *fy;
+    double *bz, *dbzdx, *dbzdy, *bgz, *dbgzdx, *dbgzdy;
+    double *frame_drag_torque, *tidal_torque;
+    double *pe;                 /* per-particle PE contribution (sum over sources) */
+    double *f1pn_x, *f1pn_y;    /* force1PN after the last 1PN pass */
+} GeonForceView;
+
+/* FP64 conservation readout: same fields as GPUPhysics.readStats
+ * (gpu-physics.js:4146-4281) / computeEnergies (energy.js:11-195). */
+typedef struct GeonStats {
+    double linear_ke, spin_ke, px, py, orbital_ang_mom, spin_ang_mom;
+    double com_x, com_y, total_mass, pe;
+    double field_energy, higgs_field_energy, axion_field_energy, pfi_energy;
+    double field_px, field_py;
+    double sim_time;
+    int32_t alive_count;
+} GeonStats;
+
+/* Work counters of the last force pass, for roofline accounting. */
+typedef struct GeonCounters {
+    int64_t pair_interactions;  /* particle-source (leaf point / direct) pairs */
+    int64_t node_interactions;  /* particle-aggregate (accepted node) pairs */
+    int64_t nodes_visited;
+    int64_t tree_nodes;         /* nodes in the last built tree */
+    int64_t tree_divided;       /* divided (internal) nodes */
+    int64_t depth_overflow;     /* leaves at the key's deepest level holding > 4 points */
+    int64_t substeps_total;     /* substeps run since create/reset */
+} GeonCounters;
+
+/* Canonical tree export: one record per node, any order.  `path` packs the
+ * quadrant digits root->node (2 bits each, NW=0 NE=1 SW=2 SE=3, first digit
+ * most significant), `level` is the depth (root = 0). */
+typedef struct GeonTreeNode {
+    uint64_t path;
+    int32_t  level;
+    int32_t  divided;           /* 1 internal, 0 leaf */
+    int32_t  count;             /* points in the subtree */
+    int32_t  charge_quanta;     /* sum of round(q/0.1) over the subtree (exact) */
+    double   mass, com_x, com_y;/* FP64 aggregates (quadtree.js:239-312) */
+} GeonTreeNode;
+
+int  geon_abi_version(void);
+
+/* Reference defaults of every constant (src/config.js). */
+void geon_default_params(GeonParams* p);
+
+/* GPUPhysics constructor + init() (gpu-physics.js, main.js:31-63). */
+int  geon_create(const GeonParams* params, int32_t max_particles, GeonHandle* out);
+/* GPUPhysics.destroy() (gpu-physics.js:4291). */
+void geon_destroy(GeonHandle h);
+/* GPUPhysics.reset() (gpu-physics.js:3973): drop all particles, sim time 0. */
+int  geon_reset(GeonHandle h);
+/* setToggles(physics) + the writable boundaryMode/topologyMode/domainW/H
+ * fields (gpu-physics.js:1969-2056). */
+int  geon_set_params(GeonHandle h, const GeonParams* params);
+int  geon_get_params(GeonHandle h, GeonParams* out);
+
+/* addParticle({x,y,vx,vy,mass,charge,angw,antimatter}) -> slot | GEON_EFULL
+ * (gpu-physics.js:1857-1926).  Charge is quantized to multiples of 0.1 (1874). */
+int32_t geon_add_particle(GeonHandle h, const GeonParticleInit* p);
+/* removeParticle(idx) (gpu-physics.js:1936-1946). */
+int  geon_remove_particle(GeonHandle h, int32_t slot);
+/* patchMass(idx, mass, baseMass) (gpu-physics.js:1955-1963). */
+int  geon_patch_mass(GeonHandle h, int32_t slot, double mass, double base_mass);
+
+/* Bulk state transfer (the SoA the reference's serialize/deserialize read and
+ * write, gpu-physics.js:3749-3836, 3886).  Upload replaces the whole system:
+ * slots 0..n-1.  Download fills slots 0..n_slots-1 and writes n_slots. */
+int  geon_upload_soa(GeonHandle h, const GeonSoAView* v, int32_t n);
+int  geon_download_soa(GeonHandle h, GeonSoAView* v, int32_t* n_slots);
+
+/* One Physics.update(particles, dt, ...) call (integrator.js:726-1825):
+ * adaptive substeps (<= max_substeps), boundary handling once at the end. */
+int  geon_update(GeonHandle h, double dt, int32_t* substeps_taken);
+
+/* The force pass alone, at the current state (integrator.js:750-782 bootstrap:
+ * computeAllForces + external fields).  For parity checks. */
+int  geon_compute_forces(GeonHandle h);
+int  geon_get_forces(GeonHandle h, GeonForceView* v, int32_t n_slots);
+
+/* requestStats + readStats (gpu-physics.js:4146-4281), in FP64. */
+int  geon_get_stats(GeonHandle h, GeonStats* out);
+
+/* Build the quadtree at the current positions and export it (quadtree.js:412-419). */
+int  geon_build_tree(GeonHandle h);
+int  geon_get_tree(GeonHandle h, GeonTreeNode* out, int64_t cap, int64_t* n_nodes);
+
+int  geon_counters(GeonHandle h, GeonCounters* out);
+/* Kernel timing: average device duration (ms) over the launches of `kernel`
+ * since the last reset, measured with HIP events on the handle's stream.
+ * kernel: 0 = force walk / direct sum, 1 = tree build, 2 = sort, 3 = integrator. */
+int  geon_timing_enable(GeonHandle h, int32_t on);
+int  geon_kernel_timing(GeonHandle h, int32_t kernel, double* total_ms, int64_t* launches);
+
+/* Last error message of this handle (or of the last failed create when h is NULL). */
+const char* geon_last_error(GeonHandle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEON_H */

@@ -1,0 +1,165 @@
+"""ctypes binding of oracle/libgeon_oracle.so (test infrastructure only)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from physsim_amd import _abi as A
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgeon_oracle.so")
+_LIB = None
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(A.GeonParams), C.c_int32]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_params.argtypes = [C.c_void_p, C.POINTER(A.GeonParams)]
+        L.orc_upload.argtypes = [C.c_void_p, C.c_int32] + [_DP] * 7
+        L.orc_count.argtypes = [C.c_void_p]
+        L.orc_count.restype = C.c_int32
+        L.orc_download.argtypes = [C.c_void_p] + [_DP] * 6 + [_IP]
+        L.orc_compute_forces.argtypes = [C.c_void_p]
+        L.orc_get_forces.argtypes = [C.c_void_p] + [_DP] * 11
+        L.orc_update.argtypes = [C.c_void_p, C.c_double, _IP]
+        L.orc_stats.argtypes = [C.c_void_p, C.POINTER(A.GeonStats)]
+        L.orc_build_tree.argtypes = [C.c_void_p]
+        L.orc_build_tree.restype = C.c_int64
+        L.orc_get_tree.argtypes = [C.c_void_p, C.POINTER(A.GeonTreeNode), C.c_int64]
+        L.orc_get_tree.restype = C.c_int64
+        L.orc_counters.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        L.orc_collision_pairs.argtypes = [C.c_void_p, _IP, _IP, C.c_int64]
+        L.orc_collision_pairs.restype = C.c_int64
+        L.orc_direct_forces.argtypes = [C.c_void_p, _IP, C.c_int32, _DP, _DP]
+        L.orc_set_threads.argtypes = [C.c_int32]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(_DP) if a is not None else None
+
+
+def default_params(**kw):
+    """Reference defaults (src/config.js) without needing the HIP library."""
+    p = A.GeonParams()
+    p.toggles0 = A.GRAVITY | A.COULOMB | A.BARNES_HUT
+    p.boundary_mode = A.BOUND_DESPAWN
+    p.topology = A.TORUS
+    p.collision_mode = A.COL_PASS
+    p.periodic_mode = A.PERIODIC_MIN_IMAGE
+    p.max_substeps = 32
+    p.domain_w = p.domain_h = 64.0
+    p.softening, p.bh_softening, p.theta = 8.0, 4.0, 0.5
+    p.yukawa_coupling, p.yukawa_mu = 14.0, 0.15
+    p.higgs_mass, p.axion_mass, p.hubble = 0.5, 0.05, 0.001
+    p.ext_gravity_angle = np.pi / 2
+    p.field_grid = 128
+    p.kahan = 1
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    def __init__(self, params, capacity=1024):
+        self.L = lib()
+        self.params = params
+        self.s = self.L.orc_create(C.byref(params), int(capacity))
+
+    def __del__(self):
+        if getattr(self, "s", None):
+            self.L.orc_destroy(self.s)
+            self.s = None
+
+    def set_params(self, p):
+        self.params = p
+        self.L.orc_set_params(self.s, C.byref(p))
+
+    def upload(self, x, y, wx, wy, mass, charge, angw=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, wx, wy, mass, charge)]
+        ang = np.ascontiguousarray(angw if angw is not None else np.zeros(len(x)), dtype=np.float64)
+        self.L.orc_upload(self.s, len(arrs[0]), *[_d(a) for a in arrs], _d(ang))
+
+    def count(self):
+        return self.L.orc_count(self.s)
+
+    def download(self):
+        n = self.count()
+        out = {k: np.zeros(n) for k in ("x", "y", "wx", "wy", "mass", "angw")}
+        ids = np.zeros(n, dtype=np.int32)
+        self.L.orc_download(self.s, *[_d(out[k]) for k in ("x", "y", "wx", "wy", "mass", "angw")],
+                            ids.ctypes.data_as(_IP))
+        out["id"] = ids
+        return out
+
+    def compute_forces(self):
+        self.L.orc_compute_forces(self.s)
+
+    def forces(self):
+        n = self.count()
+        names = ("fx", "fy", "bz", "bgz", "dbzdx", "dbzdy", "dbgzdx", "dbgzdy", "frame_drag", "tidal")
+        out = {k: np.zeros(n) for k in names}
+        pe = np.zeros(1)
+        self.L.orc_get_forces(self.s, *[_d(out[k]) for k in names], _d(pe))
+        out["pe_total"] = float(pe[0])
+        return out
+
+    def update(self, dt):
+        n = C.c_int32()
+        self.L.orc_update(self.s, float(dt), C.byref(n))
+        return n.value
+
+    def stats(self):
+        st = A.GeonStats()
+        self.L.orc_stats(self.s, C.byref(st))
+        return {k: getattr(st, k) for k, _ in A.GeonStats._fields_}
+
+    def tree(self):
+        nn = self.L.orc_build_tree(self.s)
+        arr = (A.GeonTreeNode * max(nn, 1))()
+        self.L.orc_get_tree(self.s, arr, nn)
+        return np.ctypeslib.as_array(arr)[:nn].copy()
+
+    def counters(self):
+        c = (C.c_int64 * 3)()
+        self.L.orc_counters(self.s, c)
+        return {"pair": c[0], "node": c[1], "visit": c[2]}
+
+    def collision_pairs(self):
+        n = self.L.orc_collision_pairs(self.s, None, None, 0)
+        a = np.zeros(max(n, 1), dtype=np.int32)
+        b = np.zeros(max(n, 1), dtype=np.int32)
+        self.L.orc_collision_pairs(self.s, a.ctypes.data_as(_IP), b.ctypes.data_as(_IP), n)
+        return a[:n], b[:n]
+
+    def direct_forces(self, idx=None, threads=None):
+        if threads:
+            self.L.orc_set_threads(int(threads))
+        n = self.count() if idx is None else len(idx)
+        fx = np.zeros(n)
+        fy = np.zeros(n)
+        if idx is None:
+            self.L.orc_direct_forces(self.s, None, 0, _d(fx), _d(fy))
+        else:
+            ii = np.ascontiguousarray(idx, dtype=np.int32)
+            self.L.orc_direct_forces(self.s, ii.ctypes.data_as(_IP), len(ii), _d(fx), _d(fy))
+        return fx, fy
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))

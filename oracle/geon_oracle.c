@@ -1,0 +1,1298 @@
+/*
+ * geon_oracle.c — FP64 CPU restatement of Geon's per-substep physics core.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker the HIP backend is compared against,
+ * and the CPU baseline timed by bench.py.  The product library never links it.
+ *
+ * Restates (a9lim/physsim):
+ *   src/forces.js      pairForce 163-442, calculateForce 622-694,
+ *                      computeAllForces 61-154, _accum1PN 445-506,
+ *                      _compute1PNTreeWalk 511-571, compute1PN 578-610
+ *   src/quadtree.js    pool 13-103, _subdivide 119-172, _childFor 175-179,
+ *                      insert 185-237, calculateMassDistribution 239-312,
+ *                      query 372-404, build 412-419
+ *   src/integrator.js  _syncToggles 138-150, _addGhost/_generateGhosts 178-244,
+ *                      _buildTree 247-260, external fields 553-592,
+ *                      update 726-1825 (radiation, bosons, collisions-merge,
+ *                      Hawking, disintegration and signal delay left out)
+ *   src/topology.js    minImage 18-76, wrapPosition 81-135
+ *   src/relativity.js  angwToAngVel 7-9, setVelocity 15-25
+ *   src/energy.js      computeEnergies 11-195 (passes 1-3)
+ *   src/collisions.js  overlap detection 35-60
+ *   main.js            addParticle normalization 504-539
+ *
+ * Parity pins for this file live in tests/test_oracle.py (closed forms,
+ * invariants, brute force).  Signal delay (signal-delay.js) is not restated:
+ * with GEON_RELATIVITY on, this oracle uses current-time sources (the
+ * reference couples signal delay to relativity, forces.js:85).
+ */
+#include "geon_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- constants: src/config.js ---- */
+#define EPS_ 1e-12                 /* EPSILON (config.js:30) */
+#define INERTIA_K 0.4              /* config.js:52 */
+#define MAG_MOMENT_K 0.2           /* config.js:53 */
+#define TIDAL_STRENGTH 64.0        /* config.js:54 */
+#define HIGGS_MASS_FLOOR 0.05      /* config.js:90 */
+#define MAX_SPEED_RATIO 0.999      /* config.js:32 */
+#define DESPAWN_MARGIN 64.0        /* config.js:138 */
+#define BOSON_CHARGE 0.1           /* config.js:75 */
+#define QT_CAP 4                   /* QUADTREE_CAPACITY (config.js:59) */
+
+typedef struct {
+    double x, y, vx, vy, wx, wy;
+    double mass, baseMass, charge, angw, angVel;
+    double radius, radiusSq, bodyRadiusSq, invMass;
+    double magMoment, angMomentum;
+    double axMod, yukMod, higgsMod;
+    double fx, fy;
+    double Bz, dBzdx, dBzdy, Bgz, dBgzdx, dBgzdy;
+    double frameDrag, tidal;
+    double f1x, f1y, f1ox, f1oy;
+    double pe;                     /* this particle's share of the PE accumulator */
+    int32_t id;                    /* upload index */
+    int32_t q_quanta;              /* round(charge / 0.1) */
+    int32_t isGhost, original;     /* ghost: index of the real particle */
+} P;
+
+typedef struct {
+    int cap, count;
+    double *bx, *by, *bw, *bh;
+    double *mass, *charge, *magMom, *angMom, *momX, *momY, *comX, *comY;
+    int32_t *nw, *ne, *sw, *se, *pointCount, *divided, *level, *subCount, *qq;
+    uint64_t *path;
+    int32_t *points;               /* cap * QT_CAP indices into the tree particle array */
+} Pool;
+
+struct OrcSys {
+    GeonParams prm;
+    int n, capacity;
+    P* p;                          /* real particles, array order */
+    P* tp;                         /* tree particles: real then ghosts */
+    int ntp, tpcap;
+    Pool pool;
+    int root;
+    int forcesInit;
+    double simTime;
+    double potentialEnergy;
+    int64_t cnt_pair, cnt_node, cnt_visit;
+    /* derived toggles (integrator.js:138-150) */
+    int grav, coul, mag, gm, onePN, rel, spinOrbit, bh, yuk, higgs, axion, barnesHut, expansion;
+    double softeningSq, softening, yukCutoffSq;
+    int needAxMod, periodic;
+};
+
+static int g_threads = 1;
+void orc_set_threads(int32_t n) { g_threads = n > 0 ? n : 1; }
+
+/* ------------------------------------------------------------------ */
+/* topology.js                                                         */
+/* ------------------------------------------------------------------ */
+static inline double torusWrap(double d, double full, double half) {      /* topology.js:9-12 */
+    if (d > half) d -= full; else if (d < -half) d += full;
+    return d;
+}
+
+static void minImage(double ox, double oy, double sx, double sy, int topo,
+                     double W, double H, double halfW, double halfH,
+                     double* outx, double* outy) {                        /* topology.js:18-76 */
+    double dx = sx - ox, dy = sy - oy;
+    if (topo == GEON_TORUS) {
+        *outx = torusWrap(dx, W, halfW);
+        *outy = torusWrap(dy, H, halfH);
+        return;
+    }
+    double doubleH = 2 * H;
+    if (topo == GEON_KLEIN) {
+        double bx = torusWrap(dx, W, halfW), by = dy;
+        double bestDx = bx, bestDy = by, bestSq = bx * bx + by * by;
+        double cx = torusWrap((W - sx) - ox, W, halfW);
+        double cy = torusWrap((sy + H) - oy, doubleH, H);
+        double sq = cx * cx + cy * cy;
+        if (sq < bestSq) { bestDx = cx; bestDy = cy; }
+        *outx = bestDx; *outy = bestDy;
+        return;
+    }
+    double doubleW = 2 * W;
+    double bestDx = dx, bestDy = dy, bestSq = dx * dx + dy * dy;
+    double cx = (W - sx) - ox, cy = torusWrap((sy + H) - oy, doubleH, H);
+    double sq = cx * cx + cy * cy;
+    if (sq < bestSq) { bestDx = cx; bestDy = cy; bestSq = sq; }
+    cx = torusWrap((sx + W) - ox, doubleW, W); cy = (H - sy) - oy;
+    sq = cx * cx + cy * cy;
+    if (sq < bestSq) { bestDx = cx; bestDy = cy; bestSq = sq; }
+    cx = torusWrap((2 * W - sx) - ox, doubleW, W); cy = torusWrap((2 * H - sy) - oy, doubleH, H);
+    sq = cx * cx + cy * cy;
+    if (sq < bestSq) { bestDx = cx; bestDy = cy; }
+    *outx = bestDx; *outy = bestDy;
+}
+
+static void wrapPosition(P* p, int topo, double W, double H) {            /* topology.js:81-135 */
+    if (topo == GEON_TORUS) {
+        if (p->x < 0) p->x += W; else if (p->x > W) p->x -= W;
+        if (p->y < 0) p->y += H; else if (p->y > H) p->y -= H;
+        return;
+    }
+    if (topo == GEON_KLEIN) {
+        if (p->x < 0) p->x += W; else if (p->x > W) p->x -= W;
+        if (p->y < 0 || p->y > H) {
+            p->y += (p->y < 0) ? H : -H;
+            p->x = W - p->x;
+            p->wx = -p->wx; p->vx = -p->vx; p->angw = -p->angw; p->angVel = -p->angVel;
+        }
+        return;
+    }
+    for (int iter = 0; iter < 2; iter++) {
+        int wrapped = 0;
+        if (p->x < 0 || p->x > W) {
+            p->x += (p->x < 0) ? W : -W; p->y = H - p->y;
+            p->wy = -p->wy; p->vy = -p->vy; p->angw = -p->angw; p->angVel = -p->angVel;
+            wrapped = 1;
+        }
+        if (p->y < 0 || p->y > H) {
+            p->y += (p->y < 0) ? H : -H; p->x = W - p->x;
+            p->wx = -p->wx; p->vx = -p->vx; p->angw = -p->angw; p->angVel = -p->angVel;
+            wrapped = 1;
+        }
+        if (!wrapped) break;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* relativity.js                                                       */
+/* ------------------------------------------------------------------ */
+static inline double angwToAngVel(double angw, double radius) {           /* relativity.js:7-9 */
+    return angw / sqrt(1 + angw * angw * radius * radius);
+}
+
+/* ------------------------------------------------------------------ */
+/* forces.js: pairForce                                                */
+/* ------------------------------------------------------------------ */
+/* Observer p gets the force of a source.  Restates forces.js:163-442 without
+ * the radiation jerk (radiationEnabled) and without signal-delay aberration. */
+static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, double svy,
+                      double sMass, double sCharge, double sAngVel, double sMagMoment,
+                      double sAngMomentum, double sAxMod, double sYukMod, double sHiggsMod,
+                      int accumulatePE) {
+    const GeonParams* g = &s->prm;
+    double rx, ry;
+    if (s->periodic) {
+        minImage(p->x, p->y, sx, sy, g->topology, g->domain_w, g->domain_h,
+                 g->domain_w * 0.5, g->domain_h * 0.5, &rx, &ry);
+    } else {
+        rx = sx - p->x; ry = sy - p->y;
+    }
+    const double rawRSq = rx * rx + ry * ry;
+    const double rSq = rawRSq + s->softeningSq;
+    const double invRSq = 1 / rSq;
+    const double invR = sqrt(invRSq);
+    const double invR3 = invR * invRSq;
+    const double invR5 = invR3 * invRSq;
+    const double invR3a = invR3, invR5a = invR5;     /* aberr = 1 (no signal delay) */
+    const double crossSV = svx * ry - svy * rx;
+    const double pMagMoment = p->magMoment, pAngMomentum = p->angMomentum;
+
+    if (s->grav) {                                                        /* forces.js:213-227 */
+        const double k = p->mass * sMass;
+        const double fDir = k * invR3a;
+        p->fx += rx * fDir; p->fy += ry * fDir;
+        if (accumulatePE) p->pe -= k * invR;
+    }
+    const double axModPair = s->needAxMod ? sqrt(p->axMod * sAxMod) : 1;  /* forces.js:230 */
+    if (s->coul) {                                                        /* forces.js:232-246 */
+        const double k = -(p->charge * sCharge) * axModPair;
+        const double fDir = k * invR3a;
+        p->fx += rx * fDir; p->fy += ry * fDir;
+        if (accumulatePE) p->pe += p->charge * sCharge * invR * axModPair;
+    }
+    if (s->onePN && (s->gm || s->mag)) {                                  /* forces.js:249-301 */
+        const double r = 1 / invR;
+        const double nx = rx * invR, ny = ry * invR;
+        const double pvx = p->vx, pvy = p->vy;
+        const double v1DotV2 = pvx * svx + pvy * svy;
+        if (s->gm) {
+            const double v1Sq = pvx * pvx + pvy * pvy;
+            const double v2Sq = svx * svx + svy * svy;
+            const double nDotV1 = nx * pvx + ny * pvy;
+            const double nDotV2 = nx * svx + ny * svy;
+            const double radial = -v1Sq - 2 * v2Sq + 1.5 * nDotV2 * nDotV2
+                                  + 5 * p->mass * invR + 4 * sMass * invR;
+            const double v1Coeff = 4 * nDotV1 - 3 * nDotV2;
+            const double v2Coeff = 3 * nDotV2;
+            const double base = p->mass * sMass * invR3;
+            const double fx = base * (rx * radial + (pvx * v1Coeff + svx * v2Coeff) * r);
+            const double fy = base * (ry * radial + (pvy * v1Coeff + svy * v2Coeff) * r);
+            p->fx += fx; p->fy += fy;
+            p->f1x += fx; p->f1y += fy;
+            if (accumulatePE)
+                p->pe -= p->mass * sMass * invR * (1.5 * (v1Sq + v2Sq) - 3.5 * v1DotV2
+                          - 0.5 * nDotV1 * nDotV2 + p->mass * invR + sMass * invR);
+        }
+        if (s->mag) {
+            const double v2DotN = svx * nx + svy * ny;
+            const double v1DotN = pvx * nx + pvy * ny;
+            const double coeff = 0.5 * p->charge * sCharge * invRSq;
+            const double symX = coeff * (pvx * v2DotN - 3 * nx * v1DotN * v2DotN);
+            const double symY = coeff * (pvy * v2DotN - 3 * ny * v1DotN * v2DotN);
+            p->fx += symX; p->fy += symY;
+            p->f1x += symX; p->f1y += symY;
+            if (accumulatePE) p->pe -= 0.5 * p->charge * sCharge * invR * (v1DotV2 + v1DotN * v2DotN);
+        }
+    }
+    if (s->onePN && s->gm && s->mag) {                                    /* forces.js:303-320 */
+        const double crossCoeff = p->charge * sCharge * (p->mass + sMass)
+            - (p->charge * p->charge * sMass + sCharge * sCharge * p->mass);
+        const double fDir = crossCoeff * invRSq * invRSq;
+        p->fx += rx * fDir; p->fy += ry * fDir;
+        p->f1x += rx * fDir; p->f1y += ry * fDir;
+        if (accumulatePE) p->pe += 0.5 * crossCoeff * invRSq;
+    }
+    if (s->mag) {                                                         /* forces.js:322-353 */
+        const double axMod = axModPair;
+        const double fDir = -3 * (pMagMoment * sMagMoment) * invR5a * axMod;
+        p->fx += rx * fDir; p->fy += ry * fDir;
+        if (accumulatePE) p->pe += (pMagMoment * sMagMoment) * invR3 * axMod;
+        const double BzMoving = sCharge * crossSV * invR3 * axMod;
+        p->Bz += BzMoving;
+        p->dBzdx += 3 * BzMoving * rx * invRSq + sCharge * svy * invR3 * axMod;
+        p->dBzdy += 3 * BzMoving * ry * invRSq - sCharge * svx * invR3 * axMod;
+        p->Bz -= sMagMoment * invR3 * axMod;
+        p->dBzdx -= 3 * sMagMoment * rx * invR5 * axMod;
+        p->dBzdy -= 3 * sMagMoment * ry * invR5 * axMod;
+    }
+    if (s->gm) {                                                          /* forces.js:355-387 */
+        const double fDir = 3 * (pAngMomentum * sAngMomentum) * invR5a;
+        p->fx += rx * fDir; p->fy += ry * fDir;
+        if (accumulatePE) p->pe -= (pAngMomentum * sAngMomentum) * invR3;
+        const double BgzMoving = -sMass * crossSV * invR3;
+        p->Bgz += BgzMoving;
+        p->dBgzdx += 3 * BgzMoving * rx * invRSq - sMass * svy * invR3;
+        p->dBgzdy += 3 * BgzMoving * ry * invRSq + sMass * svx * invR3;
+        p->Bgz -= 2 * sAngMomentum * invR3;
+        p->dBgzdx -= 6 * sAngMomentum * rx * invR5;
+        p->dBgzdy -= 6 * sAngMomentum * ry * invR5;
+        p->frameDrag += 2 * sAngMomentum * (sAngVel - p->angVel) * invR3;
+    }
+    if (s->yuk && rawRSq < s->yukCutoffSq) {                              /* forces.js:389-427 */
+        const double mu = s->higgs ? g->yukawa_mu * sqrt(p->higgsMod * sHiggsMod) : g->yukawa_mu;
+        const double r = 1 / invR;
+        const double expMuR = exp(-mu * r);
+        const double ym = s->needAxMod ? sqrt(p->yukMod * sYukMod) : 1;
+        const double G = g->yukawa_coupling;
+        const double fDir = G * ym * p->mass * sMass * expMuR * (invRSq + mu * invR) * invR;
+        p->fx += rx * fDir; p->fy += ry * fDir;
+        if (accumulatePE) p->pe -= G * ym * p->mass * sMass * expMuR * invR;
+        if (s->onePN) {
+            const double pvx = p->vx, pvy = p->vy;
+            const double nx = rx * invR, ny = ry * invR;
+            const double nDotV1 = nx * pvx + ny * pvy;
+            const double nDotV2 = nx * svx + ny * svy;
+            const double v1DotV2 = pvx * svx + pvy * svy;
+            const double alpha = 1 + mu * r;
+            const double beta = 0.5 * G * ym * p->mass * sMass * expMuR * invRSq;
+            const double radial = -(alpha * v1DotV2 + (alpha * alpha + alpha + 1) * nDotV1 * nDotV2);
+            const double fx = beta * (radial * nx + alpha * (nDotV2 * pvx + nDotV1 * svx));
+            const double fy = beta * (radial * ny + alpha * (nDotV2 * pvy + nDotV1 * svy));
+            p->fx += fx; p->fy += fy;
+            p->f1x += fx; p->f1y += fy;
+            if (accumulatePE)
+                p->pe += 0.5 * G * ym * p->mass * sMass * expMuR * invR * (v1DotV2 + nDotV1 * nDotV2 * alpha);
+        }
+    }
+    if (s->grav) {                                                        /* forces.js:429-441 */
+        const double crossRV = rx * (svy - p->vy) - ry * (svx - p->vx);
+        const double wOrbit = crossRV * invRSq;
+        const double dw = p->angVel - wOrbit;
+        double coupling = sMass;
+        if (s->coul && p->mass > EPS_) coupling += p->charge * sCharge / p->mass;
+        const double bodyR = sqrt(p->bodyRadiusSq);
+        const double ri5 = p->bodyRadiusSq * p->bodyRadiusSq * bodyR;
+        const double invR6 = invRSq * invRSq * invRSq;
+        p->tidal -= TIDAL_STRENGTH * coupling * coupling * ri5 * invR6 * dw;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* quadtree.js                                                         */
+/* ------------------------------------------------------------------ */
+static void pool_free(Pool* q) {
+    free(q->bx); free(q->by); free(q->bw); free(q->bh);
+    free(q->mass); free(q->charge); free(q->magMom); free(q->angMom);
+    free(q->momX); free(q->momY); free(q->comX); free(q->comY);
+    free(q->nw); free(q->ne); free(q->sw); free(q->se);
+    free(q->pointCount); free(q->divided); free(q->level); free(q->subCount); free(q->qq);
+    free(q->path); free(q->points);
+    memset(q, 0, sizeof(*q));
+}
+
+#define GROW(ptr, type, n) ptr = (type*)realloc(ptr, sizeof(type) * (size_t)(n))
+static void pool_grow(Pool* q, int newCap) {                              /* quadtree.js:81-103 */
+    GROW(q->bx, double, newCap); GROW(q->by, double, newCap);
+    GROW(q->bw, double, newCap); GROW(q->bh, double, newCap);
+    GROW(q->mass, double, newCap); GROW(q->charge, double, newCap);
+    GROW(q->magMom, double, newCap); GROW(q->angMom, double, newCap);
+    GROW(q->momX, double, newCap); GROW(q->momY, double, newCap);
+    GROW(q->comX, double, newCap); GROW(q->comY, double, newCap);
+    GROW(q->nw, int32_t, newCap); GROW(q->ne, int32_t, newCap);
+    GROW(q->sw, int32_t, newCap); GROW(q->se, int32_t, newCap);
+    GROW(q->pointCount, int32_t, newCap); GROW(q->divided, int32_t, newCap);
+    GROW(q->level, int32_t, newCap); GROW(q->subCount, int32_t, newCap);
+    GROW(q->qq, int32_t, newCap); GROW(q->path, uint64_t, newCap);
+    GROW(q->points, int32_t, (size_t)newCap * QT_CAP);
+    q->cap = newCap;
+}
+
+static int pool_alloc(Pool* q, double bx, double by, double bw, double bh) {  /* quadtree.js:52-79 */
+    if (q->count >= q->cap) pool_grow(q, q->cap ? q->cap * 2 : 512);
+    int i = q->count++;
+    q->bx[i] = bx; q->by[i] = by; q->bw[i] = bw; q->bh[i] = bh;
+    q->mass[i] = q->charge[i] = q->magMom[i] = q->angMom[i] = 0;
+    q->momX[i] = q->momY[i] = 0;
+    q->comX[i] = bx; q->comY[i] = by;
+    q->nw[i] = q->ne[i] = q->sw[i] = q->se[i] = -1;
+    q->pointCount[i] = 0; q->divided[i] = 0;
+    q->level[i] = 0; q->path[i] = 0; q->subCount[i] = 0; q->qq[i] = 0;
+    return i;
+}
+
+static void pool_subdivide(Pool* q, int idx) {                           /* quadtree.js:119-172 */
+    double x = q->bx[idx], y = q->by[idx];
+    double hw = q->bw[idx] / 2, hh = q->bh[idx] / 2;
+    if (q->count + 4 > q->cap) pool_grow(q, q->cap * 2);
+    int c = q->count;
+    q->count = c + 4;
+    const double cx[4] = {x - hw, x + hw, x - hw, x + hw};
+    const double cy[4] = {y - hh, y - hh, y + hh, y + hh};
+    for (int k = 0; k < 4; k++) {
+        int j = c + k;
+        q->bx[j] = cx[k]; q->by[j] = cy[k]; q->bw[j] = hw; q->bh[j] = hh;
+        q->mass[j] = q->charge[j] = q->magMom[j] = q->angMom[j] = 0;
+        q->momX[j] = q->momY[j] = 0;
+        q->comX[j] = cx[k]; q->comY[j] = cy[k];
+        q->nw[j] = q->ne[j] = q->sw[j] = q->se[j] = -1;
+        q->pointCount[j] = 0; q->divided[j] = 0;
+        q->level[j] = q->level[idx] + 1;
+        q->path[j] = (q->path[idx] << 2) | (uint64_t)k;
+        q->subCount[j] = 0; q->qq[j] = 0;
+    }
+    q->nw[idx] = c; q->ne[idx] = c + 1; q->sw[idx] = c + 2; q->se[idx] = c + 3;
+    q->divided[idx] = 1;
+}
+
+static inline int pool_childFor(const Pool* q, int idx, double px, double py) {  /* quadtree.js:175-179 */
+    return py <= q->by[idx] ? (px <= q->bx[idx] ? q->nw[idx] : q->ne[idx])
+                            : (px <= q->bx[idx] ? q->sw[idx] : q->se[idx]);
+}
+
+static inline int pool_contains(const Pool* q, int idx, double px, double py) {  /* quadtree.js:105-110 */
+    return px >= q->bx[idx] - q->bw[idx] && px <= q->bx[idx] + q->bw[idx] &&
+           py >= q->by[idx] - q->bh[idx] && py <= q->by[idx] + q->bh[idx];
+}
+
+/* Iterative insert with a (node, point) work stack (quadtree.js:185-237). */
+static int pool_insert(Pool* q, int idx, const P* tp, int pi, int32_t** stkN, int32_t** stkP, int* stkCap) {
+    if (!pool_contains(q, idx, tp[pi].x, tp[pi].y)) return 0;
+    int top = 0;
+    if (*stkCap < 64) { *stkCap = 64; *stkN = realloc(*stkN, 64 * 4); *stkP = realloc(*stkP, 64 * 4); }
+    (*stkN)[top] = idx; (*stkP)[top] = pi; top++;
+    while (top > 0) {
+        top--;
+        int nodeIdx = (*stkN)[top];
+        int pt = (*stkP)[top];
+        const double px = tp[pt].x, py = tp[pt].y;
+        int depth = 0;
+        while (depth <= 48) {
+            if (q->pointCount[nodeIdx] < QT_CAP && !q->divided[nodeIdx]) {
+                q->points[(size_t)nodeIdx * QT_CAP + q->pointCount[nodeIdx]] = pt;
+                q->pointCount[nodeIdx]++;
+                break;
+            }
+            if (!q->divided[nodeIdx]) {
+                pool_subdivide(q, nodeIdx);
+                if (top + QT_CAP + 1 > *stkCap) {
+                    *stkCap *= 2;
+                    *stkN = realloc(*stkN, (size_t)*stkCap * 4);
+                    *stkP = realloc(*stkP, (size_t)*stkCap * 4);
+                }
+                size_t base = (size_t)nodeIdx * QT_CAP;
+                for (int i = 0; i < q->pointCount[nodeIdx]; i++) {
+                    int pp = q->points[base + i];
+                    (*stkN)[top] = pool_childFor(q, nodeIdx, tp[pp].x, tp[pp].y);
+                    (*stkP)[top] = pp;
+                    top++;
+                }
+                q->pointCount[nodeIdx] = 0;
+            }
+            nodeIdx = pool_childFor(q, nodeIdx, px, py);
+            depth++;
+        }
+    }
+    return 1;
+}
+
+/* Post-order FP64 aggregates (quadtree.js:239-312); also subtree point counts
+ * and exact charge-quanta sums for the canonical export. */
+static void pool_massDistribution(Pool* q, const P* tp, int root) {
+    int* order = (int*)malloc(sizeof(int) * (size_t)q->count);
+    int* stack = (int*)malloc(sizeof(int) * (size_t)(q->count + 4));
+    int top = 0, len = 0;
+    stack[top++] = root;
+    while (top > 0) {
+        int idx = stack[--top];
+        order[len++] = idx;
+        if (q->divided[idx]) {
+            stack[top++] = q->nw[idx]; stack[top++] = q->ne[idx];
+            stack[top++] = q->sw[idx]; stack[top++] = q->se[idx];
+        }
+    }
+    for (int k = len - 1; k >= 0; k--) {
+        int idx = order[k];
+        if (!q->divided[idx]) {
+            int cnt = q->pointCount[idx];
+            q->subCount[idx] = cnt;
+            if (cnt == 0) continue;
+            double mass = 0, charge = 0, magMom = 0, angMom = 0, cx = 0, cy = 0, momX = 0, momY = 0;
+            int qq = 0;
+            size_t base = (size_t)idx * QT_CAP;
+            for (int i = 0; i < cnt; i++) {
+                const P* p = &tp[q->points[base + i]];
+                double pm = p->mass;
+                mass += pm; charge += p->charge; magMom += p->magMoment; angMom += p->angMomentum;
+                cx += p->x * pm; cy += p->y * pm;
+                momX += pm * p->wx; momY += pm * p->wy;
+                qq += p->q_quanta;
+            }
+            q->mass[idx] = mass; q->charge[idx] = charge;
+            q->magMom[idx] = magMom; q->angMom[idx] = angMom;
+            q->momX[idx] = momX; q->momY[idx] = momY;
+            q->qq[idx] = qq;
+            if (mass > 0) { q->comX[idx] = cx / mass; q->comY[idx] = cy / mass; }
+        } else {
+            int c0 = q->nw[idx], c1 = q->ne[idx], c2 = q->sw[idx], c3 = q->se[idx];
+            double m0 = q->mass[c0], m1 = q->mass[c1], m2 = q->mass[c2], m3 = q->mass[c3];
+            double mass = m0 + m1 + m2 + m3;
+            q->mass[idx] = mass;
+            q->charge[idx] = q->charge[c0] + q->charge[c1] + q->charge[c2] + q->charge[c3];
+            q->magMom[idx] = q->magMom[c0] + q->magMom[c1] + q->magMom[c2] + q->magMom[c3];
+            q->angMom[idx] = q->angMom[c0] + q->angMom[c1] + q->angMom[c2] + q->angMom[c3];
+            q->momX[idx] = q->momX[c0] + q->momX[c1] + q->momX[c2] + q->momX[c3];
+            q->momY[idx] = q->momY[c0] + q->momY[c1] + q->momY[c2] + q->momY[c3];
+            q->subCount[idx] = q->subCount[c0] + q->subCount[c1] + q->subCount[c2] + q->subCount[c3];
+            q->qq[idx] = q->qq[c0] + q->qq[c1] + q->qq[c2] + q->qq[c3];
+            if (mass > 0) {
+                q->comX[idx] = (q->comX[c0] * m0 + q->comX[c1] * m1 + q->comX[c2] * m2 + q->comX[c3] * m3) / mass;
+                q->comY[idx] = (q->comY[c0] * m0 + q->comY[c1] * m1 + q->comY[c2] * m2 + q->comY[c3] * m3) / mass;
+            }
+        }
+    }
+    free(order); free(stack);
+}
+
+/* ------------------------------------------------------------------ */
+/* integrator.js: ghosts and tree build                                */
+/* ------------------------------------------------------------------ */
+static void tp_reserve(OrcSys* s, int n) {
+    if (n > s->tpcap) {
+        s->tpcap = n * 2 + 16;
+        s->tp = (P*)realloc(s->tp, sizeof(P) * (size_t)s->tpcap);
+    }
+}
+
+static void addGhost(OrcSys* s, int pi, double sx, double sy, int flipVx, int flipVy) {  /* integrator.js:178-198 */
+    const P* p = &s->p[pi];
+    P* g = &s->tp[s->ntp++];
+    *g = *p;
+    g->x = sx; g->y = sy;
+    g->vx = flipVx ? -p->vx : p->vx; g->vy = flipVy ? -p->vy : p->vy;
+    g->wx = flipVx ? -p->wx : p->wx; g->wy = flipVy ? -p->wy : p->wy;
+    g->angVel = (flipVx || flipVy) ? -p->angVel : p->angVel;
+    g->angw = (flipVx || flipVy) ? -p->angw : p->angw;
+    g->magMoment = MAG_MOMENT_K * g->charge * g->angVel * g->radiusSq;
+    g->angMomentum = INERTIA_K * g->mass * g->angVel * g->radiusSq;
+    g->isGhost = 1; g->original = pi; g->id = -1;
+}
+
+static void generateGhosts(OrcSys* s) {                                    /* integrator.js:201-244 */
+    const double W = s->prm.domain_w, H = s->prm.domain_h;
+    const double margin = (W > H ? W : H) * s->prm.theta;
+    const int topo = s->prm.topology;
+    for (int i = 0; i < s->n; i++) {
+        const double x = s->p[i].x, y = s->p[i].y;
+        int nearL = x < margin, nearR = x > W - margin;
+        int nearT = y < margin, nearB = y > H - margin;
+        tp_reserve(s, s->ntp + 8);
+        if (topo == GEON_TORUS) {
+            if (nearL) addGhost(s, i, x + W, y, 0, 0);
+            if (nearR) addGhost(s, i, x - W, y, 0, 0);
+            if (nearT) addGhost(s, i, x, y + H, 0, 0);
+            if (nearB) addGhost(s, i, x, y - H, 0, 0);
+            if (nearL && nearT) addGhost(s, i, x + W, y + H, 0, 0);
+            if (nearL && nearB) addGhost(s, i, x + W, y - H, 0, 0);
+            if (nearR && nearT) addGhost(s, i, x - W, y + H, 0, 0);
+            if (nearR && nearB) addGhost(s, i, x - W, y - H, 0, 0);
+        } else if (topo == GEON_KLEIN) {
+            if (nearL) addGhost(s, i, x + W, y, 0, 0);
+            if (nearR) addGhost(s, i, x - W, y, 0, 0);
+            if (nearT) addGhost(s, i, W - x, y + H, 1, 0);
+            if (nearB) addGhost(s, i, W - x, y - H, 1, 0);
+            if (nearL && nearT) addGhost(s, i, W - x + W, y + H, 1, 0);
+            if (nearL && nearB) addGhost(s, i, W - x + W, y - H, 1, 0);
+            if (nearR && nearT) addGhost(s, i, W - x - W, y + H, 1, 0);
+            if (nearR && nearB) addGhost(s, i, W - x - W, y - H, 1, 0);
+        } else {
+            if (nearL) addGhost(s, i, x + W, H - y, 0, 1);
+            if (nearR) addGhost(s, i, x - W, H - y, 0, 1);
+            if (nearT) addGhost(s, i, W - x, y + H, 1, 0);
+            if (nearB) addGhost(s, i, W - x, y - H, 1, 0);
+            if (nearL && nearT) addGhost(s, i, W - x + W, y + H, 1, 0);
+            if (nearL && nearB) addGhost(s, i, W - x + W, y - H, 1, 0);
+            if (nearR && nearT) addGhost(s, i, W - x - W, y + H, 1, 0);
+            if (nearR && nearB) addGhost(s, i, W - x - W, y - H, 1, 0);
+        }
+    }
+}
+
+/* _buildTree (integrator.js:247-260) + QuadTreePool.build (quadtree.js:412-419).
+ * Root box: centre (W/2, H/2), half-extent (2W, 2H) (integrator.js:727-730). */
+static int buildTree(OrcSys* s) {
+    tp_reserve(s, s->n + 8);
+    s->ntp = 0;
+    for (int i = 0; i < s->n; i++) { s->tp[s->ntp] = s->p[i]; s->tp[s->ntp].isGhost = 0; s->tp[s->ntp].original = i; s->ntp++; }
+    if (s->periodic && s->prm.periodic_mode == GEON_PERIODIC_GHOSTS) generateGhosts(s);
+    Pool* q = &s->pool;
+    q->count = 0;
+    const double W = s->prm.domain_w, H = s->prm.domain_h;
+    int root = pool_alloc(q, W / 2, H / 2, W * 2, H * 2);
+    int32_t *stkN = NULL, *stkP = NULL; int stkCap = 0;
+    for (int i = 0; i < s->ntp; i++) pool_insert(q, root, s->tp, i, &stkN, &stkP, &stkCap);
+    free(stkN); free(stkP);
+    pool_massDistribution(q, s->tp, root);
+    s->root = root;
+    return root;
+}
+
+/* ------------------------------------------------------------------ */
+/* forces.js: walk, direct sum, computeAllForces                       */
+/* ------------------------------------------------------------------ */
+/* calculateForce (forces.js:622-694), sources at current time. */
+static void calculateForce(OrcSys* s, int pi, int32_t* stack, int64_t* cnt) {
+    const Pool* q = &s->pool;
+    P* p = &s->p[pi];
+    const double thetaSq = s->prm.theta * s->prm.theta;
+    const double W = s->prm.domain_w, H = s->prm.domain_h;
+    const double px = p->x, py = p->y;
+    int top = 0;
+    stack[top++] = s->root;
+    while (top > 0) {
+        int nodeIdx = stack[--top];
+        cnt[2]++;
+        if (q->mass[nodeIdx] < EPS_) continue;
+        double dx, dy;
+        if (s->periodic) minImage(px, py, q->comX[nodeIdx], q->comY[nodeIdx], s->prm.topology, W, H, W * 0.5, H * 0.5, &dx, &dy);
+        else { dx = q->comX[nodeIdx] - px; dy = q->comY[nodeIdx] - py; }
+        const double dSq = dx * dx + dy * dy;
+        const double size = q->bw[nodeIdx] * 2;
+        if (!q->divided[nodeIdx] && q->pointCount[nodeIdx] > 0) {
+            size_t base = (size_t)nodeIdx * QT_CAP;
+            for (int i = 0; i < q->pointCount[nodeIdx]; i++) {
+                int oi = q->points[base + i];
+                const P* o = &s->tp[oi];
+                if (o->original == pi) continue;    /* other === particle, or its own ghost */
+                const P* real = &s->p[o->original];
+                if (!o->isGhost) o = real;          /* a real source is the live object (fresh μ, L) */
+                pairForce(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
+                          o->magMoment, o->angMomentum, real->axMod, real->yukMod, real->higgsMod, 1);
+                cnt[0]++;
+            }
+        } else if (q->divided[nodeIdx] && (size * size < thetaSq * dSq)) {
+            const double nodeMass = q->mass[nodeIdx];
+            pairForce(s, p, q->comX[nodeIdx], q->comY[nodeIdx], q->momX[nodeIdx] / nodeMass,
+                      q->momY[nodeIdx] / nodeMass, nodeMass, q->charge[nodeIdx], 0,
+                      q->magMom[nodeIdx], q->angMom[nodeIdx], 1, 1, 1, 1);
+            cnt[1]++;
+        } else if (q->divided[nodeIdx]) {
+            stack[top++] = q->nw[nodeIdx]; stack[top++] = q->ne[nodeIdx];
+            stack[top++] = q->sw[nodeIdx]; stack[top++] = q->se[nodeIdx];
+        }
+    }
+}
+
+static void resetForces(OrcSys* s) {                                      /* forces.js:31-54 */
+    for (int i = 0; i < s->n; i++) {
+        P* p = &s->p[i];
+        p->fx = p->fy = 0;
+        p->f1x = p->f1y = 0;
+        p->Bz = p->dBzdx = p->dBzdy = 0;
+        p->Bgz = p->dBgzdx = p->dBgzdy = 0;
+        p->frameDrag = p->tidal = 0;
+        p->pe = 0;
+    }
+}
+
+/* computeAllForces (forces.js:61-154) without the dead-particle loop. */
+static void computeAllForces(OrcSys* s) {
+    const int n = s->n;
+    for (int i = 0; i < n; i++) {                                         /* forces.js:71-76 */
+        P* p = &s->p[i];
+        p->magMoment = MAG_MOMENT_K * p->charge * p->angVel * p->bodyRadiusSq;
+        p->angMomentum = INERTIA_K * p->mass * p->angVel * p->bodyRadiusSq;
+    }
+    int64_t c0 = 0, c1 = 0, c2 = 0;
+    if (s->barnesHut) {
+        if (s->root >= 0) {
+            #pragma omp parallel num_threads(g_threads) reduction(+:c0,c1,c2)
+            {
+                int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(s->pool.count * 2 + 16));
+                int64_t cnt[3] = {0, 0, 0};
+                #pragma omp for schedule(dynamic, 64)
+                for (int i = 0; i < n; i++) calculateForce(s, i, stack, cnt);
+                c0 += cnt[0]; c1 += cnt[1]; c2 += cnt[2];
+                free(stack);
+            }
+        }
+    } else {
+        #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 16) reduction(+:c0)
+        for (int i = 0; i < n; i++) {                                     /* forces.js:93-126 */
+            P* p = &s->p[i];
+            for (int j = 0; j < n; j++) {
+                if (i == j) continue;
+                const P* o = &s->p[j];
+                pairForce(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
+                          o->magMoment, o->angMomentum, o->axMod, o->yukMod, o->higgsMod, 1);
+                c0++;
+            }
+        }
+    }
+    s->cnt_pair = c0; s->cnt_node = c1; s->cnt_visit = c2;
+}
+
+/* _applyExternalFields (integrator.js:563-592). */
+static void applyExternalFields(OrcSys* s) {
+    const GeonParams* g = &s->prm;
+    if (g->ext_gravity == 0 && g->ext_electric == 0 && g->ext_bz == 0) return;
+    const double gx = g->ext_gravity * cos(g->ext_gravity_angle), gy = g->ext_gravity * sin(g->ext_gravity_angle);
+    const double ex = g->ext_electric * cos(g->ext_electric_angle), ey = g->ext_electric * sin(g->ext_electric_angle);
+    for (int i = 0; i < s->n; i++) {
+        P* p = &s->p[i];
+        if (g->ext_gravity != 0) { p->fx += p->mass * gx; p->fy += p->mass * gy; }
+        if (g->ext_electric != 0) { p->fx += p->charge * ex; p->fy += p->charge * ey; }
+        if (g->ext_bz != 0) p->Bz += g->ext_bz;
+    }
+}
+
+/* _accum1PN (forces.js:445-506): observer state passed explicitly. */
+static void accum1PN(const OrcSys* s, P* p, double sx, double sy, double svx, double svy,
+                     double sMass, double sCharge, double sYukMod, double sHiggsMod) {
+    const GeonParams* g = &s->prm;
+    double rx, ry;
+    if (s->periodic) minImage(p->x, p->y, sx, sy, g->topology, g->domain_w, g->domain_h, g->domain_w * 0.5, g->domain_h * 0.5, &rx, &ry);
+    else { rx = sx - p->x; ry = sy - p->y; }
+    const double rSq = rx * rx + ry * ry + s->softeningSq;
+    const double invRSq = 1 / rSq;
+    const double invR = sqrt(invRSq);
+    const double r = 1 / invR;
+    const double nx = rx * invR, ny = ry * invR;
+    const double pvx = p->vx, pvy = p->vy, pMass = p->mass, pCharge = p->charge;
+    if (s->gm) {
+        const double v1Sq = pvx * pvx + pvy * pvy, v2Sq = svx * svx + svy * svy;
+        const double nDotV1 = nx * pvx + ny * pvy, nDotV2 = nx * svx + ny * svy;
+        const double radial = -v1Sq - 2 * v2Sq + 1.5 * nDotV2 * nDotV2 + 5 * pMass * invR + 4 * sMass * invR;
+        const double v1Coeff = 4 * nDotV1 - 3 * nDotV2, v2Coeff = 3 * nDotV2;
+        const double base = pMass * sMass * invRSq * invR;
+        p->f1x += base * (rx * radial + (pvx * v1Coeff + svx * v2Coeff) * r);
+        p->f1y += base * (ry * radial + (pvy * v1Coeff + svy * v2Coeff) * r);
+    }
+    if (s->mag) {
+        const double v2DotN = svx * nx + svy * ny, v1DotN = pvx * nx + pvy * ny;
+        const double coeff = 0.5 * pCharge * sCharge * invRSq;
+        p->f1x += coeff * (pvx * v2DotN - 3 * nx * v1DotN * v2DotN);
+        p->f1y += coeff * (pvy * v2DotN - 3 * ny * v1DotN * v2DotN);
+    }
+    if (s->gm && s->mag) {
+        const double crossCoeff = pCharge * sCharge * (pMass + sMass) - (pCharge * pCharge * sMass + sCharge * sCharge * pMass);
+        const double fDir = crossCoeff * invRSq * invRSq;
+        p->f1x += rx * fDir; p->f1y += ry * fDir;
+    }
+    if (s->yuk) {
+        const double mu = s->higgs ? g->yukawa_mu * sqrt(p->higgsMod * sHiggsMod) : g->yukawa_mu;
+        const double expMuR = exp(-mu * r);
+        const double nDotV1 = nx * pvx + ny * pvy, nDotV2 = nx * svx + ny * svy;
+        const double v1DotV2 = pvx * svx + pvy * svy;
+        const double alpha = 1 + mu * r;
+        const double beta = 0.5 * g->yukawa_coupling * sqrt(p->yukMod * sYukMod) * pMass * sMass * expMuR * invRSq;
+        const double radial = -(alpha * v1DotV2 + (alpha * alpha + alpha + 1) * nDotV1 * nDotV2);
+        p->f1x += beta * (radial * nx + alpha * (nDotV2 * pvx + nDotV1 * svx));
+        p->f1y += beta * (radial * ny + alpha * (nDotV2 * pvy + nDotV1 * svy));
+    }
+}
+
+/* compute1PN (forces.js:578-610) and its tree walk (511-571), sources at the
+ * current time. */
+static void compute1PN(OrcSys* s) {
+    const int n = s->n;
+    for (int i = 0; i < n; i++) s->p[i].f1x = s->p[i].f1y = 0;
+    if (s->barnesHut) {
+        buildTree(s);
+        const Pool* q = &s->pool;
+        const double thetaSq = s->prm.theta * s->prm.theta;
+        const double W = s->prm.domain_w, H = s->prm.domain_h;
+        #pragma omp parallel num_threads(g_threads)
+        {
+            int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(q->count * 2 + 16));
+            #pragma omp for schedule(dynamic, 64)
+            for (int pi = 0; pi < n; pi++) {
+                P* p = &s->p[pi];
+                int top = 0;
+                stack[top++] = s->root;
+                while (top > 0) {
+                    int ni = stack[--top];
+                    if (q->mass[ni] < EPS_) continue;
+                    double dx, dy;
+                    if (s->periodic) minImage(p->x, p->y, q->comX[ni], q->comY[ni], s->prm.topology, W, H, W * 0.5, H * 0.5, &dx, &dy);
+                    else { dx = q->comX[ni] - p->x; dy = q->comY[ni] - p->y; }
+                    const double dSq = dx * dx + dy * dy, size = q->bw[ni] * 2;
+                    if (!q->divided[ni] && q->pointCount[ni] > 0) {
+                        size_t base = (size_t)ni * QT_CAP;
+                        for (int k = 0; k < q->pointCount[ni]; k++) {
+                            const P* o = &s->tp[q->points[base + k]];
+                            if (o->original == pi) continue;
+                            const P* real = &s->p[o->original];
+                            accum1PN(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, real->yukMod, real->higgsMod);
+                        }
+                    } else if (q->divided[ni] && size * size < thetaSq * dSq) {
+                        const double m = q->mass[ni];
+                        accum1PN(s, p, q->comX[ni], q->comY[ni], q->momX[ni] / m, q->momY[ni] / m, m, q->charge[ni], 1, 1);
+                    } else if (q->divided[ni]) {
+                        stack[top++] = q->nw[ni]; stack[top++] = q->ne[ni];
+                        stack[top++] = q->sw[ni]; stack[top++] = q->se[ni];
+                    }
+                }
+            }
+            free(stack);
+        }
+    } else {
+        #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 16)
+        for (int i = 0; i < n; i++) {
+            P* p = &s->p[i];
+            for (int j = 0; j < n; j++) {
+                if (i == j) continue;
+                const P* o = &s->p[j];
+                accum1PN(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->yukMod, o->higgsMod);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* public API                                                          */
+/* ------------------------------------------------------------------ */
+static void syncToggles(OrcSys* s) {                                      /* integrator.js:138-150 */
+    const uint32_t t = s->prm.toggles0;
+    s->grav = !!(t & GEON_GRAVITY); s->coul = !!(t & GEON_COULOMB);
+    s->mag = !!(t & GEON_MAGNETIC); s->gm = !!(t & GEON_GRAVITOMAG);
+    s->onePN = !!(t & GEON_ONE_PN); s->rel = !!(t & GEON_RELATIVITY);
+    s->spinOrbit = !!(t & GEON_SPIN_ORBIT); s->bh = !!(t & GEON_BLACK_HOLE);
+    s->yuk = !!(t & GEON_YUKAWA); s->higgs = !!(t & GEON_HIGGS); s->axion = !!(t & GEON_AXION);
+    s->barnesHut = !!(t & GEON_BARNES_HUT); s->expansion = !!(t & GEON_EXPANSION);
+    s->softening = s->bh ? s->prm.bh_softening : s->prm.softening;
+    s->softeningSq = s->softening * s->softening;
+    s->needAxMod = (s->coul || s->mag) && s->axion;                       /* forces.js:79 */
+    const double muMin = s->higgs ? s->prm.yukawa_mu * HIGGS_MASS_FLOOR : s->prm.yukawa_mu;
+    s->yukCutoffSq = s->yuk ? (6 / muMin) * (6 / muMin) : INFINITY;       /* forces.js:82-83 */
+    s->periodic = s->prm.boundary_mode == GEON_BOUND_LOOP;
+}
+
+OrcSys* orc_create(const GeonParams* prm, int32_t capacity) {
+    OrcSys* s = (OrcSys*)calloc(1, sizeof(OrcSys));
+    s->prm = *prm;
+    s->capacity = capacity > 0 ? capacity : 1;
+    s->p = (P*)calloc((size_t)s->capacity, sizeof(P));
+    s->root = -1;
+    syncToggles(s);
+    return s;
+}
+
+void orc_destroy(OrcSys* s) {
+    if (!s) return;
+    pool_free(&s->pool);
+    free(s->p); free(s->tp); free(s);
+}
+
+int orc_set_params(OrcSys* s, const GeonParams* prm) { s->prm = *prm; syncToggles(s); return 0; }
+
+/* addParticle normalization (main.js:504-539) with proper velocity given
+ * directly (the GPU contract passes w, gpu-physics.js:1857). */
+int orc_upload(OrcSys* s, int32_t n, const double* x, const double* y, const double* wx,
+               const double* wy, const double* mass, const double* charge, const double* angw) {
+    if (n > s->capacity) {
+        s->capacity = n;
+        s->p = (P*)realloc(s->p, sizeof(P) * (size_t)n);
+    }
+    s->n = n;
+    for (int i = 0; i < n; i++) {
+        P* p = &s->p[i];
+        memset(p, 0, sizeof(*p));
+        p->x = x[i]; p->y = y[i];
+        p->mass = mass[i]; p->baseMass = mass[i];
+        p->q_quanta = (int32_t)lround(charge[i] / BOSON_CHARGE);
+        p->charge = (double)p->q_quanta * BOSON_CHARGE;
+        p->radius = cbrt(p->mass);
+        p->radiusSq = p->radius * p->radius;
+        p->bodyRadiusSq = p->radiusSq;
+        p->invMass = 1 / p->mass;
+        p->angw = angw ? angw[i] : 0;
+        p->wx = wx[i]; p->wy = wy[i];
+        if (s->rel) {
+            const double invG = 1 / sqrt(1 + p->wx * p->wx + p->wy * p->wy);
+            p->vx = p->wx * invG; p->vy = p->wy * invG;
+        } else { p->vx = p->wx; p->vy = p->wy; }
+        p->angVel = s->rel ? angwToAngVel(p->angw, p->radius) : p->angw;
+        p->axMod = p->yukMod = p->higgsMod = 1;
+        p->id = i;
+    }
+    s->forcesInit = 0;
+    s->simTime = 0;
+    s->root = -1;
+    return 0;
+}
+
+int32_t orc_count(OrcSys* s) { return s->n; }
+
+int orc_download(OrcSys* s, double* x, double* y, double* wx, double* wy, double* mass, double* angw, int32_t* ids) {
+    for (int i = 0; i < s->n; i++) {
+        const P* p = &s->p[i];
+        if (x) x[i] = p->x; if (y) y[i] = p->y;
+        if (wx) wx[i] = p->wx; if (wy) wy[i] = p->wy;
+        if (mass) mass[i] = p->mass; if (angw) angw[i] = p->angw;
+        if (ids) ids[i] = p->id;
+    }
+    return 0;
+}
+
+/* Tree rebuild (integrator.js:1340-1348) then resetForces, _syncAxionField and
+ * computeAllForces (1490-1493).  The tree's μ and L aggregates use the moments
+ * cached by the previous force pass: computeAllForces re-caches them only after
+ * the tree is built (forces.js:71-76). */
+static void forcePass(OrcSys* s) {
+    if (s->barnesHut) buildTree(s);
+    resetForces(s);
+    for (int i = 0; i < s->n; i++) s->p[i].axMod = s->p[i].yukMod = 1;   /* _syncAxionField, no field */
+    computeAllForces(s);
+    applyExternalFields(s);
+}
+
+int orc_compute_forces(OrcSys* s) {
+    syncToggles(s);
+    for (int i = 0; i < s->n; i++)
+        s->p[i].angVel = s->rel ? angwToAngVel(s->p[i].angw, s->p[i].radius) : s->p[i].angw;
+    forcePass(s);
+    s->forcesInit = 1;
+    return 0;
+}
+
+int orc_get_forces(OrcSys* s, double* fx, double* fy, double* bz, double* bgz, double* dbzdx,
+                   double* dbzdy, double* dbgzdx, double* dbgzdy, double* frame_drag, double* tidal,
+                   double* pe_total) {
+    double pe = 0;
+    for (int i = 0; i < s->n; i++) {
+        const P* p = &s->p[i];
+        if (fx) fx[i] = p->fx; if (fy) fy[i] = p->fy;
+        if (bz) bz[i] = p->Bz; if (bgz) bgz[i] = p->Bgz;
+        if (dbzdx) dbzdx[i] = p->dBzdx; if (dbzdy) dbzdy[i] = p->dBzdy;
+        if (dbgzdx) dbgzdx[i] = p->dBgzdx; if (dbgzdy) dbgzdy[i] = p->dBgzdy;
+        if (frame_drag) frame_drag[i] = p->frameDrag; if (tidal) tidal[i] = p->tidal;
+        pe += p->pe;
+    }
+    if (pe_total) *pe_total = pe * 0.5;                                   /* getPEAccum (forces.js:26) */
+    return 0;
+}
+
+static void cyclotronScan(OrcSys* s, double* maxCyc) {                    /* integrator.js:817-849 */
+    const int hasMagnetic = s->mag, hasExtBz = s->prm.ext_bz != 0, hasGM = s->gm;
+    for (int i = 0; i < s->n; i++) {
+        const P* p = &s->p[i];
+        if (hasMagnetic || hasExtBz) {
+            const double absBz = p->Bz > 0 ? p->Bz : -p->Bz;
+            if (absBz > 0) {
+                const double absQ = p->charge > 0 ? p->charge : -p->charge;
+                const double wc = absQ * absBz * p->invMass;
+                if (wc > *maxCyc) *maxCyc = wc;
+            }
+        }
+        if (hasGM) {
+            const double absBgz = p->Bgz > 0 ? p->Bgz : -p->Bgz;
+            if (absBgz > 0) { const double wc = 4 * absBgz; if (wc > *maxCyc) *maxCyc = wc; }
+        }
+    }
+}
+
+/* Physics.update (integrator.js:726-1825), the parts on the hot path. */
+int orc_update(OrcSys* s, double dt, int32_t* substeps) {
+    syncToggles(s);
+    const GeonParams* g = &s->prm;
+    const int n = s->n;
+    const int relOn = s->rel;
+    if (!s->forcesInit && n > 0) {                                        /* integrator.js:750-782 */
+        for (int i = 0; i < n; i++) s->p[i].angVel = relOn ? angwToAngVel(s->p[i].angw, s->p[i].radius) : s->p[i].angw;
+        forcePass(s);
+        s->forcesInit = 1;
+    }
+    const int hasGrav = s->grav, hasMagnetic = s->mag, hasGM = s->gm, hasExtBz = g->ext_bz != 0;
+    double dtRemain = dt;
+    int totalSteps = 0;
+    double prevMaxAccelSq = 0;
+    const int maxSub = g->max_substeps;
+    while (dtRemain > EPS_ && totalSteps < maxSub) {                      /* integrator.js:810 */
+        double maxAccelSq = 0, maxCyc = 0;
+        if (totalSteps == 0) {
+            for (int i = 0; i < n; i++) {
+                const P* p = &s->p[i];
+                const double aSq = (p->fx * p->fx + p->fy * p->fy) * p->invMass * p->invMass;
+                if (aSq > maxAccelSq) maxAccelSq = aSq;
+            }
+            cyclotronScan(s, &maxCyc);
+        } else {
+            maxAccelSq = prevMaxAccelSq;
+            if (hasMagnetic || hasExtBz || hasGM) cyclotronScan(s, &maxCyc);
+        }
+        double aMax = sqrt(maxAccelSq);
+        if (aMax != aMax) aMax = 0;
+        double dtSafe = aMax > EPS_ ? sqrt(s->softening / aMax) : dtRemain;   /* integrator.js:864 */
+        if (maxCyc > 0) {
+            const double dtC = 0.7853981633974483 / maxCyc;
+            if (dtC < dtSafe) dtSafe = dtC;
+        }
+        const int budgetLeft = maxSub - totalSteps;
+        double need = ceil(dtRemain / dtSafe);
+        const int stepsNeeded = need < budgetLeft ? (int)need : budgetLeft;
+        const double dtSub = dtRemain / stepsNeeded;
+        totalSteps++;
+        dtRemain -= dtSub;
+        const int has1PN = s->onePN;
+        const double halfDt = dtSub * 0.5;
+        const int needBoris = hasMagnetic || hasGM || hasExtBz;
+        double nextMaxAccelSq = 0;
+        for (int i = 0; i < n; i++) {                                     /* integrator.js:884-928 */
+            P* p = &s->p[i];
+            const double invM = p->invMass, halfDtOverM = halfDt * invM;
+            if (has1PN) { p->f1ox = p->f1x; p->f1oy = p->f1y; }
+            const double fx = p->fx, fy = p->fy;
+            p->wx += fx * halfDtOverM; p->wy += fy * halfDtOverM;
+            if (needBoris) {
+                double t = 0;
+                if (hasMagnetic || hasExtBz) t += (p->charge * 0.5 * invM) * p->Bz;
+                if (hasGM) t += 2 * p->Bgz;
+                if (t != 0) {
+                    const double gamma = relOn ? sqrt(1 + p->wx * p->wx + p->wy * p->wy) : 1;
+                    t *= dtSub / gamma;
+                    const double sB = 2 * t / (1 + t * t);
+                    const double wx = p->wx, wy = p->wy;
+                    const double wpx = wx + wy * t, wpy = wy - wx * t;
+                    p->wx = wx + wpy * sB; p->wy = wy - wpx * sB;
+                }
+            }
+            p->wx += fx * halfDtOverM; p->wy += fy * halfDtOverM;
+            if (p->wx != p->wx || p->wy != p->wy) { p->wx = p->wy = 0; p->vx = p->vy = 0; }
+            const double aSq = (fx * fx + fy * fy) * invM * invM;
+            if (aSq > nextMaxAccelSq) nextMaxAccelSq = aSq;
+        }
+        prevMaxAccelSq = nextMaxAccelSq;
+        const int needSpinOrbit = s->spinOrbit && (hasMagnetic || hasGM);     /* integrator.js:930-981 */
+        const int needTorques = (hasGM && relOn) || hasGrav;
+        if (needSpinOrbit || needTorques) {
+            for (int i = 0; i < n; i++) {
+                P* p = &s->p[i];
+                if (needSpinOrbit && fabs(p->angVel) >= EPS_) {
+                    const double IxOmega = p->angMomentum;
+                    if (fabs(IxOmega) >= EPS_) {
+                        const double dtOverM = dtSub * p->invMass;
+                        if (hasMagnetic && fabs(p->charge) > EPS_) {
+                            const double mu = p->magMoment;
+                            p->angw -= mu * (p->vx * p->dBzdx + p->vy * p->dBzdy) * dtSub / IxOmega;
+                            p->wx += mu * p->dBzdx * dtOverM; p->wy += mu * p->dBzdy * dtOverM;
+                        }
+                        if (hasGM) {
+                            const double L = p->angMomentum;
+                            p->angw -= L * (p->vx * p->dBgzdx + p->vy * p->dBgzdy) * dtSub / IxOmega;
+                            p->wx -= L * p->dBgzdx * dtOverM; p->wy -= L * p->dBgzdy * dtOverM;
+                        }
+                        if (p->angw != p->angw) p->angw = 0;
+                        const double sr = p->angw * p->radius;
+                        p->angVel = relOn ? p->angw / sqrt(1 + sr * sr) : p->angw;
+                    }
+                }
+                if (needTorques) {
+                    double torque = 0;
+                    if (hasGM && relOn) torque += p->frameDrag;
+                    if (hasGrav) torque += p->tidal;
+                    if (torque != 0) {
+                        const double I = INERTIA_K * p->mass * p->radiusSq;
+                        if (I >= EPS_) {
+                            p->angw += torque * dtSub / I;
+                            if (p->angw != p->angw) p->angw = 0;
+                            const double sr = p->angw * p->radius;
+                            p->angVel = relOn ? p->angw / sqrt(1 + sr * sr) : p->angw;
+                        }
+                    }
+                }
+            }
+        }
+        for (int i = 0; i < n; i++) {                                     /* integrator.js:1270-1278 */
+            P* p = &s->p[i];
+            const double invG = relOn ? 1 / sqrt(1 + p->wx * p->wx + p->wy * p->wy) : 1;
+            p->vx = p->wx * invG; p->vy = p->wy * invG;
+            p->angVel = relOn ? angwToAngVel(p->angw, p->radius) : p->angw;
+            p->x += p->vx * dtSub; p->y += p->vy * dtSub;
+        }
+        if (s->expansion) {                                               /* integrator.js:1281-1292 */
+            const double Hh = g->hubble, cx = g->domain_w * 0.5, cy = g->domain_h * 0.5;
+            for (int i = 0; i < n; i++) {
+                P* p = &s->p[i];
+                p->x += Hh * (p->x - cx) * dtSub; p->y += Hh * (p->y - cy) * dtSub;
+                const double decay = 1 - Hh * dtSub;
+                p->wx *= decay; p->wy *= decay;
+            }
+        }
+        s->simTime += dtSub;
+        if (has1PN) {                                                     /* integrator.js:1296-1318 */
+            if (relOn && s->expansion) {
+                for (int i = 0; i < n; i++) {
+                    P* p = &s->p[i];
+                    const double invG = 1 / sqrt(1 + p->wx * p->wx + p->wy * p->wy);
+                    p->vx = p->wx * invG; p->vy = p->wy * invG;
+                }
+            }
+            compute1PN(s);
+            for (int i = 0; i < n; i++) {
+                P* p = &s->p[i];
+                const double halfDtOverM = halfDt * p->invMass;
+                p->wx += (p->f1x - p->f1ox) * halfDtOverM;
+                p->wy += (p->f1y - p->f1oy) * halfDtOverM;
+                if (p->wx != p->wx || p->wy != p->wy) { p->wx = p->wy = 0; p->vx = p->vy = 0; }
+            }
+        }
+        forcePass(s);                                                     /* integrator.js:1489-1493 */
+    }
+    s->potentialEnergy = 0;
+    for (int i = 0; i < n; i++) s->potentialEnergy += s->p[i].pe;
+    s->potentialEnergy *= 0.5;                                            /* integrator.js:1743 */
+    /* Boundary handling once per update (integrator.js:1798-1824). */
+    int w = 0;
+    const double left = 0, top = 0, right = g->domain_w, bottom = g->domain_h;
+    for (int i = 0; i < s->n; i++) {
+        P* p = &s->p[i];
+        if (g->boundary_mode == GEON_BOUND_DESPAWN) {
+            if (p->x < left - DESPAWN_MARGIN || p->x > right + DESPAWN_MARGIN ||
+                p->y < top - DESPAWN_MARGIN || p->y > bottom + DESPAWN_MARGIN) continue;
+        } else if (g->boundary_mode == GEON_BOUND_LOOP) {
+            wrapPosition(p, g->topology, g->domain_w, g->domain_h);
+        } else if (g->boundary_mode == GEON_BOUND_BOUNCE) {
+            if (p->x < left) p->x = left; else if (p->x > right) p->x = right;
+            if (p->y < top) p->y = top; else if (p->y > bottom) p->y = bottom;
+        }
+        if (w != i) s->p[w] = *p;
+        w++;
+    }
+    s->n = w;
+    if (substeps) *substeps = totalSteps;
+    return 0;
+}
+
+/* computeEnergies (energy.js:11-195), particle passes 1-3; PE from the last
+ * force pass (integrator.js:1743). */
+int orc_stats(OrcSys* s, GeonStats* out) {
+    memset(out, 0, sizeof(*out));
+    const int n = s->n, relOn = s->rel;
+    double linearKE = 0, spinKE = 0, totalMass = 0, comX = 0, comY = 0, px = 0, py = 0;
+    for (int i = 0; i < n; i++) {
+        const P* p = &s->p[i];
+        const double pm = p->mass, rSq = p->radiusSq;
+        if (relOn) {
+            const double wSq = p->wx * p->wx + p->wy * p->wy;
+            const double gamma = sqrt(1 + wSq);
+            linearKE += wSq / (gamma + 1) * pm;
+            const double srSq = p->angw * p->angw * rSq;
+            const double gammaRot = sqrt(1 + srSq);
+            spinKE += INERTIA_K * pm * srSq / (gammaRot + 1);
+        } else {
+            const double speedSq = p->vx * p->vx + p->vy * p->vy;
+            linearKE += 0.5 * pm * speedSq;
+            spinKE += 0.5 * INERTIA_K * pm * rSq * p->angVel * p->angVel;
+        }
+        px += pm * p->wx; py += pm * p->wy;
+        totalMass += pm; comX += pm * p->x; comY += pm * p->y;
+    }
+    double orb = 0, spin = 0;
+    if (totalMass > 0) {
+        comX /= totalMass; comY /= totalMass;
+        for (int i = 0; i < n; i++) {
+            const P* p = &s->p[i];
+            const double dx = p->x - comX, dy = p->y - comY;
+            orb += dx * (p->mass * p->wy) - dy * (p->mass * p->wx);
+            spin += INERTIA_K * p->mass * p->radiusSq * p->angw;
+        }
+    }
+    double fieldEnergy = 0, fieldPx = 0, fieldPy = 0;
+    if (s->mag || s->gm) {                                                /* energy.js:64-142 */
+        const int emOn = s->mag && !s->onePN, gmOn = s->gm && !s->onePN;
+        const double W = s->prm.domain_w, H = s->prm.domain_h;
+        for (int i = 0; i < n; i++) {
+            const P* pi = &s->p[i];
+            for (int j = i + 1; j < n; j++) {
+                const P* pj = &s->p[j];
+                double dx, dy;
+                if (s->periodic) minImage(pi->x, pi->y, pj->x, pj->y, s->prm.topology, W, H, W * 0.5, H * 0.5, &dx, &dy);
+                else { dx = pj->x - pi->x; dy = pj->y - pi->y; }
+                const double rSq = dx * dx + dy * dy + s->softeningSq;
+                const double invR = 1 / sqrt(rSq);
+                const double rx = dx * invR, ry = dy * invR;
+                const double viDotVj = pi->vx * pj->vx + pi->vy * pj->vy;
+                const double viDotR = pi->vx * rx + pi->vy * ry;
+                const double vjDotR = pj->vx * rx + pj->vy * ry;
+                const double velTerm = viDotVj + viDotR * vjDotR;
+                const double sumVx = pi->vx + pj->vx, sumVy = pi->vy + pj->vy;
+                const double svDotR = sumVx * rx + sumVy * ry;
+                if (s->mag) {
+                    const double qqInvR = pi->charge * pj->charge * invR * sqrt(pi->axMod * pj->axMod);
+                    if (emOn) fieldEnergy -= 0.5 * qqInvR * velTerm;
+                    const double coeff = qqInvR * 0.5;
+                    fieldPx += coeff * (sumVx + rx * svDotR);
+                    fieldPy += coeff * (sumVy + ry * svDotR);
+                }
+                if (s->gm) {
+                    const double mmInvR = pi->mass * pj->mass * invR;
+                    if (gmOn) fieldEnergy += 0.5 * mmInvR * velTerm;
+                    const double coeff = mmInvR * 0.5;
+                    fieldPx -= coeff * (sumVx + rx * svDotR);
+                    fieldPy -= coeff * (sumVy + ry * svDotR);
+                }
+                if (s->mag && s->gm && !s->onePN) {
+                    const double invRSq = invR * invR;
+                    const double cc = pi->charge * pj->charge * (pi->mass + pj->mass)
+                        - (pi->charge * pi->charge * pj->mass + pj->charge * pj->charge * pi->mass);
+                    fieldEnergy += 0.5 * cc * invRSq;
+                }
+            }
+        }
+    }
+    out->linear_ke = linearKE; out->spin_ke = spinKE; out->px = px; out->py = py;
+    out->orbital_ang_mom = orb; out->spin_ang_mom = spin;
+    out->com_x = comX; out->com_y = comY; out->total_mass = totalMass;
+    out->pe = s->potentialEnergy;
+    out->field_energy = fieldEnergy; out->field_px = fieldPx; out->field_py = fieldPy;
+    out->sim_time = s->simTime;
+    out->alive_count = n;
+    return 0;
+}
+
+int64_t orc_build_tree(OrcSys* s) {
+    syncToggles(s);
+    buildTree(s);
+    return s->pool.count;
+}
+
+int64_t orc_get_tree(OrcSys* s, GeonTreeNode* out, int64_t cap) {
+    const Pool* q = &s->pool;
+    int64_t m = q->count < cap ? q->count : cap;
+    for (int64_t i = 0; i < m; i++) {
+        out[i].path = q->path[i];
+        out[i].level = q->level[i];
+        out[i].divided = q->divided[i];
+        out[i].count = q->subCount[i];
+        out[i].charge_quanta = q->qq[i];
+        out[i].mass = q->mass[i];
+        out[i].com_x = q->comX[i];
+        out[i].com_y = q->comY[i];
+    }
+    return q->count;
+}
+
+void orc_counters(OrcSys* s, int64_t* out3) { out3[0] = s->cnt_pair; out3[1] = s->cnt_node; out3[2] = s->cnt_visit; }
+
+static int cmp_pair(const void* a, const void* b) {
+    const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* Overlap pairs found by tree query (collisions.js:35-60, quadtree.js:372-404). */
+int64_t orc_collision_pairs(OrcSys* s, int32_t* a, int32_t* b, int64_t cap) {
+    orc_build_tree(s);
+    const Pool* q = &s->pool;
+    const double W = s->prm.domain_w, H = s->prm.domain_h;
+    int64_t np = 0, pcap = 1024;
+    int64_t* pairs = (int64_t*)malloc(sizeof(int64_t) * (size_t)pcap);
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(q->count + 8));
+    for (int ci = 0; ci < s->n; ci++) {
+        const P* p1 = &s->p[ci];
+        if (p1->mass == 0) continue;
+        const double rx = p1->x, ry = p1->y, rw = p1->radius * 2, rh = p1->radius * 2;
+        int top = 0;
+        stack[top++] = s->root;
+        while (top > 0) {
+            int ni = stack[--top];
+            if (rx - rw > q->bx[ni] + q->bw[ni] || rx + rw < q->bx[ni] - q->bw[ni] ||
+                ry - rh > q->by[ni] + q->bh[ni] || ry + rh < q->by[ni] - q->bh[ni]) continue;
+            if (!q->divided[ni]) {
+                size_t base = (size_t)ni * QT_CAP;
+                for (int k = 0; k < q->pointCount[ni]; k++) {
+                    const P* p2 = &s->tp[q->points[base + k]];
+                    if (!(p2->x >= rx - rw && p2->x <= rx + rw && p2->y >= ry - rh && p2->y <= ry + rh)) continue;
+                    const P* real2 = &s->p[p2->original];
+                    if (p2->original == ci || real2->mass == 0 || p1->id >= real2->id) continue;
+                    double dx, dy;
+                    if (s->periodic) minImage(p1->x, p1->y, p2->x, p2->y, s->prm.topology, W, H, W * 0.5, H * 0.5, &dx, &dy);
+                    else { dx = p2->x - p1->x; dy = p2->y - p1->y; }
+                    const double minDist = p1->radius + real2->radius;
+                    if (dx * dx + dy * dy < minDist * minDist) {
+                        if (np == pcap) { pcap *= 2; pairs = (int64_t*)realloc(pairs, sizeof(int64_t) * (size_t)pcap); }
+                        pairs[np++] = ((int64_t)p1->id << 32) | (uint32_t)real2->id;
+                    }
+                }
+            } else {
+                stack[top++] = q->nw[ni]; stack[top++] = q->ne[ni];
+                stack[top++] = q->sw[ni]; stack[top++] = q->se[ni];
+            }
+        }
+    }
+    qsort(pairs, (size_t)np, sizeof(int64_t), cmp_pair);
+    for (int64_t k = 0; k < np && k < cap; k++) { a[k] = (int32_t)(pairs[k] >> 32); b[k] = (int32_t)(pairs[k] & 0xffffffff); }
+    free(pairs); free(stack);
+    return np;
+}
+
+/* FP64 direct sum over all sources for selected targets (forces.js:93-126),
+ * min-image when periodic.  Leaves s->p force fields of non-target particles
+ * untouched; writes totals into fx/fy (indexed like idx). */
+int orc_direct_forces(OrcSys* s, const int32_t* idx, int32_t n_idx, double* fx, double* fy) {
+    syncToggles(s);
+    const int n = s->n;
+    for (int i = 0; i < n; i++) {
+        P* p = &s->p[i];
+        p->magMoment = MAG_MOMENT_K * p->charge * p->angVel * p->bodyRadiusSq;
+        p->angMomentum = INERTIA_K * p->mass * p->angVel * p->bodyRadiusSq;
+    }
+    const int m = idx ? n_idx : n;
+    #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 8)
+    for (int k = 0; k < m; k++) {
+        const int i = idx ? idx[k] : k;
+        P tmp = s->p[i];
+        tmp.fx = tmp.fy = 0; tmp.f1x = tmp.f1y = 0; tmp.pe = 0;
+        tmp.Bz = tmp.dBzdx = tmp.dBzdy = tmp.Bgz = tmp.dBgzdx = tmp.dBgzdy = 0;
+        tmp.frameDrag = tmp.tidal = 0;
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            const P* o = &s->p[j];
+            pairForce(s, &tmp, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
+                      o->magMoment, o->angMomentum, o->axMod, o->yukMod, o->higgsMod, 1);
+        }
+        fx[k] = tmp.fx; fy[k] = tmp.fy;
+    }
+    return 0;
+}

@@ -1,0 +1,63 @@
+/*
+ * geon_oracle.h — FP64 CPU restatement of the reference's CPU physics path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP backend is
+ * compared against (and the timed CPU baseline in bench.py).  Nothing in the
+ * product path (physsim_amd/) links, loads or calls it.
+ *
+ * It follows src/integrator.js, src/forces.js, src/quadtree.js,
+ * src/topology.js, src/relativity.js, src/energy.js and src/collisions.js of
+ * a9lim/physsim; each function in geon_oracle.c cites the lines it restates.
+ */
+#ifndef GEON_ORACLE_H
+#define GEON_ORACLE_H
+#include <stdint.h>
+#include "../include/geon.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OrcSys OrcSys;
+
+OrcSys* orc_create(const GeonParams* prm, int32_t capacity);
+void    orc_destroy(OrcSys* s);
+int     orc_set_params(OrcSys* s, const GeonParams* prm);
+/* Replace the particle array (main.js:504-539 normalization applied). */
+int     orc_upload(OrcSys* s, int32_t n, const double* x, const double* y,
+                   const double* wx, const double* wy, const double* mass,
+                   const double* charge, const double* angw);
+int32_t orc_count(OrcSys* s);
+/* Read state in array order.  ids[i] is the upload index of particle i. */
+int     orc_download(OrcSys* s, double* x, double* y, double* wx, double* wy,
+                     double* mass, double* angw, int32_t* ids);
+/* Bootstrap force pass (integrator.js:750-782). */
+int     orc_compute_forces(OrcSys* s);
+/* Forces in array order; any pointer may be NULL. */
+int     orc_get_forces(OrcSys* s, double* fx, double* fy, double* bz, double* bgz,
+                       double* dbzdx, double* dbzdy, double* dbgzdx, double* dbgzdy,
+                       double* frame_drag, double* tidal, double* pe_total);
+/* One Physics.update call (integrator.js:726-1825). */
+int     orc_update(OrcSys* s, double dt, int32_t* substeps);
+int     orc_stats(OrcSys* s, GeonStats* out);
+/* Tree at the current positions (quadtree.js:412-419) in canonical form. */
+int64_t orc_build_tree(OrcSys* s);
+int64_t orc_get_tree(OrcSys* s, GeonTreeNode* out, int64_t cap);
+/* Counters of the last force pass: [pair, node, visited]. */
+void    orc_counters(OrcSys* s, int64_t* out3);
+/* Collision pair list (collisions.js:35-60): pairs (i<j by id) whose bodies
+ * overlap, found by tree query.  Returns the number of pairs; writes at most
+ * cap pairs as (a[k], b[k]) upload indices, sorted. */
+int64_t orc_collision_pairs(OrcSys* s, int32_t* a, int32_t* b, int64_t cap);
+
+/* Stand-alone FP64 direct sum of gravity+Coulomb+... (forces.js:93-126) over
+ * `n` sources for the targets listed in `idx` (all if idx == NULL).  Uses the
+ * system's toggles; min-image when the boundary is loop.  Multithreaded. */
+int     orc_direct_forces(OrcSys* s, const int32_t* idx, int32_t n_idx,
+                          double* fx, double* fy);
+void    orc_set_threads(int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,115 @@
+"""ctypes mirror of include/geon.h (structs, constants, prototypes)."""
+import ctypes as C
+
+ABI_VERSION = 1
+
+OK, EINVAL, EFULL, EDEVICE, ENOMEM, ESTATE, EUNSUPPORTED, EOVERFLOW = 0, -1, -2, -3, -4, -5, -6, -7
+
+GRAVITY, COULOMB, MAGNETIC, GRAVITOMAG = 1 << 0, 1 << 1, 1 << 2, 1 << 3
+ONE_PN, RELATIVITY, SPIN_ORBIT, RADIATION = 1 << 4, 1 << 5, 1 << 6, 1 << 7
+BLACK_HOLE, DISINTEGRATION, EXPANSION, YUKAWA = 1 << 8, 1 << 9, 1 << 10, 1 << 11
+HIGGS, AXION, BARNES_HUT, BOSON_INTER = 1 << 12, 1 << 13, 1 << 14, 1 << 15
+
+COL_PASS, COL_MERGE, COL_BOUNCE = 0, 1, 2
+BOUND_DESPAWN, BOUND_BOUNCE, BOUND_LOOP = 0, 1, 2
+TORUS, KLEIN, RP2 = 0, 1, 2
+PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS = 0, 1
+
+TOGGLE_BITS = {
+    "gravityEnabled": GRAVITY, "coulombEnabled": COULOMB, "magneticEnabled": MAGNETIC,
+    "gravitomagEnabled": GRAVITOMAG, "onePNEnabled": ONE_PN, "relativityEnabled": RELATIVITY,
+    "spinOrbitEnabled": SPIN_ORBIT, "radiationEnabled": RADIATION, "blackHoleEnabled": BLACK_HOLE,
+    "disintegrationEnabled": DISINTEGRATION, "expansionEnabled": EXPANSION, "yukawaEnabled": YUKAWA,
+    "higgsEnabled": HIGGS, "axionEnabled": AXION, "barnesHutEnabled": BARNES_HUT,
+    "bosonInterEnabled": BOSON_INTER,
+}
+
+
+class GeonParams(C.Structure):
+    _fields_ = [
+        ("toggles0", C.c_uint32), ("boundary_mode", C.c_int32), ("topology", C.c_int32),
+        ("collision_mode", C.c_int32), ("periodic_mode", C.c_int32), ("max_substeps", C.c_int32),
+        ("domain_w", C.c_double), ("domain_h", C.c_double), ("softening", C.c_double),
+        ("bh_softening", C.c_double), ("theta", C.c_double), ("yukawa_coupling", C.c_double),
+        ("yukawa_mu", C.c_double), ("higgs_mass", C.c_double), ("axion_mass", C.c_double),
+        ("hubble", C.c_double), ("ext_gravity", C.c_double), ("ext_gravity_angle", C.c_double),
+        ("ext_electric", C.c_double), ("ext_electric_angle", C.c_double), ("ext_bz", C.c_double),
+        ("field_grid", C.c_int32), ("kahan", C.c_int32),
+    ]
+
+
+class GeonParticleInit(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("wx", C.c_double), ("wy", C.c_double),
+                ("mass", C.c_double), ("charge", C.c_double), ("angw", C.c_double),
+                ("antimatter", C.c_int32)]
+
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+class GeonSoAView(C.Structure):
+    _fields_ = [("x", _DP), ("y", _DP), ("wx", _DP), ("wy", _DP), ("mass", _DP),
+                ("base_mass", _DP), ("charge", _DP), ("angw", _DP), ("alive", _IP)]
+
+
+class GeonForceView(C.Structure):
+    _fields_ = [("fx", _DP), ("fy", _DP), ("bz", _DP), ("dbzdx", _DP), ("dbzdy", _DP),
+                ("bgz", _DP), ("dbgzdx", _DP), ("dbgzdy", _DP), ("frame_drag_torque", _DP),
+                ("tidal_torque", _DP), ("pe", _DP), ("f1pn_x", _DP), ("f1pn_y", _DP)]
+
+
+class GeonStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "linear_ke", "spin_ke", "px", "py", "orbital_ang_mom", "spin_ang_mom", "com_x", "com_y",
+        "total_mass", "pe", "field_energy", "higgs_field_energy", "axion_field_energy", "pfi_energy",
+        "field_px", "field_py", "sim_time")] + [("alive_count", C.c_int32)]
+
+
+class GeonCounters(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "pair_interactions", "node_interactions", "nodes_visited", "tree_nodes", "tree_divided",
+        "depth_overflow", "substeps_total")]
+
+
+class GeonTreeNode(C.Structure):
+    _fields_ = [("path", C.c_uint64), ("level", C.c_int32), ("divided", C.c_int32),
+                ("count", C.c_int32), ("charge_quanta", C.c_int32), ("mass", C.c_double),
+                ("com_x", C.c_double), ("com_y", C.c_double)]
+
+
+H = C.c_void_p
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "geon_abi_version": (C.c_int, []),
+    "geon_default_params": (None, [C.POINTER(GeonParams)]),
+    "geon_create": (C.c_int, [C.POINTER(GeonParams), C.c_int32, C.POINTER(H)]),
+    "geon_destroy": (None, [H]),
+    "geon_reset": (C.c_int, [H]),
+    "geon_set_params": (C.c_int, [H, C.POINTER(GeonParams)]),
+    "geon_get_params": (C.c_int, [H, C.POINTER(GeonParams)]),
+    "geon_add_particle": (C.c_int32, [H, C.POINTER(GeonParticleInit)]),
+    "geon_remove_particle": (C.c_int, [H, C.c_int32]),
+    "geon_patch_mass": (C.c_int, [H, C.c_int32, C.c_double, C.c_double]),
+    "geon_upload_soa": (C.c_int, [H, C.POINTER(GeonSoAView), C.c_int32]),
+    "geon_download_soa": (C.c_int, [H, C.POINTER(GeonSoAView), C.POINTER(C.c_int32)]),
+    "geon_update": (C.c_int, [H, C.c_double, C.POINTER(C.c_int32)]),
+    "geon_compute_forces": (C.c_int, [H]),
+    "geon_get_forces": (C.c_int, [H, C.POINTER(GeonForceView), C.c_int32]),
+    "geon_get_stats": (C.c_int, [H, C.POINTER(GeonStats)]),
+    "geon_build_tree": (C.c_int, [H]),
+    "geon_get_tree": (C.c_int, [H, C.POINTER(GeonTreeNode), C.c_int64, C.POINTER(C.c_int64)]),
+    "geon_counters": (C.c_int, [H, C.POINTER(GeonCounters)]),
+    "geon_timing_enable": (C.c_int, [H, C.c_int32]),
+    "geon_kernel_timing": (C.c_int, [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "geon_last_error": (C.c_char_p, [H]),
+}
+
+
+def bind(lib):
+    for name, (res, args) in PROTOTYPES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib

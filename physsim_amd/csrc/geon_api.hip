@@ -1,0 +1,798 @@
+// geon_api.hip — the C ABI (include/geon.h) over the gfx950 kernels.
+//
+// One GeonContext owns one particle system in device memory and one HIP
+// stream.  The substep loop of Physics.update (integrator.js:726-1825) runs as
+// a chain of kernels; the adaptive-dt controller runs on the device (k_ctrl),
+// and the host only reads the controller's 64-byte record, one substep ahead,
+// to know whether to enqueue another substep.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "../../include/geon.h"
+#include "geon_kernels.h"
+
+using namespace geon;
+
+namespace {
+
+// State columns: gathered into Morton order every time the tree is rebuilt.
+enum StateCol {
+    C_X, C_Y, C_WX, C_WY, C_VX, C_VY, C_MASS, C_BASEMASS, C_CHARGE, C_ANGW, C_ANGVEL, C_RADIUS,
+    C_MU, C_L, C_F1OX, C_F1OY, C_AX, C_YUK, C_HIGGS, C_QQ, C_SLOT, N_STATE
+};
+// Force columns: results of the last force pass, kept between updates.
+enum ForceCol {
+    K_FX, K_FY, K_PE, K_BZ, K_DBZDX, K_DBZDY, K_BGZ, K_DBGZDX, K_DBGZDY, K_FDRAG, K_TIDAL, K_F1X, K_F1Y, N_FORCE
+};
+
+constexpr int kTimers = 4;
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct GeonContext {
+    GeonParams prm;
+    int cap = 0;
+    int n_host = 0;             // live particles (host mirror, exact between calls)
+    int next_slot = 0;
+    hipStream_t stream = nullptr;
+    uint32_t* st[N_STATE] = {};
+    uint32_t* st_alt[N_STATE] = {};
+    float* fc[N_FORCE] = {};
+    float* fc_alt[N_FORCE] = {};
+    int32_t* alive = nullptr;
+    int32_t* alive_scan = nullptr;
+    uint32_t* compact_perm = nullptr;
+    float4 *src4 = nullptr, *src8a = nullptr, *src8b = nullptr;
+    Ctrl* ctrl = nullptr;
+    Ctrl* ctrl_host = nullptr;  // pinned
+    hipEvent_t ctrl_ev = nullptr;
+    TreeBufs tb = {};
+    double* stat_partial = nullptr;
+    double* stat_out = nullptr;
+    bool forces_init = false;
+    bool tree_valid = false;
+    int64_t substeps_total = 0;
+    double sim_time = 0;
+    std::string err;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kTimers];
+    size_t ev_used[kTimers] = {};
+    int64_t last_counters[4] = {};
+};
+
+#define HIP_OK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            h->err = std::string(#expr) + ": " + hipGetErrorString(e_);                      \
+            return GEON_EDEVICE;                                                             \
+        }                                                                                    \
+    } while (0)
+
+namespace {
+
+float* F(GeonContext* h, int c) { return reinterpret_cast<float*>(h->st[c]); }
+int32_t* I(GeonContext* h, int c) { return reinterpret_cast<int32_t*>(h->st[c]); }
+
+int set_err(GeonContext* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+struct Timer {
+    GeonContext* h;
+    int k;
+    hipEvent_t stop = nullptr;
+    Timer(GeonContext* h_, int k_) : h(h_), k(k_) {
+        if (!h->timing) return;
+        auto& v = h->ev[k];
+        if (h->ev_used[k] == v.size()) {
+            hipEvent_t a, b;
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            v.push_back({a, b});
+        }
+        auto& pr = v[h->ev_used[k]++];
+        hipEventRecord(pr.first, h->stream);
+        stop = pr.second;
+    }
+    ~Timer() {
+        if (stop) hipEventRecord(stop, h->stream);
+    }
+};
+
+uint32_t validate(const GeonParams* p, std::string* why) {
+    const uint32_t t = p->toggles0;
+    const uint32_t bad = GEON_RADIATION | GEON_DISINTEGRATION | GEON_BOSON_INTER | GEON_BLACK_HOLE | GEON_HIGGS |
+                         GEON_AXION | GEON_ONE_PN;
+    if (t & bad) { *why = "toggle not run by this backend (radiation, disintegration, bosons, black hole, Higgs, axion, 1PN)"; return 1; }
+    if (p->collision_mode != GEON_COL_PASS) { *why = "only collision mode 'pass' is run"; return 1; }
+    if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
+    if (p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) { *why = "only the torus topology is run"; return 1; }
+    if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode != GEON_PERIODIC_MIN_IMAGE) { *why = "only min-image periodic mode is run"; return 1; }
+    if (!(p->domain_w > 0) || !(p->domain_h > 0)) { *why = "domain must be positive"; return 1; }
+    if (p->max_substeps < 1) { *why = "max_substeps must be >= 1"; return 1; }
+    return 0;
+}
+
+PhysFlags phys_flags(const GeonParams& p) {
+    PhysFlags f;
+    const uint32_t t = p.toggles0;
+    f.mag = !!(t & GEON_MAGNETIC);
+    f.gm = !!(t & GEON_GRAVITOMAG);
+    f.one_pn = !!(t & GEON_ONE_PN);
+    f.yuk = !!(t & GEON_YUKAWA);
+    f.higgs = !!(t & GEON_HIGGS);
+    f.ax_mod = ((t & GEON_COULOMB) || (t & GEON_MAGNETIC)) && (t & GEON_AXION);
+    const double soft = (t & GEON_BLACK_HOLE) ? p.bh_softening : p.softening;
+    f.soft2 = (float)(soft * soft);
+    f.yuk_g = (float)p.yukawa_coupling;
+    f.yuk_mu = (float)p.yukawa_mu;
+    const double mu_min = f.higgs ? p.yukawa_mu * 0.05 : p.yukawa_mu;
+    f.yuk_cut2 = f.yuk ? (float)((6.0 / mu_min) * (6.0 / mu_min)) : INFINITY;
+    f.theta2 = (float)(p.theta * p.theta);
+    f.dom_w = (float)p.domain_w;
+    f.dom_h = (float)p.domain_h;
+    f.periodic = p.boundary_mode == GEON_BOUND_LOOP;
+    return f;
+}
+
+uint32_t feature_mask(const GeonParams& p) {
+    const uint32_t t = p.toggles0;
+    uint32_t f = 0;
+    if (t & GEON_GRAVITY) f |= F_GRAV;
+    if (t & GEON_COULOMB) f |= F_COUL;
+    if (t & (GEON_MAGNETIC | GEON_GRAVITOMAG | GEON_YUKAWA)) f |= F_EXTRA;
+    return f;
+}
+
+KeyParams key_params(const GeonParams& p) {
+    KeyParams k;
+    const double W = p.domain_w, H = p.domain_h;
+    k.xmin = W / 2 - 2 * W;
+    k.ymin = H / 2 - 2 * H;
+    k.scale_x = (double)(1u << kKeyLevels) / (4 * W);
+    k.scale_y = (double)(1u << kKeyLevels) / (4 * H);
+    k.root_hw = 2 * W;
+    return k;
+}
+
+StepArgs step_args(GeonContext* h) {
+    StepArgs a;
+    a.x = F(h, C_X); a.y = F(h, C_Y); a.wx = F(h, C_WX); a.wy = F(h, C_WY);
+    a.vx = F(h, C_VX); a.vy = F(h, C_VY); a.angw = F(h, C_ANGW); a.ang_vel = F(h, C_ANGVEL);
+    a.mass = F(h, C_MASS); a.radius = F(h, C_RADIUS); a.charge = F(h, C_CHARGE);
+    a.mu = F(h, C_MU); a.L = F(h, C_L);
+    a.f1x = h->fc[K_F1X]; a.f1y = h->fc[K_F1Y]; a.f1ox = F(h, C_F1OX); a.f1oy = F(h, C_F1OY);
+    a.fx = h->fc[K_FX]; a.fy = h->fc[K_FY]; a.bz = h->fc[K_BZ]; a.bgz = h->fc[K_BGZ];
+    a.dbzdx = h->fc[K_DBZDX]; a.dbzdy = h->fc[K_DBZDY]; a.dbgzdx = h->fc[K_DBGZDX]; a.dbgzdy = h->fc[K_DBGZDY];
+    a.frame_drag = h->fc[K_FDRAG]; a.tidal = h->fc[K_TIDAL];
+    a.alive = h->alive;
+    a.ctrl = h->ctrl;
+    const uint32_t t = h->prm.toggles0;
+    a.rel = !!(t & GEON_RELATIVITY);
+    a.mag = !!(t & GEON_MAGNETIC);
+    a.gm = !!(t & GEON_GRAVITOMAG);
+    a.ext_bz_on = h->prm.ext_bz != 0;
+    a.grav = !!(t & GEON_GRAVITY);
+    a.spin_orbit = !!(t & GEON_SPIN_ORBIT);
+    a.one_pn = !!(t & GEON_ONE_PN);
+    a.expansion = !!(t & GEON_EXPANSION);
+    a.hubble = (float)h->prm.hubble;
+    a.cx = (float)(h->prm.domain_w * 0.5);
+    a.cy = (float)(h->prm.domain_h * 0.5);
+    a.n_upper = h->n_host;
+    return a;
+}
+
+ForceArgs force_args(GeonContext* h) {
+    ForceArgs a;
+    a.src4 = h->src4; a.src8a = h->src8a; a.src8b = h->src8b;
+    a.radius = F(h, C_RADIUS);
+    a.nodes = h->tb.nodes;
+    a.fx = h->fc[K_FX]; a.fy = h->fc[K_FY]; a.pe = h->fc[K_PE];
+    a.bz = h->fc[K_BZ]; a.dbzdx = h->fc[K_DBZDX]; a.dbzdy = h->fc[K_DBZDY];
+    a.bgz = h->fc[K_BGZ]; a.dbgzdx = h->fc[K_DBGZDX]; a.dbgzdy = h->fc[K_DBGZDY];
+    a.frame_drag = h->fc[K_FDRAG]; a.tidal = h->fc[K_TIDAL]; a.f1x = h->fc[K_F1X]; a.f1y = h->fc[K_F1Y];
+    a.mass_for_cyc = F(h, C_MASS);
+    a.charge = F(h, C_CHARGE);
+    a.ctrl = h->ctrl;
+    a.pf = phys_flags(h->prm);
+    const GeonParams& p = h->prm;
+    a.ext_gx = (float)(p.ext_gravity * cos(p.ext_gravity_angle));
+    a.ext_gy = (float)(p.ext_gravity * sin(p.ext_gravity_angle));
+    a.ext_ex = (float)(p.ext_electric * cos(p.ext_electric_angle));
+    a.ext_ey = (float)(p.ext_electric * sin(p.ext_electric_angle));
+    a.ext_bz = (float)p.ext_bz;
+    a.cyc_mag = a.pf.mag || p.ext_bz != 0;
+    a.cyc_gm = a.pf.gm;
+    a.n_upper = h->n_host;
+    a.max_nodes = 1 + 4 * h->tb.cap_divided;
+    return a;
+}
+
+__global__ void k_reset_counters(Ctrl* c) {
+    c->cnt_pair = 0; c->cnt_node = 0; c->cnt_visit = 0; c->depth_overflow = 0; c->overflow = 0;
+}
+
+__global__ void k_begin_update(Ctrl* c, double dt, int max_sub, double sim_time) {
+    c->dt_remain = dt; c->dt_total = dt; c->total_steps = 0; c->max_substeps = max_sub;
+    c->sim_time = sim_time; c->active = 0; c->deaths = 0;
+    c->max_acc2_fresh = 0; c->max_acc2_kick = 0; c->max_cyc = 0;
+}
+
+__global__ void k_set_n(Ctrl* c, int n) { c->n_alive = n; }
+
+__global__ void k_survivor_perm(const int32_t* alive, const int32_t* scan, uint32_t* perm, const Ctrl* c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c->n_alive && alive[i]) perm[scan[i]] = (uint32_t)i;
+}
+
+// Sort by Morton key and gather the state columns into that order.
+int sort_and_gather(GeonContext* h) {
+    const int n = h->n_host;
+    if (n == 0) return GEON_OK;
+    const KeyParams kp = key_params(h->prm);
+    {
+        Timer t(h, 2);
+        launch_keys(h->stream, F(h, C_X), F(h, C_Y), n, kp, h->tb.keys, h->tb.perm, h->ctrl);
+        radix_sort(h->stream, h->tb, n, 2 * kKeyLevels + 1, h->ctrl);
+        Columns cols;
+        cols.ncols = N_STATE;
+        for (int c = 0; c < N_STATE; c++) { cols.src[c] = h->st[c]; cols.dst[c] = h->st_alt[c]; }
+        launch_gather(h->stream, cols, h->tb.perm, n, h->ctrl);
+    }
+    for (int c = 0; c < N_STATE; c++) std::swap(h->st[c], h->st_alt[c]);
+    return GEON_OK;
+}
+
+// computeAllForces + external fields at the current state (forces.js:61-154,
+// integrator.js:563-592), preceded by the tree rebuild (integrator.js:1340-1348).
+int force_pass(GeonContext* h) {
+    const int n = h->n_host;
+    const bool bh = !!(h->prm.toggles0 & GEON_BARNES_HUT);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
+    if (n == 0) return GEON_OK;
+    if (bh) {
+        int rc = sort_and_gather(h);
+        if (rc) return rc;
+        Timer t(h, 1);
+        build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
+                   F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+        h->tree_valid = true;
+    }
+    launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
+                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), F(h, C_AX), F(h, C_YUK),
+                           F(h, C_HIGGS), h->src4, h->src8a, h->src8b, n, h->ctrl);
+    const ForceArgs fa = force_args(h);
+    {
+        Timer t(h, 0);
+        if (bh) launch_walk(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
+        else launch_direct(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
+    }
+    HIP_OK(hipGetLastError());
+    return GEON_OK;
+}
+
+int read_ctrl(GeonContext* h) {
+    HIP_OK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
+// Remove despawned particles, keeping the survivors' relative order
+// (integrator.js:1821-1823 compacts the array the same way).
+int compact(GeonContext* h) {
+    const int n = h->n_host;
+    int32_t* total = &h->ctrl->n_tree;    // scratch: n_tree is rebuilt before every use
+    exclusive_scan_i32(h->stream, h->alive, h->alive_scan, n, &h->ctrl->n_alive, total, h->tb.scan_tmp);
+    // perm[j] = old index of the j-th survivor
+    hipLaunchKernelGGL(k_survivor_perm, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->alive, h->alive_scan,
+                       h->compact_perm, h->ctrl);
+    HIP_OK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    const int survivors = h->ctrl_host->n_tree;
+    hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, survivors);
+    Columns cols;
+    cols.ncols = 0;
+    for (int c = 0; c < N_STATE; c++) { cols.src[cols.ncols] = h->st[c]; cols.dst[cols.ncols++] = h->st_alt[c]; }
+    for (int c = 0; c < N_FORCE; c++) {
+        cols.src[cols.ncols] = reinterpret_cast<uint32_t*>(h->fc[c]);
+        cols.dst[cols.ncols++] = reinterpret_cast<uint32_t*>(h->fc_alt[c]);
+    }
+    launch_gather(h->stream, cols, h->compact_perm, survivors, h->ctrl);
+    for (int c = 0; c < N_STATE; c++) std::swap(h->st[c], h->st_alt[c]);
+    for (int c = 0; c < N_FORCE; c++) std::swap(h->fc[c], h->fc_alt[c]);
+    h->n_host = survivors;
+    h->tree_valid = false;
+    return GEON_OK;
+}
+
+template <typename T>
+int dmalloc(GeonContext* h, T** p, size_t count) {
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) { h->err = std::string("hipMalloc: ") + hipGetErrorString(e); return GEON_ENOMEM; }
+    return GEON_OK;
+}
+
+int allocate(GeonContext* h) {
+    const size_t cap = (size_t)h->cap;
+    int rc;
+    for (int c = 0; c < N_STATE; c++) {
+        if ((rc = dmalloc(h, &h->st[c], cap))) return rc;
+        if ((rc = dmalloc(h, &h->st_alt[c], cap))) return rc;
+    }
+    for (int c = 0; c < N_FORCE; c++) {
+        if ((rc = dmalloc(h, &h->fc[c], cap))) return rc;
+        if ((rc = dmalloc(h, &h->fc_alt[c], cap))) return rc;
+    }
+    if ((rc = dmalloc(h, &h->alive, cap))) return rc;
+    if ((rc = dmalloc(h, &h->alive_scan, cap))) return rc;
+    if ((rc = dmalloc(h, &h->compact_perm, cap))) return rc;
+    if ((rc = dmalloc(h, &h->src4, cap))) return rc;
+    if ((rc = dmalloc(h, &h->src8a, cap))) return rc;
+    if ((rc = dmalloc(h, &h->src8b, cap))) return rc;
+    if ((rc = dmalloc(h, &h->ctrl, 1))) return rc;
+    TreeBufs& tb = h->tb;
+    tb.cap_particles = (int64_t)cap;
+    tb.cap_divided = (int64_t)cap + 1024;
+    const size_t nodes = 1 + 4 * (size_t)tb.cap_divided;
+    const size_t nblk = (cap + 2047) / 2048 + 1;
+    if ((rc = dmalloc(h, &tb.keys, cap))) return rc;
+    if ((rc = dmalloc(h, &tb.keys_alt, cap))) return rc;
+    if ((rc = dmalloc(h, &tb.perm, cap))) return rc;
+    if ((rc = dmalloc(h, &tb.perm_alt, cap))) return rc;
+    if ((rc = dmalloc(h, &tb.hist, 256 * nblk))) return rc;
+    if ((rc = dmalloc(h, &tb.owned, cap))) return rc;
+    if ((rc = dmalloc(h, &tb.rank_start, cap))) return rc;
+    if ((rc = dmalloc(h, &tb.divnode, (size_t)tb.cap_divided))) return rc;
+    if ((rc = dmalloc(h, &tb.pre, 8 * (cap + 1)))) return rc;
+    if ((rc = dmalloc(h, &tb.nodes, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_mass64, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_comx64, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_comy64, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_qq64, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_path, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_npts, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.scan_tmp, nblk + 1024))) return rc;
+    if ((rc = dmalloc(h, &tb.scan_tmp64, 8 * (nblk + 1024)))) return rc;
+    if ((rc = dmalloc(h, &h->stat_partial, 8 * 1024))) return rc;
+    if ((rc = dmalloc(h, &h->stat_out, 16))) return rc;
+    HIP_OK(hipHostMalloc((void**)&h->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
+    HIP_OK(hipMemset(h->ctrl, 0, sizeof(Ctrl)));
+    HIP_OK(hipEventCreateWithFlags(&h->ctrl_ev, hipEventDisableTiming));
+    return GEON_OK;
+}
+
+void release(GeonContext* h) {
+    for (int c = 0; c < N_STATE; c++) { hipFree(h->st[c]); hipFree(h->st_alt[c]); }
+    for (int c = 0; c < N_FORCE; c++) { hipFree(h->fc[c]); hipFree(h->fc_alt[c]); }
+    hipFree(h->alive); hipFree(h->alive_scan); hipFree(h->compact_perm);
+    hipFree(h->src4); hipFree(h->src8a); hipFree(h->src8b); hipFree(h->ctrl);
+    TreeBufs& tb = h->tb;
+    hipFree(tb.keys); hipFree(tb.keys_alt); hipFree(tb.perm); hipFree(tb.perm_alt); hipFree(tb.hist);
+    hipFree(tb.owned); hipFree(tb.rank_start); hipFree(tb.divnode); hipFree(tb.pre); hipFree(tb.nodes);
+    hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64); hipFree(tb.node_qq64);
+    hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
+    hipFree(h->stat_partial); hipFree(h->stat_out);
+    if (h->ctrl_host) hipHostFree(h->ctrl_host);
+    if (h->ctrl_ev) hipEventDestroy(h->ctrl_ev);
+    for (int k = 0; k < kTimers; k++)
+        for (auto& pr : h->ev[k]) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+    if (h->stream) hipStreamDestroy(h->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int geon_abi_version(void) { return GEON_ABI_VERSION; }
+
+void geon_default_params(GeonParams* p) {
+    memset(p, 0, sizeof(*p));
+    // Physics() defaults with barnesHut on (integrator.js:65-99), CPU constants (config.js)
+    p->toggles0 = GEON_GRAVITY | GEON_COULOMB | GEON_BARNES_HUT;
+    p->boundary_mode = GEON_BOUND_DESPAWN;
+    p->topology = GEON_TORUS;
+    p->collision_mode = GEON_COL_PASS;
+    p->periodic_mode = GEON_PERIODIC_MIN_IMAGE;
+    p->max_substeps = 32;
+    p->domain_w = 64; p->domain_h = 64;
+    p->softening = 8; p->bh_softening = 4;
+    p->theta = 0.5;
+    p->yukawa_coupling = 14; p->yukawa_mu = 0.15;
+    p->higgs_mass = 0.5; p->axion_mass = 0.05; p->hubble = 0.001;
+    p->ext_gravity_angle = M_PI * 0.5;
+    p->field_grid = 128;
+    p->kahan = 1;
+}
+
+int geon_create(const GeonParams* params, int32_t max_particles, GeonHandle* out) {
+    if (!params || !out || max_particles <= 0) { g_create_error = "bad argument"; return GEON_EINVAL; }
+    std::string why;
+    if (validate(params, &why)) { g_create_error = why; return GEON_EUNSUPPORTED; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { g_create_error = "no HIP device"; return GEON_EDEVICE; }
+    GeonContext* h = new GeonContext();
+    h->prm = *params;
+    h->cap = max_particles;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        delete h;
+        return GEON_EDEVICE;
+    }
+    int rc = allocate(h);
+    if (rc) { g_create_error = h->err; release(h); delete h; return rc; }
+    *out = h;
+    return GEON_OK;
+}
+
+void geon_destroy(GeonHandle h) {
+    if (!h) return;
+    hipStreamSynchronize(h->stream);
+    release(h);
+    delete h;
+}
+
+int geon_reset(GeonHandle h) {
+    if (!h) return GEON_EINVAL;
+    h->n_host = 0; h->next_slot = 0; h->forces_init = false; h->tree_valid = false;
+    h->sim_time = 0; h->substeps_total = 0;
+    hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
+int geon_set_params(GeonHandle h, const GeonParams* p) {
+    if (!h || !p) return GEON_EINVAL;
+    std::string why;
+    if (validate(p, &why)) return set_err(h, GEON_EUNSUPPORTED, why);
+    h->prm = *p;
+    h->forces_init = false;    // forces depend on the toggles: recompute at next update
+    return GEON_OK;
+}
+
+int geon_get_params(GeonHandle h, GeonParams* out) {
+    if (!h || !out) return GEON_EINVAL;
+    *out = h->prm;
+    return GEON_OK;
+}
+
+int geon_upload_soa(GeonHandle h, const GeonSoAView* v, int32_t n) {
+    if (!h || !v || n < 0) return GEON_EINVAL;
+    if (n > h->cap) return set_err(h, GEON_EFULL, "more particles than max_particles");
+    if (n > 0 && (!v->x || !v->y || !v->wx || !v->wy || !v->mass || !v->charge))
+        return set_err(h, GEON_EINVAL, "upload needs x, y, wx, wy, mass, charge");
+    const bool rel = !!(h->prm.toggles0 & GEON_RELATIVITY);
+    std::vector<std::vector<uint32_t>> col(N_STATE, std::vector<uint32_t>((size_t)n));
+    auto putf = [&](int c, int i, double val) { float f = (float)val; memcpy(&col[c][i], &f, 4); };
+    for (int i = 0; i < n; i++) {
+        const double m = v->mass[i];
+        if (!(m > 0)) return set_err(h, GEON_EINVAL, "mass must be > 0");
+        // main.js:511, gpu-physics.js:1874: charge quantized to multiples of 0.1
+        const int32_t qq = (int32_t)llround(v->charge[i] / 0.1);
+        const double q = qq * 0.1;
+        const double r = cbrt(m);   // particle.js:96
+        const double angw = v->angw ? v->angw[i] : 0.0;
+        const double wx = v->wx[i], wy = v->wy[i];
+        const double ig = rel ? 1.0 / sqrt(1 + wx * wx + wy * wy) : 1.0;
+        putf(C_X, i, v->x[i]); putf(C_Y, i, v->y[i]);
+        putf(C_WX, i, wx); putf(C_WY, i, wy);
+        putf(C_VX, i, wx * ig); putf(C_VY, i, wy * ig);
+        putf(C_MASS, i, m); putf(C_BASEMASS, i, v->base_mass ? v->base_mass[i] : m);
+        putf(C_CHARGE, i, q);
+        putf(C_ANGW, i, angw);
+        putf(C_ANGVEL, i, rel ? angw / sqrt(1 + angw * angw * r * r) : angw);
+        putf(C_RADIUS, i, r);
+        putf(C_MU, i, 0.0); putf(C_L, i, 0.0);          // particle.js:49-50: moments start at 0
+        putf(C_F1OX, i, 0.0); putf(C_F1OY, i, 0.0);
+        putf(C_AX, i, 1.0); putf(C_YUK, i, 1.0); putf(C_HIGGS, i, 1.0);
+        col[C_QQ][i] = (uint32_t)qq;
+        col[C_SLOT][i] = (uint32_t)i;
+    }
+    for (int c = 0; c < N_STATE; c++)
+        if (n) HIP_OK(hipMemcpyAsync(h->st[c], col[c].data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
+    for (int c = 0; c < N_FORCE; c++) if (n) HIP_OK(hipMemsetAsync(h->fc[c], 0, sizeof(float) * n, h->stream));
+    hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, n);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->n_host = n;
+    h->next_slot = n;
+    h->forces_init = false;
+    h->tree_valid = false;
+    h->sim_time = 0;
+    return GEON_OK;
+}
+
+int32_t geon_add_particle(GeonHandle h, const GeonParticleInit* p) {
+    if (!h || !p) return GEON_EINVAL;
+    if (h->n_host >= h->cap || h->next_slot >= h->cap) return GEON_EFULL;   // gpu-physics.js:1864
+    if (!(p->mass > 0)) return set_err(h, GEON_EINVAL, "mass must be > 0");
+    const bool rel = !!(h->prm.toggles0 & GEON_RELATIVITY);
+    uint32_t w[N_STATE];
+    auto putf = [&](int c, double val) { float f = (float)val; memcpy(&w[c], &f, 4); };
+    const int32_t qq = (int32_t)llround(p->charge / 0.1);
+    const double r = cbrt(p->mass);
+    const double ig = rel ? 1.0 / sqrt(1 + p->wx * p->wx + p->wy * p->wy) : 1.0;
+    putf(C_X, p->x); putf(C_Y, p->y); putf(C_WX, p->wx); putf(C_WY, p->wy);
+    putf(C_VX, p->wx * ig); putf(C_VY, p->wy * ig);
+    putf(C_MASS, p->mass); putf(C_BASEMASS, p->mass); putf(C_CHARGE, qq * 0.1);
+    putf(C_ANGW, p->angw);
+    putf(C_ANGVEL, rel ? p->angw / sqrt(1 + p->angw * p->angw * r * r) : p->angw);
+    putf(C_RADIUS, r); putf(C_MU, 0); putf(C_L, 0); putf(C_F1OX, 0); putf(C_F1OY, 0);
+    putf(C_AX, 1); putf(C_YUK, 1); putf(C_HIGGS, 1);
+    w[C_QQ] = (uint32_t)qq;
+    const int32_t slot = h->next_slot;
+    w[C_SLOT] = (uint32_t)slot;
+    const int i = h->n_host;
+    for (int c = 0; c < N_STATE; c++) HIP_OK(hipMemcpyAsync(h->st[c] + i, &w[c], 4, hipMemcpyHostToDevice, h->stream));
+    for (int c = 0; c < N_FORCE; c++) HIP_OK(hipMemsetAsync(h->fc[c] + i, 0, 4, h->stream));
+    hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, i + 1);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->n_host = i + 1;
+    h->next_slot = slot + 1;
+    h->forces_init = false;
+    h->tree_valid = false;
+    return slot;
+}
+
+static int find_slot(GeonHandle h, int32_t slot, int* idx) {
+    std::vector<int32_t> slots((size_t)h->n_host);
+    if (h->n_host)
+        HIP_OK(hipMemcpyAsync(slots.data(), h->st[C_SLOT], 4 * (size_t)h->n_host, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < h->n_host; i++) if (slots[i] == slot) { *idx = i; return GEON_OK; }
+    return set_err(h, GEON_EINVAL, "no live particle in that slot");
+}
+
+int geon_remove_particle(GeonHandle h, int32_t slot) {
+    if (!h) return GEON_EINVAL;
+    int idx;
+    int rc = find_slot(h, slot, &idx);
+    if (rc) return rc;
+    std::vector<int32_t> alive((size_t)h->n_host, 1);
+    alive[idx] = 0;
+    HIP_OK(hipMemcpyAsync(h->alive, alive.data(), 4 * alive.size(), hipMemcpyHostToDevice, h->stream));
+    rc = compact(h);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return rc;
+}
+
+int geon_patch_mass(GeonHandle h, int32_t slot, double mass, double base_mass) {
+    if (!h) return GEON_EINVAL;
+    if (!(mass > 0)) return set_err(h, GEON_EINVAL, "mass must be > 0");
+    int idx;
+    int rc = find_slot(h, slot, &idx);
+    if (rc) return rc;
+    const float m = (float)mass, bm = (float)base_mass, r = (float)cbrt(mass);
+    HIP_OK(hipMemcpyAsync(F(h, C_MASS) + idx, &m, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(F(h, C_BASEMASS) + idx, &bm, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(F(h, C_RADIUS) + idx, &r, 4, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
+int geon_download_soa(GeonHandle h, GeonSoAView* v, int32_t* n_slots) {
+    if (!h || !v) return GEON_EINVAL;
+    const int n = h->n_host, ns = h->next_slot;
+    std::vector<std::vector<uint32_t>> col(N_STATE, std::vector<uint32_t>((size_t)n));
+    for (int c = 0; c < N_STATE; c++)
+        if (n) HIP_OK(hipMemcpyAsync(col[c].data(), h->st[c], 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    auto getf = [&](int c, int i) { float f; memcpy(&f, &col[c][i], 4); return (double)f; };
+    if (v->alive) for (int s = 0; s < ns; s++) v->alive[s] = 0;
+    for (int i = 0; i < n; i++) {
+        const int s = (int)col[C_SLOT][i];
+        if (s < 0 || s >= ns) continue;
+        if (v->x) v->x[s] = getf(C_X, i);
+        if (v->y) v->y[s] = getf(C_Y, i);
+        if (v->wx) v->wx[s] = getf(C_WX, i);
+        if (v->wy) v->wy[s] = getf(C_WY, i);
+        if (v->mass) v->mass[s] = getf(C_MASS, i);
+        if (v->base_mass) v->base_mass[s] = getf(C_BASEMASS, i);
+        if (v->charge) v->charge[s] = (int32_t)col[C_QQ][i] * 0.1;
+        if (v->angw) v->angw[s] = getf(C_ANGW, i);
+        if (v->alive) v->alive[s] = 1;
+    }
+    if (n_slots) *n_slots = ns;
+    return GEON_OK;
+}
+
+int geon_compute_forces(GeonHandle h) {
+    if (!h) return GEON_EINVAL;
+    int rc = force_pass(h);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->forces_init = true;
+    return read_ctrl(h);
+}
+
+int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
+    if (!h) return GEON_EINVAL;
+    if (!(dt >= 0)) return set_err(h, GEON_EINVAL, "dt must be >= 0");
+    int rc;
+    if (!h->forces_init && h->n_host > 0) {          // integrator.js:750-782
+        if ((rc = force_pass(h))) return rc;
+        h->forces_init = true;
+    }
+    const int n = h->n_host;
+    hipLaunchKernelGGL(k_begin_update, dim3(1), dim3(1), 0, h->stream, h->ctrl, dt, h->prm.max_substeps, h->sim_time);
+    StepArgs sa = step_args(h);
+    const double soft = (h->prm.toggles0 & GEON_BLACK_HOLE) ? h->prm.bh_softening : h->prm.softening;
+    int steps = 0;
+    if (n > 0) {
+        launch_dt_scan(h->stream, sa);
+        for (int k = 0; k < h->prm.max_substeps; k++) {
+            launch_ctrl(h->stream, h->ctrl, soft, k == 0);
+            HIP_OK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+            HIP_OK(hipEventRecord(h->ctrl_ev, h->stream));
+            {
+                Timer t(h, 3);
+                sa = step_args(h);
+                launch_kick_drift(h->stream, sa);
+            }
+            if ((rc = force_pass(h))) return rc;
+            sa = step_args(h);
+            launch_dt_scan(h->stream, sa);
+            HIP_OK(hipEventSynchronize(h->ctrl_ev));
+            const Ctrl& c = *h->ctrl_host;
+            if (!c.active) break;          // nothing left (dt <= EPSILON)
+            steps = c.total_steps;
+            if (!(c.dt_remain > 1e-12) || c.total_steps >= c.max_substeps) break;
+        }
+    }
+    HIP_OK(hipGetLastError());
+    // boundary once per update (integrator.js:1798-1824)
+    sa = step_args(h);
+    launch_boundary(h->stream, sa, h->prm.boundary_mode, h->prm.domain_w, h->prm.domain_h);
+    if ((rc = read_ctrl(h))) return rc;
+    h->sim_time = h->ctrl_host->sim_time;
+    h->substeps_total += steps;
+    for (int k = 0; k < 3; k++) h->last_counters[k] = 0;
+    h->last_counters[0] = h->ctrl_host->cnt_pair;
+    h->last_counters[1] = h->ctrl_host->cnt_node;
+    h->last_counters[2] = h->ctrl_host->cnt_visit;
+    if (h->ctrl_host->overflow) return set_err(h, GEON_EOVERFLOW, "tree node pool or walk stack overflow");
+    if (h->ctrl_host->deaths > 0) {
+        if ((rc = compact(h))) return rc;
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+    if (substeps_taken) *substeps_taken = steps;
+    return GEON_OK;
+}
+
+int geon_get_forces(GeonHandle h, GeonForceView* v, int32_t n_slots) {
+    if (!h || !v) return GEON_EINVAL;
+    const int n = h->n_host;
+    std::vector<int32_t> slot((size_t)n);
+    std::vector<std::vector<float>> fc(N_FORCE, std::vector<float>((size_t)n));
+    if (n) {
+        HIP_OK(hipMemcpyAsync(slot.data(), h->st[C_SLOT], 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+        for (int c = 0; c < N_FORCE; c++)
+            HIP_OK(hipMemcpyAsync(fc[c].data(), h->fc[c], 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIP_OK(hipStreamSynchronize(h->stream));
+    double* outs[N_FORCE] = {v->fx, v->fy, v->pe, v->bz, v->dbzdx, v->dbzdy, v->bgz, v->dbgzdx, v->dbgzdy,
+                             v->frame_drag_torque, v->tidal_torque, v->f1pn_x, v->f1pn_y};
+    for (int i = 0; i < n; i++) {
+        const int s = slot[i];
+        if (s < 0 || s >= n_slots) continue;
+        for (int c = 0; c < N_FORCE; c++) if (outs[c]) outs[c][s] = fc[c][i];
+    }
+    return GEON_OK;
+}
+
+int geon_get_stats(GeonHandle h, GeonStats* out) {
+    if (!h || !out) return GEON_EINVAL;
+    memset(out, 0, sizeof(*out));
+    StepArgs sa = step_args(h);
+    double r[16] = {};
+    if (h->n_host > 0) {
+        launch_stats(h->stream, sa, h->fc[K_PE], h->stat_partial, h->stat_out, !!(h->prm.toggles0 & GEON_RELATIVITY));
+        HIP_OK(hipMemcpyAsync(r, h->stat_out, sizeof(double) * 10, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIP_OK(hipStreamSynchronize(h->stream));
+    out->linear_ke = r[0]; out->spin_ke = r[1]; out->px = r[2]; out->py = r[3];
+    out->total_mass = r[4];
+    out->com_x = r[4] > 0 ? r[5] / r[4] : 0; out->com_y = r[4] > 0 ? r[6] / r[4] : 0;
+    out->pe = 0.5 * r[7];                       // getPEAccum halves the double-counted sum (forces.js:26)
+    out->orbital_ang_mom = r[8]; out->spin_ang_mom = r[9];
+    out->sim_time = h->sim_time;
+    out->alive_count = h->n_host;
+    return GEON_OK;
+}
+
+int geon_build_tree(GeonHandle h) {
+    if (!h) return GEON_EINVAL;
+    if (h->n_host == 0) return GEON_OK;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
+    int rc = sort_and_gather(h);
+    if (rc) return rc;
+    build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
+               I(h, C_QQ), h->n_host, key_params(h->prm), h->ctrl);
+    h->tree_valid = true;
+    return read_ctrl(h);
+}
+
+int geon_get_tree(GeonHandle h, GeonTreeNode* out, int64_t cap, int64_t* n_nodes) {
+    if (!h) return GEON_EINVAL;
+    if (!h->tree_valid) return set_err(h, GEON_ESTATE, "no tree built");
+    int rc = read_ctrl(h);
+    if (rc) return rc;
+    const int64_t nn = h->n_host == 0 ? 0 : 1 + 4 * (int64_t)h->ctrl_host->n_divided;
+    if (n_nodes) *n_nodes = nn;
+    if (!out) return GEON_OK;
+    const int64_t m = std::min(cap, nn);
+    if (m <= 0) return GEON_OK;
+    std::vector<Node> nodes((size_t)m);
+    std::vector<double> mass((size_t)m), cx((size_t)m), cy((size_t)m), qq((size_t)m);
+    std::vector<uint64_t> path((size_t)m);
+    std::vector<int32_t> npts((size_t)m);
+    HIP_OK(hipMemcpyAsync(nodes.data(), h->tb.nodes, sizeof(Node) * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(mass.data(), h->tb.node_mass64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(cx.data(), h->tb.node_comx64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(cy.data(), h->tb.node_comy64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(qq.data(), h->tb.node_qq64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(path.data(), h->tb.node_path, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(npts.data(), h->tb.node_npts, 4 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int64_t i = 0; i < m; i++) {
+        out[i].path = path[i];
+        out[i].level = nodes[i].level;
+        out[i].divided = nodes[i].count < 0 ? 1 : 0;
+        out[i].count = npts[i];
+        out[i].charge_quanta = (int32_t)llround(qq[i]);
+        out[i].mass = mass[i];
+        out[i].com_x = cx[i];
+        out[i].com_y = cy[i];
+    }
+    return GEON_OK;
+}
+
+int geon_counters(GeonHandle h, GeonCounters* out) {
+    if (!h || !out) return GEON_EINVAL;
+    int rc = read_ctrl(h);
+    if (rc) return rc;
+    memset(out, 0, sizeof(*out));
+    out->pair_interactions = h->ctrl_host->cnt_pair;
+    out->node_interactions = h->ctrl_host->cnt_node;
+    out->nodes_visited = h->ctrl_host->cnt_visit;
+    out->tree_divided = h->ctrl_host->n_divided;
+    out->tree_nodes = h->tree_valid ? 1 + 4 * (int64_t)h->ctrl_host->n_divided : 0;
+    out->depth_overflow = h->ctrl_host->depth_overflow;
+    out->substeps_total = h->substeps_total;
+    return GEON_OK;
+}
+
+int geon_timing_enable(GeonHandle h, int32_t on) {
+    if (!h) return GEON_EINVAL;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->timing = on != 0;
+    for (int k = 0; k < kTimers; k++) h->ev_used[k] = 0;
+    return GEON_OK;
+}
+
+int geon_kernel_timing(GeonHandle h, int32_t kernel, double* total_ms, int64_t* launches) {
+    if (!h || kernel < 0 || kernel >= kTimers) return GEON_EINVAL;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    double tot = 0;
+    for (size_t i = 0; i < h->ev_used[kernel]; i++) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, h->ev[kernel][i].first, h->ev[kernel][i].second));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (int64_t)h->ev_used[kernel];
+    return GEON_OK;
+}
+
+const char* geon_last_error(GeonHandle h) {
+    if (!h) return g_create_error.c_str();
+    return h->err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+// geon_dev.h — device-side data layout and the pair interaction shared by the
+// gfx950 kernels of the Geon physics backend.
+//
+// Layout in HBM (all particle arrays in Morton order, re-sorted every substep):
+//   SoA FP32 particle state  x, y, wx, wy, vx, vy, mass, base_mass, charge,
+//                            angw, ang_vel, radius, mu_cache, l_cache, f1old
+//   int32                    qq (charge quanta), slot (caller's index)
+//   packed sources           src4[i] = {x, y, mass, charge}  (one 16 B load)
+//                            src8[i] = {vx, vy, ang_vel, mu, L, ax, yuk, higgs}
+//   tree nodes               Node (64 B, one s_load_dwordx16 per visit)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace geon {
+
+constexpr int kWave = 64;
+constexpr int kKeyLevels = 24;          // quadtree levels resolved by the Morton key
+constexpr int kLeafCap = 4;             // QUADTREE_CAPACITY (config.js:59)
+constexpr int kWalkStack = 96;          // per-wave node stack: 3 * levels + 4 < 96
+constexpr float kEpsMass = 1e-12f;      // EPSILON (config.js:30), node-mass skip
+
+// Feature bits baked into kernel instantiations.
+enum : uint32_t {
+    F_GRAV = 1u,        // gravity (+ tidal torque)
+    F_COUL = 2u,        // Coulomb
+    F_EXTRA = 4u,       // any of magnetic / GM / 1PN / Yukawa / axion / Higgs (runtime-checked)
+};
+
+// Runtime toggles of the extra forces (uniform per launch).
+struct PhysFlags {
+    uint32_t mag, gm, one_pn, yuk, ax_mod, higgs;
+    float soft2;            // softening^2
+    float yuk_g, yuk_mu;    // YUKAWA_COUPLING, yukawaMu
+    float yuk_cut2;         // (6/muMin)^2
+    float theta2;
+    float dom_w, dom_h;     // periodic box (min-image when periodic != 0)
+    uint32_t periodic;
+};
+
+// Tree node record: 16 dwords.
+struct __align__(64) Node {
+    float comx, comy, mass, charge;
+    float mu, L, avx, avy;          // avx/avy: momentum / mass (proper velocity, quadtree.js:285)
+    float size2;                    // (2 * half_width)^2  (forces.js:641)
+    int32_t child;                  // divided: index of first child; leaf: first point
+    int32_t count;                  // -1 divided, else points in leaf
+    int32_t level;
+    float qxx, qxy, qyy, pad;       // mass quadrupole about the COM (learned correction input)
+};
+
+// Compensated (Kahan) FP32 accumulator.
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
+    __device__ __forceinline__ void add(float v) {
+        float y = v - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// Observer (target particle) values held in registers for a whole walk.
+struct Obs {
+    float x, y, vx, vy, m, q, ang_vel, mu, L, ax, yuk, higgs;
+    float tidal_ri5;        // body_radius^5 (forces.js:437-438)
+    float inv_m;            // 1/m (0 if m <= EPSILON) for the tidal coupling q1q2/m
+};
+
+// Accumulated outputs of one observer.
+template <bool KAHAN>
+struct Acc {
+    Kahan fx, fy, pe;
+    float bz, dbzdx, dbzdy, bgz, dbgzdx, dbgzdy, frame_drag, tidal, f1x, f1y;
+    __device__ __forceinline__ void init() {
+        fx.init(); fy.init(); pe.init();
+        bz = dbzdx = dbzdy = bgz = dbgzdx = dbgzdy = frame_drag = tidal = f1x = f1y = 0.f;
+    }
+    __device__ __forceinline__ void addf(float x, float y) {
+        if (KAHAN) { fx.add(x); fy.add(y); } else { fx.s += x; fy.s += y; }
+    }
+    __device__ __forceinline__ void addpe(float v) {
+        if (KAHAN) pe.add(v); else pe.s += v;
+    }
+};
+
+__device__ __forceinline__ float torus_wrap(float d, float full, float half) {
+    // topology.js:9-12
+    return d > half ? d - full : (d < -half ? d + full : d);
+}
+
+// One source acting on the observer: the body of pairForce (forces.js:163-442)
+// in FP32, without radiation jerk and signal-delay aberration.  (rx, ry) is the
+// (min-imaged) displacement source - observer.
+template <uint32_t F, bool KAHAN>
+__device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const PhysFlags& pf,
+                                           float rx, float ry, float sm, float sq,
+                                           float svx, float svy, float s_ang_vel,
+                                           float s_mu, float s_L, float s_ax, float s_yuk,
+                                           float s_higgs) {
+    const float raw2 = rx * rx + ry * ry;
+    const float r2 = raw2 + pf.soft2;
+    const float inv_r = __builtin_amdgcn_rsqf(r2);   // 1/sqrt(r^2 + eps^2)
+    const float inv_r2 = inv_r * inv_r;
+    const float inv_r3 = inv_r * inv_r2;
+    if (F & F_GRAV) {
+        const float k = o.m * sm;
+        const float fd = k * inv_r3;
+        a.addf(rx * fd, ry * fd);
+        a.addpe(-k * inv_r);
+    }
+    float ax_pair = 1.f;
+    if ((F & F_EXTRA) && pf.ax_mod) ax_pair = sqrtf(o.ax * s_ax);
+    if (F & F_COUL) {
+        const float k = -(o.q * sq) * ax_pair;
+        const float fd = k * inv_r3;
+        a.addf(rx * fd, ry * fd);
+        a.addpe(o.q * sq * inv_r * ax_pair);
+    }
+    if (F & F_EXTRA) {
+        const float inv_r5 = inv_r3 * inv_r2;
+        const float cross_sv = svx * ry - svy * rx;
+        if (pf.one_pn && (pf.gm || pf.mag)) {                       // forces.js:249-301
+            const float r = 1.f / inv_r;
+            const float nx = rx * inv_r, ny = ry * inv_r;
+            const float v1v2 = o.vx * svx + o.vy * svy;
+            if (pf.gm) {
+                const float v1sq = o.vx * o.vx + o.vy * o.vy;
+                const float v2sq = svx * svx + svy * svy;
+                const float nv1 = nx * o.vx + ny * o.vy;
+                const float nv2 = nx * svx + ny * svy;
+                const float radial = -v1sq - 2.f * v2sq + 1.5f * nv2 * nv2 + 5.f * o.m * inv_r + 4.f * sm * inv_r;
+                const float c1 = 4.f * nv1 - 3.f * nv2, c2 = 3.f * nv2;
+                const float base = o.m * sm * inv_r3;
+                const float fx = base * (rx * radial + (o.vx * c1 + svx * c2) * r);
+                const float fy = base * (ry * radial + (o.vy * c1 + svy * c2) * r);
+                a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
+                a.addpe(-o.m * sm * inv_r * (1.5f * (v1sq + v2sq) - 3.5f * v1v2 - 0.5f * nv1 * nv2 + o.m * inv_r + sm * inv_r));
+            }
+            if (pf.mag) {
+                const float v2n = svx * nx + svy * ny, v1n = o.vx * nx + o.vy * ny;
+                const float coeff = 0.5f * o.q * sq * inv_r2;
+                const float fx = coeff * (o.vx * v2n - 3.f * nx * v1n * v2n);
+                const float fy = coeff * (o.vy * v2n - 3.f * ny * v1n * v2n);
+                a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
+                a.addpe(-0.5f * o.q * sq * inv_r * (v1v2 + v1n * v2n));
+            }
+        }
+        if (pf.one_pn && pf.gm && pf.mag) {                         // forces.js:303-320
+            const float cc = o.q * sq * (o.m + sm) - (o.q * o.q * sm + sq * sq * o.m);
+            const float fd = cc * inv_r2 * inv_r2;
+            a.addf(rx * fd, ry * fd); a.f1x += rx * fd; a.f1y += ry * fd;
+            a.addpe(0.5f * cc * inv_r2);
+        }
+        if (pf.mag) {                                               // forces.js:322-353
+            const float fd = -3.f * (o.mu * s_mu) * inv_r5 * ax_pair;
+            a.addf(rx * fd, ry * fd);
+            a.addpe((o.mu * s_mu) * inv_r3 * ax_pair);
+            const float bzm = sq * cross_sv * inv_r3 * ax_pair;
+            a.bz += bzm;
+            a.dbzdx += 3.f * bzm * rx * inv_r2 + sq * svy * inv_r3 * ax_pair;
+            a.dbzdy += 3.f * bzm * ry * inv_r2 - sq * svx * inv_r3 * ax_pair;
+            a.bz -= s_mu * inv_r3 * ax_pair;
+            a.dbzdx -= 3.f * s_mu * rx * inv_r5 * ax_pair;
+            a.dbzdy -= 3.f * s_mu * ry * inv_r5 * ax_pair;
+        }
+        if (pf.gm) {                                                // forces.js:355-387
+            const float fd = 3.f * (o.L * s_L) * inv_r5;
+            a.addf(rx * fd, ry * fd);
+            a.addpe(-(o.L * s_L) * inv_r3);
+            const float bgm = -sm * cross_sv * inv_r3;
+            a.bgz += bgm;
+            a.dbgzdx += 3.f * bgm * rx * inv_r2 - sm * svy * inv_r3;
+            a.dbgzdy += 3.f * bgm * ry * inv_r2 + sm * svx * inv_r3;
+            a.bgz -= 2.f * s_L * inv_r3;
+            a.dbgzdx -= 6.f * s_L * rx * inv_r5;
+            a.dbgzdy -= 6.f * s_L * ry * inv_r5;
+            a.frame_drag += 2.f * s_L * (s_ang_vel - o.ang_vel) * inv_r3;
+        }
+        if (pf.yuk && raw2 < pf.yuk_cut2) {                         // forces.js:389-427
+            const float mu = pf.higgs ? pf.yuk_mu * sqrtf(o.higgs * s_higgs) : pf.yuk_mu;
+            const float r = 1.f / inv_r;
+            const float e = __expf(-mu * r);
+            const float ym = pf.ax_mod ? sqrtf(o.yuk * s_yuk) : 1.f;
+            const float gmm = pf.yuk_g * ym * o.m * sm * e;
+            const float fd = gmm * (inv_r2 + mu * inv_r) * inv_r;
+            a.addf(rx * fd, ry * fd);
+            a.addpe(-gmm * inv_r);
+            if (pf.one_pn) {
+                const float nx = rx * inv_r, ny = ry * inv_r;
+                const float nv1 = nx * o.vx + ny * o.vy, nv2 = nx * svx + ny * svy;
+                const float v1v2 = o.vx * svx + o.vy * svy;
+                const float alpha = 1.f + mu * r;
+                const float beta = 0.5f * gmm * inv_r2;
+                const float radial = -(alpha * v1v2 + (alpha * alpha + alpha + 1.f) * nv1 * nv2);
+                const float fx = beta * (radial * nx + alpha * (nv2 * o.vx + nv1 * svx));
+                const float fy = beta * (radial * ny + alpha * (nv2 * o.vy + nv1 * svy));
+                a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
+                a.addpe(0.5f * gmm * inv_r * (v1v2 + nv1 * nv2 * alpha));
+            }
+        }
+    }
+    if (F & F_GRAV) {                                               // forces.js:429-441
+        const float cross_rv = rx * (svy - o.vy) - ry * (svx - o.vx);
+        const float dw = o.ang_vel - cross_rv * inv_r2;
+        float coupling = sm;
+        if (F & F_COUL) coupling += o.q * sq * o.inv_m;
+        const float inv_r6 = inv_r2 * inv_r2 * inv_r2;
+        a.tidal -= 64.f * coupling * coupling * o.tidal_ri5 * inv_r6 * dw;
+    }
+}
+
+// Flops of one gravity+Coulomb interaction by the GPU-Gems rule (FMA = 2,
+// rsqrt = 4): displacement 2, r^2 + eps 4, rsqrt 4, r^-2 r^-3 2, force 8,
+// PE 4.  The tidal term adds 14 more when gravity is on.
+constexpr int kFlopsGravCoul = 23;
+constexpr int kFlopsTidal = 14;
+
+}  // namespace geon

@@ -1,0 +1,110 @@
+// geon_kernels.h — host-side launchers of the gfx950 kernels (internal to
+// libgeon_hip.so; not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "geon_dev.h"
+
+namespace geon {
+
+// Device control block: the substep controller lives on the device so the
+// adaptive-dt decision never needs the host (integrator.js:810-874).
+struct Ctrl {
+    double dt_remain, dt_sub, sim_time, dt_total;
+    uint64_t max_acc2_fresh;    // bits of a non-negative double (ordered as uint64)
+    uint64_t max_acc2_kick;     // max |F/m|^2 seen by the last kick (H8, integrator.js:924-928)
+    uint64_t max_cyc;           // max cyclotron frequency (integrator.js:817-849)
+    int32_t total_steps, active, n_alive, n_tree;
+    int32_t max_substeps, n_divided, overflow, deaths;
+    int32_t first_substep, pad0, pad1, pad2;
+    int64_t cnt_pair, cnt_node, cnt_visit, depth_overflow;
+};
+
+// Word-columns gathered by a permutation (all 32-bit).
+constexpr int kMaxCols = 40;
+struct Columns {
+    uint32_t* src[kMaxCols];
+    uint32_t* dst[kMaxCols];
+    int ncols;
+};
+
+// ---- sort + tree (k_tree.hip) ----
+struct TreeBufs {
+    uint64_t *keys, *keys_alt;
+    uint32_t *perm, *perm_alt;
+    uint32_t* hist;             // 256 * max_blocks
+    int32_t *owned, *rank_start;
+    int2* divnode;              // (lo, level) per divided node, pre-order rank
+    double* pre;                // 8 columns of (cap + 1) prefix sums
+    Node* nodes;
+    double *node_mass64, *node_comx64, *node_comy64, *node_qq64;
+    uint64_t* node_path;
+    int32_t* node_npts;         // points in each node's subtree
+    int32_t* scan_tmp;          // block sums for int scans
+    double* scan_tmp64;
+    int64_t cap_particles, cap_divided;
+};
+
+struct KeyParams {
+    double xmin, ymin, scale_x, scale_y;   // u = (x - xmin) * scale (exact for dyadic boxes)
+    double root_hw;                          // root half width (node size = 4W / 2^level)
+};
+
+void launch_keys(hipStream_t s, const float* x, const float* y, int n, KeyParams kp,
+                 uint64_t* keys, uint32_t* perm, Ctrl* ctrl);
+// Sorts keys/perm in place (ping-pong internally); n is a host upper bound,
+// the exact count is ctrl->n_alive.
+void radix_sort(hipStream_t s, TreeBufs& tb, int n_upper, int key_bits, const Ctrl* ctrl);
+void launch_gather(hipStream_t s, const Columns& cols, const uint32_t* perm, int n_upper, const Ctrl* ctrl);
+void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, const float* m,
+                const float* wx, const float* wy, const float* mu, const float* L,
+                const int32_t* qq, int n_upper, KeyParams kp, Ctrl* ctrl);
+// Exclusive scan of int32 flags (n from device ctrl->n_alive) -> out, total -> *total_dev.
+void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n_upper,
+                        const int32_t* n_dev, int32_t* total_dev, int32_t* tmp);
+
+// ---- forces (k_force.hip) ----
+struct ForceArgs {
+    const float4* src4;         // {x, y, mass, charge}
+    const float4* src8a;        // {vx, vy, ang_vel, mu}
+    const float4* src8b;        // {L, ax, yuk, higgs}
+    const float* radius;
+    const Node* nodes;
+    float *fx, *fy, *pe, *bz, *dbzdx, *dbzdy, *bgz, *dbgzdx, *dbgzdy, *frame_drag, *tidal, *f1x, *f1y;
+    const float* mass_for_cyc;  // for cyclotron scan in the epilogue
+    const float* charge;
+    Ctrl* ctrl;
+    PhysFlags pf;
+    float ext_gx, ext_gy, ext_ex, ext_ey, ext_bz;
+    uint32_t cyc_mag, cyc_gm;   // which B fields enter the cyclotron estimate
+    int n_upper;
+    int64_t max_nodes;          // node records allocated (walk bounds check)
+};
+void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan);
+void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan);
+void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const float* m,
+                            const float* q, const float* vx, const float* vy, const float* ang_vel,
+                            const float* radius, float* mu, float* L, const float* ax,
+                            const float* yuk, const float* higgs, float4* src4, float4* src8a,
+                            float4* src8b, int n_upper, const Ctrl* ctrl);
+
+// ---- integrator + reductions (k_step.hip) ----
+struct StepArgs {
+    float *x, *y, *wx, *wy, *vx, *vy, *angw, *ang_vel, *mass, *radius, *charge;
+    float *mu, *L, *f1x, *f1y, *f1ox, *f1oy;
+    const float *fx, *fy, *bz, *bgz, *dbzdx, *dbzdy, *dbgzdx, *dbgzdy, *frame_drag, *tidal;
+    int32_t* alive;
+    Ctrl* ctrl;
+    uint32_t rel, mag, gm, ext_bz_on, grav, spin_orbit, one_pn, expansion;
+    float hubble, cx, cy;
+    int n_upper;
+};
+void launch_dt_scan(hipStream_t s, const StepArgs& a);
+void launch_ctrl(hipStream_t s, Ctrl* ctrl, double softening, int kick_reuse);
+void launch_kick_drift(hipStream_t s, const StepArgs& a);
+void launch_vv_kick(hipStream_t s, const StepArgs& a);
+void launch_boundary(hipStream_t s, const StepArgs& a, int mode, double w, double h);
+// FP64 two-level reductions for the conservation readout; out[0..15].
+void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel);
+
+}  // namespace geon

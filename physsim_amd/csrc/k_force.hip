@@ -1,0 +1,259 @@
+// k_force.hip — force kernels for gfx950: the Barnes-Hut walk with
+// wavefront-shared node stacks, and the LDS-tiled O(N^2) direct sum.
+//
+// Walk (forces.js:622-694 calculateForce): one lane per particle, particles in
+// Morton order so the 64 lanes of a wavefront are spatial neighbours.  The wave
+// walks ONE shared stack held in LDS; every stack entry carries the 64-bit mask
+// of the lanes whose own walk reaches that node.  At a node each lane in the
+// mask makes the reference's decision (skip light node / leaf points / accept
+// aggregate / open); the lanes that open form the mask pushed with the four
+// children.  Each lane therefore visits exactly the node sequence of the
+// reference's per-particle stack walk, in the same order (children pushed
+// NW,NE,SW,SE, popped SE first), while node records are fetched once per wave
+// through the scalar cache.
+#include "geon_kernels.h"
+
+namespace geon {
+
+constexpr int kWalkThreads = 256;
+constexpr int kWavesPerBlock = kWalkThreads / kWave;
+
+__global__ void k_prepare_sources(const float* __restrict__ x, const float* __restrict__ y,
+                                  const float* __restrict__ m, const float* __restrict__ q,
+                                  const float* __restrict__ vx, const float* __restrict__ vy,
+                                  const float* __restrict__ ang_vel, const float* __restrict__ radius,
+                                  float* __restrict__ mu, float* __restrict__ L, const float* __restrict__ ax,
+                                  const float* __restrict__ yuk, const float* __restrict__ higgs,
+                                  float4* __restrict__ src4, float4* __restrict__ src8a,
+                                  float4* __restrict__ src8b, const Ctrl* __restrict__ ctrl) {
+    const int n = ctrl->n_alive;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float r = radius[i];
+    const float r2 = r * r;
+    const float w = ang_vel[i];
+    // forces.js:71-76: cached dipole moments for this substep
+    const float mui = 0.2f * q[i] * w * r2;
+    const float Li = 0.4f * m[i] * w * r2;
+    mu[i] = mui;
+    L[i] = Li;
+    src4[i] = make_float4(x[i], y[i], m[i], q[i]);
+    src8a[i] = make_float4(vx[i], vy[i], w, mui);
+    src8b[i] = make_float4(Li, ax ? ax[i] : 1.f, yuk ? yuk[i] : 1.f, higgs ? higgs[i] : 1.f);
+}
+
+void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const float* m, const float* q,
+                            const float* vx, const float* vy, const float* ang_vel, const float* radius, float* mu,
+                            float* L, const float* ax, const float* yuk, const float* higgs, float4* src4,
+                            float4* src8a, float4* src8b, int n_upper, const Ctrl* ctrl) {
+    if (n_upper <= 0) return;
+    hipLaunchKernelGGL(k_prepare_sources, dim3((n_upper + 255) / 256), dim3(256), 0, s, x, y, m, q, vx, vy, ang_vel,
+                       radius, mu, L, ax, yuk, higgs, src4, src8a, src8b, ctrl);
+}
+
+__device__ __forceinline__ void load_obs(const ForceArgs& a, int i, Obs& o) {
+    const float4 s4 = a.src4[i];
+    const float4 s8a = a.src8a[i];
+    const float4 s8b = a.src8b[i];
+    o.x = s4.x; o.y = s4.y; o.m = s4.z; o.q = s4.w;
+    o.vx = s8a.x; o.vy = s8a.y; o.ang_vel = s8a.z; o.mu = s8a.w;
+    o.L = s8b.x; o.ax = s8b.y; o.yuk = s8b.z; o.higgs = s8b.w;
+    const float r = a.radius[i];
+    const float r2 = r * r;
+    o.tidal_ri5 = r2 * r2 * r;
+    o.inv_m = o.m > kEpsMass ? 1.f / o.m : 0.f;
+}
+
+template <bool KAHAN>
+__device__ __forceinline__ void store_acc(const ForceArgs& a, int i, const Obs& o, const Acc<KAHAN>& acc) {
+    // _applyExternalFields (integrator.js:563-592) folded into the store
+    a.fx[i] = acc.fx.s + o.m * a.ext_gx + o.q * a.ext_ex;
+    a.fy[i] = acc.fy.s + o.m * a.ext_gy + o.q * a.ext_ey;
+    a.pe[i] = acc.pe.s;
+    a.bz[i] = acc.bz + a.ext_bz;
+    a.dbzdx[i] = acc.dbzdx; a.dbzdy[i] = acc.dbzdy;
+    a.bgz[i] = acc.bgz; a.dbgzdx[i] = acc.dbgzdx; a.dbgzdy[i] = acc.dbgzdy;
+    a.frame_drag[i] = acc.frame_drag; a.tidal[i] = acc.tidal;
+    a.f1x[i] = acc.f1x; a.f1y[i] = acc.f1y;
+}
+
+__device__ __forceinline__ void wave_add_counters(Ctrl* ctrl, int64_t pairs, int64_t nodes, int64_t visits) {
+    for (int off = 32; off > 0; off >>= 1) {
+        pairs += __shfl_xor(pairs, off, 64);
+        nodes += __shfl_xor(nodes, off, 64);
+        visits += __shfl_xor(visits, off, 64);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicAdd((unsigned long long*)&ctrl->cnt_pair, (unsigned long long)pairs);
+        atomicAdd((unsigned long long*)&ctrl->cnt_node, (unsigned long long)nodes);
+        atomicAdd((unsigned long long*)&ctrl->cnt_visit, (unsigned long long)visits);
+    }
+}
+
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
+    __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
+    __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
+    const int n = a.ctrl->n_alive;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int i = blockIdx.x * kWalkThreads + threadIdx.x;
+    const bool valid = i < n;
+    const uint64_t active = __ballot(valid);
+    if (active == 0) return;                       // whole wave past the end
+    const uint64_t my_bit = 1ull << lane;
+    Obs o;
+    if (valid) load_obs(a, i, o);
+    else { o.x = o.y = o.vx = o.vy = o.m = o.q = o.ang_vel = o.mu = o.L = 0.f; o.ax = o.yuk = o.higgs = 1.f; o.tidal_ri5 = 0.f; o.inv_m = 0.f; }
+    Acc<KAHAN> acc;
+    acc.init();
+    const PhysFlags pf = a.pf;
+    const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    int64_t c_pair = 0, c_node = 0, c_visit = 0;
+    int sp = 0;
+    if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
+    __builtin_amdgcn_wave_barrier();
+    sp = 1;
+    bool overflow = false;
+    while (sp > 0) {
+        sp--;
+        const int node = __builtin_amdgcn_readfirstlane(s_node[wv][sp]);
+        const uint64_t mask = s_mask[wv][sp];
+        const bool mine = (mask & my_bit) != 0;
+        if (node < 0 || node >= a.max_nodes) { overflow = true; continue; }
+        const Node nd = a.nodes[node];
+        c_visit += mine;
+        if (nd.mass < kEpsMass) continue;          // forces.js:630
+        if (nd.count > 0) {
+            // leaf: every point except the particle itself (forces.js:643-676)
+            const int first = nd.child, cnt = nd.count;
+            for (int j = first; j < first + cnt; j++) {
+                const float4 s4 = a.src4[j];
+                const float4 s8a = a.src8a[j];
+                float4 s8b = make_float4(0.f, 1.f, 1.f, 1.f);
+                if (F & F_EXTRA) s8b = a.src8b[j];
+                if (mine && j != i) {
+                    float rx = s4.x - o.x, ry = s4.y - o.y;
+                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                    pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x,
+                                         s8b.y, s8b.z, s8b.w);
+                    c_pair++;
+                }
+            }
+        } else if (nd.count < 0) {
+            float dx = nd.comx - o.x, dy = nd.comy - o.y;
+            if (pf.periodic) { dx = torus_wrap(dx, pf.dom_w, half_w); dy = torus_wrap(dy, pf.dom_h, half_h); }
+            const float d2 = dx * dx + dy * dy;
+            const bool accept = mine && (nd.size2 < pf.theta2 * d2);   // forces.js:680
+            if (accept) {
+                // aggregate: sAngVel = 0, mods = 1 (forces.js:681-685)
+                pair_force<F, KAHAN>(o, acc, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 0.f, nd.mu, nd.L,
+                                     1.f, 1.f, 1.f);
+                c_node++;
+            }
+            const uint64_t open = __ballot(mine && !accept);
+            if (open) {
+                if (sp + 4 <= kWalkStack) {
+                    if (lane == 0) {
+                        // push nw, ne, sw, se: se is popped first (forces.js:688-691)
+                        s_node[wv][sp] = nd.child;     s_mask[wv][sp] = open;
+                        s_node[wv][sp + 1] = nd.child + 1; s_mask[wv][sp + 1] = open;
+                        s_node[wv][sp + 2] = nd.child + 2; s_mask[wv][sp + 2] = open;
+                        s_node[wv][sp + 3] = nd.child + 3; s_mask[wv][sp + 3] = open;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    sp += 4;
+                } else {
+                    overflow = true;
+                }
+            }
+        }
+    }
+    if (overflow && lane == 0) a.ctrl->overflow = 2;
+    wave_add_counters(a.ctrl, c_pair, c_node, c_visit);
+    if (valid) store_acc<KAHAN>(a, i, o, acc);
+}
+
+// Direct sum (forces.js:93-126): each block stages 256 sources at a time in LDS.
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) {
+    __shared__ float4 t4[kWalkThreads];
+    __shared__ float4 t8a[kWalkThreads];
+    __shared__ float4 t8b[kWalkThreads];
+    const int n = a.ctrl->n_alive;
+    const int i = blockIdx.x * kWalkThreads + threadIdx.x;
+    const bool valid = i < n;
+    Obs o;
+    if (valid) load_obs(a, i, o);
+    else { o.x = o.y = o.vx = o.vy = o.m = o.q = o.ang_vel = o.mu = o.L = 0.f; o.ax = o.yuk = o.higgs = 1.f; o.tidal_ri5 = 0.f; o.inv_m = 0.f; }
+    Acc<KAHAN> acc;
+    acc.init();
+    const PhysFlags pf = a.pf;
+    const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    for (int base = 0; base < n; base += kWalkThreads) {
+        const int j = base + threadIdx.x;
+        __syncthreads();
+        if (j < n) {
+            t4[threadIdx.x] = a.src4[j];
+            t8a[threadIdx.x] = a.src8a[j];
+            if (F & F_EXTRA) t8b[threadIdx.x] = a.src8b[j];
+        }
+        __syncthreads();
+        const int cnt = min(kWalkThreads, n - base);
+        if (valid) {
+            for (int k = 0; k < cnt; k++) {
+                if (base + k == i) continue;
+                const float4 s4 = t4[k];
+                const float4 s8a = t8a[k];
+                float4 s8b = make_float4(0.f, 1.f, 1.f, 1.f);
+                if (F & F_EXTRA) s8b = t8b[k];
+                float rx = s4.x - o.x, ry = s4.y - o.y;
+                if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x, s8b.y,
+                                     s8b.z, s8b.w);
+            }
+        }
+    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t nn = n;
+        atomicAdd((unsigned long long*)&a.ctrl->cnt_pair, (unsigned long long)(nn * (nn - 1)));
+    }
+    if (valid) store_acc<KAHAN>(a, i, o, acc);
+}
+
+#define GEON_FEAT_SWITCH(KERNEL, feat, kahan, grid, stream, args)                                     \
+    do {                                                                                              \
+        const dim3 g_(grid), b_(kWalkThreads);                                                        \
+        switch ((feat) & 7u) {                                                                        \
+            case 1: if (kahan) hipLaunchKernelGGL((KERNEL<1, true>), g_, b_, 0, stream, args);        \
+                    else hipLaunchKernelGGL((KERNEL<1, false>), g_, b_, 0, stream, args); break;      \
+            case 2: if (kahan) hipLaunchKernelGGL((KERNEL<2, true>), g_, b_, 0, stream, args);        \
+                    else hipLaunchKernelGGL((KERNEL<2, false>), g_, b_, 0, stream, args); break;      \
+            case 3: if (kahan) hipLaunchKernelGGL((KERNEL<3, true>), g_, b_, 0, stream, args);        \
+                    else hipLaunchKernelGGL((KERNEL<3, false>), g_, b_, 0, stream, args); break;      \
+            case 4: if (kahan) hipLaunchKernelGGL((KERNEL<4, true>), g_, b_, 0, stream, args);        \
+                    else hipLaunchKernelGGL((KERNEL<4, false>), g_, b_, 0, stream, args); break;      \
+            case 5: if (kahan) hipLaunchKernelGGL((KERNEL<5, true>), g_, b_, 0, stream, args);        \
+                    else hipLaunchKernelGGL((KERNEL<5, false>), g_, b_, 0, stream, args); break;      \
+            case 6: if (kahan) hipLaunchKernelGGL((KERNEL<6, true>), g_, b_, 0, stream, args);        \
+                    else hipLaunchKernelGGL((KERNEL<6, false>), g_, b_, 0, stream, args); break;      \
+            case 7: if (kahan) hipLaunchKernelGGL((KERNEL<7, true>), g_, b_, 0, stream, args);        \
+                    else hipLaunchKernelGGL((KERNEL<7, false>), g_, b_, 0, stream, args); break;      \
+            default: if (kahan) hipLaunchKernelGGL((KERNEL<0, true>), g_, b_, 0, stream, args);       \
+                    else hipLaunchKernelGGL((KERNEL<0, false>), g_, b_, 0, stream, args); break;      \
+        }                                                                                             \
+    } while (0)
+
+void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
+    if (a.n_upper <= 0) return;
+    const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
+    GEON_FEAT_SWITCH(k_walk, feat, kahan, grid, s, a);
+}
+
+void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
+    if (a.n_upper <= 0) return;
+    const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
+    GEON_FEAT_SWITCH(k_direct, feat, kahan, grid, s, a);
+}
+
+}  // namespace geon

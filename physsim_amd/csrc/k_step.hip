@@ -1,0 +1,337 @@
+// k_step.hip — the substep controller, the fused Boris kick/rotate/kick +
+// spin/torque + drift pass, boundary handling and the FP64 conservation
+// reductions, for gfx950.
+#include "geon_kernels.h"
+
+namespace geon {
+
+constexpr double kEps = 1e-12;          // EPSILON (config.js:30)
+
+__device__ __forceinline__ void atomic_max_pos_double(uint64_t* addr, double v) {
+    // non-negative doubles order like their bit patterns; NaN never wins
+    // (the reference's `aSq > maxAccelSq` is false for NaN, integrator.js:822)
+    if (!(v >= 0.0)) return;
+    atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Max |F/m|^2 and max cyclotron frequency over the live particles
+// (integrator.js:812-849).
+__global__ void k_dt_scan(StepArgs a) {
+    const int n = a.ctrl->n_alive;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc2 = -1.0, cyc = -1.0;
+    if (i < n) {
+        const double m = a.mass[i];
+        const double inv_m = 1.0 / m;
+        const double fx = a.fx[i], fy = a.fy[i];
+        acc2 = (fx * fx + fy * fy) * inv_m * inv_m;
+        if (acc2 != acc2) acc2 = -1.0;
+        cyc = 0.0;
+        if (a.mag || a.ext_bz_on) {
+            const double absbz = fabs((double)a.bz[i]);
+            if (absbz > 0) cyc = fmax(cyc, fabs((double)a.charge[i]) * absbz * inv_m);
+        }
+        if (a.gm) {
+            const double absbgz = fabs((double)a.bgz[i]);
+            if (absbgz > 0) cyc = fmax(cyc, 4.0 * absbgz);
+        }
+        if (cyc != cyc) cyc = -1.0;
+    }
+    acc2 = wave_max(acc2);
+    cyc = wave_max(cyc);
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_pos_double(&a.ctrl->max_acc2_fresh, acc2);
+        atomic_max_pos_double(&a.ctrl->max_cyc, cyc);
+    }
+}
+
+void launch_dt_scan(hipStream_t s, const StepArgs& a) {
+    if (a.n_upper <= 0) return;
+    hipLaunchKernelGGL(k_dt_scan, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a);
+}
+
+// One thread: the adaptive-substep decision of integrator.js:810-874.
+__global__ void k_ctrl(Ctrl* c, double softening, int first) {
+    if (c->total_steps >= c->max_substeps || !(c->dt_remain > kEps)) {
+        c->active = 0;
+        return;
+    }
+    const double acc2 = __longlong_as_double((long long)(first ? c->max_acc2_fresh : c->max_acc2_kick));
+    const double cyc = __longlong_as_double((long long)c->max_cyc);
+    double a_max = sqrt(acc2);
+    if (a_max != a_max) a_max = 0;
+    double dt_safe = a_max > kEps ? sqrt(softening / a_max) : c->dt_remain;
+    if (cyc > 0) {
+        const double dtc = 0.7853981633974483 / cyc;
+        if (dtc < dt_safe) dt_safe = dtc;
+    }
+    const int budget = c->max_substeps - c->total_steps;
+    const double need = ceil(c->dt_remain / dt_safe);
+    const int steps = need < (double)budget ? (int)need : budget;
+    const double dt_sub = c->dt_remain / steps;
+    c->total_steps++;
+    c->dt_remain -= dt_sub;
+    c->dt_sub = dt_sub;
+    c->sim_time += dt_sub;
+    c->active = 1;
+    c->max_acc2_kick = 0;
+    c->max_acc2_fresh = 0;
+    c->max_cyc = 0;
+}
+
+void launch_ctrl(hipStream_t s, Ctrl* ctrl, double softening, int first) {
+    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1), 0, s, ctrl, softening, first);
+}
+
+// Fused substep body per particle: save 1PN force, half kick, Boris rotation,
+// half kick, NaN guard (integrator.js:884-928); spin-orbit and torques
+// (930-981); drift (1270-1278); expansion (1281-1292).
+__global__ void k_kick_drift(StepArgs a) {
+    const Ctrl* c = a.ctrl;
+    if (!c->active) return;
+    const int n = c->n_alive;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc2 = -1.0;
+    if (i < n) {
+        const float dt = (float)c->dt_sub;
+        const float half_dt = 0.5f * dt;
+        const float m = a.mass[i];
+        const float inv_m = 1.f / m;
+        const float h = half_dt * inv_m;
+        if (a.one_pn) { a.f1ox[i] = a.f1x[i]; a.f1oy[i] = a.f1y[i]; }
+        const float fx = a.fx[i], fy = a.fy[i];
+        float wx = a.wx[i] + fx * h;
+        float wy = a.wy[i] + fy * h;
+        const float q = a.charge[i];
+        float vx = a.vx[i], vy = a.vy[i];
+        if (a.mag || a.gm || a.ext_bz_on) {
+            float t = 0.f;
+            if (a.mag || a.ext_bz_on) t += (q * 0.5f * inv_m) * a.bz[i];
+            if (a.gm) t += 2.f * a.bgz[i];
+            if (t != 0.f) {
+                const float gamma = a.rel ? sqrtf(1.f + wx * wx + wy * wy) : 1.f;
+                t *= dt / gamma;
+                const float s = 2.f * t / (1.f + t * t);
+                const float wpx = wx + wy * t, wpy = wy - wx * t;
+                wx = wx + wpy * s;
+                wy = wy - wpx * s;
+            }
+        }
+        wx += fx * h;
+        wy += fy * h;
+        if (wx != wx || wy != wy) { wx = 0.f; wy = 0.f; vx = 0.f; vy = 0.f; }
+        acc2 = ((double)fx * fx + (double)fy * fy) * (double)inv_m * (double)inv_m;
+        float angw = a.angw[i];
+        float ang_vel = a.ang_vel[i];
+        const float r = a.radius[i];
+        const bool need_so = a.spin_orbit && (a.mag || a.gm);
+        const bool need_tq = (a.gm && a.rel) || a.grav;
+        if (need_so && fabsf(ang_vel) >= 1e-12f) {
+            const float ixo = a.L[i];       // angMomentum cached by the last force pass
+            if (fabsf(ixo) >= 1e-12f) {
+                const float dt_m = dt * inv_m;
+                if (a.mag && fabsf(q) > 1e-12f) {
+                    const float mu = a.mu[i];
+                    angw -= mu * (vx * a.dbzdx[i] + vy * a.dbzdy[i]) * dt / ixo;
+                    wx += mu * a.dbzdx[i] * dt_m;
+                    wy += mu * a.dbzdy[i] * dt_m;
+                }
+                if (a.gm) {
+                    const float L = ixo;
+                    angw -= L * (vx * a.dbgzdx[i] + vy * a.dbgzdy[i]) * dt / ixo;
+                    wx -= L * a.dbgzdx[i] * dt_m;
+                    wy -= L * a.dbgzdy[i] * dt_m;
+                }
+                if (angw != angw) angw = 0.f;
+                const float sr = angw * r;
+                ang_vel = a.rel ? angw / sqrtf(1.f + sr * sr) : angw;
+            }
+        }
+        if (need_tq) {
+            float torque = 0.f;
+            if (a.gm && a.rel) torque += a.frame_drag[i];
+            if (a.grav) torque += a.tidal[i];
+            if (torque != 0.f) {
+                const float I = 0.4f * m * r * r;
+                if (I >= 1e-12f) {
+                    angw += torque * dt / I;
+                    if (angw != angw) angw = 0.f;
+                    const float sr = angw * r;
+                    ang_vel = a.rel ? angw / sqrtf(1.f + sr * sr) : angw;
+                }
+            }
+        }
+        // drift
+        const float inv_g = a.rel ? 1.f / sqrtf(1.f + wx * wx + wy * wy) : 1.f;
+        vx = wx * inv_g;
+        vy = wy * inv_g;
+        ang_vel = a.rel ? angw / sqrtf(1.f + angw * angw * r * r) : angw;
+        float x = a.x[i] + vx * dt;
+        float y = a.y[i] + vy * dt;
+        if (a.expansion) {
+            x += a.hubble * (x - a.cx) * dt;
+            y += a.hubble * (y - a.cy) * dt;
+            const float decay = 1.f - a.hubble * dt;
+            wx *= decay;
+            wy *= decay;
+            if (a.rel && a.one_pn) {
+                const float ig = 1.f / sqrtf(1.f + wx * wx + wy * wy);
+                vx = wx * ig; vy = wy * ig;
+            }
+        }
+        a.x[i] = x; a.y[i] = y;
+        a.wx[i] = wx; a.wy[i] = wy;
+        a.vx[i] = vx; a.vy[i] = vy;
+        a.angw[i] = angw; a.ang_vel[i] = ang_vel;
+    }
+    acc2 = wave_max(acc2);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos_double(&a.ctrl->max_acc2_kick, acc2);
+}
+
+void launch_kick_drift(hipStream_t s, const StepArgs& a) {
+    if (a.n_upper <= 0) return;
+    hipLaunchKernelGGL(k_kick_drift, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a);
+}
+
+// 1PN velocity-Verlet correction w += (F1PN_new - F1PN_old) dt/(2m)
+// (integrator.js:1310-1316).
+__global__ void k_vv_kick(StepArgs a) {
+    const Ctrl* c = a.ctrl;
+    if (!c->active) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c->n_alive) return;
+    const float h = 0.5f * (float)c->dt_sub / a.mass[i];
+    float wx = a.wx[i] + (a.f1x[i] - a.f1ox[i]) * h;
+    float wy = a.wy[i] + (a.f1y[i] - a.f1oy[i]) * h;
+    if (wx != wx || wy != wy) { wx = wy = 0.f; a.vx[i] = 0.f; a.vy[i] = 0.f; }
+    a.wx[i] = wx; a.wy[i] = wy;
+}
+
+void launch_vv_kick(hipStream_t s, const StepArgs& a) {
+    if (a.n_upper <= 0) return;
+    hipLaunchKernelGGL(k_vv_kick, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a);
+}
+
+// Boundary handling once per update (integrator.js:1798-1824, topology.js:81-90).
+__global__ void k_boundary(StepArgs a, int mode, float w, float h) {
+    const int n = a.ctrl->n_alive;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = a.x[i], y = a.y[i];
+    int alive = 1;
+    if (mode == 0) {            // despawn (DESPAWN_MARGIN = 64, config.js:138)
+        if (x < -64.f || x > w + 64.f || y < -64.f || y > h + 64.f) alive = 0;
+    } else if (mode == 2) {     // loop, torus wrap
+        if (x < 0.f) x += w; else if (x > w) x -= w;
+        if (y < 0.f) y += h; else if (y > h) y -= h;
+    } else {                    // bounce safety clamp
+        x = fminf(fmaxf(x, 0.f), w);
+        y = fminf(fmaxf(y, 0.f), h);
+    }
+    a.x[i] = x; a.y[i] = y;
+    a.alive[i] = alive;
+    if (!alive) atomicAdd(&a.ctrl->deaths, 1);
+}
+
+void launch_boundary(hipStream_t s, const StepArgs& a, int mode, double w, double h) {
+    if (a.n_upper <= 0) return;
+    hipLaunchKernelGGL(k_boundary, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a, mode, (float)w, (float)h);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 conservation reductions (energy.js:11-62), deterministic two-level.
+// ---------------------------------------------------------------------------
+constexpr int kStatThreads = 256;
+constexpr int kStatCols = 8;
+
+__global__ __launch_bounds__(kStatThreads) void k_stats_pass1(StepArgs a, const float* __restrict__ pe,
+                                                              double* __restrict__ partial, int nb, int rel) {
+    __shared__ double sh[kStatCols][kStatThreads];
+    const int n = a.ctrl->n_alive;
+    double v[kStatCols] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * kStatThreads + threadIdx.x; i < n; i += nb * kStatThreads) {
+        const double m = a.mass[i];
+        const double r = a.radius[i];
+        const double r2 = r * r;
+        const double wx = a.wx[i], wy = a.wy[i], angw = a.angw[i];
+        if (rel) {
+            const double wsq = wx * wx + wy * wy;
+            const double g = sqrt(1 + wsq);
+            v[0] += wsq / (g + 1) * m;
+            const double srsq = angw * angw * r2;
+            v[1] += 0.4 * m * srsq / (sqrt(1 + srsq) + 1);
+        } else {
+            const double vx = a.vx[i], vy = a.vy[i], av = a.ang_vel[i];
+            v[0] += 0.5 * m * (vx * vx + vy * vy);
+            v[1] += 0.5 * 0.4 * m * r2 * av * av;
+        }
+        v[2] += m * wx;
+        v[3] += m * wy;
+        v[4] += m;
+        v[5] += m * (double)a.x[i];
+        v[6] += m * (double)a.y[i];
+        v[7] += (double)pe[i];
+    }
+#pragma unroll
+    for (int c = 0; c < kStatCols; c++) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int off = kStatThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+#pragma unroll
+            for (int c = 0; c < kStatCols; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < kStatCols) partial[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_stats_final(const double* __restrict__ partial, int nb, int ncols, double* __restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= ncols) return;
+    double s = 0;
+    for (int b = 0; b < nb; b++) s += partial[c * nb + b];
+    out[c] = s;
+}
+
+__global__ __launch_bounds__(kStatThreads) void k_stats_pass2(StepArgs a, const double* __restrict__ sums,
+                                                              double* __restrict__ partial, int nb) {
+    __shared__ double sh[2][kStatThreads];
+    const int n = a.ctrl->n_alive;
+    const double tm = sums[4];
+    const double comx = tm > 0 ? sums[5] / tm : 0.0, comy = tm > 0 ? sums[6] / tm : 0.0;
+    double orb = 0, spin = 0;
+    for (int i = blockIdx.x * kStatThreads + threadIdx.x; i < n; i += nb * kStatThreads) {
+        const double m = a.mass[i];
+        const double r = a.radius[i];
+        const double dx = (double)a.x[i] - comx, dy = (double)a.y[i] - comy;
+        orb += dx * (m * (double)a.wy[i]) - dy * (m * (double)a.wx[i]);
+        spin += 0.4 * m * r * r * (double)a.angw[i];
+    }
+    sh[0][threadIdx.x] = orb;
+    sh[1][threadIdx.x] = spin;
+    __syncthreads();
+    for (int off = kStatThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) partial[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel) {
+    int nb = (a.n_upper + kStatThreads - 1) / kStatThreads;
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_stats_pass1, dim3(nb), dim3(kStatThreads), 0, s, a, pe, partial, nb, rel);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, partial, nb, kStatCols, out);
+    hipLaunchKernelGGL(k_stats_pass2, dim3(nb), dim3(kStatThreads), 0, s, a, out, partial, nb);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, partial, nb, 2, out + kStatCols);
+}
+
+}  // namespace geon

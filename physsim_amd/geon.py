@@ -1,0 +1,232 @@
+"""Python host side of the gfx950 Geon backend.
+
+`HipPhysics` mirrors the method surface the reference app calls on its WebGPU
+backend (`GPUPhysics`, src/gpu/gpu-physics.js) so a host written against that
+contract can drive this backend; every call goes through the C ABI of
+libgeon_hip.so (include/geon.h).  There is no CPU fallback: if the HIP library
+is missing or no GPU is present, construction raises.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+from . import _abi as A
+
+_LIB = None
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgeon_hip.so")
+
+
+def load_library(path=None):
+    """Load libgeon_hip.so (built in-tree by physsim_amd/build.py)."""
+    global _LIB
+    if _LIB is None or path is not None:
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError("libgeon_hip.so not built: run __graft_entry__.build() (missing %s)" % p)
+        _LIB = A.bind(C.CDLL(p))
+        if _LIB.geon_abi_version() != A.ABI_VERSION:
+            raise RuntimeError("libgeon_hip.so ABI mismatch")
+    return _LIB
+
+
+class GeonError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("geon error %d: %s" % (code, msg))
+        self.code = code
+
+
+def default_params(**kw):
+    lib = load_library()
+    p = A.GeonParams()
+    lib.geon_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class HipPhysics:
+    """Third physics backend beside the reference's CPUPhysics and GPUPhysics."""
+
+    def __init__(self, params=None, max_particles=1 << 16):
+        self.lib = load_library()
+        self.params = params if params is not None else default_params()
+        h = C.c_void_p()
+        rc = self.lib.geon_create(C.byref(self.params), int(max_particles), C.byref(h))
+        if rc != 0:
+            raise GeonError(rc, self.lib.geon_last_error(None).decode())
+        self.h = h
+        self.max_particles = int(max_particles)
+        self._stats = None
+
+    # ---- lifecycle ----------------------------------------------------
+    def destroy(self):                                      # gpu-physics.js:4291
+        if self.h:
+            self.lib.geon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise GeonError(rc, self.lib.geon_last_error(self.h).decode())
+        return rc
+
+    def reset(self):                                        # gpu-physics.js:3973
+        self._check(self.lib.geon_reset(self.h))
+
+    # ---- configuration ----------------------------------------------------
+    def set_params(self, params):
+        self._check(self.lib.geon_set_params(self.h, C.byref(params)))
+        self.params = params
+
+    def setToggles(self, physics):                          # gpu-physics.js:1969-2056
+        """Accepts an object or dict with the reference's *Enabled booleans."""
+        get = physics.get if isinstance(physics, dict) else (lambda k, d=None: getattr(physics, k, d))
+        t = 0
+        for name, bit in A.TOGGLE_BITS.items():
+            if get(name, False):
+                t |= bit
+        p = A.GeonParams.from_buffer_copy(self.params)
+        p.toggles0 = t
+        if get("yukawaMu") is not None:
+            p.yukawa_mu = float(get("yukawaMu"))
+        self.set_params(p)
+
+    # ---- particles ----------------------------------------------------
+    def addParticle(self, x, y, vx=0.0, vy=0.0, mass=1.0, charge=0.0, angw=0.0, antimatter=False):
+        """gpu-physics.js:1857: vx/vy are proper velocity w; returns slot or -1."""
+        pi = A.GeonParticleInit(x, y, vx, vy, mass, charge, angw, int(bool(antimatter)))
+        rc = self.lib.geon_add_particle(self.h, C.byref(pi))
+        if rc == A.EFULL:
+            return -1
+        return self._check(rc)
+
+    def removeParticle(self, idx):                          # gpu-physics.js:1936
+        self._check(self.lib.geon_remove_particle(self.h, int(idx)))
+
+    def patchMass(self, idx, mass, base_mass):              # gpu-physics.js:1955
+        self._check(self.lib.geon_patch_mass(self.h, int(idx), float(mass), float(base_mass)))
+
+    def upload(self, x, y, wx, wy, mass, charge, angw=None, base_mass=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (x, y, wx, wy, mass, base_mass, charge, angw)]
+        n = len(arrs[0])
+        v = A.GeonSoAView(*[_dptr(a) for a in arrs], None)
+        self._check(self.lib.geon_upload_soa(self.h, C.byref(v), n))
+        self._keep = arrs
+
+    def download(self):
+        ns = C.c_int32()
+        cap = self.max_particles
+        out = {k: np.zeros(cap) for k in ("x", "y", "wx", "wy", "mass", "base_mass", "charge", "angw")}
+        alive = np.zeros(cap, dtype=np.int32)
+        v = A.GeonSoAView(*[_dptr(out[k]) for k in ("x", "y", "wx", "wy", "mass", "base_mass", "charge", "angw")],
+                          alive.ctypes.data_as(C.POINTER(C.c_int32)))
+        self._check(self.lib.geon_download_soa(self.h, C.byref(v), C.byref(ns)))
+        n = ns.value
+        res = {k: a[:n] for k, a in out.items()}
+        res["alive"] = alive[:n].astype(bool)
+        return res
+
+    def getParticleCount(self):                             # gpu-physics.js:4287
+        st = A.GeonStats()
+        self._check(self.lib.geon_get_stats(self.h, C.byref(st)))
+        return st.alive_count
+
+    # ---- stepping --------------------------------------------------------
+    def update(self, dt):
+        """One Physics.update call: adaptive substeps, boundary once at the end."""
+        n = C.c_int32()
+        self._check(self.lib.geon_update(self.h, float(dt), C.byref(n)))
+        return n.value
+
+    def compute_forces(self):
+        self._check(self.lib.geon_compute_forces(self.h))
+
+    def forces(self, n_slots):
+        names = ("fx", "fy", "bz", "dbzdx", "dbzdy", "bgz", "dbgzdx", "dbgzdy", "frame_drag_torque",
+                 "tidal_torque", "pe", "f1pn_x", "f1pn_y")
+        out = {k: np.zeros(n_slots) for k in names}
+        v = A.GeonForceView(*[_dptr(out[k]) for k in names])
+        self._check(self.lib.geon_get_forces(self.h, C.byref(v), int(n_slots)))
+        return out
+
+    # ---- stats -----------------------------------------------------------
+    def requestStats(self, sel_idx=-1):                     # gpu-physics.js:4146
+        st = A.GeonStats()
+        self._check(self.lib.geon_get_stats(self.h, C.byref(st)))
+        self._stats = st
+
+    def readStats(self):                                    # gpu-physics.js:4215-4281
+        st = self._stats
+        if st is None:
+            return None
+        return {
+            "linearKE": st.linear_ke, "spinKE": st.spin_ke, "px": st.px, "py": st.py,
+            "orbitalAngMom": st.orbital_ang_mom, "spinAngMom": st.spin_ang_mom,
+            "comX": st.com_x, "comY": st.com_y, "totalMass": st.total_mass,
+            "aliveCount": st.alive_count, "pe": st.pe, "fieldEnergy": st.field_energy,
+            "fieldPx": st.field_px, "fieldPy": st.field_py,
+            "higgsFieldEnergy": st.higgs_field_energy, "axionFieldEnergy": st.axion_field_energy,
+            "pfiEnergy": st.pfi_energy, "simTime": st.sim_time,
+        }
+
+    def stats(self):
+        self.requestStats()
+        return self.readStats()
+
+    # ---- tree / diagnostics ---------------------------------------------
+    def build_tree(self):
+        self._check(self.lib.geon_build_tree(self.h))
+
+    def tree(self):
+        nn = C.c_int64()
+        self._check(self.lib.geon_get_tree(self.h, None, 0, C.byref(nn)))
+        arr = (A.GeonTreeNode * max(nn.value, 1))()
+        self._check(self.lib.geon_get_tree(self.h, arr, nn.value, C.byref(nn)))
+        return np.ctypeslib.as_array(arr)[: nn.value].copy() if nn.value else np.zeros(0)
+
+    def counters(self):
+        c = A.GeonCounters()
+        self._check(self.lib.geon_counters(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in A.GeonCounters._fields_}
+
+    def timing(self, on=True):
+        self._check(self.lib.geon_timing_enable(self.h, int(bool(on))))
+
+    def kernel_timing(self, kernel):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.lib.geon_kernel_timing(self.h, int(kernel), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # ---- save/load: the reference's v1 JSON (save-load.js:89-127) ----------
+    def serialize(self):
+        d = self.download()
+        particles = []
+        for i in np.nonzero(d["alive"])[0]:
+            particles.append({"x": d["x"][i], "y": d["y"][i], "wx": d["wx"][i], "wy": d["wy"][i],
+                              "mass": d["mass"][i], "baseMass": d["base_mass"][i], "charge": d["charge"][i],
+                              "angw": d["angw"][i], "antimatter": False})
+        return json.dumps({"version": 1, "particles": particles})
+
+    def deserialize(self, state):
+        s = json.loads(state) if isinstance(state, str) else state
+        if s.get("version") != 1:                           # save-load.js:130
+            raise ValueError("unsupported state version")
+        ps = s["particles"]
+        col = lambda k, d=0.0: np.array([p.get(k, d) for p in ps], dtype=np.float64)
+        self.upload(col("x"), col("y"), col("wx"), col("wy"), col("mass", 1.0), col("charge"), col("angw"),
+                    col("baseMass") if ps and "baseMass" in ps[0] else None)

@@ -1,0 +1,84 @@
+"""Seeded synthetic inputs for the benchmark and parity workloads.
+
+All generators use numpy's Philox4x32-10 bit generator with seed
+`config_id * 1000 + replicate` and round every value to FP32, so the FP32
+device path and the FP64 oracle see bit-identical inputs.  Charges are integer
+multiples of 0.1 (main.js:511); spins are surface-velocity fractions turned
+into angular celerity as in main.js:518-521.
+"""
+import numpy as np
+
+from . import _abi as A
+
+
+def rng(config_id, replicate=0):
+    return np.random.Generator(np.random.Philox(config_id * 1000 + replicate))
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def spin_to_angw(sv, mass):
+    """main.js:517-521: clamp to +-0.999, angw = sign*|sv| / (r*sqrt(1 - sv^2)), r = cbrt(m)."""
+    sv = np.clip(sv, -0.999, 0.999)
+    r = np.cbrt(mass)
+    a = np.abs(sv)
+    return np.where(a > 0, np.sign(sv) * a / (r * np.sqrt(1 - a * a)), 0.0)
+
+
+def disk(n, radius, cx, cy, soft, g, rel_noise=0.05):
+    """Uniform disk r = R*sqrt(U), lognormal(0, 0.5) masses, charges 0.1*k with
+    k in [-10, 10], circular speed from the enclosed mass plus N(0, 0.05) noise."""
+    r = radius * np.sqrt(g.random(n))
+    phi = 2 * np.pi * g.random(n)
+    x = cx + r * np.cos(phi)
+    y = cy + r * np.sin(phi)
+    m = g.lognormal(0.0, 0.5, n)
+    k = g.integers(-10, 11, n)
+    q = k * 0.1
+    order = np.argsort(r)
+    menc = np.empty(n)
+    menc[order] = np.cumsum(m[order]) - m[order]
+    vc = np.sqrt(np.maximum(menc * r * r / (r * r + soft * soft) ** 1.5, 0.0))
+    vc = np.minimum(vc, 0.5)
+    wx = -vc * np.sin(phi) + g.normal(0, rel_noise, n)
+    wy = vc * np.cos(phi) + g.normal(0, rel_noise, n)
+    return dict(x=f32(x), y=f32(y), wx=f32(wx), wy=f32(wy), mass=f32(m), charge=f32(q), angw=np.zeros(n))
+
+
+def config1(n=1024, replicate=0):
+    """Plain method, small: gravity + Coulomb, softening 0.01, theta 0.5, dt 1e-3,
+    despawn boundary; disk of radius 10 at the centre of a 64x64 domain."""
+    g = rng(1, replicate)
+    st = disk(n, 10.0, 32.0, 32.0, 0.01, g)
+    prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, boundary_mode=A.BOUND_DESPAWN,
+               domain_w=64.0, domain_h=64.0, softening=0.01, theta=0.5)
+    return st, prm, 1e-3
+
+
+def config2(n=65536, replicate=0, side=None):
+    """Relativistic torus box (min-image), gravity + Coulomb + Yukawa (mu = 1) +
+    magnetic dipole, spins uniform in [-1, 1]; softening 8, theta 0.5, dt 1/128.
+    The box side scales with sqrt(n) to keep the density of 65536 in 256x256."""
+    g = rng(2, replicate)
+    L = side if side is not None else 256.0 * np.sqrt(n / 65536.0)
+    x = g.random(n) * L
+    y = g.random(n) * L
+    m = g.lognormal(0.0, 0.5, n)
+    q = g.integers(-10, 11, n) * 0.1
+    wx = g.normal(0, 0.05, n)
+    wy = g.normal(0, 0.05, n)
+    sv = g.uniform(-1.0, 1.0, n)
+    angw = spin_to_angw(sv, f32(m))
+    st = dict(x=f32(x), y=f32(y), wx=f32(wx), wy=f32(wy), mass=f32(m), charge=f32(q), angw=f32(angw))
+    prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.YUKAWA | A.MAGNETIC | A.RELATIVITY | A.BARNES_HUT,
+               boundary_mode=A.BOUND_LOOP, topology=A.TORUS, periodic_mode=A.PERIODIC_MIN_IMAGE,
+               domain_w=float(L), domain_h=float(L), softening=8.0, theta=0.5, yukawa_mu=1.0)
+    return st, prm, 1.0 / 128
+
+
+def make_params(lib_default, **kw):
+    for k, v in kw.items():
+        setattr(lib_default, k, v)
+    return lib_default
