@@ -70,6 +70,12 @@ extern "C" {
  * once (what the reference's own direct sum does, forces.js:93-126). */
 #define GEON_PERIODIC_MIN_IMAGE 0
 #define GEON_PERIODIC_GHOSTS    1
+/* WINDOWED: as MIN_IMAGE, but a divided node is approximated only when its
+ * whole box lies inside the observer's min-image window, so every particle of
+ * the node has the image its COM has and the walk converges to the min-image
+ * direct sum as theta -> 0 (with MIN_IMAGE it does not: a node straddling the
+ * window edge is counted at its COM's image). */
+#define GEON_PERIODIC_WINDOWED  2
 
 typedef struct GeonContext* GeonHandle;
 

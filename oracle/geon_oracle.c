@@ -581,6 +581,17 @@ static int buildTree(OrcSys* s) {
 /* ------------------------------------------------------------------ */
 /* forces.js: walk, direct sum, computeAllForces                       */
 /* ------------------------------------------------------------------ */
+/* GEON_PERIODIC_WINDOWED: a node may stand for its particles only if its whole
+ * box, moved by the image shift of its COM, lies inside the observer's
+ * min-image window (not in the reference: see include/geon.h). */
+static inline int window_ok(const OrcSys* s, const Pool* q, int ni, double px, double py, double dx, double dy) {
+    if (!s->periodic || s->prm.periodic_mode != GEON_PERIODIC_WINDOWED) return 1;
+    const double W = s->prm.domain_w, H = s->prm.domain_h;
+    const double sx = dx - (q->comX[ni] - px), sy = dy - (q->comY[ni] - py);
+    const double cx = q->bx[ni] - px + sx, cy = q->by[ni] - py + sy;
+    return fabs(cx) + q->bw[ni] <= 0.5 * W && fabs(cy) + q->bh[ni] <= 0.5 * H;
+}
+
 /* calculateForce (forces.js:622-694), sources at current time. */
 static void calculateForce(OrcSys* s, int pi, int32_t* stack, int64_t* cnt) {
     const Pool* q = &s->pool;
@@ -611,7 +622,7 @@ static void calculateForce(OrcSys* s, int pi, int32_t* stack, int64_t* cnt) {
                           o->magMoment, o->angMomentum, real->axMod, real->yukMod, real->higgsMod, 1);
                 cnt[0]++;
             }
-        } else if (q->divided[nodeIdx] && (size * size < thetaSq * dSq)) {
+        } else if (q->divided[nodeIdx] && (size * size < thetaSq * dSq) && window_ok(s, q, nodeIdx, px, py, dx, dy)) {
             const double nodeMass = q->mass[nodeIdx];
             pairForce(s, p, q->comX[nodeIdx], q->comY[nodeIdx], q->momX[nodeIdx] / nodeMass,
                       q->momY[nodeIdx] / nodeMass, nodeMass, q->charge[nodeIdx], 0,
@@ -766,7 +777,7 @@ static void compute1PN(OrcSys* s) {
                             const P* real = &s->p[o->original];
                             accum1PN(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, real->yukMod, real->higgsMod);
                         }
-                    } else if (q->divided[ni] && size * size < thetaSq * dSq) {
+                    } else if (q->divided[ni] && size * size < thetaSq * dSq && window_ok(s, q, ni, p->x, p->y, dx, dy)) {
                         const double m = q->mass[ni];
                         accum1PN(s, p, q->comX[ni], q->comY[ni], q->momX[ni] / m, q->momY[ni] / m, m, q->charge[ni], 1, 1);
                     } else if (q->divided[ni]) {

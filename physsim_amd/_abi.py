@@ -13,7 +13,7 @@ HIGGS, AXION, BARNES_HUT, BOSON_INTER = 1 << 12, 1 << 13, 1 << 14, 1 << 15
 COL_PASS, COL_MERGE, COL_BOUNCE = 0, 1, 2
 BOUND_DESPAWN, BOUND_BOUNCE, BOUND_LOOP = 0, 1, 2
 TORUS, KLEIN, RP2 = 0, 1, 2
-PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS = 0, 1
+PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS, PERIODIC_WINDOWED = 0, 1, 2
 
 TOGGLE_BITS = {
     "gravityEnabled": GRAVITY, "coulombEnabled": COULOMB, "magneticEnabled": MAGNETIC,

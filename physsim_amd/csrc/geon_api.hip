@@ -114,7 +114,7 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     if (p->collision_mode != GEON_COL_PASS) { *why = "only collision mode 'pass' is run"; return 1; }
     if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
     if (p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) { *why = "only the torus topology is run"; return 1; }
-    if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode != GEON_PERIODIC_MIN_IMAGE) { *why = "only min-image periodic mode is run"; return 1; }
+    if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode == GEON_PERIODIC_GHOSTS) { *why = "ghost-image periodic mode is not run (min-image and windowed are)"; return 1; }
     if (!(p->domain_w > 0) || !(p->domain_h > 0)) { *why = "domain must be positive"; return 1; }
     if (p->max_substeps < 1) { *why = "max_substeps must be >= 1"; return 1; }
     return 0;
@@ -139,6 +139,7 @@ PhysFlags phys_flags(const GeonParams& p) {
     f.dom_w = (float)p.domain_w;
     f.dom_h = (float)p.domain_h;
     f.periodic = p.boundary_mode == GEON_BOUND_LOOP;
+    f.window = f.periodic && p.periodic_mode == GEON_PERIODIC_WINDOWED;
     return f;
 }
 

@@ -36,6 +36,7 @@ struct PhysFlags {
     float theta2;
     float dom_w, dom_h;     // periodic box (min-image when periodic != 0)
     uint32_t periodic;
+    uint32_t window;        // periodic: approximate only nodes inside the min-image window
 };
 
 // Tree node record: 16 dwords.
@@ -45,8 +46,10 @@ struct __align__(64) Node {
     float size2;                    // (2 * half_width)^2  (forces.js:641)
     int32_t child;                  // divided: index of first child; leaf: first point
     int32_t count;                  // -1 divided, else points in leaf
+    float half;                     // half width of the node box
+    float bx, by;                   // box centre
     int32_t level;
-    float qxx, qxy, qyy, pad;       // mass quadrupole about the COM (learned correction input)
+    float pad;
 };
 
 // Compensated (Kahan) FP32 accumulator.

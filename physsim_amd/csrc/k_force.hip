@@ -141,10 +141,16 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
                 }
             }
         } else if (nd.count < 0) {
-            float dx = nd.comx - o.x, dy = nd.comy - o.y;
+            const float rawx = nd.comx - o.x, rawy = nd.comy - o.y;
+            float dx = rawx, dy = rawy;
             if (pf.periodic) { dx = torus_wrap(dx, pf.dom_w, half_w); dy = torus_wrap(dy, pf.dom_h, half_h); }
             const float d2 = dx * dx + dy * dy;
-            const bool accept = mine && (nd.size2 < pf.theta2 * d2);   // forces.js:680
+            bool accept = mine && (nd.size2 < pf.theta2 * d2);   // forces.js:680
+            if (pf.window) {
+                // the node's whole box must map through the COM's image shift into the window
+                const float cx = nd.bx - o.x + (dx - rawx), cy = nd.by - o.y + (dy - rawy);
+                accept = accept && (fabsf(cx) + nd.half <= half_w) && (fabsf(cy) + nd.half * (pf.dom_h / pf.dom_w) <= half_h);
+            }
             if (accept) {
                 // aggregate: sAngVel = 0, mods = 1 (forces.js:681-685)
                 pair_force<F, KAHAN>(o, acc, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 0.f, nd.mu, nd.L,

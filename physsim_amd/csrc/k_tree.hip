@@ -526,6 +526,8 @@ struct NodeBuild {
     uint64_t* path;
     int32_t* npts;
     double root_size;           // 4W: full width of the root box
+    double xmin, ymin;          // lower corner of the root box
+    double aspect_x, aspect_y;  // box extent per unit of `size` along x and y (1 and H/W)
     int64_t cap_div;
 };
 
@@ -577,8 +579,18 @@ __device__ void write_node(const NodeBuild& b, int slot, int level, uint64_t pat
     nd.size2 = (float)(size * size);
     nd.child = divided ? child_base : a;
     nd.count = divided ? -1 : cnt;
+    nd.half = (float)(0.5 * size);
+    // box centre from the path: de-interleave the level's quadrant digits
+    uint32_t ix = 0, iy = 0;
+    for (int l = 0; l < level; l++) {
+        const uint32_t dgt = (uint32_t)(path >> (2 * (level - 1 - l))) & 3u;
+        ix = (ix << 1) | (dgt & 1u);
+        iy = (iy << 1) | (dgt >> 1);
+    }
+    nd.bx = (float)(b.xmin + (ix + 0.5) * size * b.aspect_x);
+    nd.by = (float)(b.ymin + (iy + 0.5) * size * b.aspect_y);
     nd.level = level;
-    nd.qxx = nd.qxy = nd.qyy = nd.pad = 0.f;
+    nd.pad = 0.f;
     b.nodes[slot] = nd;
     b.mass64[slot] = m; b.comx64[slot] = comx; b.comy64[slot] = comy; b.qq64[slot] = qq;
     b.path[slot] = path;
@@ -659,6 +671,10 @@ void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, con
     b.nodes = tb.nodes; b.mass64 = tb.node_mass64; b.comx64 = tb.node_comx64; b.comy64 = tb.node_comy64;
     b.qq64 = tb.node_qq64; b.path = tb.node_path; b.npts = tb.node_npts;
     b.root_size = 2.0 * kp.root_hw;
+    b.xmin = kp.xmin;
+    b.ymin = kp.ymin;
+    b.aspect_x = 1.0;
+    b.aspect_y = (kp.scale_x / kp.scale_y);
     b.cap_div = tb.cap_divided;
     int blocks = (int)((tb.cap_divided + 255) / 256);
     if (blocks > 2048) blocks = 2048;

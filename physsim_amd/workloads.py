@@ -73,7 +73,7 @@ def config2(n=65536, replicate=0, side=None):
     angw = spin_to_angw(sv, f32(m))
     st = dict(x=f32(x), y=f32(y), wx=f32(wx), wy=f32(wy), mass=f32(m), charge=f32(q), angw=f32(angw))
     prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.YUKAWA | A.MAGNETIC | A.RELATIVITY | A.BARNES_HUT,
-               boundary_mode=A.BOUND_LOOP, topology=A.TORUS, periodic_mode=A.PERIODIC_MIN_IMAGE,
+               boundary_mode=A.BOUND_LOOP, topology=A.TORUS, periodic_mode=A.PERIODIC_WINDOWED,
                domain_w=float(L), domain_h=float(L), softening=8.0, theta=0.5, yukawa_mu=1.0)
     return st, prm, 1.0 / 128
 
