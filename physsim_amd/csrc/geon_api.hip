@@ -349,6 +349,7 @@ int allocate(GeonContext* h) {
     if ((rc = dmalloc(h, &tb.perm, cap))) return rc;
     if ((rc = dmalloc(h, &tb.perm_alt, cap))) return rc;
     if ((rc = dmalloc(h, &tb.hist, 256 * nblk))) return rc;
+    if ((rc = dmalloc(h, &tb.digit_total, 256))) return rc;
     if ((rc = dmalloc(h, &tb.owned, cap))) return rc;
     if ((rc = dmalloc(h, &tb.rank_start, cap))) return rc;
     if ((rc = dmalloc(h, &tb.divnode, (size_t)tb.cap_divided))) return rc;
@@ -377,6 +378,7 @@ void release(GeonContext* h) {
     hipFree(h->src4); hipFree(h->src8a); hipFree(h->src8b); hipFree(h->ctrl);
     TreeBufs& tb = h->tb;
     hipFree(tb.keys); hipFree(tb.keys_alt); hipFree(tb.perm); hipFree(tb.perm_alt); hipFree(tb.hist);
+    hipFree(tb.digit_total);
     hipFree(tb.owned); hipFree(tb.rank_start); hipFree(tb.divnode); hipFree(tb.pre); hipFree(tb.nodes);
     hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64); hipFree(tb.node_qq64);
     hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);

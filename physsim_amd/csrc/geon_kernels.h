@@ -33,6 +33,7 @@ struct TreeBufs {
     uint64_t *keys, *keys_alt;
     uint32_t *perm, *perm_alt;
     uint32_t* hist;             // 256 * max_blocks
+    uint32_t* digit_total;      // 256
     int32_t *owned, *rank_start;
     int2* divnode;              // (lo, level) per divided node, pre-order rank
     double* pre;                // 8 columns of (cap + 1) prefix sums

@@ -19,6 +19,21 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Max over a 256-thread block, then one atomic per block (every thread of the
+// block must call it).
+__device__ __forceinline__ void block_max_atomic(uint64_t* addr, double v) {
+    __shared__ double s_part[4];
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = s_part[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, s_part[w]);
+        atomic_max_pos_double(addr, m);
+    }
+    __syncthreads();
+}
+
 // Max |F/m|^2 and max cyclotron frequency over the live particles
 // (integrator.js:812-849).
 __global__ void k_dt_scan(StepArgs a) {
@@ -42,12 +57,8 @@ __global__ void k_dt_scan(StepArgs a) {
         }
         if (cyc != cyc) cyc = -1.0;
     }
-    acc2 = wave_max(acc2);
-    cyc = wave_max(cyc);
-    if ((threadIdx.x & 63) == 0) {
-        atomic_max_pos_double(&a.ctrl->max_acc2_fresh, acc2);
-        atomic_max_pos_double(&a.ctrl->max_cyc, cyc);
-    }
+    block_max_atomic(&a.ctrl->max_acc2_fresh, acc2);
+    block_max_atomic(&a.ctrl->max_cyc, cyc);
 }
 
 void launch_dt_scan(hipStream_t s, const StepArgs& a) {
@@ -189,8 +200,7 @@ __global__ void k_kick_drift(StepArgs a) {
         a.vx[i] = vx; a.vy[i] = vy;
         a.angw[i] = angw; a.ang_vel[i] = ang_vel;
     }
-    acc2 = wave_max(acc2);
-    if ((threadIdx.x & 63) == 0) atomic_max_pos_double(&a.ctrl->max_acc2_kick, acc2);
+    block_max_atomic(&a.ctrl->max_acc2_kick, acc2);
 }
 
 void launch_kick_drift(hipStream_t s, const StepArgs& a) {

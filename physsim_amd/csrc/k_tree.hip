@@ -101,24 +101,28 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __r
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (one block).
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* __restrict__ hist, int total) {
-    __shared__ uint32_t part[1024];
+// Per digit (one block each): exclusive scan of that digit's per-block counts,
+// in place, and the digit's total.  The scatter adds the digit's base offset.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scan(uint32_t* __restrict__ hist, int nblocks,
+                                                             uint32_t* __restrict__ digit_total) {
+    __shared__ uint32_t part[kSortThreads];
     const int t = threadIdx.x;
-    const int chunk = (total + 1023) / 1024;
-    const int b = t * chunk, e = min(total, b + chunk);
+    uint32_t* h = hist + (size_t)blockIdx.x * nblocks;
+    const int chunk = (nblocks + kSortThreads - 1) / kSortThreads;
+    const int b = t * chunk, e = min(nblocks, b + chunk);
     uint32_t s = 0;
-    for (int i = b; i < e; i++) s += hist[i];
+    for (int i = b; i < e; i++) s += h[i];
     part[t] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
+    for (int off = 1; off < kSortThreads; off <<= 1) {
         uint32_t v = t >= off ? part[t - off] : 0;
         __syncthreads();
         part[t] += v;
         __syncthreads();
     }
     uint32_t run = part[t] - s;
-    for (int i = b; i < e; i++) { uint32_t v = hist[i]; hist[i] = run; run += v; }
+    for (int i = b; i < e; i++) { uint32_t v = h[i]; h[i] = run; run += v; }
+    if (t == kSortThreads - 1) digit_total[blockIdx.x] = part[t];
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
@@ -126,6 +130,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
                                                                 uint64_t* __restrict__ keys_out,
                                                                 uint32_t* __restrict__ vals_out, int shift,
                                                                 const uint32_t* __restrict__ hist, int nblocks,
+                                                                const uint32_t* __restrict__ digit_total,
                                                                 const Ctrl* __restrict__ ctrl) {
     __shared__ uint64_t s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
@@ -137,9 +142,22 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* 
     const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
     const int base = blockIdx.x * kSortTile;
     if (base >= n) return;
+    // Global offset of (digit, block): exclusive scan of the digit totals plus
+    // this block's offset inside its digit.
+    {
+        const uint32_t tot = digit_total[t];
+        s_goff[t] = tot;
+        __syncthreads();
+        for (int off = 1; off < kRadix; off <<= 1) {
+            uint32_t a = t >= off ? s_goff[t - off] : 0;
+            __syncthreads();
+            s_goff[t] += a;
+            __syncthreads();
+        }
+        s_goff[t] = s_goff[t] - tot + hist[t * nblocks + blockIdx.x];
+    }
     // Tile digit histogram -> digit start offsets inside the tile.
     s_run[t] = 0;
-    s_goff[t] = hist[t * nblocks + blockIdx.x];
     __syncthreads();
     uint64_t k_reg[kSortItems];
     uint32_t v_reg[kSortItems];
@@ -224,9 +242,9 @@ void radix_sort(hipStream_t s, TreeBufs& tb, int n_upper, int key_bits, const Ct
     for (int p = 0; p < passes; p++) {
         const int shift = p * kRadixBits;
         hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, s, kin, shift, tb.hist, nblocks, ctrl);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, tb.hist, nblocks * kRadix);
+        hipLaunchKernelGGL(k_radix_scan, dim3(kRadix), dim3(kSortThreads), 0, s, tb.hist, nblocks, tb.digit_total);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, kout, vout, shift,
-                           tb.hist, nblocks, ctrl);
+                           tb.hist, nblocks, tb.digit_total, ctrl);
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }
