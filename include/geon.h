@@ -209,6 +209,33 @@ int  geon_get_stats(GeonHandle h, GeonStats* out);
 int  geon_build_tree(GeonHandle h);
 int  geon_get_tree(GeonHandle h, GeonTreeNode* out, int64_t cap, int64_t* n_nodes);
 
+/* ---- scalar fields (Higgs, axion) on the field_grid^2 grid ---- */
+#define GEON_FIELD_HIGGS 0
+#define GEON_FIELD_AXION 1
+/* quantity selectors for geon_get_field */
+#define GEON_FQ_FIELD     0   /* phi / a */
+#define GEON_FQ_FIELD_DOT 1
+#define GEON_FQ_GRAD_X    2   /* central differences, grid units */
+#define GEON_FQ_GRAD_Y    3
+#define GEON_FQ_SG_PHI    4   /* self-gravity potential */
+#define GEON_FQ_SG_GX     5
+#define GEON_FQ_SG_GY     6
+#define GEON_FQ_SOURCE    7   /* last PQS deposit */
+#define GEON_FQ_THERMAL   8   /* Higgs only: deposited kinetic energy */
+
+/* Set a field's value and time derivative (n = grid^2 row-major); the grid
+ * gradients are recomputed.  The reference has no such call (its fields start
+ * at the vacuum, higgs-field.js:25-27, axion-field.js:39-41); this is how
+ * seeded initial fields enter.  Either pointer may be NULL (left unchanged). */
+int  geon_set_field(GeonHandle h, int32_t which, const double* field, const double* field_dot, int32_t n);
+/* getFieldBuffers / readbackFieldData (gpu-physics.js:3307, 3844). */
+int  geon_get_field(GeonHandle h, int32_t which, int32_t what, double* out, int32_t n);
+/* computeSelfGravity on the current field state (scalar-field.js:600-660). */
+int  geon_field_selfgravity(GeonHandle h, int32_t which);
+/* The self-gravity kernel's 2-D FFT applied to a caller grid in place
+ * (n x n, n a power of two <= 128; inverse scaled by 1/n per axis as fft.js:63-69). */
+int  geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse);
+
 int  geon_counters(GeonHandle h, GeonCounters* out);
 /* Kernel timing: average device duration (ms) over the launches of `kernel`
  * since the last reset, measured with HIP events on the handle's stream.

@@ -163,3 +163,64 @@ class Oracle:
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def _field_protos(L):
+    if getattr(L, "_field_bound", False):
+        return
+    L.orc_field_set.argtypes = [C.c_void_p, C.c_int32, _DP, _DP]
+    L.orc_field_get.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _DP]
+    L.orc_field_get.restype = C.c_int
+    L.orc_field_selfgrav.argtypes = [C.c_void_p, C.c_int32]
+    L.orc_field_selfgrav.restype = C.c_int
+    L.orc_pqs_deposit.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32, _DP, _DP, _DP, _DP]
+    L.orc_fft2d.argtypes = [_DP, _DP, C.c_int32, C.c_int32]
+    L._field_bound = True
+
+
+FQ = {"field": 0, "field_dot": 1, "grad_x": 2, "grad_y": 3, "sg_phi": 4, "sg_gx": 5, "sg_gy": 6,
+      "source": 7, "thermal": 8, "rho": 9}
+
+
+def fft2d(re, im, inverse=False):
+    L = lib()
+    _field_protos(L)
+    re = np.ascontiguousarray(re, dtype=np.float64).copy()
+    im = np.ascontiguousarray(im, dtype=np.float64).copy()
+    L.orc_fft2d(_d(re), _d(im), re.shape[0], int(inverse))
+    return re, im
+
+
+def pqs_deposit(G, W, H, bc, x, y, v):
+    L = lib()
+    _field_protos(L)
+    out = np.zeros((G, G))
+    xs, ys, vs = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, v))
+    L.orc_pqs_deposit(G, W, H, bc, len(xs), _d(xs), _d(ys), _d(vs), _d(out))
+    return out
+
+
+def _oracle_field_methods():
+    def field_set(self, which, field=None, field_dot=None):
+        _field_protos(self.L)
+        f = np.ascontiguousarray(field, dtype=np.float64).ravel() if field is not None else None
+        fd = np.ascontiguousarray(field_dot, dtype=np.float64).ravel() if field_dot is not None else None
+        self.L.orc_field_set(self.s, which, _d(f), _d(fd))
+
+    def field_get(self, which, what):
+        _field_protos(self.L)
+        G = self.params.field_grid
+        out = np.zeros(G * G)
+        self.L.orc_field_get(self.s, which, FQ[what], _d(out))
+        return out.reshape(G, G)
+
+    def field_selfgrav(self, which):
+        _field_protos(self.L)
+        return self.L.orc_field_selfgrav(self.s, which)
+
+    Oracle.field_set = field_set
+    Oracle.field_get = field_get
+    Oracle.field_selfgrav = field_selfgrav
+
+
+_oracle_field_methods()

@@ -45,6 +45,23 @@
 #define DESPAWN_MARGIN 64.0        /* config.js:138 */
 #define BOSON_CHARGE 0.1           /* config.js:75 */
 #define QT_CAP 4                   /* QUADTREE_CAPACITY (config.js:59) */
+#define SCALAR_FIELD_MAX 2.0       /* config.js:85 */
+#define HIGGS_COUPLING 1.0         /* config.js:89 */
+#define HIGGS_MASS_MAX_DELTA 8.0   /* config.js:91 */
+#define SELFGRAV_PHI_MAX 0.25      /* config.js:92 */
+#define AXION_COUPLING 0.05        /* config.js:96 */
+#define HIGGS_AXION_COUPLING 0.01  /* config.js:97 */
+
+/* One scalar field on a G x G grid (scalar-field.js:13-64). */
+typedef struct {
+    int G;
+    double mass, vac;
+    int higgs;
+    double *field, *fieldDot, *source, *thermal, *rho, *gradX, *gradY;
+    double *sgPhi, *sgGx, *sgGy, *fre, *fim, *gre, *gim;
+    double gW, gH, gSoft2; int gPer, gTopo, gValid;
+    int hasEnergy;
+} OField;
 
 typedef struct {
     double x, y, vx, vy, wx, wy;
@@ -87,6 +104,7 @@ struct OrcSys {
     int grav, coul, mag, gm, onePN, rel, spinOrbit, bh, yuk, higgs, axion, barnesHut, expansion;
     double softeningSq, softening, yukCutoffSq;
     int needAxMod, periodic;
+    OField higgsF, axionF;
 };
 
 static int g_threads = 1;
@@ -801,6 +819,8 @@ static void compute1PN(OrcSys* s) {
     }
 }
 
+#include "geon_oracle_fields.inc"
+
 /* ------------------------------------------------------------------ */
 /* public API                                                          */
 /* ------------------------------------------------------------------ */
@@ -827,11 +847,16 @@ OrcSys* orc_create(const GeonParams* prm, int32_t capacity) {
     s->p = (P*)calloc((size_t)s->capacity, sizeof(P));
     s->root = -1;
     syncToggles(s);
+    const int G = prm->field_grid > 0 ? prm->field_grid : 64;
+    of_init(&s->higgsF, G, 1, prm->higgs_mass);
+    of_init(&s->axionF, G, 0, prm->axion_mass);
     return s;
 }
 
 void orc_destroy(OrcSys* s) {
     if (!s) return;
+    of_free(&s->higgsF);
+    of_free(&s->axionF);
     pool_free(&s->pool);
     free(s->p); free(s->tp); free(s);
 }
@@ -894,9 +919,16 @@ int orc_download(OrcSys* s, double* x, double* y, double* wx, double* wy, double
 static void forcePass(OrcSys* s) {
     if (s->barnesHut) buildTree(s);
     resetForces(s);
-    for (int i = 0; i < s->n; i++) s->p[i].axMod = s->p[i].yukMod = 1;   /* _syncAxionField, no field */
+    if (s->axion) of_axmod(s, &s->axionF);                               /* _syncAxionField (integrator.js:595-605) */
+    else for (int i = 0; i < s->n; i++) s->p[i].axMod = s->p[i].yukMod = 1;
     computeAllForces(s);
     applyExternalFields(s);
+    if (s->higgs) of_apply_forces(s, &s->higgsF);                        /* integrator.js:1494-1507 */
+    if (s->axion) of_apply_forces(s, &s->axionF);
+    if (s->grav) {
+        if (s->higgs) of_grav_forces(s, &s->higgsF);
+        if (s->axion) of_grav_forces(s, &s->axionF);
+    }
 }
 
 int orc_compute_forces(OrcSys* s) {
@@ -1089,11 +1121,20 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
                 if (p->wx != p->wx || p->wy != p->wy) { p->wx = p->wy = 0; p->vx = p->vy = 0; }
             }
         }
+        if (s->higgs) {                                                   /* integrator.js:1320-1329 */
+            of_update(s, &s->higgsF, dtSub, s->axion ? &s->axionF : NULL);
+            of_modulate_masses(s, &s->higgsF, dtSub);
+        }
+        if (s->axion) of_update(s, &s->axionF, dtSub, s->higgs ? &s->higgsF : NULL);   /* 1331-1334 */
         forcePass(s);                                                     /* integrator.js:1489-1493 */
     }
     s->potentialEnergy = 0;
     for (int i = 0; i < n; i++) s->potentialEnergy += s->p[i].pe;
     s->potentialEnergy *= 0.5;                                            /* integrator.js:1743 */
+    if (s->grav) {                                                        /* integrator.js:1744-1749 */
+        if (s->higgs) s->potentialEnergy += of_grav_pe(s, &s->higgsF);
+        if (s->axion) s->potentialEnergy += of_grav_pe(s, &s->axionF);
+    }
     /* Boundary handling once per update (integrator.js:1798-1824). */
     int w = 0;
     const double left = 0, top = 0, right = g->domain_w, bottom = g->domain_h;
@@ -1198,6 +1239,22 @@ int orc_stats(OrcSys* s, GeonStats* out) {
     out->com_x = comX; out->com_y = comY; out->total_mass = totalMass;
     out->pe = s->potentialEnergy;
     out->field_energy = fieldEnergy; out->field_px = fieldPx; out->field_py = fieldPy;
+    const double W = s->prm.domain_w, H = s->prm.domain_h;
+    if (s->higgs) {                                                       /* energy.js:146-176 */
+        out->higgs_field_energy = of_field_energy(&s->higgsF, W, H);
+        if (s->axion) {                                                   /* higgs-field.js:353-364 */
+            const int G = s->higgsF.G;
+            double t = 0;
+            for (int i = 0; i < G * G; i++) t += s->higgsF.field[i] * s->higgsF.field[i] * s->axionF.field[i] * s->axionF.field[i];
+            out->higgs_field_energy += 0.5 * HIGGS_AXION_COUPLING * t * (W / G) * (H / G);
+        }
+        out->pfi_energy += of_pfi_energy(s, &s->higgsF);
+    }
+    if (s->axion) {
+        out->axion_field_energy = of_field_energy(&s->axionF, W, H);
+        out->pfi_energy += of_pfi_energy(s, &s->axionF);
+    }
+    out->pe += out->pfi_energy;                                           /* energy.js:186 */
     out->sim_time = s->simTime;
     out->alive_count = n;
     return 0;
@@ -1304,6 +1361,46 @@ int orc_direct_forces(OrcSys* s, const int32_t* idx, int32_t n_idx, double* fx, 
                       o->magMoment, o->angMomentum, o->axMod, o->yukMod, o->higgsMod, 1);
         }
         fx[k] = tmp.fx; fy[k] = tmp.fy;
+    }
+    return 0;
+}
+
+/* ---- scalar-field access for tests ---- */
+static OField* of_pick(OrcSys* s, int which) { return which == 0 ? &s->higgsF : &s->axionF; }
+
+int orc_field_set(OrcSys* s, int32_t which, const double* field, const double* field_dot) {
+    OField* f = of_pick(s, which);
+    const int G = f->G;
+    if (field) memcpy(f->field, field, sizeof(double) * G * G);
+    if (field_dot) memcpy(f->fieldDot, field_dot, sizeof(double) * G * G);
+    of_grid_gradients(f, s->prm.boundary_mode, s->prm.topology);
+    return 0;
+}
+
+/* what: 0 field, 1 fieldDot, 2 gradX, 3 gradY, 4 sgPhi, 5 sgGx, 6 sgGy, 7 source, 8 thermal, 9 rho */
+int orc_field_get(OrcSys* s, int32_t which, int32_t what, double* out) {
+    OField* f = of_pick(s, which);
+    const double* src[] = {f->field, f->fieldDot, f->gradX, f->gradY, f->sgPhi, f->sgGx, f->sgGy, f->source, f->thermal, f->rho};
+    if (what < 0 || what > 9) return -1;
+    memcpy(out, src[what], sizeof(double) * f->G * f->G);
+    return f->G;
+}
+
+int orc_field_selfgrav(OrcSys* s, int32_t which) {
+    syncToggles(s);
+    OField* f = of_pick(s, which);
+    of_selfgrav(f, s->prm.domain_w, s->prm.domain_h, s->softeningSq, s->prm.boundary_mode, s->prm.topology);
+    return f->hasEnergy;
+}
+
+/* PQS deposit of per-particle values into a G x G grid (scalar-field.js:145-180). */
+int orc_pqs_deposit(int32_t G, double W, double H, int32_t bc, int32_t n, const double* x, const double* y,
+                    const double* v, double* out) {
+    memset(out, 0, sizeof(double) * G * G);
+    for (int i = 0; i < n; i++) {
+        int ix, iy; double wx[4], wy[4];
+        of_pqs(x[i], y[i], G / W, G / H, &ix, &iy, wx, wy);
+        of_deposit_raw(G, out, ix, iy, wx, wy, v[i], bc, GEON_TORUS);
     }
     return 0;
 }

@@ -57,6 +57,14 @@ int     orc_direct_forces(OrcSys* s, const int32_t* idx, int32_t n_idx,
                           double* fx, double* fy);
 void    orc_set_threads(int32_t n);
 
+/* Scalar fields (which: 0 Higgs, 1 axion). */
+int     orc_field_set(OrcSys* s, int32_t which, const double* field, const double* field_dot);
+int     orc_field_get(OrcSys* s, int32_t which, int32_t what, double* out);
+int     orc_field_selfgrav(OrcSys* s, int32_t which);
+int     orc_pqs_deposit(int32_t G, double W, double H, int32_t bc, int32_t n, const double* x, const double* y,
+                        const double* v, double* out);
+void    orc_fft2d(double* re, double* im, int32_t N, int32_t inverse);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,9 @@ COL_PASS, COL_MERGE, COL_BOUNCE = 0, 1, 2
 BOUND_DESPAWN, BOUND_BOUNCE, BOUND_LOOP = 0, 1, 2
 TORUS, KLEIN, RP2 = 0, 1, 2
 PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS, PERIODIC_WINDOWED = 0, 1, 2
+FIELD_HIGGS, FIELD_AXION = 0, 1
+FQ = {"field": 0, "field_dot": 1, "grad_x": 2, "grad_y": 3, "sg_phi": 4, "sg_gx": 5, "sg_gy": 6,
+      "source": 7, "thermal": 8}
 
 TOGGLE_BITS = {
     "gravityEnabled": GRAVITY, "coulombEnabled": COULOMB, "magneticEnabled": MAGNETIC,
@@ -100,6 +103,10 @@ PROTOTYPES = {
     "geon_get_stats": (C.c_int, [H, C.POINTER(GeonStats)]),
     "geon_build_tree": (C.c_int, [H]),
     "geon_get_tree": (C.c_int, [H, C.POINTER(GeonTreeNode), C.c_int64, C.POINTER(C.c_int64)]),
+    "geon_set_field": (C.c_int, [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),
+    "geon_get_field": (C.c_int, [H, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_int32]),
+    "geon_field_selfgravity": (C.c_int, [H, C.c_int32]),
+    "geon_fft2d": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32]),
     "geon_counters": (C.c_int, [H, C.POINTER(GeonCounters)]),
     "geon_timing_enable": (C.c_int, [H, C.c_int32]),
     "geon_kernel_timing": (C.c_int, [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

@@ -26,7 +26,18 @@ enum StateCol {
 };
 // Force columns: results of the last force pass, kept between updates.
 enum ForceCol {
-    K_FX, K_FY, K_PE, K_BZ, K_DBZDX, K_DBZDY, K_BGZ, K_DBGZDX, K_DBGZDY, K_FDRAG, K_TIDAL, K_F1X, K_F1Y, N_FORCE
+    K_FX, K_FY, K_PE, K_BZ, K_DBZDX, K_DBZDY, K_BGZ, K_DBGZDX, K_DBGZDY, K_FDRAG, K_TIDAL, K_F1X, K_F1Y,
+    K_PEG, K_PFI, N_FORCE
+};
+
+// Device state of one scalar field (scalar-field.js:13-64).
+struct FieldState {
+    float *field = nullptr, *field_dot = nullptr, *field_dot_alt = nullptr, *grad_x = nullptr, *grad_y = nullptr;
+    float *source = nullptr, *thermal = nullptr, *sg_phi = nullptr, *sg_gx = nullptr, *sg_gy = nullptr;
+    unsigned long long *dep_src = nullptr, *dep_th = nullptr;
+    float2* green = nullptr;
+    int* has_energy = nullptr;
+    double green_w = -1, green_h = -1, green_soft2 = -1;
 };
 
 constexpr int kTimers = 4;
@@ -63,6 +74,9 @@ struct GeonContext {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kTimers];
     size_t ev_used[kTimers] = {};
     int64_t last_counters[4] = {};
+    FieldState fld[2];          // 0 Higgs, 1 axion
+    int field_g = 0;
+    double* field_out = nullptr;
 };
 
 #define HIP_OK(expr)                                                                         \
@@ -108,9 +122,12 @@ struct Timer {
 
 uint32_t validate(const GeonParams* p, std::string* why) {
     const uint32_t t = p->toggles0;
-    const uint32_t bad = GEON_RADIATION | GEON_DISINTEGRATION | GEON_BOSON_INTER | GEON_BLACK_HOLE | GEON_HIGGS |
-                         GEON_AXION | GEON_ONE_PN;
-    if (t & bad) { *why = "toggle not run by this backend (radiation, disintegration, bosons, black hole, Higgs, axion, 1PN)"; return 1; }
+    const uint32_t bad = GEON_RADIATION | GEON_DISINTEGRATION | GEON_BOSON_INTER | GEON_BLACK_HOLE | GEON_ONE_PN;
+    if (t & bad) { *why = "toggle not run by this backend (radiation, disintegration, bosons, black hole, 1PN)"; return 1; }
+    const int G = p->field_grid;
+    if ((t & (GEON_HIGGS | GEON_AXION)) && (G < 8 || G > 128 || (G & (G - 1)))) {
+        *why = "field_grid must be a power of two in [8, 128]"; return 1;
+    }
     if (p->collision_mode != GEON_COL_PASS) { *why = "only collision mode 'pass' is run"; return 1; }
     if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
     if (p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) { *why = "only the torus topology is run"; return 1; }
@@ -217,6 +234,92 @@ ForceArgs force_args(GeonContext* h) {
     return a;
 }
 
+FieldArgs field_args(GeonContext* h, int which) {
+    FieldState& s = h->fld[which];
+    const GeonParams& p = h->prm;
+    FieldArgs f;
+    f.G = h->field_g;
+    f.bc = p.boundary_mode;
+    f.higgs = which == 0;
+    f.cw = (float)(p.domain_w / f.G);
+    f.ch = (float)(p.domain_h / f.G);
+    f.icw = (float)(f.G / p.domain_w);
+    f.ich = (float)(f.G / p.domain_h);
+    const double area = (p.domain_w / f.G) * (p.domain_h / f.G);
+    f.inv_area = area > 1e-12 ? (float)(1.0 / area) : 0.f;
+    f.vac = which == 0 ? 1.f : 0.f;
+    f.mu2 = (float)(0.5 * p.higgs_mass * p.higgs_mass);          // higgs-field.js:79-81
+    f.m2 = (float)(p.axion_mass * p.axion_mass);                 // axion-field.js:86-88
+    f.damp = which == 0 ? (float)(2 * p.higgs_mass) : (float)(0.05 * p.axion_mass);
+    f.field = s.field; f.field_dot = s.field_dot; f.field_dot_alt = s.field_dot_alt;
+    f.grad_x = s.grad_x; f.grad_y = s.grad_y; f.source = s.source; f.thermal = s.thermal;
+    f.sg_phi = s.sg_phi; f.sg_gx = s.sg_gx; f.sg_gy = s.sg_gy;
+    f.dep_src = s.dep_src; f.dep_th = s.dep_th; f.green = s.green; f.has_energy = s.has_energy;
+    f.ctrl = h->ctrl;
+    return f;
+}
+
+ParticleView particle_view(GeonContext* h) {
+    ParticleView v;
+    v.x = F(h, C_X); v.y = F(h, C_Y); v.q = F(h, C_CHARGE);
+    v.m = F(h, C_MASS); v.base_mass = F(h, C_BASEMASS); v.wx = F(h, C_WX); v.wy = F(h, C_WY);
+    v.vx = F(h, C_VX); v.vy = F(h, C_VY); v.radius = F(h, C_RADIUS);
+    v.ctrl = h->ctrl;
+    return v;
+}
+
+bool fields_on(GeonContext* h) { return (h->prm.toggles0 & (GEON_HIGGS | GEON_AXION)) != 0; }
+
+void ensure_green(GeonContext* h, int which) {
+    FieldState& s = h->fld[which];
+    const GeonParams& p = h->prm;
+    const double soft = (p.toggles0 & GEON_BLACK_HOLE) ? p.bh_softening : p.softening;
+    if (s.green_w == p.domain_w && s.green_h == p.domain_h && s.green_soft2 == soft * soft) return;
+    field_build_green(h->stream, field_args(h, which), (float)(soft * soft));
+    s.green_w = p.domain_w; s.green_h = p.domain_h; s.green_soft2 = soft * soft;
+}
+
+// HiggsField.update + modulateMasses, AxionField.update (integrator.js:1320-1334),
+// on the device; dt from the controller is mirrored on the host (dt_sub).
+void fields_step(GeonContext* h, float dt) {
+    const uint32_t t = h->prm.toggles0;
+    const int grav = !!(t & GEON_GRAVITY), rel = !!(t & GEON_RELATIVITY);
+    const int coul = !!(t & GEON_COULOMB), yuk = !!(t & GEON_YUKAWA);
+    const ParticleView pv = particle_view(h);
+    const int n = h->n_host;
+    if (t & GEON_HIGGS) {
+        if (grav) ensure_green(h, 0);
+        FieldArgs f = field_args(h, 0);
+        field_deposit(h->stream, f, pv, n, rel, coul, yuk);
+        field_kdk(h->stream, f, (t & GEON_AXION) ? h->fld[1].field : nullptr, grav, dt);
+        field_modulate(h->stream, f, pv, F(h, C_HIGGS), dt, n);
+    }
+    if (t & GEON_AXION) {
+        if (grav) ensure_green(h, 1);
+        FieldArgs f = field_args(h, 1);
+        field_deposit(h->stream, f, pv, n, rel, coul, yuk);
+        field_kdk(h->stream, f, (t & GEON_HIGGS) ? h->fld[0].field : nullptr, grav, dt);
+    }
+}
+
+__global__ void k_fill(float* a, int n, float v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Field reset to vacuum (HiggsField.reset / AxionField.reset, scalar-field.js:66-71).
+void reset_fields(GeonContext* h) {
+    const int g2 = h->field_g * h->field_g;
+    for (int w = 0; w < 2; w++) {
+        FieldState& s = h->fld[w];
+        const dim3 gb((g2 + 255) / 256), bb(256);
+        hipLaunchKernelGGL(k_fill, gb, bb, 0, h->stream, s.field, g2, w == 0 ? 1.f : 0.f);
+        float* zero[] = {s.field_dot, s.field_dot_alt, s.grad_x, s.grad_y, s.source, s.thermal, s.sg_phi, s.sg_gx, s.sg_gy};
+        for (float* p : zero) hipLaunchKernelGGL(k_fill, gb, bb, 0, h->stream, p, g2, 0.f);
+        hipMemsetAsync(s.has_energy, 0, sizeof(int), h->stream);
+    }
+}
+
 __global__ void k_reset_counters(Ctrl* c) {
     c->cnt_pair = 0; c->cnt_node = 0; c->cnt_visit = 0; c->depth_overflow = 0; c->overflow = 0;
 }
@@ -267,15 +370,25 @@ int force_pass(GeonContext* h) {
                    F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl);
         h->tree_valid = true;
     }
+    const uint32_t tg = h->prm.toggles0;
+    const bool axion = !!(tg & GEON_AXION), higgs = !!(tg & GEON_HIGGS);
+    if (axion)      // _syncAxionField (integrator.js:595-605)
+        field_axmod(h->stream, field_args(h, 1), particle_view(h), F(h, C_AX), F(h, C_YUK), !!(tg & GEON_COULOMB),
+                    !!(tg & GEON_YUKAWA), n);
     launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
-                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), F(h, C_AX), F(h, C_YUK),
-                           F(h, C_HIGGS), h->src4, h->src8a, h->src8b, n, h->ctrl);
+                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), axion ? F(h, C_AX) : nullptr,
+                           axion ? F(h, C_YUK) : nullptr, higgs ? F(h, C_HIGGS) : nullptr, h->src4, h->src8a,
+                           h->src8b, n, h->ctrl);
     const ForceArgs fa = force_args(h);
     {
         Timer t(h, 0);
         if (bh) launch_walk(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
         else launch_direct(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
     }
+    if (higgs || axion)   // field forces after the pair forces (integrator.js:1494-1507)
+        field_forces(h->stream, field_args(h, 0), field_args(h, 1), higgs, axion, !!(tg & GEON_GRAVITY),
+                     !!(tg & GEON_COULOMB), !!(tg & GEON_YUKAWA), particle_view(h), h->fc[K_FX], h->fc[K_FY],
+                     h->fc[K_PEG], h->fc[K_PFI], n);
     HIP_OK(hipGetLastError());
     return GEON_OK;
 }
@@ -363,8 +476,30 @@ int allocate(GeonContext* h) {
     if ((rc = dmalloc(h, &tb.node_npts, nodes))) return rc;
     if ((rc = dmalloc(h, &tb.scan_tmp, nblk + 1024))) return rc;
     if ((rc = dmalloc(h, &tb.scan_tmp64, 8 * (nblk + 1024)))) return rc;
-    if ((rc = dmalloc(h, &h->stat_partial, 8 * 1024))) return rc;
+    if ((rc = dmalloc(h, &h->stat_partial, 16 * 1024))) return rc;
     if ((rc = dmalloc(h, &h->stat_out, 16))) return rc;
+    h->field_g = h->prm.field_grid > 0 ? h->prm.field_grid : 128;
+    const size_t g2 = (size_t)h->field_g * h->field_g;
+    for (int w = 0; w < 2; w++) {
+        FieldState& s = h->fld[w];
+        if ((rc = dmalloc(h, &s.field, g2))) return rc;
+        if ((rc = dmalloc(h, &s.field_dot, g2))) return rc;
+        if ((rc = dmalloc(h, &s.field_dot_alt, g2))) return rc;
+        if ((rc = dmalloc(h, &s.grad_x, g2))) return rc;
+        if ((rc = dmalloc(h, &s.grad_y, g2))) return rc;
+        if ((rc = dmalloc(h, &s.source, g2))) return rc;
+        if ((rc = dmalloc(h, &s.thermal, g2))) return rc;
+        if ((rc = dmalloc(h, &s.sg_phi, g2))) return rc;
+        if ((rc = dmalloc(h, &s.sg_gx, g2))) return rc;
+        if ((rc = dmalloc(h, &s.sg_gy, g2))) return rc;
+        if ((rc = dmalloc(h, &s.dep_src, g2))) return rc;
+        if ((rc = dmalloc(h, &s.dep_th, g2))) return rc;
+        if ((rc = dmalloc(h, &s.green, g2))) return rc;
+        if ((rc = dmalloc(h, &s.has_energy, 1))) return rc;
+        HIP_OK(hipMemset(s.dep_src, 0, 8 * g2));
+        HIP_OK(hipMemset(s.dep_th, 0, 8 * g2));
+    }
+    if ((rc = dmalloc(h, &h->field_out, 8))) return rc;
     HIP_OK(hipHostMalloc((void**)&h->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
     HIP_OK(hipMemset(h->ctrl, 0, sizeof(Ctrl)));
     HIP_OK(hipEventCreateWithFlags(&h->ctrl_ev, hipEventDisableTiming));
@@ -382,7 +517,14 @@ void release(GeonContext* h) {
     hipFree(tb.owned); hipFree(tb.rank_start); hipFree(tb.divnode); hipFree(tb.pre); hipFree(tb.nodes);
     hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64); hipFree(tb.node_qq64);
     hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
-    hipFree(h->stat_partial); hipFree(h->stat_out);
+    hipFree(h->stat_partial); hipFree(h->stat_out); hipFree(h->field_out);
+    for (int w = 0; w < 2; w++) {
+        FieldState& s = h->fld[w];
+        float* fa[] = {s.field, s.field_dot, s.field_dot_alt, s.grad_x, s.grad_y, s.source, s.thermal,
+                       s.sg_phi, s.sg_gx, s.sg_gy};
+        for (float* p : fa) hipFree(p);
+        hipFree(s.dep_src); hipFree(s.dep_th); hipFree(s.green); hipFree(s.has_energy);
+    }
     if (h->ctrl_host) hipHostFree(h->ctrl_host);
     if (h->ctrl_ev) hipEventDestroy(h->ctrl_ev);
     for (int k = 0; k < kTimers; k++)
@@ -431,6 +573,12 @@ int geon_create(const GeonParams* params, int32_t max_particles, GeonHandle* out
     }
     int rc = allocate(h);
     if (rc) { g_create_error = h->err; release(h); delete h; return rc; }
+    reset_fields(h);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+        g_create_error = "device error during field reset";
+        release(h); delete h;
+        return GEON_EDEVICE;
+    }
     *out = h;
     return GEON_OK;
 }
@@ -447,6 +595,7 @@ int geon_reset(GeonHandle h) {
     h->n_host = 0; h->next_slot = 0; h->forces_init = false; h->tree_valid = false;
     h->sim_time = 0; h->substeps_total = 0;
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0);
+    reset_fields(h);
     HIP_OK(hipStreamSynchronize(h->stream));
     return GEON_OK;
 }
@@ -638,6 +787,7 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
                 sa = step_args(h);
                 launch_kick_drift(h->stream, sa);
             }
+            if (fields_on(h)) fields_step(h, 0.f);
             if ((rc = force_pass(h))) return rc;
             sa = step_args(h);
             launch_dt_scan(h->stream, sa);
@@ -704,6 +854,28 @@ int geon_get_stats(GeonHandle h, GeonStats* out) {
     out->com_x = r[4] > 0 ? r[5] / r[4] : 0; out->com_y = r[4] > 0 ? r[6] / r[4] : 0;
     out->pe = 0.5 * r[7];                       // getPEAccum halves the double-counted sum (forces.js:26)
     out->orbital_ang_mom = r[8]; out->spin_ang_mom = r[9];
+    const uint32_t t = h->prm.toggles0;
+    if ((t & (GEON_HIGGS | GEON_AXION)) && h->n_host > 0) {
+        double pf[2] = {0, 0};
+        launch_sum2(h->stream, h->fc[K_PEG], h->fc[K_PFI], h->n_host, h->ctrl, h->stat_partial, h->stat_out + 12);
+        HIP_OK(hipMemcpyAsync(pf, h->stat_out + 12, 16, hipMemcpyDeviceToHost, h->stream));
+        double fe[2][4] = {};
+        for (int w = 0; w < 2; w++) {
+            if (!(t & (w == 0 ? GEON_HIGGS : GEON_AXION))) continue;
+            const float* partner = (w == 0 && (t & GEON_AXION)) ? h->fld[1].field : nullptr;
+            field_energy(h->stream, field_args(h, w), partner, h->field_out);
+            HIP_OK(hipMemcpyAsync(fe[w], h->field_out, 32, hipMemcpyDeviceToHost, h->stream));
+            HIP_OK(hipStreamSynchronize(h->stream));
+        }
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (t & GEON_GRAVITY) out->pe += pf[0];  // gravPE of the fields (integrator.js:1744-1749)
+        out->pfi_energy = pf[1];
+        out->pe += pf[1];                        // energy.js:186
+        out->higgs_field_energy = fe[0][0] + fe[0][3];
+        out->axion_field_energy = fe[1][0];
+        out->field_px += fe[0][1] + fe[1][1];
+        out->field_py += fe[0][2] + fe[1][2];
+    }
     out->sim_time = h->sim_time;
     out->alive_count = h->n_host;
     return GEON_OK;
@@ -753,6 +925,67 @@ int geon_get_tree(GeonHandle h, GeonTreeNode* out, int64_t cap, int64_t* n_nodes
         out[i].com_x = cx[i];
         out[i].com_y = cy[i];
     }
+    return GEON_OK;
+}
+
+int geon_set_field(GeonHandle h, int32_t which, const double* field, const double* field_dot, int32_t n) {
+    if (!h || which < 0 || which > 1) return GEON_EINVAL;
+    const int g2 = h->field_g * h->field_g;
+    if (n != g2) return set_err(h, GEON_EINVAL, "n must be field_grid^2");
+    FieldState& s = h->fld[which];
+    std::vector<float> buf((size_t)g2);
+    if (field) {
+        for (int i = 0; i < g2; i++) buf[i] = (float)field[i];
+        HIP_OK(hipMemcpyAsync(s.field, buf.data(), 4 * (size_t)g2, hipMemcpyHostToDevice, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+    if (field_dot) {
+        for (int i = 0; i < g2; i++) buf[i] = (float)field_dot[i];
+        HIP_OK(hipMemcpyAsync(s.field_dot, buf.data(), 4 * (size_t)g2, hipMemcpyHostToDevice, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+    field_grads(h->stream, field_args(h, which));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
+int geon_get_field(GeonHandle h, int32_t which, int32_t what, double* out, int32_t n) {
+    if (!h || !out || which < 0 || which > 1) return GEON_EINVAL;
+    const int g2 = h->field_g * h->field_g;
+    if (n != g2) return set_err(h, GEON_EINVAL, "n must be field_grid^2");
+    FieldState& s = h->fld[which];
+    const float* src[] = {s.field, s.field_dot, s.grad_x, s.grad_y, s.sg_phi, s.sg_gx, s.sg_gy, s.source, s.thermal};
+    if (what < 0 || what > 8) return set_err(h, GEON_EINVAL, "bad field quantity");
+    std::vector<float> buf((size_t)g2);
+    HIP_OK(hipMemcpyAsync(buf.data(), src[what], 4 * (size_t)g2, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < g2; i++) out[i] = buf[i];
+    return GEON_OK;
+}
+
+int geon_field_selfgravity(GeonHandle h, int32_t which) {
+    if (!h || which < 0 || which > 1) return GEON_EINVAL;
+    ensure_green(h, which);
+    field_selfgrav(h->stream, field_args(h, which));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
+int geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse) {
+    if (!h || !re || !im) return GEON_EINVAL;
+    if (n < 2 || n > 128 || (n & (n - 1)) || n > h->field_g) return set_err(h, GEON_EINVAL, "n must be a power of two <= field_grid");
+    // scratch: the axion field's G^ buffer is rebuilt afterwards
+    FieldState& s = h->fld[1];
+    std::vector<float2> buf((size_t)n * n);
+    for (int i = 0; i < n * n; i++) buf[i] = make_float2((float)re[i], (float)im[i]);
+    HIP_OK(hipMemcpyAsync(s.green, buf.data(), sizeof(float2) * n * n, hipMemcpyHostToDevice, h->stream));
+    FieldArgs f = field_args(h, 1);
+    f.G = n;
+    field_fft(h->stream, f, inverse);
+    HIP_OK(hipMemcpyAsync(buf.data(), s.green, sizeof(float2) * n * n, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < n * n; i++) { re[i] = buf[i].x; im[i] = buf[i].y; }
+    s.green_w = -1;     // force a rebuild of the axion G^
     return GEON_OK;
 }
 

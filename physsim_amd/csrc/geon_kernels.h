@@ -89,6 +89,39 @@ void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const
                             const float* yuk, const float* higgs, float4* src4, float4* src8a,
                             float4* src8b, int n_upper, const Ctrl* ctrl);
 
+// ---- scalar fields (k_field.hip) ----
+struct FieldArgs {
+    int G, bc, higgs;
+    float cw, ch, icw, ich, inv_area;
+    float vac;                  // vacuum value: Higgs 1, axion 0
+    float mu2, m2, damp;        // Higgs: mu^2 = m_H^2/2, damp 2 m_H; axion: m_a^2, damp g m_a
+    float *field, *field_dot, *field_dot_alt, *grad_x, *grad_y;
+    float *source, *thermal, *sg_phi, *sg_gx, *sg_gy;
+    unsigned long long *dep_src, *dep_th;   // 2^32 fixed-point deposit accumulators
+    float2* green;              // G^ (FFT of the softened Green's function)
+    int* has_energy;
+    const Ctrl* ctrl;           // non-null: time-stepping kernels take dt_sub / active from it
+};
+
+struct ParticleView {
+    const float *x, *y, *q;
+    float *m, *base_mass, *wx, *wy, *vx, *vy, *radius;
+    const Ctrl* ctrl;
+};
+
+void field_build_green(hipStream_t s, const FieldArgs& f, float soft2);
+void field_selfgrav(hipStream_t s, const FieldArgs& f);
+void field_fft(hipStream_t s, const FieldArgs& f, int inverse);   // in place on f.green
+void field_deposit(hipStream_t s, const FieldArgs& f, const ParticleView& p, int n_upper, int rel, int coul, int yuk);
+void field_grads(hipStream_t s, const FieldArgs& f);
+void field_kdk(hipStream_t s, FieldArgs f, const float* portal, int sg_on, float dt);
+void field_modulate(hipStream_t s, const FieldArgs& f, const ParticleView& p, float* higgs_mod, float dt, int n_upper);
+void field_axmod(hipStream_t s, const FieldArgs& f, const ParticleView& p, float* ax, float* yuk, int coul, int yukawa,
+                 int n_upper);
+void field_forces(hipStream_t s, const FieldArgs& h, const FieldArgs& a, int use_h, int use_a, int grav, int coul,
+                  int yukawa, const ParticleView& p, float* fx, float* fy, float* pe_grav, float* pfi, int n_upper);
+void field_energy(hipStream_t s, const FieldArgs& f, const float* partner, double* out);
+
 // ---- integrator + reductions (k_step.hip) ----
 struct StepArgs {
     float *x, *y, *wx, *wy, *vx, *vy, *angw, *ang_vel, *mass, *radius, *charge;
@@ -107,5 +140,7 @@ void launch_vv_kick(hipStream_t s, const StepArgs& a);
 void launch_boundary(hipStream_t s, const StepArgs& a, int mode, double w, double h);
 // FP64 two-level reductions for the conservation readout; out[0..15].
 void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel);
+void launch_sum2(hipStream_t s, const float* a, const float* b, int n_upper, const Ctrl* ctrl, double* partial,
+                 double* out);
 
 }  // namespace geon

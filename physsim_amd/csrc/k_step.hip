@@ -334,6 +334,39 @@ __global__ __launch_bounds__(kStatThreads) void k_stats_pass2(StepArgs a, const 
     if (threadIdx.x < 2) partial[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
 }
 
+// FP64 sums of two per-particle FP32 columns (deterministic two-level).
+__global__ __launch_bounds__(kStatThreads) void k_sum2(const float* __restrict__ a, const float* __restrict__ b,
+                                                       const Ctrl* __restrict__ ctrl, double* __restrict__ partial,
+                                                       int nb) {
+    __shared__ double sh[2][kStatThreads];
+    const int n = ctrl->n_alive;
+    double sa = 0, sb = 0;
+    for (int i = blockIdx.x * kStatThreads + threadIdx.x; i < n; i += nb * kStatThreads) {
+        sa += (double)a[i];
+        sb += (double)b[i];
+    }
+    sh[0][threadIdx.x] = sa;
+    sh[1][threadIdx.x] = sb;
+    __syncthreads();
+    for (int off = kStatThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) partial[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+void launch_sum2(hipStream_t s, const float* a, const float* b, int n_upper, const Ctrl* ctrl, double* partial,
+                 double* out) {
+    int nb = (n_upper + kStatThreads - 1) / kStatThreads;
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_sum2, dim3(nb), dim3(kStatThreads), 0, s, a, b, ctrl, partial, nb);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, partial, nb, 2, out);
+}
+
 void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel) {
     int nb = (a.n_upper + kStatThreads - 1) / kStatThreads;
     if (nb > 1024) nb = 1024;

@@ -187,6 +187,28 @@ class HipPhysics:
         self.requestStats()
         return self.readStats()
 
+    # ---- scalar fields ---------------------------------------------------
+    def set_field(self, which, field=None, field_dot=None):
+        G = self.params.field_grid
+        f = np.ascontiguousarray(field, dtype=np.float64).ravel() if field is not None else None
+        fd = np.ascontiguousarray(field_dot, dtype=np.float64).ravel() if field_dot is not None else None
+        self._check(self.lib.geon_set_field(self.h, int(which), _dptr(f), _dptr(fd), G * G))
+
+    def get_field(self, which, what="field"):              # readbackFieldData (gpu-physics.js:3844)
+        G = self.params.field_grid
+        out = np.zeros(G * G)
+        self._check(self.lib.geon_get_field(self.h, int(which), A.FQ[what], _dptr(out), G * G))
+        return out.reshape(G, G)
+
+    def field_selfgravity(self, which):
+        self._check(self.lib.geon_field_selfgravity(self.h, int(which)))
+
+    def fft2d(self, re, im, inverse=False):
+        re = np.ascontiguousarray(re, dtype=np.float64).copy()
+        im = np.ascontiguousarray(im, dtype=np.float64).copy()
+        self._check(self.lib.geon_fft2d(self.h, _dptr(re), _dptr(im), re.shape[0], int(bool(inverse))))
+        return re, im
+
     # ---- tree / diagnostics ---------------------------------------------
     def build_tree(self):
         self._check(self.lib.geon_build_tree(self.h))

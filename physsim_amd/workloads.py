@@ -82,6 +82,44 @@ def config2(n=65536, replicate=0, side=None, open_box=False):
     return st, prm, 1.0 / 128
 
 
+def gaussian_field(G, g, rms, index=-2.0):
+    """Gaussian random field with power spectrum P(k) ~ k^index, zero mean, given rms."""
+    white = np.fft.fft2(g.normal(size=(G, G)))
+    k = np.sqrt(np.fft.fftfreq(G)[:, None] ** 2 + np.fft.fftfreq(G)[None, :] ** 2)
+    k[0, 0] = np.inf
+    f = np.fft.ifft2(white * k ** (index / 2)).real
+    f -= f.mean()
+    return f * (rms / f.std())
+
+
+def config3(n=1 << 20, replicate=0, grid=128, side=1024.0, total_mass=16.0):
+    """Projected Plummer sphere (scale radius 20, truncated at 0.45 W) with
+    virialised Gaussian velocities, gravity, Higgs and axion fields on a
+    grid^2 PQS grid with FFT self-gravity; initial fields Gaussian random with
+    spectral index -2 and rms 0.01 about the vacuum (Higgs 1, axion 0).
+    Masses: lognormal(0, 0.5) scaled to total_mass, so the virial speed stays
+    well below c."""
+    g = rng(3, replicate)
+    a = 20.0
+    rmax = 0.45 * side
+    u = g.random(n) * (rmax ** 2 / (rmax ** 2 + a ** 2))
+    R = a * np.sqrt(u / (1 - u))
+    phi = 2 * np.pi * g.random(n)
+    x = side / 2 + R * np.cos(phi)
+    y = side / 2 + R * np.sin(phi)
+    m = g.lognormal(0.0, 0.5, n)
+    m *= total_mass / m.sum()
+    # virial ratio K/|W| = 1/2 with the Plummer binding energy |W| = (3 pi / 32) M^2 / a
+    sigma = np.sqrt((3 * np.pi / 32) * total_mass / (2 * a))
+    wx = g.normal(0, sigma, n)
+    wy = g.normal(0, sigma, n)
+    st = dict(x=f32(x), y=f32(y), wx=f32(wx), wy=f32(wy), mass=f32(m), charge=np.zeros(n), angw=np.zeros(n))
+    prm = dict(toggles0=A.GRAVITY | A.HIGGS | A.AXION | A.BARNES_HUT, boundary_mode=A.BOUND_DESPAWN,
+               domain_w=float(side), domain_h=float(side), softening=8.0, theta=0.5, field_grid=grid)
+    fields = dict(higgs=f32(1.0 + gaussian_field(grid, g, 0.01)), axion=f32(gaussian_field(grid, g, 0.01)))
+    return st, prm, 1.0 / 128, fields
+
+
 def make_params(lib_default, **kw):
     for k, v in kw.items():
         setattr(lib_default, k, v)

@@ -1,0 +1,125 @@
+"""Scalar-field path on the GPU (through the C ABI) against numpy and the FP64
+oracle: LDS FFT, FFT self-gravity, PQS deposit, KDK field step and the
+particle-side couplings.  Boundary modes are named in each case."""
+import numpy as np
+import pytest
+
+from physsim_amd import _abi as A
+from physsim_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(prm, n=16):
+    from physsim_amd import HipPhysics, default_params
+    print("case: toggles=0x%x grid=%d" % (prm["toggles0"], prm.get("field_grid", 128)), flush=True)
+    return HipPhysics(default_params(**prm), max_particles=max(n, 1))
+
+
+def _rel(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / max(np.sum(b ** 2), 1e-300)))
+
+
+@pytest.mark.parametrize("n", [16, 64, 128])
+def test_lds_fft_matches_numpy(n):
+    g = _gpu(dict(toggles0=A.GRAVITY, field_grid=128))
+    r = np.random.default_rng(n)
+    re, im = r.normal(size=(n, n)), r.normal(size=(n, n))
+    a, b = g.fft2d(re, im)
+    F = np.fft.fft2(re + 1j * im)
+    assert _rel(a + 1j * b, F) < 2e-6
+    c, d = g.fft2d(a, b, inverse=True)
+    assert _rel(c + 1j * d, re + 1j * im) < 2e-6
+
+
+@pytest.mark.parametrize("bc", [A.BOUND_LOOP, A.BOUND_DESPAWN])
+def test_selfgravity_matches_oracle(bc, oracle_lib):
+    G = 64
+    prm = dict(toggles0=A.GRAVITY | A.AXION, field_grid=G, domain_w=256.0, domain_h=256.0, softening=8.0,
+               boundary_mode=bc, axion_mass=0.2)
+    r = np.random.default_rng(3)
+    a = W.f32(W.gaussian_field(G, r, 0.05))
+    ad = W.f32(0.02 * r.normal(size=(G, G)))
+    g = _gpu(prm)
+    g.set_field(1, a, ad)
+    g.field_selfgravity(1)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1)
+    o.field_set(1, a, ad)
+    o.field_selfgrav(1)
+    for what in ("sg_phi", "sg_gx", "sg_gy"):
+        assert _rel(g.get_field(1, what), o.field_get(1, what)) < 2e-5, what
+
+
+def _config3_pair(oracle_lib, n=4096, G=64, toggles=None):
+    from physsim_amd import HipPhysics, default_params
+    st, prm, dt, fields = W.config3(n, grid=G, side=256.0, total_mass=4.0)
+    if toggles is not None:
+        prm["toggles0"] = toggles
+    print("case: config3 n=%d G=%d toggles=0x%x" % (n, G, prm["toggles0"]), flush=True)
+    g = HipPhysics(default_params(**prm), max_particles=n)
+    g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    g.set_field(0, fields["higgs"])
+    g.set_field(1, fields["axion"])
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), n)
+    o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    o.field_set(0, fields["higgs"])
+    o.field_set(1, fields["axion"])
+    return g, o, dt
+
+
+def test_field_forces_at_bootstrap(oracle_lib):
+    """Higgs gradient force + field gravity (no gravity walk difference: BH at the same theta)."""
+    g, o, dt = _config3_pair(oracle_lib)
+    g.compute_forces()
+    o.compute_forces()
+    n = o.count()
+    fg, fo = g.forces(n), o.forces()
+    assert _rel(np.hypot(fg["fx"], fg["fy"]), np.hypot(fo["fx"], fo["fy"])) < 1e-4
+
+
+def test_field_update_matches_oracle(oracle_lib):
+    """Deposit, KDK with self-gravity and portal coupling, mass modulation: 3 updates."""
+    g, o, dt = _config3_pair(oracle_lib)
+    for step in range(3):
+        sg, so = g.update(dt), o.update(dt)
+        assert sg == so, step
+    for which in (0, 1):
+        assert _rel(g.get_field(which, "source"), o.field_get(which, "source")) < 1e-5
+        dev = g.get_field(which, "field") - (1.0 if which == 0 else 0.0)
+        odev = o.field_get(which, "field") - (1.0 if which == 0 else 0.0)
+        assert _rel(dev, odev) < 2e-3, which
+        assert _rel(g.get_field(which, "field_dot"), o.field_get(which, "field_dot")) < 5e-3, which
+    dg, do = g.download(), o.download()
+    ids = do["id"]
+    np.testing.assert_allclose(dg["mass"][ids], do["mass"], rtol=2e-5)
+    np.testing.assert_allclose(dg["x"][ids], do["x"], atol=1e-3)
+    sg_, so_ = g.stats(), o.stats()
+    assert sg_["higgsFieldEnergy"] == pytest.approx(so_["higgs_field_energy"], rel=1e-4)
+    assert sg_["axionFieldEnergy"] == pytest.approx(so_["axion_field_energy"], rel=1e-3)
+    assert sg_["pfiEnergy"] == pytest.approx(so_["pfi_energy"], rel=1e-3, abs=1e-9)
+
+
+def test_klein_gordon_mode_on_gpu():
+    """Same closed form as the oracle's pin, FP32 on the device."""
+    from physsim_amd import HipPhysics, default_params
+    G, Wd, m = 32, 64.0, 0.3
+    prm = dict(toggles0=A.AXION, field_grid=G, domain_w=Wd, domain_h=Wd, boundary_mode=A.BOUND_LOOP, axion_mass=m)
+    g = HipPhysics(default_params(**prm), max_particles=1)
+    g.upload([1.0], [1.0], [0.0], [0.0], [1.0], [0.0])
+    dx = Wd / G
+    kx = 2 * np.pi / Wd
+    x = np.arange(G) * dx
+    A0 = 0.01
+    g.set_field(1, np.tile(A0 * np.cos(kx * x), (G, 1)), np.zeros((G, G)))
+    k2 = (2 - 2 * np.cos(kx * dx)) / dx ** 2
+    nu = 0.5 / np.sqrt(2 / dx ** 2)
+    gam = 0.05 * m + nu * k2
+    wd = np.sqrt(m * m + k2 - gam * gam / 4)
+    dt, steps = 1.0 / 512, 1600
+    for _ in range(steps):
+        g.update(dt)
+    a = g.get_field(1, "field")[0]
+    amp = np.dot(a, np.cos(kx * x)) * 2 / G
+    t = dt * steps
+    want = A0 * np.exp(-gam * t / 2) * (np.cos(wd * t) + gam / (2 * wd) * np.sin(wd * t))
+    assert amp == pytest.approx(want, rel=2e-3)

@@ -1,0 +1,365 @@
+"""Pins of the FP64 oracle (oracle/geon_oracle.c) against closed forms,
+invariants, brute force and library routines.
+
+The reference ships no tests, golden vectors or fixtures for this path, so
+these checks are what fixes the oracle: each is chosen so that a plausible
+slip (dropped term, wrong sign, swapped index, off-by-one stencil) fails.
+"""
+import numpy as np
+import pytest
+
+from physsim_amd import _abi as A
+from physsim_amd import workloads as W
+
+
+def two_body(oracle_lib, toggles, m=(2.0, 3.0), q=(0.3, -0.5), d=5.0, soft=1.0, w=((0, 0), (0, 0)), angw=(0, 0),
+             rel=False, **kw):
+    prm = oracle_lib.default_params(toggles0=toggles | (A.RELATIVITY if rel else 0), softening=soft,
+                                    domain_w=64.0, domain_h=64.0, **kw)
+    o = oracle_lib.Oracle(prm, 2)
+    o.upload(np.array([30.0, 30.0 + d]), np.array([32.0, 32.0]), np.array([w[0][0], w[1][0]]),
+             np.array([w[0][1], w[1][1]]), np.array(m), np.array(q), np.array(angw))
+    o.compute_forces()
+    return o
+
+
+def test_gravity_closed_form(oracle_lib):
+    o = two_body(oracle_lib, A.GRAVITY, d=5.0, soft=1.0)
+    f = o.forces()
+    expect = 2.0 * 3.0 * 5.0 / (25.0 + 1.0) ** 1.5      # m1 m2 r / (r^2 + eps^2)^(3/2)
+    assert f["fx"][0] == pytest.approx(expect, rel=1e-14)
+    assert f["fx"][1] == pytest.approx(-expect, rel=1e-14)
+    assert f["fy"][0] == 0.0
+    assert f["pe_total"] == pytest.approx(-6.0 / np.sqrt(26.0), rel=1e-14)
+
+
+def test_coulomb_sign_and_antisymmetry(oracle_lib):
+    o = two_body(oracle_lib, A.COULOMB, q=(0.3, -0.5), d=4.0, soft=2.0)
+    f = o.forces()
+    expect = 0.3 * 0.5 * 4.0 / (16.0 + 4.0) ** 1.5      # opposite charges attract
+    assert f["fx"][0] == pytest.approx(expect, rel=1e-14)
+    assert f["fx"][0] == pytest.approx(-f["fx"][1], rel=1e-14)
+    assert f["pe_total"] == pytest.approx(0.3 * -0.5 / np.sqrt(20.0), rel=1e-14)
+
+
+def test_yukawa_closed_form_and_cutoff(oracle_lib):
+    mu = 0.5
+    o = two_body(oracle_lib, A.YUKAWA, d=3.0, soft=0.5, yukawa_mu=mu)
+    f = o.forces()
+    r = np.sqrt(9.0 + 0.25)
+    expect = 14.0 * 6.0 * np.exp(-mu * r) * (1 / r ** 2 + mu / r) * 3.0 / r
+    assert f["fx"][0] == pytest.approx(expect, rel=1e-13)
+    far = two_body(oracle_lib, A.YUKAWA, d=13.0, soft=0.5, yukawa_mu=mu)   # r > 6/mu: skipped (forces.js:82-83)
+    assert far.forces()["fx"][0] == 0.0
+
+
+def test_magnetic_dipole_field_of_moving_charge(oracle_lib):
+    # source 1 moving along +y at v, observer 0 at distance d along -x: Bz = q v x r / r^3
+    v = 0.2
+    o = two_body(oracle_lib, A.MAGNETIC, q=(0.0, 0.7), d=5.0, soft=1.0, w=((0, 0), (0, v)))
+    f = o.forces()
+    r2 = 25.0 + 1.0
+    # cross_sv = svx*ry - svy*rx with r = source - observer = (5, 0): -v*5
+    assert f["bz"][0] == pytest.approx(0.7 * (-v * 5.0) / r2 ** 1.5, rel=1e-13)
+
+
+def test_gravitomagnetic_sign_convention(oracle_lib):
+    v = 0.2
+    o = two_body(oracle_lib, A.GRAVITOMAG, m=(1.0, 4.0), d=5.0, soft=1.0, w=((0, 0), (0, v)))
+    f = o.forces()
+    assert f["bgz"][0] == pytest.approx(-4.0 * (-v * 5.0) / 26.0 ** 1.5, rel=1e-13)
+
+
+def test_bh_theta_zero_equals_direct_sum(oracle_lib):
+    st, prm, _ = W.config1(300)
+    prm["theta"] = 1e-9                 # nothing accepted: every source visited as a leaf point
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 300)
+    o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"])
+    o.compute_forces()
+    f = o.forces()
+    fx, fy = o.direct_forces()
+    np.testing.assert_allclose(f["fx"], fx, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(f["fy"], fy, rtol=1e-12, atol=1e-12)
+    c = o.counters()
+    assert c["pair"] == 300 * 299 and c["node"] == 0
+
+
+def test_bh_root_only_monopole(oracle_lib):
+    # a compact cluster far away: the walk accepts its top node, giving the monopole force
+    n = 40
+    g = np.random.default_rng(3)
+    xs = np.concatenate([[2.0], 60 + 0.5 * g.random(n - 1)])
+    ys = np.concatenate([[2.0], 60 + 0.5 * g.random(n - 1)])
+    m = np.concatenate([[1.0], 1 + g.random(n - 1)])
+    prm = dict(toggles0=A.GRAVITY | A.BARNES_HUT, softening=0.1, theta=0.5)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), n)
+    z = np.zeros(n)
+    o.upload(xs, ys, z, z, m, z)
+    o.compute_forces()
+    f = o.forces()
+    M = m[1:].sum()
+    cx, cy = (m[1:] * xs[1:]).sum() / M, (m[1:] * ys[1:]).sum() / M
+    dx, dy = cx - 2.0, cy - 2.0
+    r3 = (dx * dx + dy * dy + 0.01) ** 1.5
+    assert f["fx"][0] == pytest.approx(M * dx / r3, rel=1e-12)
+    assert f["fy"][0] == pytest.approx(M * dy / r3, rel=1e-12)
+
+
+def test_bh_error_decreases_with_theta(oracle_lib):
+    st, prm, _ = W.config1(1500)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1500)
+    o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"])
+    fx, fy = o.direct_forces()
+    errs = []
+    for theta in (0.8, 0.5, 0.3):
+        p = oracle_lib.default_params(**dict(prm, theta=theta))
+        o.set_params(p)
+        o.compute_forces()
+        f = o.forces()
+        errs.append(np.sqrt(np.sum((f["fx"] - fx) ** 2 + (f["fy"] - fy) ** 2) / np.sum(fx ** 2 + fy ** 2)))
+    assert errs[0] > errs[1] > errs[2] > 0
+
+
+def _brute_tree(xs, ys, W_, H_):
+    """Recursive count: a node is divided iff > 4 points fall in it (ties to the
+    lower child), down to depth 48 (quadtree.js:185-237)."""
+    out = {}
+
+    def rec(idx, cx, cy, hw, hh, level, path):
+        if level > 40:
+            return
+        divided = len(idx) > 4
+        out[(level, path)] = (int(divided), len(idx))
+        if not divided:
+            return
+        west = xs[idx] <= cx
+        north = ys[idx] <= cy
+        for q, sel in enumerate((north & west, north & ~west, ~north & west, ~north & ~west)):
+            ox = -hw / 2 if q in (0, 2) else hw / 2
+            oy = -hh / 2 if q in (0, 1) else hh / 2
+            rec(idx[sel], cx + ox, cy + oy, hw / 2, hh / 2, level + 1, (path << 2) | q)
+
+    rec(np.arange(len(xs)), W_ / 2, H_ / 2, 2 * W_, 2 * H_, 0, 0)
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 5, 6, 200, 2000])
+def test_tree_matches_brute_force_count(n, oracle_lib):
+    st, prm, _ = W.config1(n)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), n)
+    o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"])
+    t = o.tree()
+    got = {(int(r["level"]), int(r["path"])): (int(r["divided"]), int(r["count"])) for r in t}
+    assert got == _brute_tree(st["x"], st["y"], 64.0, 64.0)
+    # invariants: children of every divided node exist; root holds every point
+    assert t[0]["count"] == n
+    assert t[0]["charge_quanta"] == int(np.round(st["charge"] / 0.1).sum())
+    np.testing.assert_allclose(t[0]["mass"], st["mass"].sum(), rtol=1e-13)
+    np.testing.assert_allclose(t[0]["com_x"], (st["mass"] * st["x"]).sum() / st["mass"].sum(), rtol=1e-13)
+
+
+def test_boris_rotation_preserves_speed(oracle_lib):
+    # a charge in a uniform external Bz, no other force: |w| is preserved exactly up to rounding,
+    # and the rotation angle per substep is 2 atan(t) with t = q Bz dt / (2 m)
+    prm = dict(toggles0=A.MAGNETIC, ext_bz=0.5, softening=1.0, domain_w=64.0, domain_h=64.0)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1)
+    o.upload(np.array([32.0]), np.array([32.0]), np.array([0.1]), np.array([0.0]), np.array([2.0]), np.array([0.4]))
+    steps = o.update(0.05)
+    d = o.download()
+    assert steps >= 1
+    assert np.hypot(d["wx"][0], d["wy"][0]) == pytest.approx(0.1, rel=1e-14)
+    t = 0.4 * 0.5 / (2 * 2.0) * (0.05 / steps)
+    ang = -steps * 2 * np.arctan(t)
+    assert np.arctan2(d["wy"][0], d["wx"][0]) == pytest.approx(ang, rel=1e-12)
+
+
+def test_uniform_field_exact_kick_and_drift(oracle_lib):
+    # external gravity g on a lone particle: w += g dt, x += w_new dt (kick-drift)
+    g = 0.3
+    prm = dict(toggles0=A.GRAVITY, ext_gravity=g, ext_gravity_angle=0.0, softening=1.0, domain_w=64.0, domain_h=64.0)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1)
+    o.upload(np.array([32.0]), np.array([32.0]), np.array([0.0]), np.array([0.0]), np.array([1.0]), np.array([0.0]))
+    dt = 0.01
+    n = o.update(dt)
+    d = o.download()
+    assert d["wx"][0] == pytest.approx(g * dt, rel=1e-13)
+    sub = dt / n
+    assert d["x"][0] == pytest.approx(32.0 + sum((k + 1) * g * sub * sub for k in range(n)), rel=1e-13)
+
+
+def test_adaptive_substeps_formula(oracle_lib):
+    # dtSafe = sqrt(softening / a_max); substeps = ceil(dt / dtSafe) (integrator.js:862-868).
+    # a = 32, softening 2: dtSafe = 1/4 exactly, so every quotient below is exact in FP64.
+    prm = dict(toggles0=A.GRAVITY, ext_gravity=32.0, ext_gravity_angle=0.0, softening=2.0,
+               domain_w=64.0, domain_h=64.0)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1)
+    o.upload(np.array([32.0]), np.array([32.0]), np.array([0.0]), np.array([0.0]), np.array([1.0]), np.array([0.0]))
+    assert o.update(1.0) == 4
+    assert o.update(0.6) == 3          # ceil(0.6 / 0.25)
+    # cyclotron limit: dt <= (pi/4) / (|q| Bz / m) when tighter
+    prm2 = dict(toggles0=A.MAGNETIC, ext_bz=np.pi, softening=2.0, domain_w=64.0, domain_h=64.0)
+    o2 = oracle_lib.Oracle(oracle_lib.default_params(**prm2), 1)
+    o2.upload(np.array([32.0]), np.array([32.0]), np.array([0.0]), np.array([0.0]), np.array([1.0]), np.array([1.0]))
+    o2.compute_forces()
+    assert o2.update(1.0) == 4         # (pi/4)/pi = 1/4
+
+
+def test_kepler_energy_conservation(oracle_lib):
+    # two bodies on a circular orbit: FP64 energy drift tiny at small dt
+    M, m, r = 10.0, 0.01, 8.0
+    v = np.sqrt(M / r)
+    prm = dict(toggles0=A.GRAVITY, softening=1e-3, domain_w=64.0, domain_h=64.0)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 2)
+    o.upload(np.array([32.0, 32.0 + r]), np.array([32.0, 32.0]), np.array([0.0, 0.0]), np.array([-m * v / M, v]),
+             np.array([M, m]), np.array([0.0, 0.0]))
+    o.update(0.01)
+    s0 = o.stats()
+    e0 = s0["linear_ke"] + s0["pe"]
+    for _ in range(400):
+        o.update(0.01)
+    s1 = o.stats()
+    e1 = s1["linear_ke"] + s1["pe"]
+    assert abs(e1 - e0) / abs(e0) < 1e-6
+    assert abs(s1["orbital_ang_mom"] - s0["orbital_ang_mom"]) / abs(s0["orbital_ang_mom"]) < 1e-10
+
+
+def test_collision_pairs_brute_force(oracle_lib):
+    g = np.random.default_rng(11)
+    n = 300
+    x = 20 + 24 * g.random(n)
+    y = 20 + 24 * g.random(n)
+    m = 0.2 + g.random(n)
+    o = oracle_lib.Oracle(oracle_lib.default_params(), n)
+    o.upload(x, y, np.zeros(n), np.zeros(n), m, np.zeros(n))
+    a, b = o.collision_pairs()
+    r = np.cbrt(m)
+    # collisions.js:39-60: the pair (i < j) is looked for only from i, inside i's query
+    # box of half-size 2 r_i, so an overlap farther than 2 r_i along an axis is missed
+    want = sorted((i, j) for i in range(n) for j in range(i + 1, n)
+                  if (x[i] - x[j]) ** 2 + (y[i] - y[j]) ** 2 < (r[i] + r[j]) ** 2
+                  and abs(x[j] - x[i]) <= 2 * r[i] and abs(y[j] - y[i]) <= 2 * r[i])
+    assert list(zip(a.tolist(), b.tolist())) == want
+    assert len(want) > 10
+
+
+def test_min_image_torus(oracle_lib):
+    # periodic: a source just across the boundary acts through the short way round
+    prm = oracle_lib.default_params(toggles0=A.GRAVITY, softening=0.5, boundary_mode=A.BOUND_LOOP,
+                                    domain_w=64.0, domain_h=64.0)
+    o = oracle_lib.Oracle(prm, 2)
+    o.upload(np.array([1.0, 63.0]), np.array([32.0, 32.0]), np.zeros(2), np.zeros(2), np.ones(2), np.zeros(2))
+    o.compute_forces()
+    f = o.forces()
+    assert f["fx"][0] == pytest.approx(-2.0 / (4.0 + 0.25) ** 1.5, rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# scalar fields
+# ---------------------------------------------------------------------------
+def test_fft_matches_numpy_and_parseval(oracle_lib):
+    g = np.random.default_rng(5)
+    re, im = g.normal(size=(32, 32)), g.normal(size=(32, 32))
+    a, b = oracle_lib.fft2d(re, im)
+    F = np.fft.fft2(re + 1j * im)
+    np.testing.assert_allclose(a, F.real, atol=1e-12)
+    np.testing.assert_allclose(b, F.imag, atol=1e-12)
+    assert np.sum(a ** 2 + b ** 2) == pytest.approx(32 * 32 * np.sum(re ** 2 + im ** 2), rel=1e-12)
+    ra, rb = oracle_lib.fft2d(a, b, inverse=True)
+    np.testing.assert_allclose(ra, re, atol=1e-12)
+    delta = np.zeros((16, 16)); delta[0, 0] = 1.0
+    c, d = oracle_lib.fft2d(delta, np.zeros((16, 16)))
+    np.testing.assert_allclose(c, 1.0, atol=1e-15)
+
+
+def test_pqs_partition_of_unity_and_linear_reproduction(oracle_lib):
+    g = np.random.default_rng(7)
+    G, Wd = 32, 64.0
+    x, y = Wd * g.random(500), Wd * g.random(500)
+    dep = oracle_lib.pqs_deposit(G, Wd, Wd, A.BOUND_LOOP, x, y, np.ones(500))
+    assert dep.sum() == pytest.approx(500.0, rel=1e-13)         # periodic deposit conserves the total
+    # interpolation of a linear field (interior) reproduces it exactly: use the
+    # deposit's transpose: sum_j w_j f_j with f = grid-centre coordinates
+    cell = Wd / G
+    xc = (np.arange(G) + 0.5) * cell
+    px, py = 17.3, 40.9
+    gx, gy = px / cell - 0.5, py / cell - 0.5
+    ix, iy = int(np.floor(gx)), int(np.floor(gy))
+    d = gx - ix
+    wx = np.array([(1 - d) ** 3 / 6, (4 - 6 * d * d + 3 * d ** 3) / 6, (1 + 3 * d + 3 * d * d - 3 * d ** 3) / 6, d ** 3 / 6])
+    assert wx.sum() == pytest.approx(1.0, rel=1e-15)
+    assert (wx * xc[ix - 1: ix + 3]).sum() == pytest.approx(px, rel=1e-13)
+    one = oracle_lib.pqs_deposit(G, Wd, Wd, A.BOUND_LOOP, [px], [py], [1.0])
+    assert (one.sum(axis=0)[ix - 1: ix + 3] * xc[ix - 1: ix + 3]).sum() == pytest.approx(px, rel=1e-13)
+
+
+def test_selfgravity_is_circular_convolution(oracle_lib):
+    G = 16
+    prm = oracle_lib.default_params(toggles0=A.GRAVITY | A.AXION, field_grid=G, domain_w=32.0, domain_h=32.0,
+                                    softening=1.5, boundary_mode=A.BOUND_LOOP, axion_mass=0.3)
+    o = oracle_lib.Oracle(prm, 1)
+    g = np.random.default_rng(9)
+    a = 0.2 * g.normal(size=(G, G))
+    ad = 0.1 * g.normal(size=(G, G))
+    o.field_set(1, a, ad)
+    assert o.field_selfgrav(1) == 1
+    rho = o.field_get(1, "rho")
+    phi = o.field_get(1, "sg_phi")
+    cw = 32.0 / G
+    # brute force: Phi_i = sum_j G(r_ij) rho_j dA with min-image distances
+    idx = np.arange(G)
+    d = (idx[:, None] - idx[None, :]) * cw
+    d = np.where(d > 16.0, d - 32.0, np.where(d < -16.0, d + 32.0, d))
+    want = np.zeros((G, G))
+    for iy in range(G):
+        for ix in range(G):
+            dy = d[iy][:, None]
+            dx = d[ix][None, :]
+            want[iy, ix] = np.sum(-1.0 / np.sqrt(dx ** 2 + dy ** 2 + 1.5 ** 2) * rho) * cw * cw
+    np.testing.assert_allclose(phi, want, rtol=1e-10, atol=1e-12)
+    gx = o.field_get(1, "sg_gx")
+    np.testing.assert_allclose(gx, (np.roll(want, -1, 1) - np.roll(want, 1, 1)) * 0.5, atol=1e-12)
+
+
+def test_klein_gordon_single_mode(oracle_lib):
+    """Axion without sources or gravity on a periodic grid: a single Fourier mode is a
+    damped oscillator a'' = -(m^2 + k^2) a - (g m + nu k^2) a' with the discrete
+    Laplacian's k^2 = (2 - 2 cos(k dx)) / dx^2 (axion-field.js:129-190)."""
+    G, Wd, m = 32, 64.0, 0.3
+    prm = oracle_lib.default_params(toggles0=A.AXION, field_grid=G, domain_w=Wd, domain_h=Wd,
+                                    boundary_mode=A.BOUND_LOOP, axion_mass=m)
+    o = oracle_lib.Oracle(prm, 1)
+    o.upload(np.array([1.0]), np.array([1.0]), np.zeros(1), np.zeros(1), np.ones(1), np.zeros(1))
+    # The viscosity term clamps to the edge on the grid border even on a torus
+    # (scalar-field.js:254-276), which perturbs short modes; the longest mode is
+    # nearly untouched, and the KDK error then shrinks as O(dt).
+    dx = Wd / G
+    kx = 2 * np.pi * 1 / Wd
+    x = (np.arange(G) + 0.0) * dx
+    A0 = 0.01
+    a0 = np.tile(A0 * np.cos(kx * x), (G, 1))
+    o.field_set(1, a0, np.zeros((G, G)))
+    k2 = (2 - 2 * np.cos(kx * dx)) / dx ** 2
+    nu = 0.5 / np.sqrt(2 / dx ** 2)
+    gam = 0.05 * m + nu * k2
+    w2 = m * m + k2
+    dt, steps = 1.0 / 512, 1600
+    for _ in range(steps):
+        o.update(dt)
+    a = o.field_get(1, "field")[0]
+    amp = np.dot(a, np.cos(kx * x)) * 2 / G
+    t = dt * steps
+    wd = np.sqrt(w2 - gam * gam / 4)
+    want = A0 * np.exp(-gam * t / 2) * (np.cos(wd * t) + gam / (2 * wd) * np.sin(wd * t))
+    assert amp == pytest.approx(want, rel=4e-4)
+    # without the damping and mass terms the answer would be far off: the pin is sharp
+    assert abs(A0 * np.cos(np.sqrt(k2) * t) - want) > 10 * 4e-4 * abs(want)
+
+
+def test_higgs_vacuum_stays_vacuum(oracle_lib):
+    prm = oracle_lib.default_params(toggles0=A.HIGGS | A.GRAVITY, field_grid=16, domain_w=32.0, domain_h=32.0,
+                                    boundary_mode=A.BOUND_LOOP)
+    o = oracle_lib.Oracle(prm, 1)
+    o.upload(np.array([16.0]), np.array([16.0]), np.zeros(1), np.zeros(1), np.array([1e-13]), np.zeros(1))
+    for _ in range(5):
+        o.update(1.0 / 128)
+    np.testing.assert_allclose(o.field_get(0, "field"), 1.0, atol=1e-15)
