@@ -13,7 +13,7 @@ LIB = os.path.join(ROOT, "physsim_amd", "libgeon_hip.so")
 
 def declared_functions():
     src = open(HDR).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(geon_[a-z_]+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(geon_[a-z0-9_]+)\s*\(", src, re.M)))
 
 
 @pytest.fixture(scope="module")
