@@ -122,8 +122,8 @@ struct Timer {
 
 uint32_t validate(const GeonParams* p, std::string* why) {
     const uint32_t t = p->toggles0;
-    const uint32_t bad = GEON_RADIATION | GEON_DISINTEGRATION | GEON_BOSON_INTER | GEON_BLACK_HOLE | GEON_ONE_PN;
-    if (t & bad) { *why = "toggle not run by this backend (radiation, disintegration, bosons, black hole, 1PN)"; return 1; }
+    const uint32_t bad = GEON_RADIATION | GEON_DISINTEGRATION | GEON_BOSON_INTER | GEON_BLACK_HOLE;
+    if (t & bad) { *why = "toggle not run by this backend (radiation, disintegration, bosons, black hole)"; return 1; }
     const int G = p->field_grid;
     if ((t & (GEON_HIGGS | GEON_AXION)) && (G < 8 || G > 128 || (G & (G - 1)))) {
         *why = "field_grid must be a power of two in [8, 128]"; return 1;
@@ -357,17 +357,23 @@ int sort_and_gather(GeonContext* h) {
 
 // computeAllForces + external fields at the current state (forces.js:61-154,
 // integrator.js:563-592), preceded by the tree rebuild (integrator.js:1340-1348).
-int force_pass(GeonContext* h) {
+// reuse_tree: the tree built for this substep's 1PN pass stands (integrator.js:1340-1348
+// reuses it; with Higgs on, its aggregates are recomputed for the new masses).
+int force_pass(GeonContext* h, bool reuse_tree = false) {
     const int n = h->n_host;
     const bool bh = !!(h->prm.toggles0 & GEON_BARNES_HUT);
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
     if (n == 0) return GEON_OK;
     if (bh) {
-        int rc = sort_and_gather(h);
-        if (rc) return rc;
-        Timer t(h, 1);
-        build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
-                   F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+        if (!reuse_tree) {
+            int rc = sort_and_gather(h);
+            if (rc) return rc;
+        }
+        if (!reuse_tree || (h->prm.toggles0 & GEON_HIGGS)) {
+            Timer t(h, 1);
+            build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
+                       F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+        }
         h->tree_valid = true;
     }
     const uint32_t tg = h->prm.toggles0;
@@ -389,6 +395,33 @@ int force_pass(GeonContext* h) {
         field_forces(h->stream, field_args(h, 0), field_args(h, 1), higgs, axion, !!(tg & GEON_GRAVITY),
                      !!(tg & GEON_COULOMB), !!(tg & GEON_YUKAWA), particle_view(h), h->fc[K_FX], h->fc[K_FY],
                      h->fc[K_PEG], h->fc[K_PFI], n);
+    HIP_OK(hipGetLastError());
+    return GEON_OK;
+}
+
+// 1PN velocity-Verlet correction (integrator.js:1296-1318): tree at the drifted
+// positions, compute1PN, then w += (F1PN_new - F1PN_old) dt / 2m.  The moment
+// caches are left as the last force pass wrote them, as in the reference.
+int one_pn_step(GeonContext* h) {
+    const int n = h->n_host;
+    if (n == 0) return GEON_OK;
+    const bool bh = !!(h->prm.toggles0 & GEON_BARNES_HUT);
+    if (bh) {
+        int rc = sort_and_gather(h);
+        if (rc) return rc;
+        Timer t(h, 1);
+        build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
+                   F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+        h->tree_valid = true;
+    }
+    const uint32_t tg = h->prm.toggles0;
+    launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
+                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L),
+                           (tg & GEON_AXION) ? F(h, C_AX) : nullptr, (tg & GEON_AXION) ? F(h, C_YUK) : nullptr,
+                           (tg & GEON_HIGGS) ? F(h, C_HIGGS) : nullptr, h->src4, h->src8a, h->src8b, n, h->ctrl,
+                           /*update_moments=*/0);
+    launch_walk1pn(h->stream, force_args(h), bh);
+    launch_vv_kick(h->stream, step_args(h));
     HIP_OK(hipGetLastError());
     return GEON_OK;
 }
@@ -787,8 +820,10 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
                 sa = step_args(h);
                 launch_kick_drift(h->stream, sa);
             }
+            const bool one_pn = !!(h->prm.toggles0 & GEON_ONE_PN);
+            if (one_pn && (rc = one_pn_step(h))) return rc;
             if (fields_on(h)) fields_step(h, 0.f);
-            if ((rc = force_pass(h))) return rc;
+            if ((rc = force_pass(h, one_pn && (h->prm.toggles0 & GEON_BARNES_HUT)))) return rc;
             sa = step_args(h);
             launch_dt_scan(h->stream, sa);
             HIP_OK(hipEventSynchronize(h->ctrl_ev));

@@ -87,7 +87,9 @@ void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const
                             const float* q, const float* vx, const float* vy, const float* ang_vel,
                             const float* radius, float* mu, float* L, const float* ax,
                             const float* yuk, const float* higgs, float4* src4, float4* src8a,
-                            float4* src8b, int n_upper, const Ctrl* ctrl);
+                            float4* src8b, int n_upper, const Ctrl* ctrl, int update_moments = 1);
+// 1PN recompute for the velocity-Verlet correction: tree walk or O(N^2).
+void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree);
 
 // ---- scalar fields (k_field.hip) ----
 struct FieldArgs {

@@ -25,18 +25,24 @@ __global__ void k_prepare_sources(const float* __restrict__ x, const float* __re
                                   float* __restrict__ mu, float* __restrict__ L, const float* __restrict__ ax,
                                   const float* __restrict__ yuk, const float* __restrict__ higgs,
                                   float4* __restrict__ src4, float4* __restrict__ src8a,
-                                  float4* __restrict__ src8b, const Ctrl* __restrict__ ctrl) {
+                                  float4* __restrict__ src8b, int update_moments, const Ctrl* __restrict__ ctrl) {
     const int n = ctrl->n_alive;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float r = radius[i];
     const float r2 = r * r;
     const float w = ang_vel[i];
-    // forces.js:71-76: cached dipole moments for this substep
-    const float mui = 0.2f * q[i] * w * r2;
-    const float Li = 0.4f * m[i] * w * r2;
-    mu[i] = mui;
-    L[i] = Li;
+    float mui, Li;
+    if (update_moments) {
+        // forces.js:71-76: cached dipole moments for this substep
+        mui = 0.2f * q[i] * w * r2;
+        Li = 0.4f * m[i] * w * r2;
+        mu[i] = mui;
+        L[i] = Li;
+    } else {
+        mui = mu[i];
+        Li = L[i];
+    }
     src4[i] = make_float4(x[i], y[i], m[i], q[i]);
     src8a[i] = make_float4(vx[i], vy[i], w, mui);
     src8b[i] = make_float4(Li, ax ? ax[i] : 1.f, yuk ? yuk[i] : 1.f, higgs ? higgs[i] : 1.f);
@@ -45,10 +51,10 @@ __global__ void k_prepare_sources(const float* __restrict__ x, const float* __re
 void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const float* m, const float* q,
                             const float* vx, const float* vy, const float* ang_vel, const float* radius, float* mu,
                             float* L, const float* ax, const float* yuk, const float* higgs, float4* src4,
-                            float4* src8a, float4* src8b, int n_upper, const Ctrl* ctrl) {
+                            float4* src8a, float4* src8b, int n_upper, const Ctrl* ctrl, int update_moments) {
     if (n_upper <= 0) return;
     hipLaunchKernelGGL(k_prepare_sources, dim3((n_upper + 255) / 256), dim3(256), 0, s, x, y, m, q, vx, vy, ang_vel,
-                       radius, mu, L, ax, yuk, higgs, src4, src8a, src8b, ctrl);
+                       radius, mu, L, ax, yuk, higgs, src4, src8a, src8b, update_moments, ctrl);
 }
 
 __device__ __forceinline__ void load_obs(const ForceArgs& a, int i, Obs& o) {
@@ -178,6 +184,122 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
     if (overflow && lane == 0) a.ctrl->overflow = 2;
     wave_add_counters(a.ctrl, c_pair, c_node, c_visit);
     if (valid) store_acc<KAHAN>(a, i, o, acc);
+}
+
+// 1PN recompute for the velocity-Verlet correction: the same wavefront walk
+// with _accum1PN at leaves and accepted nodes (forces.js:511-571); leaf sources
+// at the current time (the reference always takes their signal-delayed state).
+__global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
+    __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
+    __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
+    if (!a.ctrl->active) return;
+    const int n = a.ctrl->n_alive;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int i = blockIdx.x * kWalkThreads + threadIdx.x;
+    const bool valid = i < n;
+    const uint64_t active = __ballot(valid);
+    if (active == 0) return;
+    const uint64_t my_bit = 1ull << lane;
+    Obs o;
+    if (valid) load_obs(a, i, o);
+    else { o.x = o.y = o.vx = o.vy = o.m = o.q = o.ang_vel = o.mu = o.L = 0.f; o.ax = o.yuk = o.higgs = 1.f; o.tidal_ri5 = 0.f; o.inv_m = 0.f; }
+    float f1x = 0.f, f1y = 0.f;
+    const PhysFlags pf = a.pf;
+    const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    int sp = 0;
+    if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
+    __builtin_amdgcn_wave_barrier();
+    sp = 1;
+    bool overflow = false;
+    while (sp > 0) {
+        sp--;
+        const int node = __builtin_amdgcn_readfirstlane(s_node[wv][sp]);
+        const uint64_t mask = s_mask[wv][sp];
+        const bool mine = (mask & my_bit) != 0;
+        if (node < 0 || node >= a.max_nodes) { overflow = true; continue; }
+        const Node nd = a.nodes[node];
+        if (nd.mass < kEpsMass) continue;
+        if (nd.count > 0) {
+            for (int j = nd.child; j < nd.child + nd.count; j++) {
+                const float4 s4 = a.src4[j];
+                const float4 s8a = a.src8a[j];
+                const float4 s8b = a.src8b[j];
+                if (mine && j != i) {
+                    float rx = s4.x - o.x, ry = s4.y - o.y;
+                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                    accum_1pn(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8b.z, s8b.w);
+                }
+            }
+        } else if (nd.count < 0) {
+            const float rawx = nd.comx - o.x, rawy = nd.comy - o.y;
+            float dx = rawx, dy = rawy;
+            if (pf.periodic) { dx = torus_wrap(dx, pf.dom_w, half_w); dy = torus_wrap(dy, pf.dom_h, half_h); }
+            bool accept = mine && (nd.size2 < pf.theta2 * (dx * dx + dy * dy));
+            if (pf.window) {
+                const float cx = nd.bx - o.x + (dx - rawx), cy = nd.by - o.y + (dy - rawy);
+                accept = accept && (fabsf(cx) + nd.half <= half_w) && (fabsf(cy) + nd.half * (pf.dom_h / pf.dom_w) <= half_h);
+            }
+            if (accept) accum_1pn(o, f1x, f1y, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 1.f, 1.f);
+            const uint64_t open = __ballot(mine && !accept);
+            if (open) {
+                if (sp + 4 <= kWalkStack) {
+                    if (lane == 0) {
+                        s_node[wv][sp] = nd.child;     s_mask[wv][sp] = open;
+                        s_node[wv][sp + 1] = nd.child + 1; s_mask[wv][sp + 1] = open;
+                        s_node[wv][sp + 2] = nd.child + 2; s_mask[wv][sp + 2] = open;
+                        s_node[wv][sp + 3] = nd.child + 3; s_mask[wv][sp + 3] = open;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    sp += 4;
+                } else {
+                    overflow = true;
+                }
+            }
+        }
+    }
+    if (overflow && lane == 0) a.ctrl->overflow = 2;
+    if (valid) { a.f1x[i] = f1x; a.f1y[i] = f1y; }
+}
+
+// O(N^2) 1PN recompute (forces.js:593-608), sources at the current time.
+__global__ __launch_bounds__(kWalkThreads) void k_direct1pn(ForceArgs a) {
+    __shared__ float4 t4[kWalkThreads];
+    __shared__ float4 t8a[kWalkThreads];
+    __shared__ float4 t8b[kWalkThreads];
+    if (!a.ctrl->active) return;
+    const int n = a.ctrl->n_alive;
+    const int i = blockIdx.x * kWalkThreads + threadIdx.x;
+    const bool valid = i < n;
+    Obs o;
+    if (valid) load_obs(a, i, o);
+    else { o.x = o.y = o.vx = o.vy = o.m = o.q = o.ang_vel = o.mu = o.L = 0.f; o.ax = o.yuk = o.higgs = 1.f; o.tidal_ri5 = 0.f; o.inv_m = 0.f; }
+    float f1x = 0.f, f1y = 0.f;
+    const PhysFlags pf = a.pf;
+    const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    for (int base = 0; base < n; base += kWalkThreads) {
+        const int j = base + threadIdx.x;
+        __syncthreads();
+        if (j < n) { t4[threadIdx.x] = a.src4[j]; t8a[threadIdx.x] = a.src8a[j]; t8b[threadIdx.x] = a.src8b[j]; }
+        __syncthreads();
+        const int cnt = min(kWalkThreads, n - base);
+        if (valid) {
+            for (int k = 0; k < cnt; k++) {
+                if (base + k == i) continue;
+                float rx = t4[k].x - o.x, ry = t4[k].y - o.y;
+                if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                accum_1pn(o, f1x, f1y, pf, rx, ry, t4[k].z, t4[k].w, t8a[k].x, t8a[k].y, t8b[k].z, t8b[k].w);
+            }
+        }
+    }
+    if (valid) { a.f1x[i] = f1x; a.f1y[i] = f1y; }
+}
+
+void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree) {
+    if (a.n_upper <= 0) return;
+    const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
+    if (tree) hipLaunchKernelGGL(k_walk1pn, dim3(grid), dim3(kWalkThreads), 0, s, a);
+    else hipLaunchKernelGGL(k_direct1pn, dim3(grid), dim3(kWalkThreads), 0, s, a);
 }
 
 // Direct sum (forces.js:93-126): each block stages 256 sources at a time in LDS.

@@ -227,3 +227,23 @@ def test_windowed_walk_converges_to_direct_sum(oracle_lib):
         errs.append(_rel_l2(fg["fx"], fg["fy"], fx, fy))
         g.destroy()
     assert errs[1] < errs[0] and errs[1] < 0.02, errs
+
+
+@pytest.mark.parametrize("bh", [True, False])
+def test_one_pn_velocity_verlet_matches_oracle(bh, oracle_lib):
+    """Gravity + gravitomagnetism + 1PN (EIH) with the velocity-Verlet 1PN correction
+    (integrator.js:1296-1318), relativistic; BH walk or O(N^2)."""
+    st, prm, dt, _ = W.config3(2048, grid=64, side=256.0, total_mass=4.0)
+    prm["toggles0"] = A.GRAVITY | A.GRAVITOMAG | A.ONE_PN | A.RELATIVITY | (A.BARNES_HUT if bh else 0)
+    g, o = _pair(st, prm, oracle_lib)
+    g.compute_forces()
+    o.compute_forces()
+    fg, fo = g.forces(len(st["x"])), o.forces()
+    assert _rel_l2(fg["fx"], fg["fy"], fo["fx"], fo["fy"]) < 1e-4
+    for step in range(3):
+        sg, so = g.update(dt), o.update(dt)
+        assert sg == so, step
+    dg, do = g.download(), o.download()
+    ids = do["id"]
+    wg = np.hypot(dg["wx"][ids], dg["wy"][ids])
+    assert np.sqrt(np.sum((dg["wx"][ids] - do["wx"]) ** 2 + (dg["wy"][ids] - do["wy"]) ** 2) / np.sum(wg ** 2)) < 1e-4

@@ -363,3 +363,16 @@ def test_higgs_vacuum_stays_vacuum(oracle_lib):
     for _ in range(5):
         o.update(1.0 / 128)
     np.testing.assert_allclose(o.field_get(0, "field"), 1.0, atol=1e-15)
+
+
+def test_eih_two_body_at_rest(oracle_lib):
+    """1PN EIH term for two bodies at rest: F = m1 m2 (5 m1 + 4 m2) r_vec / r^4
+    (forces.js:249-285 with v = 0), on top of Newton."""
+    m1, m2, d, soft = 2.0, 3.0, 5.0, 1.0
+    o = two_body(oracle_lib, A.GRAVITY | A.GRAVITOMAG | A.ONE_PN, m=(m1, m2), d=d, soft=soft, rel=True)
+    f = o.forces()
+    r = np.sqrt(d * d + soft * soft)
+    newton = m1 * m2 * d / r ** 3
+    eih = m1 * m2 * d / r ** 3 * (5 * m1 + 4 * m2) / r
+    assert f["fx"][0] == pytest.approx(newton + eih, rel=1e-13)
+    assert f["fx"][1] == pytest.approx(-(newton + m1 * m2 * d / r ** 3 * (5 * m2 + 4 * m1) / r), rel=1e-13)
