@@ -39,24 +39,25 @@ __device__ __forceinline__ void block_max_atomic(uint64_t* addr, double v) {
 __global__ void k_dt_scan(StepArgs a) {
     const int n = a.ctrl->n_alive;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc2 = -1.0, cyc = -1.0;
+    float acc2f = -1.f, cycf = -1.f;
     if (i < n) {
-        const double m = a.mass[i];
-        const double inv_m = 1.0 / m;
-        const double fx = a.fx[i], fy = a.fy[i];
-        acc2 = (fx * fx + fy * fy) * inv_m * inv_m;
-        if (acc2 != acc2) acc2 = -1.0;
-        cyc = 0.0;
+        // FP32 per particle (the forces are FP32); only the block maxima go to FP64
+        const float inv_m = 1.f / a.mass[i];
+        const float fx = a.fx[i] * inv_m, fy = a.fy[i] * inv_m;
+        acc2f = fx * fx + fy * fy;
+        if (acc2f != acc2f) acc2f = -1.f;
+        cycf = 0.f;
         if (a.mag || a.ext_bz_on) {
-            const double absbz = fabs((double)a.bz[i]);
-            if (absbz > 0) cyc = fmax(cyc, fabs((double)a.charge[i]) * absbz * inv_m);
+            const float absbz = fabsf(a.bz[i]);
+            if (absbz > 0.f) cycf = fmaxf(cycf, fabsf(a.charge[i]) * absbz * inv_m);
         }
         if (a.gm) {
-            const double absbgz = fabs((double)a.bgz[i]);
-            if (absbgz > 0) cyc = fmax(cyc, 4.0 * absbgz);
+            const float absbgz = fabsf(a.bgz[i]);
+            if (absbgz > 0.f) cycf = fmaxf(cycf, 4.f * absbgz);
         }
-        if (cyc != cyc) cyc = -1.0;
+        if (cycf != cycf) cycf = -1.f;
     }
+    const double acc2 = acc2f, cyc = cycf;
     block_max_atomic(&a.ctrl->max_acc2_fresh, acc2);
     block_max_atomic(&a.ctrl->max_cyc, cyc);
 }
@@ -299,12 +300,14 @@ __global__ __launch_bounds__(kStatThreads) void k_stats_pass1(StepArgs a, const 
     if (threadIdx.x < kStatCols) partial[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
 }
 
+// One wave per column: lane-strided partial sums, then a fixed-order wave reduction.
 __global__ void k_stats_final(const double* __restrict__ partial, int nb, int ncols, double* __restrict__ out) {
-    const int c = threadIdx.x;
+    const int c = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (c >= ncols) return;
     double s = 0;
-    for (int b = 0; b < nb; b++) s += partial[c * nb + b];
-    out[c] = s;
+    for (int b = lane; b < nb; b += 64) s += partial[c * nb + b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) out[c] = s;
 }
 
 __global__ __launch_bounds__(kStatThreads) void k_stats_pass2(StepArgs a, const double* __restrict__ sums,
@@ -364,7 +367,7 @@ void launch_sum2(hipStream_t s, const float* a, const float* b, int n_upper, con
     if (nb > 1024) nb = 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_sum2, dim3(nb), dim3(kStatThreads), 0, s, a, b, ctrl, partial, nb);
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, partial, nb, 2, out);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(2 * 64), 0, s, partial, nb, 2, out);
 }
 
 void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel) {
@@ -372,9 +375,9 @@ void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* par
     if (nb > 1024) nb = 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_stats_pass1, dim3(nb), dim3(kStatThreads), 0, s, a, pe, partial, nb, rel);
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, partial, nb, kStatCols, out);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kStatCols * 64), 0, s, partial, nb, kStatCols, out);
     hipLaunchKernelGGL(k_stats_pass2, dim3(nb), dim3(kStatThreads), 0, s, a, out, partial, nb);
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, partial, nb, 2, out + kStatCols);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(2 * 64), 0, s, partial, nb, 2, out + kStatCols);
 }
 
 }  // namespace geon

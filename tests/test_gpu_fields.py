@@ -17,7 +17,7 @@ def _gpu(prm, n=16):
 
 
 def _rel(a, b):
-    return float(np.sqrt(np.sum((a - b) ** 2) / max(np.sum(b ** 2), 1e-300)))
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / max(np.sum(np.abs(b) ** 2), 1e-300)))
 
 
 @pytest.mark.parametrize("n", [16, 64, 128])
