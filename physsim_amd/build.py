@@ -45,7 +45,24 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_napi()
     return LIB
+
+
+NAPI_DIR = os.path.join(HERE, "napi")
+NAPI_ADDON = os.path.join(NAPI_DIR, "geon_napi.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_napi():
+    """The Node-API addon over the C ABI (needs only the node headers)."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    tmp = NAPI_ADDON + ".tmp"
+    _run(["gcc", "-O2", "-shared", "-fPIC", "-I" + NODE_INCLUDE, "-DNODE_GYP_MODULE_NAME=geon_napi",
+          os.path.join(NAPI_DIR, "geon_napi.c"), "-L" + HERE, "-lgeon_hip", "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+    os.replace(tmp, NAPI_ADDON)
+    return NAPI_ADDON
 
 
 if __name__ == "__main__":

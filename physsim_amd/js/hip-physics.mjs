@@ -1,0 +1,119 @@
+// hip-physics.mjs — the gfx950 backend as a third physics backend for the
+// reference app, beside CPUPhysics (src/cpu-physics.js) and GPUPhysics
+// (src/gpu/gpu-physics.js).  It keeps GPUPhysics' method names and argument
+// meanings (addParticle returns -1 when full, vx/vy are proper velocity,
+// charge is quantized, readStats returns the same fields); every call goes
+// through the Node-API addon to the C ABI of libgeon_hip.so.
+import { createRequire } from 'module';
+import { fileURLToPath } from 'url';
+import path from 'path';
+
+const require = createRequire(import.meta.url);
+const here = path.dirname(fileURLToPath(import.meta.url));
+const addon = require(path.join(here, '..', 'napi', 'geon_napi.node'));
+
+// toggles0 bits: src/gpu/gpu-constants.js:196-213
+export const TOGGLE_BITS = {
+    gravityEnabled: 1, coulombEnabled: 2, magneticEnabled: 4, gravitomagEnabled: 8,
+    onePNEnabled: 16, relativityEnabled: 32, spinOrbitEnabled: 64, radiationEnabled: 128,
+    blackHoleEnabled: 256, disintegrationEnabled: 512, expansionEnabled: 1024, yukawaEnabled: 2048,
+    higgsEnabled: 4096, axionEnabled: 8192, barnesHutEnabled: 16384, bosonInterEnabled: 32768,
+};
+
+export const BACKEND_HIP = 'hip';
+
+export default class HipPhysics {
+    /** @param {object} params  GeonParams fields in camelCase (missing = reference defaults) */
+    constructor(params = {}, maxParticles = 1 << 16) {
+        this._params = { ...params };
+        this._max = maxParticles;
+        this._h = null;
+        this._stats = null;
+    }
+
+    /** GPUPhysics.init(): allocate the device system. */
+    async init() {
+        this._h = addon.create(this._params, this._max);
+        return this;
+    }
+
+    get aliveCount() { return this.getParticleCount(); }
+
+    set boundaryMode(v) { this._setParam('boundaryMode', v); }
+    set topologyMode(v) { this._setParam('topology', v); }
+    set domainW(v) { this._setParam('domainW', v); }
+    set domainH(v) { this._setParam('domainH', v); }
+
+    _setParam(k, v) {
+        this._params[k] = v;
+        if (this._h) addon.setParams(this._h, this._params);
+    }
+
+    /** setToggles(physics): the *Enabled booleans of a Physics-like object (gpu-physics.js:1969). */
+    setToggles(physics) {
+        let t = 0;
+        for (const [name, bit] of Object.entries(TOGGLE_BITS)) if (physics[name]) t |= bit;
+        this._params.toggles0 = t;
+        if (physics.yukawaMu !== undefined) this._params.yukawaMu = physics.yukawaMu;
+        if (physics.higgsMass !== undefined) this._params.higgsMass = physics.higgsMass;
+        if (physics.axionMass !== undefined) this._params.axionMass = physics.axionMass;
+        if (physics.extGravity !== undefined) this._params.extGravity = physics.extGravity;
+        if (physics.extGravityAngle !== undefined) this._params.extGravityAngle = physics.extGravityAngle;
+        if (physics.extElectric !== undefined) this._params.extElectric = physics.extElectric;
+        if (physics.extElectricAngle !== undefined) this._params.extElectricAngle = physics.extElectricAngle;
+        if (physics.extBz !== undefined) this._params.extBz = physics.extBz;
+        if (this._h) addon.setParams(this._h, this._params);
+    }
+
+    /** One Physics.update call: adaptive substeps, boundary once at the end. */
+    update(dt) { return addon.update(this._h, dt); }
+
+    addParticle({ x, y, vx = 0, vy = 0, mass = 1, charge = 0, angw = 0, antimatter = false }) {
+        return addon.addParticle(this._h, { x, y, vx, vy, mass, charge, angw, antimatter });
+    }
+
+    removeParticle(idx) { addon.removeParticle(this._h, idx); }
+
+    patchMass(idx, mass, baseMass) { addon.patchMass(this._h, idx, mass, baseMass); }
+
+    uploadSoA(arrays) { addon.uploadSoA(this._h, arrays); }
+
+    downloadSoA() { return addon.downloadSoA(this._h, this._max); }
+
+    computeForces() { addon.computeForces(this._h); }
+
+    getForces(nSlots) { return addon.getForces(this._h, nSlots); }
+
+    reset() { addon.reset(this._h); }
+
+    getParticleCount() { return addon.getStats(this._h).aliveCount; }
+
+    requestStats(selIdx = -1) { this._stats = addon.getStats(this._h); }
+
+    readStats() { return this._stats; }
+
+    /** serialize(): the reference's v1 JSON particle list (save-load.js:89-127). */
+    async serialize() {
+        const d = this.downloadSoA();
+        const particles = [];
+        for (let i = 0; i < d.nSlots; i++) {
+            if (!d.alive[i]) continue;
+            particles.push({ x: d.x[i], y: d.y[i], wx: d.wx[i], wy: d.wy[i], mass: d.mass[i],
+                             baseMass: d.baseMass[i], charge: d.charge[i], angw: d.angw[i], antimatter: false });
+        }
+        return { version: 1, particles };
+    }
+
+    /** deserialize(state): requires version 1 (save-load.js:130). */
+    deserialize(state) {
+        if (!state || state.version !== 1) throw new Error('unsupported state version');
+        const ps = state.particles;
+        const col = (k, d = 0) => Float64Array.from(ps, p => (p[k] !== undefined ? p[k] : d));
+        this.uploadSoA({ x: col('x'), y: col('y'), wx: col('wx'), wy: col('wy'), mass: col('mass', 1),
+                         baseMass: col('baseMass', 1), charge: col('charge'), angw: col('angw') });
+    }
+
+    destroy() {
+        if (this._h) { addon.destroy(this._h); this._h = null; }
+    }
+}

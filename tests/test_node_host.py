@@ -1,0 +1,78 @@
+"""The JavaScript host side: the Node-API addon and the HipPhysics ES module
+drive the same C ABI as the Python tests.  Skipped where node is absent."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from physsim_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "physsim_amd", "napi", "geon_napi.node")
+HARNESS = os.path.join(ROOT, "physsim_amd", "js", "harness.mjs")
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None, reason="node not installed")
+
+
+def _ensure_addon():
+    if not os.path.exists(ADDON):
+        from physsim_amd import build
+        build.build()
+    if not os.path.exists(ADDON):
+        pytest.skip("node headers absent: addon not built")
+
+
+def test_addon_loads_and_exports():
+    _ensure_addon()
+    out = subprocess.run([NODE, "-e", "const g=require(%r); console.log(JSON.stringify(Object.keys(g)), g.abiVersion())"
+                          % ADDON], capture_output=True, text=True, check=True).stdout
+    keys, ver = out.rsplit(" ", 1)
+    assert int(ver) == 1
+    for k in ("create", "update", "addParticle", "removeParticle", "patchMass", "uploadSoA", "downloadSoA",
+              "getStats", "computeForces", "getForces", "setParams", "reset", "destroy"):
+        assert k in json.loads(keys)
+
+
+def test_create_without_gpu_throws():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    _ensure_addon()
+    r = subprocess.run([NODE, "-e", "const g=require(%r); try { g.create({}, 8); console.log('made') } "
+                                    "catch (e) { console.log('threw:', e.message) }" % ADDON],
+                       capture_output=True, text=True, check=True)
+    assert "threw: geon error -3" in r.stdout
+
+
+@pytest.mark.gpu
+def test_node_harness_matches_python_path(tmp_path):
+    """The JS host (v1 JSON in, main.js-style update ticks) gives bit-identical
+    state to the Python host over the same library."""
+    _ensure_addon()
+    from physsim_amd import HipPhysics, default_params
+    st, prm, dt = W.config1(512)
+    particles = [dict(x=float(st["x"][i]), y=float(st["y"][i]), wx=float(st["wx"][i]), wy=float(st["wy"][i]),
+                      mass=float(st["mass"][i]), baseMass=float(st["mass"][i]), charge=float(st["charge"][i]),
+                      angw=0.0) for i in range(len(st["x"]))]
+    js_params = dict(toggles0=prm["toggles0"], boundaryMode=prm["boundary_mode"], domainW=prm["domain_w"],
+                     domainH=prm["domain_h"], softening=prm["softening"], theta=prm["theta"])
+    inp = tmp_path / "in.json"
+    inp.write_text(json.dumps(dict(params=js_params, particles=particles)))
+    print("case: node harness config1 n=512", flush=True)
+    r = subprocess.run([NODE, HARNESS, str(inp), "4", repr(dt)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    g = HipPhysics(default_params(**prm), max_particles=512)
+    g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    steps = [g.update(dt) for _ in range(4)]
+    assert steps == js["substeps"]
+    d = g.download()
+    alive = np.nonzero(d["alive"])[0]
+    assert len(alive) == len(js["particles"])
+    assert np.array_equal(d["x"][alive], np.array([p["x"] for p in js["particles"]]))
+    assert np.array_equal(d["wy"][alive], np.array([p["wy"] for p in js["particles"]]))
+    assert js["stats"]["aliveCount"] == len(alive)
