@@ -236,6 +236,13 @@ int  geon_field_selfgravity(GeonHandle h, int32_t which);
  * (n x n, n a power of two <= 128; inverse scaled by 1/n per axis as fft.js:63-69). */
 int  geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse);
 
+/* Collision broadphase pair list (collisions.js:35-60, quadtree.js:372-404):
+ * all (a, b) with slot a < slot b whose bodies overlap, found by a box query of
+ * half-size 2 r_a around each particle, as the reference finds them.  Builds
+ * the tree at the current positions.  Writes min(cap, total) pairs sorted by
+ * (a, b) and the total to *n_pairs (a, b may be NULL to count only). */
+int  geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs);
+
 int  geon_counters(GeonHandle h, GeonCounters* out);
 /* Kernel timing: average device duration (ms) over the launches of `kernel`
  * since the last reset, measured with HIP events on the handle's stream.

@@ -107,6 +107,8 @@ PROTOTYPES = {
     "geon_get_field": (C.c_int, [H, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_int32]),
     "geon_field_selfgravity": (C.c_int, [H, C.c_int32]),
     "geon_fft2d": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32]),
+    "geon_collision_pairs": (C.c_int, [H, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64,
+                                        C.POINTER(C.c_int64)]),
     "geon_counters": (C.c_int, [H, C.POINTER(GeonCounters)]),
     "geon_timing_enable": (C.c_int, [H, C.c_int32]),
     "geon_kernel_timing": (C.c_int, [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

@@ -1024,6 +1024,54 @@ int geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse)
     return GEON_OK;
 }
 
+int geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs) {
+    if (!h || (cap > 0 && (!a || !b))) return GEON_EINVAL;
+    const int n = h->n_host;
+    if (n_pairs) *n_pairs = 0;
+    if (n == 0) return GEON_OK;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
+    int rc = sort_and_gather(h);
+    if (rc) return rc;
+    build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
+               I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+    h->tree_valid = true;
+    launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
+                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), nullptr, nullptr, nullptr, h->src4,
+                           h->src8a, h->src8b, n, h->ctrl, /*update_moments=*/0);
+    PairArgs pa;
+    pa.x = F(h, C_X); pa.y = F(h, C_Y); pa.mass = F(h, C_MASS); pa.radius = F(h, C_RADIUS);
+    pa.slot = I(h, C_SLOT); pa.src4 = h->src4; pa.nodes = h->tb.nodes; pa.max_nodes = 1 + 4 * h->tb.cap_divided;
+    pa.count = h->tb.owned; pa.offset = h->tb.rank_start; pa.pa = nullptr; pa.pb = nullptr;
+    pa.dom_w = (float)h->prm.domain_w; pa.dom_h = (float)h->prm.domain_h;
+    pa.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
+    pa.ctrl = h->ctrl; pa.n_upper = n;
+    launch_pairs(h->stream, pa, false);
+    int32_t* total = &h->ctrl->n_divided;     // scratch: rebuilt by the next tree build
+    exclusive_scan_i32(h->stream, h->tb.owned, h->tb.rank_start, n, &h->ctrl->n_alive, total, h->tb.scan_tmp);
+    if ((rc = read_ctrl(h))) return rc;
+    const int64_t np = h->ctrl_host->n_divided;
+    h->tree_valid = false;                    // n_divided no longer describes the tree
+    if (n_pairs) *n_pairs = np;
+    if (np == 0 || cap <= 0) return GEON_OK;
+    int32_t *da = nullptr, *db = nullptr;
+    if ((rc = dmalloc(h, &da, (size_t)np))) return rc;
+    if ((rc = dmalloc(h, &db, (size_t)np))) { hipFree(da); return rc; }
+    pa.pa = da; pa.pb = db;
+    launch_pairs(h->stream, pa, true);
+    std::vector<int64_t> keys((size_t)np);
+    std::vector<int32_t> ha((size_t)np), hb((size_t)np);
+    hipError_t e1 = hipMemcpyAsync(ha.data(), da, 4 * (size_t)np, hipMemcpyDeviceToHost, h->stream);
+    hipError_t e2 = hipMemcpyAsync(hb.data(), db, 4 * (size_t)np, hipMemcpyDeviceToHost, h->stream);
+    hipError_t e3 = hipStreamSynchronize(h->stream);
+    hipFree(da); hipFree(db);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return set_err(h, GEON_EDEVICE, "pair list copy failed");
+    for (int64_t k = 0; k < np; k++) keys[k] = ((int64_t)ha[k] << 32) | (uint32_t)hb[k];
+    std::sort(keys.begin(), keys.end());
+    const int64_t m = std::min(cap, np);
+    for (int64_t k = 0; k < m; k++) { a[k] = (int32_t)(keys[k] >> 32); b[k] = (int32_t)(keys[k] & 0xffffffff); }
+    return GEON_OK;
+}
+
 int geon_counters(GeonHandle h, GeonCounters* out) {
     if (!h || !out) return GEON_EINVAL;
     int rc = read_ctrl(h);

@@ -91,6 +91,23 @@ void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const
 // 1PN recompute for the velocity-Verlet correction: tree walk or O(N^2).
 void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree);
 
+// Collision broadphase over the tree (count pass, then write pass).
+struct PairArgs {
+    const float *x, *y, *mass, *radius;
+    const int32_t* slot;
+    const float4* src4;
+    const Node* nodes;
+    int64_t max_nodes;
+    int32_t* count;             // count pass: pairs found per particle
+    const int32_t* offset;      // write pass: exclusive scan of count
+    int32_t *pa, *pb;           // write pass: pair slots
+    float dom_w, dom_h;
+    int periodic;
+    Ctrl* ctrl;
+    int n_upper;
+};
+void launch_pairs(hipStream_t s, const PairArgs& a, bool write);
+
 // ---- scalar fields (k_field.hip) ----
 struct FieldArgs {
     int G, bc, higgs;

@@ -302,6 +302,105 @@ void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree) {
     else hipLaunchKernelGGL(k_direct1pn, dim3(grid), dim3(kWalkThreads), 0, s, a);
 }
 
+// Collision broadphase (collisions.js:35-60 with quadtree.js:372-404): for each
+// particle p1 a box query of half-size 2 r1 over the tree; a candidate p2 is a
+// pair when slot1 < slot2 and the bodies overlap, dist^2 < (r1 + r2)^2.  The
+// query rides the wavefront-shared stack of the walk, each entry carrying the
+// lanes whose box still intersects the node.  On a torus the box is also
+// queried at its wrapped images (the reference finds those neighbours through
+// ghost particles, integrator.js:201-258).  WRITE = 0 counts, 1 writes at the
+// lane's offset.
+template <int WRITE>
+__global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
+    __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
+    __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
+    const int n = a.ctrl->n_alive;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int i = blockIdx.x * kWalkThreads + threadIdx.x;
+    const bool valid = i < n && a.mass[i] != 0.f;
+    const uint64_t active = __ballot(valid);
+    if (active == 0) {
+        if (!WRITE && i < n) a.count[i] = 0;
+        return;
+    }
+    const uint64_t my_bit = 1ull << lane;
+    const float px = valid ? a.x[i] : 0.f, py = valid ? a.y[i] : 0.f;
+    const float r1 = valid ? a.radius[i] : 0.f;
+    const int slot1 = valid ? a.slot[i] : 0;
+    const float rw = 2.f * r1;
+    int found = 0;
+    int out = WRITE && valid ? a.offset[i] : 0;
+    const int n_images = a.periodic ? 4 : 1;
+    for (int img = 0; img < n_images; img++) {
+        // image shift of the query box: (0,0), (sx,0), (0,sy), (sx,sy) toward the nearer edge
+        float ox = 0.f, oy = 0.f;
+        const float sx = px < 0.5f * a.dom_w ? a.dom_w : -a.dom_w;
+        const float sy = py < 0.5f * a.dom_h ? a.dom_h : -a.dom_h;
+        if (img & 1) ox = sx;
+        if (img & 2) oy = sy;
+        const float qx = px + ox, qy = py + oy;
+        // lanes whose shifted box can touch the domain at all
+        const bool need = valid && (img == 0 || ((img & 1) == 0 || qx - rw <= a.dom_w && qx + rw >= 0.f) &&
+                                                ((img & 2) == 0 || qy - rw <= a.dom_h && qy + rw >= 0.f));
+        const uint64_t start = __ballot(need);
+        if (start == 0) continue;
+        int sp = 0;
+        if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = start; }
+        __builtin_amdgcn_wave_barrier();
+        sp = 1;
+        while (sp > 0) {
+            sp--;
+            const int node = __builtin_amdgcn_readfirstlane(s_node[wv][sp]);
+            const uint64_t mask = s_mask[wv][sp];
+            if (node < 0 || node >= a.max_nodes) continue;
+            const Node nd = a.nodes[node];
+            const float bw = nd.half, bh = nd.half * (a.dom_h / a.dom_w);
+            const bool mine = (mask & my_bit) != 0 &&
+                              !(qx - rw > nd.bx + bw || qx + rw < nd.bx - bw || qy - rw > nd.by + bh ||
+                                qy + rw < nd.by - bh);
+            if (nd.count >= 0) {
+                for (int j = nd.child; j < nd.child + nd.count; j++) {
+                    const float4 s4 = a.src4[j];
+                    if (!mine) continue;
+                    if (!(s4.x >= qx - rw && s4.x <= qx + rw && s4.y >= qy - rw && s4.y <= qy + rw)) continue;
+                    const int slot2 = a.slot[j];
+                    if (j == i || s4.z == 0.f || slot1 >= slot2) continue;
+                    float dx = s4.x - px, dy = s4.y - py;
+                    if (a.periodic) { dx = torus_wrap(dx, a.dom_w, 0.5f * a.dom_w); dy = torus_wrap(dy, a.dom_h, 0.5f * a.dom_h); }
+                    const float md = r1 + a.radius[j];
+                    if (dx * dx + dy * dy < md * md) {
+                        if (WRITE) { a.pa[out] = slot1; a.pb[out] = slot2; out++; }
+                        found++;
+                    }
+                }
+            } else {
+                const uint64_t open = __ballot(mine);
+                if (open && sp + 4 <= kWalkStack) {
+                    if (lane == 0) {
+                        s_node[wv][sp] = nd.child;     s_mask[wv][sp] = open;
+                        s_node[wv][sp + 1] = nd.child + 1; s_mask[wv][sp + 1] = open;
+                        s_node[wv][sp + 2] = nd.child + 2; s_mask[wv][sp + 2] = open;
+                        s_node[wv][sp + 3] = nd.child + 3; s_mask[wv][sp + 3] = open;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    sp += 4;
+                } else if (open) {
+                    if (lane == 0) a.ctrl->overflow = 2;
+                }
+            }
+        }
+    }
+    if (!WRITE && i < n) a.count[i] = valid ? found : 0;
+}
+
+void launch_pairs(hipStream_t s, const PairArgs& a, bool write) {
+    if (a.n_upper <= 0) return;
+    const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
+    if (write) hipLaunchKernelGGL(k_pairs<1>, dim3(grid), dim3(kWalkThreads), 0, s, a);
+    else hipLaunchKernelGGL(k_pairs<0>, dim3(grid), dim3(kWalkThreads), 0, s, a);
+}
+
 // Direct sum (forces.js:93-126): each block stages 256 sources at a time in LDS.
 template <uint32_t F, bool KAHAN>
 __global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) {

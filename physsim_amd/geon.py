@@ -220,6 +220,17 @@ class HipPhysics:
         self._check(self.lib.geon_get_tree(self.h, arr, nn.value, C.byref(nn)))
         return np.ctypeslib.as_array(arr)[: nn.value].copy() if nn.value else np.zeros(0)
 
+    def collision_pairs(self):
+        """Overlapping pairs (slot a < slot b), sorted (collisions.js:35-60)."""
+        n = C.c_int64()
+        self._check(self.lib.geon_collision_pairs(self.h, None, None, 0, C.byref(n)))
+        a = np.zeros(max(n.value, 1), dtype=np.int32)
+        b = np.zeros(max(n.value, 1), dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
+        self._check(self.lib.geon_collision_pairs(self.h, a.ctypes.data_as(ip), b.ctypes.data_as(ip), n.value,
+                                                  C.byref(n)))
+        return a[:n.value], b[:n.value]
+
     def counters(self):
         c = A.GeonCounters()
         self._check(self.lib.geon_counters(self.h, C.byref(c)))

@@ -250,3 +250,27 @@ def test_one_pn_velocity_verlet_matches_oracle(bh, oracle_lib):
     ids = do["id"]
     wg = np.hypot(dg["wx"][ids], dg["wy"][ids])
     assert np.sqrt(np.sum((dg["wx"][ids] - do["wx"]) ** 2 + (dg["wy"][ids] - do["wy"]) ** 2) / np.sum(wg ** 2)) < 1e-4
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_collision_pairs_bit_exact(periodic, oracle_lib):
+    """Overlap pair list (collisions.js:35-60) equals the oracle's, pair for pair.
+    Periodic: the oracle runs the reference's ghost images (integrator.js:201-258)."""
+    g_ = np.random.default_rng(21)
+    n = 3000
+    W_ = 64.0
+    x = W.f32(W_ * g_.random(n))
+    y = W.f32(W_ * g_.random(n))
+    m = W.f32(0.05 + 0.5 * g_.random(n))
+    z = np.zeros(n)
+    st = dict(x=x, y=y, wx=z, wy=z, mass=m, charge=z, angw=z)
+    prm = dict(toggles0=A.GRAVITY | A.BARNES_HUT, domain_w=W_, domain_h=W_)
+    if periodic:
+        prm.update(boundary_mode=A.BOUND_LOOP)
+    g, o = _pair(st, prm, oracle_lib)
+    if periodic:
+        o.set_params(oracle_lib.default_params(**dict(prm, periodic_mode=A.PERIODIC_GHOSTS)))
+    ga, gb = g.collision_pairs()
+    oa, ob = o.collision_pairs()
+    assert len(oa) > 100
+    assert np.array_equal(ga, oa) and np.array_equal(gb, ob)
