@@ -103,7 +103,18 @@ typedef struct GeonParams {
     double   ext_bz;
     int32_t  field_grid;        /* scalar-field grid side: 64 CPU, 128 GPU (config.js:84,125) */
     int32_t  kahan;             /* 1: compensated FP32 force accumulation */
+    int32_t  far_field;         /* GEON_FAR_*: correction added to accepted nodes */
+    int32_t  reserved0;
 } GeonParams;
+
+/* Far-field correction of accepted Barnes-Hut nodes (not in the reference,
+ * whose nodes carry monopoles only, quadtree.js:272-309): NONE reproduces the
+ * reference; MULTIPOLE adds the mass-quadrupole and charge dipole+quadrupole
+ * terms of gravity and Coulomb; LEARNED takes those coefficients from the
+ * network loaded with geon_mlp_set_weights. */
+#define GEON_FAR_NONE      0
+#define GEON_FAR_MULTIPOLE 1
+#define GEON_FAR_LEARNED   2
 
 /* addParticle argument (gpu-physics.js:1857): vx/vy are proper velocity w. */
 typedef struct GeonParticleInit {

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libgeon_oracle.so")
 _LIB = None
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
+FAR_STRIDE = 20
 
 
 def build():
@@ -46,6 +47,8 @@ def lib():
         L.orc_collision_pairs.restype = C.c_int64
         L.orc_direct_forces.argtypes = [C.c_void_p, _IP, C.c_int32, _DP, _DP]
         L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_far_samples.argtypes = [C.c_void_p, _IP, C.c_int32, _DP, C.c_int64]
+        L.orc_far_samples.restype = C.c_int64
         _LIB = L
     return _LIB
 
@@ -146,6 +149,15 @@ class Oracle:
         b = np.zeros(max(n, 1), dtype=np.int32)
         self.L.orc_collision_pairs(self.s, a.ctypes.data_as(_IP), b.ctypes.data_as(_IP), n)
         return a[:n], b[:n]
+
+    def far_samples(self, idx):
+        """Accepted far nodes of the observers in idx with their exact residuals
+        (one row of FAR_STRIDE values per sample; see orc_far_samples)."""
+        ii = np.ascontiguousarray(idx, dtype=np.int32)
+        m = self.L.orc_far_samples(self.s, ii.ctypes.data_as(_IP), len(ii), None, 0)
+        out = np.zeros((max(m, 1), FAR_STRIDE))
+        self.L.orc_far_samples(self.s, ii.ctypes.data_as(_IP), len(ii), _d(out), m)
+        return out[:m]
 
     def direct_forces(self, idx=None, threads=None):
         if threads:

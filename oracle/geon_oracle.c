@@ -83,6 +83,7 @@ typedef struct {
     int cap, count;
     double *bx, *by, *bw, *bh;
     double *mass, *charge, *magMom, *angMom, *momX, *momY, *comX, *comY;
+    double *mom;                   /* 9 per node: mass quadrupole, charge dipole, charge quadrupole (about the COM), sum |q| */
     int32_t *nw, *ne, *sw, *se, *pointCount, *divided, *level, *subCount, *qq;
     uint64_t *path;
     int32_t *points;               /* cap * QT_CAP indices into the tree particle array */
@@ -343,7 +344,7 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
 static void pool_free(Pool* q) {
     free(q->bx); free(q->by); free(q->bw); free(q->bh);
     free(q->mass); free(q->charge); free(q->magMom); free(q->angMom);
-    free(q->momX); free(q->momY); free(q->comX); free(q->comY);
+    free(q->momX); free(q->momY); free(q->comX); free(q->comY); free(q->mom);
     free(q->nw); free(q->ne); free(q->sw); free(q->se);
     free(q->pointCount); free(q->divided); free(q->level); free(q->subCount); free(q->qq);
     free(q->path); free(q->points);
@@ -358,6 +359,7 @@ static void pool_grow(Pool* q, int newCap) {                              /* qua
     GROW(q->magMom, double, newCap); GROW(q->angMom, double, newCap);
     GROW(q->momX, double, newCap); GROW(q->momY, double, newCap);
     GROW(q->comX, double, newCap); GROW(q->comY, double, newCap);
+    GROW(q->mom, double, (size_t)newCap * 9);
     GROW(q->nw, int32_t, newCap); GROW(q->ne, int32_t, newCap);
     GROW(q->sw, int32_t, newCap); GROW(q->se, int32_t, newCap);
     GROW(q->pointCount, int32_t, newCap); GROW(q->divided, int32_t, newCap);
@@ -513,6 +515,86 @@ static void pool_massDistribution(Pool* q, const P* tp, int root) {
     free(order); free(stack);
 }
 
+/* Second moments about each node's COM (not in the reference: the input of the
+ * far-field correction).  Leaves sum their points; a divided node combines its
+ * children by the parallel-axis shifts:
+ *   Qm += Qm_c + m_c D D^T,   Dq += Dq_c + q_c D,
+ *   Qq += Qq_c + Dq_c D^T + D Dq_c^T + q_c D D^T   (D = com_c - com). */
+static void pool_moments(Pool* q, const P* tp, int root) {
+    int* order = (int*)malloc(sizeof(int) * (size_t)q->count);
+    int* stack = (int*)malloc(sizeof(int) * (size_t)(q->count + 4));
+    int top = 0, len = 0;
+    stack[top++] = root;
+    while (top > 0) {
+        int idx = stack[--top];
+        order[len++] = idx;
+        if (q->divided[idx]) {
+            stack[top++] = q->nw[idx]; stack[top++] = q->ne[idx];
+            stack[top++] = q->sw[idx]; stack[top++] = q->se[idx];
+        }
+    }
+    for (int k = len - 1; k >= 0; k--) {
+        const int idx = order[k];
+        double* M = &q->mom[(size_t)idx * 9];
+        for (int c = 0; c < 9; c++) M[c] = 0;
+        const double cx = q->comX[idx], cy = q->comY[idx];
+        if (!q->divided[idx]) {
+            const size_t base = (size_t)idx * QT_CAP;
+            for (int i = 0; i < q->pointCount[idx]; i++) {
+                const P* p = &tp[q->points[base + i]];
+                const double dx = p->x - cx, dy = p->y - cy;
+                M[0] += p->mass * dx * dx; M[1] += p->mass * dx * dy; M[2] += p->mass * dy * dy;
+                M[3] += p->charge * dx; M[4] += p->charge * dy;
+                M[5] += p->charge * dx * dx; M[6] += p->charge * dx * dy; M[7] += p->charge * dy * dy;
+                M[8] += fabs(p->charge);
+            }
+        } else {
+            const int ch[4] = {q->nw[idx], q->ne[idx], q->sw[idx], q->se[idx]};
+            for (int c = 0; c < 4; c++) {
+                const int j = ch[c];
+                const double* C = &q->mom[(size_t)j * 9];
+                const double m = q->mass[j], qc = q->charge[j];
+                const double dx = q->comX[j] - cx, dy = q->comY[j] - cy;
+                if (m <= 0 && C[8] == 0) continue;
+                M[0] += C[0] + m * dx * dx; M[1] += C[1] + m * dx * dy; M[2] += C[2] + m * dy * dy;
+                M[3] += C[3] + qc * dx; M[4] += C[4] + qc * dy;
+                M[5] += C[5] + 2 * C[3] * dx + qc * dx * dx;
+                M[6] += C[6] + C[3] * dy + C[4] * dx + qc * dx * dy;
+                M[7] += C[7] + 2 * C[4] * dy + qc * dy * dy;
+                M[8] += C[8];
+            }
+        }
+    }
+    free(order); free(stack);
+}
+
+/* Far-field correction of an accepted node (not in the reference): the
+ * gravity and Coulomb terms beyond the monopole of the softened kernel
+ * g(d) = d (|d|^2 + eps^2)^(-3/2), with coefficients c[0..7] = mass quadrupole,
+ * charge dipole, charge quadrupole (about the COM):
+ *   1/2 Q:dd g = 2 (Q d) h' + tr(Q) d h' + 2 d (d.Q.d) h'',  D.dg = D h + 2 d (D.d) h',
+ * h = r2^-3/2, h' = -3/2 r2^-5/2, h'' = 15/4 r2^-7/2, r2 = |d|^2 + eps^2. */
+static void far_correction(const OrcSys* s, P* p, double dx, double dy, const double* c) {
+    const double r2 = dx * dx + dy * dy + s->softeningSq;
+    const double ir2 = 1 / r2, ir = sqrt(ir2);
+    const double h = ir * ir2, h1 = -1.5 * h * ir2, h2 = 3.75 * h * ir2 * ir2;
+    double fx = 0, fy = 0;
+    if (s->grav) {
+        const double qx = c[0] * dx + c[1] * dy, qy = c[1] * dx + c[2] * dy;
+        const double dqd = dx * qx + dy * qy, tr = c[0] + c[2];
+        fx += p->mass * (2 * qx * h1 + tr * dx * h1 + 2 * dx * dqd * h2);
+        fy += p->mass * (2 * qy * h1 + tr * dy * h1 + 2 * dy * dqd * h2);
+    }
+    if (s->coul) {
+        const double dd = c[3] * dx + c[4] * dy;
+        const double qx = c[5] * dx + c[6] * dy, qy = c[6] * dx + c[7] * dy;
+        const double dqd = dx * qx + dy * qy, tr = c[5] + c[7];
+        fx -= p->charge * (c[3] * h + 2 * dx * dd * h1 + 2 * qx * h1 + tr * dx * h1 + 2 * dx * dqd * h2);
+        fy -= p->charge * (c[4] * h + 2 * dy * dd * h1 + 2 * qy * h1 + tr * dy * h1 + 2 * dy * dqd * h2);
+    }
+    p->fx += fx; p->fy += fy;
+}
+
 /* ------------------------------------------------------------------ */
 /* integrator.js: ghosts and tree build                                */
 /* ------------------------------------------------------------------ */
@@ -592,6 +674,7 @@ static int buildTree(OrcSys* s) {
     for (int i = 0; i < s->ntp; i++) pool_insert(q, root, s->tp, i, &stkN, &stkP, &stkCap);
     free(stkN); free(stkP);
     pool_massDistribution(q, s->tp, root);
+    pool_moments(q, s->tp, root);
     s->root = root;
     return root;
 }
@@ -645,12 +728,88 @@ static void calculateForce(OrcSys* s, int pi, int32_t* stack, int64_t* cnt) {
             pairForce(s, p, q->comX[nodeIdx], q->comY[nodeIdx], q->momX[nodeIdx] / nodeMass,
                       q->momY[nodeIdx] / nodeMass, nodeMass, q->charge[nodeIdx], 0,
                       q->magMom[nodeIdx], q->angMom[nodeIdx], 1, 1, 1, 1);
+            if (s->prm.far_field == GEON_FAR_MULTIPOLE) far_correction(s, p, dx, dy, &q->mom[(size_t)nodeIdx * 9]);
             cnt[1]++;
         } else if (q->divided[nodeIdx]) {
             stack[top++] = q->nw[nodeIdx]; stack[top++] = q->ne[nodeIdx];
             stack[top++] = q->sw[nodeIdx]; stack[top++] = q->se[nodeIdx];
         }
     }
+}
+
+static void syncToggles(OrcSys* s);
+
+/* Sum over the points under node ni of the softened gravity (per unit observer
+ * mass) and Coulomb (per unit observer charge) kernels, each point placed at
+ * offset (dx, dy) + (x_j - com) from the observer: the exact counterpart of
+ * the node's monopole. */
+static void node_exact(const OrcSys* s, const Pool* q, int ni, double dx, double dy, double cx, double cy,
+                       int32_t* stack, double* g) {
+    int top = 0;
+    stack[top++] = ni;
+    g[0] = g[1] = g[2] = g[3] = 0;
+    while (top > 0) {
+        const int k = stack[--top];
+        if (q->divided[k]) {
+            stack[top++] = q->nw[k]; stack[top++] = q->ne[k];
+            stack[top++] = q->sw[k]; stack[top++] = q->se[k];
+            continue;
+        }
+        const size_t base = (size_t)k * QT_CAP;
+        for (int i = 0; i < q->pointCount[k]; i++) {
+            const P* o = &s->tp[q->points[base + i]];
+            const double ex = dx + (o->x - cx), ey = dy + (o->y - cy);
+            const double r2 = ex * ex + ey * ey + s->softeningSq;
+            const double h = 1 / (r2 * sqrt(r2));
+            g[0] += o->mass * ex * h; g[1] += o->mass * ey * h;
+            g[2] -= o->charge * ex * h; g[3] -= o->charge * ey * h;
+        }
+    }
+}
+
+int64_t orc_far_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double* out, int64_t cap) {
+    syncToggles(s);
+    buildTree(s);
+    const Pool* q = &s->pool;
+    const double thetaSq = s->prm.theta * s->prm.theta;
+    const double W = s->prm.domain_w, H = s->prm.domain_h;
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(q->count * 2 + 16));
+    int32_t* stack2 = (int32_t*)malloc(sizeof(int32_t) * (size_t)(q->count * 2 + 16));
+    int64_t m = 0;
+    for (int t = 0; t < n_idx; t++) {
+        const int pi = idx[t];
+        const double px = s->p[pi].x, py = s->p[pi].y;
+        int top = 0;
+        stack[top++] = s->root;
+        while (top > 0) {                   /* the acceptance of calculateForce */
+            const int ni = stack[--top];
+            if (q->mass[ni] < EPS_ || !q->divided[ni]) continue;
+            double dx, dy;
+            if (s->periodic) minImage(px, py, q->comX[ni], q->comY[ni], s->prm.topology, W, H, W * 0.5, H * 0.5, &dx, &dy);
+            else { dx = q->comX[ni] - px; dy = q->comY[ni] - py; }
+            const double size = q->bw[ni] * 2;
+            if (size * size < thetaSq * (dx * dx + dy * dy) && window_ok(s, q, ni, px, py, dx, dy)) {
+                if (out && m < cap) {
+                    double* r = out + (size_t)m * ORC_FAR_STRIDE;
+                    double g[4];
+                    node_exact(s, q, ni, dx, dy, q->comX[ni], q->comY[ni], stack2, g);
+                    const double r2 = dx * dx + dy * dy + s->softeningSq;
+                    const double h = 1 / (r2 * sqrt(r2));
+                    r[0] = pi; r[1] = dx; r[2] = dy; r[3] = r2; r[4] = size;
+                    r[5] = q->mass[ni]; r[6] = q->charge[ni];
+                    for (int c = 0; c < 9; c++) r[7 + c] = q->mom[(size_t)ni * 9 + c];
+                    r[16] = g[0] - q->mass[ni] * dx * h; r[17] = g[1] - q->mass[ni] * dy * h;
+                    r[18] = g[2] + q->charge[ni] * dx * h; r[19] = g[3] + q->charge[ni] * dy * h;
+                }
+                m++;
+            } else {
+                stack[top++] = q->nw[ni]; stack[top++] = q->ne[ni];
+                stack[top++] = q->sw[ni]; stack[top++] = q->se[ni];
+            }
+        }
+    }
+    free(stack); free(stack2);
+    return m;
 }
 
 static void resetForces(OrcSys* s) {                                      /* forces.js:31-54 */

@@ -57,6 +57,15 @@ int     orc_direct_forces(OrcSys* s, const int32_t* idx, int32_t n_idx,
                           double* fx, double* fy);
 void    orc_set_threads(int32_t n);
 
+/* Far-node samples of the learned correction (not in the reference): builds
+ * the tree, then for each observer in idx lists every divided node its walk
+ * accepts, as ORC_FAR_STRIDE doubles: observer, dx, dy, r^2 + eps^2, node size,
+ * mass, charge, the 9 moments about the COM, and the exact monopole-truncation
+ * residual (gravity x, y per unit observer mass; Coulomb x, y per unit
+ * observer charge).  Returns the number of samples; writes at most cap. */
+#define ORC_FAR_STRIDE 20
+int64_t orc_far_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double* out, int64_t cap);
+
 /* Scalar fields (which: 0 Higgs, 1 axion). */
 int     orc_field_set(OrcSys* s, int32_t which, const double* field, const double* field_dot);
 int     orc_field_get(OrcSys* s, int32_t which, int32_t what, double* out);

@@ -14,6 +14,7 @@ COL_PASS, COL_MERGE, COL_BOUNCE = 0, 1, 2
 BOUND_DESPAWN, BOUND_BOUNCE, BOUND_LOOP = 0, 1, 2
 TORUS, KLEIN, RP2 = 0, 1, 2
 PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS, PERIODIC_WINDOWED = 0, 1, 2
+FAR_NONE, FAR_MULTIPOLE, FAR_LEARNED = 0, 1, 2
 FIELD_HIGGS, FIELD_AXION = 0, 1
 FQ = {"field": 0, "field_dot": 1, "grad_x": 2, "grad_y": 3, "sg_phi": 4, "sg_gx": 5, "sg_gy": 6,
       "source": 7, "thermal": 8}
@@ -37,7 +38,7 @@ class GeonParams(C.Structure):
         ("yukawa_mu", C.c_double), ("higgs_mass", C.c_double), ("axion_mass", C.c_double),
         ("hubble", C.c_double), ("ext_gravity", C.c_double), ("ext_gravity_angle", C.c_double),
         ("ext_electric", C.c_double), ("ext_electric_angle", C.c_double), ("ext_bz", C.c_double),
-        ("field_grid", C.c_int32), ("kahan", C.c_int32),
+        ("field_grid", C.c_int32), ("kahan", C.c_int32), ("far_field", C.c_int32), ("reserved0", C.c_int32),
     ]
 
 

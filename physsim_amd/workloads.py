@@ -124,3 +124,24 @@ def make_params(lib_default, **kw):
     for k, v in kw.items():
         setattr(lib_default, k, v)
     return lib_default
+
+
+def plummer_charged(n=65536, replicate=0, a=20.0, side=512.0):
+    """Training systems of the learned far-field correction: a projected
+    Plummer sphere of scale radius a (truncated at 0.45 side), lognormal(0, 0.5)
+    masses, charges 0.1*k for k uniform in [-10, 10], at rest; gravity and
+    Coulomb with the Barnes-Hut softening of the N=65536 box workload."""
+    g = rng(5, replicate)
+    rmax = 0.45 * side
+    u = g.random(n) * (rmax ** 2 / (rmax ** 2 + a ** 2))
+    R = a * np.sqrt(u / (1 - u))
+    phi = 2 * np.pi * g.random(n)
+    x = side / 2 + R * np.cos(phi)
+    y = side / 2 + R * np.sin(phi)
+    m = g.lognormal(0.0, 0.5, n)
+    q = g.integers(-10, 11, n) * 0.1
+    z = np.zeros(n)
+    st = dict(x=f32(x), y=f32(y), wx=z, wy=z.copy(), mass=f32(m), charge=f32(q), angw=z.copy())
+    prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, boundary_mode=A.BOUND_DESPAWN,
+               domain_w=float(side), domain_h=float(side), softening=8.0, theta=0.3)
+    return st, prm

@@ -105,6 +105,40 @@ def test_bh_root_only_monopole(oracle_lib):
     assert f["fy"][0] == pytest.approx(M * dy / r3, rel=1e-12)
 
 
+def test_far_multipole_is_third_order(oracle_lib):
+    # A charged cluster of extent a accepted as one node at distance ~80.  The
+    # monopole about the mass COM errs as a^2 for gravity and as a (the charge
+    # dipole) for Coulomb; with the quadrupole/dipole correction both fall as
+    # a^3 (a dropped term, a wrong sign or a swapped index leaves a lower order).
+    n = 40
+    g = np.random.default_rng(5)
+    ux, uy = g.random(n - 1) - 0.5, g.random(n - 1) - 0.5
+    m = np.concatenate([[1.3], 1 + g.random(n - 1)])
+    q = np.concatenate([[0.7], np.round(10 * g.random(n - 1) - 3) / 10])   # charges are multiples of 0.1
+    z = np.zeros(n)
+    for tog, sign, src, plain_order in ((A.GRAVITY, 1.0, m, 2), (A.COULOMB, -1.0, q, 1)):
+        errs = {0: [], 1: []}
+        for a in (2.0, 1.0, 0.5):
+            xs = np.concatenate([[2.0], 60 + a * ux])
+            ys = np.concatenate([[2.0], 55 + a * uy])
+            dx, dy = xs[1:] - 2.0, ys[1:] - 2.0
+            r3 = (dx * dx + dy * dy + 0.01) ** 1.5
+            fx = sign * src[0] * np.sum(src[1:] * dx / r3)
+            fy = sign * src[0] * np.sum(src[1:] * dy / r3)
+            for far in (0, 1):
+                prm = dict(toggles0=tog | A.BARNES_HUT, softening=0.1, theta=0.5, far_field=far)
+                o = oracle_lib.Oracle(oracle_lib.default_params(**prm), n)
+                o.upload(xs, ys, z, z, m, q)
+                o.compute_forces()
+                f = o.forces()
+                assert o.counters()["node"] >= 1
+                errs[far].append(np.hypot(f["fx"][0] - fx, f["fy"][0] - fy) / np.hypot(fx, fy))
+        for k in range(2):
+            assert errs[0][k] / errs[0][k + 1] == pytest.approx(2.0 ** plain_order, rel=0.1)
+            assert errs[1][k] / errs[1][k + 1] == pytest.approx(8.0, rel=0.15)
+        assert errs[1][0] < 0.02 * errs[0][0]
+
+
 def test_bh_error_decreases_with_theta(oracle_lib):
     st, prm, _ = W.config1(1500)
     o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1500)
