@@ -104,14 +104,15 @@ typedef struct GeonParams {
     int32_t  field_grid;        /* scalar-field grid side: 64 CPU, 128 GPU (config.js:84,125) */
     int32_t  kahan;             /* 1: compensated FP32 force accumulation */
     int32_t  far_field;         /* GEON_FAR_*: correction added to accepted nodes */
-    int32_t  reserved0;
+    int32_t  far_list_cap;      /* LEARNED: listed (particle, node) pairs per 64 particles; 0 = 65536 */
 } GeonParams;
 
 /* Far-field correction of accepted Barnes-Hut nodes (not in the reference,
  * whose nodes carry monopoles only, quadtree.js:272-309): NONE reproduces the
  * reference; MULTIPOLE adds the mass-quadrupole and charge dipole+quadrupole
- * terms of gravity and Coulomb; LEARNED takes those coefficients from the
- * network loaded with geon_mlp_set_weights. */
+ * terms of gravity and Coulomb; LEARNED adds the gravity and Coulomb residual
+ * predicted for every (particle, accepted node) pair by the 2 x 64 GELU
+ * network set with geon_far_set_model or trained with geon_far_train. */
 #define GEON_FAR_NONE      0
 #define GEON_FAR_MULTIPOLE 1
 #define GEON_FAR_LEARNED   2
@@ -255,6 +256,36 @@ int  geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse
 int  geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs);
 
 int  geon_counters(GeonHandle h, GeonCounters* out);
+
+/* ---- Learned far-field correction (no counterpart in the reference) -------
+ * Model: GEON_FAR_MODEL_FLOATS floats = GEON_FAR_PARAMS FP32 parameters
+ * (W1[16][64] whose row 10 is the first layer's bias, W2[64][64], b2[64],
+ * W3[64][4], b3[4]) followed by the feature means[10], inverse standard
+ * deviations[10] and output scales[4].  Inputs and outputs of one pair are
+ * defined in physsim_amd/csrc/k_learn.hip. */
+#define GEON_FAR_PARAMS       5444
+#define GEON_FAR_MODEL_FLOATS 5468
+#define GEON_FAR_SAMPLE       16    /* floats per sample: 10 features, 4 targets, 2 weights */
+/* Training samples from the current state: builds the tree, lists the pairs
+ * the walk accepts, and for every group_stride-th group of 64 particles (in
+ * Morton order) appends one sample per listed pair with its exact FP64
+ * monopole-truncation residual.  *n_samples = samples held afterwards. */
+int  geon_far_collect(GeonHandle h, int32_t group_stride, int64_t* n_samples);
+int  geon_far_clear(GeonHandle h);
+/* Samples [first, first + n): out (n * GEON_FAR_SAMPLE floats) and/or the
+ * observer slot of each (either may be NULL). */
+int  geon_far_get_samples(GeonHandle h, int64_t first, int64_t n, float* out, int32_t* observer);
+/* Adam on random batches of the held samples, learning rate cosine-decayed
+ * from lr to lr_final; loss_out (may be NULL) receives `steps` losses.  The
+ * first call with no model initialises the weights from seed and the input /
+ * output statistics from the samples. */
+int  geon_far_train(GeonHandle h, int32_t steps, int32_t batch, double lr, double lr_final, uint64_t seed,
+                    float* loss_out);
+/* Loss gradient (GEON_FAR_PARAMS floats) on the given samples at the current
+ * model, with the same BF16 kernels as training. */
+int  geon_far_grad(GeonHandle h, const int64_t* rows, int32_t n_rows, float* grad_out, float* loss_out);
+int  geon_far_get_model(GeonHandle h, float* out, int32_t n);
+int  geon_far_set_model(GeonHandle h, const float* in, int32_t n);
 /* Kernel timing: average device duration (ms) over the launches of `kernel`
  * since the last reset, measured with HIP events on the handle's stream.
  * kernel: 0 = force walk / direct sum, 1 = tree build, 2 = sort, 3 = integrator. */

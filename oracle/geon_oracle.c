@@ -106,6 +106,7 @@ struct OrcSys {
     double softeningSq, softening, yukCutoffSq;
     int needAxMod, periodic;
     OField higgsF, axionF;
+    double* farModel;              /* learned far-field model (FAR_MODEL doubles) or NULL */
 };
 
 static int g_threads = 1;
@@ -596,6 +597,141 @@ static void far_correction(const OrcSys* s, P* p, double dx, double dy, const do
 }
 
 /* ------------------------------------------------------------------ */
+/* Learned far-field correction: FP64 restatement of the network of      */
+/* physsim_amd/csrc/k_learn.hip (not in the reference).                  */
+/* ------------------------------------------------------------------ */
+#define FAR_IN 10
+#define FAR_HID 64
+#define FAR_W1 0
+#define FAR_W2 1024
+#define FAR_B2 5120
+#define FAR_W3 5184
+#define FAR_B3 5440
+#define FAR_PARAMS 5444
+#define FAR_MODEL 5468
+#define FAR_SAMPLE 16
+
+static double gelu64(double z) {
+    const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
+    return z / (1.0 + exp(-2.0 * u));
+}
+
+static double gelu64_grad(double z) {
+    const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
+    const double sg = 1.0 / (1.0 + exp(-2.0 * u));
+    return sg + z * sg * (1.0 - sg) * 2.0 * 0.7978845608028654 * (1.0 + 3.0 * 0.044715 * z * z);
+}
+
+/* Features of an (observer, node) pair in the offset frame u = d/|d|,
+ * v = (-u_y, u_x); c = the oracle's 9 node moments (Qm xx xy yy, Dq x y,
+ * Qq xx xy yy, sum |q|).  sg0/sc0: the output scales without m_i, q_i. */
+static void far_features(double dx, double dy, double eps2, double s, double M, const double* c, double* f,
+                         double* ux, double* uy, double* sg0, double* sc0) {
+    const double rho2 = dx * dx + dy * dy, r2 = rho2 + eps2, r = sqrt(r2), irho = 1 / sqrt(rho2);
+    const double x = dx * irho, y = dy * irho;
+    const double sr = s / r;
+    const double iM = M > 0 ? 1 / (M * s * s) : 0, A = c[8], iA = A > 0 ? 1 / (A * s) : 0;
+    const double uu = x * x, vv = y * y, uv = x * y;
+    f[0] = sr;
+    f[1] = eps2 / r2;
+    f[2] = (uu * c[0] + 2 * uv * c[1] + vv * c[2]) * iM;
+    f[3] = (-uv * c[0] + (uu - vv) * c[1] + uv * c[2]) * iM;
+    f[4] = (vv * c[0] - 2 * uv * c[1] + uu * c[2]) * iM;
+    f[5] = (x * c[3] + y * c[4]) * iA;
+    f[6] = (-y * c[3] + x * c[4]) * iA;
+    f[7] = (uu * c[5] + 2 * uv * c[6] + vv * c[7]) * iA / s;
+    f[8] = (-uv * c[5] + (uu - vv) * c[6] + uv * c[7]) * iA / s;
+    f[9] = (vv * c[5] - 2 * uv * c[6] + uu * c[7]) * iA / s;
+    *ux = x; *uy = y;
+    *sg0 = M * sr * sr / r2;
+    *sc0 = A * sr / r2;
+}
+
+/* Forward pass; keeps the pre-activations when z1/z2 are given. */
+static void far_forward(const double* P, const double* feat, double* x, double* z1, double* a1, double* z2,
+                        double* a2, double* o) {
+    const double* mean = P + FAR_PARAMS;
+    const double* istd = mean + FAR_IN;
+    for (int k = 0; k < 16; k++) x[k] = 0;
+    for (int k = 0; k < FAR_IN; k++) x[k] = (feat[k] - mean[k]) * istd[k];
+    x[FAR_IN] = 1;
+    for (int m = 0; m < FAR_HID; m++) {
+        double z = 0;
+        for (int k = 0; k < 16; k++) z += x[k] * P[FAR_W1 + k * FAR_HID + m];
+        z1[m] = z; a1[m] = gelu64(z);
+    }
+    for (int b = 0; b < FAR_HID; b++) {
+        double z = P[FAR_B2 + b];
+        for (int a = 0; a < FAR_HID; a++) z += a1[a] * P[FAR_W2 + a * FAR_HID + b];
+        z2[b] = z; a2[b] = gelu64(z);
+    }
+    for (int j = 0; j < 4; j++) {
+        double z = P[FAR_B3 + j];
+        for (int b = 0; b < FAR_HID; b++) z += a2[b] * P[FAR_W3 + b * 4 + j];
+        o[j] = z;
+    }
+}
+
+/* The learned residual of an accepted node, added to the observer's force. */
+static void far_learned(const OrcSys* s, P* p, double dx, double dy, int ni) {
+    const Pool* q = &s->pool;
+    const double* P = s->farModel;
+    double f[FAR_IN], ux, uy, sg0, sc0, x[16], z1[FAR_HID], a1[FAR_HID], z2[FAR_HID], a2[FAR_HID], o[4];
+    far_features(dx, dy, s->softeningSq, q->bw[ni] * 2, q->mass[ni], &q->mom[(size_t)ni * 9], f, &ux, &uy, &sg0, &sc0);
+    far_forward(P, f, x, z1, a1, z2, a2, o);
+    const double* os = P + FAR_PARAMS + 2 * FAR_IN;
+    const double gu = o[0] * os[0], gv = o[1] * os[1], cu = o[2] * os[2], cv = o[3] * os[3];
+    const double sg = s->grav ? p->mass * sg0 : 0, sc = s->coul ? p->charge * sc0 : 0;
+    p->fx += sg * (ux * gu - uy * gv) + sc * (ux * cu - uy * cv);
+    p->fy += sg * (uy * gu + ux * gv) + sc * (uy * cu + ux * cv);
+}
+
+int orc_far_set_model(OrcSys* s, const float* model) {
+    if (!s->farModel) s->farModel = (double*)malloc(sizeof(double) * FAR_MODEL);
+    for (int i = 0; i < FAR_MODEL; i++) s->farModel[i] = model[i];
+    return 0;
+}
+
+/* Weighted squared-error loss of the network on n samples and its gradient
+ * (FAR_PARAMS doubles), in FP64:
+ *   L = wscale / n * sum_r sum_j w_rj (oscale_j o_rj - t_rj)^2. */
+double orc_far_loss_grad(const float* model, const float* samples, int64_t n, double wscale, double* grad) {
+    double P[FAR_MODEL];
+    for (int i = 0; i < FAR_MODEL; i++) P[i] = model[i];
+    const double* os = P + FAR_PARAMS + 2 * FAR_IN;
+    for (int i = 0; i < FAR_PARAMS; i++) grad[i] = 0;
+    double L = 0;
+    for (int64_t r = 0; r < n; r++) {
+        const float* sm = samples + r * FAR_SAMPLE;
+        double feat[FAR_IN], x[16], z1[FAR_HID], a1[FAR_HID], z2[FAR_HID], a2[FAR_HID], o[4], dO[4];
+        for (int k = 0; k < FAR_IN; k++) feat[k] = sm[k];
+        far_forward(P, feat, x, z1, a1, z2, a2, o);
+        for (int j = 0; j < 4; j++) {
+            const double w = wscale * sm[FAR_IN + 4 + (j >> 1)];
+            const double err = os[j] * o[j] - sm[FAR_IN + j];
+            L += w * err * err / n;
+            dO[j] = 2 * w * err * os[j] / n;
+        }
+        double dz2[FAR_HID], dz1[FAR_HID];
+        for (int b = 0; b < FAR_HID; b++) {
+            double da = 0;
+            for (int j = 0; j < 4; j++) { da += P[FAR_W3 + b * 4 + j] * dO[j]; grad[FAR_W3 + b * 4 + j] += a2[b] * dO[j]; }
+            dz2[b] = da * gelu64_grad(z2[b]);
+            grad[FAR_B2 + b] += dz2[b];
+        }
+        for (int j = 0; j < 4; j++) grad[FAR_B3 + j] += dO[j];
+        for (int a = 0; a < FAR_HID; a++) {
+            double da = 0;
+            for (int b = 0; b < FAR_HID; b++) { da += P[FAR_W2 + a * FAR_HID + b] * dz2[b]; grad[FAR_W2 + a * FAR_HID + b] += a1[a] * dz2[b]; }
+            dz1[a] = da * gelu64_grad(z1[a]);
+        }
+        for (int k = 0; k < 16; k++)
+            for (int m = 0; m < FAR_HID; m++) grad[FAR_W1 + k * FAR_HID + m] += x[k] * dz1[m];
+    }
+    return L;
+}
+
+/* ------------------------------------------------------------------ */
 /* integrator.js: ghosts and tree build                                */
 /* ------------------------------------------------------------------ */
 static void tp_reserve(OrcSys* s, int n) {
@@ -729,6 +865,7 @@ static void calculateForce(OrcSys* s, int pi, int32_t* stack, int64_t* cnt) {
                       q->momY[nodeIdx] / nodeMass, nodeMass, q->charge[nodeIdx], 0,
                       q->magMom[nodeIdx], q->angMom[nodeIdx], 1, 1, 1, 1);
             if (s->prm.far_field == GEON_FAR_MULTIPOLE) far_correction(s, p, dx, dy, &q->mom[(size_t)nodeIdx * 9]);
+            if (s->prm.far_field == GEON_FAR_LEARNED && s->farModel) far_learned(s, p, dx, dy, nodeIdx);
             cnt[1]++;
         } else if (q->divided[nodeIdx]) {
             stack[top++] = q->nw[nodeIdx]; stack[top++] = q->ne[nodeIdx];
@@ -809,6 +946,34 @@ int64_t orc_far_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double* ou
         }
     }
     free(stack); free(stack2);
+    return m;
+}
+
+/* Training samples in the GEON_FAR_SAMPLE layout (features, targets in the
+ * offset frame normalised by the output scales, loss weights divided by f2ref)
+ * for the observers in idx; obs[k] = the observer of sample k. */
+int64_t orc_far_training_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double f2ref, float* out,
+                                 int32_t* obs, int64_t cap) {
+    const int64_t m = orc_far_samples(s, idx, n_idx, NULL, 0);
+    double* raw = (double*)malloc(sizeof(double) * ORC_FAR_STRIDE * (size_t)(m > 0 ? m : 1));
+    orc_far_samples(s, idx, n_idx, raw, m);
+    for (int64_t k = 0; k < m && k < cap; k++) {
+        const double* r = raw + (size_t)k * ORC_FAR_STRIDE;
+        const P* p = &s->p[(int)r[0]];
+        double f[FAR_IN], ux, uy, sg0, sc0;
+        far_features(r[1], r[2], s->softeningSq, r[4], r[5], r + 7, f, &ux, &uy, &sg0, &sc0);
+        float* o = out + (size_t)k * FAR_SAMPLE;
+        for (int j = 0; j < FAR_IN; j++) o[j] = (float)f[j];
+        const double ig = sg0 > 0 ? 1 / sg0 : 0, ic = sc0 > 0 ? 1 / sc0 : 0;
+        o[FAR_IN + 0] = (float)((ux * r[16] + uy * r[17]) * ig);
+        o[FAR_IN + 1] = (float)((-uy * r[16] + ux * r[17]) * ig);
+        o[FAR_IN + 2] = (float)((ux * r[18] + uy * r[19]) * ic);
+        o[FAR_IN + 3] = (float)((-uy * r[18] + ux * r[19]) * ic);
+        o[FAR_IN + 4] = s->grav && sg0 > 0 ? (float)((p->mass * sg0) * (p->mass * sg0) / f2ref) : 0.f;
+        o[FAR_IN + 5] = s->coul && sc0 > 0 ? (float)((p->charge * sc0) * (p->charge * sc0) / f2ref) : 0.f;
+        if (obs) obs[k] = (int32_t)r[0];
+    }
+    free(raw);
     return m;
 }
 
@@ -1017,6 +1182,7 @@ void orc_destroy(OrcSys* s) {
     of_free(&s->higgsF);
     of_free(&s->axionF);
     pool_free(&s->pool);
+    free(s->farModel);
     free(s->p); free(s->tp); free(s);
 }
 

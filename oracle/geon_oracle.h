@@ -65,6 +65,14 @@ void    orc_set_threads(int32_t n);
  * observer charge).  Returns the number of samples; writes at most cap. */
 #define ORC_FAR_STRIDE 20
 int64_t orc_far_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double* out, int64_t cap);
+/* FP64 restatement of the learned far-field network (model layout as
+ * GEON_FAR_MODEL_FLOATS in include/geon.h): set the model used when
+ * far_field == GEON_FAR_LEARNED, and the training loss and its gradient on
+ * samples in the GEON_FAR_SAMPLE layout. */
+int     orc_far_set_model(OrcSys* s, const float* model);
+int64_t orc_far_training_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double f2ref, float* out,
+                                 int32_t* obs, int64_t cap);
+double  orc_far_loss_grad(const float* model, const float* samples, int64_t n, double wscale, double* grad);
 
 /* Scalar fields (which: 0 Higgs, 1 axion). */
 int     orc_field_set(OrcSys* s, int32_t which, const double* field, const double* field_dot);

@@ -15,6 +15,7 @@ BOUND_DESPAWN, BOUND_BOUNCE, BOUND_LOOP = 0, 1, 2
 TORUS, KLEIN, RP2 = 0, 1, 2
 PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS, PERIODIC_WINDOWED = 0, 1, 2
 FAR_NONE, FAR_MULTIPOLE, FAR_LEARNED = 0, 1, 2
+FAR_PARAMS, FAR_MODEL_FLOATS, FAR_SAMPLE = 5444, 5468, 16
 FIELD_HIGGS, FIELD_AXION = 0, 1
 FQ = {"field": 0, "field_dot": 1, "grad_x": 2, "grad_y": 3, "sg_phi": 4, "sg_gx": 5, "sg_gy": 6,
       "source": 7, "thermal": 8}
@@ -38,7 +39,7 @@ class GeonParams(C.Structure):
         ("yukawa_mu", C.c_double), ("higgs_mass", C.c_double), ("axion_mass", C.c_double),
         ("hubble", C.c_double), ("ext_gravity", C.c_double), ("ext_gravity_angle", C.c_double),
         ("ext_electric", C.c_double), ("ext_electric_angle", C.c_double), ("ext_bz", C.c_double),
-        ("field_grid", C.c_int32), ("kahan", C.c_int32), ("far_field", C.c_int32), ("reserved0", C.c_int32),
+        ("field_grid", C.c_int32), ("kahan", C.c_int32), ("far_field", C.c_int32), ("far_list_cap", C.c_int32),
     ]
 
 
@@ -114,6 +115,14 @@ PROTOTYPES = {
     "geon_timing_enable": (C.c_int, [H, C.c_int32]),
     "geon_kernel_timing": (C.c_int, [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "geon_last_error": (C.c_char_p, [H]),
+    "geon_far_collect": (C.c_int, [H, C.c_int32, C.POINTER(C.c_int64)]),
+    "geon_far_clear": (C.c_int, [H]),
+    "geon_far_get_samples": (C.c_int, [H, C.c_int64, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+    "geon_far_train": (C.c_int, [H, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_uint64,
+                                  C.POINTER(C.c_float)]),
+    "geon_far_grad": (C.c_int, [H, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "geon_far_get_model": (C.c_int, [H, C.POINTER(C.c_float), C.c_int32]),
+    "geon_far_set_model": (C.c_int, [H, C.POINTER(C.c_float), C.c_int32]),
 }
 
 

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <cmath>
 #include "../../include/geon.h"
 #include "geon_kernels.h"
 
@@ -40,7 +41,7 @@ struct FieldState {
     double green_w = -1, green_h = -1, green_soft2 = -1;
 };
 
-constexpr int kTimers = 4;
+constexpr int kTimers = 5;
 thread_local std::string g_create_error;
 
 }  // namespace
@@ -77,6 +78,25 @@ struct GeonContext {
     FieldState fld[2];          // 0 Higgs, 1 axion
     int field_g = 0;
     double* field_out = nullptr;
+    // learned far-field correction
+    uint32_t* far_list = nullptr;
+    int32_t* far_count = nullptr;
+    int64_t far_cap = 0, far_groups = 0;
+    void* far_frags = nullptr;
+    float *far_params = nullptr, *far_stats = nullptr;      // FP32 master parameters, input/output statistics
+    float *far_m = nullptr, *far_v = nullptr, *far_grad = nullptr, *far_loss = nullptr;
+    float *far_gpart = nullptr, *far_lpart = nullptr;
+    int64_t far_blocks_cap = 0;
+    float* far_samples = nullptr;
+    int32_t* far_obs = nullptr;         // observer slot of each sample
+    int64_t far_n_samples = 0, far_cap_samples = 0;
+    int64_t* far_sel = nullptr;
+    int64_t far_sel_cap = 0;
+    int64_t* far_rows = nullptr;
+    int64_t far_rows_cap = 0;
+    double* far_f2 = nullptr;
+    bool far_have_model = false, far_have_stats = false;
+    int64_t far_adam_t = 0;
 };
 
 #define HIP_OK(expr)                                                                         \
@@ -96,6 +116,13 @@ int32_t* I(GeonContext* h, int c) { return reinterpret_cast<int32_t*>(h->st[c]);
 int set_err(GeonContext* h, int code, const std::string& msg) {
     if (h) h->err = msg;
     return code;
+}
+
+template <typename T>
+int dmalloc(GeonContext* h, T** p, size_t count) {
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) { h->err = std::string("hipMalloc: ") + hipGetErrorString(e); return GEON_ENOMEM; }
+    return GEON_OK;
 }
 
 struct Timer {
@@ -134,6 +161,8 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode == GEON_PERIODIC_GHOSTS) { *why = "ghost-image periodic mode is not run (min-image and windowed are)"; return 1; }
     if (!(p->domain_w > 0) || !(p->domain_h > 0)) { *why = "domain must be positive"; return 1; }
     if (p->max_substeps < 1) { *why = "max_substeps must be >= 1"; return 1; }
+    if (p->far_field < GEON_FAR_NONE || p->far_field > GEON_FAR_LEARNED) { *why = "unknown far_field mode"; return 1; }
+    if (p->far_list_cap < 0) { *why = "far_list_cap must be >= 0"; return 1; }
     return 0;
 }
 
@@ -231,7 +260,74 @@ ForceArgs force_args(GeonContext* h) {
     a.cyc_gm = a.pf.gm;
     a.n_upper = h->n_host;
     a.max_nodes = 1 + 4 * h->tb.cap_divided;
+    a.far_mode = 0;
+    a.node_mom = h->tb.node_mom;
+    a.far_list = h->far_list;
+    a.far_count = h->far_count;
+    a.far_cap = h->far_cap;
     return a;
+}
+
+bool want_moments(GeonContext* h) {
+    return (h->prm.toggles0 & GEON_BARNES_HUT) && h->prm.far_field != GEON_FAR_NONE;
+}
+
+int far_list_alloc(GeonContext* h) {
+    const int64_t groups = ((int64_t)h->cap + kWave - 1) / kWave;
+    const int64_t cap = h->prm.far_list_cap > 0 ? h->prm.far_list_cap : 65536;
+    if (h->far_list && h->far_cap == cap && h->far_groups == groups) return GEON_OK;
+    if (1 + 4 * h->tb.cap_divided > (int64_t)kFarNodeMask)
+        return set_err(h, GEON_EUNSUPPORTED, "too many tree nodes for the far-list encoding");
+    hipFree(h->far_list); hipFree(h->far_count);
+    h->far_list = nullptr; h->far_count = nullptr;
+    int rc;
+    if ((rc = dmalloc(h, &h->far_list, (size_t)(groups * cap)))) return rc;
+    if ((rc = dmalloc(h, &h->far_count, (size_t)groups))) return rc;
+    h->far_cap = cap; h->far_groups = groups;
+    return GEON_OK;
+}
+
+int far_model_alloc(GeonContext* h) {
+    if (h->far_params) return GEON_OK;
+    int rc;
+    uint8_t* fr = nullptr;
+    if ((rc = dmalloc(h, &fr, far_frags_bytes()))) return rc;
+    h->far_frags = fr;
+    if ((rc = dmalloc(h, &h->far_params, kFarParams))) return rc;
+    if ((rc = dmalloc(h, &h->far_stats, kFarStats))) return rc;
+    if ((rc = dmalloc(h, &h->far_m, kFarParams))) return rc;
+    if ((rc = dmalloc(h, &h->far_v, kFarParams))) return rc;
+    if ((rc = dmalloc(h, &h->far_grad, kFarParams))) return rc;
+    if ((rc = dmalloc(h, &h->far_loss, 1))) return rc;
+    if ((rc = dmalloc(h, &h->far_f2, 1))) return rc;
+    return GEON_OK;
+}
+
+void far_release(GeonContext* h) {
+    hipFree(h->far_list); hipFree(h->far_count); hipFree(h->far_frags); hipFree(h->far_params);
+    hipFree(h->far_stats); hipFree(h->far_m); hipFree(h->far_v); hipFree(h->far_grad); hipFree(h->far_loss);
+    hipFree(h->far_gpart); hipFree(h->far_lpart); hipFree(h->far_samples); hipFree(h->far_sel);
+    hipFree(h->far_rows); hipFree(h->far_f2); hipFree(h->far_obs);
+}
+
+FarArgs far_args(GeonContext* h) {
+    FarArgs f;
+    f.frags = (const FarFrags*)h->far_frags;
+    f.src4 = h->src4;
+    f.nodes = h->tb.nodes;
+    f.node_mom = h->tb.node_mom;
+    f.far_list = h->far_list;
+    f.far_count = h->far_count;
+    f.far_cap = h->far_cap;
+    f.fx = h->fc[K_FX]; f.fy = h->fc[K_FY];
+    f.ctrl = h->ctrl;
+    const PhysFlags pf = phys_flags(h->prm);
+    f.soft2 = pf.soft2;
+    f.dom_w = pf.dom_w; f.dom_h = pf.dom_h;
+    f.periodic = pf.periodic;
+    f.grav = !!(h->prm.toggles0 & GEON_GRAVITY);
+    f.coul = !!(h->prm.toggles0 & GEON_COULOMB);
+    return f;
 }
 
 FieldArgs field_args(GeonContext* h, int which) {
@@ -359,9 +455,18 @@ int sort_and_gather(GeonContext* h) {
 // integrator.js:563-592), preceded by the tree rebuild (integrator.js:1340-1348).
 // reuse_tree: the tree built for this substep's 1PN pass stands (integrator.js:1340-1348
 // reuses it; with Higgs on, its aggregates are recomputed for the new masses).
-int force_pass(GeonContext* h, bool reuse_tree = false) {
+// collect: list the accepted (particle, node) pairs for training samples
+// instead of applying the far-field correction.
+int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
     const int n = h->n_host;
     const bool bh = !!(h->prm.toggles0 & GEON_BARNES_HUT);
+    const int far = collect ? GEON_FAR_LEARNED : (bh ? h->prm.far_field : GEON_FAR_NONE);
+    if (far == GEON_FAR_LEARNED) {
+        if (!collect && !h->far_have_model)
+            return set_err(h, GEON_ESTATE, "learned far field needs a model (geon_far_set_model or geon_far_train)");
+        int rc = far_list_alloc(h);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
     if (n == 0) return GEON_OK;
     if (bh) {
@@ -372,7 +477,7 @@ int force_pass(GeonContext* h, bool reuse_tree = false) {
         if (!reuse_tree || (h->prm.toggles0 & GEON_HIGGS)) {
             Timer t(h, 1);
             build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
-                       F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+                       F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl, far != GEON_FAR_NONE);
         }
         h->tree_valid = true;
     }
@@ -385,11 +490,17 @@ int force_pass(GeonContext* h, bool reuse_tree = false) {
                            F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), axion ? F(h, C_AX) : nullptr,
                            axion ? F(h, C_YUK) : nullptr, higgs ? F(h, C_HIGGS) : nullptr, h->src4, h->src8a,
                            h->src8b, n, h->ctrl);
-    const ForceArgs fa = force_args(h);
+    ForceArgs fa = force_args(h);
+    if (bh && far == GEON_FAR_MULTIPOLE) fa.far_mode = kFarMultipole;
+    if (bh && far == GEON_FAR_LEARNED) fa.far_mode = kFarList;
     {
         Timer t(h, 0);
         if (bh) launch_walk(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
         else launch_direct(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
+    }
+    if (bh && far == GEON_FAR_LEARNED && !collect) {
+        Timer t(h, 4);
+        launch_far_mlp(h->stream, far_args(h), n);
     }
     if (higgs || axion)   // field forces after the pair forces (integrator.js:1494-1507)
         field_forces(h->stream, field_args(h, 0), field_args(h, 1), higgs, axion, !!(tg & GEON_GRAVITY),
@@ -411,7 +522,7 @@ int one_pn_step(GeonContext* h) {
         if (rc) return rc;
         Timer t(h, 1);
         build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
-                   F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+                   F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl, want_moments(h));
         h->tree_valid = true;
     }
     const uint32_t tg = h->prm.toggles0;
@@ -424,6 +535,14 @@ int one_pn_step(GeonContext* h) {
     launch_vv_kick(h->stream, step_args(h));
     HIP_OK(hipGetLastError());
     return GEON_OK;
+}
+
+int check_overflow(GeonContext* h) {
+    switch (h->ctrl_host->overflow) {
+        case 0: return GEON_OK;
+        case 3: return set_err(h, GEON_EOVERFLOW, "far-field pair list full: raise far_list_cap");
+        default: return set_err(h, GEON_EOVERFLOW, "tree node pool or walk stack overflow");
+    }
 }
 
 int read_ctrl(GeonContext* h) {
@@ -460,12 +579,6 @@ int compact(GeonContext* h) {
     return GEON_OK;
 }
 
-template <typename T>
-int dmalloc(GeonContext* h, T** p, size_t count) {
-    hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
-    if (e != hipSuccess) { h->err = std::string("hipMalloc: ") + hipGetErrorString(e); return GEON_ENOMEM; }
-    return GEON_OK;
-}
 
 int allocate(GeonContext* h) {
     const size_t cap = (size_t)h->cap;
@@ -507,8 +620,11 @@ int allocate(GeonContext* h) {
     if ((rc = dmalloc(h, &tb.node_qq64, nodes))) return rc;
     if ((rc = dmalloc(h, &tb.node_path, nodes))) return rc;
     if ((rc = dmalloc(h, &tb.node_npts, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_first, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.node_mom, nodes))) return rc;
+    if ((rc = dmalloc(h, &tb.pre_mom, 9 * (cap + 1)))) return rc;
     if ((rc = dmalloc(h, &tb.scan_tmp, nblk + 1024))) return rc;
-    if ((rc = dmalloc(h, &tb.scan_tmp64, 8 * (nblk + 1024)))) return rc;
+    if ((rc = dmalloc(h, &tb.scan_tmp64, 9 * (nblk + 1024)))) return rc;
     if ((rc = dmalloc(h, &h->stat_partial, 16 * 1024))) return rc;
     if ((rc = dmalloc(h, &h->stat_out, 16))) return rc;
     h->field_g = h->prm.field_grid > 0 ? h->prm.field_grid : 128;
@@ -550,6 +666,8 @@ void release(GeonContext* h) {
     hipFree(tb.owned); hipFree(tb.rank_start); hipFree(tb.divnode); hipFree(tb.pre); hipFree(tb.nodes);
     hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64); hipFree(tb.node_qq64);
     hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
+    hipFree(tb.node_first); hipFree(tb.node_mom); hipFree(tb.pre_mom);
+    far_release(h);
     hipFree(h->stat_partial); hipFree(h->stat_out); hipFree(h->field_out);
     for (int w = 0; w < 2; w++) {
         FieldState& s = h->fld[w];
@@ -563,6 +681,81 @@ void release(GeonContext* h) {
     for (int k = 0; k < kTimers; k++)
         for (auto& pr : h->ev[k]) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
     if (h->stream) hipStreamDestroy(h->stream);
+}
+
+}  // namespace
+
+namespace {
+
+uint64_t splitmix(uint64_t& z) {
+    uint64_t r = (z += 0x9E3779B97F4A7C15ull);
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+    return r ^ (r >> 31);
+}
+
+// Input means / inverse deviations and output scales from (a subsample of)
+// the held samples; also the loss-weight normaliser 1 / mean(w).
+int far_sample_stats(GeonContext* h, std::vector<float>* stats, float* wscale) {
+    const int64_t ns = h->far_n_samples;
+    const int64_t take = std::min<int64_t>(ns, 1 << 18);
+    const int64_t stride = std::max<int64_t>(1, ns / std::max<int64_t>(take, 1));
+    std::vector<float> buf((size_t)take * kFarSample);
+    HIP_OK(hipMemcpy2D(buf.data(), sizeof(float) * kFarSample, h->far_samples, sizeof(float) * kFarSample * stride,
+                       sizeof(float) * kFarSample, (size_t)take, hipMemcpyDeviceToHost));
+    double sum[kFarIn] = {}, sq[kFarIn] = {}, tq[4] = {}, tn[4] = {}, wsum = 0, wn = 0;
+    for (int64_t r = 0; r < take; r++) {
+        const float* s = &buf[(size_t)r * kFarSample];
+        for (int k = 0; k < kFarIn; k++) { sum[k] += s[k]; sq[k] += (double)s[k] * s[k]; }
+        for (int j = 0; j < 4; j++) {
+            const float w = s[kFarIn + 4 + (j >> 1)];
+            if (w > 0) { tq[j] += (double)s[kFarIn + j] * s[kFarIn + j]; tn[j] += 1; }
+        }
+        for (int j = 0; j < 2; j++) if (s[kFarIn + 4 + j] > 0) { wsum += s[kFarIn + 4 + j]; wn += 1; }
+    }
+    stats->assign(kFarStats, 0.f);
+    for (int k = 0; k < kFarIn; k++) {
+        const double m = take ? sum[k] / take : 0.0;
+        const double var = take ? std::max(0.0, sq[k] / take - m * m) : 0.0;
+        (*stats)[k] = (float)m;
+        (*stats)[kFarIn + k] = (float)(1.0 / std::max(sqrt(var), 1e-6));
+    }
+    for (int j = 0; j < 4; j++) (*stats)[2 * kFarIn + j] = (float)std::max(tn[j] > 0 ? sqrt(tq[j] / tn[j]) : 1.0, 1e-6);
+    *wscale = (float)(wsum > 0 ? wn / wsum : 1.0);
+    return GEON_OK;
+}
+
+int far_pack(GeonContext* h) {
+    launch_far_pack(h->stream, h->far_params, h->far_stats, h->far_frags);
+    HIP_OK(hipGetLastError());
+    return GEON_OK;
+}
+
+int far_blocks(GeonContext* h, int64_t nblocks) {
+    if (nblocks <= h->far_blocks_cap) return GEON_OK;
+    hipFree(h->far_gpart); hipFree(h->far_lpart);
+    h->far_gpart = nullptr; h->far_lpart = nullptr;
+    int rc;
+    if ((rc = dmalloc(h, &h->far_gpart, (size_t)(nblocks * kFarParams)))) return rc;
+    if ((rc = dmalloc(h, &h->far_lpart, (size_t)nblocks))) return rc;
+    h->far_blocks_cap = nblocks;
+    return GEON_OK;
+}
+
+FarTrainArgs far_train_args(GeonContext* h, float wscale) {
+    FarTrainArgs a;
+    a.frags = (const FarFrags*)h->far_frags;
+    a.samples = h->far_samples;
+    a.n_samples = h->far_n_samples;
+    a.rows = nullptr;
+    a.batch = 0;
+    a.seed = 0;
+    a.step = 0;
+    a.wscale = wscale;
+    a.grad_part = h->far_gpart; a.loss_part = h->far_lpart;
+    a.grad = h->far_grad; a.loss_out = h->far_loss;
+    a.params = h->far_params; a.adam_m = h->far_m; a.adam_v = h->far_v;
+    return a;
 }
 
 }  // namespace
@@ -793,7 +986,8 @@ int geon_compute_forces(GeonHandle h) {
     if (rc) return rc;
     HIP_OK(hipStreamSynchronize(h->stream));
     h->forces_init = true;
-    return read_ctrl(h);
+    if ((rc = read_ctrl(h))) return rc;
+    return check_overflow(h);
 }
 
 int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
@@ -844,7 +1038,7 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
     h->last_counters[0] = h->ctrl_host->cnt_pair;
     h->last_counters[1] = h->ctrl_host->cnt_node;
     h->last_counters[2] = h->ctrl_host->cnt_visit;
-    if (h->ctrl_host->overflow) return set_err(h, GEON_EOVERFLOW, "tree node pool or walk stack overflow");
+    if ((rc = check_overflow(h))) return rc;
     if (h->ctrl_host->deaths > 0) {
         if ((rc = compact(h))) return rc;
         HIP_OK(hipStreamSynchronize(h->stream));
@@ -923,7 +1117,7 @@ int geon_build_tree(GeonHandle h) {
     int rc = sort_and_gather(h);
     if (rc) return rc;
     build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
-               I(h, C_QQ), h->n_host, key_params(h->prm), h->ctrl);
+               I(h, C_QQ), h->n_host, key_params(h->prm), h->ctrl, want_moments(h));
     h->tree_valid = true;
     return read_ctrl(h);
 }
@@ -1106,6 +1300,205 @@ int geon_kernel_timing(GeonHandle h, int32_t kernel, double* total_ms, int64_t* 
     }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = (int64_t)h->ev_used[kernel];
+    return GEON_OK;
+}
+
+// ---- learned far-field correction --------------------------------------------
+int geon_far_clear(GeonHandle h) {
+    if (!h) return GEON_EINVAL;
+    h->far_n_samples = 0;
+    return GEON_OK;
+}
+
+int geon_far_collect(GeonHandle h, int32_t group_stride, int64_t* n_samples) {
+    if (!h || group_stride < 1) return GEON_EINVAL;
+    if (!(h->prm.toggles0 & GEON_BARNES_HUT)) return set_err(h, GEON_ESTATE, "far-field samples need Barnes-Hut on");
+    const int n = h->n_host;
+    int rc;
+    if ((rc = far_model_alloc(h))) return rc;
+    if (n > 0) {
+        if ((rc = force_pass(h, false, true))) return rc;
+        h->forces_init = false;             // forces above lack the far-field correction
+        if ((rc = read_ctrl(h))) return rc;
+        if ((rc = check_overflow(h))) return rc;
+        const int64_t groups = (n + kWave - 1) / kWave;
+        std::vector<int32_t> cnt((size_t)groups);
+        HIP_OK(hipMemcpy(cnt.data(), h->far_count, 4 * (size_t)groups, hipMemcpyDeviceToHost));
+        std::vector<int64_t> sel, off;
+        int64_t rows = 0;
+        for (int64_t g = 0; g < groups; g += group_stride) { sel.push_back(g); off.push_back(rows); rows += cnt[g]; }
+        const int64_t ns = (int64_t)sel.size();
+        if (2 * ns > h->far_sel_cap) {
+            hipFree(h->far_sel);
+            h->far_sel = nullptr;
+            if ((rc = dmalloc(h, &h->far_sel, (size_t)(2 * ns)))) return rc;
+            h->far_sel_cap = 2 * ns;
+        }
+        HIP_OK(hipMemcpy(h->far_sel, sel.data(), 8 * ns, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(h->far_sel + ns, off.data(), 8 * ns, hipMemcpyHostToDevice));
+        const int64_t need = h->far_n_samples + rows;
+        if (need > h->far_cap_samples) {
+            const int64_t cap = std::max(need, 2 * h->far_cap_samples);
+            float* ns_buf = nullptr;
+            int32_t* no_buf = nullptr;
+            if ((rc = dmalloc(h, &ns_buf, (size_t)(cap * kFarSample)))) return rc;
+            if ((rc = dmalloc(h, &no_buf, (size_t)cap))) { hipFree(ns_buf); return rc; }
+            if (h->far_n_samples) {
+                HIP_OK(hipMemcpy(ns_buf, h->far_samples, sizeof(float) * kFarSample * h->far_n_samples,
+                                 hipMemcpyDeviceToDevice));
+                HIP_OK(hipMemcpy(no_buf, h->far_obs, sizeof(int32_t) * h->far_n_samples, hipMemcpyDeviceToDevice));
+            }
+            hipFree(h->far_samples);
+            hipFree(h->far_obs);
+            h->far_samples = ns_buf;
+            h->far_obs = no_buf;
+            h->far_cap_samples = cap;
+        }
+        launch_far_f2(h->stream, h->fc[K_FX], h->fc[K_FY], h->ctrl, h->far_f2);
+        FarSampleArgs a;
+        a.x = F(h, C_X); a.y = F(h, C_Y); a.m = F(h, C_MASS); a.qq = I(h, C_QQ);
+        a.nodes = h->tb.nodes; a.node_mom = h->tb.node_mom;
+        a.mass64 = h->tb.node_mass64; a.comx64 = h->tb.node_comx64; a.comy64 = h->tb.node_comy64;
+        a.qq64 = h->tb.node_qq64; a.first = h->tb.node_first; a.npts = h->tb.node_npts;
+        a.far_list = h->far_list; a.far_cap = h->far_cap;
+        a.groups = h->far_sel; a.row_off = h->far_sel + ns; a.n_groups = (int)ns; a.n_rows = rows;
+        a.f2ref = h->far_f2;
+        a.out = h->far_samples + (size_t)h->far_n_samples * kFarSample;
+        a.slot = I(h, C_SLOT);
+        a.obs = h->far_obs + h->far_n_samples;
+        const PhysFlags pf = phys_flags(h->prm);
+        a.soft2 = pf.soft2; a.dom_w = h->prm.domain_w; a.dom_h = h->prm.domain_h; a.periodic = pf.periodic;
+        a.grav = !!(h->prm.toggles0 & GEON_GRAVITY);
+        a.coul = !!(h->prm.toggles0 & GEON_COULOMB);
+        launch_far_samples(h->stream, a);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(h->stream));
+        h->far_n_samples = need;
+    }
+    if (n_samples) *n_samples = h->far_n_samples;
+    return GEON_OK;
+}
+
+int geon_far_get_samples(GeonHandle h, int64_t first, int64_t n, float* out, int32_t* observer) {
+    if (!h || first < 0 || n < 0) return GEON_EINVAL;
+    if (first + n > h->far_n_samples) return set_err(h, GEON_EINVAL, "sample range beyond the samples held");
+    if (n && out)
+        HIP_OK(hipMemcpy(out, h->far_samples + (size_t)first * kFarSample, sizeof(float) * kFarSample * n,
+                         hipMemcpyDeviceToHost));
+    if (n && observer)
+        HIP_OK(hipMemcpy(observer, h->far_obs + first, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    return GEON_OK;
+}
+
+int geon_far_train(GeonHandle h, int32_t steps, int32_t batch, double lr, double lr_final, uint64_t seed,
+                   float* loss_out) {
+    if (!h || steps < 0 || batch < 1) return GEON_EINVAL;
+    if (h->far_n_samples == 0) return set_err(h, GEON_ESTATE, "no training samples (geon_far_collect)");
+    int rc;
+    if ((rc = far_model_alloc(h))) return rc;
+    std::vector<float> stats;
+    float wscale = 1.f;
+    if ((rc = far_sample_stats(h, &stats, &wscale))) return rc;
+    if (!h->far_have_stats) {
+        HIP_OK(hipMemcpy(h->far_stats, stats.data(), sizeof(float) * kFarStats, hipMemcpyHostToDevice));
+        h->far_have_stats = true;
+    }
+    if (!h->far_have_model) {
+        // U(-1/sqrt(fan_in), 1/sqrt(fan_in)) per layer; W1 rows 0..10 (10 inputs + bias), rows 11..15 zero
+        std::vector<float> p(kFarParams, 0.f);
+        uint64_t z = seed;
+        auto uni = [&](double a) { return (float)(a * (2.0 * ((splitmix(z) >> 11) * 0x1.0p-53) - 1.0)); };
+        for (int k = 0; k <= kFarIn; k++)
+            for (int m = 0; m < kFarHid; m++) p[kFarOffW1 + k * kFarHid + m] = uni(1.0 / sqrt((double)kFarIn));
+        for (int k = 0; k < kFarHid * kFarHid; k++) p[kFarOffW2 + k] = uni(0.125);
+        for (int k = 0; k < kFarHid; k++) p[kFarOffB2 + k] = uni(0.125);
+        for (int k = 0; k < kFarHid * 4; k++) p[kFarOffW3 + k] = uni(0.125);
+        for (int k = 0; k < 4; k++) p[kFarOffB3 + k] = uni(0.125);
+        HIP_OK(hipMemcpy(h->far_params, p.data(), sizeof(float) * kFarParams, hipMemcpyHostToDevice));
+        HIP_OK(hipMemset(h->far_m, 0, sizeof(float) * kFarParams));
+        HIP_OK(hipMemset(h->far_v, 0, sizeof(float) * kFarParams));
+        h->far_adam_t = 0;
+        h->far_have_model = true;
+    }
+    const int64_t nblocks = (batch + kWave - 1) / kWave;
+    if ((rc = far_blocks(h, nblocks))) return rc;
+    float* dloss = nullptr;
+    if (steps > 0 && (rc = dmalloc(h, &dloss, (size_t)steps))) return rc;
+    FarTrainArgs a = far_train_args(h, wscale);
+    a.batch = batch;
+    a.seed = seed;
+    for (int s = 0; s < steps; s++) {
+        const double c = 0.5 * (1.0 + cos(M_PI * s / std::max(1, steps)));
+        const float lr_s = (float)(lr_final + (lr - lr_final) * c);
+        a.step = h->far_adam_t;
+        a.loss_out = dloss + s;
+        if ((rc = far_pack(h))) { hipFree(dloss); return rc; }
+        launch_far_train(h->stream, a, (int)nblocks, 1, lr_s, ++h->far_adam_t);
+    }
+    if ((rc = far_pack(h))) { hipFree(dloss); return rc; }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && loss_out && steps > 0)
+        e = hipMemcpyAsync(loss_out, dloss, sizeof(float) * steps, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(dloss);
+    if (e != hipSuccess) return set_err(h, GEON_EDEVICE, std::string("far training: ") + hipGetErrorString(e));
+    return GEON_OK;
+}
+
+int geon_far_grad(GeonHandle h, const int64_t* rows, int32_t n_rows, float* grad_out, float* loss_out) {
+    if (!h || !rows || n_rows < 1) return GEON_EINVAL;
+    if (!h->far_have_model) return set_err(h, GEON_ESTATE, "no far-field model");
+    for (int32_t i = 0; i < n_rows; i++)
+        if (rows[i] < 0 || rows[i] >= h->far_n_samples) return set_err(h, GEON_EINVAL, "sample row out of range");
+    int rc;
+    const int64_t nblocks = (n_rows + kWave - 1) / kWave;
+    if ((rc = far_blocks(h, nblocks))) return rc;
+    if (n_rows > h->far_rows_cap) {
+        hipFree(h->far_rows);
+        h->far_rows = nullptr;
+        if ((rc = dmalloc(h, &h->far_rows, (size_t)n_rows))) return rc;
+        h->far_rows_cap = n_rows;
+    }
+    HIP_OK(hipMemcpy(h->far_rows, rows, 8 * (size_t)n_rows, hipMemcpyHostToDevice));
+    std::vector<float> stats;
+    float wscale = 1.f;
+    if ((rc = far_sample_stats(h, &stats, &wscale))) return rc;
+    FarTrainArgs a = far_train_args(h, wscale);
+    a.rows = h->far_rows;
+    a.batch = n_rows;
+    if ((rc = far_pack(h))) return rc;
+    launch_far_train(h->stream, a, (int)nblocks, 0, 0.f, 1);
+    HIP_OK(hipGetLastError());
+    if (grad_out) HIP_OK(hipMemcpyAsync(grad_out, h->far_grad, sizeof(float) * kFarParams, hipMemcpyDeviceToHost, h->stream));
+    if (loss_out) HIP_OK(hipMemcpyAsync(loss_out, h->far_loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
+int geon_far_get_model(GeonHandle h, float* out, int32_t n) {
+    if (!h || !out || n != GEON_FAR_MODEL_FLOATS) return GEON_EINVAL;
+    if (!h->far_have_model) return set_err(h, GEON_ESTATE, "no far-field model");
+    HIP_OK(hipMemcpy(out, h->far_params, sizeof(float) * kFarParams, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(out + kFarParams, h->far_stats, sizeof(float) * kFarStats, hipMemcpyDeviceToHost));
+    return GEON_OK;
+}
+
+int geon_far_set_model(GeonHandle h, const float* in, int32_t n) {
+    if (!h || !in || n != GEON_FAR_MODEL_FLOATS) return GEON_EINVAL;
+    for (int32_t i = 0; i < n; i++)
+        if (!std::isfinite(in[i])) return set_err(h, GEON_EINVAL, "model holds a non-finite value");
+    int rc;
+    if ((rc = far_model_alloc(h))) return rc;
+    HIP_OK(hipMemcpy(h->far_params, in, sizeof(float) * kFarParams, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->far_stats, in + kFarParams, sizeof(float) * kFarStats, hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(h->far_m, 0, sizeof(float) * kFarParams));
+    HIP_OK(hipMemset(h->far_v, 0, sizeof(float) * kFarParams));
+    h->far_adam_t = 0;
+    h->far_have_model = true;
+    h->far_have_stats = true;
+    h->forces_init = false;
+    if ((rc = far_pack(h))) return rc;
+    HIP_OK(hipStreamSynchronize(h->stream));
     return GEON_OK;
 }
 

@@ -40,6 +40,17 @@ struct PhysFlags {
 };
 
 // Tree node record: 16 dwords.
+constexpr double kChargeQuantum = 0.1;   // charges are integer multiples (main.js:504-539)
+
+// Moments of a divided node's points about its COM (not in the reference: the
+// inputs of the far-field correction).  Mass quadrupole q**, charge dipole d*,
+// charge quadrupole c**, and the node's total |charge| aq.
+struct __align__(16) NodeMom {
+    float qxx, qxy, qyy, aq;
+    float dx, dy, cxx, cxy;
+    float cyy, pad0, pad1, pad2;
+};
+
 struct __align__(64) Node {
     float comx, comy, mass, charge;
     float mu, L, avx, avy;          // avx/avy: momentum / mass (proper velocity, quadtree.js:285)
@@ -91,6 +102,39 @@ struct Acc {
 __device__ __forceinline__ float torus_wrap(float d, float full, float half) {
     // topology.js:9-12
     return d > half ? d - full : (d < -half ? d + full : d);
+}
+
+// Far-field terms of an accepted node beyond its monopole (not in the
+// reference): with the softened kernel g(d) = d h(r2), h = r2^-3/2,
+// r2 = |d|^2 + eps^2, the node's points at COM + delta_j contribute
+//   gravity  m_i [ 2 (Q d) h' + tr(Q) d h' + 2 d (d.Q.d) h'' ]          (1/2 Q : grad grad g)
+//   Coulomb -q_i [ D h + 2 d (D.d) h' + (the same quadrupole form in C) ]
+// with h' = dh/dr2 = -3/2 r2^-5/2 and h'' = 15/4 r2^-7/2; Q, D, C are the
+// node's mass quadrupole, charge dipole and charge quadrupole about its COM.
+template <uint32_t F, bool KAHAN>
+__device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const PhysFlags& pf, float dx, float dy,
+                                              const NodeMom& c) {
+    const float r2 = dx * dx + dy * dy + pf.soft2;
+    const float ir2 = 1.f / r2;
+    const float h = __builtin_amdgcn_rsqf(r2) * ir2;
+    const float h1 = -1.5f * h * ir2, h2 = 3.75f * h * ir2 * ir2;
+    float fx = 0.f, fy = 0.f;
+    if (F & F_GRAV) {
+        const float qx = c.qxx * dx + c.qxy * dy, qy = c.qxy * dx + c.qyy * dy;
+        const float dqd = dx * qx + dy * qy, tr = c.qxx + c.qyy;
+        fx += o.m * (2.f * qx * h1 + tr * dx * h1 + 2.f * dx * dqd * h2);
+        fy += o.m * (2.f * qy * h1 + tr * dy * h1 + 2.f * dy * dqd * h2);
+    }
+    if (F & F_COUL) {
+        float qi = o.q;
+        if ((F & F_EXTRA) && pf.ax_mod) qi *= sqrtf(o.ax);   // node sources carry axMod = 1
+        const float dd = c.dx * dx + c.dy * dy;
+        const float qx = c.cxx * dx + c.cxy * dy, qy = c.cxy * dx + c.cyy * dy;
+        const float dqd = dx * qx + dy * qy, tr = c.cxx + c.cyy;
+        fx -= qi * (c.dx * h + 2.f * dx * dd * h1 + 2.f * qx * h1 + tr * dx * h1 + 2.f * dx * dqd * h2);
+        fy -= qi * (c.dy * h + 2.f * dy * dd * h1 + 2.f * qy * h1 + tr * dy * h1 + 2.f * dy * dqd * h2);
+    }
+    a.addf(fx, fy);
 }
 
 // One source acting on the observer: the body of pairForce (forces.js:163-442)

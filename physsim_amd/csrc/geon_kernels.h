@@ -41,6 +41,9 @@ struct TreeBufs {
     double *node_mass64, *node_comx64, *node_comy64, *node_qq64;
     uint64_t* node_path;
     int32_t* node_npts;         // points in each node's subtree
+    int32_t* node_first;        // first sorted point of each node's subtree
+    NodeMom* node_mom;          // far-field moments of divided nodes (when requested)
+    double* pre_mom;            // 9 columns of (cap + 1) moment prefix sums
     int32_t* scan_tmp;          // block sums for int scans
     double* scan_tmp64;
     int64_t cap_particles, cap_divided;
@@ -59,7 +62,7 @@ void radix_sort(hipStream_t s, TreeBufs& tb, int n_upper, int key_bits, const Ct
 void launch_gather(hipStream_t s, const Columns& cols, const uint32_t* perm, int n_upper, const Ctrl* ctrl);
 void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, const float* m,
                 const float* wx, const float* wy, const float* mu, const float* L,
-                const int32_t* qq, int n_upper, KeyParams kp, Ctrl* ctrl);
+                const int32_t* qq, int n_upper, KeyParams kp, Ctrl* ctrl, bool moments = false);
 // Exclusive scan of int32 flags (n from device ctrl->n_alive) -> out, total -> *total_dev.
 void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n_upper,
                         const int32_t* n_dev, int32_t* total_dev, int32_t* tmp);
@@ -80,7 +83,17 @@ struct ForceArgs {
     uint32_t cyc_mag, cyc_gm;   // which B fields enter the cyclotron estimate
     int n_upper;
     int64_t max_nodes;          // node records allocated (walk bounds check)
+    // far field of accepted nodes: 0 monopole only, 1 + quadrupole/dipole
+    // terms from node_mom, 2 also list (node, lane) per 64-particle group
+    int far_mode;
+    const NodeMom* node_mom;
+    uint32_t* far_list;         // far_cap entries per group: node | lane << 26
+    int32_t* far_count;         // entries per group
+    int64_t far_cap;
 };
+constexpr int kFarMultipole = 1, kFarList = 2;
+constexpr int kFarLaneShift = 26;
+constexpr uint32_t kFarNodeMask = (1u << kFarLaneShift) - 1;
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan);
 void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan);
 void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const float* m,
@@ -107,6 +120,74 @@ struct PairArgs {
     int n_upper;
 };
 void launch_pairs(hipStream_t s, const PairArgs& a, bool write);
+
+// ---- learned far-field correction (k_learn.hip) ----
+constexpr int kFarIn = 10;          // features of a pair
+constexpr int kFarHid = 64;         // hidden width (2 layers)
+constexpr int kFarSample = 16;      // floats per training sample: 10 features, 4 targets, 2 weights
+constexpr int kFarOffW1 = 0;        // W1[16][64] (input 10 = bias of layer 1)
+constexpr int kFarOffW2 = 1024;     // W2[64][64]
+constexpr int kFarOffB2 = 5120;     // b2[64]
+constexpr int kFarOffW3 = 5184;     // W3[64][4]
+constexpr int kFarOffB3 = 5440;     // b3[4]
+constexpr int kFarParams = 5444;
+constexpr int kFarStats = 24;       // feature mean[10], 1/std[10], output scale[4]
+
+struct FarFrags;
+struct FarArgs {
+    const FarFrags* frags;
+    const float4* src4;
+    const Node* nodes;
+    const NodeMom* node_mom;
+    const uint32_t* far_list;
+    const int32_t* far_count;
+    int64_t far_cap;
+    float *fx, *fy;
+    const Ctrl* ctrl;
+    float soft2, dom_w, dom_h;
+    int periodic, grav, coul;
+};
+struct FarSampleArgs {
+    const float *x, *y, *m;
+    const int32_t* qq;
+    const Node* nodes;
+    const NodeMom* node_mom;
+    const double *mass64, *comx64, *comy64, *qq64;
+    const int32_t *first, *npts;
+    const uint32_t* far_list;
+    int64_t far_cap;
+    const int64_t* groups;      // selected groups
+    const int64_t* row_off;     // first sample row of each selected group
+    int n_groups;
+    int64_t n_rows;
+    const double* f2ref;        // mean |F|^2 of the system (loss weight normaliser)
+    float* out;                 // n_rows * kFarSample
+    const int32_t* slot;        // observer slots (optional record)
+    int32_t* obs;               // n_rows observer slots, or null
+    double soft2, dom_w, dom_h;
+    int periodic, grav, coul;
+};
+struct FarTrainArgs {
+    const FarFrags* frags;
+    const float* samples;
+    int64_t n_samples;
+    const int64_t* rows;        // explicit rows (gradient query) or null (random batch)
+    int64_t batch;
+    uint64_t seed;
+    int64_t step;
+    float wscale;
+    float* grad_part;           // blocks * kFarParams
+    float* loss_part;           // blocks
+    float* grad;                // kFarParams
+    float* loss_out;            // 1
+    float *params, *adam_m, *adam_v;
+};
+size_t far_frags_bytes();
+void launch_far_pack(hipStream_t s, const float* params, const float* stats, void* frags);
+void launch_far_mlp(hipStream_t s, const FarArgs& a, int n_upper);
+void launch_far_samples(hipStream_t s, const FarSampleArgs& a);
+void launch_far_train(hipStream_t s, const FarTrainArgs& a, int nblocks, int apply, float lr, int64_t t);
+void launch_far_f2(hipStream_t s, const float* fx, const float* fy, const Ctrl* ctrl, double* out);
 
 // ---- scalar fields (k_field.hip) ----
 struct FieldArgs {

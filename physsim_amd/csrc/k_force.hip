@@ -116,6 +116,9 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
     const PhysFlags pf = a.pf;
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
     int64_t c_pair = 0, c_node = 0, c_visit = 0;
+    // far list of this 64-particle group (far_mode == kFarList)
+    const int64_t group = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+    int64_t far_n = 0;
     int sp = 0;
     if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
     __builtin_amdgcn_wave_barrier();
@@ -162,6 +165,17 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
                 pair_force<F, KAHAN>(o, acc, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 0.f, nd.mu, nd.L,
                                      1.f, 1.f, 1.f);
                 c_node++;
+                if (a.far_mode == kFarMultipole) far_multipole<F, KAHAN>(o, acc, pf, dx, dy, a.node_mom[node]);
+            }
+            if (a.far_mode == kFarList) {
+                // the accepting lanes append (node, lane) in lane order: one coalesced run
+                const uint64_t took = __ballot(accept);
+                if (took) {
+                    const int64_t pos = far_n + __popcll(took & (my_bit - 1));
+                    if (accept && pos < a.far_cap)
+                        a.far_list[group * a.far_cap + pos] = (uint32_t)node | ((uint32_t)lane << kFarLaneShift);
+                    far_n += __popcll(took);
+                }
             }
             const uint64_t open = __ballot(mine && !accept);
             if (open) {
@@ -182,6 +196,10 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
         }
     }
     if (overflow && lane == 0) a.ctrl->overflow = 2;
+    if (a.far_mode == kFarList && lane == 0) {
+        a.far_count[group] = (int32_t)(far_n < a.far_cap ? far_n : a.far_cap);
+        if (far_n > a.far_cap) a.ctrl->overflow = 3;
+    }
     wave_add_counters(a.ctrl, c_pair, c_node, c_visit);
     if (valid) store_acc<KAHAN>(a, i, o, acc);
 }

@@ -1,0 +1,665 @@
+// k_learn.hip — learned far-field correction of the Barnes-Hut walk (not in the
+// reference; the reference's accepted node is a monopole, forces.js:680-685).
+//
+// For every (observer i, accepted divided node n) pair the walk lists, a
+// 2 x 64 GELU MLP predicts the monopole-truncation residual from the node's
+// quadrupole moments and the offset normalised by the node size.  The inputs
+// and outputs are expressed in the frame of the offset direction u = d/|d|,
+// which makes the map rotation-equivariant by construction:
+//   f0 = s/r, f1 = eps^2/r^2,
+//   f2..f4 = (Q_uu, Q_uv, Q_vv) / (M s^2)        mass quadrupole about the COM
+//   f5, f6 = (D_u, D_v) / (A s)                  charge dipole about the COM
+//   f7..f9 = (C_uu, C_uv, C_vv) / (A s^2)        charge quadrupole about the COM
+// with r^2 = |d|^2 + eps^2, s the node width, M its mass, A its total |charge|,
+// v = (-u_y, u_x).  Outputs o = (g_u, g_v, c_u, c_v) (times oscale) give
+//   F_i += m_i M (s/r)^2 / r^2 (g_u u + g_v v) + q_i A (s/r) / r^2 (c_u u + c_v v).
+// Network: x = ((f - mean) * istd, 1, 0...) in 16 inputs (input 10 carries
+// the first layer's bias), z1 = W1^T x, a1 = gelu(z1), z2 = W2^T a1 + b2,
+// a2 = gelu(z2), o = W3^T a2 + b3; gelu is the tanh form.  BF16 operands,
+// FP32 accumulation, on v_mfma_f32_32x32x16_bf16.
+//
+// MFMA 32x32x16 lane layout (gfx950, lane l, h = l / 32, c = l % 32):
+//   A[m][k]: lane holds A[c][8h + j], j = 0..7
+//   B[k][n]: lane holds B[8h + j][c]
+//   C[m][n]: lane holds C[8(r/4) + 4h + r%4][c] in accumulator r = 0..15.
+// All activations are kept transposed (hidden x rows, rows = pairs in the
+// N dimension), so a C tile of one layer is the B operand of the next once
+// the K order of the next layer's weights is permuted to the C layout:
+// K-step t takes accumulators r = 8(t%2) + j of M-block t/2, i.e. hidden unit
+//   kperm(t, h, j) = 32(t/2) + 16(t%2) + 8(j/4) + 4h + (j%4).
+#include "geon_kernels.h"
+
+namespace geon {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+
+__device__ __forceinline__ floatx16 mfma32(const bf16x8& a, const bf16x8& b, const floatx16& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__host__ __device__ __forceinline__ int kperm(int t, int h, int j) {
+    return 32 * (t >> 1) + 16 * (t & 1) + 8 * (j >> 2) + 4 * h + (j & 3);
+}
+
+// tanh-form GELU: z * sigma(2u), u = sqrt(2/pi) (z + 0.044715 z^3)
+constexpr float kGeluC0 = 0.7978845608028654f;
+constexpr float kGeluK0 = -2.f * kGeluC0 * 1.4426950408889634f;   // exp(-2u) = exp2(z (k0 + k1 z^2))
+constexpr float kGeluK1 = kGeluK0 * 0.044715f;
+
+__device__ __forceinline__ float gelu_sig(float z) {
+    const float e = exp2f(z * (kGeluK0 + kGeluK1 * z * z));
+    return __builtin_amdgcn_rcpf(1.f + e);
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
+__device__ __forceinline__ float gelu_grad(float z) {
+    const float s = gelu_sig(z);
+    return s + z * s * (1.f - s) * (2.f * kGeluC0) * (1.f + 3.f * 0.044715f * z * z);
+}
+
+__device__ __forceinline__ bf16x8 pack8(const float* v) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = (__bf16)v[j];
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Features of one (observer, node) pair
+// ---------------------------------------------------------------------------
+struct PairFeat {
+    float f[kFarIn];
+    float ux, uy, sg0, sc0;     // direction and the output scales without m_i, q_i
+};
+
+template <typename T>
+__device__ __forceinline__ void pair_features(T dx, T dy, T eps2, T s, T M, const NodeMom& c, T* f, T& ux, T& uy,
+                                              T& sg0, T& sc0) {
+    const T rho2 = dx * dx + dy * dy;
+    const T r2 = rho2 + eps2;
+    const T ir2 = T(1) / r2;
+    const T r = sqrt(r2);
+    const T irho = T(1) / sqrt(rho2);
+    ux = dx * irho; uy = dy * irho;
+    const T sr = s / r;
+    const T iM = M > T(0) ? T(1) / (M * s * s) : T(0);
+    const T A = (T)c.aq;
+    const T iA = A > T(0) ? T(1) / (A * s) : T(0);
+    const T uu = ux * ux, vv = uy * uy, uv = ux * uy;
+    f[0] = sr;
+    f[1] = eps2 * ir2;
+    f[2] = (uu * c.qxx + T(2) * uv * c.qxy + vv * c.qyy) * iM;
+    f[3] = (-uv * c.qxx + (uu - vv) * c.qxy + uv * c.qyy) * iM;
+    f[4] = (vv * c.qxx - T(2) * uv * c.qxy + uu * c.qyy) * iM;
+    f[5] = (ux * c.dx + uy * c.dy) * iA;
+    f[6] = (-uy * c.dx + ux * c.dy) * iA;
+    f[7] = (uu * c.cxx + T(2) * uv * c.cxy + vv * c.cyy) * iA / s;
+    f[8] = (-uv * c.cxx + (uu - vv) * c.cxy + uv * c.cyy) * iA / s;
+    f[9] = (vv * c.cxx - T(2) * uv * c.cxy + uu * c.cyy) * iA / s;
+    sg0 = M * sr * sr * ir2;
+    sc0 = A * sr * ir2;
+}
+
+// ---------------------------------------------------------------------------
+// Model fragments: the network's weights arranged per lane for the MFMAs
+// ---------------------------------------------------------------------------
+struct FarFrags {
+    bf16x8 w1[2][kWave];            // layer 1, A = W1^T, M-block mb
+    bf16x8 w2[2][4][kWave];         // layer 2, A = W2^T, K permuted
+    bf16x8 w3[4][kWave];            // layer 3, A = W3^T padded to 32 rows, K permuted
+    bf16x8 w3t[2][kWave];           // backward: A = W3 (hidden x 4 outputs, K padded to 16)
+    bf16x8 w2t[2][4][kWave];        // backward: A = W2 (in x out), K (out) permuted
+    float b2[kFarHid];
+    float b3[4];
+    float mean[kFarIn], istd[kFarIn], oscale[4];
+};
+
+__global__ void k_far_pack(const float* __restrict__ P, const float* __restrict__ stats, FarFrags* __restrict__ fr) {
+    const int l = threadIdx.x;          // one wave
+    const int h = l >> 5, c = l & 31;
+    const float* W1 = P + kFarOffW1;
+    const float* W2 = P + kFarOffW2;
+    const float* W3 = P + kFarOffW3;
+    for (int mb = 0; mb < 2; mb++) {
+        float v[8];
+        for (int j = 0; j < 8; j++) v[j] = W1[(8 * h + j) * kFarHid + 32 * mb + c];
+        fr->w1[mb][l] = pack8(v);
+        for (int t = 0; t < 4; t++) {
+            for (int j = 0; j < 8; j++) v[j] = W2[kperm(t, h, j) * kFarHid + 32 * mb + c];
+            fr->w2[mb][t][l] = pack8(v);
+            for (int j = 0; j < 8; j++) v[j] = W2[(32 * mb + c) * kFarHid + kperm(t, h, j)];
+            fr->w2t[mb][t][l] = pack8(v);
+        }
+        for (int j = 0; j < 8; j++) v[j] = (h == 0 && j < 4) ? W3[(32 * mb + c) * 4 + j] : 0.f;
+        fr->w3t[mb][l] = pack8(v);
+    }
+    for (int t = 0; t < 4; t++) {
+        float v[8];
+        for (int j = 0; j < 8; j++) v[j] = c < 4 ? W3[kperm(t, h, j) * 4 + c] : 0.f;
+        fr->w3[t][l] = pack8(v);
+    }
+    fr->b2[l] = P[kFarOffB2 + l];
+    if (l < 4) fr->b3[l] = P[kFarOffB3 + l];
+    if (l < kFarIn) {
+        fr->mean[l] = stats[l];
+        fr->istd[l] = stats[kFarIn + l];
+    }
+    if (l < 4) fr->oscale[l] = stats[2 * kFarIn + l];
+}
+
+// Standardised inputs of one row as the 16 BF16 values of the layer-1 operand.
+__device__ __forceinline__ void std_inputs(const FarFrags& fr, const float* f, bool valid, float* x) {
+#pragma unroll
+    for (int k = 0; k < kFarIn; k++) x[k] = valid ? (f[k] - fr.mean[k]) * fr.istd[k] : 0.f;
+    x[kFarIn] = valid ? 1.f : 0.f;
+#pragma unroll
+    for (int k = kFarIn + 1; k < 16; k++) x[k] = 0.f;
+}
+
+// Accumulator r of M-block mb holds hidden unit 32mb + 8(r/4) + 4h + r%4.
+__device__ __forceinline__ int c_row(int mb, int r, int h) { return 32 * mb + 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// Forward pass of one 64-row tile whose standardised inputs are in LDS
+// (xs[row][16] BF16).  Returns the layer-3 tiles; lanes with h = 0 hold
+// outputs 0..3 of row 32 nb + c in acc[0..3] of o[nb].  If z1/z2 are given,
+// the pre-activations are kept (training).
+struct Fwd {
+    floatx16 z1[2][2], z2[2][2];
+    bf16x8 a1[4][2], a2[4][2];
+    floatx16 o[2];
+};
+
+__device__ __forceinline__ void mlp_forward(const FarFrags& fr, const __bf16* xs, int l, Fwd& F) {
+    const int h = l >> 5, c = l & 31;
+    bf16x8 bx[2];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) bx[nb] = *reinterpret_cast<const bf16x8*>(xs + (32 * nb + c) * 16 + 8 * h);
+    const floatx16 zero = {};
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) F.z1[mb][nb] = mfma32(fr.w1[mb][l], bx[nb], zero);
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = gelu(F.z1[t >> 1][nb][8 * (t & 1) + j]);
+            F.a1[t][nb] = pack8(v);
+        }
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++) {
+        floatx16 binit;
+#pragma unroll
+        for (int r = 0; r < 16; r++) binit[r] = fr.b2[c_row(mb, r, h)];
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) {
+            floatx16 acc = binit;
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc = mfma32(fr.w2[mb][t][l], F.a1[t][nb], acc);
+            F.z2[mb][nb] = acc;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = gelu(F.z2[t >> 1][nb][8 * (t & 1) + j]);
+            F.a2[t][nb] = pack8(v);
+        }
+    floatx16 binit3 = {};
+    if (h == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) binit3[r] = fr.b3[r];
+    }
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+        floatx16 acc = binit3;
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc = mfma32(fr.w3[t][l], F.a2[t][nb], acc);
+        F.o[nb] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inference: add the predicted residual of every listed pair to the forces
+// ---------------------------------------------------------------------------
+struct FarRow {
+    float ux, uy, sg, sc;
+};
+
+__global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
+    __shared__ __attribute__((aligned(16))) __bf16 xs[kWave * 16];
+    __shared__ FarRow rows[kWave];
+    __shared__ int rlane[kWave];
+    __shared__ float sacc[kWave][2];
+    const int l = threadIdx.x;
+    const int h = l >> 5, c = l & 31;
+    const int64_t g = blockIdx.x;
+    const int n = a.ctrl->n_alive;
+    if (g * kWave >= n) return;
+    const FarFrags& fr = *a.frags;
+    sacc[l][0] = 0.f; sacc[l][1] = 0.f;
+    const int cnt = a.far_count[g];
+    const uint32_t* list = a.far_list + g * a.far_cap;
+    const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
+    for (int base = 0; base < cnt; base += kWave) {
+        // 1. features of row l
+        const bool valid = base + l < cnt;
+        float f[kFarIn];
+        FarRow rw = {0.f, 0.f, 0.f, 0.f};
+        int lane_of = 0;
+        if (valid) {
+            const uint32_t e = list[base + l];
+            const int node = (int)(e & kFarNodeMask);
+            lane_of = (int)(e >> kFarLaneShift);
+            const int i = (int)(g * kWave) + lane_of;
+            const float4 s4 = a.src4[i];
+            const Node nd = a.nodes[node];
+            const NodeMom mo = a.node_mom[node];
+            float dx = nd.comx - s4.x, dy = nd.comy - s4.y;
+            if (a.periodic) { dx = torus_wrap(dx, a.dom_w, half_w); dy = torus_wrap(dy, a.dom_h, half_h); }
+            float sg0, sc0;
+            pair_features<float>(dx, dy, a.soft2, 2.f * nd.half, nd.mass, mo, f, rw.ux, rw.uy, sg0, sc0);
+            rw.sg = a.grav ? s4.z * sg0 : 0.f;
+            rw.sc = a.coul ? s4.w * sc0 : 0.f;
+        }
+        float x[16];
+        std_inputs(fr, f, valid, x);
+        *reinterpret_cast<bf16x8*>(xs + l * 16) = pack8(x);
+        *reinterpret_cast<bf16x8*>(xs + l * 16 + 8) = pack8(x + 8);
+        rows[l] = rw;
+        rlane[l] = lane_of;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // 2. the network
+        Fwd F;
+        mlp_forward(fr, xs, l, F);
+        // 3. rotate, scale and accumulate per observer lane
+        if (h == 0) {
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                const int row = 32 * nb + c;
+                if (base + row < cnt) {
+                    const FarRow q = rows[row];
+                    const float gu = F.o[nb][0] * fr.oscale[0], gv = F.o[nb][1] * fr.oscale[1];
+                    const float cu = F.o[nb][2] * fr.oscale[2], cv = F.o[nb][3] * fr.oscale[3];
+                    const float fx = q.sg * (q.ux * gu - q.uy * gv) + q.sc * (q.ux * cu - q.uy * cv);
+                    const float fy = q.sg * (q.uy * gu + q.ux * gv) + q.sc * (q.uy * cu + q.ux * cv);
+                    atomicAdd(&sacc[rlane[row]][0], fx);
+                    atomicAdd(&sacc[rlane[row]][1], fy);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const int i = (int)(g * kWave) + l;
+    if (i < n) {
+        a.fx[i] += sacc[l][0];
+        a.fy[i] += sacc[l][1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training samples: features, exact residual targets and loss weights
+// ---------------------------------------------------------------------------
+// One wave per listed pair of the selected groups.  The residual is the FP64
+// sum over the node's points (its Morton range) of the softened gravity and
+// Coulomb kernels minus the node's monopole, each point at offset
+// d + (x_j - COM) from the observer.
+__global__ __launch_bounds__(256) void k_far_samples(FarSampleArgs a) {
+    const int l = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+    if (row >= a.n_rows) return;
+    // selected group owning this row
+    int lo = 0, hi = a.n_groups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.row_off[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    const int64_t g = a.groups[lo];
+    const uint32_t e = a.far_list[g * a.far_cap + (row - a.row_off[lo])];
+    const int node = (int)(e & kFarNodeMask);
+    const int i = (int)(g * kWave) + (int)(e >> kFarLaneShift);
+    const double xi = a.x[i], yi = a.y[i];
+    const double cx = a.comx64[node], cy = a.comy64[node];
+    const double M = a.mass64[node], Q = a.qq64[node] * kChargeQuantum;
+    double dx = cx - xi, dy = cy - yi;
+    if (a.periodic) {
+        const double W = a.dom_w, H = a.dom_h;
+        dx = dx > 0.5 * W ? dx - W : (dx < -0.5 * W ? dx + W : dx);
+        dy = dy > 0.5 * H ? dy - H : (dy < -0.5 * H ? dy + H : dy);
+    }
+    const int first = a.first[node], np = a.npts[node];
+    double gx = 0, gy = 0, qx = 0, qy = 0;
+    for (int j = first + l; j < first + np; j += kWave) {
+        const double ex = dx + ((double)a.x[j] - cx), ey = dy + ((double)a.y[j] - cy);
+        const double r2 = ex * ex + ey * ey + a.soft2;
+        const double hh = 1.0 / (r2 * sqrt(r2));
+        const double mj = a.m[j], qj = a.qq[j] * kChargeQuantum;
+        gx += mj * ex * hh; gy += mj * ey * hh;
+        qx += qj * ex * hh; qy += qj * ey * hh;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        gx += __shfl_xor(gx, off, 64); gy += __shfl_xor(gy, off, 64);
+        qx += __shfl_xor(qx, off, 64); qy += __shfl_xor(qy, off, 64);
+    }
+    if (l != 0) return;
+    const double r2 = dx * dx + dy * dy + a.soft2;
+    const double hh = 1.0 / (r2 * sqrt(r2));
+    const double rgx = gx - M * dx * hh, rgy = gy - M * dy * hh;          // per unit observer mass
+    const double rcx = -(qx - Q * dx * hh), rcy = -(qy - Q * dy * hh);    // per unit observer charge
+    const NodeMom mo = a.node_mom[node];
+    double f[kFarIn], ux, uy, sg0, sc0;
+    pair_features<double>(dx, dy, (double)a.soft2, 2.0 * (double)a.nodes[node].half, M, mo, f, ux, uy, sg0, sc0);
+    float* out = a.out + (size_t)row * kFarSample;
+    for (int k = 0; k < kFarIn; k++) out[k] = (float)f[k];
+    const double ig = sg0 > 0 ? 1.0 / sg0 : 0.0, ic = sc0 > 0 ? 1.0 / sc0 : 0.0;
+    out[kFarIn + 0] = (float)((ux * rgx + uy * rgy) * ig);
+    out[kFarIn + 1] = (float)((-uy * rgx + ux * rgy) * ig);
+    out[kFarIn + 2] = (float)((ux * rcx + uy * rcy) * ic);
+    out[kFarIn + 3] = (float)((-uy * rcx + ux * rcy) * ic);
+    if (a.obs) a.obs[row] = a.slot[i];
+    const double mi = a.m[i], qi = a.qq[i] * kChargeQuantum;
+    const double iF2 = 1.0 / (*a.f2ref > 0 ? *a.f2ref : 1.0);
+    out[kFarIn + 4] = a.grav && sg0 > 0 ? (float)((mi * sg0) * (mi * sg0) * iF2) : 0.f;
+    out[kFarIn + 5] = a.coul && sc0 > 0 ? (float)((qi * sc0) * (qi * sc0) * iF2) : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Training: forward + backward of the weighted squared error on a batch
+// ---------------------------------------------------------------------------
+// Loss L = wscale / B * sum_rows sum_j w_j (oscale_j o_j - t_j)^2 (w = wg for
+// j < 2, wc for j >= 2).  Each block (one wave) takes one tile of 64 rows and
+// writes its partial gradient (kFarParams floats) and loss.
+constexpr int kHS = 72;      // LDS row stride (BF16) of the hidden-major tiles
+
+__device__ __forceinline__ void store_hmajor(__bf16* dst, const floatx16& v, int mb, int nb, int h, int c) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) dst[c_row(mb, r, h) * kHS + 32 * nb + c] = (__bf16)v[r];
+}
+
+__device__ __forceinline__ bf16x8 ld8(const __bf16* p) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = p[j];
+    return r;
+}
+
+__global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
+    __shared__ __attribute__((aligned(16))) __bf16 xs[kWave * 16];     // [row][16]
+    __shared__ __bf16 xf[16 * kHS];                                     // [input][row]
+    __shared__ __bf16 a1f[kFarHid * kHS], a2f[kFarHid * kHS];           // [hidden][row]
+    __shared__ __bf16 dz2f[kFarHid * kHS], dz1f[kFarHid * kHS];
+    __shared__ __bf16 dof[4 * kHS];                                     // [output][row]
+    __shared__ float tg[kWave][4], wt[kWave][2];
+    __shared__ float g1s[4 * 16 * kWave], g2s[4 * 16 * kWave];         // gelu'(z) in accumulator order
+    const int l = threadIdx.x;
+    const int h = l >> 5, c = l & 31;
+    const FarFrags& fr = *a.frags;
+    const floatx16 zero = {};
+    floatx16 gw1[2], gw2[2][2], gw3[2];
+    float gb2[2][16], gb3[4] = {0.f, 0.f, 0.f, 0.f};
+    gw1[0] = gw1[1] = zero;
+    gw2[0][0] = gw2[0][1] = gw2[1][0] = gw2[1][1] = zero;
+    gw3[0] = gw3[1] = zero;
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) gb2[mb][r] = 0.f;
+    float loss = 0.f;
+    const float inv_b = 1.f / (float)a.batch;
+    {
+        const int64_t brow = (int64_t)blockIdx.x * kWave + l;
+        bool valid = brow < a.batch;
+        int64_t s = 0;
+        if (valid) {
+            if (a.rows) s = a.rows[brow];
+            else {
+                // counter-based pick of a sample (splitmix64 of seed, step, row)
+                uint64_t z = a.seed + 0x9E3779B97F4A7C15ull * (uint64_t)(brow + 1) + 0xD1B54A32D192ED03ull * a.step;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                s = (int64_t)(z % (uint64_t)a.n_samples);
+            }
+        }
+        float f[kFarIn];
+        const float* smp = a.samples + (size_t)s * kFarSample;
+#pragma unroll
+        for (int k = 0; k < kFarIn; k++) f[k] = valid ? smp[k] : 0.f;
+        float x[16];
+        std_inputs(fr, f, valid, x);
+        *reinterpret_cast<bf16x8*>(xs + l * 16) = pack8(x);
+        *reinterpret_cast<bf16x8*>(xs + l * 16 + 8) = pack8(x + 8);
+#pragma unroll
+        for (int k = 0; k < 16; k++) xf[k * kHS + l] = (__bf16)x[k];
+#pragma unroll
+        for (int j = 0; j < 4; j++) tg[l][j] = valid ? smp[kFarIn + j] : 0.f;
+        wt[l][0] = valid ? smp[kFarIn + 4] * a.wscale : 0.f;
+        wt[l][1] = valid ? smp[kFarIn + 5] * a.wscale : 0.f;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        Fwd F;
+        mlp_forward(fr, xs, l, F);
+        // activations, hidden-major, for the weight gradients
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                floatx16 v1, v2;
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const float z1 = F.z1[mb][nb][r], z2 = F.z2[mb][nb][r];
+                    v1[r] = gelu(z1); v2[r] = gelu(z2);
+                    g1s[((mb * 2 + nb) * 16 + r) * kWave + l] = gelu_grad(z1);
+                    g2s[((mb * 2 + nb) * 16 + r) * kWave + l] = gelu_grad(z2);
+                }
+                store_hmajor(a1f, v1, mb, nb, h, c);
+                store_hmajor(a2f, v2, mb, nb, h, c);
+            }
+        // output error (lanes h = 0 hold the 4 outputs of row 32 nb + c)
+        bf16x8 bdo[2];
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) {
+            float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (h == 0) {
+                const int row = 32 * nb + c;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float os = fr.oscale[j];
+                    const float err = os * F.o[nb][j] - tg[row][j];
+                    const float w = wt[row][j >> 1];
+                    loss += w * err * err * inv_b;
+                    v[j] = 2.f * w * err * os * inv_b;
+                    gb3[j] += v[j];
+                    dof[j * kHS + row] = (__bf16)v[j];
+                }
+            }
+            bdo[nb] = pack8(v);
+        }
+        // dA2 = W3 dO, dZ2 = dA2 * gelu'(z2)
+        floatx16 dz2[2][2];
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                floatx16 d = mfma32(fr.w3t[mb][l], bdo[nb], zero);
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    d[r] *= g2s[((mb * 2 + nb) * 16 + r) * kWave + l];
+                    gb2[mb][r] += d[r];
+                }
+                dz2[mb][nb] = d;
+                store_hmajor(dz2f, d, mb, nb, h, c);
+            }
+        // dA1 = W2 dZ2 (K over layer-2 units, permuted), dZ1 = dA1 * gelu'(z1)
+        bf16x8 bdz2[4][2];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                float v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[j] = dz2[t >> 1][nb][8 * (t & 1) + j];
+                bdz2[t][nb] = pack8(v);
+            }
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                floatx16 d = zero;
+#pragma unroll
+                for (int t = 0; t < 4; t++) d = mfma32(fr.w2t[mb][t][l], bdz2[t][nb], d);
+#pragma unroll
+                for (int r = 0; r < 16; r++) d[r] *= g1s[((mb * 2 + nb) * 16 + r) * kWave + l];
+                store_hmajor(dz1f, d, mb, nb, h, c);
+            }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // weight gradients: sums over the tile's rows (K = rows)
+#pragma unroll 1
+        for (int ks = 0; ks < 4; ks++) {
+            const int k0 = 16 * ks + 8 * h;
+            bf16x8 ax = c < 16 ? ld8(xf + c * kHS + k0) : bf16x8{};
+            bf16x8 a1[2], a2[2], bz1[2], bz2[2];
+#pragma unroll
+            for (int mb = 0; mb < 2; mb++) {
+                a1[mb] = ld8(a1f + (32 * mb + c) * kHS + k0);
+                a2[mb] = ld8(a2f + (32 * mb + c) * kHS + k0);
+                bz1[mb] = ld8(dz1f + (32 * mb + c) * kHS + k0);
+                bz2[mb] = ld8(dz2f + (32 * mb + c) * kHS + k0);
+            }
+            bf16x8 bo = c < 4 ? ld8(dof + c * kHS + k0) : bf16x8{};
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) gw1[nb] = mfma32(ax, bz1[nb], gw1[nb]);
+#pragma unroll
+            for (int ma = 0; ma < 2; ma++) {
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++) gw2[ma][nb] = mfma32(a1[ma], bz2[nb], gw2[ma][nb]);
+                gw3[ma] = mfma32(a2[ma], bo, gw3[ma]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // bias sums over the 32 row lanes of each half
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            for (int off = 16; off > 0; off >>= 1) gb2[mb][r] += __shfl_xor(gb2[mb][r], off, 64);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        for (int off = 16; off > 0; off >>= 1) gb3[j] += __shfl_xor(gb3[j], off, 64);
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    float* G = a.grad_part + (size_t)blockIdx.x * kFarParams;
+    // W1[k][m]: gw1[nb] has M = input k (rows c_row), N = hidden 32 nb + c
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int k = c_row(0, r, h);
+            if (k < 16) G[kFarOffW1 + k * kFarHid + 32 * nb + c] = gw1[nb][r];
+        }
+#pragma unroll
+    for (int ma = 0; ma < 2; ma++) {
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) G[kFarOffW2 + c_row(ma, r, h) * kFarHid + 32 * nb + c] = gw2[ma][nb][r];
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            if (c < 4) G[kFarOffW3 + c_row(ma, r, h) * 4 + c] = gw3[ma][r];
+    }
+    if (c == 0) {
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) G[kFarOffB2 + c_row(mb, r, h)] = gb2[mb][r];
+    }
+    if (l == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) G[kFarOffB3 + j] = gb3[j];
+        a.loss_part[blockIdx.x] = loss;
+    }
+}
+
+// Sum of the block partials (fixed order), then optionally one Adam step
+// with bias correction on the FP32 master parameters.
+__global__ void k_far_adam(FarTrainArgs a, int nblocks, int apply, float lr, int64_t t) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        float L = 0.f;
+        for (int b = 0; b < nblocks; b++) L += a.loss_part[b];
+        if (a.loss_out) a.loss_out[0] = L;
+    }
+    if (p >= kFarParams) return;
+    float g = 0.f;
+    for (int b = 0; b < nblocks; b++) g += a.grad_part[(size_t)b * kFarParams + p];
+    a.grad[p] = g;
+    if (!apply) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float m = b1 * a.adam_m[p] + (1.f - b1) * g;
+    const float v = b2 * a.adam_v[p] + (1.f - b2) * g * g;
+    a.adam_m[p] = m;
+    a.adam_v[p] = v;
+    const float mh = m / (1.f - powf(b1, (float)t)), vh = v / (1.f - powf(b2, (float)t));
+    a.params[p] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Mean |F|^2 over the live particles (FP64), the loss-weight normaliser.
+__global__ __launch_bounds__(1024) void k_far_f2(const float* __restrict__ fx, const float* __restrict__ fy,
+                                                 const Ctrl* __restrict__ ctrl, double* __restrict__ out) {
+    __shared__ double sh[1024];
+    const int n = ctrl->n_alive;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) acc += (double)fx[i] * fx[i] + (double)fy[i] * fy[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = n > 0 ? sh[0] / n : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+void launch_far_f2(hipStream_t s, const float* fx, const float* fy, const Ctrl* ctrl, double* out) {
+    hipLaunchKernelGGL(k_far_f2, dim3(1), dim3(1024), 0, s, fx, fy, ctrl, out);
+}
+
+size_t far_frags_bytes() { return sizeof(FarFrags); }
+
+void launch_far_pack(hipStream_t s, const float* params, const float* stats, void* frags) {
+    hipLaunchKernelGGL(k_far_pack, dim3(1), dim3(kWave), 0, s, params, stats, (FarFrags*)frags);
+}
+
+void launch_far_mlp(hipStream_t s, const FarArgs& a, int n_upper) {
+    const int groups = (n_upper + kWave - 1) / kWave;
+    if (groups > 0) hipLaunchKernelGGL(k_far_mlp, dim3(groups), dim3(kWave), 0, s, a);
+}
+
+void launch_far_samples(hipStream_t s, const FarSampleArgs& a) {
+    if (a.n_rows <= 0) return;
+    const int64_t blocks = (a.n_rows + 3) / 4;
+    hipLaunchKernelGGL(k_far_samples, dim3((unsigned)blocks), dim3(256), 0, s, a);
+}
+
+void launch_far_train(hipStream_t s, const FarTrainArgs& a, int nblocks, int apply, float lr, int64_t t) {
+    hipLaunchKernelGGL(k_far_train, dim3(nblocks), dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL(k_far_adam, dim3((kFarParams + 255) / 256), dim3(256), 0, s, a, nblocks, apply, lr, t);
+}
+
+}  // namespace geon

@@ -371,54 +371,77 @@ void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n_up
 // FP64 multi-column prefix sums over the sorted points (tree aggregates)
 // ---------------------------------------------------------------------------
 constexpr int kPreCols = 8;   // m, m*x, m*y, mu, L, m*wx, m*wy, charge quanta
+constexpr int kMomCols = 9;   // m X^2, m XY, m Y^2, q X, q Y, q X^2, q XY, q Y^2, |q|  (X = x - x0)
 
 struct PreIn {
+    static constexpr int kCols = kPreCols;
     const float *x, *y, *m, *wx, *wy, *mu, *L;
     const int32_t* qq;
+    __device__ __forceinline__ void item(int i, double* v) const {
+        const double mi = m[i];
+        v[0] = mi;
+        v[1] = mi * (double)x[i];
+        v[2] = mi * (double)y[i];
+        v[3] = mu[i];
+        v[4] = L[i];
+        v[5] = mi * (double)wx[i];
+        v[6] = mi * (double)wy[i];
+        v[7] = (double)qq[i];
+    }
 };
 
-__device__ __forceinline__ void pre_item(const PreIn& in, int i, double* v) {
-    const double m = in.m[i];
-    v[0] = m;
-    v[1] = m * (double)in.x[i];
-    v[2] = m * (double)in.y[i];
-    v[3] = in.mu[i];
-    v[4] = in.L[i];
-    v[5] = m * (double)in.wx[i];
-    v[6] = m * (double)in.wy[i];
-    v[7] = (double)in.qq[i];
-}
+// Second moments about a fixed origin (x0, y0) near the points, for the
+// far-field inputs of nodes too large for a direct sum.
+struct MomIn {
+    static constexpr int kCols = kMomCols;
+    const float *x, *y, *m;
+    const int32_t* qq;
+    double x0, y0;
+    __device__ __forceinline__ void item(int i, double* v) const {
+        const double X = (double)x[i] - x0, Y = (double)y[i] - y0;
+        const double mi = m[i], q = (double)qq[i] * kChargeQuantum;
+        v[0] = mi * X * X; v[1] = mi * X * Y; v[2] = mi * Y * Y;
+        v[3] = q * X; v[4] = q * Y;
+        v[5] = q * X * X; v[6] = q * X * Y; v[7] = q * Y * Y;
+        v[8] = fabs(q);
+    }
+};
 
-__global__ __launch_bounds__(kScanThreads) void k_pre_reduce(PreIn in, double* __restrict__ bsum, int nb,
+template <class IN>
+__global__ __launch_bounds__(kScanThreads) void k_pre_reduce(IN in, double* __restrict__ bsum, int nb,
                                                              const Ctrl* __restrict__ ctrl) {
-    __shared__ double sh[kPreCols][kScanThreads];
+    constexpr int NC = IN::kCols;
+    __shared__ double sh[NC][kScanThreads];
     const int n = ctrl->n_tree;
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    double acc[kPreCols] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
     for (int k = 0; k < kScanItems; k++) {
         const int i = base + k;
         if (i < n) {
-            double v[kPreCols];
-            pre_item(in, i, v);
+            double v[NC];
+            in.item(i, v);
 #pragma unroll
-            for (int c = 0; c < kPreCols; c++) acc[c] += v[c];
+            for (int c = 0; c < NC; c++) acc[c] += v[c];
         }
     }
 #pragma unroll
-    for (int c = 0; c < kPreCols; c++) sh[c][threadIdx.x] = acc[c];
+    for (int c = 0; c < NC; c++) sh[c][threadIdx.x] = acc[c];
     __syncthreads();
     for (int off = kScanThreads / 2; off > 0; off >>= 1) {
         if (threadIdx.x < off)
 #pragma unroll
-            for (int c = 0; c < kPreCols; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+            for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x < kPreCols) bsum[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < NC) bsum[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(kPreCols * 64) void k_pre_bsums(double* __restrict__ bsum, int nb) {
+__global__ __launch_bounds__(1024) void k_pre_bsums(double* __restrict__ bsum, int nb, int ncols) {
     // one wave per column: serial chunks + wave scan
     const int c = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (c >= ncols) return;
     double* b = bsum + (size_t)c * nb;
     const int chunk = (nb + 63) / 64;
     const int lo = lane * chunk, hi = min(nb, lo + chunk);
@@ -433,52 +456,64 @@ __global__ __launch_bounds__(kPreCols * 64) void k_pre_bsums(double* __restrict_
     for (int i = lo; i < hi; i++) { const double v = b[i]; b[i] = run; run += v; }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_pre_apply(PreIn in, const double* __restrict__ bsum, int nb,
+template <class IN>
+__global__ __launch_bounds__(kScanThreads) void k_pre_apply(IN in, const double* __restrict__ bsum, int nb,
                                                             double* __restrict__ pre, int64_t stride,
                                                             const Ctrl* __restrict__ ctrl) {
-    __shared__ double sh[kPreCols][kScanThreads];
+    constexpr int NC = IN::kCols;
+    __shared__ double sh[NC][kScanThreads];
     const int n = ctrl->n_tree;
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    double acc[kPreCols] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
     for (int k = 0; k < kScanItems; k++) {
         const int i = base + k;
         if (i < n) {
-            double v[kPreCols];
-            pre_item(in, i, v);
+            double v[NC];
+            in.item(i, v);
 #pragma unroll
-            for (int c = 0; c < kPreCols; c++) acc[c] += v[c];
+            for (int c = 0; c < NC; c++) acc[c] += v[c];
         }
     }
 #pragma unroll
-    for (int c = 0; c < kPreCols; c++) sh[c][threadIdx.x] = acc[c];
+    for (int c = 0; c < NC; c++) sh[c][threadIdx.x] = acc[c];
     __syncthreads();
     for (int off = 1; off < kScanThreads; off <<= 1) {
-        double a[kPreCols];
+        double a[NC];
 #pragma unroll
-        for (int c = 0; c < kPreCols; c++) a[c] = threadIdx.x >= (unsigned)off ? sh[c][threadIdx.x - off] : 0.0;
+        for (int c = 0; c < NC; c++) a[c] = threadIdx.x >= (unsigned)off ? sh[c][threadIdx.x - off] : 0.0;
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < kPreCols; c++) sh[c][threadIdx.x] += a[c];
+        for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += a[c];
         __syncthreads();
     }
-    double run[kPreCols];
+    double run[NC];
 #pragma unroll
-    for (int c = 0; c < kPreCols; c++) run[c] = sh[c][threadIdx.x] - acc[c] + bsum[(size_t)c * nb + blockIdx.x];
+    for (int c = 0; c < NC; c++) run[c] = sh[c][threadIdx.x] - acc[c] + bsum[(size_t)c * nb + blockIdx.x];
     if (blockIdx.x == 0 && threadIdx.x == 0)
 #pragma unroll
-        for (int c = 0; c < kPreCols; c++) pre[(size_t)c * stride] = 0.0;
+        for (int c = 0; c < NC; c++) pre[(size_t)c * stride] = 0.0;
     for (int k = 0; k < kScanItems; k++) {
         const int i = base + k;
         if (i < n) {
-            double v[kPreCols];
-            pre_item(in, i, v);
+            double v[NC];
+            in.item(i, v);
 #pragma unroll
-            for (int c = 0; c < kPreCols; c++) {
+            for (int c = 0; c < NC; c++) {
                 run[c] += v[c];
                 pre[(size_t)c * stride + i + 1] = run[c];
             }
         }
     }
+}
+
+template <class IN>
+static void prefix_columns(hipStream_t s, const IN& in, double* bsum, double* pre, int64_t stride, int nb,
+                           const Ctrl* ctrl) {
+    hipLaunchKernelGGL(k_pre_reduce<IN>, dim3(nb), dim3(kScanThreads), 0, s, in, bsum, nb, ctrl);
+    hipLaunchKernelGGL(k_pre_bsums, dim3(1), dim3(IN::kCols * 64), 0, s, bsum, nb, IN::kCols);
+    hipLaunchKernelGGL(k_pre_apply<IN>, dim3(nb), dim3(kScanThreads), 0, s, in, bsum, nb, pre, stride, ctrl);
 }
 
 // ---------------------------------------------------------------------------
@@ -543,6 +578,10 @@ struct NodeBuild {
     double *mass64, *comx64, *comy64, *qq64;
     uint64_t* path;
     int32_t* npts;
+    int32_t* first;             // first sorted point of each node
+    NodeMom* mom;               // null: moments not wanted
+    const double* pre_mom;      // kMomCols columns of prefix sums about (mom_in.x0, mom_in.y0)
+    MomIn mom_in;
     double root_size;           // 4W: full width of the root box
     double xmin, ymin;          // lower corner of the root box
     double aspect_x, aspect_y;  // box extent per unit of `size` along x and y (1 and H/W)
@@ -562,6 +601,44 @@ __device__ __forceinline__ int lower_bound_prefix(const uint64_t* keys, int lo, 
     return lo;
 }
 
+// Moments of the points [a, e) about their COM: the inputs of the far-field
+// correction.  Small nodes sum their points directly in FP64; large ones take
+// prefix-sum differences about a fixed origin, then shift to the COM
+// (parallel-axis terms).
+constexpr int kMomDirect = 64;
+
+__device__ NodeMom node_moments(const NodeBuild& b, int a, int e, double m, double comx, double comy, double qtot) {
+    double S[kMomCols];
+    double cX, cY;
+    if (e - a <= kMomDirect) {
+        MomIn in = b.mom_in;
+        in.x0 = comx; in.y0 = comy;
+        for (int c = 0; c < kMomCols; c++) S[c] = 0.0;
+        for (int j = a; j < e; j++) {
+            double v[kMomCols];
+            in.item(j, v);
+            for (int c = 0; c < kMomCols; c++) S[c] += v[c];
+        }
+        cX = 0.0; cY = 0.0;
+    } else {
+        const int64_t st = b.stride;
+        for (int c = 0; c < kMomCols; c++) S[c] = b.pre_mom[c * st + e] - b.pre_mom[c * st + a];
+        cX = comx - b.mom_in.x0; cY = comy - b.mom_in.y0;
+    }
+    NodeMom nm;
+    nm.qxx = (float)(S[0] - m * cX * cX);
+    nm.qxy = (float)(S[1] - m * cX * cY);
+    nm.qyy = (float)(S[2] - m * cY * cY);
+    nm.aq = (float)S[8];
+    nm.dx = (float)(S[3] - qtot * cX);
+    nm.dy = (float)(S[4] - qtot * cY);
+    nm.cxx = (float)(S[5] - 2.0 * cX * S[3] + qtot * cX * cX);
+    nm.cxy = (float)(S[6] - cX * S[4] - cY * S[3] + qtot * cX * cY);
+    nm.cyy = (float)(S[7] - 2.0 * cY * S[4] + qtot * cY * cY);
+    nm.pad0 = nm.pad1 = nm.pad2 = 0.f;
+    return nm;
+}
+
 __device__ void write_node(const NodeBuild& b, int slot, int level, uint64_t path, int a, int e, int ctrl_n_tree,
                            bool divided, int child_base) {
     double m, mx, my, mu, L, px, py, qq;
@@ -570,7 +647,7 @@ __device__ void write_node(const NodeBuild& b, int slot, int level, uint64_t pat
         m = mx = my = mu = L = px = py = qq = 0.0;
         for (int j = a; j < e; j++) {
             double v[kPreCols];
-            pre_item(b.pts, j, v);
+            b.pts.item(j, v);
             m += v[0]; mx += v[1]; my += v[2]; mu += v[3]; L += v[4]; px += v[5]; py += v[6]; qq += v[7];
         }
     } else {
@@ -613,6 +690,8 @@ __device__ void write_node(const NodeBuild& b, int slot, int level, uint64_t pat
     b.mass64[slot] = m; b.comx64[slot] = comx; b.comy64[slot] = comy; b.qq64[slot] = qq;
     b.path[slot] = path;
     b.npts[slot] = cnt;
+    b.first[slot] = a;
+    if (b.mom != nullptr && divided) b.mom[slot] = node_moments(b, a, e, m, comx, comy, qq * kChargeQuantum);
 }
 
 __device__ __forceinline__ int divided_rank(const NodeBuild& b, int a, int level) {
@@ -666,16 +745,17 @@ __global__ void k_tree_nodes(NodeBuild b, Ctrl* __restrict__ ctrl) {
 
 void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, const float* m, const float* wx,
                 const float* wy, const float* mu, const float* L, const int32_t* qq, int n_upper, KeyParams kp,
-                Ctrl* ctrl) {
+                Ctrl* ctrl, bool moments) {
     // n_tree (points inside the root box) from the sorted keys
     hipLaunchKernelGGL(k_count_tree, dim3(1), dim3(1), 0, s, tb.keys, ctrl);
     const int nb = (n_upper + kScanTile - 1) / kScanTile;
     PreIn in{x, y, m, wx, wy, mu, L, qq};
+    // origin of the moment prefix sums: the centre of the root box
+    MomIn mom_in{x, y, m, qq, kp.xmin + kp.root_hw, kp.ymin + kp.root_hw * (kp.scale_x / kp.scale_y)};
     const int64_t stride = tb.cap_particles + 1;
     if (nb > 0) {
-        hipLaunchKernelGGL(k_pre_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, tb.scan_tmp64, nb, ctrl);
-        hipLaunchKernelGGL(k_pre_bsums, dim3(1), dim3(kPreCols * 64), 0, s, tb.scan_tmp64, nb);
-        hipLaunchKernelGGL(k_pre_apply, dim3(nb), dim3(kScanThreads), 0, s, in, tb.scan_tmp64, nb, tb.pre, stride, ctrl);
+        prefix_columns(s, in, tb.scan_tmp64, tb.pre, stride, nb, ctrl);
+        if (moments) prefix_columns(s, mom_in, tb.scan_tmp64, tb.pre_mom, stride, nb, ctrl);
         hipLaunchKernelGGL(k_tree_levels, dim3((n_upper + 255) / 256), dim3(256), 0, s, tb.keys, tb.owned, ctrl);
         exclusive_scan_i32(s, tb.owned, tb.rank_start, n_upper, &ctrl->n_tree, &ctrl->n_divided, tb.scan_tmp);
         hipLaunchKernelGGL(k_tree_scatter, dim3((n_upper + 255) / 256), dim3(256), 0, s, tb.keys, tb.rank_start,
@@ -688,6 +768,10 @@ void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, con
     b.pre = tb.pre; b.stride = stride; b.pts = in;
     b.nodes = tb.nodes; b.mass64 = tb.node_mass64; b.comx64 = tb.node_comx64; b.comy64 = tb.node_comy64;
     b.qq64 = tb.node_qq64; b.path = tb.node_path; b.npts = tb.node_npts;
+    b.first = tb.node_first;
+    b.mom = moments ? tb.node_mom : nullptr;
+    b.pre_mom = tb.pre_mom;
+    b.mom_in = mom_in;
     b.root_size = 2.0 * kp.root_hw;
     b.xmin = kp.xmin;
     b.ymin = kp.ymin;

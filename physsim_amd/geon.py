@@ -231,6 +231,52 @@ class HipPhysics:
                                                   C.byref(n)))
         return a[:n.value], b[:n.value]
 
+    # ---- learned far-field correction ------------------------------------
+    def far_collect(self, group_stride=1):
+        """Append training samples from the current state; returns the count held."""
+        n = C.c_int64()
+        self._check(self.lib.geon_far_collect(self.h, int(group_stride), C.byref(n)))
+        self._far_n = n.value
+        return n.value
+
+    def far_clear(self):
+        self._check(self.lib.geon_far_clear(self.h))
+        self._far_n = 0
+
+    def far_samples(self, first=0, n=None, observers=False):
+        """Samples (n x 16 float32: 10 features, 4 targets, 2 weights) and
+        optionally the observer slot of each."""
+        if n is None:
+            n = getattr(self, "_far_n", 0) - first
+        out = np.zeros((max(n, 1), A.FAR_SAMPLE), dtype=np.float32)
+        obs = np.zeros(max(n, 1), dtype=np.int32)
+        self._check(self.lib.geon_far_get_samples(self.h, int(first), int(n), out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                  obs.ctypes.data_as(C.POINTER(C.c_int32))))
+        return (out[:n], obs[:n]) if observers else out[:n]
+
+    def far_train(self, steps, batch=8192, lr=2e-3, lr_final=2e-5, seed=1):
+        loss = np.zeros(max(int(steps), 1), dtype=np.float32)
+        self._check(self.lib.geon_far_train(self.h, int(steps), int(batch), float(lr), float(lr_final), int(seed),
+                                            loss.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss[:steps]
+
+    def far_grad(self, rows):
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        g = np.zeros(A.FAR_PARAMS, dtype=np.float32)
+        L = C.c_float()
+        self._check(self.lib.geon_far_grad(self.h, r.ctypes.data_as(C.POINTER(C.c_int64)), len(r),
+                                           g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(L)))
+        return L.value, g
+
+    def far_get_model(self):
+        m = np.zeros(A.FAR_MODEL_FLOATS, dtype=np.float32)
+        self._check(self.lib.geon_far_get_model(self.h, m.ctypes.data_as(C.POINTER(C.c_float)), len(m)))
+        return m
+
+    def far_set_model(self, model):
+        m = np.ascontiguousarray(model, dtype=np.float32)
+        self._check(self.lib.geon_far_set_model(self.h, m.ctypes.data_as(C.POINTER(C.c_float)), len(m)))
+
     def counters(self):
         c = A.GeonCounters()
         self._check(self.lib.geon_counters(self.h, C.byref(c)))

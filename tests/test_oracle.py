@@ -410,3 +410,109 @@ def test_eih_two_body_at_rest(oracle_lib):
     eih = m1 * m2 * d / r ** 3 * (5 * m1 + 4 * m2) / r
     assert f["fx"][0] == pytest.approx(newton + eih, rel=1e-13)
     assert f["fx"][1] == pytest.approx(-(newton + m1 * m2 * d / r ** 3 * (5 * m2 + 4 * m1) / r), rel=1e-13)
+
+
+def _far_system(oracle_lib, n=3000, theta=0.5, seed=0):
+    st, prm, _ = W.config2(n, replicate=seed, open_box=True)
+    prm = dict(prm, theta=theta, toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), n)
+    o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    return o, st
+
+
+def test_far_samples_reconstruct_direct_sum(oracle_lib):
+    # BH force + the sum of the exact residuals of every accepted node = FP64 direct sum
+    o, st = _far_system(oracle_lib)
+    idx = np.arange(0, 3000, 37, dtype=np.int32)
+    S = o.far_samples(idx)
+    o.compute_forces()
+    f = o.forces()
+    fx, fy = o.direct_forces(idx)
+    pos = {int(p): k for k, p in enumerate(idx)}
+    k = np.array([pos[int(v)] for v in S[:, 0]])
+    m = st["mass"][idx][k]
+    q = np.round(st["charge"][idx] * 10)[k] / 10
+    cx = np.bincount(k, m * S[:, 16] + q * S[:, 18], len(idx))
+    cy = np.bincount(k, m * S[:, 17] + q * S[:, 19], len(idx))
+    plain = np.hypot(f["fx"][idx] - fx, f["fy"][idx] - fy)
+    fixed = np.hypot(f["fx"][idx] + cx - fx, f["fy"][idx] + cy - fy)
+    assert plain.max() > 1e-4 * np.abs(fx).max()
+    assert fixed.max() < 1e-11 * np.hypot(fx, fy).max()
+
+
+def _random_model(rng):
+    model = np.zeros(A.FAR_MODEL_FLOATS, dtype=np.float32)
+    model[:A.FAR_PARAMS] = rng.normal(0, 0.3, A.FAR_PARAMS)
+    model[A.FAR_PARAMS:A.FAR_PARAMS + 10] = rng.normal(0, 0.1, 10)          # means
+    model[A.FAR_PARAMS + 10:A.FAR_PARAMS + 20] = rng.uniform(0.5, 2.0, 10)  # inverse deviations
+    model[A.FAR_PARAMS + 20:] = rng.uniform(0.5, 2.0, 4)                    # output scales
+    return model
+
+
+def test_far_loss_grad_matches_finite_differences(oracle_lib):
+    rng = np.random.default_rng(11)
+    o, _ = _far_system(oracle_lib, n=2000)
+    S, _ = o.far_training_samples(np.arange(0, 2000, 50, dtype=np.int32))
+    S = S[:300]
+    model = _random_model(rng)
+    L, g = oracle_lib.far_loss_grad(model, S, wscale=3.0)
+    assert L > 0
+    # parameters of every block: W1 (incl. the bias row 10), W2, b2, W3, b3
+    picks = np.concatenate([rng.integers(0, 11 * 64, 8), rng.integers(1024, 5120, 8), rng.integers(5120, 5184, 4),
+                            rng.integers(5184, 5440, 6), np.arange(5440, 5444)])
+    for p in picks:
+        h = 1e-3 * max(1.0, abs(float(model[p])))
+        mp = model.astype(np.float64).copy()
+        # central difference in FP64 through float32 storage: perturb by an exactly representable step
+        step = np.float32(h)
+        m1 = model.copy(); m1[p] = np.float32(mp[p] + step)
+        m0 = model.copy(); m0[p] = np.float32(mp[p] - step)
+        d = (float(m1[p]) - float(m0[p]))
+        fd = (oracle_lib.far_loss_grad(m1, S, 3.0)[0] - oracle_lib.far_loss_grad(m0, S, 3.0)[0]) / d
+        assert fd == pytest.approx(g[p], rel=2e-5, abs=1e-9 * abs(L) + 1e-14), p
+
+
+def test_far_learned_correction_wiring(oracle_lib):
+    # The learned force minus the plain force equals, per observer, the sum over
+    # its accepted nodes of the network output rotated out of the offset frame
+    # and scaled by m_i M (s/r)^2/r^2 and q_i A (s/r)/r^2 (network evaluated here in numpy).
+    rng = np.random.default_rng(5)
+    o, st = _far_system(oracle_lib, n=2000)
+    model = _random_model(rng)
+    idx = np.arange(0, 2000, 41, dtype=np.int32)
+    raw = o.far_samples(idx)
+    S, obs = o.far_training_samples(idx)
+    P = model.astype(np.float64)
+    x = np.zeros((len(S), 16))
+    x[:, :10] = (S[:, :10] - P[5444:5454]) * P[5454:5464]
+    x[:, 10] = 1
+    gl = lambda z: 0.5 * z * (1 + np.tanh(np.sqrt(2 / np.pi) * (z + 0.044715 * z ** 3)))
+    a1 = gl(x @ P[:1024].reshape(16, 64))
+    a2 = gl(a1 @ P[1024:5120].reshape(64, 64) + P[5120:5184])
+    out = (a2 @ P[5184:5440].reshape(64, 4) + P[5440:5444]) * P[5464:5468]
+    dx, dy, r2, size, M = raw[:, 1], raw[:, 2], raw[:, 3], raw[:, 4], raw[:, 5]
+    rho = np.hypot(dx, dy)
+    ux, uy = dx / rho, dy / rho
+    sr = size / np.sqrt(r2)
+    m_i = st["mass"][obs]
+    q_i = np.round(st["charge"][obs] * 10) / 10
+    sg = m_i * M * sr ** 2 / r2
+    sc = q_i * raw[:, 15] * sr / r2
+    cx = sg * (ux * out[:, 0] - uy * out[:, 1]) + sc * (ux * out[:, 2] - uy * out[:, 3])
+    cy = sg * (uy * out[:, 0] + ux * out[:, 1]) + sc * (uy * out[:, 2] + ux * out[:, 3])
+    pos = {int(p): k for k, p in enumerate(idx)}
+    k = np.array([pos[int(v)] for v in obs])
+    o.compute_forces()
+    f0 = o.forces()
+    p = A.GeonParams.from_buffer_copy(o.params)
+    p.far_field = A.FAR_LEARNED
+    o.set_params(p)
+    o.far_set_model(model)
+    o.compute_forces()
+    f1 = o.forces()
+    # the features reach numpy through float32 samples: agreement to float32 rounding
+    sx, sy = np.bincount(k, cx, len(idx)), np.bincount(k, cy, len(idx))
+    tol = 2e-6 * np.abs(np.concatenate([sx, sy])).max()
+    np.testing.assert_allclose(f1["fx"][idx] - f0["fx"][idx], sx, rtol=0, atol=tol)
+    np.testing.assert_allclose(f1["fy"][idx] - f0["fy"][idx], sy, rtol=0, atol=tol)
+    assert np.abs(sx).max() > 1e-3 * np.abs(f0["fx"][idx]).max()
