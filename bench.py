@@ -1,19 +1,31 @@
-"""Headline benchmark: particle-steps/sec of the Barnes-Hut substep loop at N = 1,048,576.
+"""Headline benchmark: particle-steps/sec at N = 1,048,576, plain vs
+learned-correction Barnes-Hut at force error <= 1e-4.
 
-Workload (default): 2^20 particles with the relativistic dynamics of the
-N=65536 configuration scaled to keep its density (1024 x 1024 box): gravity +
-Coulomb + Yukawa (mu = 1) + magnetic dipole, softening 8, spins uniform in
-[-1, 1], plain monopole BH at theta = 0.3.  The box is open (despawn) by
-default, where BH converges to the FP64 direct sum the force error is judged
-against; --boundary torus runs the min-image torus of the reference instead.  One step = one
-Physics.update(dt = 1/128) (adaptive substeps); particle-steps = sum over
+Workload: 2^20 particles with the relativistic dynamics of the N=65536
+configuration scaled to keep its density (1024 x 1024 open box): gravity +
+Coulomb + Yukawa (mu = 1) + magnetic dipole, softening 8 (Barnes-Hut 4),
+spins uniform in [-1, 1].  The box is open (despawn), where Barnes-Hut
+converges to the FP64 direct sum the force error is judged against.  One step
+= one Physics.update(dt = 1/128) (adaptive substeps); particle-steps = sum over
 substeps of live particles.  Inputs are resident in HBM before timing.
 
-Also reported: the force walk's roofline (FP32 vector FLOP/s from the kernel's
-own interaction counters x flops per interaction, over its HIP-event duration),
-the global relative L2 force error of the GPU forces against an FP64 direct sum
-on a subsample, and the reference CPU algorithm (the FP64 oracle) timed on this
-host on a bounded sample.
+Three force methods run the same updates from the same initial state:
+  plain      monopole nodes (the reference's walk) at --theta-plain, the
+             coarsest opening angle whose error is <= 1e-4;
+  learned    the same walk at --theta-learned plus the 2 x 64 GELU network's
+             residual for every accepted node; the network is trained here,
+             on the GPU, on exact residuals of 16 Plummer spheres of 65536
+             (untimed setup; time and sample count reported);
+  multipole  the same walk at --theta-multipole plus the analytic
+             quadrupole / dipole terms (reported beside the two).
+The global relative L2 force error of each against the FP64 direct sum on a
+4096-particle subsample is measured in the same run; `value` is the learned
+method's throughput.
+
+Also reported: the dominant kernel's roofline (HIP events around its
+launches; algorithmic flops from the kernel's own counters), the walk's, and
+the reference algorithm (the FP64 oracle) timed on this host on a bounded
+sample.
 """
 import argparse
 import json
@@ -27,6 +39,10 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+# network flops per (particle, node) pair: 2 (11 x 64 + 64 x 64 + 64 x 4)
+MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 4)
+TIMER_WALK, TIMER_TREE, TIMER_SORT, TIMER_KICK, TIMER_MLP = 0, 1, 2, 3, 4
 
 
 def flops_per_interaction(toggles):
@@ -49,121 +65,191 @@ def flops_per_interaction(toggles):
     return c
 
 
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def train_far_model(theta, seeds, n, stride, steps, batch):
+    """The learned correction's network, trained on the GPU from the exact
+    residuals of `seeds` Plummer spheres of n particles."""
+    from physsim_amd import HipPhysics, default_params, workloads as W
+    _, tprm = W.plummer_charged(n, 0)
+    T = HipPhysics(default_params(**dict(tprm, theta=theta, far_list_cap=1 << 18)), max_particles=n)
+    t0 = time.perf_counter()
+    ns = 0
+    for seed in range(seeds):
+        st, _ = W.plummer_charged(n, seed)
+        T.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+        ns = T.far_collect(stride)
+    t1 = time.perf_counter()
+    loss = T.far_train(steps, batch=batch, lr=2e-3, lr_final=1e-5, seed=3)
+    t2 = time.perf_counter()
+    model = T.far_get_model()
+    T.destroy()
+    info = {"systems": "%d Plummer spheres (scale radius 20) of N=%d, gravity + Coulomb" % (seeds, n),
+            "theta": theta, "samples": int(ns), "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+            "steps": steps, "batch": batch, "loss_first": float(loss[:50].mean()), "loss_last": float(loss[-50:].mean())}
+    return model, info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--theta", type=float, default=0.3)
+    ap.add_argument("--theta-plain", type=float, default=0.04)
+    ap.add_argument("--theta-learned", type=float, default=0.25)
+    ap.add_argument("--theta-multipole", type=float, default=0.3)
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--train-seeds", type=int, default=16)
+    ap.add_argument("--err-sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
-    ap.add_argument("--err-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--boundary", choices=("open", "torus"), default="open")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the Barnes-Hut path here is single-GPU "
               "(see DESIGN.md for how it would shard); got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    from physsim_amd import HipPhysics, default_params, workloads as W
-    st, prm, dt = W.config2(args.n, open_box=(args.boundary == "open"))
-    prm["theta"] = args.theta
+    from physsim_amd import HipPhysics, default_params, workloads as W, _abi as A
+    from oracle import bind
+    threads = min(16, os.cpu_count() or 1)
+    bind.set_threads(threads)
     n = args.n
-    g = HipPhysics(default_params(**prm), max_particles=n)
-    g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
-    print("bench: n=%d theta=%.2f warmup=%d steps=%d" % (n, args.theta, args.warmup, args.steps), file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        g.update(dt)
-    g.timing(True)
-    alive0 = g.getParticleCount()
-    t0 = time.perf_counter()
-    substeps = 0
-    for k in range(args.steps):
-        s = g.update(dt)
-        substeps += s
-        print("bench: step %d substeps %d" % (k, s), file=sys.stderr, flush=True)
-    g.getParticleCount()       # synchronizes the stream
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    alive1 = g.getParticleCount()
-    psteps = substeps * alive1  # torus: no particle leaves
-    value = psteps / wall
-    walk_ms, walk_n = g.kernel_timing(0)
-    tree_ms, tree_n = g.kernel_timing(1)
-    sort_ms, sort_n = g.kernel_timing(2)
-    kick_ms, kick_n = g.kernel_timing(3)
-    g.timing(False)
-    cnt = g.counters()
-    inter = cnt["pair_interactions"] + cnt["node_interactions"]
-    c_f = flops_per_interaction(prm["toggles0"])
-    walk_avg_s = walk_ms / max(walk_n, 1) / 1e3
-    achieved_tflops = inter * c_f / walk_avg_s / 1e12 if walk_avg_s > 0 else 0.0
+    st, prm, dt = W.config2(n, open_box=True)
+    g = HipPhysics(default_params(**dict(prm, far_list_cap=1 << 17)), max_particles=n)
 
-    # ---- force error of the GPU forces vs the FP64 direct sum (checker) ----
-    result_err = None
+    def upload():
+        g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+
+    def set_method(theta, far):
+        p = A.GeonParams.from_buffer_copy(g.params)
+        p.theta, p.far_field = theta, far
+        g.set_params(p)
+
+    log("n=%d: training the far-field network" % n)
+    model, train_info = train_far_model(args.theta_learned, args.train_seeds, 65536, 8, args.train_steps, 16384)
+    g.far_set_model(model)
+    upload()
+
+    # ---- force error of each method vs the FP64 direct sum (the checker) ----
+    rs = np.sort(np.random.Generator(np.random.Philox(5000)).choice(n, args.err_sample, replace=False)).astype(np.int32)
+    orc = bind.Oracle(bind.default_params(**prm), n)
+    orc.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    te = time.perf_counter()
+    fx, fy = orc.direct_forces(rs)
+    te = time.perf_counter() - te
+    del orc
+    methods = {"plain": (args.theta_plain, A.FAR_NONE), "learned": (args.theta_learned, A.FAR_LEARNED),
+               "multipole": (args.theta_multipole, A.FAR_MULTIPOLE)}
+    res = {}
+    for name, (theta, far) in methods.items():
+        set_method(theta, far)
+        g.compute_forces()
+        f = g.forces(n)
+        dx, dy = f["fx"][rs] - fx, f["fy"][rs] - fy
+        rel = np.sqrt((dx ** 2 + dy ** 2) / np.maximum(fx ** 2 + fy ** 2, 1e-300))
+        res[name] = {"theta": theta, "force_rel_l2": float(np.sqrt(np.sum(dx ** 2 + dy ** 2) / np.sum(fx ** 2 + fy ** 2))),
+                     "force_median": float(np.median(rel)), "force_p99": float(np.percentile(rel, 99))}
+        log("%s theta %.3f: force error %.3e" % (name, theta, res[name]["force_rel_l2"]))
+
+    # ---- timed updates, each method from the same initial state ----
+    c_f = flops_per_interaction(prm["toggles0"])
+    for name, (theta, far) in methods.items():
+        upload()
+        set_method(theta, far)
+        for _ in range(args.warmup):
+            g.update(dt)
+        upload()                              # timed steps start from the initial state again
+        g.compute_forces()
+        g.timing(True)
+        t0 = time.perf_counter()
+        substeps = 0
+        for k in range(args.steps):
+            s = g.update(dt)
+            substeps += s
+            log("%s step %d substeps %d" % (name, k, s))
+        g.getParticleCount()                  # synchronizes the stream
+        wall = time.perf_counter() - t0
+        alive = g.getParticleCount()
+        ph = {}
+        for key, tid in (("walk", TIMER_WALK), ("mlp", TIMER_MLP), ("tree", TIMER_TREE), ("sort_gather", TIMER_SORT),
+                         ("kick_drift", TIMER_KICK)):
+            ms, cnt = g.kernel_timing(tid)
+            ph[key] = ms / cnt if cnt else 0.0
+        g.timing(False)
+        cnt = g.counters()                    # counters of the last force pass
+        r = res[name]
+        r.update({"particle_steps_per_s": substeps * alive / wall, "ms_per_step": wall / args.steps * 1e3,
+                  "substeps": substeps, "phases_ms_avg": ph,
+                  "pair_interactions_per_particle": cnt["pair_interactions"] / alive,
+                  "node_interactions_per_particle": cnt["node_interactions"] / alive})
+        walk_s = ph["walk"] * 1e-3
+        inter = cnt["pair_interactions"] + cnt["node_interactions"]
+        r["walk_tflops"] = inter * c_f / walk_s / 1e12 if walk_s > 0 else 0.0
+        if name == "learned" and ph["mlp"] > 0:
+            r["mlp_tflops"] = cnt["node_interactions"] * MLP_FLOPS_PER_PAIR / (ph["mlp"] * 1e-3) / 1e12
+        log("%s: %.3e particle-steps/s" % (name, r["particle_steps_per_s"]))
+
+    L, P = res["learned"], res["plain"]
+    dominant_mlp = L["phases_ms_avg"]["mlp"] >= L["phases_ms_avg"]["walk"]
+    if dominant_mlp:
+        roof = {"bound": "mfma", "kernel": "k_far_mlp", "achieved": L["mlp_tflops"], "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": L["mlp_tflops"] / BF16_PEAK_TFLOPS, "traffic": None,
+                "pairs_per_launch": L["node_interactions_per_particle"] * n, "flops_per_pair": MLP_FLOPS_PER_PAIR,
+                "ms_avg": L["phases_ms_avg"]["mlp"]}
+    else:
+        roof = {"bound": "valu", "kernel": "k_walk", "achieved": L["walk_tflops"], "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": L["walk_tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
+                "ms_avg": L["phases_ms_avg"]["walk"]}
+    walk_roof = {"bound": "valu", "kernel": "k_walk (learned run)", "achieved": L["walk_tflops"],
+                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": L["walk_tflops"] / FP32_PEAK_TFLOPS,
+                 "flops_per_interaction": c_f, "ms_avg": L["phases_ms_avg"]["walk"]}
+
     cpu = None
     if not args.no_cpu:
-        from oracle import bind
-        threads = min(16, os.cpu_count() or 1)
-        bind.set_threads(threads)
-        d = g.download()
-        g.compute_forces()
-        fg = g.forces(n)
-        orc = bind.Oracle(bind.default_params(**prm), n)
-        orc.upload(d["x"], d["y"], d["wx"], d["wy"], d["mass"], d["charge"], d["angw"])
-        rs = np.random.Generator(np.random.Philox(5000)).choice(n, args.err_sample, replace=False)
-        rs = np.sort(rs).astype(np.int32)
-        te = time.perf_counter()
-        fx, fy = orc.direct_forces(rs)
-        te = time.perf_counter() - te
-        num = np.sum((fg["fx"][rs] - fx) ** 2 + (fg["fy"][rs] - fy) ** 2)
-        den = np.sum(fx ** 2 + fy ** 2)
-        rel = np.sqrt(((fg["fx"][rs] - fx) ** 2 + (fg["fy"][rs] - fy) ** 2) / np.maximum(fx ** 2 + fy ** 2, 1e-300))
-        result_err = {"rel_l2": float(np.sqrt(num / den)), "median": float(np.median(rel)),
-                      "p99": float(np.percentile(rel, 99)), "sample": int(args.err_sample),
-                      "reference": "FP64 direct sum (forces.js:93-126, min-image)", "seconds": round(te, 2)}
-        del orc
         # ---- CPU baseline: the reference algorithm in FP64 on this host ----
-        cs, cprm, _ = W.config2(args.cpu_sample, open_box=(args.boundary == "open"))
-        cprm["theta"] = args.theta
+        cs, cprm, _ = W.config2(args.cpu_sample, open_box=True)
+        cprm["theta"] = args.theta_plain
         oc = bind.Oracle(bind.default_params(**cprm), args.cpu_sample)
         oc.upload(cs["x"], cs["y"], cs["wx"], cs["wy"], cs["mass"], cs["charge"], cs["angw"])
         tc = time.perf_counter()
-        oc.compute_forces()                     # bootstrap force pass (part of the first update)
+        oc.compute_forces()                    # bootstrap force pass (part of the first update)
         tb = time.perf_counter()
         steps_c = oc.update(dt)
         tc2 = time.perf_counter()
         cpu = {"value": steps_c * oc.count() / (tc2 - tb), "unit": "particle-steps/s", "cores": threads,
                "kind": "port",
-               "sample": "FP64 oracle (integrator.js/forces.js/quadtree.js restated in C, OpenMP over "
-                         "particles) on the same dynamics at N=%d (same density), one update of %d substeps, "
-                         "%.1f s; bootstrap force pass %.1f s untimed" % (args.cpu_sample, steps_c, tc2 - tb, tb - tc)}
+               "sample": "FP64 oracle (integrator.js/forces.js/quadtree.js restated in C, OpenMP over particles), "
+                         "plain monopole BH at theta %.2f, same dynamics at N=%d (same density), one update of %d "
+                         "substeps, %.1f s; bootstrap force pass %.1f s untimed"
+                         % (args.theta_plain, args.cpu_sample, steps_c, tc2 - tb, tb - tc)}
 
     line = {
-        "metric": "particle-steps/sec at N=1M, plain BH (monopole, theta=%.2f)" % args.theta,
-        "value": value,
+        "metric": "particle-steps/sec at N=1M, plain vs learned-correction BH at force err <=1e-4",
+        "value": L["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": L["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (Philox seed 2000: uniform torus box, lognormal masses, 0.1-quantized charges)",
-        "config": {"workload": "%s 1024x1024 box, N=%d uniform, gravity+Coulomb+Yukawa(mu=1)+magnetic dipole, "
-                               "relativistic, spins U[-1,1], softening 8, theta %.2f, dt 1/128 per update"
-                               % ("open (despawn)" if args.boundary == "open" else "torus min-image", n, args.theta),
-                   "n": n, "theta": args.theta, "substeps": substeps},
-        "roofline": {"bound": "valu", "kernel": "k_walk", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / FP32_PEAK_TFLOPS, "traffic": None,
-                     "interactions_per_launch": inter, "flops_per_interaction": c_f,
-                     "walk_ms_avg": walk_avg_s * 1e3},
-        "phases_ms_avg": {"walk": walk_ms / max(walk_n, 1), "tree": tree_ms / max(tree_n, 1),
-                          "sort_gather": sort_ms / max(sort_n, 1), "kick_drift": kick_ms / max(kick_n, 1)},
-        "force_error": result_err,
+        "dtype": "f32 (forces, Kahan), bf16 MFMA with f32 accumulation (network)",
+        "data": "synthetic (Philox seed 2000: uniform open box, lognormal masses, 0.1-quantized charges)",
+        "config": {"workload": "open 1024x1024 box, N=%d uniform, gravity+Coulomb+Yukawa(mu=1)+magnetic dipole, "
+                               "relativistic, spins U[-1,1], softening 8 (BH 4), dt 1/128 per update; learned BH at "
+                               "theta %.2f vs plain BH at theta %.2f" % (n, args.theta_learned, args.theta_plain),
+                   "n": n},
+        "speedup_learned_vs_plain": L["particle_steps_per_s"] / P["particle_steps_per_s"],
+        "methods": res,
+        "force_error_reference": {"kind": "FP64 direct sum (forces.js:93-126)", "sample": int(args.err_sample),
+                                  "seconds": round(te, 2)},
+        "training": train_info,
+        "roofline": roof,
+        "roofline_walk": walk_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -47,8 +47,10 @@ constexpr float kGeluC0 = 0.7978845608028654f;
 constexpr float kGeluK0 = -2.f * kGeluC0 * 1.4426950408889634f;   // exp(-2u) = exp2(z (k0 + k1 z^2))
 constexpr float kGeluK1 = kGeluK0 * 0.044715f;
 
+// v_exp_f32 / v_rcp_f32 directly: exp2 of a large argument gives +inf and
+// rcp(inf) = 0, so no range reduction is needed for sigma in [0, 1].
 __device__ __forceinline__ float gelu_sig(float z) {
-    const float e = exp2f(z * (kGeluK0 + kGeluK1 * z * z));
+    const float e = __builtin_amdgcn_exp2f(z * (kGeluK0 + kGeluK1 * z * z));
     return __builtin_amdgcn_rcpf(1.f + e);
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
@@ -72,19 +74,25 @@ struct PairFeat {
     float ux, uy, sg0, sc0;     // direction and the output scales without m_i, q_i
 };
 
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rsq_t(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq_t(double x) { return 1.0 / sqrt(x); }
+
 template <typename T>
 __device__ __forceinline__ void pair_features(T dx, T dy, T eps2, T s, T M, const NodeMom& c, T* f, T& ux, T& uy,
                                               T& sg0, T& sc0) {
     const T rho2 = dx * dx + dy * dy;
     const T r2 = rho2 + eps2;
-    const T ir2 = T(1) / r2;
-    const T r = sqrt(r2);
-    const T irho = T(1) / sqrt(rho2);
+    const T ir = rsq_t(r2);
+    const T ir2 = ir * ir;
+    const T irho = rsq_t(rho2);
     ux = dx * irho; uy = dy * irho;
-    const T sr = s / r;
-    const T iM = M > T(0) ? T(1) / (M * s * s) : T(0);
+    const T sr = s * ir;
+    const T is = rcp_t(s);
+    const T iM = M > T(0) ? rcp_t(M) * is * is : T(0);
     const T A = (T)c.aq;
-    const T iA = A > T(0) ? T(1) / (A * s) : T(0);
+    const T iA = A > T(0) ? rcp_t(A) * is : T(0);
     const T uu = ux * ux, vv = uy * uy, uv = ux * uy;
     f[0] = sr;
     f[1] = eps2 * ir2;
@@ -93,9 +101,9 @@ __device__ __forceinline__ void pair_features(T dx, T dy, T eps2, T s, T M, cons
     f[4] = (vv * c.qxx - T(2) * uv * c.qxy + uu * c.qyy) * iM;
     f[5] = (ux * c.dx + uy * c.dy) * iA;
     f[6] = (-uy * c.dx + ux * c.dy) * iA;
-    f[7] = (uu * c.cxx + T(2) * uv * c.cxy + vv * c.cyy) * iA / s;
-    f[8] = (-uv * c.cxx + (uu - vv) * c.cxy + uv * c.cyy) * iA / s;
-    f[9] = (vv * c.cxx - T(2) * uv * c.cxy + uu * c.cyy) * iA / s;
+    f[7] = (uu * c.cxx + T(2) * uv * c.cxy + vv * c.cyy) * iA * is;
+    f[8] = (-uv * c.cxx + (uu - vv) * c.cxy + uv * c.cyy) * iA * is;
+    f[9] = (vv * c.cxx - T(2) * uv * c.cxy + uu * c.cyy) * iA * is;
     sg0 = M * sr * sr * ir2;
     sc0 = A * sr * ir2;
 }
