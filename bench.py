@@ -42,6 +42,9 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 # network flops per (particle, node) pair: 2 (11 x 64 + 64 x 64 + 64 x 4)
 MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 4)
+# HBM bytes of one k_far_mlp launch at the default workload (N=2^20, theta 0.25), from the
+# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (profiles/k_far_mlp_open1M_theta0.25.md)
+MLP_TRAFFIC_DEFAULT = 4015.4e6
 TIMER_WALK, TIMER_TREE, TIMER_SORT, TIMER_KICK, TIMER_MLP = 0, 1, 2, 3, 4
 
 
@@ -98,14 +101,20 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--theta-plain", type=float, default=0.04)
-    ap.add_argument("--theta-learned", type=float, default=0.25)
-    ap.add_argument("--theta-multipole", type=float, default=0.3)
+    # opening-angle ladders, coarsest first: each method runs at the coarsest
+    # theta whose force error (measured below) is <= --err-target
+    ap.add_argument("--theta-plain", default="0.05,0.045,0.04,0.035,0.03")
+    ap.add_argument("--theta-learned", default="0.3,0.28,0.26,0.24,0.22")
+    ap.add_argument("--theta-multipole", default="0.35,0.32,0.3,0.28,0.25")
+    ap.add_argument("--theta-train", type=float, default=0.3)
+    ap.add_argument("--err-target", type=float, default=1e-4)
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--err-sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--methods", default="plain,learned,multipole",
+                    help="comma list of force methods to run (the learned one is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the Barnes-Hut path here is single-GPU "
@@ -129,7 +138,7 @@ def main():
         g.set_params(p)
 
     log("n=%d: training the far-field network" % n)
-    model, train_info = train_far_model(args.theta_learned, args.train_seeds, 65536, 8, args.train_steps, 16384)
+    model, train_info = train_far_model(args.theta_train, args.train_seeds, 65536, 8, args.train_steps, 16384)
     g.far_set_model(model)
     upload()
 
@@ -141,18 +150,31 @@ def main():
     fx, fy = orc.direct_forces(rs)
     te = time.perf_counter() - te
     del orc
-    methods = {"plain": (args.theta_plain, A.FAR_NONE), "learned": (args.theta_learned, A.FAR_LEARNED),
-               "multipole": (args.theta_multipole, A.FAR_MULTIPOLE)}
-    res = {}
-    for name, (theta, far) in methods.items():
-        set_method(theta, far)
-        g.compute_forces()
-        f = g.forces(n)
-        dx, dy = f["fx"][rs] - fx, f["fy"][rs] - fy
-        rel = np.sqrt((dx ** 2 + dy ** 2) / np.maximum(fx ** 2 + fy ** 2, 1e-300))
-        res[name] = {"theta": theta, "force_rel_l2": float(np.sqrt(np.sum(dx ** 2 + dy ** 2) / np.sum(fx ** 2 + fy ** 2))),
-                     "force_median": float(np.median(rel)), "force_p99": float(np.percentile(rel, 99))}
-        log("%s theta %.3f: force error %.3e" % (name, theta, res[name]["force_rel_l2"]))
+    ladders = {"plain": ([float(t) for t in args.theta_plain.split(",")], A.FAR_NONE),
+               "learned": ([float(t) for t in args.theta_learned.split(",")], A.FAR_LEARNED),
+               "multipole": ([float(t) for t in args.theta_multipole.split(",")], A.FAR_MULTIPOLE)}
+    keep = set(args.methods.split(",")) | {"learned"}
+    res, methods = {}, {}
+    for name, (ladder, far) in ladders.items():
+        if name not in keep:
+            continue
+        tried = []
+        for theta in ladder:
+            set_method(theta, far)
+            g.compute_forces()
+            f = g.forces(n)
+            dx, dy = f["fx"][rs] - fx, f["fy"][rs] - fy
+            rel = np.sqrt((dx ** 2 + dy ** 2) / np.maximum(fx ** 2 + fy ** 2, 1e-300))
+            r = {"theta": theta, "force_rel_l2": float(np.sqrt(np.sum(dx ** 2 + dy ** 2) / np.sum(fx ** 2 + fy ** 2))),
+                 "force_median": float(np.median(rel)), "force_p99": float(np.percentile(rel, 99))}
+            tried.append((theta, r["force_rel_l2"]))
+            log("%s theta %.3f: force error %.3e" % (name, theta, r["force_rel_l2"]))
+            if r["force_rel_l2"] <= args.err_target or theta == ladder[-1]:
+                break
+        r["ladder"] = tried
+        r["meets_target"] = r["force_rel_l2"] <= args.err_target
+        res[name] = r
+        methods[name] = (r["theta"], far)
 
     # ---- timed updates, each method from the same initial state ----
     c_f = flops_per_interaction(prm["toggles0"])
@@ -192,11 +214,16 @@ def main():
             r["mlp_tflops"] = cnt["node_interactions"] * MLP_FLOPS_PER_PAIR / (ph["mlp"] * 1e-3) / 1e12
         log("%s: %.3e particle-steps/s" % (name, r["particle_steps_per_s"]))
 
-    L, P = res["learned"], res["plain"]
+    L, P = res["learned"], res.get("plain")
     dominant_mlp = L["phases_ms_avg"]["mlp"] >= L["phases_ms_avg"]["walk"]
     if dominant_mlp:
+        default = n == 1 << 20 and abs(L["theta"] - 0.25) < 1e-9
         roof = {"bound": "mfma", "kernel": "k_far_mlp", "achieved": L["mlp_tflops"], "peak": BF16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": L["mlp_tflops"] / BF16_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": L["mlp_tflops"] / BF16_PEAK_TFLOPS,
+                "traffic": MLP_TRAFFIC_DEFAULT if default else None,
+                "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch (copy-calibrated), "
+                                  "profiles/k_far_mlp_open1M_theta0.25.md" if default else None,
+                "note": "GELU on the vector ALU (2 transcendentals per hidden unit) bounds this kernel, not MFMA",
                 "pairs_per_launch": L["node_interactions_per_particle"] * n, "flops_per_pair": MLP_FLOPS_PER_PAIR,
                 "ms_avg": L["phases_ms_avg"]["mlp"]}
     else:
@@ -211,7 +238,8 @@ def main():
     if not args.no_cpu:
         # ---- CPU baseline: the reference algorithm in FP64 on this host ----
         cs, cprm, _ = W.config2(args.cpu_sample, open_box=True)
-        cprm["theta"] = args.theta_plain
+        theta_c = P["theta"] if P else 0.04
+        cprm["theta"] = theta_c
         oc = bind.Oracle(bind.default_params(**cprm), args.cpu_sample)
         oc.upload(cs["x"], cs["y"], cs["wx"], cs["wy"], cs["mass"], cs["charge"], cs["angw"])
         tc = time.perf_counter()
@@ -224,7 +252,7 @@ def main():
                "sample": "FP64 oracle (integrator.js/forces.js/quadtree.js restated in C, OpenMP over particles), "
                          "plain monopole BH at theta %.2f, same dynamics at N=%d (same density), one update of %d "
                          "substeps, %.1f s; bootstrap force pass %.1f s untimed"
-                         % (args.theta_plain, args.cpu_sample, steps_c, tc2 - tb, tb - tc)}
+                         % (theta_c, args.cpu_sample, steps_c, tc2 - tb, tb - tc)}
 
     line = {
         "metric": "particle-steps/sec at N=1M, plain vs learned-correction BH at force err <=1e-4",
@@ -241,9 +269,10 @@ def main():
         "data": "synthetic (Philox seed 2000: uniform open box, lognormal masses, 0.1-quantized charges)",
         "config": {"workload": "open 1024x1024 box, N=%d uniform, gravity+Coulomb+Yukawa(mu=1)+magnetic dipole, "
                                "relativistic, spins U[-1,1], softening 8 (BH 4), dt 1/128 per update; learned BH at "
-                               "theta %.2f vs plain BH at theta %.2f" % (n, args.theta_learned, args.theta_plain),
-                   "n": n},
-        "speedup_learned_vs_plain": L["particle_steps_per_s"] / P["particle_steps_per_s"],
+                               "theta %.3f vs plain BH at theta %.3f, each the coarsest of its ladder with force "
+                               "error <= %.0e" % (n, L["theta"], P["theta"] if P else float("nan"), args.err_target),
+                   "n": n, "err_target": args.err_target},
+        "speedup_learned_vs_plain": L["particle_steps_per_s"] / P["particle_steps_per_s"] if P else None,
         "methods": res,
         "force_error_reference": {"kind": "FP64 direct sum (forces.js:93-126)", "sample": int(args.err_sample),
                                   "seconds": round(te, 2)},

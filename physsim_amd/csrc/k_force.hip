@@ -96,8 +96,10 @@ __device__ __forceinline__ void wave_add_counters(Ctrl* ctrl, int64_t pairs, int
     }
 }
 
-template <uint32_t F, bool KAHAN>
-__global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
+// FAR: 0 monopole nodes (the reference), kFarMultipole, kFarList (compile-time,
+// so the reference walk carries no far-field registers).
+template <uint32_t F, bool KAHAN, int FAR>
+__device__ __forceinline__ void walk_body(const ForceArgs& a) {
     __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
     __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
     const int n = a.ctrl->n_alive;
@@ -165,9 +167,9 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
                 pair_force<F, KAHAN>(o, acc, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 0.f, nd.mu, nd.L,
                                      1.f, 1.f, 1.f);
                 c_node++;
-                if (a.far_mode == kFarMultipole) far_multipole<F, KAHAN>(o, acc, pf, dx, dy, a.node_mom[node]);
+                if (FAR == kFarMultipole) far_multipole<F, KAHAN>(o, acc, pf, dx, dy, a.node_mom[node]);
             }
-            if (a.far_mode == kFarList) {
+            if (FAR == kFarList) {
                 // the accepting lanes append (node, lane) in lane order: one coalesced run
                 const uint64_t took = __ballot(accept);
                 if (took) {
@@ -196,13 +198,20 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) {
         }
     }
     if (overflow && lane == 0) a.ctrl->overflow = 2;
-    if (a.far_mode == kFarList && lane == 0) {
+    if (FAR == kFarList && lane == 0) {
         a.far_count[group] = (int32_t)(far_n < a.far_cap ? far_n : a.far_cap);
         if (far_n > a.far_cap) a.ctrl->overflow = 3;
     }
     wave_add_counters(a.ctrl, c_pair, c_node, c_visit);
     if (valid) store_acc<KAHAN>(a, i, o, acc);
 }
+
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) { walk_body<F, KAHAN, 0>(a); }
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk_multipole(ForceArgs a) { walk_body<F, KAHAN, kFarMultipole>(a); }
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk_list(ForceArgs a) { walk_body<F, KAHAN, kFarList>(a); }
 
 // 1PN recompute for the velocity-Verlet correction: the same wavefront walk
 // with _accum1PN at leaves and accepted nodes (forces.js:511-571); leaf sources
@@ -492,7 +501,9 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) {
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    GEON_FEAT_SWITCH(k_walk, feat, kahan, grid, s, a);
+    if (a.far_mode == kFarMultipole) GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
+    else if (a.far_mode == kFarList) GEON_FEAT_SWITCH(k_walk_list, feat, kahan, grid, s, a);
+    else GEON_FEAT_SWITCH(k_walk, feat, kahan, grid, s, a);
 }
 
 void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {

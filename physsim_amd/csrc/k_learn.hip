@@ -15,8 +15,9 @@
 //   F_i += m_i M (s/r)^2 / r^2 (g_u u + g_v v) + q_i A (s/r) / r^2 (c_u u + c_v v).
 // Network: x = ((f - mean) * istd, 1, 0...) in 16 inputs (input 10 carries
 // the first layer's bias), z1 = W1^T x, a1 = gelu(z1), z2 = W2^T a1 + b2,
-// a2 = gelu(z2), o = W3^T a2 + b3; gelu is the tanh form.  BF16 operands,
-// FP32 accumulation, on v_mfma_f32_32x32x16_bf16.
+// a2 = gelu(z2), o = W3^T a2 + b3; gelu is the tanh form (evaluated by a
+// polynomial, below).  BF16 operands, FP32 accumulation, on
+// v_mfma_f32_32x32x16_bf16.
 //
 // MFMA 32x32x16 lane layout (gfx950, lane l, h = l / 32, c = l % 32):
 //   A[m][k]: lane holds A[c][8h + j], j = 0..7
@@ -42,21 +43,35 @@ __host__ __device__ __forceinline__ int kperm(int t, int h, int j) {
     return 32 * (t >> 1) + 16 * (t & 1) + 8 * (j >> 2) + 4 * h + (j & 3);
 }
 
-// tanh-form GELU: z * sigma(2u), u = sqrt(2/pi) (z + 0.044715 z^3)
-constexpr float kGeluC0 = 0.7978845608028654f;
-constexpr float kGeluK0 = -2.f * kGeluC0 * 1.4426950408889634f;   // exp(-2u) = exp2(z (k0 + k1 z^2))
-constexpr float kGeluK1 = kGeluK0 * 0.044715f;
+// GELU, tanh form: gelu(z) = z phi(z), phi(z) = sigma(2u), u = sqrt(2/pi) (z + 0.044715 z^3).
+// phi is evaluated without transcendentals as 0.5 + zc P(zc^2), zc = clamp(z, -4, 4),
+// P of degree 6 fitted (weighted least squares, phi(+-4) pinned to 1 and 0) to the
+// tanh form on [-4, 4]: |phi error| <= 1.3e-4 and |gelu error| <= 5e-4 for all z,
+// below the BF16 rounding of the activations it feeds.  Outside [-4, 4] the tanh
+// form differs from relu by < 7e-5 |z|.
+constexpr float kGeluP[7] = {3.984482262e-01f, -6.551867656e-02f, 9.199989143e-03f, -8.964315729e-04f,
+                             5.576349423e-05f, -1.958261534e-06f, 2.922456199e-08f};
+constexpr float kGeluClamp = 4.f;
 
-// v_exp_f32 / v_rcp_f32 directly: exp2 of a large argument gives +inf and
-// rcp(inf) = 0, so no range reduction is needed for sigma in [0, 1].
-__device__ __forceinline__ float gelu_sig(float z) {
-    const float e = __builtin_amdgcn_exp2f(z * (kGeluK0 + kGeluK1 * z * z));
-    return __builtin_amdgcn_rcpf(1.f + e);
+__device__ __forceinline__ float gelu_q(float t) {
+    float q = kGeluP[6];
+#pragma unroll
+    for (int k = 5; k >= 0; k--) q = fmaf(q, t, kGeluP[k]);
+    return q;
 }
-__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
+__device__ __forceinline__ float gelu(float z) {
+    const float zc = __builtin_amdgcn_fmed3f(z, -kGeluClamp, kGeluClamp);
+    return z * fmaf(zc, gelu_q(zc * zc), 0.5f);
+}
 __device__ __forceinline__ float gelu_grad(float z) {
-    const float s = gelu_sig(z);
-    return s + z * s * (1.f - s) * (2.f * kGeluC0) * (1.f + 3.f * 0.044715f * z * z);
+    const float zc = __builtin_amdgcn_fmed3f(z, -kGeluClamp, kGeluClamp);
+    const float t = zc * zc;
+    const float q = gelu_q(t);
+    float dq = 6.f * kGeluP[6];
+#pragma unroll
+    for (int k = 5; k >= 1; k--) dq = fmaf(dq, t, (float)k * kGeluP[k]);
+    const float inner = fabsf(z) < kGeluClamp ? q + 2.f * t * dq : 0.f;   // d(zc q)/dz
+    return fmaf(zc, q, 0.5f) + z * inner;
 }
 
 __device__ __forceinline__ bf16x8 pack8(const float* v) {
@@ -602,18 +617,25 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
     }
 }
 
-// Sum of the block partials (fixed order), then optionally one Adam step
-// with bias correction on the FP32 master parameters.
-__global__ void k_far_adam(FarTrainArgs a, int nblocks, int apply, float lr, int64_t t) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
+// Sum of the block partials (fixed order: four interleaved partial sums per
+// parameter, combined in LDS), then optionally one Adam step with bias
+// correction on the FP32 master parameters.  Block: 64 parameters x 4 waves.
+__global__ __launch_bounds__(256) void k_far_adam(FarTrainArgs a, int nblocks, int apply, float lr, int64_t t) {
+    __shared__ float part[4][kWave];
+    const int pl = threadIdx.x & (kWave - 1), q = threadIdx.x / kWave;
+    const int p = blockIdx.x * kWave + pl;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         float L = 0.f;
         for (int b = 0; b < nblocks; b++) L += a.loss_part[b];
         if (a.loss_out) a.loss_out[0] = L;
     }
-    if (p >= kFarParams) return;
     float g = 0.f;
-    for (int b = 0; b < nblocks; b++) g += a.grad_part[(size_t)b * kFarParams + p];
+    if (p < kFarParams)
+        for (int b = q; b < nblocks; b += 4) g += a.grad_part[(size_t)b * kFarParams + p];
+    part[q][pl] = g;
+    __syncthreads();
+    if (q != 0 || p >= kFarParams) return;
+    g = (part[0][pl] + part[1][pl]) + (part[2][pl] + part[3][pl]);
     a.grad[p] = g;
     if (!apply) return;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -667,7 +689,7 @@ void launch_far_samples(hipStream_t s, const FarSampleArgs& a) {
 
 void launch_far_train(hipStream_t s, const FarTrainArgs& a, int nblocks, int apply, float lr, int64_t t) {
     hipLaunchKernelGGL(k_far_train, dim3(nblocks), dim3(kWave), 0, s, a);
-    hipLaunchKernelGGL(k_far_adam, dim3((kFarParams + 255) / 256), dim3(256), 0, s, a, nblocks, apply, lr, t);
+    hipLaunchKernelGGL(k_far_adam, dim3((kFarParams + kWave - 1) / kWave), dim3(256), 0, s, a, nblocks, apply, lr, t);
 }
 
 }  // namespace geon
