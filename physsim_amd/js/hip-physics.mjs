@@ -86,6 +86,28 @@ export default class HipPhysics {
 
     reset() { addon.reset(this._h); }
 
+    // ---- learned far-field correction (no GPUPhysics counterpart) ----
+    /** Far-field mode: 'none' (the reference's monopole nodes), 'multipole' or 'learned'. */
+    setFarField(mode, listCap = 0) {
+        this._params.farField = { none: 0, multipole: 1, learned: 2 }[mode];
+        this._params.farListCap = listCap;
+        addon.setParams(this._h, this._params);
+    }
+
+    /** Training samples from the current state; returns the count held. */
+    farCollect(groupStride = 1) { return addon.farCollect(this._h, groupStride); }
+
+    farClear() { addon.farClear(this._h); }
+
+    /** Adam steps on the held samples; returns the per-step losses. */
+    farTrain(steps, { batch = 16384, lr = 2e-3, lrFinal = 1e-5, seed = 3 } = {}) {
+        return addon.farTrain(this._h, steps, batch, lr, lrFinal, seed);
+    }
+
+    farGetModel() { return addon.farGetModel(this._h); }
+
+    farSetModel(model) { addon.farSetModel(this._h, model); }
+
     getParticleCount() { return addon.getStats(this._h).aliveCount; }
 
     requestStats(selIdx = -1) { this._stats = addon.getStats(this._h); }

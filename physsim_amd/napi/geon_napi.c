@@ -81,6 +81,8 @@ static void read_params(napi_env env, napi_value obj, GeonParams* p) {
     p->ext_bz = get_num(env, obj, "extBz", p->ext_bz);
     p->field_grid = (int32_t)get_num(env, obj, "fieldGrid", p->field_grid);
     p->kahan = (int32_t)get_num(env, obj, "kahan", p->kahan);
+    p->far_field = (int32_t)get_num(env, obj, "farField", p->far_field);
+    p->far_list_cap = (int32_t)get_num(env, obj, "farListCap", p->far_list_cap);
 }
 
 static napi_value args_n(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
@@ -342,6 +344,81 @@ static napi_value js_abi_version(napi_env env, napi_callback_info info) {
     return i32(env, geon_abi_version());
 }
 
+/* ---- learned far-field correction ---- */
+
+/* farCollect(h, groupStride) -> samples held */
+static napi_value js_far_collect(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!args_n(env, info, 2, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    int32_t stride;
+    CHECK(napi_get_value_int32(env, argv[1], &stride));
+    int64_t n = 0;
+    int rc = geon_far_collect(h, stride, &n);
+    if (rc) return throw_geon(env, h, rc);
+    return num(env, (double)n);
+}
+
+static napi_value js_far_clear(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    if (!args_n(env, info, 1, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    int rc = geon_far_clear(h);
+    if (rc) return throw_geon(env, h, rc);
+    return NULL;
+}
+
+/* farTrain(h, steps, batch, lr, lrFinal, seed) -> Float32Array of losses */
+static napi_value js_far_train(napi_env env, napi_callback_info info) {
+    napi_value argv[6];
+    if (!args_n(env, info, 6, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    int32_t steps, batch;
+    double lr, lr_final, seed;
+    CHECK(napi_get_value_int32(env, argv[1], &steps));
+    CHECK(napi_get_value_int32(env, argv[2], &batch));
+    CHECK(napi_get_value_double(env, argv[3], &lr));
+    CHECK(napi_get_value_double(env, argv[4], &lr_final));
+    CHECK(napi_get_value_double(env, argv[5], &seed));
+    if (steps < 0) steps = 0;
+    float* loss = NULL;
+    napi_value ab, ta;
+    CHECK(napi_create_arraybuffer(env, (size_t)(steps > 0 ? steps : 1) * sizeof(float), (void**)&loss, &ab));
+    CHECK(napi_create_typedarray(env, napi_float32_array, (size_t)steps, ab, 0, &ta));
+    int rc = geon_far_train(h, steps, batch, lr, lr_final, (uint64_t)seed, loss);
+    if (rc) return throw_geon(env, h, rc);
+    return ta;
+}
+
+/* farGetModel(h) -> Float32Array(GEON_FAR_MODEL_FLOATS) */
+static napi_value js_far_get_model(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    if (!args_n(env, info, 1, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    float* m = NULL;
+    napi_value ab, ta;
+    CHECK(napi_create_arraybuffer(env, GEON_FAR_MODEL_FLOATS * sizeof(float), (void**)&m, &ab));
+    CHECK(napi_create_typedarray(env, napi_float32_array, GEON_FAR_MODEL_FLOATS, ab, 0, &ta));
+    int rc = geon_far_get_model(h, m, GEON_FAR_MODEL_FLOATS);
+    if (rc) return throw_geon(env, h, rc);
+    return ta;
+}
+
+/* farSetModel(h, Float32Array(GEON_FAR_MODEL_FLOATS)) */
+static napi_value js_far_set_model(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!args_n(env, info, 2, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    napi_typedarray_type t;
+    size_t len = 0;
+    void* data = NULL;
+    CHECK(napi_get_typedarray_info(env, argv[1], &t, &len, &data, NULL, NULL));
+    if (t != napi_float32_array) { napi_throw_type_error(env, NULL, "farSetModel wants a Float32Array"); return NULL; }
+    int rc = geon_far_set_model(h, (const float*)data, (int32_t)len);
+    if (rc) return throw_geon(env, h, rc);
+    return NULL;
+}
+
 #define EXPORT(name, fn) do { napi_value f; napi_create_function(env, name, NAPI_AUTO_LENGTH, fn, NULL, &f); \
                               napi_set_named_property(env, exports, name, f); } while (0)
 
@@ -361,6 +438,11 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("getForces", js_get_forces);
     EXPORT("getStats", js_get_stats);
     EXPORT("lastError", js_last_error);
+    EXPORT("farCollect", js_far_collect);
+    EXPORT("farClear", js_far_clear);
+    EXPORT("farTrain", js_far_train);
+    EXPORT("farGetModel", js_far_get_model);
+    EXPORT("farSetModel", js_far_set_model);
     return exports;
 }
 
