@@ -3,7 +3,7 @@ learned-correction Barnes-Hut at force error <= 1e-4.
 
 Workload: 2^20 particles with the relativistic dynamics of the N=65536
 configuration scaled to keep its density (1024 x 1024 open box): gravity +
-Coulomb + Yukawa (mu = 1) + magnetic dipole, softening 8 (Barnes-Hut 4),
+Coulomb + Yukawa (mu = 1) + magnetic dipole, softening 8,
 spins uniform in [-1, 1].  The box is open (despawn), where Barnes-Hut
 converges to the FP64 direct sum the force error is judged against.  One step
 = one Physics.update(dt = 1/128) (adaptive substeps); particle-steps = sum over
@@ -268,7 +268,7 @@ def main():
         "dtype": "f32 (forces, Kahan), bf16 MFMA with f32 accumulation (network)",
         "data": "synthetic (Philox seed 2000: uniform open box, lognormal masses, 0.1-quantized charges)",
         "config": {"workload": "open 1024x1024 box, N=%d uniform, gravity+Coulomb+Yukawa(mu=1)+magnetic dipole, "
-                               "relativistic, spins U[-1,1], softening 8 (BH 4), dt 1/128 per update; learned BH at "
+                               "relativistic, spins U[-1,1], softening 8, dt 1/128 per update; learned BH at "
                                "theta %.3f vs plain BH at theta %.3f, each the coarsest of its ladder with force "
                                "error <= %.0e" % (n, L["theta"], P["theta"] if P else float("nan"), args.err_target),
                    "n": n, "err_target": args.err_target},

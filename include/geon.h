@@ -295,6 +295,42 @@ int  geon_kernel_timing(GeonHandle h, int32_t kernel, double* total_ms, int64_t*
 /* Last error message of this handle (or of the last failed create when h is NULL). */
 const char* geon_last_error(GeonHandle h);
 
+
+/* ---- Batched ensembles (no counterpart in the reference) -----------------
+ * Many independent systems that share one GeonParams, each with the semantics
+ * of one Physics.update per call (its own adaptive substeps, despawn once per
+ * update), stepped together: every substep of all systems is one pass of the
+ * kernels, and the max_substeps bodies are replayed from one captured graph.
+ * A system's results do not depend on the other systems of the batch.  Runs
+ * gravity, Coulomb, magnetic, gravitomagnetic, Yukawa, relativity and
+ * spin-orbit with the Barnes-Hut walk (GEON_EUNSUPPORTED otherwise). */
+typedef struct GeonBatch_* GeonBatchHandle;
+typedef struct GeonBatchTune {
+    int32_t walk_threads;       /* persistent walk block size: 64, 128 or 256 */
+    int32_t waves_per_cu;       /* persistent walk grid: resident waves per compute unit */
+    int32_t stack;              /* walk node-stack entries per wave: 64 or 96 */
+    int32_t use_graph;          /* 1: replay the captured substep loop (even max_substeps) */
+    double  walk_ms;            /* geon_batch_autotune: the chosen configuration's walk time */
+} GeonBatchTune;
+int  geon_batch_create(const GeonParams* params, int32_t n_systems, const int32_t* max_particles,
+                       GeonBatchHandle* out);
+void geon_batch_destroy(GeonBatchHandle h);
+/* One system's particles, as geon_upload_soa. */
+int  geon_batch_upload(GeonBatchHandle h, int32_t system, const GeonSoAView* v, int32_t n);
+int  geon_batch_download(GeonBatchHandle h, int32_t system, GeonSoAView* v, int32_t* n_slots);
+/* One Physics.update(dt) of every system; substeps (n_systems entries, may be
+ * NULL) receives each system's substep count. */
+int  geon_batch_update(GeonBatchHandle h, double dt, int32_t* substeps);
+int  geon_batch_compute_forces(GeonBatchHandle h);
+int  geon_batch_get_forces(GeonBatchHandle h, int32_t system, GeonForceView* v, int32_t n_slots);
+int  geon_batch_get_stats(GeonBatchHandle h, int32_t system, GeonStats* out);
+/* Tile and stack sizes of the persistent walk: autotune times every candidate
+ * on the current state and keeps the fastest. */
+int  geon_batch_autotune(GeonBatchHandle h, GeonBatchTune* out);
+int  geon_batch_set_tune(GeonBatchHandle h, const GeonBatchTune* t);
+int  geon_batch_get_tune(GeonBatchHandle h, GeonBatchTune* out);
+const char* geon_batch_last_error(GeonBatchHandle h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,11 @@ class GeonTreeNode(C.Structure):
                 ("com_x", C.c_double), ("com_y", C.c_double)]
 
 
+class GeonBatchTune(C.Structure):
+    _fields_ = [("walk_threads", C.c_int32), ("waves_per_cu", C.c_int32), ("stack", C.c_int32),
+                ("use_graph", C.c_int32), ("walk_ms", C.c_double)]
+
+
 H = C.c_void_p
 
 # name -> (restype, argtypes)
@@ -123,6 +128,18 @@ PROTOTYPES = {
     "geon_far_grad": (C.c_int, [H, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "geon_far_get_model": (C.c_int, [H, C.POINTER(C.c_float), C.c_int32]),
     "geon_far_set_model": (C.c_int, [H, C.POINTER(C.c_float), C.c_int32]),
+    "geon_batch_create": (C.c_int, [C.POINTER(GeonParams), C.c_int32, C.POINTER(C.c_int32), C.POINTER(H)]),
+    "geon_batch_destroy": (None, [H]),
+    "geon_batch_upload": (C.c_int, [H, C.c_int32, C.POINTER(GeonSoAView), C.c_int32]),
+    "geon_batch_download": (C.c_int, [H, C.c_int32, C.POINTER(GeonSoAView), C.POINTER(C.c_int32)]),
+    "geon_batch_update": (C.c_int, [H, C.c_double, C.POINTER(C.c_int32)]),
+    "geon_batch_compute_forces": (C.c_int, [H]),
+    "geon_batch_get_forces": (C.c_int, [H, C.c_int32, C.POINTER(GeonForceView), C.c_int32]),
+    "geon_batch_get_stats": (C.c_int, [H, C.c_int32, C.POINTER(GeonStats)]),
+    "geon_batch_autotune": (C.c_int, [H, C.POINTER(GeonBatchTune)]),
+    "geon_batch_set_tune": (C.c_int, [H, C.POINTER(GeonBatchTune)]),
+    "geon_batch_get_tune": (C.c_int, [H, C.POINTER(GeonBatchTune)]),
+    "geon_batch_last_error": (C.c_char_p, [H]),
 }
 
 

@@ -261,6 +261,7 @@ ForceArgs force_args(GeonContext* h) {
     a.n_upper = h->n_host;
     a.max_nodes = 1 + 4 * h->tb.cap_divided;
     a.far_mode = 0;
+    a.bsys = nullptr; a.bvalid = nullptr; a.bctrl = nullptr; a.broot = nullptr;
     a.node_mom = h->tb.node_mom;
     a.far_list = h->far_list;
     a.far_count = h->far_count;

@@ -63,6 +63,20 @@ void launch_gather(hipStream_t s, const Columns& cols, const uint32_t* perm, int
 void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, const float* m,
                 const float* wx, const float* wy, const float* mu, const float* L,
                 const int32_t* qq, int n_upper, KeyParams kp, Ctrl* ctrl, bool moments = false);
+// Batch of independent systems (slices of the sorted arrays, kScanTile-aligned).
+struct BatchTreeArgs {
+    int S, total;
+    const int32_t* soff;        // S + 1 slice offsets
+    int32_t* n_in;              // per system: tree points (written)
+    int32_t* seg_first;         // per slot: first slot of its system (written)
+    int32_t* root_slot;         // per system: root node slot (written)
+};
+void launch_batch_keys(hipStream_t s, const float* x, const float* y, const int32_t* valid, const int32_t* sys,
+                       KeyParams kp, uint64_t* keys, uint32_t* perm, int total);
+void build_tree_batch(hipStream_t s, TreeBufs& tb, const BatchTreeArgs& ba, const float* x, const float* y,
+                      const float* m, const float* wx, const float* wy, const float* mu, const float* L,
+                      const int32_t* qq, KeyParams kp, Ctrl* ctrl);
+constexpr int kBatchAlign = 2048;   // slice alignment (the prefix-scan tile)
 // Exclusive scan of int32 flags (n from device ctrl->n_alive) -> out, total -> *total_dev.
 void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n_upper,
                         const int32_t* n_dev, int32_t* total_dev, int32_t* tmp);
@@ -90,11 +104,21 @@ struct ForceArgs {
     uint32_t* far_list;         // far_cap entries per group: node | lane << 26
     int32_t* far_count;         // entries per group
     int64_t far_cap;
+    // batch of systems (null for one system): per slot system id and liveness,
+    // per system controller (its `active` flag) and root node slot
+    const int32_t* bsys;
+    const int32_t* bvalid;
+    const Ctrl* bctrl;
+    const int32_t* broot;
 };
 constexpr int kFarMultipole = 1, kFarList = 2;
 constexpr int kFarLaneShift = 26;
 constexpr uint32_t kFarNodeMask = (1u << kFarLaneShift) - 1;
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan);
+// Persistent walk: `grid` blocks of `threads` pull 64-particle groups from *queue
+// (reset to 0 before the launch) until all n_groups are done.
+void launch_walk_persistent(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan, int* queue,
+                            int64_t n_groups, int grid, int threads, int stack);
 void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan);
 void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const float* m,
                             const float* q, const float* vx, const float* vy, const float* ang_vel,
@@ -234,6 +258,21 @@ struct StepArgs {
     int n_upper;
 };
 void launch_dt_scan(hipStream_t s, const StepArgs& a);
+// Batch of systems: per-slot system id and liveness, per-system controllers.
+struct BatchStepArgs {
+    int S, total;
+    const int32_t* sys;
+    int32_t* valid;
+    const int32_t* soff;
+    Ctrl* ctrl;
+};
+void launch_batch_begin(hipStream_t s, Ctrl* ctrls, int S, double dt, int max_sub);
+void launch_batch_ctrl(hipStream_t s, Ctrl* ctrls, int S, double softening, int first);
+void launch_batch_kick_drift(hipStream_t s, const StepArgs& a, const BatchStepArgs& b);
+void launch_batch_dt_scan(hipStream_t s, const StepArgs& a, const BatchStepArgs& b);
+void launch_batch_boundary(hipStream_t s, const StepArgs& a, const BatchStepArgs& b, int mode, double w, double h);
+void launch_batch_stats(hipStream_t s, const StepArgs& a, const BatchStepArgs& b, const float* pe, int rel,
+                        double* out);
 void launch_ctrl(hipStream_t s, Ctrl* ctrl, double softening, int kick_reuse);
 void launch_kick_drift(hipStream_t s, const StepArgs& a);
 void launch_vv_kick(hipStream_t s, const StepArgs& a);

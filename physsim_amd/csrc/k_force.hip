@@ -98,17 +98,23 @@ __device__ __forceinline__ void wave_add_counters(Ctrl* ctrl, int64_t pairs, int
 
 // FAR: 0 monopole nodes (the reference), kFarMultipole, kFarList (compile-time,
 // so the reference walk carries no far-field registers).
-template <uint32_t F, bool KAHAN, int FAR>
-__device__ __forceinline__ void walk_body(const ForceArgs& a) {
-    __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
-    __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
+template <uint32_t F, bool KAHAN, int FAR, int STACK = kWalkStack>
+__device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
+    __shared__ int32_t s_node[kWavesPerBlock][STACK];
+    __shared__ uint64_t s_mask[kWavesPerBlock][STACK];
     const int n = a.ctrl->n_alive;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
-    const int i = blockIdx.x * kWalkThreads + threadIdx.x;
-    const bool valid = i < n;
+    const int i = (int)(group * kWave) + lane;
+    bool valid = i < n;
+    int my_sys = -1;
+    if (a.bsys != nullptr && valid) {
+        // batch: live particles of systems still inside their update
+        my_sys = a.bsys[i];
+        valid = a.bvalid[i] && a.bctrl[my_sys].active;
+    }
     const uint64_t active = __ballot(valid);
-    if (active == 0) return;                       // whole wave past the end
+    if (active == 0) return;                       // whole wave past the end (or idle)
     const uint64_t my_bit = 1ull << lane;
     Obs o;
     if (valid) load_obs(a, i, o);
@@ -118,13 +124,23 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a) {
     const PhysFlags pf = a.pf;
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
     int64_t c_pair = 0, c_node = 0, c_visit = 0;
-    // far list of this 64-particle group (far_mode == kFarList)
-    const int64_t group = (int64_t)blockIdx.x * kWavesPerBlock + wv;
-    int64_t far_n = 0;
+    int64_t far_n = 0;                             // far list of this group (FAR == kFarList)
     int sp = 0;
-    if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
+    if (a.bsys == nullptr) {
+        if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
+        sp = 1;
+    } else {
+        // one root per system among the wave's lanes, each with its lanes' mask
+        uint64_t rem = active;
+        while (rem != 0 && sp < STACK - 4) {
+            const int sy = __shfl(my_sys, (int)__builtin_ctzll(rem), 64);
+            const uint64_t m = __ballot(valid && my_sys == sy);
+            if (lane == 0) { s_node[wv][sp] = a.broot[sy]; s_mask[wv][sp] = m; }
+            sp++;
+            rem &= ~m;
+        }
+    }
     __builtin_amdgcn_wave_barrier();
-    sp = 1;
     bool overflow = false;
     while (sp > 0) {
         sp--;
@@ -181,7 +197,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a) {
             }
             const uint64_t open = __ballot(mine && !accept);
             if (open) {
-                if (sp + 4 <= kWalkStack) {
+                if (sp + 4 <= STACK) {
                     if (lane == 0) {
                         // push nw, ne, sw, se: se is popped first (forces.js:688-691)
                         s_node[wv][sp] = nd.child;     s_mask[wv][sp] = open;
@@ -206,12 +222,42 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a) {
     if (valid) store_acc<KAHAN>(a, i, o, acc);
 }
 
+__device__ __forceinline__ int64_t walk_group() { return (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; }
+
 template <uint32_t F, bool KAHAN>
-__global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) { walk_body<F, KAHAN, 0>(a); }
+__global__ __launch_bounds__(kWalkThreads) void k_walk(ForceArgs a) { walk_body<F, KAHAN, 0>(a, walk_group()); }
 template <uint32_t F, bool KAHAN>
-__global__ __launch_bounds__(kWalkThreads) void k_walk_multipole(ForceArgs a) { walk_body<F, KAHAN, kFarMultipole>(a); }
+__global__ __launch_bounds__(kWalkThreads) void k_walk_multipole(ForceArgs a) {
+    walk_body<F, KAHAN, kFarMultipole>(a, walk_group());
+}
 template <uint32_t F, bool KAHAN>
-__global__ __launch_bounds__(kWalkThreads) void k_walk_list(ForceArgs a) { walk_body<F, KAHAN, kFarList>(a); }
+__global__ __launch_bounds__(kWalkThreads) void k_walk_list(ForceArgs a) {
+    walk_body<F, KAHAN, kFarList>(a, walk_group());
+}
+
+// Persistent walk: every wave pulls 64-particle groups from a queue until none
+// are left, so uneven work (systems of a batch, dense regions) balances itself.
+// Node-stack entries per wave: 64 or 96 (LDS per block = 4 waves x 12 B x entries).
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk_persistent64(ForceArgs a, int* queue, int64_t n_groups) {
+    for (;;) {
+        int g = 0;
+        if ((threadIdx.x & (kWave - 1)) == 0) g = atomicAdd(queue, 1);
+        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+        if (g >= n_groups) break;
+        walk_body<F, KAHAN, 0, 64>(a, g);
+    }
+}
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk_persistent96(ForceArgs a, int* queue, int64_t n_groups) {
+    for (;;) {
+        int g = 0;
+        if ((threadIdx.x & (kWave - 1)) == 0) g = atomicAdd(queue, 1);
+        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+        if (g >= n_groups) break;
+        walk_body<F, KAHAN, 0, 96>(a, g);
+    }
+}
 
 // 1PN recompute for the velocity-Verlet correction: the same wavefront walk
 // with _accum1PN at leaves and accepted nodes (forces.js:511-571); leaf sources
@@ -504,6 +550,28 @@ void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.far_mode == kFarMultipole) GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
     else if (a.far_mode == kFarList) GEON_FEAT_SWITCH(k_walk_list, feat, kahan, grid, s, a);
     else GEON_FEAT_SWITCH(k_walk, feat, kahan, grid, s, a);
+}
+
+#define GEON_FEAT_SWITCH_P(KERNEL, feat, kahan, grid, threads, stream, ...)                              \
+    do {                                                                                              \
+        const dim3 g_(grid), b_(threads);                                                             \
+        switch ((feat) & 7u) {                                                                        \
+            case 1: if (kahan) hipLaunchKernelGGL((KERNEL<1, true>), g_, b_, 0, stream, __VA_ARGS__); \
+                    else hipLaunchKernelGGL((KERNEL<1, false>), g_, b_, 0, stream, __VA_ARGS__); break; \
+            case 2: if (kahan) hipLaunchKernelGGL((KERNEL<2, true>), g_, b_, 0, stream, __VA_ARGS__); \
+                    else hipLaunchKernelGGL((KERNEL<2, false>), g_, b_, 0, stream, __VA_ARGS__); break; \
+            case 3: if (kahan) hipLaunchKernelGGL((KERNEL<3, true>), g_, b_, 0, stream, __VA_ARGS__); \
+                    else hipLaunchKernelGGL((KERNEL<3, false>), g_, b_, 0, stream, __VA_ARGS__); break; \
+            default: if (kahan) hipLaunchKernelGGL((KERNEL<7, true>), g_, b_, 0, stream, __VA_ARGS__); \
+                    else hipLaunchKernelGGL((KERNEL<7, false>), g_, b_, 0, stream, __VA_ARGS__); break; \
+        }                                                                                             \
+    } while (0)
+
+void launch_walk_persistent(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan, int* queue,
+                            int64_t n_groups, int grid, int threads, int stack) {
+    if (n_groups <= 0) return;
+    if (stack <= 64) GEON_FEAT_SWITCH_P(k_walk_persistent64, feat, kahan, grid, threads, s, a, queue, n_groups);
+    else GEON_FEAT_SWITCH_P(k_walk_persistent96, feat, kahan, grid, threads, s, a, queue, n_groups);
 }
 
 void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {

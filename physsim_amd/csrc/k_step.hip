@@ -36,12 +36,8 @@ __device__ __forceinline__ void block_max_atomic(uint64_t* addr, double v) {
 
 // Max |F/m|^2 and max cyclotron frequency over the live particles
 // (integrator.js:812-849).
-__global__ void k_dt_scan(StepArgs a) {
-    const int n = a.ctrl->n_alive;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    float acc2f = -1.f, cycf = -1.f;
-    if (i < n) {
-        // FP32 per particle (the forces are FP32); only the block maxima go to FP64
+// FP32 per particle (the forces are FP32); only the maxima go to FP64.
+__device__ __forceinline__ void dt_scan_one(const StepArgs& a, int i, float& acc2f, float& cycf) {
         const float inv_m = 1.f / a.mass[i];
         const float fx = a.fx[i] * inv_m, fy = a.fy[i] * inv_m;
         acc2f = fx * fx + fy * fy;
@@ -56,7 +52,13 @@ __global__ void k_dt_scan(StepArgs a) {
             if (absbgz > 0.f) cycf = fmaxf(cycf, 4.f * absbgz);
         }
         if (cycf != cycf) cycf = -1.f;
-    }
+}
+
+__global__ void k_dt_scan(StepArgs a) {
+    const int n = a.ctrl->n_alive;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float acc2f = -1.f, cycf = -1.f;
+    if (i < n) dt_scan_one(a, i, acc2f, cycf);
     const double acc2 = acc2f, cyc = cycf;
     block_max_atomic(&a.ctrl->max_acc2_fresh, acc2);
     block_max_atomic(&a.ctrl->max_cyc, cyc);
@@ -67,8 +69,8 @@ void launch_dt_scan(hipStream_t s, const StepArgs& a) {
     hipLaunchKernelGGL(k_dt_scan, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a);
 }
 
-// One thread: the adaptive-substep decision of integrator.js:810-874.
-__global__ void k_ctrl(Ctrl* c, double softening, int first) {
+// The adaptive-substep decision of integrator.js:810-874 for one system.
+__device__ __forceinline__ void ctrl_step(Ctrl* c, double softening, int first) {
     if (c->total_steps >= c->max_substeps || !(c->dt_remain > kEps)) {
         c->active = 0;
         return;
@@ -96,6 +98,8 @@ __global__ void k_ctrl(Ctrl* c, double softening, int first) {
     c->max_cyc = 0;
 }
 
+__global__ void k_ctrl(Ctrl* c, double softening, int first) { ctrl_step(c, softening, first); }
+
 void launch_ctrl(hipStream_t s, Ctrl* ctrl, double softening, int first) {
     hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1), 0, s, ctrl, softening, first);
 }
@@ -103,14 +107,8 @@ void launch_ctrl(hipStream_t s, Ctrl* ctrl, double softening, int first) {
 // Fused substep body per particle: save 1PN force, half kick, Boris rotation,
 // half kick, NaN guard (integrator.js:884-928); spin-orbit and torques
 // (930-981); drift (1270-1278); expansion (1281-1292).
-__global__ void k_kick_drift(StepArgs a) {
-    const Ctrl* c = a.ctrl;
-    if (!c->active) return;
-    const int n = c->n_alive;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc2 = -1.0;
-    if (i < n) {
-        const float dt = (float)c->dt_sub;
+__device__ __forceinline__ double kick_drift_one(const StepArgs& a, int i, float dt) {
+        double acc2;
         const float half_dt = 0.5f * dt;
         const float m = a.mass[i];
         const float inv_m = 1.f / m;
@@ -200,7 +198,16 @@ __global__ void k_kick_drift(StepArgs a) {
         a.wx[i] = wx; a.wy[i] = wy;
         a.vx[i] = vx; a.vy[i] = vy;
         a.angw[i] = angw; a.ang_vel[i] = ang_vel;
-    }
+        return acc2;
+}
+
+__global__ void k_kick_drift(StepArgs a) {
+    const Ctrl* c = a.ctrl;
+    if (!c->active) return;
+    const int n = c->n_alive;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc2 = -1.0;
+    if (i < n) acc2 = kick_drift_one(a, i, (float)c->dt_sub);
     block_max_atomic(&a.ctrl->max_acc2_kick, acc2);
 }
 
@@ -378,6 +385,155 @@ void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* par
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kStatCols * 64), 0, s, partial, nb, kStatCols, out);
     hipLaunchKernelGGL(k_stats_pass2, dim3(nb), dim3(kStatThreads), 0, s, a, out, partial, nb);
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(2 * 64), 0, s, partial, nb, 2, out + kStatCols);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batches: the same per-particle bodies with each system's own controller.
+// Slices are kBatchAlign-aligned, so every wavefront lies in one system.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_max_to(uint64_t* addr, double v) {
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos_double(addr, v);
+}
+
+__global__ void k_batch_begin(Ctrl* __restrict__ c, int S, double dt, int max_sub) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    c[s].dt_remain = dt; c[s].dt_total = dt; c[s].total_steps = 0; c[s].max_substeps = max_sub;
+    c[s].active = 0; c[s].deaths = 0;
+    c[s].max_acc2_fresh = 0; c[s].max_acc2_kick = 0; c[s].max_cyc = 0;
+}
+
+__global__ void k_batch_ctrl(Ctrl* __restrict__ c, int S, double softening, int first) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) ctrl_step(&c[s], softening, first);
+}
+
+__global__ void k_batch_kick_drift(StepArgs a, BatchStepArgs b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.total) return;
+    const int s = b.sys[i];
+    const Ctrl* c = &b.ctrl[s];
+    if (!c->active) return;                          // the whole wave shares s
+    double acc2 = -1.0;
+    if (b.valid[i]) acc2 = kick_drift_one(a, i, (float)c->dt_sub);
+    wave_max_to(&b.ctrl[s].max_acc2_kick, acc2);
+}
+
+__global__ void k_batch_dt_scan(StepArgs a, BatchStepArgs b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.total) return;
+    const int s = b.sys[i];
+    float acc2f = -1.f, cycf = -1.f;
+    if (b.valid[i]) dt_scan_one(a, i, acc2f, cycf);
+    wave_max_to(&b.ctrl[s].max_acc2_fresh, (double)acc2f);
+    wave_max_to(&b.ctrl[s].max_cyc, (double)cycf);
+}
+
+// Despawn once per update (integrator.js:1798-1824): the particle stops being live.
+__global__ void k_batch_boundary(StepArgs a, BatchStepArgs b, int mode, float w, float h) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.total || !b.valid[i]) return;
+    float x = a.x[i], y = a.y[i];
+    if (mode == 0) {
+        if (x < -64.f || x > w + 64.f || y < -64.f || y > h + 64.f) {
+            b.valid[i] = 0;
+            atomicAdd(&b.ctrl[b.sys[i]].deaths, 1);
+        }
+    } else if (mode == 2) {
+        if (x < 0.f) x += w; else if (x > w) x -= w;
+        if (y < 0.f) y += h; else if (y > h) y -= h;
+    } else {
+        x = fminf(fmaxf(x, 0.f), w);
+        y = fminf(fmaxf(y, 0.f), h);
+    }
+    a.x[i] = x; a.y[i] = y;
+}
+
+// FP64 conservation sums of each system (energy.js:11-62): one block per
+// system, fixed-order reductions; out[s * 10 + k] in the launch_stats order.
+__global__ __launch_bounds__(kStatThreads) void k_batch_stats(StepArgs a, BatchStepArgs b, const float* __restrict__ pe,
+                                                              int rel, double* __restrict__ out) {
+    __shared__ double sh[kStatCols][kStatThreads];
+    const int s = blockIdx.x;
+    const int lo = b.soff[s], hi = b.soff[s + 1];
+    double v[kStatCols] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = lo + threadIdx.x; i < hi; i += kStatThreads) {
+        if (!b.valid[i]) continue;
+        const double m = a.mass[i];
+        const double r = a.radius[i];
+        const double r2 = r * r;
+        const double wx = a.wx[i], wy = a.wy[i], angw = a.angw[i];
+        if (rel) {
+            const double wsq = wx * wx + wy * wy;
+            const double g = sqrt(1 + wsq);
+            v[0] += wsq / (g + 1) * m;
+            const double srsq = angw * angw * r2;
+            v[1] += 0.4 * m * srsq / (sqrt(1 + srsq) + 1);
+        } else {
+            const double vx = a.vx[i], vy = a.vy[i], av = a.ang_vel[i];
+            v[0] += 0.5 * m * (vx * vx + vy * vy);
+            v[1] += 0.5 * 0.4 * m * r2 * av * av;
+        }
+        v[2] += m * wx;
+        v[3] += m * wy;
+        v[4] += m;
+        v[5] += m * (double)a.x[i];
+        v[6] += m * (double)a.y[i];
+        v[7] += (double)pe[i];
+    }
+    for (int c = 0; c < kStatCols; c++) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int off = kStatThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int c = 0; c < kStatCols; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+        __syncthreads();
+    }
+    const double tm = sh[4][0];
+    const double comx = tm > 0 ? sh[5][0] / tm : 0.0, comy = tm > 0 ? sh[6][0] / tm : 0.0;
+    if (threadIdx.x < kStatCols) out[s * 10 + threadIdx.x] = sh[threadIdx.x][0];
+    __syncthreads();
+    double orb = 0, spin = 0;
+    for (int i = lo + threadIdx.x; i < hi; i += kStatThreads) {
+        if (!b.valid[i]) continue;
+        const double m = a.mass[i];
+        const double r = a.radius[i];
+        const double dx = (double)a.x[i] - comx, dy = (double)a.y[i] - comy;
+        orb += dx * (m * (double)a.wy[i]) - dy * (m * (double)a.wx[i]);
+        spin += 0.4 * m * r * r * (double)a.angw[i];
+    }
+    sh[0][threadIdx.x] = orb;
+    sh[1][threadIdx.x] = spin;
+    __syncthreads();
+    for (int off = kStatThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[s * 10 + 8] = sh[0][0]; out[s * 10 + 9] = sh[1][0]; }
+}
+
+void launch_batch_begin(hipStream_t s, Ctrl* ctrls, int S, double dt, int max_sub) {
+    hipLaunchKernelGGL(k_batch_begin, dim3((S + 255) / 256), dim3(256), 0, s, ctrls, S, dt, max_sub);
+}
+void launch_batch_ctrl(hipStream_t s, Ctrl* ctrls, int S, double softening, int first) {
+    hipLaunchKernelGGL(k_batch_ctrl, dim3((S + 255) / 256), dim3(256), 0, s, ctrls, S, softening, first);
+}
+void launch_batch_kick_drift(hipStream_t s, const StepArgs& a, const BatchStepArgs& b) {
+    hipLaunchKernelGGL(k_batch_kick_drift, dim3((b.total + 255) / 256), dim3(256), 0, s, a, b);
+}
+void launch_batch_dt_scan(hipStream_t s, const StepArgs& a, const BatchStepArgs& b) {
+    hipLaunchKernelGGL(k_batch_dt_scan, dim3((b.total + 255) / 256), dim3(256), 0, s, a, b);
+}
+void launch_batch_boundary(hipStream_t s, const StepArgs& a, const BatchStepArgs& b, int mode, double w, double h) {
+    hipLaunchKernelGGL(k_batch_boundary, dim3((b.total + 255) / 256), dim3(256), 0, s, a, b, mode, (float)w, (float)h);
+}
+void launch_batch_stats(hipStream_t s, const StepArgs& a, const BatchStepArgs& b, const float* pe, int rel,
+                        double* out) {
+    hipLaunchKernelGGL(k_batch_stats, dim3(b.S), dim3(kStatThreads), 0, s, a, b, pe, rel, out);
 }
 
 }  // namespace geon

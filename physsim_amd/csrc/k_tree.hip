@@ -456,6 +456,23 @@ __global__ __launch_bounds__(1024) void k_pre_bsums(double* __restrict__ bsum, i
     for (int i = lo; i < hi; i++) { const double v = b[i]; b[i] = run; run += v; }
 }
 
+// Exclusive scan of the block sums restarting at every system of a batch
+// (slices aligned to kScanTile): one thread per system, fixed order.
+__global__ void k_pre_bsums_seg(double* __restrict__ bsum, int nb, int ncols, const int32_t* __restrict__ soff,
+                                int S) {
+    const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sidx >= S) return;
+    const int b0 = soff[sidx] / kScanTile, b1 = min(nb, (soff[sidx + 1] + kScanTile - 1) / kScanTile);
+    for (int c = 0; c < ncols; c++) {
+        double run = 0.0;
+        for (int bk = b0; bk < b1; bk++) {
+            const double v = bsum[(size_t)c * nb + bk];
+            bsum[(size_t)c * nb + bk] = run;
+            run += v;
+        }
+    }
+}
+
 template <class IN>
 __global__ __launch_bounds__(kScanThreads) void k_pre_apply(IN in, const double* __restrict__ bsum, int nb,
                                                             double* __restrict__ pre, int64_t stride,
@@ -519,9 +536,12 @@ static void prefix_columns(hipStream_t s, const IN& in, double* bsum, double* pr
 // ---------------------------------------------------------------------------
 // Tree structure
 // ---------------------------------------------------------------------------
+// Common prefix of two keys in tree levels; -1 when they differ above the
+// Morton bits (the outside-the-root-box bit, or a batch's system id).
 __device__ __forceinline__ int cpl_levels(uint64_t a, uint64_t b) {
     const uint64_t d = a ^ b;
     if (d == 0) return kKeyLevels;
+    if (d >> (2 * kKeyLevels)) return -1;
     return (__clzll((long long)d) - (64 - 2 * kKeyLevels)) >> 1;
 }
 
@@ -579,6 +599,7 @@ struct NodeBuild {
     uint64_t* path;
     int32_t* npts;
     int32_t* first;             // first sorted point of each node
+    const int32_t* seg_first;   // batch: first slot of each point's system (prefix sums restart there)
     NodeMom* mom;               // null: moments not wanted
     const double* pre_mom;      // kMomCols columns of prefix sums about (mom_in.x0, mom_in.y0)
     MomIn mom_in;
@@ -586,10 +607,11 @@ struct NodeBuild {
     double xmin, ymin;          // lower corner of the root box
     double aspect_x, aspect_y;  // box extent per unit of `size` along x and y (1 and H/W)
     int64_t cap_div;
+    bool batch;                 // roots are written per system by k_batch_roots, not at slot 0
 };
 
 __device__ __forceinline__ uint64_t prefix_at(uint64_t key, int level) {
-    return level == 0 ? 0ull : (key >> (2 * (kKeyLevels - level)));
+    return key >> (2 * (kKeyLevels - level));      // level 0 keeps the bits above the Morton code
 }
 
 // First index j in [lo, hi) with prefix_at(keys[j], level) >= target (keys sorted).
@@ -653,14 +675,16 @@ __device__ void write_node(const NodeBuild& b, int slot, int level, uint64_t pat
     } else {
         const double* P = b.pre;
         const int64_t s = b.stride;
-        m = P[e] - P[a];
-        mx = P[s + e] - P[s + a];
-        my = P[2 * s + e] - P[2 * s + a];
-        mu = P[3 * s + e] - P[3 * s + a];
-        L = P[4 * s + e] - P[4 * s + a];
-        px = P[5 * s + e] - P[5 * s + a];
-        py = P[6 * s + e] - P[6 * s + a];
-        qq = P[7 * s + e] - P[7 * s + a];
+        // a batch's prefix sums restart at each system: its first point subtracts 0
+        const bool z = b.seg_first != nullptr && b.seg_first[a] == a;
+        m = P[e] - (z ? 0.0 : P[a]);
+        mx = P[s + e] - (z ? 0.0 : P[s + a]);
+        my = P[2 * s + e] - (z ? 0.0 : P[2 * s + a]);
+        mu = P[3 * s + e] - (z ? 0.0 : P[3 * s + a]);
+        L = P[4 * s + e] - (z ? 0.0 : P[4 * s + a]);
+        px = P[5 * s + e] - (z ? 0.0 : P[5 * s + a]);
+        py = P[6 * s + e] - (z ? 0.0 : P[6 * s + a]);
+        qq = P[7 * s + e] - (z ? 0.0 : P[7 * s + a]);
     }
     Node nd;
     const double comx = m > 0 ? mx / m : 0.0, comy = m > 0 ? my / m : 0.0;
@@ -703,7 +727,7 @@ __global__ void k_tree_nodes(NodeBuild b, Ctrl* __restrict__ ctrl) {
     const int ndiv = ctrl->n_divided;
     const int n = ctrl->n_tree;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid == 0) {
+    if (gid == 0 && !b.batch) {
         // root (quadtree.js:412-419): divided iff more than 4 points were inserted
         const bool div = n > kLeafCap;
         write_node(b, 0, 0, 0ull, 0, n, n, div, 1);
@@ -769,6 +793,8 @@ void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, con
     b.nodes = tb.nodes; b.mass64 = tb.node_mass64; b.comx64 = tb.node_comx64; b.comy64 = tb.node_comy64;
     b.qq64 = tb.node_qq64; b.path = tb.node_path; b.npts = tb.node_npts;
     b.first = tb.node_first;
+    b.seg_first = nullptr;
+    b.batch = false;
     b.mom = moments ? tb.node_mom : nullptr;
     b.pre_mom = tb.pre_mom;
     b.mom_in = mom_in;
@@ -782,6 +808,149 @@ void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, con
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_tree_nodes, dim3(blocks), dim3(256), 0, s, b, ctrl);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batches: many independent systems in one sorted array.  Each system owns a
+// slice of slots aligned to kScanTile; its keys carry the system id above the
+// outside bit, so one sort orders by (system, inside, Morton), prefix sums
+// restart at every slice, and the tree of one system never depends on the
+// others (a system's results are the same alone or in any batch).
+// ---------------------------------------------------------------------------
+constexpr int kBatchSysShift = 2 * kKeyLevels + 1;    // system id above the outside bit
+static_assert(kScanTile == kBatchAlign, "batch slices must align to the prefix-scan tile");
+
+__global__ void k_batch_keys(const float* __restrict__ x, const float* __restrict__ y,
+                             const int32_t* __restrict__ valid, const int32_t* __restrict__ sys, KeyParams kp,
+                             uint64_t* __restrict__ keys, uint32_t* __restrict__ perm, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double px = (double)x[i], py = (double)y[i];
+    const double span_x = (double)(1u << kKeyLevels) / kp.scale_x;
+    const double span_y = (double)(1u << kKeyLevels) / kp.scale_y;
+    const bool inside = valid[i] && px >= kp.xmin && px <= kp.xmin + span_x && py >= kp.ymin && py <= kp.ymin + span_y;
+    uint64_t key = 1ull << (2 * kKeyLevels);
+    if (inside) {
+        const uint32_t cx = cell_index(px, kp.xmin, kp.scale_x, kKeyLevels);
+        const uint32_t cy = cell_index(py, kp.ymin, kp.scale_y, kKeyLevels);
+        key = spread2(cx) | (spread2(cy) << 1);
+    }
+    keys[i] = key | ((uint64_t)sys[i] << kBatchSysShift);
+    perm[i] = (uint32_t)i;
+}
+
+// Per system: points inside the root box (the first outside key of its slice).
+__global__ void k_batch_count(const uint64_t* __restrict__ keys, const int32_t* __restrict__ soff,
+                              int32_t* __restrict__ n_in, int S) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint64_t out_bit = 1ull << (2 * kKeyLevels);
+    int lo = soff[s], hi = soff[s + 1];
+    const int a = lo;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] & out_bit) hi = mid; else lo = mid + 1;
+    }
+    n_in[s] = lo - a;
+}
+
+__global__ void k_batch_levels(const uint64_t* __restrict__ keys, const int32_t* __restrict__ soff,
+                               int32_t* __restrict__ owned, int32_t* __restrict__ seg_first, int total,
+                               Ctrl* __restrict__ ctrl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t k = keys[i];
+    seg_first[i] = soff[k >> kBatchSysShift];
+    if (k & (1ull << (2 * kKeyLevels))) { owned[i] = 0; return; }
+    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k);
+    int D = -1;
+    if (i + kLeafCap < total) {
+        const int cp = cpl_levels(k, keys[i + kLeafCap]);
+        if (cp == kKeyLevels && c < kKeyLevels) atomicAdd((unsigned long long*)&ctrl->depth_overflow, 1ull);
+        D = min(cp, kKeyLevels - 1);
+    }
+    owned[i] = max(0, D - c);
+}
+
+__global__ void k_batch_scatter(const uint64_t* __restrict__ keys, const int32_t* __restrict__ rank_start,
+                                int2* __restrict__ divnode, int64_t cap_div, int total, Ctrl* __restrict__ ctrl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t k = keys[i];
+    if (k & (1ull << (2 * kKeyLevels))) return;
+    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k);
+    int D = -1;
+    if (i + kLeafCap < total) D = min(cpl_levels(k, keys[i + kLeafCap]), kKeyLevels - 1);
+    int r = rank_start[i];
+    for (int d = c + 1; d <= D; d++, r++) {
+        if (r < cap_div) divnode[r] = make_int2(i, d);
+        else ctrl->overflow = 1;
+    }
+}
+
+// The root of each system (quadtree.js:412-419) at slot root_base + s.
+__global__ void k_batch_roots(NodeBuild b, const int32_t* __restrict__ soff, const int32_t* __restrict__ n_in,
+                              int32_t* __restrict__ root_slot, int64_t root_base, int S, Ctrl* __restrict__ ctrl) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int a = soff[s], e = a + n_in[s];
+    bool div = e - a > kLeafCap;
+    int child_base = 0;
+    if (div) {
+        const int r = b.rank_start[a];          // the level-0 node owned by the system's first point
+        if (r >= b.cap_div) { div = false; ctrl->overflow = 1; }
+        else child_base = 1 + 4 * r;
+    }
+    write_node(b, (int)(root_base + s), 0, (uint64_t)s << 1, a, e, e, div, child_base);
+    root_slot[s] = (int32_t)(root_base + s);
+}
+
+void build_tree_batch(hipStream_t s, TreeBufs& tb, const BatchTreeArgs& ba, const float* x, const float* y,
+                      const float* m, const float* wx, const float* wy, const float* mu, const float* L,
+                      const int32_t* qq, KeyParams kp, Ctrl* ctrl) {
+    const int total = ba.total;
+    const dim3 gt((total + 255) / 256), bt(256), gs((ba.S + 255) / 256);
+    hipLaunchKernelGGL(k_batch_count, gs, bt, 0, s, tb.keys, ba.soff, ba.n_in, ba.S);
+    const int nb = (total + kScanTile - 1) / kScanTile;
+    PreIn in{x, y, m, wx, wy, mu, L, qq};
+    const int64_t stride = tb.cap_particles + 1;
+    // FP64 prefix sums with a segmented block-sum scan: each system restarts at 0
+    hipLaunchKernelGGL(k_pre_reduce<PreIn>, dim3(nb), dim3(kScanThreads), 0, s, in, tb.scan_tmp64, nb, ctrl);
+    hipLaunchKernelGGL(k_pre_bsums_seg, gs, bt, 0, s, tb.scan_tmp64, nb, kPreCols, ba.soff, ba.S);
+    hipLaunchKernelGGL(k_pre_apply<PreIn>, dim3(nb), dim3(kScanThreads), 0, s, in, tb.scan_tmp64, nb, tb.pre, stride,
+                       ctrl);
+    hipLaunchKernelGGL(k_batch_levels, gt, bt, 0, s, tb.keys, ba.soff, tb.owned, ba.seg_first, total, ctrl);
+    exclusive_scan_i32(s, tb.owned, tb.rank_start, total, &ctrl->n_tree, &ctrl->n_divided, tb.scan_tmp);
+    hipLaunchKernelGGL(k_batch_scatter, gt, bt, 0, s, tb.keys, tb.rank_start, tb.divnode, tb.cap_divided, total, ctrl);
+    NodeBuild b;
+    b.keys = tb.keys; b.rank_start = tb.rank_start; b.divnode = tb.divnode;
+    b.pre = tb.pre; b.stride = stride; b.pts = in;
+    b.nodes = tb.nodes; b.mass64 = tb.node_mass64; b.comx64 = tb.node_comx64; b.comy64 = tb.node_comy64;
+    b.qq64 = tb.node_qq64; b.path = tb.node_path; b.npts = tb.node_npts;
+    b.first = tb.node_first;
+    b.seg_first = ba.seg_first;
+    b.batch = true;
+    b.mom = nullptr;
+    b.pre_mom = nullptr;
+    b.root_size = 2.0 * kp.root_hw;
+    b.xmin = kp.xmin;
+    b.ymin = kp.ymin;
+    b.aspect_x = 1.0;
+    b.aspect_y = (kp.scale_x / kp.scale_y);
+    b.cap_div = tb.cap_divided;
+    hipLaunchKernelGGL(k_batch_roots, gs, bt, 0, s, b, ba.soff, ba.n_in, ba.root_slot, 1 + 4 * tb.cap_divided, ba.S,
+                       ctrl);
+    int blocks = (int)((tb.cap_divided + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_tree_nodes, dim3(blocks), dim3(256), 0, s, b, ctrl);
+}
+
+void launch_batch_keys(hipStream_t s, const float* x, const float* y, const int32_t* valid, const int32_t* sys,
+                       KeyParams kp, uint64_t* keys, uint32_t* perm, int total) {
+    hipLaunchKernelGGL(k_batch_keys, dim3((total + 255) / 256), dim3(256), 0, s, x, y, valid, sys, kp, keys, perm,
+                       total);
 }
 
 }  // namespace geon

@@ -309,3 +309,94 @@ class HipPhysics:
         col = lambda k, d=0.0: np.array([p.get(k, d) for p in ps], dtype=np.float64)
         self.upload(col("x"), col("y"), col("wx"), col("wy"), col("mass", 1.0), col("charge"), col("angw"),
                     col("baseMass") if ps and "baseMass" in ps[0] else None)
+
+
+class HipBatch:
+    """Many independent systems sharing one GeonParams, stepped together
+    (include/geon.h, geon_batch_*): each system keeps the semantics of one
+    Physics.update per call; results of a system do not depend on the batch."""
+
+    def __init__(self, params, max_particles):
+        self.lib = load_library()
+        self.params = params
+        caps = np.ascontiguousarray(max_particles, dtype=np.int32)
+        self.n_systems = len(caps)
+        h = C.c_void_p()
+        rc = self.lib.geon_batch_create(C.byref(params), self.n_systems, caps.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        C.byref(h))
+        if rc != 0:
+            raise GeonError(rc, self.lib.geon_batch_last_error(None).decode())
+        self.h = h
+        self.caps = caps
+
+    def destroy(self):
+        if self.h:
+            self.lib.geon_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise GeonError(rc, self.lib.geon_batch_last_error(self.h).decode())
+        return rc
+
+    def upload(self, system, x, y, wx, wy, mass, charge, angw=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (x, y, wx, wy, mass, None, charge, angw)]
+        v = A.GeonSoAView(*[_dptr(a) for a in arrs], None)
+        self._check(self.lib.geon_batch_upload(self.h, int(system), C.byref(v), len(arrs[0])))
+        self._keep = arrs
+
+    def download(self, system):
+        cap = int(self.caps[system])
+        out = {k: np.zeros(cap) for k in ("x", "y", "wx", "wy", "mass", "charge", "angw")}
+        alive = np.zeros(cap, dtype=np.int32)
+        v = A.GeonSoAView(_dptr(out["x"]), _dptr(out["y"]), _dptr(out["wx"]), _dptr(out["wy"]), _dptr(out["mass"]),
+                          None, _dptr(out["charge"]), _dptr(out["angw"]), alive.ctypes.data_as(C.POINTER(C.c_int32)))
+        ns = C.c_int32()
+        self._check(self.lib.geon_batch_download(self.h, int(system), C.byref(v), C.byref(ns)))
+        n = ns.value
+        res = {k: a[:n] for k, a in out.items()}
+        res["alive"] = alive[:n].astype(bool)
+        return res
+
+    def update(self, dt):
+        """One Physics.update(dt) of every system; returns each system's substeps."""
+        sub = np.zeros(self.n_systems, dtype=np.int32)
+        self._check(self.lib.geon_batch_update(self.h, float(dt), sub.ctypes.data_as(C.POINTER(C.c_int32))))
+        return sub
+
+    def compute_forces(self):
+        self._check(self.lib.geon_batch_compute_forces(self.h))
+
+    def forces(self, system, n_slots):
+        names = ("fx", "fy", "bz", "dbzdx", "dbzdy", "bgz", "dbgzdx", "dbgzdy", "frame_drag_torque",
+                 "tidal_torque", "pe", "f1pn_x", "f1pn_y")
+        out = {k: np.zeros(n_slots) for k in names}
+        v = A.GeonForceView(*[_dptr(out[k]) for k in names])
+        self._check(self.lib.geon_batch_get_forces(self.h, int(system), C.byref(v), int(n_slots)))
+        return out
+
+    def stats(self, system):
+        st = A.GeonStats()
+        self._check(self.lib.geon_batch_get_stats(self.h, int(system), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in A.GeonStats._fields_}
+
+    def autotune(self):
+        t = A.GeonBatchTune()
+        self._check(self.lib.geon_batch_autotune(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in A.GeonBatchTune._fields_}
+
+    def tune(self, **kw):
+        t = A.GeonBatchTune()
+        self._check(self.lib.geon_batch_get_tune(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        if kw:
+            self._check(self.lib.geon_batch_set_tune(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in A.GeonBatchTune._fields_}

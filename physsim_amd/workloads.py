@@ -145,3 +145,19 @@ def plummer_charged(n=65536, replicate=0, a=20.0, side=512.0):
     prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, boundary_mode=A.BOUND_DESPAWN,
                domain_w=float(side), domain_h=float(side), softening=8.0, theta=0.3)
     return st, prm
+
+
+def config4(n_systems=512, replicate=0, n_min=256, n_max=16384):
+    """Batched ensemble: n_systems independent config-1 disks (gravity +
+    Coulomb, softening 0.01, theta 0.5, dt 1e-3, despawn) with N_s = floor(n_min
+    (n_max / n_min)^U), log-uniform in [n_min, n_max].  Returns the per-system
+    states, the shared params and dt."""
+    g = rng(4, replicate)
+    u = g.random(n_systems)
+    sizes = np.minimum(np.floor(n_min * (n_max / n_min) ** u).astype(int), n_max)
+    states = []
+    prm = None
+    for s, n in enumerate(sizes):
+        st, prm, dt = config1(int(n), replicate=1000 * (replicate + 1) + s)
+        states.append(st)
+    return states, prm, dt
