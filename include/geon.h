@@ -300,7 +300,8 @@ const char* geon_last_error(GeonHandle h);
  * Many independent systems that share one GeonParams, each with the semantics
  * of one Physics.update per call (its own adaptive substeps, despawn once per
  * update), stepped together: every substep of all systems is one pass of the
- * kernels, and the max_substeps bodies are replayed from one captured graph.
+ * kernels, replayed from captured graphs of four substep bodies until no
+ * system can take another substep.
  * A system's results do not depend on the other systems of the batch.  Runs
  * gravity, Coulomb, magnetic, gravitomagnetic, Yukawa, relativity and
  * spin-orbit with the Barnes-Hut walk (GEON_EUNSUPPORTED otherwise). */
@@ -309,7 +310,7 @@ typedef struct GeonBatchTune {
     int32_t walk_threads;       /* persistent walk block size: 64, 128 or 256 */
     int32_t waves_per_cu;       /* persistent walk grid: resident waves per compute unit */
     int32_t stack;              /* walk node-stack entries per wave: 64 or 96 */
-    int32_t use_graph;          /* 1: replay the captured substep loop (even max_substeps) */
+    int32_t use_graph;          /* 1: replay the substep loop from captured graphs of 4 bodies */
     double  walk_ms;            /* geon_batch_autotune: the chosen configuration's walk time */
 } GeonBatchTune;
 int  geon_batch_create(const GeonParams* params, int32_t n_systems, const int32_t* max_particles,

@@ -8,9 +8,10 @@
 // under each system's dt, one sort by (system, Morton key), one tree build
 // whose prefix sums restart at every slice, and a persistent walk whose waves
 // take 64-particle groups from a queue (a wave never crosses a slice).  The
-// max_substeps bodies are captured once into a hipGraph and replayed; systems
-// that finished their update are predicated off inside the kernels.  A
-// system's results do not depend on the other systems in the batch.
+// substep bodies are captured into hipGraphs of four and replayed until no
+// system can take another substep; systems that finished their update are
+// predicated off inside the kernels (sort and tree skip entirely once none is
+// active).  A system's results do not depend on the other systems in the batch.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -33,6 +34,7 @@ enum BForce {
 };
 constexpr int kBatchKeyBits = 2 * kKeyLevels + 1 + 9 + 1;   // Morton + outside bit + system id (< 2^10)
 constexpr int kMaxSystems = 1 << 10;
+constexpr int kChunk = 4;           // substep bodies per captured graph (even: the state ping-pong returns)
 
 thread_local std::string g_batch_create_error;
 
@@ -52,11 +54,15 @@ struct GeonBatch_ {
     Ctrl* bctrl_host = nullptr;     // pinned mirror
     Ctrl* gctrl = nullptr;          // whole batch (counts, counters, overflow)
     Ctrl* gctrl_host = nullptr;
+    Ctrl* actrl = nullptr;          // counts gated on any system being active (sort, tree)
     int32_t *d_soff = nullptr, *n_in = nullptr, *seg_first = nullptr, *root_slot = nullptr, *queue = nullptr;
     double* stats = nullptr;        // 10 per system
-    hipGraphExec_t graph = nullptr;
-    int graph_substeps = 0;
-    uint32_t* graph_st0 = nullptr;  // the state buffer the graph starts from (ping-pong parity)
+    // the substep loop as captured chunks of kChunk bodies: the first chunk
+    // (controller's first-substep rule) and the one replayed after it
+    hipGraphExec_t graph = nullptr, graph_next = nullptr;
+    uint32_t* graph_st0 = nullptr;  // the state buffer the graphs start from (ping-pong parity)
+    int32_t* pending = nullptr;     // device flag: a system still inside its update
+    int32_t* pending_host = nullptr;
     GeonBatchTune tune = {};
     int cus = 0;
     bool forces_init = false;
@@ -227,14 +233,43 @@ __global__ void k_batch_activate(Ctrl* c, int S) {
     if (s < S) c[s].active = 1;
 }
 
+// *flag = 1 if a system can still take a substep (integrator.js:810-812 loop test).
+__global__ __launch_bounds__(1024) void k_batch_pending(const Ctrl* __restrict__ c, int S, int32_t* flag) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < S; s += blockDim.x)
+        if (c[s].total_steps < c[s].max_substeps && c[s].dt_remain > 1e-12) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) *flag = any;
+}
+
+// a->n_alive = a->n_tree = total while any system is inside its update, else 0.
+__global__ __launch_bounds__(1024) void k_batch_any(const Ctrl* __restrict__ c, int S, Ctrl* __restrict__ a, int total) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < S; s += blockDim.x)
+        if (c[s].active) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a->n_alive = any ? total : 0;
+        a->n_tree = any ? total : 0;
+        a->active = any;
+    }
+}
+
 // Sort by (system, Morton key), gather the state, build every system's tree,
 // then the persistent walk.  Launches only: it is captured into the graph.
 void force_pass(GeonBatch_* h, bool timed = false) {
     const hipStream_t s = h->stream;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, h->gctrl, h->queue);
+    hipLaunchKernelGGL(k_batch_any, dim3(1), dim3(1024), 0, s, h->bctrl, h->S, h->actrl, h->total);
     const KeyParams kp = keyp(h->prm);
+    // keys and an identity permutation always; the sort only while a system is
+    // active, so idle substeps gather with the identity
     launch_batch_keys(s, F(h, B_X), F(h, B_Y), I(h, B_VALID), I(h, B_SYS), kp, h->tb.keys, h->tb.perm, h->total);
-    radix_sort(s, h->tb, h->total, kBatchKeyBits, h->gctrl);
+    radix_sort(s, h->tb, h->total, kBatchKeyBits, h->actrl);
     Columns cols;
     cols.ncols = NB_STATE;
     for (int c = 0; c < NB_STATE; c++) { cols.src[c] = h->st[c]; cols.dst[c] = h->st_alt[c]; }
@@ -244,10 +279,10 @@ void force_pass(GeonBatch_* h, bool timed = false) {
     ba.S = h->S; ba.total = h->total; ba.soff = h->d_soff; ba.n_in = h->n_in; ba.seg_first = h->seg_first;
     ba.root_slot = h->root_slot;
     build_tree_batch(s, h->tb, ba, F(h, B_X), F(h, B_Y), F(h, B_MASS), F(h, B_WX), F(h, B_WY), F(h, B_MU), F(h, B_L),
-                     I(h, B_QQ), kp, h->gctrl);
+                     I(h, B_QQ), kp, h->actrl);
     launch_prepare_sources(s, F(h, B_X), F(h, B_Y), F(h, B_MASS), F(h, B_CHARGE), F(h, B_VX), F(h, B_VY),
                            F(h, B_ANGVEL), F(h, B_RADIUS), F(h, B_MU), F(h, B_L), nullptr, nullptr, nullptr, h->src4,
-                           h->src8a, h->src8b, h->total, h->gctrl);
+                           h->src8a, h->src8b, h->total, h->actrl);
     const int threads = h->tune.walk_threads;
     const int grid = std::max(1, h->cus * h->tune.waves_per_cu / (threads / kWave));
     if (timed) hipEventRecord(h->ev0, s);
@@ -267,28 +302,47 @@ void substep(GeonBatch_* h, int k) {
 int read_ctrls(GeonBatch_* h) {
     HIP_OK(hipMemcpyAsync(h->bctrl_host, h->bctrl, sizeof(Ctrl) * h->S, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipMemcpyAsync(h->gctrl_host, h->gctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    Ctrl a;
+    HIP_OK(hipMemcpyAsync(&a, h->actrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
-    if (h->gctrl_host->overflow) return set_err(h, GEON_EOVERFLOW, "tree node pool or walk stack overflow");
+    if (h->gctrl_host->overflow || a.overflow) return set_err(h, GEON_EOVERFLOW, "tree node pool or walk stack overflow");
     return GEON_OK;
 }
 
 void drop_graph(GeonBatch_* h) {
     if (h->graph) hipGraphExecDestroy(h->graph);
+    if (h->graph_next) hipGraphExecDestroy(h->graph_next);
     h->graph = nullptr;
+    h->graph_next = nullptr;
+}
+
+int capture_chunk(GeonBatch_* h, int first_body, hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kChunk; k++) substep(h, first_body + k);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess) return set_err(h, GEON_EDEVICE, std::string("substep graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) { *out = nullptr; return set_err(h, GEON_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+    return GEON_OK;
 }
 
 int capture(GeonBatch_* h) {
     drop_graph(h);
-    hipGraph_t g = nullptr;
-    HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < h->prm.max_substeps; k++) substep(h, k);
-    hipError_t e = hipStreamEndCapture(h->stream, &g);
-    if (e != hipSuccess) return set_err(h, GEON_EDEVICE, std::string("substep graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (e != hipSuccess) { h->graph = nullptr; return set_err(h, GEON_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
-    h->graph_substeps = h->prm.max_substeps;
+    int rc;
+    if ((rc = capture_chunk(h, 0, &h->graph))) return rc;
+    if ((rc = capture_chunk(h, kChunk, &h->graph_next))) { drop_graph(h); return rc; }
     h->graph_st0 = h->st[0];
+    return GEON_OK;
+}
+
+// Are there systems left inside their update?  (one small kernel + a pinned read)
+int pending(GeonBatch_* h, bool* any) {
+    hipLaunchKernelGGL(k_batch_pending, dim3(1), dim3(1024), 0, h->stream, h->bctrl, h->S, h->pending);
+    HIP_OK(hipMemcpyAsync(h->pending_host, h->pending, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    *any = *h->pending_host != 0;
     return GEON_OK;
 }
 
@@ -357,12 +411,15 @@ int geon_batch_create(const GeonParams* params, int32_t n_systems, const int32_t
     tb.node_mom = nullptr;
     tb.pre_mom = nullptr;
     if ((rc = dmalloc(h, &h->bctrl, (size_t)n_systems)) || (rc = dmalloc(h, &h->gctrl, 1)) ||
+        (rc = dmalloc(h, &h->actrl, 1)) ||
         (rc = dmalloc(h, &h->d_soff, (size_t)n_systems + 1)) || (rc = dmalloc(h, &h->n_in, (size_t)n_systems)) ||
         (rc = dmalloc(h, &h->seg_first, T)) || (rc = dmalloc(h, &h->root_slot, (size_t)n_systems)) ||
-        (rc = dmalloc(h, &h->queue, 1)) || (rc = dmalloc(h, &h->stats, 10 * (size_t)n_systems)))
+        (rc = dmalloc(h, &h->queue, 1)) || (rc = dmalloc(h, &h->stats, 10 * (size_t)n_systems)) ||
+        (rc = dmalloc(h, &h->pending, 1)))
         return fail(rc);
     if (hipHostMalloc((void**)&h->bctrl_host, sizeof(Ctrl) * n_systems, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h->gctrl_host, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void**)&h->gctrl_host, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h->pending_host, sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return fail(GEON_ENOMEM);
     if (hipMemset(h->bctrl, 0, sizeof(Ctrl) * n_systems) != hipSuccess) return fail(GEON_EDEVICE);
     Ctrl g;
@@ -370,7 +427,9 @@ int geon_batch_create(const GeonParams* params, int32_t n_systems, const int32_t
     g.n_alive = (int32_t)total;
     g.n_tree = (int32_t)total;
     g.active = 1;
-    if (hipMemcpy(h->gctrl, &g, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess) return fail(GEON_EDEVICE);
+    if (hipMemcpy(h->gctrl, &g, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->actrl, &g, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(GEON_EDEVICE);
     // slot -> system, all slots empty
     std::vector<int32_t> sys(T);
     for (int s = 0; s < n_systems; s++)
@@ -401,8 +460,9 @@ void geon_batch_destroy(GeonBatchHandle h) {
     hipFree(tb.nodes); hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64);
     hipFree(tb.node_qq64); hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.node_first);
     hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
-    hipFree(h->bctrl); hipFree(h->gctrl); hipFree(h->d_soff); hipFree(h->n_in); hipFree(h->seg_first);
-    hipFree(h->root_slot); hipFree(h->queue); hipFree(h->stats);
+    hipFree(h->bctrl); hipFree(h->gctrl); hipFree(h->actrl); hipFree(h->d_soff); hipFree(h->n_in); hipFree(h->seg_first);
+    hipFree(h->root_slot); hipFree(h->queue); hipFree(h->stats); hipFree(h->pending);
+    if (h->pending_host) hipHostFree(h->pending_host);
     if (h->bctrl_host) hipHostFree(h->bctrl_host);
     if (h->gctrl_host) hipHostFree(h->gctrl_host);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -500,17 +560,26 @@ int geon_batch_update(GeonBatchHandle h, double dt, int32_t* substeps) {
     }
     launch_batch_begin(h->stream, h->bctrl, h->S, dt, h->prm.max_substeps);
     launch_batch_dt_scan(h->stream, step_args(h), bstep(h));
-    // a replayed graph must leave the state in the buffers it started from:
-    // an even number of sorts (one per substep)
-    if (h->tune.use_graph && h->prm.max_substeps % 2 == 0) {
-        // a force pass outside the graph (bootstrap, compute_forces) flips the ping-pong
-        if (!h->graph || h->graph_substeps != h->prm.max_substeps || h->graph_st0 != h->st[0]) {
-            HIP_OK(hipStreamSynchronize(h->stream));
-            if ((rc = capture(h))) return rc;
+    // The substep loop in chunks of kChunk bodies (systems that finished are
+    // predicated off inside the kernels); after each chunk the host stops once
+    // no system can take another substep.
+    const int chunks = (h->prm.max_substeps + kChunk - 1) / kChunk;
+    for (int c = 0; c < chunks; c++) {
+        if (c > 0) {
+            bool any = false;
+            if ((rc = pending(h, &any))) return rc;
+            if (!any) break;
         }
-        HIP_OK(hipGraphLaunch(h->graph, h->stream));
-    } else {
-        for (int k = 0; k < h->prm.max_substeps; k++) substep(h, k);
+        if (h->tune.use_graph) {
+            // a force pass outside the graphs (bootstrap, compute_forces) flips the ping-pong
+            if (!h->graph || h->graph_st0 != h->st[0]) {
+                HIP_OK(hipStreamSynchronize(h->stream));
+                if ((rc = capture(h))) return rc;
+            }
+            HIP_OK(hipGraphLaunch(c == 0 ? h->graph : h->graph_next, h->stream));
+        } else {
+            for (int k = 0; k < kChunk; k++) substep(h, c * kChunk + k);
+        }
     }
     launch_batch_boundary(h->stream, step_args(h), bstep(h), h->prm.boundary_mode, h->prm.domain_w, h->prm.domain_h);
     HIP_OK(hipGetLastError());
@@ -601,6 +670,7 @@ int geon_batch_autotune(GeonBatchHandle h, GeonBatchTune* out) {
                 int rc = read_ctrls(h);
                 if (rc == GEON_EOVERFLOW) {                 // this stack is too small: not a candidate
                     HIP_OK(hipMemsetAsync(&h->gctrl->overflow, 0, sizeof(int32_t), h->stream));
+                    HIP_OK(hipMemsetAsync(&h->actrl->overflow, 0, sizeof(int32_t), h->stream));
                     continue;
                 }
                 if (rc) return rc;

@@ -459,9 +459,9 @@ __global__ __launch_bounds__(1024) void k_pre_bsums(double* __restrict__ bsum, i
 // Exclusive scan of the block sums restarting at every system of a batch
 // (slices aligned to kScanTile): one thread per system, fixed order.
 __global__ void k_pre_bsums_seg(double* __restrict__ bsum, int nb, int ncols, const int32_t* __restrict__ soff,
-                                int S) {
+                                int S, const Ctrl* __restrict__ ctrl) {
     const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sidx >= S) return;
+    if (sidx >= S || ctrl->n_tree == 0) return;
     const int b0 = soff[sidx] / kScanTile, b1 = min(nb, (soff[sidx + 1] + kScanTile - 1) / kScanTile);
     for (int c = 0; c < ncols; c++) {
         double run = 0.0;
@@ -842,9 +842,9 @@ __global__ void k_batch_keys(const float* __restrict__ x, const float* __restric
 
 // Per system: points inside the root box (the first outside key of its slice).
 __global__ void k_batch_count(const uint64_t* __restrict__ keys, const int32_t* __restrict__ soff,
-                              int32_t* __restrict__ n_in, int S) {
+                              int32_t* __restrict__ n_in, int S, const Ctrl* __restrict__ ctrl) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
+    if (s >= S || ctrl->n_tree == 0) return;       // no system active: nothing to build
     const uint64_t out_bit = 1ull << (2 * kKeyLevels);
     int lo = soff[s], hi = soff[s + 1];
     const int a = lo;
@@ -856,8 +856,8 @@ __global__ void k_batch_count(const uint64_t* __restrict__ keys, const int32_t* 
 }
 
 __global__ void k_batch_levels(const uint64_t* __restrict__ keys, const int32_t* __restrict__ soff,
-                               int32_t* __restrict__ owned, int32_t* __restrict__ seg_first, int total,
-                               Ctrl* __restrict__ ctrl) {
+                               int32_t* __restrict__ owned, int32_t* __restrict__ seg_first, Ctrl* __restrict__ ctrl) {
+    const int total = ctrl->n_tree;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const uint64_t k = keys[i];
@@ -874,7 +874,8 @@ __global__ void k_batch_levels(const uint64_t* __restrict__ keys, const int32_t*
 }
 
 __global__ void k_batch_scatter(const uint64_t* __restrict__ keys, const int32_t* __restrict__ rank_start,
-                                int2* __restrict__ divnode, int64_t cap_div, int total, Ctrl* __restrict__ ctrl) {
+                                int2* __restrict__ divnode, int64_t cap_div, Ctrl* __restrict__ ctrl) {
+    const int total = ctrl->n_tree;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const uint64_t k = keys[i];
@@ -893,7 +894,7 @@ __global__ void k_batch_scatter(const uint64_t* __restrict__ keys, const int32_t
 __global__ void k_batch_roots(NodeBuild b, const int32_t* __restrict__ soff, const int32_t* __restrict__ n_in,
                               int32_t* __restrict__ root_slot, int64_t root_base, int S, Ctrl* __restrict__ ctrl) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
+    if (s >= S || ctrl->n_tree == 0) return;
     const int a = soff[s], e = a + n_in[s];
     bool div = e - a > kLeafCap;
     int child_base = 0;
@@ -906,23 +907,25 @@ __global__ void k_batch_roots(NodeBuild b, const int32_t* __restrict__ soff, con
     root_slot[s] = (int32_t)(root_base + s);
 }
 
+// ctrl->n_tree is the slot count, or 0 when no system is active (then every
+// kernel of the build returns at once: the graph replays idle substeps cheaply).
 void build_tree_batch(hipStream_t s, TreeBufs& tb, const BatchTreeArgs& ba, const float* x, const float* y,
                       const float* m, const float* wx, const float* wy, const float* mu, const float* L,
                       const int32_t* qq, KeyParams kp, Ctrl* ctrl) {
     const int total = ba.total;
     const dim3 gt((total + 255) / 256), bt(256), gs((ba.S + 255) / 256);
-    hipLaunchKernelGGL(k_batch_count, gs, bt, 0, s, tb.keys, ba.soff, ba.n_in, ba.S);
+    hipLaunchKernelGGL(k_batch_count, gs, bt, 0, s, tb.keys, ba.soff, ba.n_in, ba.S, ctrl);
     const int nb = (total + kScanTile - 1) / kScanTile;
     PreIn in{x, y, m, wx, wy, mu, L, qq};
     const int64_t stride = tb.cap_particles + 1;
     // FP64 prefix sums with a segmented block-sum scan: each system restarts at 0
     hipLaunchKernelGGL(k_pre_reduce<PreIn>, dim3(nb), dim3(kScanThreads), 0, s, in, tb.scan_tmp64, nb, ctrl);
-    hipLaunchKernelGGL(k_pre_bsums_seg, gs, bt, 0, s, tb.scan_tmp64, nb, kPreCols, ba.soff, ba.S);
+    hipLaunchKernelGGL(k_pre_bsums_seg, gs, bt, 0, s, tb.scan_tmp64, nb, kPreCols, ba.soff, ba.S, ctrl);
     hipLaunchKernelGGL(k_pre_apply<PreIn>, dim3(nb), dim3(kScanThreads), 0, s, in, tb.scan_tmp64, nb, tb.pre, stride,
                        ctrl);
-    hipLaunchKernelGGL(k_batch_levels, gt, bt, 0, s, tb.keys, ba.soff, tb.owned, ba.seg_first, total, ctrl);
+    hipLaunchKernelGGL(k_batch_levels, gt, bt, 0, s, tb.keys, ba.soff, tb.owned, ba.seg_first, ctrl);
     exclusive_scan_i32(s, tb.owned, tb.rank_start, total, &ctrl->n_tree, &ctrl->n_divided, tb.scan_tmp);
-    hipLaunchKernelGGL(k_batch_scatter, gt, bt, 0, s, tb.keys, tb.rank_start, tb.divnode, tb.cap_divided, total, ctrl);
+    hipLaunchKernelGGL(k_batch_scatter, gt, bt, 0, s, tb.keys, tb.rank_start, tb.divnode, tb.cap_divided, ctrl);
     NodeBuild b;
     b.keys = tb.keys; b.rank_start = tb.rank_start; b.divnode = tb.divnode;
     b.pre = tb.pre; b.stride = stride; b.pts = in;
