@@ -95,6 +95,32 @@ def train_far_model(theta, seeds, n, stride, steps, batch):
     return model, info
 
 
+def run_batch(n_systems, steps):
+    """Batched ensemble (config 4): n_systems config-1 disks, N log-uniform in
+    [256, 16384], gravity + Coulomb, theta 0.5, dt 1e-3 per update, stepped
+    together (captured substep graphs, persistent walk with autotuned tile and
+    stack).  systems-steps/s = systems x updates / wall time."""
+    from physsim_amd import HipBatch, default_params, workloads as W
+    states, prm, dt = W.config4(n_systems)
+    sizes = np.array([len(s["x"]) for s in states])
+    b = HipBatch(default_params(**prm), sizes)
+    for k, st in enumerate(states):
+        b.upload(k, st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    b.compute_forces()
+    tune = b.autotune()
+    b.update(dt)                                  # warmup (captures the graphs)
+    t0 = time.perf_counter()
+    subs = np.array([b.update(dt) for _ in range(steps)])
+    wall = time.perf_counter() - t0
+    log("batch: %d systems, %.3e systems-steps/s" % (n_systems, n_systems * steps / wall))
+    return {"metric": "systems-steps/sec", "value": n_systems * steps / wall, "systems": n_systems,
+            "particles": int(sizes.sum()), "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "particle_steps_per_s": float((subs * sizes[None, :]).sum() / wall),
+            "substeps_mean": float(subs.mean()), "substeps_max": int(subs.max()), "tune": tune,
+            "workload": "config-1 disks, N = floor(256 * 64^U), gravity + Coulomb, softening 0.01, theta 0.5, "
+                        "dt 1e-3 per update, despawn"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +139,8 @@ def main():
     ap.add_argument("--err-sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=512,
+                    help="batched-ensemble leg (config 4): systems; 0 skips it")
     ap.add_argument("--methods", default="plain,learned,multipole",
                     help="comma list of force methods to run (the learned one is always run)")
     args = ap.parse_args()
@@ -234,6 +262,8 @@ def main():
                  "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": L["walk_tflops"] / FP32_PEAK_TFLOPS,
                  "flops_per_interaction": c_f, "ms_avg": L["phases_ms_avg"]["walk"]}
 
+    batch = run_batch(args.batch_systems, args.steps) if args.batch_systems > 0 else None
+
     cpu = None
     if not args.no_cpu:
         # ---- CPU baseline: the reference algorithm in FP64 on this host ----
@@ -279,6 +309,7 @@ def main():
         "training": train_info,
         "roofline": roof,
         "roofline_walk": walk_roof,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

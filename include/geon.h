@@ -260,12 +260,14 @@ int  geon_counters(GeonHandle h, GeonCounters* out);
 /* ---- Learned far-field correction (no counterpart in the reference) -------
  * Model: GEON_FAR_MODEL_FLOATS floats = GEON_FAR_PARAMS FP32 parameters
  * (W1[16][64] whose row 10 is the first layer's bias, W2[64][64], b2[64],
- * W3[64][4], b3[4]) followed by the feature means[10], inverse standard
- * deviations[10] and output scales[4].  Inputs and outputs of one pair are
- * defined in physsim_amd/csrc/k_learn.hip. */
-#define GEON_FAR_PARAMS       5444
-#define GEON_FAR_MODEL_FLOATS 5468
-#define GEON_FAR_SAMPLE       16    /* floats per sample: 10 features, 4 targets, 2 weights */
+ * W3[64][6], b3[6]) followed by the feature means[10], inverse standard
+ * deviations[10] and output scales[6].  The six outputs are the gravity and
+ * Coulomb force residuals (two components each, in the offset frame) and the
+ * gravity and Coulomb potential residuals of one (particle, node) pair; the
+ * inputs and outputs are defined in physsim_amd/csrc/k_learn.hip. */
+#define GEON_FAR_PARAMS       5574
+#define GEON_FAR_MODEL_FLOATS 5600
+#define GEON_FAR_SAMPLE       20    /* floats per sample: 10 features, 6 targets, 4 weights */
 /* Training samples from the current state: builds the tree, lists the pairs
  * the walk accepts, and for every group_stride-th group of 64 particles (in
  * Morton order) appends one sample per listed pair with its exact FP64

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libgeon_oracle.so")
 _LIB = None
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
-FAR_STRIDE = 20
+FAR_STRIDE = 22
 
 
 def build():
@@ -50,8 +50,8 @@ def lib():
         L.orc_far_samples.argtypes = [C.c_void_p, _IP, C.c_int32, _DP, C.c_int64]
         L.orc_far_samples.restype = C.c_int64
         L.orc_far_set_model.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
-        L.orc_far_training_samples.argtypes = [C.c_void_p, _IP, C.c_int32, C.c_double, C.POINTER(C.c_float), _IP,
-                                               C.c_int64]
+        L.orc_far_training_samples.argtypes = [C.c_void_p, _IP, C.c_int32, C.c_double, C.c_double,
+                                               C.POINTER(C.c_float), _IP, C.c_int64]
         L.orc_far_training_samples.restype = C.c_int64
         L.orc_far_loss_grad.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_double, _DP]
         L.orc_far_loss_grad.restype = C.c_double
@@ -67,7 +67,7 @@ def far_loss_grad(model, samples, wscale=1.0):
     """FP64 loss and gradient of the learned far-field network on samples."""
     m = np.ascontiguousarray(model, dtype=np.float32)
     sm = np.ascontiguousarray(samples, dtype=np.float32)
-    g = np.zeros(5444)
+    g = np.zeros(A.FAR_PARAMS)
     L = lib().orc_far_loss_grad(m.ctypes.data_as(C.POINTER(C.c_float)), sm.ctypes.data_as(C.POINTER(C.c_float)),
                                 len(sm), float(wscale), _d(g))
     return L, g
@@ -175,14 +175,14 @@ class Oracle:
         self.L.orc_far_samples(self.s, ii.ctypes.data_as(_IP), len(ii), _d(out), m)
         return out[:m]
 
-    def far_training_samples(self, idx, f2ref=1.0):
-        """Samples of the observers in idx in the 16-float training layout, and
+    def far_training_samples(self, idx, f2ref=1.0, pe2ref=1.0):
+        """Samples of the observers in idx in the 20-float training layout, and
         the observer of each."""
         ii = np.ascontiguousarray(idx, dtype=np.int32)
         m = self.far_samples(ii).shape[0]
-        out = np.zeros((max(m, 1), 16), dtype=np.float32)
+        out = np.zeros((max(m, 1), A.FAR_SAMPLE), dtype=np.float32)
         obs = np.zeros(max(m, 1), dtype=np.int32)
-        self.L.orc_far_training_samples(self.s, ii.ctypes.data_as(_IP), len(ii), float(f2ref),
+        self.L.orc_far_training_samples(self.s, ii.ctypes.data_as(_IP), len(ii), float(f2ref), float(pe2ref),
                                         out.ctypes.data_as(C.POINTER(C.c_float)), obs.ctypes.data_as(_IP), m)
         return out[:m], obs[:m]
 

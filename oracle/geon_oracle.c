@@ -570,7 +570,7 @@ static void pool_moments(Pool* q, const P* tp, int root) {
 }
 
 /* Far-field correction of an accepted node (not in the reference): the
- * gravity and Coulomb terms beyond the monopole of the softened kernel
+ * gravity and Coulomb force and potential terms beyond the monopole of the softened kernel
  * g(d) = d (|d|^2 + eps^2)^(-3/2), with coefficients c[0..7] = mass quadrupole,
  * charge dipole, charge quadrupole (about the COM):
  *   1/2 Q:dd g = 2 (Q d) h' + tr(Q) d h' + 2 d (d.Q.d) h'',  D.dg = D h + 2 d (D.d) h',
@@ -585,6 +585,7 @@ static void far_correction(const OrcSys* s, P* p, double dx, double dy, const do
         const double dqd = dx * qx + dy * qy, tr = c[0] + c[2];
         fx += p->mass * (2 * qx * h1 + tr * dx * h1 + 2 * dx * dqd * h2);
         fy += p->mass * (2 * qy * h1 + tr * dy * h1 + 2 * dy * dqd * h2);
+        p->pe += p->mass * (0.5 * tr * h + dqd * h1);           /* -m_i (1/2) Q : grad grad r2^-1/2 */
     }
     if (s->coul) {
         const double dd = c[3] * dx + c[4] * dy;
@@ -592,6 +593,7 @@ static void far_correction(const OrcSys* s, P* p, double dx, double dy, const do
         const double dqd = dx * qx + dy * qy, tr = c[5] + c[7];
         fx -= p->charge * (c[3] * h + 2 * dx * dd * h1 + 2 * qx * h1 + tr * dx * h1 + 2 * dx * dqd * h2);
         fy -= p->charge * (c[4] * h + 2 * dy * dd * h1 + 2 * qy * h1 + tr * dy * h1 + 2 * dy * dqd * h2);
+        p->pe -= p->charge * (dd * h + 0.5 * tr * h + dqd * h1);
     }
     p->fx += fx; p->fy += fy;
 }
@@ -605,11 +607,13 @@ static void far_correction(const OrcSys* s, P* p, double dx, double dy, const do
 #define FAR_W1 0
 #define FAR_W2 1024
 #define FAR_B2 5120
+#define FAR_OUT 6
 #define FAR_W3 5184
-#define FAR_B3 5440
-#define FAR_PARAMS 5444
-#define FAR_MODEL 5468
-#define FAR_SAMPLE 16
+#define FAR_B3 5568
+#define FAR_PARAMS 5574
+#define FAR_MODEL 5600
+#define FAR_SAMPLE 20
+#define FAR_WSEL(j) ((j) < 4 ? (j) >> 1 : (j) - 2)
 
 static double gelu64(double z) {
     const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
@@ -626,7 +630,7 @@ static double gelu64_grad(double z) {
  * v = (-u_y, u_x); c = the oracle's 9 node moments (Qm xx xy yy, Dq x y,
  * Qq xx xy yy, sum |q|).  sg0/sc0: the output scales without m_i, q_i. */
 static void far_features(double dx, double dy, double eps2, double s, double M, const double* c, double* f,
-                         double* ux, double* uy, double* sg0, double* sc0) {
+                         double* ux, double* uy, double* sg0, double* sc0, double* sp0, double* cp0) {
     const double rho2 = dx * dx + dy * dy, r2 = rho2 + eps2, r = sqrt(r2), irho = 1 / sqrt(rho2);
     const double x = dx * irho, y = dy * irho;
     const double sr = s / r;
@@ -645,6 +649,8 @@ static void far_features(double dx, double dy, double eps2, double s, double M, 
     *ux = x; *uy = y;
     *sg0 = M * sr * sr / r2;
     *sc0 = A * sr / r2;
+    *sp0 = M * sr * sr / r;
+    *cp0 = A * sr / r;
 }
 
 /* Forward pass; keeps the pre-activations when z1/z2 are given. */
@@ -665,9 +671,9 @@ static void far_forward(const double* P, const double* feat, double* x, double* 
         for (int a = 0; a < FAR_HID; a++) z += a1[a] * P[FAR_W2 + a * FAR_HID + b];
         z2[b] = z; a2[b] = gelu64(z);
     }
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < FAR_OUT; j++) {
         double z = P[FAR_B3 + j];
-        for (int b = 0; b < FAR_HID; b++) z += a2[b] * P[FAR_W3 + b * 4 + j];
+        for (int b = 0; b < FAR_HID; b++) z += a2[b] * P[FAR_W3 + b * FAR_OUT + j];
         o[j] = z;
     }
 }
@@ -676,14 +682,17 @@ static void far_forward(const double* P, const double* feat, double* x, double* 
 static void far_learned(const OrcSys* s, P* p, double dx, double dy, int ni) {
     const Pool* q = &s->pool;
     const double* P = s->farModel;
-    double f[FAR_IN], ux, uy, sg0, sc0, x[16], z1[FAR_HID], a1[FAR_HID], z2[FAR_HID], a2[FAR_HID], o[4];
-    far_features(dx, dy, s->softeningSq, q->bw[ni] * 2, q->mass[ni], &q->mom[(size_t)ni * 9], f, &ux, &uy, &sg0, &sc0);
+    double f[FAR_IN], ux, uy, sg0, sc0, sp0, cp0, x[16], z1[FAR_HID], a1[FAR_HID], z2[FAR_HID], a2[FAR_HID], o[FAR_OUT];
+    far_features(dx, dy, s->softeningSq, q->bw[ni] * 2, q->mass[ni], &q->mom[(size_t)ni * 9], f, &ux, &uy, &sg0, &sc0,
+                 &sp0, &cp0);
     far_forward(P, f, x, z1, a1, z2, a2, o);
     const double* os = P + FAR_PARAMS + 2 * FAR_IN;
     const double gu = o[0] * os[0], gv = o[1] * os[1], cu = o[2] * os[2], cv = o[3] * os[3];
     const double sg = s->grav ? p->mass * sg0 : 0, sc = s->coul ? p->charge * sc0 : 0;
+    const double sp = s->grav ? p->mass * sp0 : 0, cp = s->coul ? p->charge * cp0 : 0;
     p->fx += sg * (ux * gu - uy * gv) + sc * (ux * cu - uy * cv);
     p->fy += sg * (uy * gu + ux * gv) + sc * (uy * cu + ux * cv);
+    p->pe += sp * o[4] * os[4] + cp * o[5] * os[5];
 }
 
 int orc_far_set_model(OrcSys* s, const float* model) {
@@ -703,11 +712,11 @@ double orc_far_loss_grad(const float* model, const float* samples, int64_t n, do
     double L = 0;
     for (int64_t r = 0; r < n; r++) {
         const float* sm = samples + r * FAR_SAMPLE;
-        double feat[FAR_IN], x[16], z1[FAR_HID], a1[FAR_HID], z2[FAR_HID], a2[FAR_HID], o[4], dO[4];
+        double feat[FAR_IN], x[16], z1[FAR_HID], a1[FAR_HID], z2[FAR_HID], a2[FAR_HID], o[FAR_OUT], dO[FAR_OUT];
         for (int k = 0; k < FAR_IN; k++) feat[k] = sm[k];
         far_forward(P, feat, x, z1, a1, z2, a2, o);
-        for (int j = 0; j < 4; j++) {
-            const double w = wscale * sm[FAR_IN + 4 + (j >> 1)];
+        for (int j = 0; j < FAR_OUT; j++) {
+            const double w = wscale * sm[FAR_IN + FAR_OUT + FAR_WSEL(j)];
             const double err = os[j] * o[j] - sm[FAR_IN + j];
             L += w * err * err / n;
             dO[j] = 2 * w * err * os[j] / n;
@@ -715,11 +724,14 @@ double orc_far_loss_grad(const float* model, const float* samples, int64_t n, do
         double dz2[FAR_HID], dz1[FAR_HID];
         for (int b = 0; b < FAR_HID; b++) {
             double da = 0;
-            for (int j = 0; j < 4; j++) { da += P[FAR_W3 + b * 4 + j] * dO[j]; grad[FAR_W3 + b * 4 + j] += a2[b] * dO[j]; }
+            for (int j = 0; j < FAR_OUT; j++) {
+                da += P[FAR_W3 + b * FAR_OUT + j] * dO[j];
+                grad[FAR_W3 + b * FAR_OUT + j] += a2[b] * dO[j];
+            }
             dz2[b] = da * gelu64_grad(z2[b]);
             grad[FAR_B2 + b] += dz2[b];
         }
-        for (int j = 0; j < 4; j++) grad[FAR_B3 + j] += dO[j];
+        for (int j = 0; j < FAR_OUT; j++) grad[FAR_B3 + j] += dO[j];
         for (int a = 0; a < FAR_HID; a++) {
             double da = 0;
             for (int b = 0; b < FAR_HID; b++) { da += P[FAR_W2 + a * FAR_HID + b] * dz2[b]; grad[FAR_W2 + a * FAR_HID + b] += a1[a] * dz2[b]; }
@@ -884,7 +896,7 @@ static void node_exact(const OrcSys* s, const Pool* q, int ni, double dx, double
                        int32_t* stack, double* g) {
     int top = 0;
     stack[top++] = ni;
-    g[0] = g[1] = g[2] = g[3] = 0;
+    g[0] = g[1] = g[2] = g[3] = g[4] = g[5] = 0;
     while (top > 0) {
         const int k = stack[--top];
         if (q->divided[k]) {
@@ -897,9 +909,10 @@ static void node_exact(const OrcSys* s, const Pool* q, int ni, double dx, double
             const P* o = &s->tp[q->points[base + i]];
             const double ex = dx + (o->x - cx), ey = dy + (o->y - cy);
             const double r2 = ex * ex + ey * ey + s->softeningSq;
-            const double h = 1 / (r2 * sqrt(r2));
+            const double ir = 1 / sqrt(r2), h = ir / r2;
             g[0] += o->mass * ex * h; g[1] += o->mass * ey * h;
             g[2] -= o->charge * ex * h; g[3] -= o->charge * ey * h;
+            g[4] -= o->mass * ir; g[5] += o->charge * ir;
         }
     }
 }
@@ -928,15 +941,16 @@ int64_t orc_far_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double* ou
             if (size * size < thetaSq * (dx * dx + dy * dy) && window_ok(s, q, ni, px, py, dx, dy)) {
                 if (out && m < cap) {
                     double* r = out + (size_t)m * ORC_FAR_STRIDE;
-                    double g[4];
+                    double g[6];
                     node_exact(s, q, ni, dx, dy, q->comX[ni], q->comY[ni], stack2, g);
                     const double r2 = dx * dx + dy * dy + s->softeningSq;
-                    const double h = 1 / (r2 * sqrt(r2));
+                    const double ir = 1 / sqrt(r2), h = ir / r2;
                     r[0] = pi; r[1] = dx; r[2] = dy; r[3] = r2; r[4] = size;
                     r[5] = q->mass[ni]; r[6] = q->charge[ni];
                     for (int c = 0; c < 9; c++) r[7 + c] = q->mom[(size_t)ni * 9 + c];
                     r[16] = g[0] - q->mass[ni] * dx * h; r[17] = g[1] - q->mass[ni] * dy * h;
                     r[18] = g[2] + q->charge[ni] * dx * h; r[19] = g[3] + q->charge[ni] * dy * h;
+                    r[20] = g[4] + q->mass[ni] * ir; r[21] = g[5] - q->charge[ni] * ir;
                 }
                 m++;
             } else {
@@ -952,25 +966,32 @@ int64_t orc_far_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double* ou
 /* Training samples in the GEON_FAR_SAMPLE layout (features, targets in the
  * offset frame normalised by the output scales, loss weights divided by f2ref)
  * for the observers in idx; obs[k] = the observer of sample k. */
-int64_t orc_far_training_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double f2ref, float* out,
-                                 int32_t* obs, int64_t cap) {
+int64_t orc_far_training_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double f2ref, double pe2ref,
+                                 float* out, int32_t* obs, int64_t cap) {
     const int64_t m = orc_far_samples(s, idx, n_idx, NULL, 0);
     double* raw = (double*)malloc(sizeof(double) * ORC_FAR_STRIDE * (size_t)(m > 0 ? m : 1));
     orc_far_samples(s, idx, n_idx, raw, m);
     for (int64_t k = 0; k < m && k < cap; k++) {
         const double* r = raw + (size_t)k * ORC_FAR_STRIDE;
         const P* p = &s->p[(int)r[0]];
-        double f[FAR_IN], ux, uy, sg0, sc0;
-        far_features(r[1], r[2], s->softeningSq, r[4], r[5], r + 7, f, &ux, &uy, &sg0, &sc0);
+        double f[FAR_IN], ux, uy, sg0, sc0, sp0, cp0;
+        far_features(r[1], r[2], s->softeningSq, r[4], r[5], r + 7, f, &ux, &uy, &sg0, &sc0, &sp0, &cp0);
         float* o = out + (size_t)k * FAR_SAMPLE;
         for (int j = 0; j < FAR_IN; j++) o[j] = (float)f[j];
         const double ig = sg0 > 0 ? 1 / sg0 : 0, ic = sc0 > 0 ? 1 / sc0 : 0;
-        o[FAR_IN + 0] = (float)((ux * r[16] + uy * r[17]) * ig);
-        o[FAR_IN + 1] = (float)((-uy * r[16] + ux * r[17]) * ig);
-        o[FAR_IN + 2] = (float)((ux * r[18] + uy * r[19]) * ic);
-        o[FAR_IN + 3] = (float)((-uy * r[18] + ux * r[19]) * ic);
-        o[FAR_IN + 4] = s->grav && sg0 > 0 ? (float)((p->mass * sg0) * (p->mass * sg0) / f2ref) : 0.f;
-        o[FAR_IN + 5] = s->coul && sc0 > 0 ? (float)((p->charge * sc0) * (p->charge * sc0) / f2ref) : 0.f;
+        const double igp = sp0 > 0 ? 1 / sp0 : 0, icp = cp0 > 0 ? 1 / cp0 : 0;
+        float* t = o + FAR_IN;
+        t[0] = (float)((ux * r[16] + uy * r[17]) * ig);
+        t[1] = (float)((-uy * r[16] + ux * r[17]) * ig);
+        t[2] = (float)((ux * r[18] + uy * r[19]) * ic);
+        t[3] = (float)((-uy * r[18] + ux * r[19]) * ic);
+        t[4] = (float)(r[20] * igp);
+        t[5] = (float)(r[21] * icp);
+        float* w = o + FAR_IN + FAR_OUT;
+        w[0] = s->grav && sg0 > 0 ? (float)((p->mass * sg0) * (p->mass * sg0) / f2ref) : 0.f;
+        w[1] = s->coul && sc0 > 0 ? (float)((p->charge * sc0) * (p->charge * sc0) / f2ref) : 0.f;
+        w[2] = s->grav && sp0 > 0 ? (float)((p->mass * sp0) * (p->mass * sp0) / pe2ref) : 0.f;
+        w[3] = s->coul && cp0 > 0 ? (float)((p->charge * cp0) * (p->charge * cp0) / pe2ref) : 0.f;
         if (obs) obs[k] = (int32_t)r[0];
     }
     free(raw);

@@ -61,17 +61,18 @@ void    orc_set_threads(int32_t n);
  * the tree, then for each observer in idx lists every divided node its walk
  * accepts, as ORC_FAR_STRIDE doubles: observer, dx, dy, r^2 + eps^2, node size,
  * mass, charge, the 9 moments about the COM, and the exact monopole-truncation
- * residual (gravity x, y per unit observer mass; Coulomb x, y per unit
- * observer charge).  Returns the number of samples; writes at most cap. */
-#define ORC_FAR_STRIDE 20
+ * residuals: force (gravity x, y per unit observer mass; Coulomb x, y per unit
+ * observer charge) and potential (gravity, Coulomb).  Returns the number of
+ * samples; writes at most cap. */
+#define ORC_FAR_STRIDE 22
 int64_t orc_far_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double* out, int64_t cap);
 /* FP64 restatement of the learned far-field network (model layout as
  * GEON_FAR_MODEL_FLOATS in include/geon.h): set the model used when
  * far_field == GEON_FAR_LEARNED, and the training loss and its gradient on
  * samples in the GEON_FAR_SAMPLE layout. */
 int     orc_far_set_model(OrcSys* s, const float* model);
-int64_t orc_far_training_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double f2ref, float* out,
-                                 int32_t* obs, int64_t cap);
+int64_t orc_far_training_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, double f2ref, double pe2ref,
+                                 float* out, int32_t* obs, int64_t cap);
 double  orc_far_loss_grad(const float* model, const float* samples, int64_t n, double wscale, double* grad);
 
 /* Scalar fields (which: 0 Higgs, 1 axion). */

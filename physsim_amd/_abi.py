@@ -15,7 +15,7 @@ BOUND_DESPAWN, BOUND_BOUNCE, BOUND_LOOP = 0, 1, 2
 TORUS, KLEIN, RP2 = 0, 1, 2
 PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS, PERIODIC_WINDOWED = 0, 1, 2
 FAR_NONE, FAR_MULTIPOLE, FAR_LEARNED = 0, 1, 2
-FAR_PARAMS, FAR_MODEL_FLOATS, FAR_SAMPLE = 5444, 5468, 16
+FAR_PARAMS, FAR_MODEL_FLOATS, FAR_SAMPLE = 5574, 5600, 20
 FIELD_HIGGS, FIELD_AXION = 0, 1
 FQ = {"field": 0, "field_dot": 1, "grad_x": 2, "grad_y": 3, "sg_phi": 4, "sg_gx": 5, "sg_gy": 6,
       "source": 7, "thermal": 8}

@@ -300,7 +300,7 @@ int far_model_alloc(GeonContext* h) {
     if ((rc = dmalloc(h, &h->far_v, kFarParams))) return rc;
     if ((rc = dmalloc(h, &h->far_grad, kFarParams))) return rc;
     if ((rc = dmalloc(h, &h->far_loss, 1))) return rc;
-    if ((rc = dmalloc(h, &h->far_f2, 1))) return rc;
+    if ((rc = dmalloc(h, &h->far_f2, 2))) return rc;
     return GEON_OK;
 }
 
@@ -320,7 +320,7 @@ FarArgs far_args(GeonContext* h) {
     f.far_list = h->far_list;
     f.far_count = h->far_count;
     f.far_cap = h->far_cap;
-    f.fx = h->fc[K_FX]; f.fy = h->fc[K_FY];
+    f.fx = h->fc[K_FX]; f.fy = h->fc[K_FY]; f.pe = h->fc[K_PE];
     f.ctrl = h->ctrl;
     const PhysFlags pf = phys_flags(h->prm);
     f.soft2 = pf.soft2;
@@ -704,15 +704,16 @@ int far_sample_stats(GeonContext* h, std::vector<float>* stats, float* wscale) {
     std::vector<float> buf((size_t)take * kFarSample);
     HIP_OK(hipMemcpy2D(buf.data(), sizeof(float) * kFarSample, h->far_samples, sizeof(float) * kFarSample * stride,
                        sizeof(float) * kFarSample, (size_t)take, hipMemcpyDeviceToHost));
-    double sum[kFarIn] = {}, sq[kFarIn] = {}, tq[4] = {}, tn[4] = {}, wsum = 0, wn = 0;
+    double sum[kFarIn] = {}, sq[kFarIn] = {}, tq[kFarOut] = {}, tn[kFarOut] = {}, wsum = 0, wn = 0;
     for (int64_t r = 0; r < take; r++) {
         const float* s = &buf[(size_t)r * kFarSample];
+        const float* w = s + kFarIn + kFarOut;
         for (int k = 0; k < kFarIn; k++) { sum[k] += s[k]; sq[k] += (double)s[k] * s[k]; }
-        for (int j = 0; j < 4; j++) {
-            const float w = s[kFarIn + 4 + (j >> 1)];
-            if (w > 0) { tq[j] += (double)s[kFarIn + j] * s[kFarIn + j]; tn[j] += 1; }
+        for (int j = 0; j < kFarOut; j++) {
+            const float wj = w[j < 4 ? (j >> 1) : j - 2];
+            if (wj > 0) { tq[j] += (double)s[kFarIn + j] * s[kFarIn + j]; tn[j] += 1; }
         }
-        for (int j = 0; j < 2; j++) if (s[kFarIn + 4 + j] > 0) { wsum += s[kFarIn + 4 + j]; wn += 1; }
+        for (int j = 0; j < kFarW; j++) if (w[j] > 0) { wsum += w[j]; wn += 1; }
     }
     stats->assign(kFarStats, 0.f);
     for (int k = 0; k < kFarIn; k++) {
@@ -721,7 +722,8 @@ int far_sample_stats(GeonContext* h, std::vector<float>* stats, float* wscale) {
         (*stats)[k] = (float)m;
         (*stats)[kFarIn + k] = (float)(1.0 / std::max(sqrt(var), 1e-6));
     }
-    for (int j = 0; j < 4; j++) (*stats)[2 * kFarIn + j] = (float)std::max(tn[j] > 0 ? sqrt(tq[j] / tn[j]) : 1.0, 1e-6);
+    for (int j = 0; j < kFarOut; j++)
+        (*stats)[2 * kFarIn + j] = (float)std::max(tn[j] > 0 ? sqrt(tq[j] / tn[j]) : 1.0, 1e-6);
     *wscale = (float)(wsum > 0 ? wn / wsum : 1.0);
     return GEON_OK;
 }
@@ -1355,7 +1357,7 @@ int geon_far_collect(GeonHandle h, int32_t group_stride, int64_t* n_samples) {
             h->far_obs = no_buf;
             h->far_cap_samples = cap;
         }
-        launch_far_f2(h->stream, h->fc[K_FX], h->fc[K_FY], h->ctrl, h->far_f2);
+        launch_far_f2(h->stream, h->fc[K_FX], h->fc[K_FY], h->fc[K_PE], h->ctrl, h->far_f2);
         FarSampleArgs a;
         a.x = F(h, C_X); a.y = F(h, C_Y); a.m = F(h, C_MASS); a.qq = I(h, C_QQ);
         a.nodes = h->tb.nodes; a.node_mom = h->tb.node_mom;
@@ -1413,8 +1415,8 @@ int geon_far_train(GeonHandle h, int32_t steps, int32_t batch, double lr, double
             for (int m = 0; m < kFarHid; m++) p[kFarOffW1 + k * kFarHid + m] = uni(1.0 / sqrt((double)kFarIn));
         for (int k = 0; k < kFarHid * kFarHid; k++) p[kFarOffW2 + k] = uni(0.125);
         for (int k = 0; k < kFarHid; k++) p[kFarOffB2 + k] = uni(0.125);
-        for (int k = 0; k < kFarHid * 4; k++) p[kFarOffW3 + k] = uni(0.125);
-        for (int k = 0; k < 4; k++) p[kFarOffB3 + k] = uni(0.125);
+        for (int k = 0; k < kFarHid * kFarOut; k++) p[kFarOffW3 + k] = uni(0.125);
+        for (int k = 0; k < kFarOut; k++) p[kFarOffB3 + k] = uni(0.125);
         HIP_OK(hipMemcpy(h->far_params, p.data(), sizeof(float) * kFarParams, hipMemcpyHostToDevice));
         HIP_OK(hipMemset(h->far_m, 0, sizeof(float) * kFarParams));
         HIP_OK(hipMemset(h->far_v, 0, sizeof(float) * kFarParams));

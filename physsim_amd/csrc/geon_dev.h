@@ -111,6 +111,8 @@ __device__ __forceinline__ float torus_wrap(float d, float full, float half) {
 //   Coulomb -q_i [ D h + 2 d (D.d) h' + (the same quadrupole form in C) ]
 // with h' = dh/dr2 = -3/2 r2^-5/2 and h'' = 15/4 r2^-7/2; Q, D, C are the
 // node's mass quadrupole, charge dipole and charge quadrupole about its COM.
+// The matching potential terms (f = r2^-1/2, grad f = -d h) go to the PE:
+//   gravity  m_i (tr(Q) h / 2 + (d.Q.d) h'),  Coulomb  -q_i ((D.d) h + tr(C) h / 2 + (d.C.d) h').
 template <uint32_t F, bool KAHAN>
 __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const PhysFlags& pf, float dx, float dy,
                                               const NodeMom& c) {
@@ -118,12 +120,13 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
     const float ir2 = 1.f / r2;
     const float h = __builtin_amdgcn_rsqf(r2) * ir2;
     const float h1 = -1.5f * h * ir2, h2 = 3.75f * h * ir2 * ir2;
-    float fx = 0.f, fy = 0.f;
+    float fx = 0.f, fy = 0.f, pe = 0.f;
     if (F & F_GRAV) {
         const float qx = c.qxx * dx + c.qxy * dy, qy = c.qxy * dx + c.qyy * dy;
         const float dqd = dx * qx + dy * qy, tr = c.qxx + c.qyy;
         fx += o.m * (2.f * qx * h1 + tr * dx * h1 + 2.f * dx * dqd * h2);
         fy += o.m * (2.f * qy * h1 + tr * dy * h1 + 2.f * dy * dqd * h2);
+        pe += o.m * (0.5f * tr * h + dqd * h1);
     }
     if (F & F_COUL) {
         float qi = o.q;
@@ -133,8 +136,10 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
         const float dqd = dx * qx + dy * qy, tr = c.cxx + c.cyy;
         fx -= qi * (c.dx * h + 2.f * dx * dd * h1 + 2.f * qx * h1 + tr * dx * h1 + 2.f * dx * dqd * h2);
         fy -= qi * (c.dy * h + 2.f * dy * dd * h1 + 2.f * qy * h1 + tr * dy * h1 + 2.f * dy * dqd * h2);
+        pe -= qi * (dd * h + 0.5f * tr * h + dqd * h1);
     }
     a.addf(fx, fy);
+    a.addpe(pe);
 }
 
 // One source acting on the observer: the body of pairForce (forces.js:163-442)

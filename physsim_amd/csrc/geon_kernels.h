@@ -148,14 +148,16 @@ void launch_pairs(hipStream_t s, const PairArgs& a, bool write);
 // ---- learned far-field correction (k_learn.hip) ----
 constexpr int kFarIn = 10;          // features of a pair
 constexpr int kFarHid = 64;         // hidden width (2 layers)
-constexpr int kFarSample = 16;      // floats per training sample: 10 features, 4 targets, 2 weights
+constexpr int kFarOut = 6;          // gravity force (u, v), Coulomb force (u, v), gravity PE, Coulomb PE
+constexpr int kFarW = 4;            // loss weights: gravity force, Coulomb force, gravity PE, Coulomb PE
+constexpr int kFarSample = kFarIn + kFarOut + kFarW;   // floats per training sample
 constexpr int kFarOffW1 = 0;        // W1[16][64] (input 10 = bias of layer 1)
 constexpr int kFarOffW2 = 1024;     // W2[64][64]
 constexpr int kFarOffB2 = 5120;     // b2[64]
-constexpr int kFarOffW3 = 5184;     // W3[64][4]
-constexpr int kFarOffB3 = 5440;     // b3[4]
-constexpr int kFarParams = 5444;
-constexpr int kFarStats = 24;       // feature mean[10], 1/std[10], output scale[4]
+constexpr int kFarOffW3 = 5184;     // W3[64][6]
+constexpr int kFarOffB3 = 5568;     // b3[6]
+constexpr int kFarParams = 5574;
+constexpr int kFarStats = 2 * kFarIn + kFarOut;   // feature mean[10], 1/std[10], output scale[6]
 
 struct FarFrags;
 struct FarArgs {
@@ -166,7 +168,7 @@ struct FarArgs {
     const uint32_t* far_list;
     const int32_t* far_count;
     int64_t far_cap;
-    float *fx, *fy;
+    float *fx, *fy, *pe;
     const Ctrl* ctrl;
     float soft2, dom_w, dom_h;
     int periodic, grav, coul;
@@ -184,7 +186,7 @@ struct FarSampleArgs {
     const int64_t* row_off;     // first sample row of each selected group
     int n_groups;
     int64_t n_rows;
-    const double* f2ref;        // mean |F|^2 of the system (loss weight normaliser)
+    const double* f2ref;        // mean |F|^2 and 2 mean PE^2 of the system (loss weight normalisers)
     float* out;                 // n_rows * kFarSample
     const int32_t* slot;        // observer slots (optional record)
     int32_t* obs;               // n_rows observer slots, or null
@@ -211,7 +213,7 @@ void launch_far_pack(hipStream_t s, const float* params, const float* stats, voi
 void launch_far_mlp(hipStream_t s, const FarArgs& a, int n_upper);
 void launch_far_samples(hipStream_t s, const FarSampleArgs& a);
 void launch_far_train(hipStream_t s, const FarTrainArgs& a, int nblocks, int apply, float lr, int64_t t);
-void launch_far_f2(hipStream_t s, const float* fx, const float* fy, const Ctrl* ctrl, double* out);
+void launch_far_f2(hipStream_t s, const float* fx, const float* fy, const float* pe, const Ctrl* ctrl, double* out);
 
 // ---- scalar fields (k_field.hip) ----
 struct FieldArgs {

@@ -11,8 +11,10 @@
 //   f5, f6 = (D_u, D_v) / (A s)                  charge dipole about the COM
 //   f7..f9 = (C_uu, C_uv, C_vv) / (A s^2)        charge quadrupole about the COM
 // with r^2 = |d|^2 + eps^2, s the node width, M its mass, A its total |charge|,
-// v = (-u_y, u_x).  Outputs o = (g_u, g_v, c_u, c_v) (times oscale) give
-//   F_i += m_i M (s/r)^2 / r^2 (g_u u + g_v v) + q_i A (s/r) / r^2 (c_u u + c_v v).
+// v = (-u_y, u_x).  Outputs o = (g_u, g_v, c_u, c_v, p_g, p_c) (times oscale) give
+//   F_i  += m_i M (s/r)^2 / r^2 (g_u u + g_v v) + q_i A (s/r) / r^2 (c_u u + c_v v),
+//   PE_i += m_i M (s/r)^2 / r p_g + q_i A (s/r) / r p_c
+// (the potential residual keeps the energy readout consistent with the forces).
 // Network: x = ((f - mean) * istd, 1, 0...) in 16 inputs (input 10 carries
 // the first layer's bias), z1 = W1^T x, a1 = gelu(z1), z2 = W2^T a1 + b2,
 // a2 = gelu(z2), o = W3^T a2 + b3; gelu is the tanh form (evaluated by a
@@ -84,19 +86,15 @@ __device__ __forceinline__ bf16x8 pack8(const float* v) {
 // ---------------------------------------------------------------------------
 // Features of one (observer, node) pair
 // ---------------------------------------------------------------------------
-struct PairFeat {
-    float f[kFarIn];
-    float ux, uy, sg0, sc0;     // direction and the output scales without m_i, q_i
-};
-
 __device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 __device__ __forceinline__ float rsq_t(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsq_t(double x) { return 1.0 / sqrt(x); }
 
+// sg0, sc0: force scales, sp0, cp0: potential scales (without m_i, q_i).
 template <typename T>
 __device__ __forceinline__ void pair_features(T dx, T dy, T eps2, T s, T M, const NodeMom& c, T* f, T& ux, T& uy,
-                                              T& sg0, T& sc0) {
+                                              T& sg0, T& sc0, T& sp0, T& cp0) {
     const T rho2 = dx * dx + dy * dy;
     const T r2 = rho2 + eps2;
     const T ir = rsq_t(r2);
@@ -121,6 +119,8 @@ __device__ __forceinline__ void pair_features(T dx, T dy, T eps2, T s, T M, cons
     f[9] = (vv * c.cxx - T(2) * uv * c.cxy + uu * c.cyy) * iA * is;
     sg0 = M * sr * sr * ir2;
     sc0 = A * sr * ir2;
+    sp0 = M * sr * sr * ir;
+    cp0 = A * sr * ir;
 }
 
 // ---------------------------------------------------------------------------
@@ -130,12 +130,15 @@ struct FarFrags {
     bf16x8 w1[2][kWave];            // layer 1, A = W1^T, M-block mb
     bf16x8 w2[2][4][kWave];         // layer 2, A = W2^T, K permuted
     bf16x8 w3[4][kWave];            // layer 3, A = W3^T padded to 32 rows, K permuted
-    bf16x8 w3t[2][kWave];           // backward: A = W3 (hidden x 4 outputs, K padded to 16)
+    bf16x8 w3t[2][kWave];           // backward: A = W3 (hidden x outputs; K = 8h + j <-> output 4h + j)
     bf16x8 w2t[2][4][kWave];        // backward: A = W2 (in x out), K (out) permuted
     float b2[kFarHid];
-    float b3[4];
-    float mean[kFarIn], istd[kFarIn], oscale[4];
+    float b3[kFarOut];
+    float mean[kFarIn], istd[kFarIn], oscale[kFarOut];
 };
+
+// Layer-3 outputs live in accumulator r of lane half h as output 4h + r (r < 4).
+__host__ __device__ __forceinline__ int out_of(int h, int r) { return 4 * h + r; }
 
 __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict__ stats, FarFrags* __restrict__ fr) {
     const int l = threadIdx.x;          // one wave
@@ -153,21 +156,22 @@ __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict_
             for (int j = 0; j < 8; j++) v[j] = W2[(32 * mb + c) * kFarHid + kperm(t, h, j)];
             fr->w2t[mb][t][l] = pack8(v);
         }
-        for (int j = 0; j < 8; j++) v[j] = (h == 0 && j < 4) ? W3[(32 * mb + c) * 4 + j] : 0.f;
+        for (int j = 0; j < 8; j++)
+            v[j] = (j < 4 && out_of(h, j) < kFarOut) ? W3[(32 * mb + c) * kFarOut + out_of(h, j)] : 0.f;
         fr->w3t[mb][l] = pack8(v);
     }
     for (int t = 0; t < 4; t++) {
         float v[8];
-        for (int j = 0; j < 8; j++) v[j] = c < 4 ? W3[kperm(t, h, j) * 4 + c] : 0.f;
+        for (int j = 0; j < 8; j++) v[j] = c < kFarOut ? W3[kperm(t, h, j) * kFarOut + c] : 0.f;
         fr->w3[t][l] = pack8(v);
     }
     fr->b2[l] = P[kFarOffB2 + l];
-    if (l < 4) fr->b3[l] = P[kFarOffB3 + l];
+    if (l < kFarOut) fr->b3[l] = P[kFarOffB3 + l];
     if (l < kFarIn) {
         fr->mean[l] = stats[l];
         fr->istd[l] = stats[kFarIn + l];
     }
-    if (l < 4) fr->oscale[l] = stats[2 * kFarIn + l];
+    if (l < kFarOut) fr->oscale[l] = stats[2 * kFarIn + l];
 }
 
 // Standardised inputs of one row as the 16 BF16 values of the layer-1 operand.
@@ -183,9 +187,9 @@ __device__ __forceinline__ void std_inputs(const FarFrags& fr, const float* f, b
 __device__ __forceinline__ int c_row(int mb, int r, int h) { return 32 * mb + 8 * (r >> 2) + 4 * h + (r & 3); }
 
 // Forward pass of one 64-row tile whose standardised inputs are in LDS
-// (xs[row][16] BF16).  Returns the layer-3 tiles; lanes with h = 0 hold
-// outputs 0..3 of row 32 nb + c in acc[0..3] of o[nb].  If z1/z2 are given,
-// the pre-activations are kept (training).
+// (xs[row][16] BF16).  Returns the layer-3 tiles: lane (h, c) holds outputs
+// 4h + r of row 32 nb + c in acc[r] of o[nb] (r < 4, 4h + r < kFarOut); the
+// pre-activations z1, z2 are kept for training.
 struct Fwd {
     floatx16 z1[2][2], z2[2][2];
     bf16x8 a1[4][2], a2[4][2];
@@ -234,10 +238,9 @@ __device__ __forceinline__ void mlp_forward(const FarFrags& fr, const __bf16* xs
             F.a2[t][nb] = pack8(v);
         }
     floatx16 binit3 = {};
-    if (h == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) binit3[r] = fr.b3[r];
-    }
+    for (int r = 0; r < 4; r++)
+        if (out_of(h, r) < kFarOut) binit3[r] = fr.b3[out_of(h, r)];
 #pragma unroll
     for (int nb = 0; nb < 2; nb++) {
         floatx16 acc = binit3;
@@ -251,21 +254,21 @@ __device__ __forceinline__ void mlp_forward(const FarFrags& fr, const __bf16* xs
 // Inference: add the predicted residual of every listed pair to the forces
 // ---------------------------------------------------------------------------
 struct FarRow {
-    float ux, uy, sg, sc;
+    float ux, uy, sg, sc, sp, cp;
 };
 
 __global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 xs[kWave * 16];
     __shared__ FarRow rows[kWave];
     __shared__ int rlane[kWave];
-    __shared__ float sacc[kWave][2];
+    __shared__ float sacc[kWave][3];                // fx, fy, PE of each observer lane
     const int l = threadIdx.x;
     const int h = l >> 5, c = l & 31;
     const int64_t g = blockIdx.x;
     const int n = a.ctrl->n_alive;
     if (g * kWave >= n) return;
     const FarFrags& fr = *a.frags;
-    sacc[l][0] = 0.f; sacc[l][1] = 0.f;
+    sacc[l][0] = 0.f; sacc[l][1] = 0.f; sacc[l][2] = 0.f;
     const int cnt = a.far_count[g];
     const uint32_t* list = a.far_list + g * a.far_cap;
     const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
         // 1. features of row l
         const bool valid = base + l < cnt;
         float f[kFarIn];
-        FarRow rw = {0.f, 0.f, 0.f, 0.f};
+        FarRow rw = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         int lane_of = 0;
         if (valid) {
             const uint32_t e = list[base + l];
@@ -285,10 +288,12 @@ __global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
             const NodeMom mo = a.node_mom[node];
             float dx = nd.comx - s4.x, dy = nd.comy - s4.y;
             if (a.periodic) { dx = torus_wrap(dx, a.dom_w, half_w); dy = torus_wrap(dy, a.dom_h, half_h); }
-            float sg0, sc0;
-            pair_features<float>(dx, dy, a.soft2, 2.f * nd.half, nd.mass, mo, f, rw.ux, rw.uy, sg0, sc0);
+            float sg0, sc0, sp0, cp0;
+            pair_features<float>(dx, dy, a.soft2, 2.f * nd.half, nd.mass, mo, f, rw.ux, rw.uy, sg0, sc0, sp0, cp0);
             rw.sg = a.grav ? s4.z * sg0 : 0.f;
             rw.sc = a.coul ? s4.w * sc0 : 0.f;
+            rw.sp = a.grav ? s4.z * sp0 : 0.f;
+            rw.cp = a.coul ? s4.w * cp0 : 0.f;
         }
         float x[16];
         std_inputs(fr, f, valid, x);
@@ -302,19 +307,23 @@ __global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
         // 2. the network
         Fwd F;
         mlp_forward(fr, xs, l, F);
-        // 3. rotate, scale and accumulate per observer lane
-        if (h == 0) {
+        // 3. rotate, scale and accumulate per observer lane: the force from
+        //    outputs 0..3 (lanes h = 0), the potential from outputs 4, 5 (h = 1)
 #pragma unroll
-            for (int nb = 0; nb < 2; nb++) {
-                const int row = 32 * nb + c;
-                if (base + row < cnt) {
-                    const FarRow q = rows[row];
+        for (int nb = 0; nb < 2; nb++) {
+            const int row = 32 * nb + c;
+            if (base + row < cnt) {
+                const FarRow q = rows[row];
+                if (h == 0) {
                     const float gu = F.o[nb][0] * fr.oscale[0], gv = F.o[nb][1] * fr.oscale[1];
                     const float cu = F.o[nb][2] * fr.oscale[2], cv = F.o[nb][3] * fr.oscale[3];
                     const float fx = q.sg * (q.ux * gu - q.uy * gv) + q.sc * (q.ux * cu - q.uy * cv);
                     const float fy = q.sg * (q.uy * gu + q.ux * gv) + q.sc * (q.uy * cu + q.ux * cv);
                     atomicAdd(&sacc[rlane[row]][0], fx);
                     atomicAdd(&sacc[rlane[row]][1], fy);
+                } else {
+                    const float pe = q.sp * F.o[nb][0] * fr.oscale[4] + q.cp * F.o[nb][1] * fr.oscale[5];
+                    atomicAdd(&sacc[rlane[row]][2], pe);
                 }
             }
         }
@@ -326,6 +335,7 @@ __global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
     if (i < n) {
         a.fx[i] += sacc[l][0];
         a.fy[i] += sacc[l][1];
+        a.pe[i] += sacc[l][2];
     }
 }
 
@@ -360,47 +370,65 @@ __global__ __launch_bounds__(256) void k_far_samples(FarSampleArgs a) {
         dy = dy > 0.5 * H ? dy - H : (dy < -0.5 * H ? dy + H : dy);
     }
     const int first = a.first[node], np = a.npts[node];
-    double gx = 0, gy = 0, qx = 0, qy = 0;
+    double gx = 0, gy = 0, qx = 0, qy = 0, pg = 0, pq = 0;
     for (int j = first + l; j < first + np; j += kWave) {
         const double ex = dx + ((double)a.x[j] - cx), ey = dy + ((double)a.y[j] - cy);
         const double r2 = ex * ex + ey * ey + a.soft2;
-        const double hh = 1.0 / (r2 * sqrt(r2));
+        const double ir = 1.0 / sqrt(r2);
+        const double hh = ir / r2;
         const double mj = a.m[j], qj = a.qq[j] * kChargeQuantum;
         gx += mj * ex * hh; gy += mj * ey * hh;
         qx += qj * ex * hh; qy += qj * ey * hh;
+        pg -= mj * ir;                               // pair potentials (forces.js:199, 216)
+        pq += qj * ir;
     }
     for (int off = 32; off > 0; off >>= 1) {
         gx += __shfl_xor(gx, off, 64); gy += __shfl_xor(gy, off, 64);
         qx += __shfl_xor(qx, off, 64); qy += __shfl_xor(qy, off, 64);
+        pg += __shfl_xor(pg, off, 64); pq += __shfl_xor(pq, off, 64);
     }
     if (l != 0) return;
     const double r2 = dx * dx + dy * dy + a.soft2;
-    const double hh = 1.0 / (r2 * sqrt(r2));
-    const double rgx = gx - M * dx * hh, rgy = gy - M * dy * hh;          // per unit observer mass
-    const double rcx = -(qx - Q * dx * hh), rcy = -(qy - Q * dy * hh);    // per unit observer charge
+    const double ir = 1.0 / sqrt(r2);
+    const double hh = ir / r2;
+    // residuals per unit observer mass (gravity) and charge (Coulomb)
+    const double rgx = gx - M * dx * hh, rgy = gy - M * dy * hh;
+    const double rcx = -(qx - Q * dx * hh), rcy = -(qy - Q * dy * hh);
+    const double rpg = pg + M * ir, rpq = pq - Q * ir;
     const NodeMom mo = a.node_mom[node];
-    double f[kFarIn], ux, uy, sg0, sc0;
-    pair_features<double>(dx, dy, (double)a.soft2, 2.0 * (double)a.nodes[node].half, M, mo, f, ux, uy, sg0, sc0);
+    double f[kFarIn], ux, uy, sg0, sc0, sp0, cp0;
+    pair_features<double>(dx, dy, (double)a.soft2, 2.0 * (double)a.nodes[node].half, M, mo, f, ux, uy, sg0, sc0,
+                          sp0, cp0);
     float* out = a.out + (size_t)row * kFarSample;
     for (int k = 0; k < kFarIn; k++) out[k] = (float)f[k];
     const double ig = sg0 > 0 ? 1.0 / sg0 : 0.0, ic = sc0 > 0 ? 1.0 / sc0 : 0.0;
-    out[kFarIn + 0] = (float)((ux * rgx + uy * rgy) * ig);
-    out[kFarIn + 1] = (float)((-uy * rgx + ux * rgy) * ig);
-    out[kFarIn + 2] = (float)((ux * rcx + uy * rcy) * ic);
-    out[kFarIn + 3] = (float)((-uy * rcx + ux * rcy) * ic);
+    const double igp = sp0 > 0 ? 1.0 / sp0 : 0.0, icp = cp0 > 0 ? 1.0 / cp0 : 0.0;
+    float* t = out + kFarIn;
+    t[0] = (float)((ux * rgx + uy * rgy) * ig);
+    t[1] = (float)((-uy * rgx + ux * rgy) * ig);
+    t[2] = (float)((ux * rcx + uy * rcy) * ic);
+    t[3] = (float)((-uy * rcx + ux * rcy) * ic);
+    t[4] = (float)(rpg * igp);
+    t[5] = (float)(rpq * icp);
     if (a.obs) a.obs[row] = a.slot[i];
     const double mi = a.m[i], qi = a.qq[i] * kChargeQuantum;
-    const double iF2 = 1.0 / (*a.f2ref > 0 ? *a.f2ref : 1.0);
-    out[kFarIn + 4] = a.grav && sg0 > 0 ? (float)((mi * sg0) * (mi * sg0) * iF2) : 0.f;
-    out[kFarIn + 5] = a.coul && sc0 > 0 ? (float)((qi * sc0) * (qi * sc0) * iF2) : 0.f;
+    const double iF2 = 1.0 / (a.f2ref[0] > 0 ? a.f2ref[0] : 1.0);
+    const double iP2 = 1.0 / (a.f2ref[1] > 0 ? a.f2ref[1] : 1.0);
+    float* w = out + kFarIn + kFarOut;
+    w[0] = a.grav && sg0 > 0 ? (float)((mi * sg0) * (mi * sg0) * iF2) : 0.f;
+    w[1] = a.coul && sc0 > 0 ? (float)((qi * sc0) * (qi * sc0) * iF2) : 0.f;
+    w[2] = a.grav && sp0 > 0 ? (float)((mi * sp0) * (mi * sp0) * iP2) : 0.f;
+    w[3] = a.coul && cp0 > 0 ? (float)((qi * cp0) * (qi * cp0) * iP2) : 0.f;
 }
 
 // ---------------------------------------------------------------------------
 // Training: forward + backward of the weighted squared error on a batch
 // ---------------------------------------------------------------------------
-// Loss L = wscale / B * sum_rows sum_j w_j (oscale_j o_j - t_j)^2 (w = wg for
-// j < 2, wc for j >= 2).  Each block (one wave) takes one tile of 64 rows and
-// writes its partial gradient (kFarParams floats) and loss.
+// Loss L = wscale / B * sum_rows sum_j w_j (oscale_j o_j - t_j)^2 with the
+// sample's weight of output j: gravity force (j < 2), Coulomb force (j < 4),
+// gravity PE (j = 4), Coulomb PE (j = 5).  Each block (one wave) takes one
+// tile of 64 rows and writes its partial gradient (kFarParams floats) and loss.
+__host__ __device__ __forceinline__ int wsel(int j) { return j < 4 ? (j >> 1) : j - 2; }
 constexpr int kHS = 72;      // LDS row stride (BF16) of the hidden-major tiles
 
 __device__ __forceinline__ void store_hmajor(__bf16* dst, const floatx16& v, int mb, int nb, int h, int c) {
@@ -420,15 +448,15 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
     __shared__ __bf16 xf[16 * kHS];                                     // [input][row]
     __shared__ __bf16 a1f[kFarHid * kHS], a2f[kFarHid * kHS];           // [hidden][row]
     __shared__ __bf16 dz2f[kFarHid * kHS], dz1f[kFarHid * kHS];
-    __shared__ __bf16 dof[4 * kHS];                                     // [output][row]
-    __shared__ float tg[kWave][4], wt[kWave][2];
+    __shared__ __bf16 dof[kFarOut * kHS];                               // [output][row]
+    __shared__ float tg[kWave][kFarOut], wt[kWave][kFarW];
     __shared__ float g1s[4 * 16 * kWave], g2s[4 * 16 * kWave];         // gelu'(z) in accumulator order
     const int l = threadIdx.x;
     const int h = l >> 5, c = l & 31;
     const FarFrags& fr = *a.frags;
     const floatx16 zero = {};
     floatx16 gw1[2], gw2[2][2], gw3[2];
-    float gb2[2][16], gb3[4] = {0.f, 0.f, 0.f, 0.f};
+    float gb2[2][16], gb3[4] = {0.f, 0.f, 0.f, 0.f};  // gb3[r]: output 4h + r
     gw1[0] = gw1[1] = zero;
     gw2[0][0] = gw2[0][1] = gw2[1][0] = gw2[1][1] = zero;
     gw3[0] = gw3[1] = zero;
@@ -464,9 +492,9 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
 #pragma unroll
         for (int k = 0; k < 16; k++) xf[k * kHS + l] = (__bf16)x[k];
 #pragma unroll
-        for (int j = 0; j < 4; j++) tg[l][j] = valid ? smp[kFarIn + j] : 0.f;
-        wt[l][0] = valid ? smp[kFarIn + 4] * a.wscale : 0.f;
-        wt[l][1] = valid ? smp[kFarIn + 5] * a.wscale : 0.f;
+        for (int j = 0; j < kFarOut; j++) tg[l][j] = valid ? smp[kFarIn + j] : 0.f;
+#pragma unroll
+        for (int j = 0; j < kFarW; j++) wt[l][j] = valid ? smp[kFarIn + kFarOut + j] * a.wscale : 0.f;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -488,22 +516,24 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
                 store_hmajor(a1f, v1, mb, nb, h, c);
                 store_hmajor(a2f, v2, mb, nb, h, c);
             }
-        // output error (lanes h = 0 hold the 4 outputs of row 32 nb + c)
+        // output error: lane (h, c) holds outputs 4h + r of row 32 nb + c; as the
+        // B operand of dA2 its K index 8h + r carries output 4h + r
         bf16x8 bdo[2];
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) {
             float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (h == 0) {
-                const int row = 32 * nb + c;
+            const int row = 32 * nb + c;
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
+            for (int r = 0; r < 4; r++) {
+                const int j = out_of(h, r);
+                if (j < kFarOut) {
                     const float os = fr.oscale[j];
-                    const float err = os * F.o[nb][j] - tg[row][j];
-                    const float w = wt[row][j >> 1];
+                    const float err = os * F.o[nb][r] - tg[row][j];
+                    const float w = wt[row][wsel(j)];
                     loss += w * err * err * inv_b;
-                    v[j] = 2.f * w * err * os * inv_b;
-                    gb3[j] += v[j];
-                    dof[j * kHS + row] = (__bf16)v[j];
+                    v[r] = 2.f * w * err * os * inv_b;
+                    gb3[r] += v[r];
+                    dof[j * kHS + row] = (__bf16)v[r];
                 }
             }
             bdo[nb] = pack8(v);
@@ -561,7 +591,7 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
                 bz1[mb] = ld8(dz1f + (32 * mb + c) * kHS + k0);
                 bz2[mb] = ld8(dz2f + (32 * mb + c) * kHS + k0);
             }
-            bf16x8 bo = c < 4 ? ld8(dof + c * kHS + k0) : bf16x8{};
+            bf16x8 bo = c < kFarOut ? ld8(dof + c * kHS + k0) : bf16x8{};
 #pragma unroll
             for (int nb = 0; nb < 2; nb++) gw1[nb] = mfma32(ax, bz1[nb], gw1[nb]);
 #pragma unroll
@@ -582,8 +612,8 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
         for (int r = 0; r < 16; r++)
             for (int off = 16; off > 0; off >>= 1) gb2[mb][r] += __shfl_xor(gb2[mb][r], off, 64);
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-        for (int off = 16; off > 0; off >>= 1) gb3[j] += __shfl_xor(gb3[j], off, 64);
+    for (int r = 0; r < 4; r++)
+        for (int off = 16; off > 0; off >>= 1) gb3[r] += __shfl_xor(gb3[r], off, 64);
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
     float* G = a.grad_part + (size_t)blockIdx.x * kFarParams;
     // W1[k][m]: gw1[nb] has M = input k (rows c_row), N = hidden 32 nb + c
@@ -602,7 +632,7 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
             for (int r = 0; r < 16; r++) G[kFarOffW2 + c_row(ma, r, h) * kFarHid + 32 * nb + c] = gw2[ma][nb][r];
 #pragma unroll
         for (int r = 0; r < 16; r++)
-            if (c < 4) G[kFarOffW3 + c_row(ma, r, h) * 4 + c] = gw3[ma][r];
+            if (c < kFarOut) G[kFarOffW3 + c_row(ma, r, h) * kFarOut + c] = gw3[ma][r];
     }
     if (c == 0) {
 #pragma unroll
@@ -610,11 +640,12 @@ __global__ __launch_bounds__(kWave) void k_far_train(FarTrainArgs a) {
 #pragma unroll
             for (int r = 0; r < 16; r++) G[kFarOffB2 + c_row(mb, r, h)] = gb2[mb][r];
     }
-    if (l == 0) {
+    if (c == 0) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) G[kFarOffB3 + j] = gb3[j];
-        a.loss_part[blockIdx.x] = loss;
+        for (int r = 0; r < 4; r++)
+            if (out_of(h, r) < kFarOut) G[kFarOffB3 + out_of(h, r)] = gb3[r];
     }
+    if (l == 0) a.loss_part[blockIdx.x] = loss;
 }
 
 // Sum of the block partials (fixed order: four interleaved partial sums per
@@ -647,27 +678,32 @@ __global__ __launch_bounds__(256) void k_far_adam(FarTrainArgs a, int nblocks, i
     a.params[p] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-// Mean |F|^2 over the live particles (FP64), the loss-weight normaliser.
+// Mean |F|^2 and mean PE^2 over the live particles (FP64), the loss-weight normalisers.
 __global__ __launch_bounds__(1024) void k_far_f2(const float* __restrict__ fx, const float* __restrict__ fy,
-                                                 const Ctrl* __restrict__ ctrl, double* __restrict__ out) {
-    __shared__ double sh[1024];
+                                                 const float* __restrict__ pe, const Ctrl* __restrict__ ctrl,
+                                                 double* __restrict__ out) {
+    __shared__ double sh[2][1024];
     const int n = ctrl->n_alive;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) acc += (double)fx[i] * fx[i] + (double)fy[i] * fy[i];
-    sh[threadIdx.x] = acc;
+    double acc = 0.0, accp = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        acc += (double)fx[i] * fx[i] + (double)fy[i] * fy[i];
+        accp += (double)pe[i] * pe[i];
+    }
+    sh[0][threadIdx.x] = acc;
+    sh[1][threadIdx.x] = accp;
     __syncthreads();
     for (int off = 512; off > 0; off >>= 1) {
-        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        if (threadIdx.x < off) { sh[0][threadIdx.x] += sh[0][threadIdx.x + off]; sh[1][threadIdx.x] += sh[1][threadIdx.x + off]; }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = n > 0 ? sh[0] / n : 0.0;
+    if (threadIdx.x == 0) { out[0] = n > 0 ? sh[0][0] / n : 0.0; out[1] = n > 0 ? sh[1][0] / n : 0.0; }
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-void launch_far_f2(hipStream_t s, const float* fx, const float* fy, const Ctrl* ctrl, double* out) {
-    hipLaunchKernelGGL(k_far_f2, dim3(1), dim3(1024), 0, s, fx, fy, ctrl, out);
+void launch_far_f2(hipStream_t s, const float* fx, const float* fy, const float* pe, const Ctrl* ctrl, double* out) {
+    hipLaunchKernelGGL(k_far_f2, dim3(1), dim3(1024), 0, s, fx, fy, pe, ctrl, out);
 }
 
 size_t far_frags_bytes() { return sizeof(FarFrags); }

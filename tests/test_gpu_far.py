@@ -50,6 +50,7 @@ def test_multipole_forces_match_oracle(theta, oracle_lib):
     o.compute_forces()
     fg, fo = g.forces(4096), o.forces()
     assert _rel_l2(fg["fx"], fg["fy"], fo["fx"], fo["fy"]) < 5e-5
+    assert 0.5 * fg["pe"].sum() == pytest.approx(fo["pe_total"], rel=2e-5)
     idx = np.arange(0, 4096, 8, dtype=np.int32)
     fx, fy = o.direct_forces(idx)
     _set(g, far_field=A.FAR_NONE)
@@ -80,8 +81,9 @@ def test_far_samples_match_oracle(oracle_lib):
     assert ns == len(S) > 0
     f = g.forces(2048)
     f2ref = float(np.mean(f["fx"].astype(np.float64) ** 2 + f["fy"].astype(np.float64) ** 2))
+    pe2ref = float(np.mean(f["pe"].astype(np.float64) ** 2))
     o = _orc(st, prm, oracle_lib)
-    So, obso = o.far_training_samples(np.arange(2048, dtype=np.int32), f2ref)
+    So, obso = o.far_training_samples(np.arange(2048, dtype=np.int32), f2ref, pe2ref)
     assert abs(len(So) - len(S)) <= 0.002 * len(So)
     G, O = _by_observer(S, obs), _by_observer(So, obso)
     same = [k for k in O if k in G and len(G[k]) == len(O[k])]
@@ -89,12 +91,12 @@ def test_far_samples_match_oracle(oracle_lib):
     a = np.concatenate([G[k] for k in same])
     b = np.concatenate([O[k] for k in same])
     np.testing.assert_allclose(a[:, :10], b[:, :10], rtol=0, atol=2e-5 * np.abs(b[:, :10]).max())
-    np.testing.assert_allclose(a[:, 10:14], b[:, 10:14], rtol=0, atol=1e-4 * np.abs(b[:, 10:14]).max())
-    np.testing.assert_allclose(a[:, 14:], b[:, 14:], rtol=1e-4, atol=1e-9 * b[:, 14:].max())
+    np.testing.assert_allclose(a[:, 10:16], b[:, 10:16], rtol=0, atol=1e-4 * np.abs(b[:, 10:16]).max())
+    np.testing.assert_allclose(a[:, 16:], b[:, 16:], rtol=1e-4, atol=1e-9 * b[:, 16:].max())
 
 
 def _wscale(S):
-    w = S[:, 14:].ravel()
+    w = S[:, 16:].ravel()
     w = w[w > 0]
     return len(w) / w.sum()
 
@@ -141,8 +143,11 @@ def test_learned_forces_match_oracle(oracle_lib):
     o0 = _orc(st, prm, oracle_lib)
     o0.compute_forces()
     fo0 = o0.forces()
-    # the correction itself, BF16 network against the FP64 one
+    # the correction itself, BF16 network against the FP64 one: forces and potential
     assert _rel_l2(f1["fx"] - f0["fx"], f1["fy"] - f0["fy"], fo["fx"] - fo0["fx"], fo["fy"] - fo0["fy"]) < 3e-2
+    dpe_g = 0.5 * (f1["pe"].sum() - f0["pe"].sum())
+    dpe_o = fo["pe_total"] - fo0["pe_total"]
+    assert dpe_g == pytest.approx(dpe_o, rel=3e-2)
 
 
 def test_learned_correction_gain(oracle_lib):

@@ -438,6 +438,23 @@ def test_far_samples_reconstruct_direct_sum(oracle_lib):
     fixed = np.hypot(f["fx"][idx] + cx - fx, f["fy"][idx] + cy - fy)
     assert plain.max() > 1e-4 * np.abs(fx).max()
     assert fixed.max() < 1e-11 * np.hypot(fx, fy).max()
+    # the potential residuals close the total PE: every observer's accepted nodes
+    all_idx = np.arange(3000, dtype=np.int32)
+    S2 = o.far_samples(all_idx)
+    k2 = S2[:, 0].astype(int)
+    m2 = st["mass"][k2]
+    q2 = np.round(st["charge"][k2] * 10) / 10
+    pe_fix = 0.5 * np.sum(m2 * S2[:, 20] + q2 * S2[:, 21])
+    x, y = st["x"], st["y"]
+    mm, qq = st["mass"], np.round(st["charge"] * 10) / 10
+    pe_exact = 0.0
+    for i in range(0, 3000, 500):                   # FP64 pair sum, i < j
+        d = np.hypot(x[i:i + 500, None] - x[None, :], y[i:i + 500, None] - y[None, :])
+        ir = 1 / np.sqrt(d * d + 64.0)
+        jj = np.arange(3000)[None, :] > np.arange(i, min(i + 500, 3000))[:, None]
+        pe_exact += np.sum(np.where(jj, (-mm[i:i + 500, None] * mm[None, :] + qq[i:i + 500, None] * qq[None, :]) * ir, 0))
+    assert abs(f["pe_total"] - pe_exact) > 1e-7 * abs(pe_exact)
+    assert f["pe_total"] + pe_fix == pytest.approx(pe_exact, rel=1e-12)
 
 
 def _random_model(rng):
@@ -445,7 +462,7 @@ def _random_model(rng):
     model[:A.FAR_PARAMS] = rng.normal(0, 0.3, A.FAR_PARAMS)
     model[A.FAR_PARAMS:A.FAR_PARAMS + 10] = rng.normal(0, 0.1, 10)          # means
     model[A.FAR_PARAMS + 10:A.FAR_PARAMS + 20] = rng.uniform(0.5, 2.0, 10)  # inverse deviations
-    model[A.FAR_PARAMS + 20:] = rng.uniform(0.5, 2.0, 4)                    # output scales
+    model[A.FAR_PARAMS + 20:] = rng.uniform(0.5, 2.0, 6)                    # output scales
     return model
 
 
@@ -459,7 +476,7 @@ def test_far_loss_grad_matches_finite_differences(oracle_lib):
     assert L > 0
     # parameters of every block: W1 (incl. the bias row 10), W2, b2, W3, b3
     picks = np.concatenate([rng.integers(0, 11 * 64, 8), rng.integers(1024, 5120, 8), rng.integers(5120, 5184, 4),
-                            rng.integers(5184, 5440, 6), np.arange(5440, 5444)])
+                            rng.integers(5184, 5568, 8), np.arange(5568, 5574)])
     for p in picks:
         h = 1e-3 * max(1.0, abs(float(model[p])))
         mp = model.astype(np.float64).copy()
@@ -473,23 +490,24 @@ def test_far_loss_grad_matches_finite_differences(oracle_lib):
 
 
 def test_far_learned_correction_wiring(oracle_lib):
-    # The learned force minus the plain force equals, per observer, the sum over
-    # its accepted nodes of the network output rotated out of the offset frame
-    # and scaled by m_i M (s/r)^2/r^2 and q_i A (s/r)/r^2 (network evaluated here in numpy).
+    # The learned force and PE minus the plain ones equal, summed over every
+    # observer's accepted nodes, the network output (evaluated here in numpy)
+    # rotated out of the offset frame and scaled by m_i M (s/r)^2/r^2 and
+    # q_i A (s/r)/r^2 (forces), m_i M (s/r)^2/r and q_i A (s/r)/r (potentials).
     rng = np.random.default_rng(5)
     o, st = _far_system(oracle_lib, n=2000)
     model = _random_model(rng)
-    idx = np.arange(0, 2000, 41, dtype=np.int32)
+    idx = np.arange(2000, dtype=np.int32)
     raw = o.far_samples(idx)
     S, obs = o.far_training_samples(idx)
     P = model.astype(np.float64)
     x = np.zeros((len(S), 16))
-    x[:, :10] = (S[:, :10] - P[5444:5454]) * P[5454:5464]
+    x[:, :10] = (S[:, :10] - P[5574:5584]) * P[5584:5594]
     x[:, 10] = 1
     gl = lambda z: 0.5 * z * (1 + np.tanh(np.sqrt(2 / np.pi) * (z + 0.044715 * z ** 3)))
     a1 = gl(x @ P[:1024].reshape(16, 64))
     a2 = gl(a1 @ P[1024:5120].reshape(64, 64) + P[5120:5184])
-    out = (a2 @ P[5184:5440].reshape(64, 4) + P[5440:5444]) * P[5464:5468]
+    out = (a2 @ P[5184:5568].reshape(64, 6) + P[5568:5574]) * P[5594:5600]
     dx, dy, r2, size, M = raw[:, 1], raw[:, 2], raw[:, 3], raw[:, 4], raw[:, 5]
     rho = np.hypot(dx, dy)
     ux, uy = dx / rho, dy / rho
@@ -500,8 +518,7 @@ def test_far_learned_correction_wiring(oracle_lib):
     sc = q_i * raw[:, 15] * sr / r2
     cx = sg * (ux * out[:, 0] - uy * out[:, 1]) + sc * (ux * out[:, 2] - uy * out[:, 3])
     cy = sg * (uy * out[:, 0] + ux * out[:, 1]) + sc * (uy * out[:, 2] + ux * out[:, 3])
-    pos = {int(p): k for k, p in enumerate(idx)}
-    k = np.array([pos[int(v)] for v in obs])
+    pe = (sg * np.sqrt(r2)) * out[:, 4] + (sc * np.sqrt(r2)) * out[:, 5]
     o.compute_forces()
     f0 = o.forces()
     p = A.GeonParams.from_buffer_copy(o.params)
@@ -511,8 +528,9 @@ def test_far_learned_correction_wiring(oracle_lib):
     o.compute_forces()
     f1 = o.forces()
     # the features reach numpy through float32 samples: agreement to float32 rounding
-    sx, sy = np.bincount(k, cx, len(idx)), np.bincount(k, cy, len(idx))
+    sx, sy = np.bincount(obs, cx, 2000), np.bincount(obs, cy, 2000)
     tol = 2e-6 * np.abs(np.concatenate([sx, sy])).max()
-    np.testing.assert_allclose(f1["fx"][idx] - f0["fx"][idx], sx, rtol=0, atol=tol)
-    np.testing.assert_allclose(f1["fy"][idx] - f0["fy"][idx], sy, rtol=0, atol=tol)
-    assert np.abs(sx).max() > 1e-3 * np.abs(f0["fx"][idx]).max()
+    np.testing.assert_allclose(f1["fx"] - f0["fx"], sx, rtol=0, atol=tol)
+    np.testing.assert_allclose(f1["fy"] - f0["fy"], sy, rtol=0, atol=tol)
+    assert np.abs(sx).max() > 1e-3 * np.abs(f0["fx"]).max()
+    assert f1["pe_total"] - f0["pe_total"] == pytest.approx(0.5 * pe.sum(), rel=1e-5)
