@@ -40,8 +40,8 @@ import numpy as np  # noqa: E402
 
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
-# network flops per (particle, node) pair: 2 (11 x 64 + 64 x 64 + 64 x 4)
-MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 4)
+# network flops per (particle, node) pair: 2 (11 x 64 + 64 x 64 + 64 x 6)
+MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 6)
 # HBM bytes of one k_far_mlp launch at the default workload (N=2^20, theta 0.25), from the
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (profiles/k_far_mlp_open1M_theta0.25.md)
 MLP_TRAFFIC_DEFAULT = 4015.4e6
@@ -95,6 +95,36 @@ def train_far_model(theta, seeds, n, stride, steps, batch):
     return model, info
 
 
+def run_drift(n, steps, theta, model):
+    """Relative energy drift |E(t) - E(0)| / |E(0)| (E = KE + spin KE + PE,
+    energy.js:11-62, FP64 reductions) of the learned method over `steps`
+    updates of a virialised projected Plummer sphere (scale radius 20,
+    gravity, softening 8, dt 1/128), E(0) after the first update."""
+    from physsim_amd import HipPhysics, default_params, workloads as W, _abi as A
+    st, prm, dt, _ = W.config3(n, side=1024.0, total_mass=16.0)
+    prm = dict(prm, toggles0=A.GRAVITY | A.BARNES_HUT, theta=theta, far_field=A.FAR_LEARNED, far_list_cap=1 << 17)
+    g = HipPhysics(default_params(**prm), max_particles=n)
+    g.far_set_model(model)
+    g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    g.update(dt)
+    energy = lambda: (lambda s: s["linearKE"] + s["spinKE"] + s["pe"])(g.stats())
+    e0 = energy()
+    t0 = time.perf_counter()
+    worst, trace = 0.0, []
+    for k in range(1, steps + 1):
+        g.update(dt)
+        if k % max(1, steps // 10) == 0:
+            d = (energy() - e0) / abs(e0)
+            worst = max(worst, abs(d))
+            trace.append([k, d])
+            log("drift step %d: %.3e" % (k, d))
+    g.destroy()
+    return {"max_abs": worst, "trace": trace, "steps": steps, "n": n, "theta": theta, "method": "learned",
+            "seconds": round(time.perf_counter() - t0, 1),
+            "workload": "projected Plummer sphere (scale radius 20, truncated at 0.45 W, W = 1024), lognormal masses "
+                        "(total 16), virialised Gaussian velocities, gravity, softening 8, dt 1/128 per update"}
+
+
 def run_batch(n_systems, steps):
     """Batched ensemble (config 4): n_systems config-1 disks, N log-uniform in
     [256, 16384], gravity + Coulomb, theta 0.5, dt 1e-3 per update, stepped
@@ -139,6 +169,9 @@ def main():
     ap.add_argument("--err-sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=1000,
+                    help="energy-drift leg: updates of a virialised Plummer sphere with the learned method; 0 skips")
+    ap.add_argument("--drift-n", type=int, default=1 << 20)
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="batched-ensemble leg (config 4): systems; 0 skips it")
     ap.add_argument("--methods", default="plain,learned,multipole",
@@ -263,6 +296,7 @@ def main():
                  "flops_per_interaction": c_f, "ms_avg": L["phases_ms_avg"]["walk"]}
 
     batch = run_batch(args.batch_systems, args.steps) if args.batch_systems > 0 else None
+    drift = run_drift(args.drift_n, args.drift_steps, L["theta"], model) if args.drift_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -310,6 +344,7 @@ def main():
         "roofline": roof,
         "roofline_walk": walk_roof,
         "batch": batch,
+        "energy_drift": drift,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
