@@ -105,7 +105,24 @@ typedef struct GeonParams {
     int32_t  kahan;             /* 1: compensated FP32 force accumulation */
     int32_t  far_field;         /* GEON_FAR_*: correction added to accepted nodes */
     int32_t  far_list_cap;      /* LEARNED: listed (particle, node) pairs per 64 particles; 0 = 65536 */
+    int32_t  signal_delay;      /* GEON_DELAY_*: FOLLOW couples it to GEON_RELATIVITY (forces.js:85) */
+    int32_t  history_stride;    /* HISTORY_STRIDE = 64 updates per history sample (config.js:122); 0 = 64 */
 } GeonParams;
+
+/* Signal delay (signal-delay.js:16-221): every leaf source acts from the point
+ * where the observer's past light cone crosses its recorded trajectory, a
+ * 256-sample history per particle taken every history_stride updates
+ * (integrator.js:1549-1565).  A source with fewer than 2 samples exerts no
+ * leaf force (forces.js:652-655); despawned and removed particles keep acting
+ * from their history until 2 sqrt(W^2 + H^2) time units after their death
+ * (forces.js:128-153; main.js:875-885).  The relativistic aberration factor
+ * (1 - n.v)^-3 scales the delayed pair's position-dependent forces
+ * (forces.js:182-189).  Accepted nodes use current-time aggregates, as in
+ * the reference.  FOLLOW (the reference) turns it on exactly when
+ * GEON_RELATIVITY is on. */
+#define GEON_DELAY_FOLLOW 0
+#define GEON_DELAY_OFF    1
+#define GEON_DELAY_ON     2
 
 /* Far-field correction of accepted Barnes-Hut nodes (not in the reference,
  * whose nodes carry monopoles only, quadtree.js:272-309): NONE reproduces the
@@ -150,6 +167,7 @@ typedef struct GeonStats {
     double field_px, field_py;
     double sim_time;
     int32_t alive_count;
+    int32_t dead_count;         /* retired particles still acting through signal delay */
 } GeonStats;
 
 /* Work counters of the last force pass, for roofline accounting. */
@@ -213,6 +231,17 @@ int  geon_update(GeonHandle h, double dt, int32_t* substeps_taken);
  * computeAllForces + external fields).  For parity checks. */
 int  geon_compute_forces(GeonHandle h);
 int  geon_get_forces(GeonHandle h, GeonForceView* v, int32_t n_slots);
+
+/* One slot's signal-delay history, oldest sample first (a live particle or
+ * one retired within the last 2 sqrt(W^2 + H^2) time units): position,
+ * coordinate velocity, angw and time, as Particle.histX/Y/Vx/Vy/AngW/Time
+ * hold them (particle.js:77-85).  Up to 256 samples; the count goes to
+ * *count.  No counterpart in the reference's interface: for inspection and
+ * parity checks. */
+typedef struct GeonHistoryView {
+    double *x, *y, *vx, *vy, *angw, *t;
+} GeonHistoryView;
+int  geon_get_history(GeonHandle h, int32_t slot, GeonHistoryView* out, int32_t* count);
 
 /* requestStats + readStats (gpu-physics.js:4146-4281), in FP64. */
 int  geon_get_stats(GeonHandle h, GeonStats* out);

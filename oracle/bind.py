@@ -55,6 +55,16 @@ def lib():
         L.orc_far_training_samples.restype = C.c_int64
         L.orc_far_loss_grad.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_double, _DP]
         L.orc_far_loss_grad.restype = C.c_double
+        L.orc_get_history.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [_DP] * 6
+        L.orc_get_history.restype = C.c_int32
+        L.orc_set_history.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [_DP] * 6
+        L.orc_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.orc_get_f1pn.argtypes = [C.c_void_p, _DP, _DP]
+        L.orc_dead_count.argtypes = [C.c_void_p]
+        L.orc_dead_count.restype = C.c_int32
+        L.orc_set_lifetime.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
+        L.orc_delayed_state.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, _DP]
+        L.orc_delayed_state.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -202,6 +212,42 @@ class Oracle:
             ii = np.ascontiguousarray(idx, dtype=np.int32)
             self.L.orc_direct_forces(self.s, ii.ctypes.data_as(_IP), len(ii), _d(fx), _d(fy))
         return fx, fy
+
+
+    # ---- signal delay ----
+    HIST_KEYS = ("x", "y", "vx", "vy", "angw", "t")
+
+    def history(self, i, dead=False):
+        """Particle i's history (or the i-th dead record's), oldest first."""
+        out = {k: np.zeros(A.HISTORY_SIZE) for k in self.HIST_KEYS}
+        n = self.L.orc_get_history(self.s, int(i), int(dead), *[_d(out[k]) for k in self.HIST_KEYS])
+        return {k: v[:n] for k, v in out.items()}
+
+    def set_history(self, i, h):
+        n = len(h["t"])
+        arrs = [np.ascontiguousarray(h[k], dtype=np.float64) for k in self.HIST_KEYS]
+        self.L.orc_set_history(self.s, int(i), n, *[_d(a) for a in arrs])
+
+    def f1pn(self):
+        n = self.count()
+        fx, fy = np.zeros(n), np.zeros(n)
+        self.L.orc_get_f1pn(self.s, _d(fx), _d(fy))
+        return fx, fy
+
+    def set_time(self, t):
+        self.L.orc_set_time(self.s, float(t))
+
+    def dead_count(self):
+        return self.L.orc_dead_count(self.s)
+
+    def set_lifetime(self, i, creation=-np.inf, death=np.inf):
+        self.L.orc_set_lifetime(self.s, int(i), float(creation), float(death))
+
+    def delayed_state(self, i, ox, oy):
+        """getDelayedState of source i seen from (ox, oy) now: (x, y, vx, vy, angw) or None."""
+        out = np.zeros(5)
+        ok = self.L.orc_delayed_state(self.s, int(i), float(ox), float(oy), _d(out))
+        return out if ok == 1 else None
 
 
 def set_threads(n):

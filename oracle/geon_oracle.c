@@ -14,7 +14,10 @@
  *   src/integrator.js  _syncToggles 138-150, _addGhost/_generateGhosts 178-244,
  *                      _buildTree 247-260, external fields 553-592,
  *                      update 726-1825 (radiation, bosons, collisions-merge,
- *                      Hawking, disintegration and signal delay left out)
+ *                      Hawking and disintegration left out), _retireParticle
+ *                      152-173, history sampling 1549-1565
+ *   src/signal-delay.js getDelayedState 16-221
+ *   main.js            dead-particle GC 874-885
  *   src/topology.js    minImage 18-76, wrapPosition 81-135
  *   src/relativity.js  angwToAngVel 7-9, setVelocity 15-25
  *   src/energy.js      computeEnergies 11-195 (passes 1-3)
@@ -22,9 +25,9 @@
  *   main.js            addParticle normalization 504-539
  *
  * Parity pins for this file live in tests/test_oracle.py (closed forms,
- * invariants, brute force).  Signal delay (signal-delay.js) is not restated:
- * with GEON_RELATIVITY on, this oracle uses current-time sources (the
- * reference couples signal delay to relativity, forces.js:85).
+ * invariants, brute force).  Signal delay follows GeonParams.signal_delay:
+ * FOLLOW couples it to GEON_RELATIVITY as the reference does (forces.js:85);
+ * OFF gives current-time leaf sources.
  */
 #include "geon_oracle.h"
 #include <math.h>
@@ -51,6 +54,14 @@
 #define SELFGRAV_PHI_MAX 0.25      /* config.js:92 */
 #define AXION_COUPLING 0.05        /* config.js:96 */
 #define HIGGS_AXION_COUPLING 0.01  /* config.js:97 */
+#define HIST_N 256                 /* HISTORY_SIZE (config.js:120) */
+#define HIST_MASK (HIST_N - 1)     /* HISTORY_MASK (config.js:121) */
+
+/* Signal-delay history ring of one particle (particle.js:77-85, 124-134). */
+typedef struct {
+    double x[HIST_N], y[HIST_N], vx[HIST_N], vy[HIST_N], aw[HIST_N], t[HIST_N];
+    int head, count;
+} Hist;
 
 /* One scalar field on a G x G grid (scalar-field.js:13-64). */
 typedef struct {
@@ -77,6 +88,8 @@ typedef struct {
     int32_t id;                    /* upload index */
     int32_t q_quanta;              /* round(charge / 0.1) */
     int32_t isGhost, original;     /* ghost: index of the real particle */
+    Hist* hist;                    /* NULL until the first sample (particle.js:124) */
+    double creationTime, deathTime, deathMass;   /* particle.js:17-18; _deathMass */
 } P;
 
 typedef struct {
@@ -107,6 +120,10 @@ struct OrcSys {
     int needAxMod, periodic;
     OField higgsF, axionF;
     double* farModel;              /* learned far-field model (FAR_MODEL doubles) or NULL */
+    int delay;                     /* signal delay in effect */
+    int histStride;                /* _histStride (integrator.js:111) */
+    P* dead;                       /* sim.deadParticles (main.js:103) */
+    int ndead, deadcap;
 };
 
 static int g_threads = 1;
@@ -196,11 +213,11 @@ static inline double angwToAngVel(double angw, double radius) {           /* rel
 /* forces.js: pairForce                                                */
 /* ------------------------------------------------------------------ */
 /* Observer p gets the force of a source.  Restates forces.js:163-442 without
- * the radiation jerk (radiationEnabled) and without signal-delay aberration. */
+ * the radiation jerk (radiationEnabled). */
 static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, double svy,
                       double sMass, double sCharge, double sAngVel, double sMagMoment,
                       double sAngMomentum, double sAxMod, double sYukMod, double sHiggsMod,
-                      int accumulatePE) {
+                      int accumulatePE, int signalDelayed) {
     const GeonParams* g = &s->prm;
     double rx, ry;
     if (s->periodic) {
@@ -215,7 +232,14 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
     const double invR = sqrt(invRSq);
     const double invR3 = invR * invRSq;
     const double invR5 = invR3 * invRSq;
-    const double invR3a = invR3, invR5a = invR5;     /* aberr = 1 (no signal delay) */
+    double aberr = 1;                                                     /* forces.js:182-189 */
+    if (signalDelayed) {
+        const double nDotV = -(rx * svx + ry * svy) * invR;
+        const double denom = fmax(1 - nDotV, 0.01);
+        aberr = fmin(1 / (denom * denom * denom), 100);
+    }
+    const double invR3a = signalDelayed ? invR3 * aberr : invR3;
+    const double invR5a = signalDelayed ? invR5 * aberr : invR5;
     const double crossSV = svx * ry - svy * rx;
     const double pMagMoment = p->magMoment, pAngMomentum = p->angMomentum;
 
@@ -306,7 +330,7 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
         const double expMuR = exp(-mu * r);
         const double ym = s->needAxMod ? sqrt(p->yukMod * sYukMod) : 1;
         const double G = g->yukawa_coupling;
-        const double fDir = G * ym * p->mass * sMass * expMuR * (invRSq + mu * invR) * invR;
+        const double fDir = G * ym * p->mass * sMass * expMuR * (invRSq + mu * invR) * (signalDelayed ? invR * aberr : invR);
         p->fx += rx * fDir; p->fy += ry * fDir;
         if (accumulatePE) p->pe -= G * ym * p->mass * sMass * expMuR * invR;
         if (s->onePN) {
@@ -338,6 +362,8 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
         p->tidal -= TIDAL_STRENGTH * coupling * coupling * ri5 * invR6 * dw;
     }
 }
+
+#include "geon_oracle_delay.inc"
 
 /* ------------------------------------------------------------------ */
 /* quadtree.js                                                         */
@@ -841,7 +867,8 @@ static inline int window_ok(const OrcSys* s, const Pool* q, int ni, double px, d
     return fabs(cx) + q->bw[ni] <= 0.5 * W && fabs(cy) + q->bh[ni] <= 0.5 * H;
 }
 
-/* calculateForce (forces.js:622-694), sources at current time. */
+/* calculateForce (forces.js:622-694); leaf sources from their past light cone
+ * when signal delay is on (651-672), accepted nodes at the current time. */
 static void calculateForce(OrcSys* s, int pi, int32_t* stack, int64_t* cnt) {
     const Pool* q = &s->pool;
     P* p = &s->p[pi];
@@ -867,15 +894,25 @@ static void calculateForce(OrcSys* s, int pi, int32_t* stack, int64_t* cnt) {
                 if (o->original == pi) continue;    /* other === particle, or its own ghost */
                 const P* real = &s->p[o->original];
                 if (!o->isGhost) o = real;          /* a real source is the live object (fresh μ, L) */
-                pairForce(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
-                          o->magMoment, o->angMomentum, real->axMod, real->yukMod, real->higgsMod, 1);
+                if (s->delay) {
+                    Delayed d;
+                    if (!delayedState(s, real, px, py, &d)) continue;
+                    const double sx = d.x + (o->x - real->x), sy = d.y + (o->y - real->y);   /* ghost shift */
+                    const double r2 = real->bodyRadiusSq;
+                    const double av = d.angw / sqrt(1 + d.angw * d.angw * r2);
+                    pairForce(s, p, sx, sy, d.vx, d.vy, o->mass, o->charge, av, MAG_MOMENT_K * o->charge * av * r2,
+                              INERTIA_K * o->mass * av * r2, real->axMod, real->yukMod, real->higgsMod, 1, 1);
+                } else {
+                    pairForce(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
+                              o->magMoment, o->angMomentum, real->axMod, real->yukMod, real->higgsMod, 1, 0);
+                }
                 cnt[0]++;
             }
         } else if (q->divided[nodeIdx] && (size * size < thetaSq * dSq) && window_ok(s, q, nodeIdx, px, py, dx, dy)) {
             const double nodeMass = q->mass[nodeIdx];
             pairForce(s, p, q->comX[nodeIdx], q->comY[nodeIdx], q->momX[nodeIdx] / nodeMass,
                       q->momY[nodeIdx] / nodeMass, nodeMass, q->charge[nodeIdx], 0,
-                      q->magMom[nodeIdx], q->angMom[nodeIdx], 1, 1, 1, 1);
+                      q->magMom[nodeIdx], q->angMom[nodeIdx], 1, 1, 1, 1, 0);
             if (s->prm.far_field == GEON_FAR_MULTIPOLE) far_correction(s, p, dx, dy, &q->mom[(size_t)nodeIdx * 9]);
             if (s->prm.far_field == GEON_FAR_LEARNED && s->farModel) far_learned(s, p, dx, dy, nodeIdx);
             cnt[1]++;
@@ -1010,7 +1047,7 @@ static void resetForces(OrcSys* s) {                                      /* for
     }
 }
 
-/* computeAllForces (forces.js:61-154) without the dead-particle loop. */
+/* computeAllForces (forces.js:61-154). */
 static void computeAllForces(OrcSys* s) {
     const int n = s->n;
     for (int i = 0; i < n; i++) {                                         /* forces.js:71-76 */
@@ -1038,10 +1075,27 @@ static void computeAllForces(OrcSys* s) {
             for (int j = 0; j < n; j++) {
                 if (i == j) continue;
                 const P* o = &s->p[j];
-                pairForce(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
-                          o->magMoment, o->angMomentum, o->axMod, o->yukMod, o->higgsMod, 1);
+                if (s->delay) {
+                    Delayed d;
+                    if (!delayedState(s, o, p->x, p->y, &d)) continue;
+                    const double r2 = o->bodyRadiusSq;
+                    const double av = d.angw / sqrt(1 + d.angw * d.angw * r2);
+                    pairForce(s, p, d.x, d.y, d.vx, d.vy, o->mass, o->charge, av, MAG_MOMENT_K * o->charge * av * r2,
+                              INERTIA_K * o->mass * av * r2, o->axMod, o->yukMod, o->higgsMod, 1, 1);
+                } else {
+                    pairForce(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
+                              o->magMoment, o->angMomentum, o->axMod, o->yukMod, o->higgsMod, 1, 0);
+                }
                 c0++;
             }
+        }
+    }
+    if (s->delay && s->ndead > 0) {                                       /* forces.js:128-153 */
+        #pragma omp parallel for num_threads(g_threads) schedule(dynamic, 16) reduction(+:c0)
+        for (int i = 0; i < n; i++) {
+            int64_t cnt[1] = {0};
+            deadForces(s, &s->p[i], cnt);
+            c0 += cnt[0];
         }
     }
     s->cnt_pair = c0; s->cnt_node = c1; s->cnt_visit = c2;
@@ -1107,8 +1161,9 @@ static void accum1PN(const OrcSys* s, P* p, double sx, double sy, double svx, do
     }
 }
 
-/* compute1PN (forces.js:578-610) and its tree walk (511-571), sources at the
- * current time. */
+/* compute1PN (forces.js:578-610) and its tree walk (511-571).  With signal
+ * delay on, non-ghost leaf sources (and every direct-sum source) act from
+ * their past light cone (542-546, 600-602); otherwise at the current time. */
 static void compute1PN(OrcSys* s) {
     const int n = s->n;
     for (int i = 0; i < n; i++) s->p[i].f1x = s->p[i].f1y = 0;
@@ -1138,7 +1193,13 @@ static void compute1PN(OrcSys* s) {
                             const P* o = &s->tp[q->points[base + k]];
                             if (o->original == pi) continue;
                             const P* real = &s->p[o->original];
-                            accum1PN(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, real->yukMod, real->higgsMod);
+                            if (s->delay && !o->isGhost) {
+                                Delayed d;
+                                if (!delayedState(s, real, p->x, p->y, &d)) continue;
+                                accum1PN(s, p, d.x, d.y, d.vx, d.vy, o->mass, o->charge, real->yukMod, real->higgsMod);
+                            } else {
+                                accum1PN(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, real->yukMod, real->higgsMod);
+                            }
                         }
                     } else if (q->divided[ni] && size * size < thetaSq * dSq && window_ok(s, q, ni, p->x, p->y, dx, dy)) {
                         const double m = q->mass[ni];
@@ -1158,7 +1219,13 @@ static void compute1PN(OrcSys* s) {
             for (int j = 0; j < n; j++) {
                 if (i == j) continue;
                 const P* o = &s->p[j];
-                accum1PN(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->yukMod, o->higgsMod);
+                if (s->delay) {
+                    Delayed d;
+                    if (!delayedState(s, o, p->x, p->y, &d)) continue;
+                    accum1PN(s, p, d.x, d.y, d.vx, d.vy, o->mass, o->charge, o->yukMod, o->higgsMod);
+                } else {
+                    accum1PN(s, p, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->yukMod, o->higgsMod);
+                }
             }
         }
     }
@@ -1183,6 +1250,7 @@ static void syncToggles(OrcSys* s) {                                      /* int
     const double muMin = s->higgs ? s->prm.yukawa_mu * HIGGS_MASS_FLOOR : s->prm.yukawa_mu;
     s->yukCutoffSq = s->yuk ? (6 / muMin) * (6 / muMin) : INFINITY;       /* forces.js:82-83 */
     s->periodic = s->prm.boundary_mode == GEON_BOUND_LOOP;
+    s->delay = s->prm.signal_delay == GEON_DELAY_ON || (s->prm.signal_delay == GEON_DELAY_FOLLOW && s->rel);
 }
 
 OrcSys* orc_create(const GeonParams* prm, int32_t capacity) {
@@ -1204,6 +1272,9 @@ void orc_destroy(OrcSys* s) {
     of_free(&s->axionF);
     pool_free(&s->pool);
     free(s->farModel);
+    for (int i = 0; i < s->n; i++) free(s->p[i].hist);
+    dead_free(s);
+    free(s->dead);
     free(s->p); free(s->tp); free(s);
 }
 
@@ -1213,6 +1284,9 @@ int orc_set_params(OrcSys* s, const GeonParams* prm) { s->prm = *prm; syncToggle
  * directly (the GPU contract passes w, gpu-physics.js:1857). */
 int orc_upload(OrcSys* s, int32_t n, const double* x, const double* y, const double* wx,
                const double* wy, const double* mass, const double* charge, const double* angw) {
+    for (int i = 0; i < s->n; i++) free(s->p[i].hist);
+    dead_free(s);
+    s->histStride = 0;
     if (n > s->capacity) {
         s->capacity = n;
         s->p = (P*)realloc(s->p, sizeof(P) * (size_t)n);
@@ -1238,6 +1312,8 @@ int orc_upload(OrcSys* s, int32_t n, const double* x, const double* y, const dou
         p->angVel = s->rel ? angwToAngVel(p->angw, p->radius) : p->angw;
         p->axMod = p->yukMod = p->higgsMod = 1;
         p->id = i;
+        p->creationTime = -INFINITY;                /* loaded particles always existed (save-load.js:231) */
+        p->deathTime = INFINITY;
     }
     s->forcesInit = 0;
     s->simTime = 0;
@@ -1261,9 +1337,23 @@ int orc_download(OrcSys* s, double* x, double* y, double* wx, double* wy, double
 /* Tree rebuild (integrator.js:1340-1348) then resetForces, _syncAxionField and
  * computeAllForces (1490-1493).  The tree's μ and L aggregates use the moments
  * cached by the previous force pass: computeAllForces re-caches them only after
- * the tree is built (forces.js:71-76). */
-static void forcePass(OrcSys* s) {
-    if (s->barnesHut) buildTree(s);
+ * the tree is built (forces.js:71-76).  reuseTree: the tree compute1PN built at
+ * these positions stands, its aggregates from before the 1PN kick, refreshed
+ * only when Higgs changed the masses (integrator.js:1341-1344). */
+static void forcePass(OrcSys* s, int reuseTree) {
+    if (s->barnesHut) {
+        if (!reuseTree) buildTree(s);
+        else if (s->higgs) {
+            for (int i = 0; i < s->ntp; i++) {
+                if (s->tp[i].isGhost) continue;
+                const int orig = s->tp[i].original;
+                s->tp[i] = s->p[orig];
+                s->tp[i].isGhost = 0; s->tp[i].original = orig;
+            }
+            pool_massDistribution(&s->pool, s->tp, s->root);
+            pool_moments(&s->pool, s->tp, s->root);
+        }
+    }
     resetForces(s);
     if (s->axion) of_axmod(s, &s->axionF);                               /* _syncAxionField (integrator.js:595-605) */
     else for (int i = 0; i < s->n; i++) s->p[i].axMod = s->p[i].yukMod = 1;
@@ -1281,7 +1371,7 @@ int orc_compute_forces(OrcSys* s) {
     syncToggles(s);
     for (int i = 0; i < s->n; i++)
         s->p[i].angVel = s->rel ? angwToAngVel(s->p[i].angw, s->p[i].radius) : s->p[i].angw;
-    forcePass(s);
+    forcePass(s, 0);
     s->forcesInit = 1;
     return 0;
 }
@@ -1330,7 +1420,7 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
     const int relOn = s->rel;
     if (!s->forcesInit && n > 0) {                                        /* integrator.js:750-782 */
         for (int i = 0; i < n; i++) s->p[i].angVel = relOn ? angwToAngVel(s->p[i].angw, s->p[i].radius) : s->p[i].angw;
-        forcePass(s);
+        forcePass(s, 0);
         s->forcesInit = 1;
     }
     const int hasGrav = s->grav, hasMagnetic = s->mag, hasGM = s->gm, hasExtBz = g->ext_bz != 0;
@@ -1472,7 +1562,14 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
             of_modulate_masses(s, &s->higgsF, dtSub);
         }
         if (s->axion) of_update(s, &s->axionF, dtSub, s->higgs ? &s->higgsF : NULL);   /* 1331-1334 */
-        forcePass(s);                                                     /* integrator.js:1489-1493 */
+        forcePass(s, has1PN && s->barnesHut);                             /* integrator.js:1489-1493 */
+    }
+    if (s->delay && n > 0) {                                              /* integrator.js:1549-1565 */
+        const int stride = g->history_stride > 0 ? g->history_stride : 64;
+        if (++s->histStride >= stride) {
+            s->histStride -= stride;
+            for (int i = 0; i < n; i++) hist_push(&s->p[i], s->simTime);
+        }
     }
     s->potentialEnergy = 0;
     for (int i = 0; i < n; i++) s->potentialEnergy += s->p[i].pe;
@@ -1488,7 +1585,10 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
         P* p = &s->p[i];
         if (g->boundary_mode == GEON_BOUND_DESPAWN) {
             if (p->x < left - DESPAWN_MARGIN || p->x > right + DESPAWN_MARGIN ||
-                p->y < top - DESPAWN_MARGIN || p->y > bottom + DESPAWN_MARGIN) continue;
+                p->y < top - DESPAWN_MARGIN || p->y > bottom + DESPAWN_MARGIN) {
+                retireParticle(s, p);
+                continue;
+            }
         } else if (g->boundary_mode == GEON_BOUND_LOOP) {
             wrapPosition(p, g->topology, g->domain_w, g->domain_h);
         } else if (g->boundary_mode == GEON_BOUND_BOUNCE) {
@@ -1499,8 +1599,61 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
         w++;
     }
     s->n = w;
+    dead_gc(s);                 /* once per update here; once per frame in main.js:874-885 */
     if (substeps) *substeps = totalSteps;
     return 0;
+}
+
+/* ---- signal-delay state for tests ---- */
+int32_t orc_get_history(OrcSys* s, int32_t i, int32_t dead, double* x, double* y, double* vx, double* vy,
+                        double* aw, double* t) {
+    const P* p = dead ? (i < s->ndead ? &s->dead[i] : NULL) : (i < s->n ? &s->p[i] : NULL);
+    if (!p || !p->hist) return 0;
+    const Hist* h = p->hist;
+    const int start = (h->head - h->count + HIST_N) & HIST_MASK;
+    for (int k = 0; k < h->count; k++) {
+        const int j = (start + k) & HIST_MASK;
+        x[k] = h->x[j]; y[k] = h->y[j]; vx[k] = h->vx[j]; vy[k] = h->vy[j]; aw[k] = h->aw[j]; t[k] = h->t[j];
+    }
+    return h->count;
+}
+
+int orc_set_history(OrcSys* s, int32_t i, int32_t count, const double* x, const double* y, const double* vx,
+                    const double* vy, const double* aw, const double* t) {
+    if (i < 0 || i >= s->n || count < 0 || count > HIST_N) return -1;
+    P* p = &s->p[i];
+    hist_init(p);
+    memset(p->hist, 0, sizeof(Hist));
+    for (int k = 0; k < count; k++) {
+        Hist* h = p->hist;
+        h->x[k] = x[k]; h->y[k] = y[k]; h->vx[k] = vx[k]; h->vy[k] = vy[k]; h->aw[k] = aw[k]; h->t[k] = t[k];
+    }
+    p->hist->count = count;
+    p->hist->head = count & HIST_MASK;
+    return 0;
+}
+
+int orc_get_f1pn(OrcSys* s, double* f1x, double* f1y) {
+    for (int i = 0; i < s->n; i++) { f1x[i] = s->p[i].f1x; f1y[i] = s->p[i].f1y; }
+    return 0;
+}
+
+void orc_set_time(OrcSys* s, double t) { s->simTime = t; }
+int32_t orc_dead_count(OrcSys* s) { return s->ndead; }
+int orc_set_lifetime(OrcSys* s, int32_t i, double creation, double death) {
+    if (i < 0 || i >= s->n) return -1;
+    s->p[i].creationTime = creation;
+    s->p[i].deathTime = death;
+    return 0;
+}
+
+int orc_delayed_state(OrcSys* s, int32_t i, double ox, double oy, double* out5) {
+    syncToggles(s);
+    if (i < 0 || i >= s->n) return -1;
+    Delayed d;
+    if (!delayedState(s, &s->p[i], ox, oy, &d)) return 0;
+    out5[0] = d.x; out5[1] = d.y; out5[2] = d.vx; out5[3] = d.vy; out5[4] = d.angw;
+    return 1;
 }
 
 /* computeEnergies (energy.js:11-195), particle passes 1-3; PE from the last
@@ -1704,7 +1857,7 @@ int orc_direct_forces(OrcSys* s, const int32_t* idx, int32_t n_idx, double* fx, 
             if (j == i) continue;
             const P* o = &s->p[j];
             pairForce(s, &tmp, o->x, o->y, o->vx, o->vy, o->mass, o->charge, o->angVel,
-                      o->magMoment, o->angMomentum, o->axMod, o->yukMod, o->higgsMod, 1);
+                      o->magMoment, o->angMomentum, o->axMod, o->yukMod, o->higgsMod, 1, 0);
         }
         fx[k] = tmp.fx; fy[k] = tmp.fy;
     }

@@ -75,6 +75,24 @@ int64_t orc_far_training_samples(OrcSys* s, const int32_t* idx, int32_t n_idx, d
                                  float* out, int32_t* obs, int64_t cap);
 double  orc_far_loss_grad(const float* model, const float* samples, int64_t n, double wscale, double* grad);
 
+/* Signal delay (signal-delay.js).  History of live particle i (dead = 0) or
+ * of the i-th dead record (dead = 1), oldest first; returns the count.
+ * set_history replaces live particle i's ring with `count` samples in time
+ * order; set_time sets the simulation time; set_lifetime the creation and
+ * death times; delayed_state evaluates getDelayedState for source i and the
+ * observer (ox, oy) at the current time: 1 and out5 = x, y, vx, vy, angw, or
+ * 0 where the reference returns null. */
+int32_t orc_get_history(OrcSys* s, int32_t i, int32_t dead, double* x, double* y, double* vx, double* vy,
+                        double* aw, double* t);
+int     orc_set_history(OrcSys* s, int32_t i, int32_t count, const double* x, const double* y,
+                        const double* vx, const double* vy, const double* aw, const double* t);
+void    orc_set_time(OrcSys* s, double t);
+/* force1PN of the last force pass, array order. */
+int     orc_get_f1pn(OrcSys* s, double* f1x, double* f1y);
+int32_t orc_dead_count(OrcSys* s);
+int     orc_set_lifetime(OrcSys* s, int32_t i, double creation, double death);
+int     orc_delayed_state(OrcSys* s, int32_t i, double ox, double oy, double* out5);
+
 /* Scalar fields (which: 0 Higgs, 1 axion). */
 int     orc_field_set(OrcSys* s, int32_t which, const double* field, const double* field_dot);
 int     orc_field_get(OrcSys* s, int32_t which, int32_t what, double* out);

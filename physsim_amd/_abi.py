@@ -15,6 +15,8 @@ BOUND_DESPAWN, BOUND_BOUNCE, BOUND_LOOP = 0, 1, 2
 TORUS, KLEIN, RP2 = 0, 1, 2
 PERIODIC_MIN_IMAGE, PERIODIC_GHOSTS, PERIODIC_WINDOWED = 0, 1, 2
 FAR_NONE, FAR_MULTIPOLE, FAR_LEARNED = 0, 1, 2
+DELAY_FOLLOW, DELAY_OFF, DELAY_ON = 0, 1, 2
+HISTORY_SIZE = 256
 FAR_PARAMS, FAR_MODEL_FLOATS, FAR_SAMPLE = 5574, 5600, 20
 FIELD_HIGGS, FIELD_AXION = 0, 1
 FQ = {"field": 0, "field_dot": 1, "grad_x": 2, "grad_y": 3, "sg_phi": 4, "sg_gx": 5, "sg_gy": 6,
@@ -40,6 +42,7 @@ class GeonParams(C.Structure):
         ("hubble", C.c_double), ("ext_gravity", C.c_double), ("ext_gravity_angle", C.c_double),
         ("ext_electric", C.c_double), ("ext_electric_angle", C.c_double), ("ext_bz", C.c_double),
         ("field_grid", C.c_int32), ("kahan", C.c_int32), ("far_field", C.c_int32), ("far_list_cap", C.c_int32),
+        ("signal_delay", C.c_int32), ("history_stride", C.c_int32),
     ]
 
 
@@ -68,7 +71,11 @@ class GeonStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "linear_ke", "spin_ke", "px", "py", "orbital_ang_mom", "spin_ang_mom", "com_x", "com_y",
         "total_mass", "pe", "field_energy", "higgs_field_energy", "axion_field_energy", "pfi_energy",
-        "field_px", "field_py", "sim_time")] + [("alive_count", C.c_int32)]
+        "field_px", "field_py", "sim_time")] + [("alive_count", C.c_int32), ("dead_count", C.c_int32)]
+
+
+class GeonHistoryView(C.Structure):
+    _fields_ = [("x", _DP), ("y", _DP), ("vx", _DP), ("vy", _DP), ("angw", _DP), ("t", _DP)]
 
 
 class GeonCounters(C.Structure):
@@ -108,6 +115,7 @@ PROTOTYPES = {
     "geon_compute_forces": (C.c_int, [H]),
     "geon_get_forces": (C.c_int, [H, C.POINTER(GeonForceView), C.c_int32]),
     "geon_get_stats": (C.c_int, [H, C.POINTER(GeonStats)]),
+    "geon_get_history": (C.c_int, [H, C.c_int32, C.POINTER(GeonHistoryView), C.POINTER(C.c_int32)]),
     "geon_build_tree": (C.c_int, [H]),
     "geon_get_tree": (C.c_int, [H, C.POINTER(GeonTreeNode), C.c_int64, C.POINTER(C.c_int64)]),
     "geon_set_field": (C.c_int, [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),

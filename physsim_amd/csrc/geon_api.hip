@@ -97,6 +97,19 @@ struct GeonContext {
     double* far_f2 = nullptr;
     bool far_have_model = false, far_have_stats = false;
     int64_t far_adam_t = 0;
+    // signal delay: per-slot history rings (allocated at the first sample),
+    // slot metadata, event times, retired sources (device + host mirror)
+    float4* hist4 = nullptr;
+    float* hist_aw = nullptr;
+    int2* hist_fe = nullptr;
+    double2* hist_cd = nullptr;
+    double* ev_time = nullptr;
+    int64_t ev_cap = 0, n_events = 0;
+    int hist_stride_ctr = 0;
+    DeadRec* dead = nullptr;
+    int n_dead = 0;
+    std::vector<DeadRec> dead_host;
+    std::vector<double> dead_time;
 };
 
 #define HIP_OK(expr)                                                                         \
@@ -163,7 +176,15 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     if (p->max_substeps < 1) { *why = "max_substeps must be >= 1"; return 1; }
     if (p->far_field < GEON_FAR_NONE || p->far_field > GEON_FAR_LEARNED) { *why = "unknown far_field mode"; return 1; }
     if (p->far_list_cap < 0) { *why = "far_list_cap must be >= 0"; return 1; }
+    if (p->signal_delay < GEON_DELAY_FOLLOW || p->signal_delay > GEON_DELAY_ON) { *why = "unknown signal_delay mode"; return 1; }
+    if (p->history_stride < 0) { *why = "history_stride must be >= 0"; return 1; }
+    const bool delay = p->signal_delay == GEON_DELAY_ON || (p->signal_delay == GEON_DELAY_FOLLOW && (t & GEON_RELATIVITY));
+    if (delay && p->far_field != GEON_FAR_NONE) { *why = "signal delay runs with monopole nodes only (far_field NONE)"; return 1; }
     return 0;
+}
+
+bool delay_on(const GeonParams& p) {
+    return p.signal_delay == GEON_DELAY_ON || (p.signal_delay == GEON_DELAY_FOLLOW && (p.toggles0 & GEON_RELATIVITY));
 }
 
 PhysFlags phys_flags(const GeonParams& p) {
@@ -266,7 +287,57 @@ ForceArgs force_args(GeonContext* h) {
     a.far_list = h->far_list;
     a.far_count = h->far_count;
     a.far_cap = h->far_cap;
+    a.hist.h4 = h->hist4; a.hist.haw = h->hist_aw;
+    a.hist.fe = h->hist_fe; a.hist.cd = h->hist_cd;
+    a.hist.ev_time = h->ev_time;
+    a.hist.slot = I(h, C_SLOT);
+    a.hist.dead = h->dead;
+    a.hist.n_events = h->n_events;
+    a.hist.n_dead = h->n_dead;
+    // no leaf source has two samples before the rings exist (every leaf skipped)
+    a.hist.on = delay_on(h->prm) ? 1 : 0;
     return a;
+}
+
+HistWrite hist_write(GeonContext* h) {
+    HistWrite w;
+    w.h4 = h->hist4; w.haw = h->hist_aw; w.fe = h->hist_fe; w.cd = h->hist_cd; w.ev_time = h->ev_time;
+    w.dead = h->dead;
+    return w;
+}
+
+int hist_alloc(GeonContext* h) {
+    if (h->hist4) return GEON_OK;
+    int rc;
+    if ((rc = dmalloc(h, &h->hist4, (size_t)h->cap * kHistN))) return rc;
+    if ((rc = dmalloc(h, &h->hist_aw, (size_t)h->cap * kHistN))) return rc;
+    return GEON_OK;
+}
+
+// Room for event n_events in ev_time (grown by doubling, contents kept).
+int ev_reserve(GeonContext* h) {
+    if (h->n_events < h->ev_cap) return GEON_OK;
+    const int64_t cap = h->ev_cap ? 2 * h->ev_cap : 4096;
+    double* nb = nullptr;
+    int rc;
+    if ((rc = dmalloc(h, &nb, (size_t)cap))) return rc;
+    if (h->ev_time) {
+        HIP_OK(hipMemcpyAsync(nb, h->ev_time, sizeof(double) * (size_t)h->n_events, hipMemcpyDeviceToDevice, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        hipFree(h->ev_time);
+    }
+    h->ev_time = nb;
+    h->ev_cap = cap;
+    return GEON_OK;
+}
+
+// Forget every history and retired source (upload, reset).
+void hist_clear(GeonContext* h) {
+    h->n_events = 0;
+    h->hist_stride_ctr = 0;
+    h->n_dead = 0;
+    h->dead_host.clear();
+    h->dead_time.clear();
 }
 
 bool want_moments(GeonContext* h) {
@@ -428,6 +499,7 @@ __global__ void k_begin_update(Ctrl* c, double dt, int max_sub, double sim_time)
 }
 
 __global__ void k_set_n(Ctrl* c, int n) { c->n_alive = n; }
+__global__ void k_set_time(Ctrl* c, double t) { c->sim_time = t; }
 
 __global__ void k_survivor_perm(const int32_t* alive, const int32_t* scan, uint32_t* perm, const Ctrl* c) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -503,6 +575,7 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
         Timer t(h, 4);
         launch_far_mlp(h->stream, far_args(h), n);
     }
+    if (fa.hist.on && h->n_dead > 0) launch_dead_forces(h->stream, fa, feature_mask(h->prm));
     if (higgs || axion)   // field forces after the pair forces (integrator.js:1494-1507)
         field_forces(h->stream, field_args(h, 0), field_args(h, 1), higgs, axion, !!(tg & GEON_GRAVITY),
                      !!(tg & GEON_COULOMB), !!(tg & GEON_YUKAWA), particle_view(h), h->fc[K_FX], h->fc[K_FY],
@@ -546,6 +619,30 @@ int check_overflow(GeonContext* h) {
     }
 }
 
+// Dead-particle GC (main.js:874-885): a retired source is dropped
+// 2 sqrt(W^2 + H^2) time units after its death (once per update here, once
+// per frame in the reference).
+int dead_gc(GeonContext* h) {
+    if (h->n_dead == 0) return GEON_OK;
+    const double W = h->prm.domain_w, H = h->prm.domain_h;
+    const double max_dist = 2.0 * sqrt(W * W + H * H);
+    int w = 0;
+    for (int k = 0; k < h->n_dead; k++) {
+        if (h->sim_time - h->dead_time[k] < max_dist) {
+            h->dead_host[w] = h->dead_host[k];
+            h->dead_time[w] = h->dead_time[k];
+            w++;
+        }
+    }
+    if (w == h->n_dead) return GEON_OK;
+    h->n_dead = w;
+    h->dead_host.resize((size_t)w);
+    h->dead_time.resize((size_t)w);
+    if (w) HIP_OK(hipMemcpyAsync(h->dead, h->dead_host.data(), sizeof(DeadRec) * w, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
 int read_ctrl(GeonContext* h) {
     HIP_OK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
@@ -564,6 +661,24 @@ int compact(GeonContext* h) {
     HIP_OK(hipMemcpyAsync(h->ctrl_host, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     const int survivors = h->ctrl_host->n_tree;
+    const int deaths = n - survivors;
+    if (deaths > 0 && delay_on(h->prm)) {
+        // _retireParticle (integrator.js:152-173): death snapshots and dead records
+        int rc = hist_alloc(h);
+        if (rc) return rc;
+        const uint32_t tg = h->prm.toggles0;
+        const bool axion = !!(tg & GEON_AXION), higgs = !!(tg & GEON_HIGGS);
+        launch_retire(h->stream, hist_write(h), F(h, C_X), F(h, C_Y), F(h, C_VX), F(h, C_VY), F(h, C_ANGW),
+                      F(h, C_MASS), F(h, C_CHARGE), F(h, C_RADIUS), axion ? F(h, C_AX) : nullptr,
+                      axion ? F(h, C_YUK) : nullptr, higgs ? F(h, C_HIGGS) : nullptr, I(h, C_SLOT), h->alive,
+                      h->alive_scan, h->n_events, h->n_dead, n, h->ctrl);
+        h->dead_host.resize((size_t)(h->n_dead + deaths));
+        HIP_OK(hipMemcpyAsync(h->dead_host.data() + h->n_dead, h->dead + h->n_dead, sizeof(DeadRec) * deaths,
+                              hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        h->dead_time.resize((size_t)(h->n_dead + deaths), h->sim_time);
+        h->n_dead += deaths;
+    }
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, survivors);
     Columns cols;
     cols.ncols = 0;
@@ -599,6 +714,9 @@ int allocate(GeonContext* h) {
     if ((rc = dmalloc(h, &h->src8a, cap))) return rc;
     if ((rc = dmalloc(h, &h->src8b, cap))) return rc;
     if ((rc = dmalloc(h, &h->ctrl, 1))) return rc;
+    if ((rc = dmalloc(h, &h->hist_fe, cap))) return rc;
+    if ((rc = dmalloc(h, &h->hist_cd, cap))) return rc;
+    if ((rc = dmalloc(h, &h->dead, cap))) return rc;
     TreeBufs& tb = h->tb;
     tb.cap_particles = (int64_t)cap;
     tb.cap_divided = (int64_t)cap + 1024;
@@ -661,6 +779,8 @@ void release(GeonContext* h) {
     for (int c = 0; c < N_FORCE; c++) { hipFree(h->fc[c]); hipFree(h->fc_alt[c]); }
     hipFree(h->alive); hipFree(h->alive_scan); hipFree(h->compact_perm);
     hipFree(h->src4); hipFree(h->src8a); hipFree(h->src8b); hipFree(h->ctrl);
+    hipFree(h->hist4); hipFree(h->hist_aw); hipFree(h->hist_fe); hipFree(h->hist_cd); hipFree(h->ev_time);
+    hipFree(h->dead);
     TreeBufs& tb = h->tb;
     hipFree(tb.keys); hipFree(tb.keys_alt); hipFree(tb.perm); hipFree(tb.perm_alt); hipFree(tb.hist);
     hipFree(tb.digit_total);
@@ -823,7 +943,9 @@ int geon_reset(GeonHandle h) {
     if (!h) return GEON_EINVAL;
     h->n_host = 0; h->next_slot = 0; h->forces_init = false; h->tree_valid = false;
     h->sim_time = 0; h->substeps_total = 0;
+    hist_clear(h);
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0);
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0.0);
     reset_fields(h);
     HIP_OK(hipStreamSynchronize(h->stream));
     return GEON_OK;
@@ -880,6 +1002,10 @@ int geon_upload_soa(GeonHandle h, const GeonSoAView* v, int32_t n) {
         if (n) HIP_OK(hipMemcpyAsync(h->st[c], col[c].data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
     for (int c = 0; c < N_FORCE; c++) if (n) HIP_OK(hipMemsetAsync(h->fc[c], 0, sizeof(float) * n, h->stream));
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, n);
+    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0.0);
+    // loaded particles are treated as always existing (save-load.js:231)
+    hist_clear(h);
+    launch_hist_reset(h->stream, h->hist_fe, h->hist_cd, 0, n, -INFINITY);
     HIP_OK(hipStreamSynchronize(h->stream));
     h->n_host = n;
     h->next_slot = n;
@@ -913,6 +1039,7 @@ int32_t geon_add_particle(GeonHandle h, const GeonParticleInit* p) {
     for (int c = 0; c < N_STATE; c++) HIP_OK(hipMemcpyAsync(h->st[c] + i, &w[c], 4, hipMemcpyHostToDevice, h->stream));
     for (int c = 0; c < N_FORCE; c++) HIP_OK(hipMemsetAsync(h->fc[c] + i, 0, 4, h->stream));
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, i + 1);
+    launch_hist_reset(h->stream, h->hist_fe, h->hist_cd, slot, 1, h->sim_time);   // main.js:514 creationTime
     HIP_OK(hipStreamSynchronize(h->stream));
     h->n_host = i + 1;
     h->next_slot = slot + 1;
@@ -1031,6 +1158,17 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
         }
     }
     HIP_OK(hipGetLastError());
+    if (delay_on(h->prm) && n > 0) {
+        // signal-delay history every history_stride updates (integrator.js:1549-1565)
+        const int stride = h->prm.history_stride > 0 ? h->prm.history_stride : 64;
+        if (++h->hist_stride_ctr >= stride) {
+            h->hist_stride_ctr -= stride;
+            if ((rc = hist_alloc(h)) || (rc = ev_reserve(h))) return rc;
+            launch_hist_record(h->stream, hist_write(h), F(h, C_X), F(h, C_Y), F(h, C_VX), F(h, C_VY), F(h, C_ANGW),
+                               I(h, C_SLOT), h->n_events, n, h->ctrl);
+            h->n_events++;
+        }
+    }
     // boundary once per update (integrator.js:1798-1824)
     sa = step_args(h);
     launch_boundary(h->stream, sa, h->prm.boundary_mode, h->prm.domain_w, h->prm.domain_h);
@@ -1046,6 +1184,7 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
         if ((rc = compact(h))) return rc;
         HIP_OK(hipStreamSynchronize(h->stream));
     }
+    if ((rc = dead_gc(h))) return rc;
     if (substeps_taken) *substeps_taken = steps;
     return GEON_OK;
 }
@@ -1110,6 +1249,42 @@ int geon_get_stats(GeonHandle h, GeonStats* out) {
     }
     out->sim_time = h->sim_time;
     out->alive_count = h->n_host;
+    out->dead_count = h->n_dead;
+    return GEON_OK;
+}
+
+int geon_get_history(GeonHandle h, int32_t slot, GeonHistoryView* out, int32_t* count) {
+    if (!h || !out || !count) return GEON_EINVAL;
+    if (slot < 0 || slot >= h->next_slot) return set_err(h, GEON_EINVAL, "slot out of range");
+    int2 fe;
+    double2 cd;
+    HIP_OK(hipMemcpyAsync(&fe, h->hist_fe + slot, sizeof(fe), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(&cd, h->hist_cd + slot, sizeof(cd), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    *count = 0;
+    if (fe.x < 0 || !h->hist4) return GEON_OK;
+    const bool dead = fe.y >= 0;
+    const int64_t end = dead ? fe.y : h->n_events;
+    const int c = (int)std::min<int64_t>(end - fe.x, kHistN);
+    if (c <= 0) return GEON_OK;
+    std::vector<float4> h4(kHistN);
+    std::vector<float> aw(kHistN);
+    std::vector<double> et((size_t)h->n_events);
+    HIP_OK(hipMemcpyAsync(h4.data(), h->hist4 + (size_t)slot * kHistN, sizeof(float4) * kHistN, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(aw.data(), h->hist_aw + (size_t)slot * kHistN, sizeof(float) * kHistN, hipMemcpyDeviceToHost, h->stream));
+    if (h->n_events) HIP_OK(hipMemcpyAsync(et.data(), h->ev_time, sizeof(double) * h->n_events, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < c; k++) {
+        const int64_t e = end - c + k;
+        const int j = (int)(e & (kHistN - 1));
+        if (out->x) out->x[k] = h4[j].x;
+        if (out->y) out->y[k] = h4[j].y;
+        if (out->vx) out->vx[k] = h4[j].z;
+        if (out->vy) out->vy[k] = h4[j].w;
+        if (out->angw) out->angw[k] = aw[j];
+        if (out->t) out->t[k] = (dead && e == end - 1) ? cd.y : et[(size_t)e];
+    }
+    *count = c;
     return GEON_OK;
 }
 
@@ -1316,6 +1491,7 @@ int geon_far_clear(GeonHandle h) {
 int geon_far_collect(GeonHandle h, int32_t group_stride, int64_t* n_samples) {
     if (!h || group_stride < 1) return GEON_EINVAL;
     if (!(h->prm.toggles0 & GEON_BARNES_HUT)) return set_err(h, GEON_ESTATE, "far-field samples need Barnes-Hut on");
+    if (delay_on(h->prm)) return set_err(h, GEON_EUNSUPPORTED, "far-field samples are taken without signal delay");
     const int n = h->n_host;
     int rc;
     if ((rc = far_model_alloc(h))) return rc;

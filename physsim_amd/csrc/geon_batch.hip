@@ -102,6 +102,9 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     if (p->toggles0 & ~ok) { *why = "a batch runs gravity, Coulomb, magnetic, gravitomagnetic, Yukawa, relativity and spin-orbit only"; return 1; }
     if (!(p->toggles0 & GEON_BARNES_HUT)) { *why = "a batch runs the Barnes-Hut walk"; return 1; }
     if (p->far_field != GEON_FAR_NONE) { *why = "a batch runs monopole nodes only"; return 1; }
+    if (p->signal_delay == GEON_DELAY_ON || (p->signal_delay == GEON_DELAY_FOLLOW && (p->toggles0 & GEON_RELATIVITY))) {
+        *why = "a batch runs without signal delay (relativity needs signal_delay OFF)"; return 1;
+    }
     if (p->collision_mode != GEON_COL_PASS) { *why = "only collision mode 'pass' is run"; return 1; }
     if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
     if (p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) { *why = "only the torus topology is run"; return 1; }
@@ -220,6 +223,7 @@ ForceArgs force_args(GeonBatch_* h) {
     a.bvalid = I(h, B_VALID);
     a.bctrl = h->bctrl;
     a.broot = h->root_slot;
+    a.hist = HistArgs{};
     return a;
 }
 

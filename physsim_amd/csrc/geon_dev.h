@@ -143,9 +143,12 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
 }
 
 // One source acting on the observer: the body of pairForce (forces.js:163-442)
-// in FP32, without radiation jerk and signal-delay aberration.  (rx, ry) is the
-// (min-imaged) displacement source - observer.
-template <uint32_t F, bool KAHAN>
+// in FP32, without the radiation jerk.  (rx, ry) is the (min-imaged)
+// displacement source - observer.  ABERR: the source is signal-delayed, and the
+// Lienard-Wiechert factor min((max(1 - n.v_s, 0.01))^-3, 100) scales its
+// position-dependent forces (forces.js:182-189); PE and fields are unscaled.
+// PE = false: no potential energy (dead sources, forces.js:130).
+template <uint32_t F, bool KAHAN, bool ABERR = false, bool PE = true>
 __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const PhysFlags& pf,
                                            float rx, float ry, float sm, float sq,
                                            float svx, float svy, float s_ang_vel,
@@ -156,22 +159,30 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
     const float inv_r = __builtin_amdgcn_rsqf(r2);   // 1/sqrt(r^2 + eps^2)
     const float inv_r2 = inv_r * inv_r;
     const float inv_r3 = inv_r * inv_r2;
+    float aberr = 1.f;
+    if (ABERR) {
+        const float n_dot_v = -(rx * svx + ry * svy) * inv_r;
+        const float den = fmaxf(1.f - n_dot_v, 0.01f);
+        aberr = fminf(1.f / (den * den * den), 100.f);
+    }
+    const float inv_r3a = ABERR ? inv_r3 * aberr : inv_r3;
     if (F & F_GRAV) {
         const float k = o.m * sm;
-        const float fd = k * inv_r3;
+        const float fd = k * inv_r3a;
         a.addf(rx * fd, ry * fd);
-        a.addpe(-k * inv_r);
+        if (PE) a.addpe(-k * inv_r);
     }
     float ax_pair = 1.f;
     if ((F & F_EXTRA) && pf.ax_mod) ax_pair = sqrtf(o.ax * s_ax);
     if (F & F_COUL) {
         const float k = -(o.q * sq) * ax_pair;
-        const float fd = k * inv_r3;
+        const float fd = k * inv_r3a;
         a.addf(rx * fd, ry * fd);
-        a.addpe(o.q * sq * inv_r * ax_pair);
+        if (PE) a.addpe(o.q * sq * inv_r * ax_pair);
     }
     if (F & F_EXTRA) {
         const float inv_r5 = inv_r3 * inv_r2;
+        const float inv_r5a = ABERR ? inv_r5 * aberr : inv_r5;
         const float cross_sv = svx * ry - svy * rx;
         if (pf.one_pn && (pf.gm || pf.mag)) {                       // forces.js:249-301
             const float r = 1.f / inv_r;
@@ -188,7 +199,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float fx = base * (rx * radial + (o.vx * c1 + svx * c2) * r);
                 const float fy = base * (ry * radial + (o.vy * c1 + svy * c2) * r);
                 a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
-                a.addpe(-o.m * sm * inv_r * (1.5f * (v1sq + v2sq) - 3.5f * v1v2 - 0.5f * nv1 * nv2 + o.m * inv_r + sm * inv_r));
+                if (PE) a.addpe(-o.m * sm * inv_r * (1.5f * (v1sq + v2sq) - 3.5f * v1v2 - 0.5f * nv1 * nv2 + o.m * inv_r + sm * inv_r));
             }
             if (pf.mag) {
                 const float v2n = svx * nx + svy * ny, v1n = o.vx * nx + o.vy * ny;
@@ -196,19 +207,19 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float fx = coeff * (o.vx * v2n - 3.f * nx * v1n * v2n);
                 const float fy = coeff * (o.vy * v2n - 3.f * ny * v1n * v2n);
                 a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
-                a.addpe(-0.5f * o.q * sq * inv_r * (v1v2 + v1n * v2n));
+                if (PE) a.addpe(-0.5f * o.q * sq * inv_r * (v1v2 + v1n * v2n));
             }
         }
         if (pf.one_pn && pf.gm && pf.mag) {                         // forces.js:303-320
             const float cc = o.q * sq * (o.m + sm) - (o.q * o.q * sm + sq * sq * o.m);
             const float fd = cc * inv_r2 * inv_r2;
             a.addf(rx * fd, ry * fd); a.f1x += rx * fd; a.f1y += ry * fd;
-            a.addpe(0.5f * cc * inv_r2);
+            if (PE) a.addpe(0.5f * cc * inv_r2);
         }
         if (pf.mag) {                                               // forces.js:322-353
-            const float fd = -3.f * (o.mu * s_mu) * inv_r5 * ax_pair;
+            const float fd = -3.f * (o.mu * s_mu) * inv_r5a * ax_pair;
             a.addf(rx * fd, ry * fd);
-            a.addpe((o.mu * s_mu) * inv_r3 * ax_pair);
+            if (PE) a.addpe((o.mu * s_mu) * inv_r3 * ax_pair);
             const float bzm = sq * cross_sv * inv_r3 * ax_pair;
             a.bz += bzm;
             a.dbzdx += 3.f * bzm * rx * inv_r2 + sq * svy * inv_r3 * ax_pair;
@@ -218,9 +229,9 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             a.dbzdy -= 3.f * s_mu * ry * inv_r5 * ax_pair;
         }
         if (pf.gm) {                                                // forces.js:355-387
-            const float fd = 3.f * (o.L * s_L) * inv_r5;
+            const float fd = 3.f * (o.L * s_L) * inv_r5a;
             a.addf(rx * fd, ry * fd);
-            a.addpe(-(o.L * s_L) * inv_r3);
+            if (PE) a.addpe(-(o.L * s_L) * inv_r3);
             const float bgm = -sm * cross_sv * inv_r3;
             a.bgz += bgm;
             a.dbgzdx += 3.f * bgm * rx * inv_r2 - sm * svy * inv_r3;
@@ -236,9 +247,9 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             const float e = __expf(-mu * r);
             const float ym = pf.ax_mod ? sqrtf(o.yuk * s_yuk) : 1.f;
             const float gmm = pf.yuk_g * ym * o.m * sm * e;
-            const float fd = gmm * (inv_r2 + mu * inv_r) * inv_r;
+            const float fd = gmm * (inv_r2 + mu * inv_r) * (ABERR ? inv_r * aberr : inv_r);
             a.addf(rx * fd, ry * fd);
-            a.addpe(-gmm * inv_r);
+            if (PE) a.addpe(-gmm * inv_r);
             if (pf.one_pn) {
                 const float nx = rx * inv_r, ny = ry * inv_r;
                 const float nv1 = nx * o.vx + ny * o.vy, nv2 = nx * svx + ny * svy;
@@ -249,7 +260,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float fx = beta * (radial * nx + alpha * (nv2 * o.vx + nv1 * svx));
                 const float fy = beta * (radial * ny + alpha * (nv2 * o.vy + nv1 * svy));
                 a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
-                a.addpe(0.5f * gmm * inv_r * (v1v2 + nv1 * nv2 * alpha));
+                if (PE) a.addpe(0.5f * gmm * inv_r * (v1v2 + nv1 * nv2 * alpha));
             }
         }
     }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "geon_dev.h"
+#include "geon_delay.h"
 
 namespace geon {
 
@@ -110,6 +111,8 @@ struct ForceArgs {
     const int32_t* bvalid;
     const Ctrl* bctrl;
     const int32_t* broot;
+    // signal delay of leaf sources (hist.on), and the retired sources
+    HistArgs hist;
 };
 constexpr int kFarMultipole = 1, kFarList = 2;
 constexpr int kFarLaneShift = 26;
@@ -127,6 +130,34 @@ void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const
                             float4* src8b, int n_upper, const Ctrl* ctrl, int update_moments = 1);
 // 1PN recompute for the velocity-Verlet correction: tree walk or O(N^2).
 void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree);
+
+// ---- signal delay (k_delay.hip) ----
+// Forces of the retired sources in a.hist.dead on every live particle, added
+// to the force outputs (no PE, forces.js:128-153).
+void launch_dead_forces(hipStream_t s, const ForceArgs& a, uint32_t feat);
+// History sample of every live particle as event `ev` (integrator.js:1549-1565);
+// ev_time[ev] takes the device simulation time.
+struct HistWrite {
+    float4* h4;
+    float* haw;
+    int2* fe;
+    double2* cd;
+    double* ev_time;
+    DeadRec* dead;
+};
+void launch_hist_record(hipStream_t s, const HistWrite& w, const float* x, const float* y, const float* vx,
+                        const float* vy, const float* angw, const int32_t* slot, int64_t ev, int n,
+                        const Ctrl* ctrl);
+// _retireParticle (integrator.js:152-173) for every particle with alive[i] == 0:
+// death snapshot as its private event `ev`, dead record at dead_base + its rank
+// among the dying (alive_scan: exclusive scan of alive).
+void launch_retire(hipStream_t s, const HistWrite& w, const float* x, const float* y, const float* vx,
+                   const float* vy, const float* angw, const float* mass, const float* charge,
+                   const float* radius, const float* ax, const float* yuk, const float* higgs,
+                   const int32_t* slot, const int32_t* alive, const int32_t* alive_scan, int64_t ev,
+                   int dead_base, int n, const Ctrl* ctrl);
+// Slot metadata of new particles: no history, created at `t` (-inf: loaded).
+void launch_hist_reset(hipStream_t s, int2* fe, double2* cd, int first_slot, int count, double t);
 
 // Collision broadphase over the tree (count pass, then write pass).
 struct PairArgs {

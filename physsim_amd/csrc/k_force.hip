@@ -97,8 +97,9 @@ __device__ __forceinline__ void wave_add_counters(Ctrl* ctrl, int64_t pairs, int
 }
 
 // FAR: 0 monopole nodes (the reference), kFarMultipole, kFarList (compile-time,
-// so the reference walk carries no far-field registers).
-template <uint32_t F, bool KAHAN, int FAR, int STACK = kWalkStack>
+// so the reference walk carries no far-field registers).  DELAY: leaf sources
+// act from their past light cone (forces.js:651-672).
+template <uint32_t F, bool KAHAN, int FAR, int STACK = kWalkStack, bool DELAY = false>
 __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
     __shared__ int32_t s_node[kWavesPerBlock][STACK];
     __shared__ uint64_t s_mask[kWavesPerBlock][STACK];
@@ -123,6 +124,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
     acc.init();
     const PhysFlags pf = a.pf;
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    const double now = DELAY ? a.ctrl->sim_time : 0.0;
     int64_t c_pair = 0, c_node = 0, c_visit = 0;
     int64_t far_n = 0;                             // far list of this group (FAR == kFarList)
     int sp = 0;
@@ -159,7 +161,20 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
                 const float4 s8a = a.src8a[j];
                 float4 s8b = make_float4(0.f, 1.f, 1.f, 1.f);
                 if (F & F_EXTRA) s8b = a.src8b[j];
-                if (mine && j != i) {
+                if (DELAY) {
+                    const int sj = a.hist.slot[j];
+                    Retarded rt;
+                    if (mine && j != i && delayed_state(a.hist, pf, sj, o.x, o.y, now, rt)) {
+                        const float rx = rt.dx, ry = rt.dy;
+                        const float rad = a.radius[j];
+                        const float r2 = rad * rad;
+                        const float av = retarded_ang_vel(rt.aw, r2);
+                        pair_force<F, KAHAN, true>(o, acc, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, av,
+                                                   0.2f * s4.w * av * r2, 0.4f * s4.z * av * r2, s8b.y, s8b.z,
+                                                   s8b.w);
+                        c_pair++;
+                    }
+                } else if (mine && j != i) {
                     float rx = s4.x - o.x, ry = s4.y - o.y;
                     if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
                     pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x,
@@ -234,6 +249,10 @@ template <uint32_t F, bool KAHAN>
 __global__ __launch_bounds__(kWalkThreads) void k_walk_list(ForceArgs a) {
     walk_body<F, KAHAN, kFarList>(a, walk_group());
 }
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk_delay(ForceArgs a) {
+    walk_body<F, KAHAN, 0, kWalkStack, true>(a, walk_group());
+}
 
 // Persistent walk: every wave pulls 64-particle groups from a queue until none
 // are left, so uneven work (systems of a batch, dense regions) balances itself.
@@ -261,7 +280,8 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk_persistent96(ForceArgs a,
 
 // 1PN recompute for the velocity-Verlet correction: the same wavefront walk
 // with _accum1PN at leaves and accepted nodes (forces.js:511-571); leaf sources
-// at the current time (the reference always takes their signal-delayed state).
+// from their past light cone when DELAY (542-546), else at the current time.
+template <bool DELAY>
 __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
     __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
     __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
@@ -280,6 +300,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
     float f1x = 0.f, f1y = 0.f;
     const PhysFlags pf = a.pf;
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    const double now = DELAY ? a.ctrl->sim_time : 0.0;
     int sp = 0;
     if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
     __builtin_amdgcn_wave_barrier();
@@ -298,7 +319,13 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
                 const float4 s4 = a.src4[j];
                 const float4 s8a = a.src8a[j];
                 const float4 s8b = a.src8b[j];
-                if (mine && j != i) {
+                if (DELAY) {
+                    Retarded rt;
+                    if (mine && j != i && delayed_state(a.hist, pf, a.hist.slot[j], o.x, o.y, now, rt)) {
+                        const float rx = rt.dx, ry = rt.dy;
+                        accum_1pn(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, s8b.z, s8b.w);
+                    }
+                } else if (mine && j != i) {
                     float rx = s4.x - o.x, ry = s4.y - o.y;
                     if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
                     accum_1pn(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8b.z, s8b.w);
@@ -335,11 +362,14 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
     if (valid) { a.f1x[i] = f1x; a.f1y[i] = f1y; }
 }
 
-// O(N^2) 1PN recompute (forces.js:593-608), sources at the current time.
+// O(N^2) 1PN recompute (forces.js:593-608); sources from their past light
+// cone when DELAY (600-602), else at the current time.
+template <bool DELAY>
 __global__ __launch_bounds__(kWalkThreads) void k_direct1pn(ForceArgs a) {
     __shared__ float4 t4[kWalkThreads];
     __shared__ float4 t8a[kWalkThreads];
     __shared__ float4 t8b[kWalkThreads];
+    __shared__ int32_t tslot[kWalkThreads];
     if (!a.ctrl->active) return;
     const int n = a.ctrl->n_alive;
     const int i = blockIdx.x * kWalkThreads + threadIdx.x;
@@ -350,18 +380,29 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct1pn(ForceArgs a) {
     float f1x = 0.f, f1y = 0.f;
     const PhysFlags pf = a.pf;
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    const double now = DELAY ? a.ctrl->sim_time : 0.0;
     for (int base = 0; base < n; base += kWalkThreads) {
         const int j = base + threadIdx.x;
         __syncthreads();
-        if (j < n) { t4[threadIdx.x] = a.src4[j]; t8a[threadIdx.x] = a.src8a[j]; t8b[threadIdx.x] = a.src8b[j]; }
+        if (j < n) {
+            t4[threadIdx.x] = a.src4[j]; t8a[threadIdx.x] = a.src8a[j]; t8b[threadIdx.x] = a.src8b[j];
+            if (DELAY) tslot[threadIdx.x] = a.hist.slot[j];
+        }
         __syncthreads();
         const int cnt = min(kWalkThreads, n - base);
         if (valid) {
             for (int k = 0; k < cnt; k++) {
                 if (base + k == i) continue;
-                float rx = t4[k].x - o.x, ry = t4[k].y - o.y;
-                if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
-                accum_1pn(o, f1x, f1y, pf, rx, ry, t4[k].z, t4[k].w, t8a[k].x, t8a[k].y, t8b[k].z, t8b[k].w);
+                float rx, ry, svx = t8a[k].x, svy = t8a[k].y;
+                if (DELAY) {
+                    Retarded rt;
+                    if (!delayed_state(a.hist, pf, tslot[k], o.x, o.y, now, rt)) continue;
+                    rx = rt.dx; ry = rt.dy; svx = rt.vx; svy = rt.vy;
+                } else {
+                    rx = t4[k].x - o.x; ry = t4[k].y - o.y;
+                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                }
+                accum_1pn(o, f1x, f1y, pf, rx, ry, t4[k].z, t4[k].w, svx, svy, t8b[k].z, t8b[k].w);
             }
         }
     }
@@ -371,8 +412,13 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct1pn(ForceArgs a) {
 void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    if (tree) hipLaunchKernelGGL(k_walk1pn, dim3(grid), dim3(kWalkThreads), 0, s, a);
-    else hipLaunchKernelGGL(k_direct1pn, dim3(grid), dim3(kWalkThreads), 0, s, a);
+    if (a.hist.on) {
+        if (tree) hipLaunchKernelGGL(k_walk1pn<true>, dim3(grid), dim3(kWalkThreads), 0, s, a);
+        else hipLaunchKernelGGL(k_direct1pn<true>, dim3(grid), dim3(kWalkThreads), 0, s, a);
+    } else {
+        if (tree) hipLaunchKernelGGL(k_walk1pn<false>, dim3(grid), dim3(kWalkThreads), 0, s, a);
+        else hipLaunchKernelGGL(k_direct1pn<false>, dim3(grid), dim3(kWalkThreads), 0, s, a);
+    }
 }
 
 // Collision broadphase (collisions.js:35-60 with quadtree.js:372-404): for each
@@ -474,12 +520,16 @@ void launch_pairs(hipStream_t s, const PairArgs& a, bool write) {
     else hipLaunchKernelGGL(k_pairs<0>, dim3(grid), dim3(kWalkThreads), 0, s, a);
 }
 
-// Direct sum (forces.js:93-126): each block stages 256 sources at a time in LDS.
-template <uint32_t F, bool KAHAN>
-__global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) {
+// Direct sum (forces.js:93-126): each block stages 256 sources at a time in
+// LDS.  DELAY: every source from its past light cone, skipped without history
+// (forces.js:99-110).
+template <uint32_t F, bool KAHAN, bool DELAY>
+__device__ __forceinline__ void direct_body(const ForceArgs& a) {
     __shared__ float4 t4[kWalkThreads];
     __shared__ float4 t8a[kWalkThreads];
     __shared__ float4 t8b[kWalkThreads];
+    __shared__ int32_t tslot[kWalkThreads];
+    __shared__ float tr2[kWalkThreads];
     const int n = a.ctrl->n_alive;
     const int i = blockIdx.x * kWalkThreads + threadIdx.x;
     const bool valid = i < n;
@@ -490,13 +540,20 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) {
     acc.init();
     const PhysFlags pf = a.pf;
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    const double now = DELAY ? a.ctrl->sim_time : 0.0;
+    int64_t c_pair = 0;
     for (int base = 0; base < n; base += kWalkThreads) {
         const int j = base + threadIdx.x;
         __syncthreads();
         if (j < n) {
             t4[threadIdx.x] = a.src4[j];
             t8a[threadIdx.x] = a.src8a[j];
-            if (F & F_EXTRA) t8b[threadIdx.x] = a.src8b[j];
+            if ((F & F_EXTRA) || DELAY) t8b[threadIdx.x] = a.src8b[j];
+            if (DELAY) {
+                tslot[threadIdx.x] = a.hist.slot[j];
+                const float rad = a.radius[j];
+                tr2[threadIdx.x] = rad * rad;
+            }
         }
         __syncthreads();
         const int cnt = min(kWalkThreads, n - base);
@@ -506,19 +563,76 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) {
                 const float4 s4 = t4[k];
                 const float4 s8a = t8a[k];
                 float4 s8b = make_float4(0.f, 1.f, 1.f, 1.f);
-                if (F & F_EXTRA) s8b = t8b[k];
-                float rx = s4.x - o.x, ry = s4.y - o.y;
-                if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
-                pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x, s8b.y,
-                                     s8b.z, s8b.w);
+                if ((F & F_EXTRA) || DELAY) s8b = t8b[k];
+                if (DELAY) {
+                    Retarded rt;
+                    if (!delayed_state(a.hist, pf, tslot[k], o.x, o.y, now, rt)) continue;
+                    const float rx = rt.dx, ry = rt.dy;
+                    const float r2 = tr2[k];
+                    const float av = retarded_ang_vel(rt.aw, r2);
+                    pair_force<F, KAHAN, true>(o, acc, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, av,
+                                               0.2f * s4.w * av * r2, 0.4f * s4.z * av * r2, s8b.y, s8b.z, s8b.w);
+                    c_pair++;
+                } else {
+                    float rx = s4.x - o.x, ry = s4.y - o.y;
+                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                    pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x, s8b.y,
+                                         s8b.z, s8b.w);
+                }
             }
         }
     }
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (DELAY) {
+        wave_add_counters(a.ctrl, c_pair, 0, 0);
+    } else if (threadIdx.x == 0 && blockIdx.x == 0) {
         const int64_t nn = n;
         atomicAdd((unsigned long long*)&a.ctrl->cnt_pair, (unsigned long long)(nn * (nn - 1)));
     }
     if (valid) store_acc<KAHAN>(a, i, o, acc);
+}
+
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) { direct_body<F, KAHAN, false>(a); }
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_direct_delay(ForceArgs a) { direct_body<F, KAHAN, true>(a); }
+
+// Retired sources (forces.js:128-153): every live particle against every dead
+// record, each from its past light cone with its death mass, no PE; added to
+// the force outputs the walk or direct sum wrote.
+template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_dead_forces(ForceArgs a) {
+    const int n = a.ctrl->n_alive;
+    const int i = blockIdx.x * kWalkThreads + threadIdx.x;
+    const bool valid = i < n;
+    if (!__ballot(valid)) return;
+    Obs o;
+    if (valid) load_obs(a, i, o);
+    else { o.x = o.y = o.vx = o.vy = o.m = o.q = o.ang_vel = o.mu = o.L = 0.f; o.ax = o.yuk = o.higgs = 1.f; o.tidal_ri5 = 0.f; o.inv_m = 0.f; }
+    Acc<false> acc;
+    acc.init();
+    const PhysFlags pf = a.pf;
+    const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
+    const double now = a.ctrl->sim_time;
+    int64_t c_pair = 0;
+    for (int d = 0; d < a.hist.n_dead; d++) {
+        const DeadRec r = a.hist.dead[d];
+        Retarded rt;
+        if (!valid || !delayed_state(a.hist, pf, r.slot, o.x, o.y, now, rt)) continue;
+        const float rx = rt.dx, ry = rt.dy;
+        const float av = retarded_ang_vel(rt.aw, r.r2);
+        pair_force<F, false, true, false>(o, acc, pf, rx, ry, r.mass, r.charge, rt.vx, rt.vy, av,
+                                          0.2f * r.charge * av * r.r2, 0.4f * r.mass * av * r.r2,
+                                          r.ax != 0.f ? r.ax : 1.f, r.yuk != 0.f ? r.yuk : 1.f,
+                                          r.higgs != 0.f ? r.higgs : 1.f);
+        c_pair++;
+    }
+    wave_add_counters(a.ctrl, c_pair, 0, 0);
+    if (!valid) return;
+    a.fx[i] += acc.fx.s; a.fy[i] += acc.fy.s;
+    a.bz[i] += acc.bz; a.dbzdx[i] += acc.dbzdx; a.dbzdy[i] += acc.dbzdy;
+    a.bgz[i] += acc.bgz; a.dbgzdx[i] += acc.dbgzdx; a.dbgzdy[i] += acc.dbgzdy;
+    a.frame_drag[i] += acc.frame_drag; a.tidal[i] += acc.tidal;
+    a.f1x[i] += acc.f1x; a.f1y[i] += acc.f1y;
 }
 
 #define GEON_FEAT_SWITCH(KERNEL, feat, kahan, grid, stream, args)                                     \
@@ -547,7 +661,8 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct(ForceArgs a) {
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    if (a.far_mode == kFarMultipole) GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
+    if (a.hist.on) GEON_FEAT_SWITCH(k_walk_delay, feat, kahan, grid, s, a);
+    else if (a.far_mode == kFarMultipole) GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
     else if (a.far_mode == kFarList) GEON_FEAT_SWITCH(k_walk_list, feat, kahan, grid, s, a);
     else GEON_FEAT_SWITCH(k_walk, feat, kahan, grid, s, a);
 }
@@ -577,7 +692,14 @@ void launch_walk_persistent(hipStream_t s, const ForceArgs& a, uint32_t feat, bo
 void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    GEON_FEAT_SWITCH(k_direct, feat, kahan, grid, s, a);
+    if (a.hist.on) GEON_FEAT_SWITCH(k_direct_delay, feat, kahan, grid, s, a);
+    else GEON_FEAT_SWITCH(k_direct, feat, kahan, grid, s, a);
+}
+
+void launch_dead_forces(hipStream_t s, const ForceArgs& a, uint32_t feat) {
+    if (a.n_upper <= 0 || a.hist.n_dead <= 0) return;
+    const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
+    GEON_FEAT_SWITCH(k_dead_forces, feat, false, grid, s, a);
 }
 
 }  // namespace geon

@@ -180,12 +180,22 @@ class HipPhysics:
             "aliveCount": st.alive_count, "pe": st.pe, "fieldEnergy": st.field_energy,
             "fieldPx": st.field_px, "fieldPy": st.field_py,
             "higgsFieldEnergy": st.higgs_field_energy, "axionFieldEnergy": st.axion_field_energy,
-            "pfiEnergy": st.pfi_energy, "simTime": st.sim_time,
+            "pfiEnergy": st.pfi_energy, "simTime": st.sim_time, "deadCount": st.dead_count,
         }
 
     def stats(self):
         self.requestStats()
         return self.readStats()
+
+    def history(self, slot):
+        """Signal-delay history of `slot`, oldest sample first: dict of x, y,
+        vx, vy, angw, t arrays (particle.js:77-85)."""
+        keys = ("x", "y", "vx", "vy", "angw", "t")
+        out = {k: np.zeros(A.HISTORY_SIZE) for k in keys}
+        v = A.GeonHistoryView(*[_dptr(out[k]) for k in keys])
+        n = C.c_int32()
+        self._check(self.lib.geon_get_history(self.h, int(slot), C.byref(v), C.byref(n)))
+        return {k: a[:n.value] for k, a in out.items()}
 
     # ---- scalar fields ---------------------------------------------------
     def set_field(self, which, field=None, field_dot=None):

@@ -86,6 +86,15 @@ export default class HipPhysics {
 
     reset() { addon.reset(this._h); }
 
+    /** Signal delay of leaf sources: 'follow' (on exactly when relativity is,
+     *  forces.js:85), 'off' or 'on'; a history sample every `stride` updates
+     *  (HISTORY_STRIDE, config.js:122). */
+    setSignalDelay(mode, stride = 64) {
+        this._params.signalDelay = { follow: 0, off: 1, on: 2 }[mode];
+        this._params.historyStride = stride;
+        addon.setParams(this._h, this._params);
+    }
+
     // ---- learned far-field correction (no GPUPhysics counterpart) ----
     /** Far-field mode: 'none' (the reference's monopole nodes), 'multipole' or 'learned'. */
     setFarField(mode, listCap = 0) {

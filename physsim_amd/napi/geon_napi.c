@@ -83,6 +83,8 @@ static void read_params(napi_env env, napi_value obj, GeonParams* p) {
     p->kahan = (int32_t)get_num(env, obj, "kahan", p->kahan);
     p->far_field = (int32_t)get_num(env, obj, "farField", p->far_field);
     p->far_list_cap = (int32_t)get_num(env, obj, "farListCap", p->far_list_cap);
+    p->signal_delay = (int32_t)get_num(env, obj, "signalDelay", p->signal_delay);
+    p->history_stride = (int32_t)get_num(env, obj, "historyStride", p->history_stride);
 }
 
 static napi_value args_n(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {
@@ -326,6 +328,7 @@ static napi_value js_get_stats(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, o, "fieldPy", num(env, s.field_py));
     napi_set_named_property(env, o, "simTime", num(env, s.sim_time));
     napi_set_named_property(env, o, "aliveCount", i32(env, s.alive_count));
+    napi_set_named_property(env, o, "deadCount", i32(env, s.dead_count));
     return o;
 }
 

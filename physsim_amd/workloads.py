@@ -61,6 +61,9 @@ def config2(n=65536, replicate=0, side=None, open_box=False):
     """Relativistic torus box (min-image), gravity + Coulomb + Yukawa (mu = 1) +
     magnetic dipole, spins uniform in [-1, 1]; softening 8, theta 0.5, dt 1/128.
     The box side scales with sqrt(n) to keep the density of 65536 in 256x256.
+    Leaf sources at the current time (signal delay off): the reference would
+    take them from the light cone, and from a history that needs 128 updates
+    before any leaf acts at all (forces.js:652-655, integrator.js:1549-1565).
     open_box: the same particles and forces in an open (despawn) square, where
     Barnes-Hut converges to the direct sum."""
     g = rng(2, replicate)
@@ -76,7 +79,8 @@ def config2(n=65536, replicate=0, side=None, open_box=False):
     st = dict(x=f32(x), y=f32(y), wx=f32(wx), wy=f32(wy), mass=f32(m), charge=f32(q), angw=f32(angw))
     prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.YUKAWA | A.MAGNETIC | A.RELATIVITY | A.BARNES_HUT,
                boundary_mode=A.BOUND_LOOP, topology=A.TORUS, periodic_mode=A.PERIODIC_MIN_IMAGE,
-               domain_w=float(L), domain_h=float(L), softening=8.0, theta=0.5, yukawa_mu=1.0)
+               domain_w=float(L), domain_h=float(L), softening=8.0, theta=0.5, yukawa_mu=1.0,
+               signal_delay=A.DELAY_OFF)
     if open_box:
         prm["boundary_mode"] = A.BOUND_DESPAWN
     return st, prm, 1.0 / 128

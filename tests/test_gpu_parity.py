@@ -241,6 +241,7 @@ def test_one_pn_velocity_verlet_matches_oracle(bh, oracle_lib):
     (integrator.js:1296-1318), relativistic; BH walk or O(N^2)."""
     st, prm, dt, _ = W.config3(2048, grid=64, side=256.0, total_mass=4.0)
     prm["toggles0"] = A.GRAVITY | A.GRAVITOMAG | A.ONE_PN | A.RELATIVITY | (A.BARNES_HUT if bh else 0)
+    prm["signal_delay"] = A.DELAY_OFF               # current-time leaves (light-cone leaves: test_gpu_delay.py)
     g, o = _pair(st, prm, oracle_lib)
     g.compute_forces()
     o.compute_forces()

@@ -14,6 +14,7 @@ from physsim_amd import workloads as W
 
 def two_body(oracle_lib, toggles, m=(2.0, 3.0), q=(0.3, -0.5), d=5.0, soft=1.0, w=((0, 0), (0, 0)), angw=(0, 0),
              rel=False, **kw):
+    kw.setdefault("signal_delay", A.DELAY_OFF)      # closed forms of current-time sources
     prm = oracle_lib.default_params(toggles0=toggles | (A.RELATIVITY if rel else 0), softening=soft,
                                     domain_w=64.0, domain_h=64.0, **kw)
     o = oracle_lib.Oracle(prm, 2)
