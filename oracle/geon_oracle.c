@@ -1698,17 +1698,23 @@ int orc_stats(OrcSys* s, GeonStats* out) {
             const P* pi = &s->p[i];
             for (int j = i + 1; j < n; j++) {
                 const P* pj = &s->p[j];
+                double sx = pj->x, sy = pj->y, svx = pj->vx, svy = pj->vy;
+                if (s->delay) {                                           /* energy.js:85-93 */
+                    Delayed d;
+                    if (!delayedState(s, pj, pi->x, pi->y, &d)) continue;
+                    sx = d.x; sy = d.y; svx = d.vx; svy = d.vy;
+                }
                 double dx, dy;
-                if (s->periodic) minImage(pi->x, pi->y, pj->x, pj->y, s->prm.topology, W, H, W * 0.5, H * 0.5, &dx, &dy);
-                else { dx = pj->x - pi->x; dy = pj->y - pi->y; }
+                if (s->periodic) minImage(pi->x, pi->y, sx, sy, s->prm.topology, W, H, W * 0.5, H * 0.5, &dx, &dy);
+                else { dx = sx - pi->x; dy = sy - pi->y; }
                 const double rSq = dx * dx + dy * dy + s->softeningSq;
                 const double invR = 1 / sqrt(rSq);
                 const double rx = dx * invR, ry = dy * invR;
-                const double viDotVj = pi->vx * pj->vx + pi->vy * pj->vy;
+                const double viDotVj = pi->vx * svx + pi->vy * svy;
                 const double viDotR = pi->vx * rx + pi->vy * ry;
-                const double vjDotR = pj->vx * rx + pj->vy * ry;
+                const double vjDotR = svx * rx + svy * ry;
                 const double velTerm = viDotVj + viDotR * vjDotR;
-                const double sumVx = pi->vx + pj->vx, sumVy = pi->vy + pj->vy;
+                const double sumVx = pi->vx + svx, sumVy = pi->vy + svy;
                 const double svDotR = sumVx * rx + sumVy * ry;
                 if (s->mag) {
                     const double qqInvR = pi->charge * pj->charge * invR * sqrt(pi->axMod * pj->axMod);

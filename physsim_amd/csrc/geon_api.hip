@@ -744,8 +744,8 @@ int allocate(GeonContext* h) {
     if ((rc = dmalloc(h, &tb.pre_mom, 9 * (cap + 1)))) return rc;
     if ((rc = dmalloc(h, &tb.scan_tmp, nblk + 1024))) return rc;
     if ((rc = dmalloc(h, &tb.scan_tmp64, 9 * (nblk + 1024)))) return rc;
-    if ((rc = dmalloc(h, &h->stat_partial, 16 * 1024))) return rc;
-    if ((rc = dmalloc(h, &h->stat_out, 16))) return rc;
+    if ((rc = dmalloc(h, &h->stat_partial, std::max<size_t>(16 * 1024, 3 * ((cap + 255) / 256) + 64)))) return rc;
+    if ((rc = dmalloc(h, &h->stat_out, 32))) return rc;
     h->field_g = h->prm.field_grid > 0 ? h->prm.field_grid : 128;
     const size_t g2 = (size_t)h->field_g * h->field_g;
     for (int w = 0; w < 2; w++) {
@@ -1246,6 +1246,30 @@ int geon_get_stats(GeonHandle h, GeonStats* out) {
         out->axion_field_energy = fe[1][0];
         out->field_px += fe[0][1] + fe[1][1];
         out->field_py += fe[0][2] + fe[1][2];
+    }
+    if ((t & (GEON_MAGNETIC | GEON_GRAVITOMAG)) && h->n_host > 0) {
+        // Darwin field energy and momentum, pass 3 of computeEnergies (energy.js:64-142)
+        DarwinArgs d;
+        d.x = F(h, C_X); d.y = F(h, C_Y); d.vx = F(h, C_VX); d.vy = F(h, C_VY);
+        d.mass = F(h, C_MASS); d.charge = F(h, C_CHARGE);
+        d.ax = (t & GEON_AXION) ? F(h, C_AX) : nullptr;
+        d.slot = I(h, C_SLOT);
+        d.hist = force_args(h).hist;
+        d.pf = phys_flags(h->prm);
+        const double soft = (t & GEON_BLACK_HOLE) ? h->prm.bh_softening : h->prm.softening;
+        d.soft2 = soft * soft;
+        const bool one_pn = !!(t & GEON_ONE_PN);
+        d.mag = !!(t & GEON_MAGNETIC); d.gm = !!(t & GEON_GRAVITOMAG);
+        d.em_energy = d.mag && !one_pn; d.gm_energy = d.gm && !one_pn;
+        d.bazanski = d.mag && d.gm && !one_pn;
+        d.ctrl = h->ctrl;
+        launch_darwin(h->stream, d, h->n_host, h->stat_partial, h->stat_out + 16);
+        double dw[3];
+        HIP_OK(hipMemcpyAsync(dw, h->stat_out + 16, sizeof(dw), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        out->field_energy = dw[0];
+        out->field_px += dw[1];
+        out->field_py += dw[2];
     }
     out->sim_time = h->sim_time;
     out->alive_count = h->n_host;

@@ -72,8 +72,15 @@ __device__ __forceinline__ void delay_disp64(const PhysFlags& pf, float ox, floa
     }
 }
 
-__device__ __forceinline__ void set_disp(const PhysFlags& pf, double dx, double dy, Retarded& out) {
-    if (pf.periodic) {                               // pairForce minimum-images the result (forces.js:165-167)
+// The retarded point (xr, yr), an absolute position as getDelayedState returns
+// it, seen from the observer through pairForce's minimum image: one
+// conditional shift of the absolute difference (forces.js:165-167,
+// topology.js:9-12), whose image can differ from the one the light-cone solve
+// used when the point was extrapolated far.
+__device__ __forceinline__ void set_disp(const PhysFlags& pf, double xr, double yr, float ox, float oy,
+                                         Retarded& out) {
+    double dx = xr - (double)ox, dy = yr - (double)oy;
+    if (pf.periodic) {
         dx = wrap64(dx, (double)pf.dom_w);
         dy = wrap64(dy, (double)pf.dom_h);
     }
@@ -166,7 +173,7 @@ __device__ inline bool delayed_state(const HistArgs& H, const PhysFlags& pf, int
                     continue;
                 u = fmin(fmax(u, 0.0), seg);
                 const double fd = u / seg;
-                set_disp(pf, dx + fd * sx, dy + fd * sy, out);
+                set_disp(pf, (double)a4.x + fd * sx, (double)a4.y + fd * sy, ox, oy, out);
                 const float f = (float)fd;
                 out.vx = a4.z + f * (b4.z - a4.z);
                 out.vy = a4.w + f * (b4.w - a4.w);
@@ -187,7 +194,7 @@ __device__ inline bool delayed_state(const HistArgs& H, const PhysFlags& pf, int
         return false;
     if (u > 0.0) u = 0.0;
     if (t_old + u < H.cd[s].x) return false;
-    set_disp(pf, dx + vx * u, dy + vy * u, out);
+    set_disp(pf, (double)o4.x + vx * u, (double)o4.y + vy * u, ox, oy, out);
     out.vx = o4.z;
     out.vy = o4.w;
     out.aw = haw[i0];

@@ -312,6 +312,18 @@ void launch_vv_kick(hipStream_t s, const StepArgs& a);
 void launch_boundary(hipStream_t s, const StepArgs& a, int mode, double w, double h);
 // FP64 two-level reductions for the conservation readout; out[0..15].
 void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel);
+// Darwin field energy and momentum (energy.js:64-142): out[0..2] = fieldEnergy,
+// fieldPx, fieldPy.  partial holds 3 * ceil(n / 256) doubles.
+struct DarwinArgs {
+    const float *x, *y, *vx, *vy, *mass, *charge, *ax;   // ax null: 1
+    const int32_t* slot;
+    HistArgs hist;              // on: sources at their retarded state
+    PhysFlags pf;               // periodic box
+    double soft2;
+    int mag, gm, em_energy, gm_energy, bazanski;
+    const Ctrl* ctrl;
+};
+void launch_darwin(hipStream_t s, const DarwinArgs& d, int n_upper, double* partial, double* out);
 void launch_sum2(hipStream_t s, const float* a, const float* b, int n_upper, const Ctrl* ctrl, double* partial,
                  double* out);
 

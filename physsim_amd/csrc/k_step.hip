@@ -377,6 +377,105 @@ void launch_sum2(hipStream_t s, const float* a, const float* b, int n_upper, con
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(2 * 64), 0, s, partial, nb, 2, out);
 }
 
+// Darwin field energy and momentum, O(v^2/c^2) (energy.js:64-142): every
+// unordered pair once, in FP64.  The pair is oriented as the reference's array
+// order orients it: the particle added first (lower slot) observes, and with
+// signal delay on sees the other at its retarded state (energy.js:85-93).
+// Pairs are visited as j > i in the sorted order, so each thread's row starts
+// at its own tile; per-thread sums, then fixed-order block and final
+// reductions (deterministic).
+constexpr int kDarwinTile = 256;
+
+template <bool DELAY>
+__global__ __launch_bounds__(kDarwinTile) void k_darwin(DarwinArgs d, double* __restrict__ partial, int nb) {
+    __shared__ float4 tpv[kDarwinTile];     // x, y, vx, vy
+    __shared__ float4 tmq[kDarwinTile];     // m, q, axMod
+    __shared__ int32_t tsl[kDarwinTile];    // slot
+    __shared__ double sh[3][kDarwinTile];
+    const int n = d.ctrl->n_alive;
+    const int i = blockIdx.x * kDarwinTile + threadIdx.x;
+    const bool valid = i < n;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), mi = make_float4(0.f, 0.f, 1.f, 0.f);
+    int si = 0;
+    if (valid) {
+        pi = make_float4(d.x[i], d.y[i], d.vx[i], d.vy[i]);
+        mi = make_float4(d.mass[i], d.charge[i], d.ax ? d.ax[i] : 1.f, 0.f);
+        si = d.slot[i];
+    }
+    const double now = DELAY ? d.ctrl->sim_time : 0.0;
+    const double soft2 = d.soft2;
+    double e = 0.0, fpx = 0.0, fpy = 0.0;
+    for (int base = blockIdx.x * kDarwinTile; base < n; base += kDarwinTile) {
+        __syncthreads();
+        const int jj = base + threadIdx.x;
+        if (jj < n) {
+            tpv[threadIdx.x] = make_float4(d.x[jj], d.y[jj], d.vx[jj], d.vy[jj]);
+            tmq[threadIdx.x] = make_float4(d.mass[jj], d.charge[jj], d.ax ? d.ax[jj] : 1.f, 0.f);
+            tsl[threadIdx.x] = d.slot[jj];
+        }
+        __syncthreads();
+        if (!valid) continue;
+        const int cnt = min(kDarwinTile, n - base);
+        for (int k = 0; k < cnt; k++) {
+            if (base + k <= i) continue;
+            const float4 pj = tpv[k], mj = tmq[k];
+            const int sj = tsl[k];
+            const bool i_obs = si < sj;
+            const float4 po = i_obs ? pi : pj, ps = i_obs ? pj : pi;
+            const float4 mo = i_obs ? mi : mj, ms = i_obs ? mj : mi;
+            double dx, dy, svx = ps.z, svy = ps.w;
+            if (DELAY) {
+                Retarded rt;
+                if (!delayed_state(d.hist, d.pf, i_obs ? sj : si, po.x, po.y, now, rt)) continue;
+                dx = rt.dx; dy = rt.dy; svx = rt.vx; svy = rt.vy;
+            } else {
+                delay_disp64(d.pf, po.x, po.y, ps.x, ps.y, dx, dy);
+            }
+            const double ovx = po.z, ovy = po.w;
+            const double inv_r = 1.0 / sqrt(dx * dx + dy * dy + soft2);
+            const double rx = dx * inv_r, ry = dy * inv_r;
+            const double vel_term = (ovx * svx + ovy * svy) + (ovx * rx + ovy * ry) * (svx * rx + svy * ry);
+            const double sx = ovx + svx, sy = ovy + svy;
+            const double sdr = sx * rx + sy * ry;
+            const double mo_ = mo.x, ms_ = ms.x, qo = mo.y, qs = ms.y;
+            if (d.mag) {
+                const double qq = qo * qs * inv_r * sqrt((double)mo.z * (double)ms.z);
+                if (d.em_energy) e -= 0.5 * qq * vel_term;
+                fpx += 0.5 * qq * (sx + rx * sdr);
+                fpy += 0.5 * qq * (sy + ry * sdr);
+            }
+            if (d.gm) {
+                const double mm = mo_ * ms_ * inv_r;
+                if (d.gm_energy) e += 0.5 * mm * vel_term;
+                fpx -= 0.5 * mm * (sx + rx * sdr);
+                fpy -= 0.5 * mm * (sy + ry * sdr);
+            }
+            if (d.bazanski) {
+                const double cc = qo * qs * (mo_ + ms_) - (qo * qo * ms_ + qs * qs * mo_);
+                e += 0.5 * cc * inv_r * inv_r;
+            }
+        }
+    }
+    sh[0][threadIdx.x] = e;
+    sh[1][threadIdx.x] = fpx;
+    sh[2][threadIdx.x] = fpy;
+    __syncthreads();
+    for (int off = kDarwinTile / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int c = 0; c < 3; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) partial[threadIdx.x * nb + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+void launch_darwin(hipStream_t s, const DarwinArgs& d, int n_upper, double* partial, double* out) {
+    if (n_upper <= 0) return;
+    const int nb = (n_upper + kDarwinTile - 1) / kDarwinTile;
+    if (d.hist.on) hipLaunchKernelGGL(k_darwin<true>, dim3(nb), dim3(kDarwinTile), 0, s, d, partial, nb);
+    else hipLaunchKernelGGL(k_darwin<false>, dim3(nb), dim3(kDarwinTile), 0, s, d, partial, nb);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(3 * 64), 0, s, partial, nb, 3, out);
+}
+
 void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel) {
     int nb = (a.n_upper + kStatThreads - 1) / kStatThreads;
     if (nb > 1024) nb = 1024;

@@ -98,15 +98,17 @@ def _load_gpu_state_into_oracle(g, prm, oracle_lib, n):
     return o
 
 
-@pytest.mark.parametrize("bh", [True, False])
-def test_delayed_forces_match_oracle_on_gpu_history(bh, oracle_lib):
-    # 24 updates of dt 0.25 with a sample every update: a 6-time-unit history,
-    # so near sources are found inside the buffer and far ones by extrapolation.
+@pytest.mark.parametrize("bh,seed,updates", [(True, 4, 24), (False, 3, 24), (False, 8, 10)])
+def test_delayed_forces_match_oracle_on_gpu_history(bh, seed, updates, oracle_lib):
+    # dt 0.25 with a sample every update: near sources are found inside the
+    # buffer, far ones by extrapolation.  The short history (seed 8) extrapolates
+    # far enough that the retarded point's minimum image differs from the one
+    # its light-cone solve used (pairForce re-images it, forces.js:165-167).
     n = 1024 if bh else 512
-    st = _box(n, 3 + bh, L=48.0)
+    st = _box(n, seed, L=48.0)
     prm = _prm(48.0, bh=bh)
     g = _gpu(st, prm)
-    for _ in range(24):
+    for _ in range(updates):
         g.update(0.25)
     o = _load_gpu_state_into_oracle(g, prm, oracle_lib, n)
     g.compute_forces()
@@ -217,3 +219,25 @@ def test_one_pn_delayed_leaves_match_oracle(bh, oracle_lib):
     print("1PN delayed bh=%d: force %.3e, 1PN force %.3e, Bgz %.3e, w %.3e" % (bh, ef, e1, eb, ew))
     assert ef < 2e-5 and e1 < 1e-4 and eb < 1e-4
     assert ew < 1e-5
+
+
+@pytest.mark.parametrize("delay", [A.DELAY_OFF, A.DELAY_FOLLOW])
+def test_darwin_field_energy_matches_oracle(delay, oracle_lib):
+    # Pass 3 of computeEnergies (energy.js:64-142): Darwin EM and GM field
+    # energy, the Bazanski cross term and the field momentum, every pair once
+    # with the lower slot observing; sources retarded when signal delay is on.
+    n = 2048
+    st = _box(n, 8, L=64.0, wsig=0.3)
+    prm = _prm(64.0, toggles=A.GRAVITY | A.COULOMB | A.MAGNETIC | A.GRAVITOMAG | A.RELATIVITY, softening=4.0,
+               signal_delay=delay)
+    g = _gpu(st, prm)
+    for _ in range(10):
+        g.update(0.25)
+    o = _load_gpu_state_into_oracle(g, prm, oracle_lib, n)
+    sg, so = g.stats(), o.stats()
+    print("delay %d: Darwin E %.9e / %.9e, P (%.6e, %.6e) / (%.6e, %.6e)" % (
+        delay, sg["fieldEnergy"], so["field_energy"], sg["fieldPx"], sg["fieldPy"], so["field_px"], so["field_py"]))
+    assert so["field_energy"] != 0.0
+    assert sg["fieldEnergy"] == pytest.approx(so["field_energy"], rel=1e-6)
+    scale = np.hypot(so["field_px"], so["field_py"])
+    assert abs(sg["fieldPx"] - so["field_px"]) < 1e-6 * scale and abs(sg["fieldPy"] - so["field_py"]) < 1e-6 * scale
