@@ -85,7 +85,7 @@ typedef struct GeonContext* GeonHandle;
 typedef struct GeonParams {
     uint32_t toggles0;          /* GEON_GRAVITY | ... */
     int32_t  boundary_mode;     /* GEON_BOUND_* */
-    int32_t  topology;          /* GEON_TORUS only on this backend */
+    int32_t  topology;          /* GEON_TORUS, GEON_KLEIN, GEON_RP2 (loop boundary, topology.js) */
     int32_t  collision_mode;    /* GEON_COL_PASS only on this backend */
     int32_t  periodic_mode;     /* GEON_PERIODIC_* */
     int32_t  max_substeps;      /* MAX_SUBSTEPS = 32 (config.js:36) */

@@ -60,6 +60,7 @@ def lib():
         L.orc_set_history.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [_DP] * 6
         L.orc_set_time.argtypes = [C.c_void_p, C.c_double]
         L.orc_get_f1pn.argtypes = [C.c_void_p, _DP, _DP]
+        L.orc_min_image.argtypes = [C.c_void_p] + [C.c_double] * 4 + [_DP]
         L.orc_dead_count.argtypes = [C.c_void_p]
         L.orc_dead_count.restype = C.c_int32
         L.orc_set_lifetime.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
@@ -227,6 +228,11 @@ class Oracle:
         n = len(h["t"])
         arrs = [np.ascontiguousarray(h[k], dtype=np.float64) for k in self.HIST_KEYS]
         self.L.orc_set_history(self.s, int(i), n, *[_d(a) for a in arrs])
+
+    def min_image(self, ox, oy, sx, sy):
+        out = np.zeros(2)
+        self.L.orc_min_image(self.s, float(ox), float(oy), float(sx), float(sy), _d(out))
+        return out
 
     def f1pn(self):
         n = self.count()

@@ -1638,6 +1638,14 @@ int orc_get_f1pn(OrcSys* s, double* f1x, double* f1y) {
     return 0;
 }
 
+int orc_min_image(OrcSys* s, double ox, double oy, double sx, double sy, double* out2) {
+    syncToggles(s);
+    const GeonParams* g = &s->prm;
+    if (s->periodic) minImage(ox, oy, sx, sy, g->topology, g->domain_w, g->domain_h, g->domain_w * 0.5, g->domain_h * 0.5, &out2[0], &out2[1]);
+    else { out2[0] = sx - ox; out2[1] = sy - oy; }
+    return 0;
+}
+
 void orc_set_time(OrcSys* s, double t) { s->simTime = t; }
 int32_t orc_dead_count(OrcSys* s) { return s->ndead; }
 int orc_set_lifetime(OrcSys* s, int32_t i, double creation, double death) {

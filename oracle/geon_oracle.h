@@ -87,6 +87,8 @@ int32_t orc_get_history(OrcSys* s, int32_t i, int32_t dead, double* x, double* y
 int     orc_set_history(OrcSys* s, int32_t i, int32_t count, const double* x, const double* y,
                         const double* vx, const double* vy, const double* aw, const double* t);
 void    orc_set_time(OrcSys* s, double t);
+/* minImage (topology.js:18-76) under the system's boundary and topology. */
+int     orc_min_image(OrcSys* s, double ox, double oy, double sx, double sy, double* out2);
 /* force1PN of the last force pass, array order. */
 int     orc_get_f1pn(OrcSys* s, double* f1x, double* f1y);
 int32_t orc_dead_count(OrcSys* s);

@@ -39,6 +39,7 @@ struct FieldState {
     float2* green = nullptr;
     int* has_energy = nullptr;
     double green_w = -1, green_h = -1, green_soft2 = -1;
+    int green_topo = -1;        // 0 torus / open, else the glide topology the grid was built for
 };
 
 constexpr int kTimers = 5;
@@ -170,8 +171,12 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     }
     if (p->collision_mode != GEON_COL_PASS) { *why = "only collision mode 'pass' is run"; return 1; }
     if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
-    if (p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) { *why = "only the torus topology is run"; return 1; }
+    if (p->topology < GEON_TORUS || p->topology > GEON_RP2) { *why = "unknown topology"; return 1; }
     if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode == GEON_PERIODIC_GHOSTS) { *why = "ghost-image periodic mode is not run (min-image and windowed are)"; return 1; }
+    if (p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) {
+        if (p->periodic_mode != GEON_PERIODIC_MIN_IMAGE) { *why = "Klein and RP2 run min-image periodic mode"; return 1; }
+        if (p->far_field != GEON_FAR_NONE) { *why = "far-field corrections run on open or torus domains"; return 1; }
+    }
     if (!(p->domain_w > 0) || !(p->domain_h > 0)) { *why = "domain must be positive"; return 1; }
     if (p->max_substeps < 1) { *why = "max_substeps must be >= 1"; return 1; }
     if (p->far_field < GEON_FAR_NONE || p->far_field > GEON_FAR_LEARNED) { *why = "unknown far_field mode"; return 1; }
@@ -207,6 +212,7 @@ PhysFlags phys_flags(const GeonParams& p) {
     f.dom_h = (float)p.domain_h;
     f.periodic = p.boundary_mode == GEON_BOUND_LOOP;
     f.window = f.periodic && p.periodic_mode == GEON_PERIODIC_WINDOWED;
+    f.topology = f.periodic ? (uint32_t)p.topology : 0u;
     return f;
 }
 
@@ -408,6 +414,7 @@ FieldArgs field_args(GeonContext* h, int which) {
     FieldArgs f;
     f.G = h->field_g;
     f.bc = p.boundary_mode;
+    f.topo = p.boundary_mode == GEON_BOUND_LOOP ? p.topology : 0;
     f.higgs = which == 0;
     f.cw = (float)(p.domain_w / f.G);
     f.ch = (float)(p.domain_h / f.G);
@@ -442,9 +449,10 @@ void ensure_green(GeonContext* h, int which) {
     FieldState& s = h->fld[which];
     const GeonParams& p = h->prm;
     const double soft = (p.toggles0 & GEON_BLACK_HOLE) ? p.bh_softening : p.softening;
-    if (s.green_w == p.domain_w && s.green_h == p.domain_h && s.green_soft2 == soft * soft) return;
+    const int topo = p.boundary_mode == GEON_BOUND_LOOP ? p.topology : 0;
+    if (s.green_w == p.domain_w && s.green_h == p.domain_h && s.green_soft2 == soft * soft && s.green_topo == topo) return;
     field_build_green(h->stream, field_args(h, which), (float)(soft * soft));
-    s.green_w = p.domain_w; s.green_h = p.domain_h; s.green_soft2 = soft * soft;
+    s.green_w = p.domain_w; s.green_h = p.domain_h; s.green_soft2 = soft * soft; s.green_topo = topo;
 }
 
 // HiggsField.update + modulateMasses, AxionField.update (integrator.js:1320-1334),
@@ -1171,7 +1179,7 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
     }
     // boundary once per update (integrator.js:1798-1824)
     sa = step_args(h);
-    launch_boundary(h->stream, sa, h->prm.boundary_mode, h->prm.domain_w, h->prm.domain_h);
+    launch_boundary(h->stream, sa, h->prm.boundary_mode, h->prm.topology, h->prm.domain_w, h->prm.domain_h);
     if ((rc = read_ctrl(h))) return rc;
     h->sim_time = h->ctrl_host->sim_time;
     h->substeps_total += steps;
@@ -1422,6 +1430,8 @@ int geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse)
 
 int geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs) {
     if (!h || (cap > 0 && (!a || !b))) return GEON_EINVAL;
+    if (h->prm.boundary_mode == GEON_BOUND_LOOP && h->prm.topology != GEON_TORUS)
+        return set_err(h, GEON_EUNSUPPORTED, "the pair list runs on open and torus domains");
     const int n = h->n_host;
     if (n_pairs) *n_pairs = 0;
     if (n == 0) return GEON_OK;

@@ -132,6 +132,7 @@ PhysFlags flags(const GeonParams& p) {
     f.dom_w = (float)p.domain_w;
     f.dom_h = (float)p.domain_h;
     f.periodic = p.boundary_mode == GEON_BOUND_LOOP;
+    f.topology = 0;
     f.window = 0;
     return f;
 }

@@ -48,28 +48,14 @@ struct Retarded {
 
 __device__ __forceinline__ void delay_disp(const PhysFlags& pf, float ox, float oy, float px, float py, float& dx,
                                            float& dy) {
-    dx = px - ox;
-    dy = py - oy;
-    if (pf.periodic) {
-        dx = torus_wrap(dx, pf.dom_w, 0.5f * pf.dom_w);
-        dy = torus_wrap(dy, pf.dom_h, 0.5f * pf.dom_h);
-    }
-}
-
-__device__ __forceinline__ double wrap64(double d, double full) {
-    return d > 0.5 * full ? d - full : (d < -0.5 * full ? d + full : d);
+    min_image(pf, ox, oy, px, py, dx, dy);
 }
 
 // The same displacement in FP64: exact for FP32 inputs, so the ill-conditioned
 // light-cone geometry sees no rounding of the recorded positions.
-__device__ __forceinline__ void delay_disp64(const PhysFlags& pf, float ox, float oy, float px, float py, double& dx,
-                                             double& dy) {
-    dx = (double)px - (double)ox;
-    dy = (double)py - (double)oy;
-    if (pf.periodic) {
-        dx = wrap64(dx, (double)pf.dom_w);
-        dy = wrap64(dy, (double)pf.dom_h);
-    }
+__device__ __forceinline__ void delay_disp64(const PhysFlags& pf, double ox, double oy, double px, double py,
+                                             double& dx, double& dy) {
+    min_image_t<double>(pf.periodic, pf.topology, (double)pf.dom_w, (double)pf.dom_h, ox, oy, px, py, dx, dy);
 }
 
 // The retarded point (xr, yr), an absolute position as getDelayedState returns
@@ -79,11 +65,8 @@ __device__ __forceinline__ void delay_disp64(const PhysFlags& pf, float ox, floa
 // used when the point was extrapolated far.
 __device__ __forceinline__ void set_disp(const PhysFlags& pf, double xr, double yr, float ox, float oy,
                                          Retarded& out) {
-    double dx = xr - (double)ox, dy = yr - (double)oy;
-    if (pf.periodic) {
-        dx = wrap64(dx, (double)pf.dom_w);
-        dy = wrap64(dy, (double)pf.dom_h);
-    }
+    double dx, dy;
+    delay_disp64(pf, ox, oy, xr, yr, dx, dy);
     out.dx = (float)dx;
     out.dy = (float)dy;
 }

@@ -37,6 +37,7 @@ struct PhysFlags {
     float dom_w, dom_h;     // periodic box (min-image when periodic != 0)
     uint32_t periodic;
     uint32_t window;        // periodic: approximate only nodes inside the min-image window
+    uint32_t topology;      // periodic: 0 torus, 1 Klein bottle, 2 RP^2 (config.js:22-26)
 };
 
 // Tree node record: 16 dwords.
@@ -102,6 +103,55 @@ struct Acc {
 __device__ __forceinline__ float torus_wrap(float d, float full, float half) {
     // topology.js:9-12
     return d > half ? d - full : (d < -half ? d + full : d);
+}
+
+// minImage (topology.js:18-76): displacement observer (ox, oy) -> source
+// (sx, sy) on the periodic domain.  Torus: each axis wrapped once.  Klein
+// bottle (x periodic, y glide (x, y) ~ (W - x, y + H)) and RP^2 (glides on
+// both axes): the nearest of the reference's candidate images, strict `<` in
+// its order.  T is float or double.
+template <typename T>
+__device__ __forceinline__ T wrap_t(T d, T full, T half) { return d > half ? d - full : (d < -half ? d + full : d); }
+
+template <typename T>
+__device__ __forceinline__ void min_image_t(uint32_t periodic, uint32_t topo, T W, T H, T ox, T oy, T sx, T sy, T& dx,
+                                            T& dy) {
+    dx = sx - ox;
+    dy = sy - oy;
+    if (!periodic) return;
+    const T hw = T(0.5) * W, hh = T(0.5) * H;
+    if (topo == 0) {
+        dx = wrap_t(dx, W, hw);
+        dy = wrap_t(dy, H, hh);
+        return;
+    }
+    if (topo == 1) {
+        T bx = wrap_t(dx, W, hw), by = dy;
+        const T cx = wrap_t((W - sx) - ox, W, hw), cy = wrap_t((sy + H) - oy, T(2) * H, H);
+        if (cx * cx + cy * cy < bx * bx + by * by) { bx = cx; by = cy; }
+        dx = bx;
+        dy = by;
+        return;
+    }
+    T bx = dx, by = dy, best = dx * dx + dy * dy;
+    T cx = (W - sx) - ox, cy = wrap_t((sy + H) - oy, T(2) * H, H);
+    T q = cx * cx + cy * cy;
+    if (q < best) { bx = cx; by = cy; best = q; }
+    cx = wrap_t((sx + W) - ox, T(2) * W, W);
+    cy = (H - sy) - oy;
+    q = cx * cx + cy * cy;
+    if (q < best) { bx = cx; by = cy; best = q; }
+    cx = wrap_t((T(2) * W - sx) - ox, T(2) * W, W);
+    cy = wrap_t((T(2) * H - sy) - oy, T(2) * H, H);
+    q = cx * cx + cy * cy;
+    if (q < best) { bx = cx; by = cy; }
+    dx = bx;
+    dy = by;
+}
+
+__device__ __forceinline__ void min_image(const PhysFlags& pf, float ox, float oy, float sx, float sy, float& dx,
+                                          float& dy) {
+    min_image_t<float>(pf.periodic, pf.topology, pf.dom_w, pf.dom_h, ox, oy, sx, sy, dx, dy);
 }
 
 // Far-field terms of an accepted node beyond its monopole (not in the

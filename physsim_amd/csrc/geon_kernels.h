@@ -249,6 +249,7 @@ void launch_far_f2(hipStream_t s, const float* fx, const float* fy, const float*
 // ---- scalar fields (k_field.hip) ----
 struct FieldArgs {
     int G, bc, higgs;
+    int topo;                   // loop boundary: 0 torus, 1 Klein bottle, 2 RP^2
     float cw, ch, icw, ich, inv_area;
     float vac;                  // vacuum value: Higgs 1, axion 0
     float mu2, m2, damp;        // Higgs: mu^2 = m_H^2/2, damp 2 m_H; axion: m_a^2, damp g m_a
@@ -309,7 +310,7 @@ void launch_batch_stats(hipStream_t s, const StepArgs& a, const BatchStepArgs& b
 void launch_ctrl(hipStream_t s, Ctrl* ctrl, double softening, int kick_reuse);
 void launch_kick_drift(hipStream_t s, const StepArgs& a);
 void launch_vv_kick(hipStream_t s, const StepArgs& a);
-void launch_boundary(hipStream_t s, const StepArgs& a, int mode, double w, double h);
+void launch_boundary(hipStream_t s, const StepArgs& a, int mode, int topo, double w, double h);
 // FP64 two-level reductions for the conservation readout; out[0..15].
 void launch_stats(hipStream_t s, const StepArgs& a, const float* pe, double* partial, double* out, int rel);
 // Darwin field energy and momentum (energy.js:64-142): out[0..2] = fieldEnergy,

@@ -23,11 +23,20 @@ constexpr float kPortal = 0.01f;               // HIGGS_AXION_COUPLING (config.j
 constexpr float kHiggsMassFloor = 0.05f;       // HIGGS_MASS_FLOOR (config.js:90)
 constexpr float kHiggsMaxDelta = 8.f;          // HIGGS_MASS_MAX_DELTA (config.js:91)
 
-// _nb (scalar-field.js:78-108) for torus / bounce / despawn; -1 = Dirichlet.
-__device__ __forceinline__ int nb_index(int G, int nx, int ny, int bc) {
+// _nb (scalar-field.js:78-108): loop (torus; Klein bottle and RP^2 mirror the
+// other index across a glide edge), bounce (clamp), despawn (-1 = Dirichlet).
+__device__ __forceinline__ int nb_index(int G, int nx, int ny, int bc, int topo) {
     if (bc == 2) {
-        nx = nx < 0 ? nx + G : (nx >= G ? nx - G : nx);
-        ny = ny < 0 ? ny + G : (ny >= G ? ny - G : ny);
+        if (topo == 0) {
+            nx = nx < 0 ? nx + G : (nx >= G ? nx - G : nx);
+            ny = ny < 0 ? ny + G : (ny >= G ? ny - G : ny);
+        } else if (topo == 1) {
+            nx = nx < 0 ? nx + G : (nx >= G ? nx - G : nx);
+            if (ny < 0) { ny += G; nx = G - 1 - nx; } else if (ny >= G) { ny -= G; nx = G - 1 - nx; }
+        } else {
+            if (nx < 0) { nx += G; ny = G - 1 - ny; } else if (nx >= G) { nx -= G; ny = G - 1 - ny; }
+            if (ny < 0) { ny += G; nx = G - 1 - nx; } else if (ny >= G) { ny -= G; nx = G - 1 - nx; }
+        }
         return ny * G + nx;
     }
     if (bc == 1) {
@@ -60,13 +69,13 @@ __device__ __forceinline__ void pqs_weights(float x, float y, float icw, float i
 
 // Sum of arr over the 4x4 stencil; `outside` at Dirichlet nodes
 // (interpolate / gradient / gravity gather, scalar-field.js:284-415, 665-722).
-__device__ __forceinline__ float pqs_gather(const float* arr, int G, int bc, int ix, int iy, const float* wx,
+__device__ __forceinline__ float pqs_gather(const float* arr, int G, int bc, int topo, int ix, int iy, const float* wx,
                                             const float* wy, float outside) {
     float v = 0.f;
     for (int jy = 0; jy < 4; jy++) {
         float row = 0.f;
         for (int jx = 0; jx < 4; jx++) {
-            const int idx = nb_index(G, ix + jx - 1, iy + jy - 1, bc);
+            const int idx = nb_index(G, ix + jx - 1, iy + jy - 1, bc, topo);
             row += (idx >= 0 ? arr[idx] : outside) * wx[jx];
         }
         v += row * wy[jy];
@@ -74,11 +83,11 @@ __device__ __forceinline__ float pqs_gather(const float* arr, int G, int bc, int
     return v;
 }
 
-__device__ __forceinline__ void pqs_scatter(unsigned long long* out, int G, int bc, int ix, int iy, const float* wx,
+__device__ __forceinline__ void pqs_scatter(unsigned long long* out, int G, int bc, int topo, int ix, int iy, const float* wx,
                                             const float* wy, float value) {
     for (int jy = 0; jy < 4; jy++)
         for (int jx = 0; jx < 4; jx++) {
-            const int idx = nb_index(G, ix + jx - 1, iy + jy - 1, bc);
+            const int idx = nb_index(G, ix + jx - 1, iy + jy - 1, bc, topo);
             if (idx < 0) continue;
             const double v = (double)value * (double)wx[jx] * (double)wy[jy];
             atomicAdd(out + idx, (unsigned long long)(long long)llrint(v * kFix));
@@ -108,15 +117,15 @@ __global__ void k_field_deposit(FieldArgs f, const float* __restrict__ x, const 
             ke = 0.5f * m[i] * (vx[i] * vx[i] + vy[i] * vy[i]);
         }
         pqs_weights(x[i], y[i], f.icw, f.ich, ix, iy, pw, qw);
-        pqs_scatter(f.dep_src, f.G, f.bc, ix, iy, pw, qw, kHiggsCoupling * bm);
-        if (ke >= 1e-12f) pqs_scatter(f.dep_th, f.G, f.bc, ix, iy, pw, qw, ke);
+        pqs_scatter(f.dep_src, f.G, f.bc, f.topo, ix, iy, pw, qw, kHiggsCoupling * bm);
+        if (ke >= 1e-12f) pqs_scatter(f.dep_th, f.G, f.bc, f.topo, ix, iy, pw, qw, ke);
     } else {
         float s = 0.f;
         if (coul) { const float q2 = q[i] * q[i]; if (q2 > 1e-12f) s += kAxionCoupling * q2; }
         if (yuk && m[i] > 1e-12f) s += kAxionCoupling * m[i];
         if (s == 0.f) return;
         pqs_weights(x[i], y[i], f.icw, f.ich, ix, iy, pw, qw);
-        pqs_scatter(f.dep_src, f.G, f.bc, ix, iy, pw, qw, s);
+        pqs_scatter(f.dep_src, f.G, f.bc, f.topo, ix, iy, pw, qw, s);
     }
 }
 
@@ -215,11 +224,19 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
     if (t == 0) s.has_energy = 0;
     __syncthreads();
     if (mode == 1) {
+        const bool glide = f.bc == 2 && f.topo != 0;
+        const double W = (double)f.cw * N, H = (double)f.ch * N;
         for (int e = t; e < N * N; e += kFftThreads) {
             const int iy = e / N, ix = e % N;
-            const float dx = (float)(ix <= N / 2 ? ix : ix - N) * f.cw;
-            const float dy = (float)(iy <= N / 2 ? iy : iy - N) * f.ch;
-            s.re[iy * kPitch + ix] = (float)(-1.0 / sqrt((double)dx * dx + (double)dy * dy + (double)soft2));
+            double ddx, ddy;
+            if (glide) {
+                // Klein / RP^2: the reference takes minImage of the grid offset (scalar-field.js:538-548)
+                min_image_t<double>(1u, (uint32_t)f.topo, W, H, 0.0, 0.0, ix * (double)f.cw, iy * (double)f.ch, ddx, ddy);
+            } else {
+                ddx = (float)(ix <= N / 2 ? ix : ix - N) * f.cw;
+                ddy = (float)(iy <= N / 2 ? iy : iy - N) * f.ch;
+            }
+            s.re[iy * kPitch + ix] = (float)(-1.0 / sqrt(ddx * ddx + ddy * ddy + (double)soft2));
             s.im[iy * kPitch + ix] = 0.f;
         }
         __syncthreads();
@@ -280,8 +297,8 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
     // Phi = Re, central-difference gradient (outside = 0) (scalar-field.js:567-596)
     for (int e = t; e < N * N; e += kFftThreads) {
         const int iy = e / N, ix = e % N;
-        const int iL = nb_index(N, ix - 1, iy, f.bc), iR = nb_index(N, ix + 1, iy, f.bc);
-        const int iT = nb_index(N, ix, iy - 1, f.bc), iB = nb_index(N, ix, iy + 1, f.bc);
+        const int iL = nb_index(N, ix - 1, iy, f.bc, f.topo), iR = nb_index(N, ix + 1, iy, f.bc, f.topo);
+        const int iT = nb_index(N, ix, iy - 1, f.bc, f.topo), iB = nb_index(N, ix, iy + 1, f.bc, f.topo);
         auto P = [&](int idx) { return idx >= 0 ? s.re[(idx / N) * kPitch + (idx % N)] : 0.f; };
         f.sg_gx[e] = (P(iR) - P(iL)) * 0.5f;
         f.sg_gy[e] = (P(iB) - P(iT)) * 0.5f;
@@ -307,8 +324,8 @@ __global__ void k_field_kick(FieldArgs f, const float* __restrict__ fd_in, float
     const float fC = fld[i], fdC = fd_in[i];
     float lap, visc;
     {
-        const int iL = nb_index(G, ix - 1, iy, f.bc), iR = nb_index(G, ix + 1, iy, f.bc);
-        const int iT = nb_index(G, ix, iy - 1, f.bc), iB = nb_index(G, ix, iy + 1, f.bc);
+        const int iL = nb_index(G, ix - 1, iy, f.bc, f.topo), iR = nb_index(G, ix + 1, iy, f.bc, f.topo);
+        const int iT = nb_index(G, ix, iy - 1, f.bc, f.topo), iB = nb_index(G, ix, iy + 1, f.bc, f.topo);
         lap = ((iL >= 0 ? fld[iL] : f.vac) + (iR >= 0 ? fld[iR] : f.vac) - 2.f * fC) * icw2 +
               ((iT >= 0 ? fld[iT] : f.vac) + (iB >= 0 ? fld[iB] : f.vac) - 2.f * fC) * ich2;
         // viscosity: clamp-to-edge on the border whatever the boundary (scalar-field.js:254-276)
@@ -364,8 +381,8 @@ __global__ void k_field_grads(FieldArgs f) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G * G) return;
     const int iy = i / G, ix = i % G;
-    const int iL = nb_index(G, ix - 1, iy, f.bc), iR = nb_index(G, ix + 1, iy, f.bc);
-    const int iT = nb_index(G, ix, iy - 1, f.bc), iB = nb_index(G, ix, iy + 1, f.bc);
+    const int iL = nb_index(G, ix - 1, iy, f.bc, f.topo), iR = nb_index(G, ix + 1, iy, f.bc, f.topo);
+    const int iT = nb_index(G, ix, iy - 1, f.bc, f.topo), iB = nb_index(G, ix, iy + 1, f.bc, f.topo);
     const float* fl = f.field;
     f.grad_x[i] = ((iR >= 0 ? fl[iR] : f.vac) - (iL >= 0 ? fl[iL] : f.vac)) * 0.5f;
     f.grad_y[i] = ((iB >= 0 ? fl[iB] : f.vac) - (iT >= 0 ? fl[iT] : f.vac)) * 0.5f;
@@ -390,7 +407,7 @@ __global__ void k_field_modulate(FieldArgs f, const float* __restrict__ x,
     int ix, iy;
     float pw[4], qw[4];
     pqs_weights(x[i], y[i], f.icw, f.ich, ix, iy, pw, qw);
-    const float phi = pqs_gather(f.field, f.G, f.bc, ix, iy, pw, qw, f.vac);
+    const float phi = pqs_gather(f.field, f.G, f.bc, f.topo, ix, iy, pw, qw, f.vac);
     higgs_mod[i] = fmaxf(fabsf(phi), kHiggsMassFloor);
     const float target = fmaxf(bm * fabsf(phi), kHiggsMassFloor * bm);
     if (target != target) return;
@@ -418,7 +435,7 @@ __global__ void k_field_axmod(FieldArgs f, const float* __restrict__ x, const fl
     int ix, iy;
     float pw[4], qw[4];
     pqs_weights(x[i], y[i], f.icw, f.ich, ix, iy, pw, qw);
-    const float ga = kAxionCoupling * pqs_gather(f.field, f.G, f.bc, ix, iy, pw, qw, f.vac);
+    const float ga = kAxionCoupling * pqs_gather(f.field, f.G, f.bc, f.topo, ix, iy, pw, qw, f.vac);
     ax[i] = coul ? (ga > -1.f ? 1.f + ga : 0.f) : 1.f;
     yuk[i] = yukawa ? (ga > -1.f ? 1.f + ga : 0.f) : 1.f;
 }
@@ -441,10 +458,10 @@ __global__ void k_field_forces(FieldArgs h, FieldArgs a, int use_h, int use_a, i
         int ix, iy;
         float pw[4], qw[4];
         pqs_weights(x[i], y[i], h.icw, h.ich, ix, iy, pw, qw);
-        const float val = pqs_gather(h.field, h.G, h.bc, ix, iy, pw, qw, h.vac);
+        const float val = pqs_gather(h.field, h.G, h.bc, h.topo, ix, iy, pw, qw, h.vac);
         if (bm >= 1e-12f) {
-            const float gx = pqs_gather(h.grad_x, h.G, h.bc, ix, iy, pw, qw, 0.f) * h.icw;
-            const float gy = pqs_gather(h.grad_y, h.G, h.bc, ix, iy, pw, qw, 0.f) * h.ich;
+            const float gx = pqs_gather(h.grad_x, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.icw;
+            const float gy = pqs_gather(h.grad_y, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.ich;
             if (gx == gx && gy == gy) {
                 const float sgn = val >= 0.f ? 1.f : -1.f;
                 Fx += kHiggsCoupling * bm * sgn * gx;
@@ -453,9 +470,9 @@ __global__ void k_field_forces(FieldArgs h, FieldArgs a, int use_h, int use_a, i
             e_pfi -= bm * (fabsf(val) - 1.f);
         }
         if (grav && *h.has_energy && mi >= 1e-12f) {
-            Fx += -mi * pqs_gather(h.sg_gx, h.G, h.bc, ix, iy, pw, qw, 0.f) * h.icw;
-            Fy += -mi * pqs_gather(h.sg_gy, h.G, h.bc, ix, iy, pw, qw, 0.f) * h.ich;
-            peg += mi * pqs_gather(h.sg_phi, h.G, h.bc, ix, iy, pw, qw, 0.f);
+            Fx += -mi * pqs_gather(h.sg_gx, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.icw;
+            Fy += -mi * pqs_gather(h.sg_gy, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.ich;
+            peg += mi * pqs_gather(h.sg_phi, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f);
         }
     }
     if (use_a) {
@@ -466,20 +483,20 @@ __global__ void k_field_forces(FieldArgs h, FieldArgs a, int use_h, int use_a, i
         if (coul) { const float q2 = qi * qi; if (q2 > 1e-12f) coupling += q2; }
         if (yukawa && mi > 1e-12f) coupling += mi;
         if (coupling != 0.f) {
-            const float gx = pqs_gather(a.grad_x, a.G, a.bc, ix, iy, pw, qw, 0.f) * a.icw;
-            const float gy = pqs_gather(a.grad_y, a.G, a.bc, ix, iy, pw, qw, 0.f) * a.ich;
+            const float gx = pqs_gather(a.grad_x, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.icw;
+            const float gy = pqs_gather(a.grad_y, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.ich;
             if (gx == gx && gy == gy) {
                 Fx += kAxionCoupling * coupling * gx;
                 Fy += kAxionCoupling * coupling * gy;
             }
         }
-        const float av = pqs_gather(a.field, a.G, a.bc, ix, iy, pw, qw, a.vac);
+        const float av = pqs_gather(a.field, a.G, a.bc, a.topo, ix, iy, pw, qw, a.vac);
         if (coul) e_pfi -= kAxionCoupling * qi * qi * av;
         if (yukawa && mi > 1e-12f) e_pfi -= kAxionCoupling * mi * av;
         if (grav && *a.has_energy && mi >= 1e-12f) {
-            Fx += -mi * pqs_gather(a.sg_gx, a.G, a.bc, ix, iy, pw, qw, 0.f) * a.icw;
-            Fy += -mi * pqs_gather(a.sg_gy, a.G, a.bc, ix, iy, pw, qw, 0.f) * a.ich;
-            peg += mi * pqs_gather(a.sg_phi, a.G, a.bc, ix, iy, pw, qw, 0.f);
+            Fx += -mi * pqs_gather(a.sg_gx, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.icw;
+            Fy += -mi * pqs_gather(a.sg_gy, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.ich;
+            peg += mi * pqs_gather(a.sg_phi, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f);
         }
     }
     fx[i] += Fx;

@@ -175,8 +175,8 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
                         c_pair++;
                     }
                 } else if (mine && j != i) {
-                    float rx = s4.x - o.x, ry = s4.y - o.y;
-                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                    float rx, ry;
+                    min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
                     pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x,
                                          s8b.y, s8b.z, s8b.w);
                     c_pair++;
@@ -184,8 +184,8 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
             }
         } else if (nd.count < 0) {
             const float rawx = nd.comx - o.x, rawy = nd.comy - o.y;
-            float dx = rawx, dy = rawy;
-            if (pf.periodic) { dx = torus_wrap(dx, pf.dom_w, half_w); dy = torus_wrap(dy, pf.dom_h, half_h); }
+            float dx, dy;
+            min_image(pf, o.x, o.y, nd.comx, nd.comy, dx, dy);
             const float d2 = dx * dx + dy * dy;
             bool accept = mine && (nd.size2 < pf.theta2 * d2);   // forces.js:680
             if (pf.window) {
@@ -326,15 +326,15 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
                         accum_1pn(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, s8b.z, s8b.w);
                     }
                 } else if (mine && j != i) {
-                    float rx = s4.x - o.x, ry = s4.y - o.y;
-                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                    float rx, ry;
+                    min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
                     accum_1pn(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8b.z, s8b.w);
                 }
             }
         } else if (nd.count < 0) {
             const float rawx = nd.comx - o.x, rawy = nd.comy - o.y;
-            float dx = rawx, dy = rawy;
-            if (pf.periodic) { dx = torus_wrap(dx, pf.dom_w, half_w); dy = torus_wrap(dy, pf.dom_h, half_h); }
+            float dx, dy;
+            min_image(pf, o.x, o.y, nd.comx, nd.comy, dx, dy);
             bool accept = mine && (nd.size2 < pf.theta2 * (dx * dx + dy * dy));
             if (pf.window) {
                 const float cx = nd.bx - o.x + (dx - rawx), cy = nd.by - o.y + (dy - rawy);
@@ -399,8 +399,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct1pn(ForceArgs a) {
                     if (!delayed_state(a.hist, pf, tslot[k], o.x, o.y, now, rt)) continue;
                     rx = rt.dx; ry = rt.dy; svx = rt.vx; svy = rt.vy;
                 } else {
-                    rx = t4[k].x - o.x; ry = t4[k].y - o.y;
-                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                    min_image(pf, o.x, o.y, t4[k].x, t4[k].y, rx, ry);
                 }
                 accum_1pn(o, f1x, f1y, pf, rx, ry, t4[k].z, t4[k].w, svx, svy, t8b[k].z, t8b[k].w);
             }
@@ -574,8 +573,8 @@ __device__ __forceinline__ void direct_body(const ForceArgs& a) {
                                                0.2f * s4.w * av * r2, 0.4f * s4.z * av * r2, s8b.y, s8b.z, s8b.w);
                     c_pair++;
                 } else {
-                    float rx = s4.x - o.x, ry = s4.y - o.y;
-                    if (pf.periodic) { rx = torus_wrap(rx, pf.dom_w, half_w); ry = torus_wrap(ry, pf.dom_h, half_h); }
+                    float rx, ry;
+                    min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
                     pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x, s8b.y,
                                          s8b.z, s8b.w);
                 }

@@ -235,8 +235,45 @@ void launch_vv_kick(hipStream_t s, const StepArgs& a) {
     hipLaunchKernelGGL(k_vv_kick, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a);
 }
 
-// Boundary handling once per update (integrator.js:1798-1824, topology.js:81-90).
-__global__ void k_boundary(StepArgs a, int mode, float w, float h) {
+// wrapPosition (topology.js:81-135) for the Klein bottle and RP^2: a crossing
+// of a glide edge mirrors the other coordinate and negates that axis's w and
+// v, angw and the angular velocity.
+__device__ __forceinline__ void glide_wrap(const StepArgs& a, int i, int topo, float w, float h, float& x, float& y) {
+    float wx = a.wx[i], wy = a.wy[i], vx = a.vx[i], vy = a.vy[i], aw = a.angw[i], av = a.ang_vel[i];
+    bool flipped = false;
+    if (topo == 1) {
+        if (x < 0.f) x += w; else if (x > w) x -= w;
+        if (y < 0.f || y > h) {
+            y += y < 0.f ? h : -h;
+            x = w - x;
+            wx = -wx; vx = -vx; aw = -aw; av = -av;
+            flipped = true;
+        }
+    } else {
+        for (int it = 0; it < 2; it++) {
+            bool wrapped = false;
+            if (x < 0.f || x > w) {
+                x += x < 0.f ? w : -w;
+                y = h - y;
+                wy = -wy; vy = -vy; aw = -aw; av = -av;
+                wrapped = flipped = true;
+            }
+            if (y < 0.f || y > h) {
+                y += y < 0.f ? h : -h;
+                x = w - x;
+                wx = -wx; vx = -vx; aw = -aw; av = -av;
+                wrapped = flipped = true;
+            }
+            if (!wrapped) break;
+        }
+    }
+    if (flipped) {
+        a.wx[i] = wx; a.wy[i] = wy; a.vx[i] = vx; a.vy[i] = vy; a.angw[i] = aw; a.ang_vel[i] = av;
+    }
+}
+
+// Boundary handling once per update (integrator.js:1798-1824, topology.js:81-135).
+__global__ void k_boundary(StepArgs a, int mode, int topo, float w, float h) {
     const int n = a.ctrl->n_alive;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -244,9 +281,11 @@ __global__ void k_boundary(StepArgs a, int mode, float w, float h) {
     int alive = 1;
     if (mode == 0) {            // despawn (DESPAWN_MARGIN = 64, config.js:138)
         if (x < -64.f || x > w + 64.f || y < -64.f || y > h + 64.f) alive = 0;
-    } else if (mode == 2) {     // loop, torus wrap
+    } else if (mode == 2 && topo == 0) {     // loop, torus wrap
         if (x < 0.f) x += w; else if (x > w) x -= w;
         if (y < 0.f) y += h; else if (y > h) y -= h;
+    } else if (mode == 2) {
+        glide_wrap(a, i, topo, w, h, x, y);
     } else {                    // bounce safety clamp
         x = fminf(fmaxf(x, 0.f), w);
         y = fminf(fmaxf(y, 0.f), h);
@@ -256,9 +295,9 @@ __global__ void k_boundary(StepArgs a, int mode, float w, float h) {
     if (!alive) atomicAdd(&a.ctrl->deaths, 1);
 }
 
-void launch_boundary(hipStream_t s, const StepArgs& a, int mode, double w, double h) {
+void launch_boundary(hipStream_t s, const StepArgs& a, int mode, int topo, double w, double h) {
     if (a.n_upper <= 0) return;
-    hipLaunchKernelGGL(k_boundary, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a, mode, (float)w, (float)h);
+    hipLaunchKernelGGL(k_boundary, dim3((a.n_upper + 255) / 256), dim3(256), 0, s, a, mode, topo, (float)w, (float)h);
 }
 
 // ---------------------------------------------------------------------------
