@@ -12,7 +12,7 @@ substeps of live particles.  Inputs are resident in HBM before timing.
 Three force methods run the same updates from the same initial state:
   plain      monopole nodes (the reference's walk) at --theta-plain, the
              coarsest opening angle whose error is <= 1e-4;
-  learned    the same walk at --theta-learned plus the 2 x 64 GELU network's
+  learned    the same walk at --theta-learned plus the 2 x 64 hard-swish network's
              residual for every accepted node; the network is trained here,
              on the GPU, on exact residuals of 16 Plummer spheres of 65536
              (untimed setup; time and sample count reported);
@@ -284,7 +284,7 @@ def main():
                 "traffic": MLP_TRAFFIC_DEFAULT if default else None,
                 "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch (copy-calibrated), "
                                   "profiles/k_far_mlp_open1M_theta0.25.md" if default else None,
-                "note": "GELU on the vector ALU (2 transcendentals per hidden unit) bounds this kernel, not MFMA",
+                "note": "the vector ALU (features, hard-swish, output rotation) bounds this kernel, not MFMA",
                 "pairs_per_launch": L["node_interactions_per_particle"] * n, "flops_per_pair": MLP_FLOPS_PER_PAIR,
                 "ms_avg": L["phases_ms_avg"]["mlp"]}
     else:

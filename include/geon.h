@@ -128,7 +128,7 @@ typedef struct GeonParams {
  * whose nodes carry monopoles only, quadtree.js:272-309): NONE reproduces the
  * reference; MULTIPOLE adds the mass-quadrupole and charge dipole+quadrupole
  * terms of gravity and Coulomb; LEARNED adds the gravity and Coulomb residual
- * predicted for every (particle, accepted node) pair by the 2 x 64 GELU
+ * predicted for every (particle, accepted node) pair by the 2 x 64 hard-swish
  * network set with geon_far_set_model or trained with geon_far_train. */
 #define GEON_FAR_NONE      0
 #define GEON_FAR_MULTIPOLE 1
@@ -242,6 +242,10 @@ typedef struct GeonHistoryView {
     double *x, *y, *vx, *vy, *angw, *t;
 } GeonHistoryView;
 int  geon_get_history(GeonHandle h, int32_t slot, GeonHistoryView* out, int32_t* count);
+
+/* getParticleCount() (gpu-physics.js:4287): live particles, after the work
+ * queued on the handle's stream has finished. */
+int  geon_particle_count(GeonHandle h, int32_t* alive);
 
 /* requestStats + readStats (gpu-physics.js:4146-4281), in FP64. */
 int  geon_get_stats(GeonHandle h, GeonStats* out);

@@ -641,16 +641,13 @@ static void far_correction(const OrcSys* s, P* p, double dx, double dy, const do
 #define FAR_SAMPLE 20
 #define FAR_WSEL(j) ((j) < 4 ? (j) >> 1 : (j) - 2)
 
-static double gelu64(double z) {
-    const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
-    return z / (1.0 + exp(-2.0 * u));
+/* Hard-swish activation of the network: z clamp((z + 3)/6, 0, 1). */
+static double act64(double z) {
+    const double t = (z + 3.0) / 6.0;
+    return z * (t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t));
 }
 
-static double gelu64_grad(double z) {
-    const double u = 0.7978845608028654 * (z + 0.044715 * z * z * z);
-    const double sg = 1.0 / (1.0 + exp(-2.0 * u));
-    return sg + z * sg * (1.0 - sg) * 2.0 * 0.7978845608028654 * (1.0 + 3.0 * 0.044715 * z * z);
-}
+static double act64_grad(double z) { return z < -3.0 ? 0.0 : (z > 3.0 ? 1.0 : (2.0 * z + 3.0) / 6.0); }
 
 /* Features of an (observer, node) pair in the offset frame u = d/|d|,
  * v = (-u_y, u_x); c = the oracle's 9 node moments (Qm xx xy yy, Dq x y,
@@ -690,12 +687,12 @@ static void far_forward(const double* P, const double* feat, double* x, double* 
     for (int m = 0; m < FAR_HID; m++) {
         double z = 0;
         for (int k = 0; k < 16; k++) z += x[k] * P[FAR_W1 + k * FAR_HID + m];
-        z1[m] = z; a1[m] = gelu64(z);
+        z1[m] = z; a1[m] = act64(z);
     }
     for (int b = 0; b < FAR_HID; b++) {
         double z = P[FAR_B2 + b];
         for (int a = 0; a < FAR_HID; a++) z += a1[a] * P[FAR_W2 + a * FAR_HID + b];
-        z2[b] = z; a2[b] = gelu64(z);
+        z2[b] = z; a2[b] = act64(z);
     }
     for (int j = 0; j < FAR_OUT; j++) {
         double z = P[FAR_B3 + j];
@@ -754,14 +751,14 @@ double orc_far_loss_grad(const float* model, const float* samples, int64_t n, do
                 da += P[FAR_W3 + b * FAR_OUT + j] * dO[j];
                 grad[FAR_W3 + b * FAR_OUT + j] += a2[b] * dO[j];
             }
-            dz2[b] = da * gelu64_grad(z2[b]);
+            dz2[b] = da * act64_grad(z2[b]);
             grad[FAR_B2 + b] += dz2[b];
         }
         for (int j = 0; j < FAR_OUT; j++) grad[FAR_B3 + j] += dO[j];
         for (int a = 0; a < FAR_HID; a++) {
             double da = 0;
             for (int b = 0; b < FAR_HID; b++) { da += P[FAR_W2 + a * FAR_HID + b] * dz2[b]; grad[FAR_W2 + a * FAR_HID + b] += a1[a] * dz2[b]; }
-            dz1[a] = da * gelu64_grad(z1[a]);
+            dz1[a] = da * act64_grad(z1[a]);
         }
         for (int k = 0; k < 16; k++)
             for (int m = 0; m < FAR_HID; m++) grad[FAR_W1 + k * FAR_HID + m] += x[k] * dz1[m];

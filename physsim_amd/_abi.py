@@ -115,6 +115,7 @@ PROTOTYPES = {
     "geon_compute_forces": (C.c_int, [H]),
     "geon_get_forces": (C.c_int, [H, C.POINTER(GeonForceView), C.c_int32]),
     "geon_get_stats": (C.c_int, [H, C.POINTER(GeonStats)]),
+    "geon_particle_count": (C.c_int, [H, C.POINTER(C.c_int32)]),
     "geon_get_history": (C.c_int, [H, C.c_int32, C.POINTER(GeonHistoryView), C.POINTER(C.c_int32)]),
     "geon_build_tree": (C.c_int, [H]),
     "geon_get_tree": (C.c_int, [H, C.POINTER(GeonTreeNode), C.c_int64, C.POINTER(C.c_int64)]),

@@ -1285,6 +1285,13 @@ int geon_get_stats(GeonHandle h, GeonStats* out) {
     return GEON_OK;
 }
 
+int geon_particle_count(GeonHandle h, int32_t* alive) {
+    if (!h || !alive) return GEON_EINVAL;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    *alive = h->n_host;
+    return GEON_OK;
+}
+
 int geon_get_history(GeonHandle h, int32_t slot, GeonHistoryView* out, int32_t* count) {
     if (!h || !out || !count) return GEON_EINVAL;
     if (slot < 0 || slot >= h->next_slot) return set_err(h, GEON_EINVAL, "slot out of range");

@@ -141,9 +141,9 @@ class HipPhysics:
         return res
 
     def getParticleCount(self):                             # gpu-physics.js:4287
-        st = A.GeonStats()
-        self._check(self.lib.geon_get_stats(self.h, C.byref(st)))
-        return st.alive_count
+        n = C.c_int32()
+        self._check(self.lib.geon_particle_count(self.h, C.byref(n)))
+        return n.value
 
     # ---- stepping --------------------------------------------------------
     def update(self, dt):

@@ -117,7 +117,7 @@ export default class HipPhysics {
 
     farSetModel(model) { addon.farSetModel(this._h, model); }
 
-    getParticleCount() { return addon.getStats(this._h).aliveCount; }
+    getParticleCount() { return addon.particleCount(this._h); }
 
     requestStats(selIdx = -1) { this._stats = addon.getStats(this._h); }
 

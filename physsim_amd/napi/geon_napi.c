@@ -300,6 +300,17 @@ static napi_value js_get_forces(napi_env env, napi_callback_info info) {
     return out;
 }
 
+/* particleCount(h) -> live particles (gpu-physics.js:4287) */
+static napi_value js_particle_count(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    if (!args_n(env, info, 1, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    int32_t n = 0;
+    int rc = geon_particle_count(h, &n);
+    if (rc) return throw_geon(env, h, rc);
+    return i32(env, n);
+}
+
 /* getStats(h) -> readStats-shaped object (gpu-physics.js:4215-4281) */
 static napi_value js_get_stats(napi_env env, napi_callback_info info) {
     napi_value argv[1];
@@ -440,6 +451,7 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("computeForces", js_compute_forces);
     EXPORT("getForces", js_get_forces);
     EXPORT("getStats", js_get_stats);
+    EXPORT("particleCount", js_particle_count);
     EXPORT("lastError", js_last_error);
     EXPORT("farCollect", js_far_collect);
     EXPORT("farClear", js_far_clear);
