@@ -151,7 +151,8 @@ __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict_
 }
 
 // Standardised inputs of one row as the 16 BF16 values of the layer-1 operand.
-__device__ __forceinline__ void std_inputs(const FarFrags& fr, const float* f, bool valid, float* x) {
+template <typename FR>
+__device__ __forceinline__ void std_inputs(const FR& fr, const float* f, bool valid, float* x) {
 #pragma unroll
     for (int k = 0; k < kFarIn; k++) x[k] = valid ? (f[k] - fr.mean[k]) * fr.istd[k] : 0.f;
     x[kFarIn] = valid ? 1.f : 0.f;
@@ -233,17 +234,94 @@ struct FarRow {
     float ux, uy, sg, sc, sp, cp;
 };
 
-__global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 xs[kWave * 16];
-    __shared__ FarRow rows[kWave];
-    __shared__ int rlane[kWave];
-    __shared__ float sacc[kWave][3];                // fx, fy, PE of each observer lane
-    const int l = threadIdx.x;
+// The inference part of FarFrags, staged once per block in LDS so that no wave
+// holds the weights in registers across its tiles.
+struct FarInfer {
+    bf16x8 w1[2][kWave];
+    bf16x8 w2[2][4][kWave];
+    bf16x8 w3[4][kWave];
+    float b2[kFarHid];
+    float b3[8];
+    float mean[kFarIn], istd[kFarIn], oscale[8];
+};
+
+constexpr int kMlpWaves = 4;                   // waves per block, one 64-particle group each
+
+// Layers 1-3 for the 32 rows 32 nb .. 32 nb + 31 of the tile in xs: lane (h, c)
+// gets outputs 4h + r of row 32 nb + c in accumulator r (r < 4, 4h + r < kFarOut).
+__device__ __forceinline__ floatx16 mlp_half(const FarInfer& fr, const __bf16* xs, int l, int nb) {
     const int h = l >> 5, c = l & 31;
-    const int64_t g = blockIdx.x;
+    const bf16x8 bx = *reinterpret_cast<const bf16x8*>(xs + (32 * nb + c) * 16 + 8 * h);
+    const floatx16 zero = {};
+    bf16x8 a1[4], a2[4];
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++) {
+        const floatx16 z1 = mfma32(fr.w1[mb][l], bx, zero);
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = act(z1[8 * tt + j]);
+            a1[2 * mb + tt] = pack8(v);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++) {
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = fr.b2[c_row(mb, r, h)];
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc = mfma32(fr.w2[mb][t][l], a1[t], acc);
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = act(acc[8 * tt + j]);
+            a2[2 * mb + tt] = pack8(v);
+        }
+    }
+    floatx16 o = {};
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        if (out_of(h, r) < kFarOut) o[r] = fr.b3[out_of(h, r)];
+#pragma unroll
+    for (int t = 0; t < 4; t++) o = mfma32(fr.w3[t][l], a2[t], o);
+    return o;
+}
+
+__global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_far_mlp(FarArgs a) {
+    __shared__ FarInfer wfr;
+    __shared__ __attribute__((aligned(16))) __bf16 xs_all[kMlpWaves][kWave * 16];
+    __shared__ FarRow rows_all[kMlpWaves][kWave];
+    __shared__ int rlane_all[kMlpWaves][kWave];
+    __shared__ float sacc_all[kMlpWaves][kWave][3];     // fx, fy, PE of each observer lane
+    {
+        // stage the weights: every field is a whole number of dwords
+        const FarFrags& g = *a.frags;
+        auto copy = [&](uint32_t* dst, const uint32_t* src, int ndw) {
+            for (int k = threadIdx.x; k < ndw; k += kMlpWaves * kWave) dst[k] = src[k];
+        };
+        copy(reinterpret_cast<uint32_t*>(wfr.w1), reinterpret_cast<const uint32_t*>(g.w1), sizeof(g.w1) / 4);
+        copy(reinterpret_cast<uint32_t*>(wfr.w2), reinterpret_cast<const uint32_t*>(g.w2), sizeof(g.w2) / 4);
+        copy(reinterpret_cast<uint32_t*>(wfr.w3), reinterpret_cast<const uint32_t*>(g.w3), sizeof(g.w3) / 4);
+        copy(reinterpret_cast<uint32_t*>(wfr.b2), reinterpret_cast<const uint32_t*>(g.b2), kFarHid);
+        copy(reinterpret_cast<uint32_t*>(wfr.b3), reinterpret_cast<const uint32_t*>(g.b3), kFarOut);
+        copy(reinterpret_cast<uint32_t*>(wfr.mean), reinterpret_cast<const uint32_t*>(g.mean), kFarIn);
+        copy(reinterpret_cast<uint32_t*>(wfr.istd), reinterpret_cast<const uint32_t*>(g.istd), kFarIn);
+        copy(reinterpret_cast<uint32_t*>(wfr.oscale), reinterpret_cast<const uint32_t*>(g.oscale), kFarOut);
+    }
+    __syncthreads();
+    const int l = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int h = l >> 5, c = l & 31;
+    const int64_t g = (int64_t)blockIdx.x * kMlpWaves + wv;
     const int n = a.ctrl->n_alive;
-    if (g * kWave >= n) return;
-    const FarFrags& fr = *a.frags;
+    if (g * kWave >= n) return;                  // no barrier below: a whole wave leaves
+    __bf16* xs = xs_all[wv];
+    FarRow* rows = rows_all[wv];
+    int* rlane = rlane_all[wv];
+    float (*sacc)[3] = sacc_all[wv];
+    const FarInfer& fr = wfr;
     sacc[l][0] = 0.f; sacc[l][1] = 0.f; sacc[l][2] = 0.f;
     const int cnt = a.far_count[g];
     const uint32_t* list = a.far_list + g * a.far_cap;
@@ -280,25 +358,25 @@ __global__ __launch_bounds__(kWave) void k_far_mlp(FarArgs a) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // 2. the network
-        Fwd F;
-        mlp_forward(fr, xs, l, F);
-        // 3. rotate, scale and accumulate per observer lane: the force from
-        //    outputs 0..3 (lanes h = 0), the potential from outputs 4, 5 (h = 1)
-#pragma unroll
+        // 2. the network, 32 rows at a time, then 3. rotate, scale and
+        //    accumulate per observer lane: the force from outputs 0..3 (lanes
+        //    h = 0), the potential from outputs 4, 5 (h = 1)
+#pragma unroll 1
         for (int nb = 0; nb < 2; nb++) {
+            if (base + 32 * nb >= cnt) break;
+            const floatx16 o = mlp_half(fr, xs, l, nb);
             const int row = 32 * nb + c;
             if (base + row < cnt) {
                 const FarRow q = rows[row];
                 if (h == 0) {
-                    const float gu = F.o[nb][0] * fr.oscale[0], gv = F.o[nb][1] * fr.oscale[1];
-                    const float cu = F.o[nb][2] * fr.oscale[2], cv = F.o[nb][3] * fr.oscale[3];
+                    const float gu = o[0] * fr.oscale[0], gv = o[1] * fr.oscale[1];
+                    const float cu = o[2] * fr.oscale[2], cv = o[3] * fr.oscale[3];
                     const float fx = q.sg * (q.ux * gu - q.uy * gv) + q.sc * (q.ux * cu - q.uy * cv);
                     const float fy = q.sg * (q.uy * gu + q.ux * gv) + q.sc * (q.uy * cu + q.ux * cv);
                     atomicAdd(&sacc[rlane[row]][0], fx);
                     atomicAdd(&sacc[rlane[row]][1], fy);
                 } else {
-                    const float pe = q.sp * F.o[nb][0] * fr.oscale[4] + q.cp * F.o[nb][1] * fr.oscale[5];
+                    const float pe = q.sp * o[0] * fr.oscale[4] + q.cp * o[1] * fr.oscale[5];
                     atomicAdd(&sacc[rlane[row]][2], pe);
                 }
             }
@@ -690,7 +768,8 @@ void launch_far_pack(hipStream_t s, const float* params, const float* stats, voi
 
 void launch_far_mlp(hipStream_t s, const FarArgs& a, int n_upper) {
     const int groups = (n_upper + kWave - 1) / kWave;
-    if (groups > 0) hipLaunchKernelGGL(k_far_mlp, dim3(groups), dim3(kWave), 0, s, a);
+    const int blocks = (groups + kMlpWaves - 1) / kMlpWaves;
+    if (groups > 0) hipLaunchKernelGGL(k_far_mlp, dim3(blocks), dim3(kMlpWaves * kWave), 0, s, a);
 }
 
 void launch_far_samples(hipStream_t s, const FarSampleArgs& a) {
