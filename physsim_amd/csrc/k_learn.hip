@@ -240,7 +240,7 @@ struct FarInfer {
     bf16x8 w1[2][kWave];
     bf16x8 w2[2][4][kWave];
     bf16x8 w3[4][kWave];
-    float b2[kFarHid];
+    float4 b2[kFarHid / 4];             // 16-byte rows: accumulators 4q .. 4q + 3 of a block share one
     float b3[8];
     float mean[kFarIn], istd[kFarIn], oscale[8];
 };
@@ -269,7 +269,10 @@ __device__ __forceinline__ floatx16 mlp_half(const FarInfer& fr, const __bf16* x
     for (int mb = 0; mb < 2; mb++) {
         floatx16 acc;
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = fr.b2[c_row(mb, r, h)];
+        for (int q = 0; q < 4; q++) {
+            const float4 b = fr.b2[c_row(mb, 4 * q, h) / 4];      // hidden units c_row(mb, 4q .. 4q + 3, h)
+            acc[4 * q] = b.x; acc[4 * q + 1] = b.y; acc[4 * q + 2] = b.z; acc[4 * q + 3] = b.w;
+        }
 #pragma unroll
         for (int t = 0; t < 4; t++) acc = mfma32(fr.w2[mb][t][l], a1[t], acc);
 #pragma unroll
@@ -326,14 +329,16 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
     const int cnt = a.far_count[g];
     const uint32_t* list = a.far_list + g * a.far_cap;
     const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
+    uint32_t e_next = l < cnt ? list[l] : 0u;     // the next tile's list entries, one tile ahead
     for (int base = 0; base < cnt; base += kWave) {
         // 1. features of row l
         const bool valid = base + l < cnt;
+        const uint32_t e = e_next;
+        e_next = base + kWave + l < cnt ? list[base + kWave + l] : 0u;
         float f[kFarIn];
         FarRow rw = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         int lane_of = 0;
         if (valid) {
-            const uint32_t e = list[base + l];
             const int node = (int)(e & kFarNodeMask);
             lane_of = (int)(e >> kFarLaneShift);
             const int i = (int)(g * kWave) + lane_of;
