@@ -35,9 +35,14 @@ namespace geon {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ floatx16 mfma32(const bf16x8& a, const bf16x8& b, const floatx16& c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx16 mfma32(const halfx8& a, const halfx8& b, const floatx16& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
 __host__ __device__ __forceinline__ int kperm(int t, int h, int j) {
@@ -56,6 +61,30 @@ __device__ __forceinline__ bf16x8 pack8(const float* v) {
     bf16x8 r;
 #pragma unroll
     for (int j = 0; j < 8; j++) r[j] = (__bf16)v[j];
+    return r;
+}
+__device__ __forceinline__ halfx8 pack8h(const float* v) {
+    halfx8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = (_Float16)v[j];
+    return r;
+}
+
+// The activation of 8 accumulators in packed FP16 for an FP16 MFMA operand:
+// per two values one conversion, one fma whose clamp modifier does the
+// [0, 1] clamp, and one multiply (the FP32 form takes three per value).
+__device__ __forceinline__ halfx8 act8h(const float* z) {
+    const halfx2 k = {(_Float16)(1.f / 6.f), (_Float16)(1.f / 6.f)}, c = {(_Float16)0.5f, (_Float16)0.5f};
+    const halfx2 lo = {(_Float16)0.f, (_Float16)0.f}, hi = {(_Float16)1.f, (_Float16)1.f};
+    halfx8 r;
+#pragma unroll
+    for (int j = 0; j < 8; j += 2) {
+        const halfx2 zz = {(_Float16)z[j], (_Float16)z[j + 1]};
+        const halfx2 t = __builtin_elementwise_min(__builtin_elementwise_max(__builtin_elementwise_fma(zz, k, c), lo), hi);
+        const halfx2 a = zz * t;
+        r[j] = a.x;
+        r[j + 1] = a.y;
+    }
     return r;
 }
 
@@ -106,6 +135,8 @@ struct FarFrags {
     bf16x8 w1[2][kWave];            // layer 1, A = W1^T, M-block mb
     bf16x8 w2[2][4][kWave];         // layer 2, A = W2^T, K permuted
     bf16x8 w3[4][kWave];            // layer 3, A = W3^T padded to 32 rows, K permuted
+    halfx8 w2h[2][4][kWave];        // inference: layers 2 and 3 in FP16 (as w2, w3)
+    halfx8 w3h[4][kWave];
     bf16x8 w3t[2][kWave];           // backward: A = W3 (hidden x outputs; K = 8h + j <-> output 4h + j)
     bf16x8 w2t[2][4][kWave];        // backward: A = W2 (in x out), K (out) permuted
     float b2[kFarHid];
@@ -129,6 +160,7 @@ __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict_
         for (int t = 0; t < 4; t++) {
             for (int j = 0; j < 8; j++) v[j] = W2[kperm(t, h, j) * kFarHid + 32 * mb + c];
             fr->w2[mb][t][l] = pack8(v);
+            fr->w2h[mb][t][l] = pack8h(v);
             for (int j = 0; j < 8; j++) v[j] = W2[(32 * mb + c) * kFarHid + kperm(t, h, j)];
             fr->w2t[mb][t][l] = pack8(v);
         }
@@ -140,6 +172,7 @@ __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict_
         float v[8];
         for (int j = 0; j < 8; j++) v[j] = c < kFarOut ? W3[kperm(t, h, j) * kFarOut + c] : 0.f;
         fr->w3[t][l] = pack8(v);
+        fr->w3h[t][l] = pack8h(v);
     }
     fr->b2[l] = P[kFarOffB2 + l];
     if (l < kFarOut) fr->b3[l] = P[kFarOffB3 + l];
@@ -238,8 +271,8 @@ struct FarRow {
 // holds the weights in registers across its tiles.
 struct FarInfer {
     bf16x8 w1[2][kWave];
-    bf16x8 w2[2][4][kWave];
-    bf16x8 w3[4][kWave];
+    halfx8 w2[2][4][kWave];             // FP16: the packed-FP16 activations feed them
+    halfx8 w3[4][kWave];
     float4 b2[kFarHid / 4];             // 16-byte rows: accumulators 4q .. 4q + 3 of a block share one
     float b3[8];
     float mean[kFarIn], istd[kFarIn], oscale[8];
@@ -247,23 +280,39 @@ struct FarInfer {
 
 constexpr int kMlpWaves = 4;                   // waves per block, one 64-particle group each
 
+// The gathered inputs of one listed pair: the observer's {x, y, m, q}, the
+// node's {COM x, y, mass, charge} and half width, its moments.
+struct FarIn {
+    float4 s4, c4;
+    float half;
+    NodeMom mo;
+    int lane;
+};
+
+__device__ __forceinline__ void far_gather(const FarArgs& a, int64_t g, uint32_t e, FarIn& in) {
+    const int node = (int)(e & kFarNodeMask);
+    in.lane = (int)(e >> kFarLaneShift);
+    in.s4 = a.src4[(int)(g * kWave) + in.lane];
+    in.c4 = *reinterpret_cast<const float4*>(&a.nodes[node].comx);
+    in.half = a.nodes[node].half;
+    in.mo = a.node_mom[node];
+}
+
 // Layers 1-3 for the 32 rows 32 nb .. 32 nb + 31 of the tile in xs: lane (h, c)
 // gets outputs 4h + r of row 32 nb + c in accumulator r (r < 4, 4h + r < kFarOut).
 __device__ __forceinline__ floatx16 mlp_half(const FarInfer& fr, const __bf16* xs, int l, int nb) {
     const int h = l >> 5, c = l & 31;
     const bf16x8 bx = *reinterpret_cast<const bf16x8*>(xs + (32 * nb + c) * 16 + 8 * h);
     const floatx16 zero = {};
-    bf16x8 a1[4], a2[4];
+    halfx8 a1[4], a2[4];
 #pragma unroll
     for (int mb = 0; mb < 2; mb++) {
         const floatx16 z1 = mfma32(fr.w1[mb][l], bx, zero);
+        float v[16];
 #pragma unroll
-        for (int tt = 0; tt < 2; tt++) {
-            float v[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = act(z1[8 * tt + j]);
-            a1[2 * mb + tt] = pack8(v);
-        }
+        for (int r = 0; r < 16; r++) v[r] = z1[r];
+        a1[2 * mb] = act8h(v);
+        a1[2 * mb + 1] = act8h(v + 8);
     }
 #pragma unroll
     for (int mb = 0; mb < 2; mb++) {
@@ -275,13 +324,11 @@ __device__ __forceinline__ floatx16 mlp_half(const FarInfer& fr, const __bf16* x
         }
 #pragma unroll
         for (int t = 0; t < 4; t++) acc = mfma32(fr.w2[mb][t][l], a1[t], acc);
+        float v[16];
 #pragma unroll
-        for (int tt = 0; tt < 2; tt++) {
-            float v[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = act(acc[8 * tt + j]);
-            a2[2 * mb + tt] = pack8(v);
-        }
+        for (int r = 0; r < 16; r++) v[r] = acc[r];
+        a2[2 * mb] = act8h(v);
+        a2[2 * mb + 1] = act8h(v + 8);
     }
     floatx16 o = {};
 #pragma unroll
@@ -305,8 +352,8 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
             for (int k = threadIdx.x; k < ndw; k += kMlpWaves * kWave) dst[k] = src[k];
         };
         copy(reinterpret_cast<uint32_t*>(wfr.w1), reinterpret_cast<const uint32_t*>(g.w1), sizeof(g.w1) / 4);
-        copy(reinterpret_cast<uint32_t*>(wfr.w2), reinterpret_cast<const uint32_t*>(g.w2), sizeof(g.w2) / 4);
-        copy(reinterpret_cast<uint32_t*>(wfr.w3), reinterpret_cast<const uint32_t*>(g.w3), sizeof(g.w3) / 4);
+        copy(reinterpret_cast<uint32_t*>(wfr.w2), reinterpret_cast<const uint32_t*>(g.w2h), sizeof(g.w2h) / 4);
+        copy(reinterpret_cast<uint32_t*>(wfr.w3), reinterpret_cast<const uint32_t*>(g.w3h), sizeof(g.w3h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.b2), reinterpret_cast<const uint32_t*>(g.b2), kFarHid);
         copy(reinterpret_cast<uint32_t*>(wfr.b3), reinterpret_cast<const uint32_t*>(g.b3), kFarOut);
         copy(reinterpret_cast<uint32_t*>(wfr.mean), reinterpret_cast<const uint32_t*>(g.mean), kFarIn);
@@ -329,26 +376,27 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
     const int cnt = a.far_count[g];
     const uint32_t* list = a.far_list + g * a.far_cap;
     const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
-    uint32_t e_next = l < cnt ? list[l] : 0u;     // the next tile's list entries, one tile ahead
+    // software pipeline: the gathers of tile k + 1 are in flight while tile k
+    // computes (their latency, not the vector ALU, bounds this kernel)
+    FarIn cur = {}, nxt = {};
+    if (l < cnt) far_gather(a, g, list[l], cur);
+    uint32_t e_next = kWave + l < cnt ? list[kWave + l] : 0u;     // tile 1's entries
     for (int base = 0; base < cnt; base += kWave) {
+        if (base + kWave + l < cnt) far_gather(a, g, e_next, nxt);
+        e_next = base + 2 * kWave + l < cnt ? list[base + 2 * kWave + l] : 0u;
         // 1. features of row l
         const bool valid = base + l < cnt;
-        const uint32_t e = e_next;
-        e_next = base + kWave + l < cnt ? list[base + kWave + l] : 0u;
         float f[kFarIn];
         FarRow rw = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         int lane_of = 0;
         if (valid) {
-            const int node = (int)(e & kFarNodeMask);
-            lane_of = (int)(e >> kFarLaneShift);
-            const int i = (int)(g * kWave) + lane_of;
-            const float4 s4 = a.src4[i];
-            const Node nd = a.nodes[node];
-            const NodeMom mo = a.node_mom[node];
-            float dx = nd.comx - s4.x, dy = nd.comy - s4.y;
+            lane_of = cur.lane;
+            const float4 s4 = cur.s4;
+            float dx = cur.c4.x - s4.x, dy = cur.c4.y - s4.y;
             if (a.periodic) { dx = torus_wrap(dx, a.dom_w, half_w); dy = torus_wrap(dy, a.dom_h, half_h); }
             float sg0, sc0, sp0, cp0;
-            pair_features<float>(dx, dy, a.soft2, 2.f * nd.half, nd.mass, mo, f, rw.ux, rw.uy, sg0, sc0, sp0, cp0);
+            pair_features<float>(dx, dy, a.soft2, 2.f * cur.half, cur.c4.z, cur.mo, f, rw.ux, rw.uy, sg0, sc0, sp0,
+                                 cp0);
             rw.sg = a.grav ? s4.z * sg0 : 0.f;
             rw.sc = a.coul ? s4.w * sc0 : 0.f;
             rw.sp = a.grav ? s4.z * sp0 : 0.f;
@@ -389,6 +437,7 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        cur = nxt;
     }
     const int i = (int)(g * kWave) + l;
     if (i < n) {
