@@ -298,48 +298,71 @@ __device__ __forceinline__ void far_gather(const FarArgs& a, int64_t g, uint32_t
     in.mo = a.node_mom[node];
 }
 
-// Layers 1-3 for the 32 rows 32 nb .. 32 nb + 31 of the tile in xs: lane (h, c)
-// gets outputs 4h + r of row 32 nb + c in accumulator r (r < 4, 4h + r < kFarOut).
-__device__ __forceinline__ floatx16 mlp_half(const FarInfer& fr, const __bf16* xs, int l, int nb) {
+// Layers 1-3 for the 64 rows of the tile in xs, both 32-row halves per weight
+// fragment read from LDS (LDS bandwidth, not the ALUs, bounds this kernel):
+// lane (h, c) gets outputs 4h + r of row 32 nb + c in accumulator r of o[nb]
+// (r < 4, 4h + r < kFarOut).
+__device__ __forceinline__ void mlp_tile(const FarInfer& fr, const __bf16* xs, int l, floatx16* o) {
     const int h = l >> 5, c = l & 31;
-    const bf16x8 bx = *reinterpret_cast<const bf16x8*>(xs + (32 * nb + c) * 16 + 8 * h);
+    bf16x8 bx[2];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) bx[nb] = *reinterpret_cast<const bf16x8*>(xs + (32 * nb + c) * 16 + 8 * h);
     const floatx16 zero = {};
-    halfx8 a1[4], a2[4];
+    halfx8 a1[4][2], a2[4][2];
 #pragma unroll
     for (int mb = 0; mb < 2; mb++) {
-        const floatx16 z1 = mfma32(fr.w1[mb][l], bx, zero);
-        float v[16];
+        const bf16x8 w = fr.w1[mb][l];
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = z1[r];
-        a1[2 * mb] = act8h(v);
-        a1[2 * mb + 1] = act8h(v + 8);
+        for (int nb = 0; nb < 2; nb++) {
+            const floatx16 z1 = mfma32(w, bx[nb], zero);
+            float v[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) v[r] = z1[r];
+            a1[2 * mb][nb] = act8h(v);
+            a1[2 * mb + 1][nb] = act8h(v + 8);
+        }
     }
 #pragma unroll
     for (int mb = 0; mb < 2; mb++) {
-        floatx16 acc;
+        floatx16 acc[2];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const float4 b = fr.b2[c_row(mb, 4 * q, h) / 4];      // hidden units c_row(mb, 4q .. 4q + 3, h)
-            acc[4 * q] = b.x; acc[4 * q + 1] = b.y; acc[4 * q + 2] = b.z; acc[4 * q + 3] = b.w;
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                acc[nb][4 * q] = b.x; acc[nb][4 * q + 1] = b.y; acc[nb][4 * q + 2] = b.z; acc[nb][4 * q + 3] = b.w;
+            }
         }
 #pragma unroll
-        for (int t = 0; t < 4; t++) acc = mfma32(fr.w2[mb][t][l], a1[t], acc);
-        float v[16];
+        for (int t = 0; t < 4; t++) {
+            const halfx8 w = fr.w2[mb][t][l];
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = acc[r];
-        a2[2 * mb] = act8h(v);
-        a2[2 * mb + 1] = act8h(v + 8);
+            for (int nb = 0; nb < 2; nb++) acc[nb] = mfma32(w, a1[t][nb], acc[nb]);
+        }
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) {
+            float v[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) v[r] = acc[nb][r];
+            a2[2 * mb][nb] = act8h(v);
+            a2[2 * mb + 1][nb] = act8h(v + 8);
+        }
     }
-    floatx16 o = {};
+    floatx16 b3 = {};
 #pragma unroll
     for (int r = 0; r < 4; r++)
-        if (out_of(h, r) < kFarOut) o[r] = fr.b3[out_of(h, r)];
+        if (out_of(h, r) < kFarOut) b3[r] = fr.b3[out_of(h, r)];
+    o[0] = b3;
+    o[1] = b3;
 #pragma unroll
-    for (int t = 0; t < 4; t++) o = mfma32(fr.w3[t][l], a2[t], o);
-    return o;
+    for (int t = 0; t < 4; t++) {
+        const halfx8 w = fr.w3[t][l];
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++) o[nb] = mfma32(w, a2[t][nb], o[nb]);
+    }
 }
 
-__global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_far_mlp(FarArgs a) {
+__global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_far_mlp(FarArgs a) {
     __shared__ FarInfer wfr;
     __shared__ __attribute__((aligned(16))) __bf16 xs_all[kMlpWaves][kWave * 16];
     __shared__ FarRow rows_all[kMlpWaves][kWave];
@@ -411,25 +434,25 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // 2. the network, 32 rows at a time, then 3. rotate, scale and
-        //    accumulate per observer lane: the force from outputs 0..3 (lanes
-        //    h = 0), the potential from outputs 4, 5 (h = 1)
-#pragma unroll 1
+        // 2. the network, then 3. rotate, scale and accumulate per observer
+        //    lane: the force from outputs 0..3 (lanes h = 0), the potential
+        //    from outputs 4, 5 (h = 1)
+        floatx16 o[2];
+        mlp_tile(fr, xs, l, o);
+#pragma unroll
         for (int nb = 0; nb < 2; nb++) {
-            if (base + 32 * nb >= cnt) break;
-            const floatx16 o = mlp_half(fr, xs, l, nb);
             const int row = 32 * nb + c;
             if (base + row < cnt) {
                 const FarRow q = rows[row];
                 if (h == 0) {
-                    const float gu = o[0] * fr.oscale[0], gv = o[1] * fr.oscale[1];
-                    const float cu = o[2] * fr.oscale[2], cv = o[3] * fr.oscale[3];
+                    const float gu = o[nb][0] * fr.oscale[0], gv = o[nb][1] * fr.oscale[1];
+                    const float cu = o[nb][2] * fr.oscale[2], cv = o[nb][3] * fr.oscale[3];
                     const float fx = q.sg * (q.ux * gu - q.uy * gv) + q.sc * (q.ux * cu - q.uy * cv);
                     const float fy = q.sg * (q.uy * gu + q.ux * gv) + q.sc * (q.uy * cu + q.ux * cv);
                     atomicAdd(&sacc[rlane[row]][0], fx);
                     atomicAdd(&sacc[rlane[row]][1], fy);
                 } else {
-                    const float pe = q.sp * o[0] * fr.oscale[4] + q.cp * o[1] * fr.oscale[5];
+                    const float pe = q.sp * o[nb][0] * fr.oscale[4] + q.cp * o[nb][1] * fr.oscale[5];
                     atomicAdd(&sacc[rlane[row]][2], pe);
                 }
             }
