@@ -137,6 +137,7 @@ struct FarFrags {
     bf16x8 w3[4][kWave];            // layer 3, A = W3^T padded to 32 rows, K permuted
     halfx8 w2h[2][4][kWave];        // inference: layers 2 and 3 in FP16 (as w2, w3)
     halfx8 w3h[4][kWave];
+    halfx8 b2h[2][kWave];           // inference: A[m][0], A[m][1] = b2 split into FP16 hi + lo, K 2..15 zero
     bf16x8 w3t[2][kWave];           // backward: A = W3 (hidden x outputs; K = 8h + j <-> output 4h + j)
     bf16x8 w2t[2][4][kWave];        // backward: A = W2 (in x out), K (out) permuted
     float b2[kFarHid];
@@ -175,6 +176,16 @@ __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict_
         fr->w3h[t][l] = pack8h(v);
     }
     fr->b2[l] = P[kFarOffB2 + l];
+    for (int mb = 0; mb < 2; mb++) {
+        float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (h == 0) {
+            const float b = P[kFarOffB2 + 32 * mb + c];
+            const _Float16 hi = (_Float16)b;
+            v[0] = (float)hi;
+            v[1] = b - (float)hi;
+        }
+        fr->b2h[mb][l] = pack8h(v);
+    }
     if (l < kFarOut) fr->b3[l] = P[kFarOffB3 + l];
     if (l < kFarIn) {
         fr->mean[l] = stats[l];
@@ -273,7 +284,7 @@ struct FarInfer {
     bf16x8 w1[2][kWave];
     halfx8 w2[2][4][kWave];             // FP16: the packed-FP16 activations feed them
     halfx8 w3[4][kWave];
-    float4 b2[kFarHid / 4];             // 16-byte rows: accumulators 4q .. 4q + 3 of a block share one
+    halfx8 b2h[2][kWave];               // layer-2 bias as an MFMA operand (times a column of ones)
     float b3[8];
     float mean[kFarIn], istd[kFarIn], oscale[8];
 };
@@ -308,6 +319,9 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const __bf16* xs, i
 #pragma unroll
     for (int nb = 0; nb < 2; nb++) bx[nb] = *reinterpret_cast<const bf16x8*>(xs + (32 * nb + c) * 16 + 8 * h);
     const floatx16 zero = {};
+    // B operand of the bias product: B[0][n] = B[1][n] = 1, the rest 0
+    halfx8 ones = {};
+    if (h == 0) { ones[0] = (_Float16)1.f; ones[1] = (_Float16)1.f; }
     halfx8 a1[4][2], a2[4][2];
 #pragma unroll
     for (int mb = 0; mb < 2; mb++) {
@@ -325,14 +339,8 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const __bf16* xs, i
 #pragma unroll
     for (int mb = 0; mb < 2; mb++) {
         floatx16 acc[2];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float4 b = fr.b2[c_row(mb, 4 * q, h) / 4];      // hidden units c_row(mb, 4q .. 4q + 3, h)
-#pragma unroll
-            for (int nb = 0; nb < 2; nb++) {
-                acc[nb][4 * q] = b.x; acc[nb][4 * q + 1] = b.y; acc[nb][4 * q + 2] = b.z; acc[nb][4 * q + 3] = b.w;
-            }
-        }
+        acc[0] = mfma32(fr.b2h[mb][l], ones, zero);             // b2 in every column, hi + lo in FP32
+        acc[1] = acc[0];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const halfx8 w = fr.w2[mb][t][l];
@@ -377,7 +385,7 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         copy(reinterpret_cast<uint32_t*>(wfr.w1), reinterpret_cast<const uint32_t*>(g.w1), sizeof(g.w1) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.w2), reinterpret_cast<const uint32_t*>(g.w2h), sizeof(g.w2h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.w3), reinterpret_cast<const uint32_t*>(g.w3h), sizeof(g.w3h) / 4);
-        copy(reinterpret_cast<uint32_t*>(wfr.b2), reinterpret_cast<const uint32_t*>(g.b2), kFarHid);
+        copy(reinterpret_cast<uint32_t*>(wfr.b2h), reinterpret_cast<const uint32_t*>(g.b2h), sizeof(g.b2h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.b3), reinterpret_cast<const uint32_t*>(g.b3), kFarOut);
         copy(reinterpret_cast<uint32_t*>(wfr.mean), reinterpret_cast<const uint32_t*>(g.mean), kFarIn);
         copy(reinterpret_cast<uint32_t*>(wfr.istd), reinterpret_cast<const uint32_t*>(g.istd), kFarIn);
