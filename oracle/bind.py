@@ -66,6 +66,14 @@ def lib():
         L.orc_set_lifetime.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
         L.orc_delayed_state.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, _DP]
         L.orc_delayed_state.restype = C.c_int
+        L.orc_collision_pairs_ordered.argtypes = [C.c_void_p, _IP, _IP, C.c_int64]
+        L.orc_collision_pairs_ordered.restype = C.c_int64
+        L.orc_collision_events.argtypes = [C.c_void_p, _DP, C.c_int64]
+        L.orc_collision_events.restype = C.c_int64
+        L.orc_set_antimatter.argtypes = [C.c_void_p, _IP]
+        L.orc_download_ex.argtypes = [C.c_void_p, _DP, _DP, _IP, _DP]
+        L.orc_leaf_points.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), _IP, C.c_int64]
+        L.orc_leaf_points.restype = C.c_int64
         _LIB = L
     return _LIB
 
@@ -176,6 +184,41 @@ class Oracle:
         b = np.zeros(max(n, 1), dtype=np.int32)
         self.L.orc_collision_pairs(self.s, a.ctypes.data_as(_IP), b.ctypes.data_as(_IP), n)
         return a[:n], b[:n]
+
+    def collision_pairs_ordered(self):
+        n = self.L.orc_collision_pairs_ordered(self.s, None, None, 0)
+        a = np.zeros(max(n, 1), dtype=np.int32)
+        b = np.zeros(max(n, 1), dtype=np.int32)
+        self.L.orc_collision_pairs_ordered(self.s, a.ctypes.data_as(_IP), b.ctypes.data_as(_IP), n)
+        return a[:n], b[:n]
+
+    def collision_events(self):
+        """Events of the last update: rows {type (1 merge, 2 annihilation), x, y, energy, px, py}."""
+        n = self.L.orc_collision_events(self.s, None, 0)
+        out = np.zeros((max(n, 1), 6))
+        self.L.orc_collision_events(self.s, _d(out), n)
+        return out[:n]
+
+    def set_antimatter(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        self.L.orc_set_antimatter(self.s, f.ctypes.data_as(_IP))
+
+    def leaf_points(self):
+        """(key, id) rows of every leaf's stored point order; key = level << 56 | path."""
+        m = self.L.orc_leaf_points(self.s, None, None, 0)
+        key = np.zeros(max(m, 1), dtype=np.uint64)
+        ids = np.zeros(max(m, 1), dtype=np.int32)
+        self.L.orc_leaf_points(self.s, key.ctypes.data_as(C.POINTER(C.c_uint64)), ids.ctypes.data_as(_IP), m)
+        return key[:m], ids[:m]
+
+    def download_ex(self):
+        n = self.count()
+        out = {k: np.zeros(n) for k in ("base_mass", "charge", "radius")}
+        anti = np.zeros(n, dtype=np.int32)
+        self.L.orc_download_ex(self.s, _d(out["base_mass"]), _d(out["charge"]), anti.ctypes.data_as(_IP),
+                               _d(out["radius"]))
+        out["antimatter"] = anti
+        return out
 
     def far_samples(self, idx):
         """Accepted far nodes of the observers in idx with their exact residuals

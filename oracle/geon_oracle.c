@@ -88,6 +88,7 @@ typedef struct {
     int32_t id;                    /* upload index */
     int32_t q_quanta;              /* round(charge / 0.1) */
     int32_t isGhost, original;     /* ghost: index of the real particle */
+    int32_t antimatter;            /* particle.js:21 */
     Hist* hist;                    /* NULL until the first sample (particle.js:124) */
     double creationTime, deathTime, deathMass;   /* particle.js:17-18; _deathMass */
 } P;
@@ -124,6 +125,9 @@ struct OrcSys {
     int histStride;                /* _histStride (integrator.js:111) */
     P* dead;                       /* sim.deadParticles (main.js:103) */
     int ndead, deadcap;
+    int32_t next_id;               /* Particle.nextId (particle.js:13) */
+    double* ev;                    /* collision events of the last update, ORC_EVENT_STRIDE doubles each */
+    int nev, evcap;
 };
 
 static int g_threads = 1;
@@ -832,11 +836,13 @@ static void generateGhosts(OrcSys* s) {                                    /* in
 
 /* _buildTree (integrator.js:247-260) + QuadTreePool.build (quadtree.js:412-419).
  * Root box: centre (W/2, H/2), half-extent (2W, 2H) (integrator.js:727-730). */
-static int buildTree(OrcSys* s) {
+/* ghosts: the periodic images of _buildTree (integrator.js:247-258) go into
+ * the tree; the force walk takes them only in GEON_PERIODIC_GHOSTS mode. */
+static int buildTreeG(OrcSys* s, int ghosts) {
     tp_reserve(s, s->n + 8);
     s->ntp = 0;
     for (int i = 0; i < s->n; i++) { s->tp[s->ntp] = s->p[i]; s->tp[s->ntp].isGhost = 0; s->tp[s->ntp].original = i; s->ntp++; }
-    if (s->periodic && s->prm.periodic_mode == GEON_PERIODIC_GHOSTS) generateGhosts(s);
+    if (ghosts) generateGhosts(s);
     Pool* q = &s->pool;
     q->count = 0;
     const double W = s->prm.domain_w, H = s->prm.domain_h;
@@ -849,6 +855,8 @@ static int buildTree(OrcSys* s) {
     s->root = root;
     return root;
 }
+
+static int buildTree(OrcSys* s) { return buildTreeG(s, s->periodic && s->prm.periodic_mode == GEON_PERIODIC_GHOSTS); }
 
 /* ------------------------------------------------------------------ */
 /* forces.js: walk, direct sum, computeAllForces                       */
@@ -1272,6 +1280,7 @@ void orc_destroy(OrcSys* s) {
     for (int i = 0; i < s->n; i++) free(s->p[i].hist);
     dead_free(s);
     free(s->dead);
+    free(s->ev);
     free(s->p); free(s->tp); free(s);
 }
 
@@ -1312,6 +1321,8 @@ int orc_upload(OrcSys* s, int32_t n, const double* x, const double* y, const dou
         p->creationTime = -INFINITY;                /* loaded particles always existed (save-load.js:231) */
         p->deathTime = INFINITY;
     }
+    s->next_id = n;
+    s->nev = 0;
     s->forcesInit = 0;
     s->simTime = 0;
     s->root = -1;
@@ -1410,10 +1421,13 @@ static void cyclotronScan(OrcSys* s, double* maxCyc) {                    /* int
 }
 
 /* Physics.update (integrator.js:726-1825), the parts on the hot path. */
+#include "geon_oracle_merge.inc"
+
 int orc_update(OrcSys* s, double dt, int32_t* substeps) {
     syncToggles(s);
     const GeonParams* g = &s->prm;
-    const int n = s->n;
+    int n = s->n;                  /* particles.length, refreshed after collisions (integrator.js:1386) */
+    s->nev = 0;
     const int relOn = s->rel;
     if (!s->forcesInit && n > 0) {                                        /* integrator.js:750-782 */
         for (int i = 0; i < n; i++) s->p[i].angVel = relOn ? angwToAngVel(s->p[i].angw, s->p[i].radius) : s->p[i].angw;
@@ -1559,7 +1573,21 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
             of_modulate_masses(s, &s->higgsF, dtSub);
         }
         if (s->axion) of_update(s, &s->axionF, dtSub, s->higgs ? &s->higgsF : NULL);   /* 1331-1334 */
-        forcePass(s, has1PN && s->barnesHut);                             /* integrator.js:1489-1493 */
+        int reuse = has1PN && s->barnesHut;
+        if (g->collision_mode == GEON_COL_MERGE && n > 0) {              /* integrator.js:1336-1416 */
+            /* the query finds neighbours across a periodic edge through the
+             * ghosts of the step-5 tree (integrator.js:247-258), also where the
+             * force walk runs without them */
+            const int ghostQuery = s->periodic && g->periodic_mode != GEON_PERIODIC_GHOSTS;
+            if (!reuse || ghostQuery) buildTreeG(s, s->periodic);
+            if (ghostQuery) reuse = 0;
+            /* After a change the next force pass walks a fresh tree: the
+             * reference walks the step-5 tree, where removed bodies still sit
+             * (mass 0) and spawned ones are missing, for one substep. */
+            if (mergeCollisions(s)) reuse = 0;
+            n = s->n;
+        }
+        forcePass(s, reuse);                                              /* integrator.js:1489-1493 */
     }
     if (s->delay && n > 0) {                                              /* integrator.js:1549-1565 */
         const int stride = g->history_stride > 0 ? g->history_stride : 64;

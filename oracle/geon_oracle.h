@@ -49,6 +49,22 @@ void    orc_counters(OrcSys* s, int64_t* out3);
  * overlap, found by tree query.  Returns the number of pairs; writes at most
  * cap pairs as (a[k], b[k]) upload indices, sorted. */
 int64_t orc_collision_pairs(OrcSys* s, int32_t* a, int32_t* b, int64_t cap);
+/* The same pairs unsorted, in the order handleCollisions meets them: array
+ * order of the first member, then the tree query order (quadtree.js:372-398). */
+int64_t orc_collision_pairs_ordered(OrcSys* s, int32_t* a, int32_t* b, int64_t cap);
+/* Collision events of the last update in merge mode (collisions.js:62-92):
+ * merges with lost kinetic energy first, then annihilations, each in the
+ * order they happened, as ORC_EVENT_STRIDE doubles {type, x, y, energy, px, py}.
+ * Returns the count; writes at most cap. */
+#define ORC_EVENT_STRIDE 6
+#define ORC_EV_MERGE 1
+#define ORC_EV_ANNIHILATION 2
+int64_t orc_collision_events(OrcSys* s, double* out, int64_t cap);
+/* Antimatter flags in array order (particle.js:21); extra per-particle state. */
+int     orc_set_antimatter(OrcSys* s, const int32_t* flags);
+int     orc_download_ex(OrcSys* s, double* base_mass, double* charge, int32_t* antimatter, double* radius);
+/* Leaf contents in stored order: rows (level << 56 | path, id). */
+int64_t orc_leaf_points(OrcSys* s, uint64_t* key, int32_t* id, int64_t cap);
 
 /* Stand-alone FP64 direct sum of gravity+Coulomb+... (forces.js:93-126) over
  * `n` sources for the targets listed in `idx` (all if idx == NULL).  Uses the
