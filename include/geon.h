@@ -86,7 +86,7 @@ typedef struct GeonParams {
     uint32_t toggles0;          /* GEON_GRAVITY | ... */
     int32_t  boundary_mode;     /* GEON_BOUND_* */
     int32_t  topology;          /* GEON_TORUS, GEON_KLEIN, GEON_RP2 (loop boundary, topology.js) */
-    int32_t  collision_mode;    /* GEON_COL_PASS only on this backend */
+    int32_t  collision_mode;    /* GEON_COL_PASS, or GEON_COL_MERGE on open and torus domains */
     int32_t  periodic_mode;     /* GEON_PERIODIC_* */
     int32_t  max_substeps;      /* MAX_SUBSTEPS = 32 (config.js:36) */
     double   domain_w, domain_h;/* width/height passed to Physics.update */
@@ -146,6 +146,7 @@ typedef struct GeonParticleInit {
 typedef struct GeonSoAView {
     double *x, *y, *wx, *wy, *mass, *base_mass, *charge, *angw;
     int32_t *alive;             /* download only: 1 alive, 0 despawned/removed */
+    int32_t *antimatter;        /* 1 antimatter (particle.js:21); NULL on upload: all matter */
 } GeonSoAView;
 
 /* Per-particle results of the last force pass, slot order (any may be NULL).
@@ -287,6 +288,19 @@ int  geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse
  * the tree at the current positions.  Writes min(cap, total) pairs sorted by
  * (a, b) and the total to *n_pairs (a, b may be NULL to count only). */
 int  geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs);
+
+/* Collision events of the last geon_update in collision mode merge
+ * (collisions.js:62-92, integrator.js:1388-1415): merges that lost kinetic
+ * energy (the field excitations' source), then annihilations (the photon
+ * bursts' source), each in the order the fold met them, as rows of
+ * GEON_EVENT_STRIDE doubles {type, x, y, energy, px, py}; type 1 merge, 2
+ * annihilation.  Photons are not simulated by this backend: a host that
+ * keeps bosons emits the bursts from these rows (emitPhotonBurst).  Writes
+ * min(cap, total) rows and the total to *n_events. */
+#define GEON_EVENT_STRIDE 6
+#define GEON_EVENT_MERGE 1
+#define GEON_EVENT_ANNIHILATION 2
+int  geon_collision_events(GeonHandle h, double* out, int64_t cap, int64_t* n_events);
 
 int  geon_counters(GeonHandle h, GeonCounters* out);
 

@@ -50,7 +50,8 @@ void    orc_counters(OrcSys* s, int64_t* out3);
  * cap pairs as (a[k], b[k]) upload indices, sorted. */
 int64_t orc_collision_pairs(OrcSys* s, int32_t* a, int32_t* b, int64_t cap);
 /* The same pairs unsorted, in the order handleCollisions meets them: array
- * order of the first member, then the tree query order (quadtree.js:372-398). */
+ * order of the first member, then the tree query order (quadtree.js:372-398)
+ * over the tree with its periodic ghosts (integrator.js:247-258). */
 int64_t orc_collision_pairs_ordered(OrcSys* s, int32_t* a, int32_t* b, int64_t cap);
 /* Collision events of the last update in merge mode (collisions.js:62-92):
  * merges with lost kinetic energy first, then annihilations, each in the

@@ -58,7 +58,7 @@ _IP = C.POINTER(C.c_int32)
 
 class GeonSoAView(C.Structure):
     _fields_ = [("x", _DP), ("y", _DP), ("wx", _DP), ("wy", _DP), ("mass", _DP),
-                ("base_mass", _DP), ("charge", _DP), ("angw", _DP), ("alive", _IP)]
+                ("base_mass", _DP), ("charge", _DP), ("angw", _DP), ("alive", _IP), ("antimatter", _IP)]
 
 
 class GeonForceView(C.Structure):
@@ -125,6 +125,7 @@ PROTOTYPES = {
     "geon_fft2d": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32]),
     "geon_collision_pairs": (C.c_int, [H, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64,
                                         C.POINTER(C.c_int64)]),
+    "geon_collision_events": (C.c_int, [H, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]),
     "geon_counters": (C.c_int, [H, C.POINTER(GeonCounters)]),
     "geon_timing_enable": (C.c_int, [H, C.c_int32]),
     "geon_kernel_timing": (C.c_int, [H, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

@@ -23,7 +23,7 @@ namespace {
 // State columns: gathered into Morton order every time the tree is rebuilt.
 enum StateCol {
     C_X, C_Y, C_WX, C_WY, C_VX, C_VY, C_MASS, C_BASEMASS, C_CHARGE, C_ANGW, C_ANGVEL, C_RADIUS,
-    C_MU, C_L, C_F1OX, C_F1OY, C_AX, C_YUK, C_HIGGS, C_QQ, C_SLOT, N_STATE
+    C_MU, C_L, C_F1OX, C_F1OY, C_AX, C_YUK, C_HIGGS, C_QQ, C_SLOT, C_FLAGS, N_STATE
 };
 // Force columns: results of the last force pass, kept between updates.
 enum ForceCol {
@@ -111,6 +111,19 @@ struct GeonContext {
     int n_dead = 0;
     std::vector<DeadRec> dead_host;
     std::vector<double> dead_time;
+    // collision mode merge: pair list, fold scratch, spawn records and events
+    int64_t mg_cap_pairs = 0;
+    int32_t *mg_p1 = nullptr, *mg_p2 = nullptr, *mg_leaf = nullptr;
+    TreeBufs mg_sort = {};              // keys, perm and radix scratch of the pair sort
+    int32_t *mg_fl = nullptr;           // 6 x pairs: spawn / merge-event / annihilation flags and offsets
+    int32_t* mg_scan_tmp = nullptr;
+    MergeRes *mg_res = nullptr, *mg_spawns = nullptr;
+    double* mg_events = nullptr;
+    int32_t *mg_deg = nullptr, *mg_rank = nullptr, *mg_off = nullptr, *mg_s5 = nullptr;
+    double* mg_work = nullptr;          // 4 x particles: mass, base mass, radius, death mass
+    int32_t* mg_cnt = nullptr;          // 8 ints: totals, deaths, same-leaf flag
+    Ctrl* mg_ctrl = nullptr;            // n_alive = pairs of the current fold
+    std::vector<double> events_host;    // events of the last update, 6 doubles each
 };
 
 #define HIP_OK(expr)                                                                         \
@@ -169,7 +182,12 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     if ((t & (GEON_HIGGS | GEON_AXION)) && (G < 8 || G > 128 || (G & (G - 1)))) {
         *why = "field_grid must be a power of two in [8, 128]"; return 1;
     }
-    if (p->collision_mode != GEON_COL_PASS) { *why = "only collision mode 'pass' is run"; return 1; }
+    if (p->collision_mode != GEON_COL_PASS && p->collision_mode != GEON_COL_MERGE) {
+        *why = "collision mode 'bounce' (Hertz contact forces) is not run"; return 1;
+    }
+    if (p->collision_mode == GEON_COL_MERGE && p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) {
+        *why = "merge collisions run on open and torus domains"; return 1;
+    }
     if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
     if (p->topology < GEON_TORUS || p->topology > GEON_RP2) { *why = "unknown topology"; return 1; }
     if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode == GEON_PERIODIC_GHOSTS) { *why = "ghost-image periodic mode is not run (min-image and windowed are)"; return 1; }
@@ -518,6 +536,7 @@ __global__ void k_survivor_perm(const int32_t* alive, const int32_t* scan, uint3
 int sort_and_gather(GeonContext* h) {
     const int n = h->n_host;
     if (n == 0) return GEON_OK;
+    const bool one_pn = !!(h->prm.toggles0 & GEON_ONE_PN);
     const KeyParams kp = key_params(h->prm);
     {
         Timer t(h, 2);
@@ -526,9 +545,18 @@ int sort_and_gather(GeonContext* h) {
         Columns cols;
         cols.ncols = N_STATE;
         for (int c = 0; c < N_STATE; c++) { cols.src[c] = h->st[c]; cols.dst[c] = h->st_alt[c]; }
+        // the 1PN force outlives the force pass (the next kick takes it as the
+        // old value): it moves with its particle
+        if (one_pn) {
+            for (int c : {(int)K_F1X, (int)K_F1Y}) {
+                cols.src[cols.ncols] = reinterpret_cast<uint32_t*>(h->fc[c]);
+                cols.dst[cols.ncols++] = reinterpret_cast<uint32_t*>(h->fc_alt[c]);
+            }
+        }
         launch_gather(h->stream, cols, h->tb.perm, n, h->ctrl);
     }
     for (int c = 0; c < N_STATE; c++) std::swap(h->st[c], h->st_alt[c]);
+    if (one_pn) { std::swap(h->fc[K_F1X], h->fc_alt[K_F1X]); std::swap(h->fc[K_F1Y], h->fc_alt[K_F1Y]); }
     return GEON_OK;
 }
 
@@ -659,7 +687,7 @@ int read_ctrl(GeonContext* h) {
 
 // Remove despawned particles, keeping the survivors' relative order
 // (integrator.js:1821-1823 compacts the array the same way).
-int compact(GeonContext* h) {
+int compact(GeonContext* h, double t_death) {
     const int n = h->n_host;
     int32_t* total = &h->ctrl->n_tree;    // scratch: n_tree is rebuilt before every use
     exclusive_scan_i32(h->stream, h->alive, h->alive_scan, n, &h->ctrl->n_alive, total, h->tb.scan_tmp);
@@ -684,7 +712,7 @@ int compact(GeonContext* h) {
         HIP_OK(hipMemcpyAsync(h->dead_host.data() + h->n_dead, h->dead + h->n_dead, sizeof(DeadRec) * deaths,
                               hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
-        h->dead_time.resize((size_t)(h->n_dead + deaths), h->sim_time);
+        h->dead_time.resize((size_t)(h->n_dead + deaths), t_death);
         h->n_dead += deaths;
     }
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, survivors);
@@ -782,6 +810,8 @@ int allocate(GeonContext* h) {
     return GEON_OK;
 }
 
+void merge_release(GeonContext* h);
+
 void release(GeonContext* h) {
     for (int c = 0; c < N_STATE; c++) { hipFree(h->st[c]); hipFree(h->st_alt[c]); }
     for (int c = 0; c < N_FORCE; c++) { hipFree(h->fc[c]); hipFree(h->fc_alt[c]); }
@@ -796,6 +826,7 @@ void release(GeonContext* h) {
     hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64); hipFree(tb.node_qq64);
     hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
     hipFree(tb.node_first); hipFree(tb.node_mom); hipFree(tb.pre_mom);
+    merge_release(h);
     far_release(h);
     hipFree(h->stat_partial); hipFree(h->stat_out); hipFree(h->field_out);
     for (int w = 0; w < 2; w++) {
@@ -887,6 +918,189 @@ FarTrainArgs far_train_args(GeonContext* h, float wscale) {
     a.grad = h->far_grad; a.loss_out = h->far_loss;
     a.params = h->far_params; a.adam_m = h->far_m; a.adam_v = h->far_v;
     return a;
+}
+
+}  // namespace
+
+namespace {
+
+// ---- collision mode merge (collisions.js:24-158, integrator.js:1336-1416) ----
+
+void merge_free_pairs(GeonContext* h) {
+    hipFree(h->mg_p1); hipFree(h->mg_p2); hipFree(h->mg_leaf);
+    hipFree(h->mg_sort.keys); hipFree(h->mg_sort.keys_alt); hipFree(h->mg_sort.perm); hipFree(h->mg_sort.perm_alt);
+    hipFree(h->mg_sort.hist); hipFree(h->mg_sort.digit_total);
+    hipFree(h->mg_fl); hipFree(h->mg_scan_tmp); hipFree(h->mg_res); hipFree(h->mg_spawns); hipFree(h->mg_events);
+    h->mg_p1 = h->mg_p2 = h->mg_leaf = nullptr;
+    h->mg_sort = {};
+    h->mg_fl = h->mg_scan_tmp = nullptr;
+    h->mg_res = h->mg_spawns = nullptr;
+    h->mg_events = nullptr;
+    h->mg_cap_pairs = 0;
+}
+
+void merge_release(GeonContext* h) {
+    merge_free_pairs(h);
+    hipFree(h->mg_deg); hipFree(h->mg_rank); hipFree(h->mg_off); hipFree(h->mg_s5); hipFree(h->mg_work);
+    hipFree(h->mg_cnt); hipFree(h->mg_ctrl);
+    h->mg_deg = h->mg_rank = h->mg_off = h->mg_s5 = h->mg_cnt = nullptr;
+    h->mg_work = nullptr;
+    h->mg_ctrl = nullptr;
+}
+
+// Scratch sized by the particle capacity, allocated at the first merge pass.
+int merge_alloc(GeonContext* h) {
+    if (h->mg_deg) return GEON_OK;
+    const size_t cap = (size_t)h->cap;
+    int rc;
+    if ((rc = dmalloc(h, &h->mg_deg, cap)) || (rc = dmalloc(h, &h->mg_rank, cap)) ||
+        (rc = dmalloc(h, &h->mg_off, cap)) || (rc = dmalloc(h, &h->mg_work, 4 * cap)) ||
+        (rc = dmalloc(h, &h->mg_cnt, 8)) || (rc = dmalloc(h, &h->mg_ctrl, 1)))
+        return rc;
+    HIP_OK(hipMemset(h->mg_ctrl, 0, sizeof(Ctrl)));
+    return GEON_OK;
+}
+
+// Buffers sized by the number of pairs, grown when a pass needs more.
+int merge_reserve_pairs(GeonContext* h, int64_t np) {
+    if (np <= h->mg_cap_pairs) return GEON_OK;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    merge_free_pairs(h);
+    const size_t cap = (size_t)std::max<int64_t>(std::max<int64_t>(np, 2 * h->mg_cap_pairs), 1024);
+    const size_t nblk = cap / 2048 + 1;
+    int rc;
+    TreeBufs& sb = h->mg_sort;
+    if ((rc = dmalloc(h, &h->mg_p1, cap)) || (rc = dmalloc(h, &h->mg_p2, cap)) || (rc = dmalloc(h, &h->mg_leaf, cap)) ||
+        (rc = dmalloc(h, &sb.keys, cap)) || (rc = dmalloc(h, &sb.keys_alt, cap)) || (rc = dmalloc(h, &sb.perm, cap)) ||
+        (rc = dmalloc(h, &sb.perm_alt, cap)) || (rc = dmalloc(h, &sb.hist, 256 * nblk)) ||
+        (rc = dmalloc(h, &sb.digit_total, 256)) || (rc = dmalloc(h, &h->mg_fl, 6 * cap)) ||
+        (rc = dmalloc(h, &h->mg_scan_tmp, nblk + 16)) || (rc = dmalloc(h, &h->mg_res, cap)) ||
+        (rc = dmalloc(h, &h->mg_spawns, cap)) || (rc = dmalloc(h, &h->mg_events, 6 * cap)))
+        return rc;
+    h->mg_cap_pairs = (int64_t)cap;
+    return GEON_OK;
+}
+
+// Step 6 of a substep in collision mode merge: the pair list over a tree at
+// the current positions, the fold, removal (retired for signal delay) and the
+// spawned particles, field excitations.  *changed: the particle set changed.
+int merge_step(GeonContext* h, bool* changed) {
+    *changed = false;
+    const int n = h->n_host;
+    if (n == 0) return GEON_OK;
+    int rc;
+    if ((rc = merge_alloc(h))) return rc;
+    if ((rc = sort_and_gather(h))) return rc;
+    build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
+               I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+    launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
+                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), nullptr, nullptr, nullptr, h->src4,
+                           h->src8a, h->src8b, n, h->ctrl, /*update_moments=*/0);
+    HIP_OK(hipMemsetAsync(h->mg_cnt, 0, 8 * sizeof(int32_t), h->stream));
+    PairArgs pa;
+    pa.x = F(h, C_X); pa.y = F(h, C_Y); pa.mass = F(h, C_MASS); pa.radius = F(h, C_RADIUS);
+    pa.slot = I(h, C_SLOT); pa.src4 = h->src4; pa.nodes = h->tb.nodes; pa.max_nodes = 1 + 4 * h->tb.cap_divided;
+    pa.count = h->mg_deg; pa.offset = h->mg_off; pa.pa = nullptr; pa.pb = nullptr; pa.pl = nullptr;
+    pa.index_out = 1;
+    pa.slack = 1.0f + 1.0f / 4096;      // a superset: the fold tests every pair again in FP64
+    pa.dom_w = (float)h->prm.domain_w; pa.dom_h = (float)h->prm.domain_h;
+    pa.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
+    pa.ctrl = h->ctrl; pa.n_upper = n;
+    launch_pairs(h->stream, pa, false);
+    exclusive_scan_i32(h->stream, h->mg_deg, h->mg_off, n, &h->ctrl->n_alive, h->mg_cnt + 4, h->tb.scan_tmp);
+    int32_t cnt[8];
+    HIP_OK(hipMemcpyAsync(cnt, h->mg_cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    if ((rc = read_ctrl(h))) return rc;
+    if (h->ctrl_host->overflow) return check_overflow(h);
+    const int np = cnt[4];
+    const int n_div = h->ctrl_host->n_divided;
+    const double t_now = h->ctrl_host->sim_time;
+    if (np == 0) return GEON_OK;
+    if ((rc = merge_reserve_pairs(h, np))) return rc;
+    hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->mg_ctrl, np);
+    pa.pa = h->mg_p1; pa.pb = h->mg_p2; pa.pl = h->mg_leaf;
+    launch_pairs(h->stream, pa, true);
+    // leaf orders only where a particle meets two candidates in one leaf
+    merge_pair_same_leaf(h->stream, h->mg_p1, h->mg_leaf, h->mg_ctrl, h->mg_cnt + 5, np);
+    HIP_OK(hipMemcpyAsync(cnt, h->mg_cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    const bool need_rank = cnt[5] != 0;
+    if (need_rank) {
+        if (!h->mg_s5 && (rc = dmalloc(h, &h->mg_s5, 5 * (size_t)(1 + 4 * h->tb.cap_divided)))) return rc;
+        merge_leaf_ranks(h->stream, h->tb, I(h, C_SLOT), 1 + 4 * n_div, h->mg_s5, h->mg_p1, h->mg_leaf, h->mg_ctrl,
+                         h->mg_rank, np);
+    }
+    merge_pair_keys(h->stream, h->mg_p1, h->mg_p2, h->mg_leaf, I(h, C_SLOT), need_rank ? h->mg_rank : nullptr,
+                    h->mg_ctrl, h->ctrl, h->mg_sort.keys, h->mg_sort.perm, np);
+    radix_sort(h->stream, h->mg_sort, np, 64, h->mg_ctrl);
+    MergeArgs m;
+    m.x = F(h, C_X); m.y = F(h, C_Y); m.wx = F(h, C_WX); m.wy = F(h, C_WY);
+    m.mass = F(h, C_MASS); m.base_mass = F(h, C_BASEMASS); m.radius = F(h, C_RADIUS); m.angw = F(h, C_ANGW);
+    m.qq = I(h, C_QQ); m.flags = h->st[C_FLAGS];
+    m.p1 = h->mg_p1; m.p2 = h->mg_p2; m.order = h->mg_sort.perm;
+    m.deg = h->mg_deg;
+    const size_t cap = (size_t)h->cap;
+    m.wm = h->mg_work; m.wbm = h->mg_work + cap; m.wr = h->mg_work + 2 * cap; m.wdm = h->mg_work + 3 * cap;
+    m.res = h->mg_res; m.spawns = h->mg_spawns;
+    const size_t pc = (size_t)h->mg_cap_pairs;
+    m.f_spawn = h->mg_fl; m.f_mev = h->mg_fl + pc; m.f_aev = h->mg_fl + 2 * pc;
+    m.o_spawn = h->mg_fl + 3 * pc; m.o_mev = h->mg_fl + 4 * pc; m.o_aev = h->mg_fl + 5 * pc;
+    m.cnt = h->mg_cnt; m.scan_tmp = h->mg_scan_tmp; m.events = h->mg_events;
+    m.mc = h->mg_ctrl; m.ctrl = h->ctrl;
+    m.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
+    m.dom_w = h->prm.domain_w; m.dom_h = h->prm.domain_h;
+    merge_resolve(h->stream, m, np, n);
+    merge_apply(h->stream, m, F(h, C_MASS), F(h, C_BASEMASS), F(h, C_RADIUS), h->alive, n);
+    HIP_OK(hipMemcpyAsync(cnt, h->mg_cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    const int n_spawn = cnt[0], n_ev = cnt[1] + cnt[2], deaths = cnt[3];
+    if (n_ev > 0) {
+        const size_t old = h->events_host.size();
+        h->events_host.resize(old + 6 * (size_t)n_ev);
+        HIP_OK(hipMemcpyAsync(h->events_host.data() + old, h->mg_events, sizeof(double) * 6 * n_ev,
+                              hipMemcpyDeviceToHost, h->stream));
+    }
+    if (n_spawn > 0 && h->next_slot + n_spawn > h->cap)
+        return set_err(h, GEON_EFULL, "slot space exhausted: merged particles need new slots (capacity bounds all slots ever used)");
+    if (deaths > 0 && (rc = compact(h, t_now))) return rc;
+    if (n_spawn > 0) {
+        SpawnArgs sa;
+        sa.spawns = h->mg_spawns;
+        sa.count = n_spawn;
+        sa.base = h->n_host;
+        sa.first_slot = h->next_slot;
+        sa.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
+        sa.rel = !!(h->prm.toggles0 & GEON_RELATIVITY);
+        sa.dom_w = h->prm.domain_w; sa.dom_h = h->prm.domain_h;
+        sa.x = F(h, C_X); sa.y = F(h, C_Y); sa.wx = F(h, C_WX); sa.wy = F(h, C_WY); sa.vx = F(h, C_VX);
+        sa.vy = F(h, C_VY); sa.mass = F(h, C_MASS); sa.base_mass = F(h, C_BASEMASS); sa.charge = F(h, C_CHARGE);
+        sa.angw = F(h, C_ANGW); sa.angvel = F(h, C_ANGVEL); sa.radius = F(h, C_RADIUS); sa.mu = F(h, C_MU);
+        sa.L = F(h, C_L); sa.f1ox = F(h, C_F1OX); sa.f1oy = F(h, C_F1OY); sa.ax = F(h, C_AX); sa.yuk = F(h, C_YUK);
+        sa.higgs = F(h, C_HIGGS); sa.qq = I(h, C_QQ); sa.slot = I(h, C_SLOT); sa.flags = h->st[C_FLAGS];
+        sa.n_force = N_FORCE;
+        for (int c = 0; c < N_FORCE; c++) sa.force[c] = h->fc[c];
+        merge_spawn(h->stream, sa);
+        hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, h->n_host + n_spawn);
+        launch_hist_reset(h->stream, h->hist_fe, h->hist_cd, h->next_slot, n_spawn, t_now);   // main.js:514
+        h->n_host += n_spawn;
+        h->next_slot += n_spawn;
+    }
+    const uint32_t t = h->prm.toggles0;
+    const bool hg = !!(t & GEON_HIGGS), ax = !!(t & GEON_AXION);
+    if (n_ev > 0 && (hg || ax)) {              // integrator.js:1394-1415
+        const double g2h = 1.0, g2a = 0.05 * 0.05;        // HIGGS_COUPLING^2, AXION_COUPLING^2 (config.js:90, 97)
+        const double fh = (hg && ax) ? g2h / (g2h + g2a) : 1.0, fa = (hg && ax) ? g2a / (g2h + g2a) : 1.0;
+        for (int w = 0; w < 2; w++) {
+            if (!(w == 0 ? hg : ax)) continue;
+            const FieldArgs f = field_args(h, w);
+            merge_excite(h->stream, f.field_dot, f.G, h->prm.domain_w, h->prm.domain_h, h->mg_events, n_ev,
+                         w == 0 ? fh : fa);
+        }
+    }
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->tree_valid = false;
+    *changed = deaths > 0 || n_spawn > 0;
+    return GEON_OK;
 }
 
 }  // namespace
@@ -1005,6 +1219,7 @@ int geon_upload_soa(GeonHandle h, const GeonSoAView* v, int32_t n) {
         putf(C_AX, i, 1.0); putf(C_YUK, i, 1.0); putf(C_HIGGS, i, 1.0);
         col[C_QQ][i] = (uint32_t)qq;
         col[C_SLOT][i] = (uint32_t)i;
+        col[C_FLAGS][i] = (v->antimatter && v->antimatter[i]) ? 1u : 0u;
     }
     for (int c = 0; c < N_STATE; c++)
         if (n) HIP_OK(hipMemcpyAsync(h->st[c], col[c].data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
@@ -1043,6 +1258,7 @@ int32_t geon_add_particle(GeonHandle h, const GeonParticleInit* p) {
     w[C_QQ] = (uint32_t)qq;
     const int32_t slot = h->next_slot;
     w[C_SLOT] = (uint32_t)slot;
+    w[C_FLAGS] = p->antimatter ? 1u : 0u;
     const int i = h->n_host;
     for (int c = 0; c < N_STATE; c++) HIP_OK(hipMemcpyAsync(h->st[c] + i, &w[c], 4, hipMemcpyHostToDevice, h->stream));
     for (int c = 0; c < N_FORCE; c++) HIP_OK(hipMemsetAsync(h->fc[c] + i, 0, 4, h->stream));
@@ -1073,7 +1289,7 @@ int geon_remove_particle(GeonHandle h, int32_t slot) {
     std::vector<int32_t> alive((size_t)h->n_host, 1);
     alive[idx] = 0;
     HIP_OK(hipMemcpyAsync(h->alive, alive.data(), 4 * alive.size(), hipMemcpyHostToDevice, h->stream));
-    rc = compact(h);
+    rc = compact(h, h->sim_time);
     HIP_OK(hipStreamSynchronize(h->stream));
     return rc;
 }
@@ -1112,6 +1328,7 @@ int geon_download_soa(GeonHandle h, GeonSoAView* v, int32_t* n_slots) {
         if (v->base_mass) v->base_mass[s] = getf(C_BASEMASS, i);
         if (v->charge) v->charge[s] = (int32_t)col[C_QQ][i] * 0.1;
         if (v->angw) v->angw[s] = getf(C_ANGW, i);
+        if (v->antimatter) v->antimatter[s] = (int32_t)(col[C_FLAGS][i] & 1u);
         if (v->alive) v->alive[s] = 1;
     }
     if (n_slots) *n_slots = ns;
@@ -1137,6 +1354,7 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
         h->forces_init = true;
     }
     const int n = h->n_host;
+    h->events_host.clear();
     hipLaunchKernelGGL(k_begin_update, dim3(1), dim3(1), 0, h->stream, h->ctrl, dt, h->prm.max_substeps, h->sim_time);
     StepArgs sa = step_args(h);
     const double soft = (h->prm.toggles0 & GEON_BLACK_HOLE) ? h->prm.bh_softening : h->prm.softening;
@@ -1155,7 +1373,16 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
             const bool one_pn = !!(h->prm.toggles0 & GEON_ONE_PN);
             if (one_pn && (rc = one_pn_step(h))) return rc;
             if (fields_on(h)) fields_step(h, 0.f);
-            if ((rc = force_pass(h, one_pn && (h->prm.toggles0 & GEON_BARNES_HUT)))) return rc;
+            bool reuse = one_pn && (h->prm.toggles0 & GEON_BARNES_HUT);
+            if (h->prm.collision_mode == GEON_COL_MERGE) {
+                // After a change the force pass walks a fresh tree; the reference
+                // walks its step-5 tree, where removed bodies still sit (mass 0)
+                // and spawned ones are missing, for this one substep.
+                bool changed = false;
+                if ((rc = merge_step(h, &changed))) return rc;
+                if (changed) reuse = false;
+            }
+            if ((rc = force_pass(h, reuse))) return rc;
             sa = step_args(h);
             launch_dt_scan(h->stream, sa);
             HIP_OK(hipEventSynchronize(h->ctrl_ev));
@@ -1166,14 +1393,14 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
         }
     }
     HIP_OK(hipGetLastError());
-    if (delay_on(h->prm) && n > 0) {
+    if (delay_on(h->prm) && h->n_host > 0) {
         // signal-delay history every history_stride updates (integrator.js:1549-1565)
         const int stride = h->prm.history_stride > 0 ? h->prm.history_stride : 64;
         if (++h->hist_stride_ctr >= stride) {
             h->hist_stride_ctr -= stride;
             if ((rc = hist_alloc(h)) || (rc = ev_reserve(h))) return rc;
             launch_hist_record(h->stream, hist_write(h), F(h, C_X), F(h, C_Y), F(h, C_VX), F(h, C_VY), F(h, C_ANGW),
-                               I(h, C_SLOT), h->n_events, n, h->ctrl);
+                               I(h, C_SLOT), h->n_events, h->n_host, h->ctrl);
             h->n_events++;
         }
     }
@@ -1189,7 +1416,7 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
     h->last_counters[2] = h->ctrl_host->cnt_visit;
     if ((rc = check_overflow(h))) return rc;
     if (h->ctrl_host->deaths > 0) {
-        if ((rc = compact(h))) return rc;
+        if ((rc = compact(h, h->sim_time))) return rc;
         HIP_OK(hipStreamSynchronize(h->stream));
     }
     if ((rc = dead_gc(h))) return rc;
@@ -1455,6 +1682,7 @@ int geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int6
     pa.x = F(h, C_X); pa.y = F(h, C_Y); pa.mass = F(h, C_MASS); pa.radius = F(h, C_RADIUS);
     pa.slot = I(h, C_SLOT); pa.src4 = h->src4; pa.nodes = h->tb.nodes; pa.max_nodes = 1 + 4 * h->tb.cap_divided;
     pa.count = h->tb.owned; pa.offset = h->tb.rank_start; pa.pa = nullptr; pa.pb = nullptr;
+    pa.pl = nullptr; pa.index_out = 0; pa.slack = 1.f;
     pa.dom_w = (float)h->prm.domain_w; pa.dom_h = (float)h->prm.domain_h;
     pa.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
     pa.ctrl = h->ctrl; pa.n_upper = n;
@@ -1482,6 +1710,15 @@ int geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int6
     std::sort(keys.begin(), keys.end());
     const int64_t m = std::min(cap, np);
     for (int64_t k = 0; k < m; k++) { a[k] = (int32_t)(keys[k] >> 32); b[k] = (int32_t)(keys[k] & 0xffffffff); }
+    return GEON_OK;
+}
+
+int geon_collision_events(GeonHandle h, double* out, int64_t cap, int64_t* n_events) {
+    if (!h || (cap > 0 && !out)) return GEON_EINVAL;
+    const int64_t n = (int64_t)(h->events_host.size() / GEON_EVENT_STRIDE);
+    if (n_events) *n_events = n;
+    const int64_t m = std::min(cap, n);
+    if (m > 0) memcpy(out, h->events_host.data(), sizeof(double) * GEON_EVENT_STRIDE * (size_t)m);
     return GEON_OK;
 }
 

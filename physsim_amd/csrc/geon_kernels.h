@@ -168,13 +168,62 @@ struct PairArgs {
     int64_t max_nodes;
     int32_t* count;             // count pass: pairs found per particle
     const int32_t* offset;      // write pass: exclusive scan of count
-    int32_t *pa, *pb;           // write pass: pair slots
+    int32_t *pa, *pb;           // write pass: pair slots (index_out: sorted indices)
+    int32_t* pl;                // write pass, optional: first sorted point of the candidate's leaf
+    int index_out;
+    float slack;                // box and overlap tests widened by this factor (1: exact)
     float dom_w, dom_h;
     int periodic;
     Ctrl* ctrl;
     int n_upper;
 };
 void launch_pairs(hipStream_t s, const PairArgs& a, bool write);
+
+// ---- collision mode merge (k_merge.hip) ----
+// Outcome of one pair of the fold; spawn records use the same layout.
+struct MergeRes {
+    double x, y, wx, wy, mass, base_mass, angw, e, px, py;
+    int32_t qq, anti, type, mev;    // type 0 none, 1 merge, 2 annihilation; mev: merge with lost KE
+};
+struct MergeArgs {
+    const float *x, *y, *wx, *wy, *mass, *base_mass, *radius, *angw;
+    const int32_t* qq;
+    const uint32_t* flags;          // bit 0: antimatter
+    const int32_t *p1, *p2;         // pairs as sorted particle indices
+    const uint32_t* order;          // fold position -> pair
+    int32_t* deg;                   // pairs per particle
+    double *wm, *wbm, *wr, *wdm;    // working mass, base mass, radius, death mass (FP64)
+    MergeRes *res, *spawns;
+    int32_t *f_spawn, *f_mev, *f_aev, *o_spawn, *o_mev, *o_aev;
+    int32_t* cnt;                   // totals: spawns, merge events, annihilation events, deaths
+    int32_t* scan_tmp;
+    double* events;                 // 6 doubles each: type, x, y, energy, px, py
+    Ctrl* mc;                       // n_alive = number of pairs
+    const Ctrl* ctrl;               // n_alive = number of particles
+    int periodic;
+    double dom_w, dom_h;
+};
+struct SpawnArgs {
+    const MergeRes* spawns;
+    int count, base, first_slot, periodic, rel, n_force;
+    double dom_w, dom_h;
+    float *x, *y, *wx, *wy, *vx, *vy, *mass, *base_mass, *charge, *angw, *angvel, *radius;
+    float *mu, *L, *f1ox, *f1oy, *ax, *yuk, *higgs;
+    int32_t *qq, *slot;
+    uint32_t* flags;
+    float* force[32];
+};
+void merge_pair_same_leaf(hipStream_t s, const int32_t* p1, const int32_t* leaf, const Ctrl* mc, int32_t* need, int np);
+void merge_leaf_ranks(hipStream_t s, const TreeBufs& tb, const int32_t* slot, int n_nodes, int32_t* s5,
+                      const int32_t* p1, const int32_t* leaf, const Ctrl* mc, int32_t* rank, int np);
+void merge_pair_keys(hipStream_t s, const int32_t* p1, const int32_t* p2, const int32_t* leaf, const int32_t* slot,
+                     const int32_t* rank, const Ctrl* mc, const Ctrl* ctrl, uint64_t* keys, uint32_t* perm, int np);
+void merge_resolve(hipStream_t s, const MergeArgs& m, int np, int n);
+void merge_apply(hipStream_t s, const MergeArgs& m, float* mass, float* base_mass, float* radius, int32_t* alive,
+                 int n);
+void merge_spawn(hipStream_t s, const SpawnArgs& a);
+void merge_excite(hipStream_t s, float* field_dot, int G, double W, double H, const double* events, int n_events,
+                  double frac);
 
 // ---- learned far-field correction (k_learn.hip) ----
 constexpr int kFarIn = 10;          // features of a pair

@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
     }
     const uint64_t my_bit = 1ull << lane;
     const float px = valid ? a.x[i] : 0.f, py = valid ? a.y[i] : 0.f;
-    const float r1 = valid ? a.radius[i] : 0.f;
+    const float r1 = valid ? a.radius[i] * a.slack : 0.f;
     const int slot1 = valid ? a.slot[i] : 0;
     const float rw = 2.f * r1;
     int found = 0;
@@ -486,9 +486,14 @@ __global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
                     if (j == i || s4.z == 0.f || slot1 >= slot2) continue;
                     float dx = s4.x - px, dy = s4.y - py;
                     if (a.periodic) { dx = torus_wrap(dx, a.dom_w, 0.5f * a.dom_w); dy = torus_wrap(dy, a.dom_h, 0.5f * a.dom_h); }
-                    const float md = r1 + a.radius[j];
+                    const float md = r1 + a.radius[j] * a.slack;
                     if (dx * dx + dy * dy < md * md) {
-                        if (WRITE) { a.pa[out] = slot1; a.pb[out] = slot2; out++; }
+                        if (WRITE) {
+                            a.pa[out] = a.index_out ? i : slot1;
+                            a.pb[out] = a.index_out ? j : slot2;
+                            if (a.pl) a.pl[out] = nd.child;
+                            out++;
+                        }
                         found++;
                     }
                 }

@@ -119,25 +119,30 @@ class HipPhysics:
     def patchMass(self, idx, mass, base_mass):              # gpu-physics.js:1955
         self._check(self.lib.geon_patch_mass(self.h, int(idx), float(mass), float(base_mass)))
 
-    def upload(self, x, y, wx, wy, mass, charge, angw=None, base_mass=None):
+    def upload(self, x, y, wx, wy, mass, charge, angw=None, base_mass=None, antimatter=None):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
                 for a in (x, y, wx, wy, mass, base_mass, charge, angw)]
         n = len(arrs[0])
-        v = A.GeonSoAView(*[_dptr(a) for a in arrs], None)
+        anti = np.ascontiguousarray(antimatter, dtype=np.int32) if antimatter is not None else None
+        v = A.GeonSoAView(*[_dptr(a) for a in arrs], None,
+                          anti.ctypes.data_as(C.POINTER(C.c_int32)) if anti is not None else None)
         self._check(self.lib.geon_upload_soa(self.h, C.byref(v), n))
-        self._keep = arrs
+        self._keep = (arrs, anti)
 
     def download(self):
         ns = C.c_int32()
         cap = self.max_particles
         out = {k: np.zeros(cap) for k in ("x", "y", "wx", "wy", "mass", "base_mass", "charge", "angw")}
         alive = np.zeros(cap, dtype=np.int32)
+        anti = np.zeros(cap, dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
         v = A.GeonSoAView(*[_dptr(out[k]) for k in ("x", "y", "wx", "wy", "mass", "base_mass", "charge", "angw")],
-                          alive.ctypes.data_as(C.POINTER(C.c_int32)))
+                          alive.ctypes.data_as(ip), anti.ctypes.data_as(ip))
         self._check(self.lib.geon_download_soa(self.h, C.byref(v), C.byref(ns)))
         n = ns.value
         res = {k: a[:n] for k, a in out.items()}
         res["alive"] = alive[:n].astype(bool)
+        res["antimatter"] = anti[:n].astype(bool)
         return res
 
     def getParticleCount(self):                             # gpu-physics.js:4287
@@ -240,6 +245,16 @@ class HipPhysics:
         self._check(self.lib.geon_collision_pairs(self.h, a.ctypes.data_as(ip), b.ctypes.data_as(ip), n.value,
                                                   C.byref(n)))
         return a[:n.value], b[:n.value]
+
+    def collision_events(self):
+        """Events of the last update in merge mode: rows {type (1 merge, 2
+        annihilation), x, y, energy, px, py}, merges first (collisions.js:62-92)."""
+        n = C.c_int64()
+        self._check(self.lib.geon_collision_events(self.h, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 6))
+        self._check(self.lib.geon_collision_events(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), n.value,
+                                                   C.byref(n)))
+        return out[:n.value]
 
     # ---- learned far-field correction ------------------------------------
     def far_collect(self, group_stride=1):

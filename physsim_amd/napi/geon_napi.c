@@ -206,7 +206,23 @@ static double* f64_of(napi_env env, napi_value obj, const char* key, size_t* len
     return type == napi_float64_array ? (double*)data : NULL;
 }
 
-/* uploadSoA(h, {x, y, wx, wy, mass, charge, angw?, baseMass?} Float64Arrays) */
+static int32_t* i32_of(napi_env env, napi_value obj, const char* key, size_t* len) {
+    bool has = false;
+    if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return NULL;
+    napi_value v;
+    napi_get_named_property(env, obj, key, &v);
+    bool is_ta = false;
+    napi_is_typedarray(env, v, &is_ta);
+    if (!is_ta) return NULL;
+    napi_typedarray_type type;
+    void* data;
+    napi_value ab;
+    size_t off;
+    napi_get_typedarray_info(env, v, &type, len, &data, &ab, &off);
+    return type == napi_int32_array ? (int32_t*)data : NULL;
+}
+
+/* uploadSoA(h, {x, y, wx, wy, mass, charge, angw?, baseMass?} Float64Arrays, antimatter? Int32Array) */
 static napi_value js_upload(napi_env env, napi_callback_info info) {
     napi_value argv[2];
     if (!args_n(env, info, 2, argv)) return NULL;
@@ -222,6 +238,7 @@ static napi_value js_upload(napi_env env, napi_callback_info info) {
     v.charge = f64_of(env, argv[1], "charge", &k);
     v.angw = f64_of(env, argv[1], "angw", &k);
     v.base_mass = f64_of(env, argv[1], "baseMass", &k);
+    v.antimatter = i32_of(env, argv[1], "antimatter", &k);
     int rc = geon_upload_soa(h, &v, (int32_t)n);
     if (rc) return throw_geon(env, h, rc);
     return NULL;
@@ -244,6 +261,7 @@ static napi_value js_download(napi_env env, napi_callback_info info) {
     napi_value out;
     napi_create_object(env, &out);
     GeonSoAView v;
+    memset(&v, 0, sizeof(v));
     const char* names[] = {"x", "y", "wx", "wy", "mass", "baseMass", "charge", "angw"};
     double** ptrs[] = {&v.x, &v.y, &v.wx, &v.wy, &v.mass, &v.base_mass, &v.charge, &v.angw};
     for (int i = 0; i < 8; i++) napi_set_named_property(env, out, names[i], new_f64(env, (size_t)cap, ptrs[i]));
@@ -251,6 +269,9 @@ static napi_value js_download(napi_env env, napi_callback_info info) {
     napi_create_arraybuffer(env, (size_t)cap * 4, (void**)&v.alive, &ab);
     napi_create_typedarray(env, napi_int32_array, (size_t)cap, ab, 0, &ta);
     napi_set_named_property(env, out, "alive", ta);
+    napi_create_arraybuffer(env, (size_t)cap * 4, (void**)&v.antimatter, &ab);
+    napi_create_typedarray(env, napi_int32_array, (size_t)cap, ab, 0, &ta);
+    napi_set_named_property(env, out, "antimatter", ta);
     int32_t ns = 0;
     int rc = geon_download_soa(h, &v, &ns);
     if (rc) return throw_geon(env, h, rc);
