@@ -13,6 +13,7 @@ import os
 import numpy as np
 
 from . import _abi as A
+from . import state_v1
 
 _LIB = None
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgeon_hip.so")
@@ -316,24 +317,34 @@ class HipPhysics:
         self._check(self.lib.geon_kernel_timing(self.h, int(kernel), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
-    # ---- save/load: the reference's v1 JSON (save-load.js:89-127) ----------
-    def serialize(self):
+    # ---- save/load: the reference's v1 state (save-load.js:89-262) ---------
+    def serialize(self, sim=None):
+        """GPUPhysics.serialize (gpu-physics.js:3749-3836): the v1 state of the
+        live particles in slot order, toggles, parameters and modes.  `sim`
+        may carry the app's speedScale, friction and camera."""
         d = self.download()
-        particles = []
-        for i in np.nonzero(d["alive"])[0]:
-            particles.append({"x": d["x"][i], "y": d["y"][i], "wx": d["wx"][i], "wy": d["wy"][i],
-                              "mass": d["mass"][i], "baseMass": d["base_mass"][i], "charge": d["charge"][i],
-                              "angw": d["angw"][i], "antimatter": False})
-        return json.dumps({"version": 1, "particles": particles})
+        particles = [{"x": float(d["x"][i]), "y": float(d["y"][i]), "wx": float(d["wx"][i]),
+                      "wy": float(d["wy"][i]), "mass": float(d["mass"][i]), "baseMass": float(d["base_mass"][i]),
+                      "charge": float(d["charge"][i]), "angw": float(d["angw"][i]),
+                      "antimatter": bool(d["antimatter"][i])}
+                     for i in np.nonzero(d["alive"])[0]]
+        return state_v1.params_to_state(self.params, particles, sim)
 
     def deserialize(self, state):
-        s = json.loads(state) if isinstance(state, str) else state
-        if s.get("version") != 1:                           # save-load.js:130
-            raise ValueError("unsupported state version")
-        ps = s["particles"]
-        col = lambda k, d=0.0: np.array([p.get(k, d) for p in ps], dtype=np.float64)
-        self.upload(col("x"), col("y"), col("wx"), col("wy"), col("mass", 1.0), col("charge"), col("angw"),
-                    col("baseMass") if ps and "baseMass" in ps[0] else None)
+        """GPUPhysics.deserialize (gpu-physics.js:3886-3960) with loadState's
+        restore of toggles, parameters and modes (save-load.js:129-190):
+        False for anything but version 1.  Particles are renumbered 0..n-1 and
+        taken as always existing (no signal-delay history)."""
+        if isinstance(state, (str, bytes)):
+            state = json.loads(state)
+        if not state or state.get("version") != 1:
+            return False
+        self.set_params(state_v1.apply_state(self.params, state))
+        c = state_v1.particles_from_state(state, self.max_particles)
+        arr = lambda k: np.asarray(c[k], dtype=np.float64)
+        self.upload(arr("x"), arr("y"), arr("wx"), arr("wy"), arr("mass"), arr("charge"), arr("angw"),
+                    arr("base_mass"), antimatter=np.asarray(c["antimatter"], dtype=np.int32))
+        return True
 
 
 class HipBatch:

@@ -8,17 +8,13 @@ import { createRequire } from 'module';
 import { fileURLToPath } from 'url';
 import path from 'path';
 
+import { TOGGLE_BITS, paramsToState, applyState, particlesFromState } from './state-v1.mjs';
+
 const require = createRequire(import.meta.url);
 const here = path.dirname(fileURLToPath(import.meta.url));
 const addon = require(path.join(here, '..', 'napi', 'geon_napi.node'));
 
-// toggles0 bits: src/gpu/gpu-constants.js:196-213
-export const TOGGLE_BITS = {
-    gravityEnabled: 1, coulombEnabled: 2, magneticEnabled: 4, gravitomagEnabled: 8,
-    onePNEnabled: 16, relativityEnabled: 32, spinOrbitEnabled: 64, radiationEnabled: 128,
-    blackHoleEnabled: 256, disintegrationEnabled: 512, expansionEnabled: 1024, yukawaEnabled: 2048,
-    higgsEnabled: 4096, axionEnabled: 8192, barnesHutEnabled: 16384, bosonInterEnabled: 32768,
-};
+export { TOGGLE_BITS };
 
 export const BACKEND_HIP = 'hip';
 
@@ -123,25 +119,43 @@ export default class HipPhysics {
 
     readStats() { return this._stats; }
 
-    /** serialize(): the reference's v1 JSON particle list (save-load.js:89-127). */
-    async serialize() {
+    /** Events of the last update in collision mode merge (collisions.js:62-92):
+     *  [{type: 'merge'|'annihilation', x, y, energy, px, py}], merges first.
+     *  A host that keeps photons emits the annihilation bursts from these
+     *  (integrator.js:1388-1392). */
+    getCollisionEvents() {
+        const a = addon.collisionEvents(this._h);
+        const out = [];
+        for (let k = 0; k + 6 <= a.length; k += 6)
+            out.push({ type: a[k] === 1 ? 'merge' : 'annihilation', x: a[k + 1], y: a[k + 2], energy: a[k + 3],
+                       px: a[k + 4], py: a[k + 5] });
+        return out;
+    }
+
+    /** serialize(sim): the v1 state (gpu-physics.js:3749-3836): live particles
+     *  in slot order, toggles, parameters and modes; `sim` may carry the app's
+     *  speedScale, friction and camera. */
+    async serialize(sim = {}) {
         const d = this.downloadSoA();
         const particles = [];
         for (let i = 0; i < d.nSlots; i++) {
             if (!d.alive[i]) continue;
             particles.push({ x: d.x[i], y: d.y[i], wx: d.wx[i], wy: d.wy[i], mass: d.mass[i],
-                             baseMass: d.baseMass[i], charge: d.charge[i], angw: d.angw[i], antimatter: false });
+                             baseMass: d.baseMass[i], charge: d.charge[i], angw: d.angw[i],
+                             antimatter: !!d.antimatter[i] });
         }
-        return { version: 1, particles };
+        return paramsToState(this._params, particles, sim);
     }
 
-    /** deserialize(state): requires version 1 (save-load.js:130). */
+    /** deserialize(state): GPUPhysics.deserialize (gpu-physics.js:3886-3960)
+     *  with loadState's restore of toggles, parameters and modes
+     *  (save-load.js:129-190).  False for anything but version 1. */
     deserialize(state) {
-        if (!state || state.version !== 1) throw new Error('unsupported state version');
-        const ps = state.particles;
-        const col = (k, d = 0) => Float64Array.from(ps, p => (p[k] !== undefined ? p[k] : d));
-        this.uploadSoA({ x: col('x'), y: col('y'), wx: col('wx'), wy: col('wy'), mass: col('mass', 1),
-                         baseMass: col('baseMass', 1), charge: col('charge'), angw: col('angw') });
+        if (!state || state.version !== 1) return false;
+        this._params = applyState(this._params, state);
+        addon.setParams(this._h, this._params);
+        this.uploadSoA(particlesFromState(state, this._max));
+        return true;
     }
 
     destroy() {

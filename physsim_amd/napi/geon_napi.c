@@ -332,6 +332,21 @@ static napi_value js_particle_count(napi_env env, napi_callback_info info) {
     return i32(env, n);
 }
 
+/* collisionEvents(h) -> Float64Array of GEON_EVENT_STRIDE doubles per event
+ * {type, x, y, energy, px, py} of the last update (collisions.js:62-92) */
+static napi_value js_collision_events(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    if (!args_n(env, info, 1, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    int64_t n = 0;
+    int rc = geon_collision_events(h, NULL, 0, &n);
+    if (rc) return throw_geon(env, h, rc);
+    double* data;
+    napi_value ta = new_f64(env, (size_t)n * GEON_EVENT_STRIDE, &data);
+    if (n > 0 && (rc = geon_collision_events(h, data, n, &n))) return throw_geon(env, h, rc);
+    return ta;
+}
+
 /* getStats(h) -> readStats-shaped object (gpu-physics.js:4215-4281) */
 static napi_value js_get_stats(napi_env env, napi_callback_info info) {
     napi_value argv[1];
@@ -467,6 +482,7 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("removeParticle", js_remove_particle);
     EXPORT("patchMass", js_patch_mass);
     EXPORT("uploadSoA", js_upload);
+    EXPORT("collisionEvents", js_collision_events);
     EXPORT("downloadSoA", js_download);
     EXPORT("update", js_update);
     EXPORT("computeForces", js_compute_forces);
