@@ -169,6 +169,9 @@ typedef struct GeonStats {
     double sim_time;
     int32_t alive_count;
     int32_t dead_count;         /* retired particles still acting through signal delay */
+    /* sim.totalRadiated, totalRadiatedPx/Py (radiation on): energy and momentum
+     * taken from the particles by radiation reaction since the last upload */
+    double radiated_energy, radiated_px, radiated_py;
 } GeonStats;
 
 /* Work counters of the last force pass, for roofline accounting. */

@@ -74,6 +74,7 @@ def lib():
         L.orc_download_ex.argtypes = [C.c_void_p, _DP, _DP, _IP, _DP]
         L.orc_leaf_points.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), _IP, C.c_int64]
         L.orc_leaf_points.restype = C.c_int64
+        L.orc_get_jerk.argtypes = [C.c_void_p, _DP, _DP]
         _LIB = L
     return _LIB
 
@@ -210,6 +211,12 @@ class Oracle:
         ids = np.zeros(max(m, 1), dtype=np.int32)
         self.L.orc_leaf_points(self.s, key.ctypes.data_as(C.POINTER(C.c_uint64)), ids.ctypes.data_as(_IP), m)
         return key[:m], ids[:m]
+
+    def jerk(self):
+        n = self.count()
+        jx, jy = np.zeros(n), np.zeros(n)
+        self.L.orc_get_jerk(self.s, _d(jx), _d(jy))
+        return jx, jy
 
     def download_ex(self):
         n = self.count()

@@ -84,6 +84,8 @@ typedef struct {
     double Bz, dBzdx, dBzdy, Bgz, dBgzdx, dBgzdy;
     double frameDrag, tidal;
     double f1x, f1y, f1ox, f1oy;
+    double jx, jy;                 /* analytic jerk dF/dt (radiation on, forces.js:198-402) */
+    double radAccum, quadAccum, emQuadAccum;   /* _radAccum, _quadAccum, _emQuadAccum */
     double pe;                     /* this particle's share of the PE accumulator */
     int32_t id;                    /* upload index */
     int32_t q_quanta;              /* round(charge / 0.1) */
@@ -116,7 +118,8 @@ struct OrcSys {
     double potentialEnergy;
     int64_t cnt_pair, cnt_node, cnt_visit;
     /* derived toggles (integrator.js:138-150) */
-    int grav, coul, mag, gm, onePN, rel, spinOrbit, bh, yuk, higgs, axion, barnesHut, expansion;
+    int grav, coul, mag, gm, onePN, rel, spinOrbit, bh, yuk, higgs, axion, barnesHut, expansion, rad;
+    double totalRadiated, totalRadiatedPx, totalRadiatedPy;   /* sim.totalRadiated* (main.js) */
     double softeningSq, softening, yukCutoffSq;
     int needAxMod, periodic;
     OField higgsF, axionF;
@@ -216,8 +219,9 @@ static inline double angwToAngVel(double angw, double radius) {           /* rel
 /* ------------------------------------------------------------------ */
 /* forces.js: pairForce                                                */
 /* ------------------------------------------------------------------ */
-/* Observer p gets the force of a source.  Restates forces.js:163-442 without
- * the radiation jerk (radiationEnabled). */
+/* Observer p gets the force of a source.  Restates forces.js:163-442; with
+ * radiation on it also accumulates the analytic jerk dF/dt of each term
+ * (forces.js:198-402), the input of the radiation reaction. */
 static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, double svy,
                       double sMass, double sCharge, double sAngVel, double sMagMoment,
                       double sAngMomentum, double sAxMod, double sYukMod, double sHiggsMod,
@@ -246,11 +250,18 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
     const double invR5a = signalDelayed ? invR5 * aberr : invR5;
     const double crossSV = svx * ry - svy * rx;
     const double pMagMoment = p->magMoment, pAngMomentum = p->angMomentum;
+    const int rad = s->rad;
+    double vrx = 0, vry = 0, rDotVr = 0;                                  /* forces.js:198-203 */
+    if (rad) { vrx = svx - p->vx; vry = svy - p->vy; rDotVr = rx * vrx + ry * vry; }
 
     if (s->grav) {                                                        /* forces.js:213-227 */
         const double k = p->mass * sMass;
         const double fDir = k * invR3a;
         p->fx += rx * fDir; p->fy += ry * fDir;
+        if (rad) {
+            const double jRadial = -3 * k * rDotVr * invR5a;
+            p->jx += vrx * fDir + rx * jRadial; p->jy += vry * fDir + ry * jRadial;
+        }
         if (accumulatePE) p->pe -= k * invR;
     }
     const double axModPair = s->needAxMod ? sqrt(p->axMod * sAxMod) : 1;  /* forces.js:230 */
@@ -258,6 +269,10 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
         const double k = -(p->charge * sCharge) * axModPair;
         const double fDir = k * invR3a;
         p->fx += rx * fDir; p->fy += ry * fDir;
+        if (rad) {
+            const double jRadial = -3 * k * rDotVr * invR5a;
+            p->jx += vrx * fDir + rx * jRadial; p->jy += vry * fDir + ry * jRadial;
+        }
         if (accumulatePE) p->pe += p->charge * sCharge * invR * axModPair;
     }
     if (s->onePN && (s->gm || s->mag)) {                                  /* forces.js:249-301 */
@@ -279,6 +294,12 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
             const double fy = base * (ry * radial + (pvy * v1Coeff + svy * v2Coeff) * r);
             p->fx += fx; p->fy += fy;
             p->f1x += fx; p->f1y += fy;
+            if (rad) {                      /* position-only EIH term m1 m2 (5 m1 + 4 m2) r / r^4 */
+                const double kEIH = p->mass * sMass * (5 * p->mass + 4 * sMass);
+                const double fDirEIH = kEIH * invRSq * invRSq;
+                const double jRadialEIH = -4 * kEIH * rDotVr * invRSq * invRSq * invRSq;
+                p->jx += vrx * fDirEIH + rx * jRadialEIH; p->jy += vry * fDirEIH + ry * jRadialEIH;
+            }
             if (accumulatePE)
                 p->pe -= p->mass * sMass * invR * (1.5 * (v1Sq + v2Sq) - 3.5 * v1DotV2
                           - 0.5 * nDotV1 * nDotV2 + p->mass * invR + sMass * invR);
@@ -300,12 +321,21 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
         const double fDir = crossCoeff * invRSq * invRSq;
         p->fx += rx * fDir; p->fy += ry * fDir;
         p->f1x += rx * fDir; p->f1y += ry * fDir;
+        if (rad) {
+            const double jRadial = -4 * crossCoeff * rDotVr * invRSq * invRSq * invRSq;
+            p->jx += vrx * fDir + rx * jRadial; p->jy += vry * fDir + ry * jRadial;
+        }
         if (accumulatePE) p->pe += 0.5 * crossCoeff * invRSq;
     }
     if (s->mag) {                                                         /* forces.js:322-353 */
         const double axMod = axModPair;
         const double fDir = -3 * (pMagMoment * sMagMoment) * invR5a * axMod;
         p->fx += rx * fDir; p->fy += ry * fDir;
+        if (rad) {
+            const double invR7a = invR5a * invRSq;
+            const double jRadial = 5 * 3 * (pMagMoment * sMagMoment) * rDotVr * invR7a * axMod;
+            p->jx += vrx * fDir + rx * jRadial; p->jy += vry * fDir + ry * jRadial;
+        }
         if (accumulatePE) p->pe += (pMagMoment * sMagMoment) * invR3 * axMod;
         const double BzMoving = sCharge * crossSV * invR3 * axMod;
         p->Bz += BzMoving;
@@ -318,6 +348,11 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
     if (s->gm) {                                                          /* forces.js:355-387 */
         const double fDir = 3 * (pAngMomentum * sAngMomentum) * invR5a;
         p->fx += rx * fDir; p->fy += ry * fDir;
+        if (rad) {
+            const double invR7a = invR5a * invRSq;
+            const double jRadial = -5 * 3 * (pAngMomentum * sAngMomentum) * rDotVr * invR7a;
+            p->jx += vrx * fDir + rx * jRadial; p->jy += vry * fDir + ry * jRadial;
+        }
         if (accumulatePE) p->pe -= (pAngMomentum * sAngMomentum) * invR3;
         const double BgzMoving = -sMass * crossSV * invR3;
         p->Bgz += BgzMoving;
@@ -336,6 +371,11 @@ static void pairForce(const OrcSys* s, P* p, double sx, double sy, double svx, d
         const double G = g->yukawa_coupling;
         const double fDir = G * ym * p->mass * sMass * expMuR * (invRSq + mu * invR) * (signalDelayed ? invR * aberr : invR);
         p->fx += rx * fDir; p->fy += ry * fDir;
+        if (rad) {
+            const double jRadial = -(3 * invRSq + 3 * mu * invR + mu * mu) * rDotVr * G * ym * p->mass * sMass * expMuR
+                                   * invRSq * (signalDelayed ? invR * aberr : invR);
+            p->jx += vrx * fDir + rx * jRadial; p->jy += vry * fDir + ry * jRadial;
+        }
         if (accumulatePE) p->pe -= G * ym * p->mass * sMass * expMuR * invR;
         if (s->onePN) {
             const double pvx = p->vx, pvy = p->vy;
@@ -1048,6 +1088,7 @@ static void resetForces(OrcSys* s) {                                      /* for
         p->Bz = p->dBzdx = p->dBzdy = 0;
         p->Bgz = p->dBgzdx = p->dBgzdy = 0;
         p->frameDrag = p->tidal = 0;
+        p->jx = p->jy = 0;
         p->pe = 0;
     }
 }
@@ -1249,6 +1290,7 @@ static void syncToggles(OrcSys* s) {                                      /* int
     s->spinOrbit = !!(t & GEON_SPIN_ORBIT); s->bh = !!(t & GEON_BLACK_HOLE);
     s->yuk = !!(t & GEON_YUKAWA); s->higgs = !!(t & GEON_HIGGS); s->axion = !!(t & GEON_AXION);
     s->barnesHut = !!(t & GEON_BARNES_HUT); s->expansion = !!(t & GEON_EXPANSION);
+    s->rad = !!(t & GEON_RADIATION);
     s->softening = s->bh ? s->prm.bh_softening : s->prm.softening;
     s->softeningSq = s->softening * s->softening;
     s->needAxMod = (s->coul || s->mag) && s->axion;                       /* forces.js:79 */
@@ -1323,6 +1365,7 @@ int orc_upload(OrcSys* s, int32_t n, const double* x, const double* y, const dou
     }
     s->next_id = n;
     s->nev = 0;
+    s->totalRadiated = s->totalRadiatedPx = s->totalRadiatedPy = 0;
     s->forcesInit = 0;
     s->simTime = 0;
     s->root = -1;
@@ -1422,6 +1465,7 @@ static void cyclotronScan(OrcSys* s, double* maxCyc) {                    /* int
 
 /* Physics.update (integrator.js:726-1825), the parts on the hot path. */
 #include "geon_oracle_merge.inc"
+#include "geon_oracle_rad.inc"
 
 int orc_update(OrcSys* s, double dt, int32_t* substeps) {
     syncToggles(s);
@@ -1534,6 +1578,7 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
                 }
             }
         }
+        if (s->rad && s->coul) llRadiation(s, dtSub);                    /* integrator.js:983-1086 */
         for (int i = 0; i < n; i++) {                                     /* integrator.js:1270-1278 */
             P* p = &s->p[i];
             const double invG = relOn ? 1 / sqrt(1 + p->wx * p->wx + p->wy * p->wy) : 1;
@@ -1596,6 +1641,7 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
             for (int i = 0; i < n; i++) hist_push(&s->p[i], s->simTime);
         }
     }
+    if (s->rad && n >= 2 && (s->grav || s->coul)) quadRadiation(s, dt);   /* integrator.js:1567-1740 */
     s->potentialEnergy = 0;
     for (int i = 0; i < n; i++) s->potentialEnergy += s->p[i].pe;
     s->potentialEnergy *= 0.5;                                            /* integrator.js:1743 */
@@ -1693,6 +1739,9 @@ int orc_delayed_state(OrcSys* s, int32_t i, double ox, double oy, double* out5) 
  * force pass (integrator.js:1743). */
 int orc_stats(OrcSys* s, GeonStats* out) {
     memset(out, 0, sizeof(*out));
+    out->radiated_energy = s->totalRadiated;
+    out->radiated_px = s->totalRadiatedPx;
+    out->radiated_py = s->totalRadiatedPy;
     const int n = s->n, relOn = s->rel;
     double linearKE = 0, spinKE = 0, totalMass = 0, comX = 0, comY = 0, px = 0, py = 0;
     for (int i = 0; i < n; i++) {

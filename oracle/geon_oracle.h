@@ -66,6 +66,8 @@ int     orc_set_antimatter(OrcSys* s, const int32_t* flags);
 int     orc_download_ex(OrcSys* s, double* base_mass, double* charge, int32_t* antimatter, double* radius);
 /* Leaf contents in stored order: rows (level << 56 | path, id). */
 int64_t orc_leaf_points(OrcSys* s, uint64_t* key, int32_t* id, int64_t cap);
+/* Analytic jerk dF/dt of the last force pass (radiation on), array order. */
+int     orc_get_jerk(OrcSys* s, double* jx, double* jy);
 
 /* Stand-alone FP64 direct sum of gravity+Coulomb+... (forces.js:93-126) over
  * `n` sources for the targets listed in `idx` (all if idx == NULL).  Uses the

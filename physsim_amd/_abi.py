@@ -71,7 +71,8 @@ class GeonStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "linear_ke", "spin_ke", "px", "py", "orbital_ang_mom", "spin_ang_mom", "com_x", "com_y",
         "total_mass", "pe", "field_energy", "higgs_field_energy", "axion_field_energy", "pfi_energy",
-        "field_px", "field_py", "sim_time")] + [("alive_count", C.c_int32), ("dead_count", C.c_int32)]
+        "field_px", "field_py", "sim_time")] + [("alive_count", C.c_int32), ("dead_count", C.c_int32)] + [
+        (n, C.c_double) for n in ("radiated_energy", "radiated_px", "radiated_py")]
 
 
 class GeonHistoryView(C.Structure):

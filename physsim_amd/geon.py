@@ -187,6 +187,7 @@ class HipPhysics:
             "fieldPx": st.field_px, "fieldPy": st.field_py,
             "higgsFieldEnergy": st.higgs_field_energy, "axionFieldEnergy": st.axion_field_energy,
             "pfiEnergy": st.pfi_energy, "simTime": st.sim_time, "deadCount": st.dead_count,
+            "totalRadiated": st.radiated_energy, "totalRadiatedPx": st.radiated_px, "totalRadiatedPy": st.radiated_py,
         }
 
     def stats(self):

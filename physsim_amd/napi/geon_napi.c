@@ -376,6 +376,9 @@ static napi_value js_get_stats(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, o, "simTime", num(env, s.sim_time));
     napi_set_named_property(env, o, "aliveCount", i32(env, s.alive_count));
     napi_set_named_property(env, o, "deadCount", i32(env, s.dead_count));
+    napi_set_named_property(env, o, "totalRadiated", num(env, s.radiated_energy));
+    napi_set_named_property(env, o, "totalRadiatedPx", num(env, s.radiated_px));
+    napi_set_named_property(env, o, "totalRadiatedPy", num(env, s.radiated_py));
     return o;
 }
 
