@@ -157,6 +157,7 @@ typedef struct GeonForceView {
     double *frame_drag_torque, *tidal_torque;
     double *pe;                 /* per-particle PE contribution (sum over sources) */
     double *f1pn_x, *f1pn_y;    /* force1PN after the last 1PN pass */
+    double *jerk_x, *jerk_y;    /* analytic dF/dt (radiation on, forces.js:198-402) */
 } GeonForceView;
 
 /* FP64 conservation readout: same fields as GPUPhysics.readStats

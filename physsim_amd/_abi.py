@@ -64,7 +64,8 @@ class GeonSoAView(C.Structure):
 class GeonForceView(C.Structure):
     _fields_ = [("fx", _DP), ("fy", _DP), ("bz", _DP), ("dbzdx", _DP), ("dbzdy", _DP),
                 ("bgz", _DP), ("dbgzdx", _DP), ("dbgzdy", _DP), ("frame_drag_torque", _DP),
-                ("tidal_torque", _DP), ("pe", _DP), ("f1pn_x", _DP), ("f1pn_y", _DP)]
+                ("tidal_torque", _DP), ("pe", _DP), ("f1pn_x", _DP), ("f1pn_y", _DP),
+                ("jerk_x", _DP), ("jerk_y", _DP)]
 
 
 class GeonStats(C.Structure):

@@ -28,7 +28,7 @@ enum StateCol {
 // Force columns: results of the last force pass, kept between updates.
 enum ForceCol {
     K_FX, K_FY, K_PE, K_BZ, K_DBZDX, K_DBZDY, K_BGZ, K_DBGZDX, K_DBGZDY, K_FDRAG, K_TIDAL, K_F1X, K_F1Y,
-    K_PEG, K_PFI, N_FORCE
+    K_PEG, K_PFI, K_JX, K_JY, N_FORCE
 };
 
 // Device state of one scalar field (scalar-field.js:13-64).
@@ -124,6 +124,9 @@ struct GeonContext {
     int32_t* mg_cnt = nullptr;          // 8 ints: totals, deaths, same-leaf flag
     Ctrl* mg_ctrl = nullptr;            // n_alive = pairs of the current fold
     std::vector<double> events_host;    // events of the last update, 6 doubles each
+    // radiation: per-particle FP64 scratch (9 columns), block sums, quadrupole
+    // sums and the radiated energy / momentum since the last upload
+    double *rad_scr = nullptr, *rad_partial = nullptr, *rad_sums = nullptr, *rad_tot = nullptr;
 };
 
 #define HIP_OK(expr)                                                                         \
@@ -176,8 +179,12 @@ struct Timer {
 
 uint32_t validate(const GeonParams* p, std::string* why) {
     const uint32_t t = p->toggles0;
-    const uint32_t bad = GEON_RADIATION | GEON_DISINTEGRATION | GEON_BOSON_INTER | GEON_BLACK_HOLE;
-    if (t & bad) { *why = "toggle not run by this backend (radiation, disintegration, bosons, black hole)"; return 1; }
+    const uint32_t bad = GEON_DISINTEGRATION | GEON_BOSON_INTER | GEON_BLACK_HOLE;
+    if (t & bad) { *why = "toggle not run by this backend (disintegration, bosons, black hole)"; return 1; }
+    if ((t & GEON_RADIATION) && (t & GEON_YUKAWA)) {
+        *why = "radiation with Yukawa is not run (its scalar radiation emits pions of random charge)"; return 1;
+    }
+    if ((t & GEON_RADIATION) && p->far_field != GEON_FAR_NONE) { *why = "radiation runs with monopole nodes only (far_field NONE)"; return 1; }
     const int G = p->field_grid;
     if ((t & (GEON_HIGGS | GEON_AXION)) && (G < 8 || G > 128 || (G & (G - 1)))) {
         *why = "field_grid must be a power of two in [8, 128]"; return 1;
@@ -231,6 +238,8 @@ PhysFlags phys_flags(const GeonParams& p) {
     f.periodic = p.boundary_mode == GEON_BOUND_LOOP;
     f.window = f.periodic && p.periodic_mode == GEON_PERIODIC_WINDOWED;
     f.topology = f.periodic ? (uint32_t)p.topology : 0u;
+    f.grav_on = !!(t & GEON_GRAVITY);
+    f.coul_on = !!(t & GEON_COULOMB);
     return f;
 }
 
@@ -240,6 +249,7 @@ uint32_t feature_mask(const GeonParams& p) {
     if (t & GEON_GRAVITY) f |= F_GRAV;
     if (t & GEON_COULOMB) f |= F_COUL;
     if (t & (GEON_MAGNETIC | GEON_GRAVITOMAG | GEON_YUKAWA)) f |= F_EXTRA;
+    if (t & GEON_RADIATION) f = F_GRAV | F_COUL | F_EXTRA | F_JERK;   // forces runtime-checked, plus the jerk
     return f;
 }
 
@@ -279,6 +289,10 @@ StepArgs step_args(GeonContext* h) {
     a.cx = (float)(h->prm.domain_w * 0.5);
     a.cy = (float)(h->prm.domain_h * 0.5);
     a.n_upper = h->n_host;
+    a.rad = (t & GEON_RADIATION) && (t & GEON_COULOMB) && h->rad_scr;
+    a.jx = h->fc[K_JX]; a.jy = h->fc[K_JY];
+    a.rad_e = h->rad_scr;
+    a.rad_stride = h->cap;
     return a;
 }
 
@@ -291,6 +305,7 @@ ForceArgs force_args(GeonContext* h) {
     a.bz = h->fc[K_BZ]; a.dbzdx = h->fc[K_DBZDX]; a.dbzdy = h->fc[K_DBZDY];
     a.bgz = h->fc[K_BGZ]; a.dbgzdx = h->fc[K_DBGZDX]; a.dbgzdy = h->fc[K_DBGZDY];
     a.frame_drag = h->fc[K_FDRAG]; a.tidal = h->fc[K_TIDAL]; a.f1x = h->fc[K_F1X]; a.f1y = h->fc[K_F1Y];
+    a.jx = h->fc[K_JX]; a.jy = h->fc[K_JY];
     a.mass_for_cyc = F(h, C_MASS);
     a.charge = F(h, C_CHARGE);
     a.ctrl = h->ctrl;
@@ -811,6 +826,7 @@ int allocate(GeonContext* h) {
 }
 
 void merge_release(GeonContext* h);
+void rad_release(GeonContext* h);
 
 void release(GeonContext* h) {
     for (int c = 0; c < N_STATE; c++) { hipFree(h->st[c]); hipFree(h->st_alt[c]); }
@@ -827,6 +843,7 @@ void release(GeonContext* h) {
     hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
     hipFree(tb.node_first); hipFree(tb.node_mom); hipFree(tb.pre_mom);
     merge_release(h);
+    rad_release(h);
     far_release(h);
     hipFree(h->stat_partial); hipFree(h->stat_out); hipFree(h->field_out);
     for (int w = 0; w < 2; w++) {
@@ -923,6 +940,39 @@ FarTrainArgs far_train_args(GeonContext* h, float wscale) {
 }  // namespace
 
 namespace {
+
+// ---- radiation reaction (integrator.js:983-1086, 1567-1740) ----
+
+void rad_release(GeonContext* h) {
+    hipFree(h->rad_scr); hipFree(h->rad_partial); hipFree(h->rad_sums); hipFree(h->rad_tot);
+    h->rad_scr = h->rad_partial = h->rad_sums = h->rad_tot = nullptr;
+}
+
+// Scratch of the radiation passes, allocated when radiation is first on.
+int rad_alloc(GeonContext* h) {
+    if (h->rad_scr) return GEON_OK;
+    const size_t cap = (size_t)h->cap, nb = cap / 256 + 1;
+    int rc;
+    if ((rc = dmalloc(h, &h->rad_scr, 9 * cap)) || (rc = dmalloc(h, &h->rad_partial, 9 * nb)) ||
+        (rc = dmalloc(h, &h->rad_sums, 16)) || (rc = dmalloc(h, &h->rad_tot, 4)))
+        return rc;
+    HIP_OK(hipMemset(h->rad_tot, 0, 4 * sizeof(double)));
+    return GEON_OK;
+}
+
+QuadArgs quad_args(GeonContext* h) {
+    QuadArgs q;
+    q.x = F(h, C_X); q.y = F(h, C_Y); q.vx = F(h, C_VX); q.vy = F(h, C_VY);
+    q.fx = h->fc[K_FX]; q.fy = h->fc[K_FY]; q.jx = h->fc[K_JX]; q.jy = h->fc[K_JY];
+    q.mass = F(h, C_MASS); q.charge = F(h, C_CHARGE);
+    q.wx = F(h, C_WX); q.wy = F(h, C_WY);
+    q.scr = h->rad_scr; q.stride = h->cap;
+    q.partial = h->rad_partial; q.sums = h->rad_sums; q.rad_tot = h->rad_tot;
+    q.ctrl = h->ctrl;
+    q.gw = !!(h->prm.toggles0 & GEON_GRAVITY);
+    q.em = !!(h->prm.toggles0 & GEON_COULOMB);
+    return q;
+}
 
 // ---- collision mode merge (collisions.js:24-158, integrator.js:1336-1416) ----
 
@@ -1166,6 +1216,7 @@ int geon_reset(GeonHandle h) {
     h->n_host = 0; h->next_slot = 0; h->forces_init = false; h->tree_valid = false;
     h->sim_time = 0; h->substeps_total = 0;
     hist_clear(h);
+    if (h->rad_tot) HIP_OK(hipMemsetAsync(h->rad_tot, 0, 4 * sizeof(double), h->stream));
     hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0);
     hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0.0);
     reset_fields(h);
@@ -1228,6 +1279,7 @@ int geon_upload_soa(GeonHandle h, const GeonSoAView* v, int32_t n) {
     hipLaunchKernelGGL(k_set_time, dim3(1), dim3(1), 0, h->stream, h->ctrl, 0.0);
     // loaded particles are treated as always existing (save-load.js:231)
     hist_clear(h);
+    if (h->rad_tot) HIP_OK(hipMemsetAsync(h->rad_tot, 0, 4 * sizeof(double), h->stream));
     launch_hist_reset(h->stream, h->hist_fe, h->hist_cd, 0, n, -INFINITY);
     HIP_OK(hipStreamSynchronize(h->stream));
     h->n_host = n;
@@ -1355,6 +1407,8 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
     }
     const int n = h->n_host;
     h->events_host.clear();
+    const bool rad = !!(h->prm.toggles0 & GEON_RADIATION);
+    if (rad && (rc = rad_alloc(h))) return rc;
     hipLaunchKernelGGL(k_begin_update, dim3(1), dim3(1), 0, h->stream, h->ctrl, dt, h->prm.max_substeps, h->sim_time);
     StepArgs sa = step_args(h);
     const double soft = (h->prm.toggles0 & GEON_BLACK_HOLE) ? h->prm.bh_softening : h->prm.softening;
@@ -1369,6 +1423,9 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
                 Timer t(h, 3);
                 sa = step_args(h);
                 launch_kick_drift(h->stream, sa);
+                if (sa.rad)         // the substep's radiated energy and momentum (integrator.js:1044-1060)
+                    launch_sum_cols(h->stream, h->rad_scr, 3, h->cap, h->n_host, h->ctrl, h->rad_partial, h->rad_tot,
+                                    true, true);
             }
             const bool one_pn = !!(h->prm.toggles0 & GEON_ONE_PN);
             if (one_pn && (rc = one_pn_step(h))) return rc;
@@ -1404,6 +1461,9 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
             h->n_events++;
         }
     }
+    const uint32_t tg = h->prm.toggles0;
+    if (rad && h->n_host >= 2 && (tg & (GEON_GRAVITY | GEON_COULOMB)))     // integrator.js:1567-1740
+        launch_quad_radiation(h->stream, quad_args(h), dt, h->n_host);
     // boundary once per update (integrator.js:1798-1824)
     sa = step_args(h);
     launch_boundary(h->stream, sa, h->prm.boundary_mode, h->prm.topology, h->prm.domain_w, h->prm.domain_h);
@@ -1436,7 +1496,8 @@ int geon_get_forces(GeonHandle h, GeonForceView* v, int32_t n_slots) {
     }
     HIP_OK(hipStreamSynchronize(h->stream));
     double* outs[N_FORCE] = {v->fx, v->fy, v->pe, v->bz, v->dbzdx, v->dbzdy, v->bgz, v->dbgzdx, v->dbgzdy,
-                             v->frame_drag_torque, v->tidal_torque, v->f1pn_x, v->f1pn_y};
+                             v->frame_drag_torque, v->tidal_torque, v->f1pn_x, v->f1pn_y, nullptr, nullptr,
+                             v->jerk_x, v->jerk_y};
     for (int i = 0; i < n; i++) {
         const int s = slot[i];
         if (s < 0 || s >= n_slots) continue;
@@ -1505,6 +1566,12 @@ int geon_get_stats(GeonHandle h, GeonStats* out) {
         out->field_energy = dw[0];
         out->field_px += dw[1];
         out->field_py += dw[2];
+    }
+    if (h->rad_tot) {
+        double rt[4];
+        HIP_OK(hipMemcpyAsync(rt, h->rad_tot, sizeof(rt), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        out->radiated_energy = rt[0]; out->radiated_px = rt[1]; out->radiated_py = rt[2];
     }
     out->sim_time = h->sim_time;
     out->alive_count = h->n_host;

@@ -25,6 +25,7 @@ enum : uint32_t {
     F_GRAV = 1u,        // gravity (+ tidal torque)
     F_COUL = 2u,        // Coulomb
     F_EXTRA = 4u,       // any of magnetic / GM / 1PN / Yukawa / axion / Higgs (runtime-checked)
+    F_JERK = 8u,        // radiation on: the analytic jerk too; gravity and Coulomb runtime-checked
 };
 
 // Runtime toggles of the extra forces (uniform per launch).
@@ -38,6 +39,7 @@ struct PhysFlags {
     uint32_t periodic;
     uint32_t window;        // periodic: approximate only nodes inside the min-image window
     uint32_t topology;      // periodic: 0 torus, 1 Klein bottle, 2 RP^2 (config.js:22-26)
+    uint32_t grav_on = 0, coul_on = 0;  // gravity / Coulomb, read by the F_JERK variants
 };
 
 // Tree node record: 16 dwords.
@@ -88,9 +90,15 @@ template <bool KAHAN>
 struct Acc {
     Kahan fx, fy, pe;
     float bz, dbzdx, dbzdy, bgz, dbgzdx, dbgzdy, frame_drag, tidal, f1x, f1y;
+    float jx, jy;           // analytic jerk dF/dt (F_JERK only)
     __device__ __forceinline__ void init() {
         fx.init(); fy.init(); pe.init();
         bz = dbzdx = dbzdy = bgz = dbgzdx = dbgzdy = frame_drag = tidal = f1x = f1y = 0.f;
+        jx = jy = 0.f;
+    }
+    __device__ __forceinline__ void addj(float fd, float jr, float vrx, float vry, float rx, float ry) {
+        jx += vrx * fd + rx * jr;
+        jy += vry * fd + ry * jr;
     }
     __device__ __forceinline__ void addf(float x, float y) {
         if (KAHAN) { fx.add(x); fy.add(y); } else { fx.s += x; fy.s += y; }
@@ -193,7 +201,8 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
 }
 
 // One source acting on the observer: the body of pairForce (forces.js:163-442)
-// in FP32, without the radiation jerk.  (rx, ry) is the (min-imaged)
+// in FP32; F_JERK adds each term's analytic jerk dF/dt (forces.js:198-402),
+// the input of the radiation reaction.  (rx, ry) is the (min-imaged)
 // displacement source - observer.  ABERR: the source is signal-delayed, and the
 // Lienard-Wiechert factor min((max(1 - n.v_s, 0.01))^-3, 100) scales its
 // position-dependent forces (forces.js:182-189); PE and fields are unscaled.
@@ -216,18 +225,29 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         aberr = fminf(1.f / (den * den * den), 100.f);
     }
     const float inv_r3a = ABERR ? inv_r3 * aberr : inv_r3;
-    if (F & F_GRAV) {
+    const bool do_grav = (F & F_JERK) ? pf.grav_on != 0u : (F & F_GRAV) != 0u;
+    const bool do_coul = (F & F_JERK) ? pf.coul_on != 0u : (F & F_COUL) != 0u;
+    float vrx = 0.f, vry = 0.f, rdv = 0.f, inv_r5j = 0.f;
+    if (F & F_JERK) {                                               // forces.js:198-203
+        vrx = svx - o.vx;
+        vry = svy - o.vy;
+        rdv = rx * vrx + ry * vry;
+        inv_r5j = ABERR ? inv_r3 * inv_r2 * aberr : inv_r3 * inv_r2;
+    }
+    if (do_grav) {
         const float k = o.m * sm;
         const float fd = k * inv_r3a;
         a.addf(rx * fd, ry * fd);
+        if (F & F_JERK) a.addj(fd, -3.f * k * rdv * inv_r5j, vrx, vry, rx, ry);
         if (PE) a.addpe(-k * inv_r);
     }
     float ax_pair = 1.f;
     if ((F & F_EXTRA) && pf.ax_mod) ax_pair = sqrtf(o.ax * s_ax);
-    if (F & F_COUL) {
+    if (do_coul) {
         const float k = -(o.q * sq) * ax_pair;
         const float fd = k * inv_r3a;
         a.addf(rx * fd, ry * fd);
+        if (F & F_JERK) a.addj(fd, -3.f * k * rdv * inv_r5j, vrx, vry, rx, ry);
         if (PE) a.addpe(o.q * sq * inv_r * ax_pair);
     }
     if (F & F_EXTRA) {
@@ -249,6 +269,11 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float fx = base * (rx * radial + (o.vx * c1 + svx * c2) * r);
                 const float fy = base * (ry * radial + (o.vy * c1 + svy * c2) * r);
                 a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
+                if (F & F_JERK) {               // position-only EIH term m1 m2 (5 m1 + 4 m2) r / r^4
+                    const float ke = o.m * sm * (5.f * o.m + 4.f * sm);
+                    const float r4 = inv_r2 * inv_r2;
+                    a.addj(ke * r4, -4.f * ke * rdv * r4 * inv_r2, vrx, vry, rx, ry);
+                }
                 if (PE) a.addpe(-o.m * sm * inv_r * (1.5f * (v1sq + v2sq) - 3.5f * v1v2 - 0.5f * nv1 * nv2 + o.m * inv_r + sm * inv_r));
             }
             if (pf.mag) {
@@ -264,11 +289,13 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             const float cc = o.q * sq * (o.m + sm) - (o.q * o.q * sm + sq * sq * o.m);
             const float fd = cc * inv_r2 * inv_r2;
             a.addf(rx * fd, ry * fd); a.f1x += rx * fd; a.f1y += ry * fd;
+            if (F & F_JERK) a.addj(fd, -4.f * cc * rdv * inv_r2 * inv_r2 * inv_r2, vrx, vry, rx, ry);
             if (PE) a.addpe(0.5f * cc * inv_r2);
         }
         if (pf.mag) {                                               // forces.js:322-353
             const float fd = -3.f * (o.mu * s_mu) * inv_r5a * ax_pair;
             a.addf(rx * fd, ry * fd);
+            if (F & F_JERK) a.addj(fd, 15.f * (o.mu * s_mu) * rdv * inv_r5a * inv_r2 * ax_pair, vrx, vry, rx, ry);
             if (PE) a.addpe((o.mu * s_mu) * inv_r3 * ax_pair);
             const float bzm = sq * cross_sv * inv_r3 * ax_pair;
             a.bz += bzm;
@@ -281,6 +308,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         if (pf.gm) {                                                // forces.js:355-387
             const float fd = 3.f * (o.L * s_L) * inv_r5a;
             a.addf(rx * fd, ry * fd);
+            if (F & F_JERK) a.addj(fd, -15.f * (o.L * s_L) * rdv * inv_r5a * inv_r2, vrx, vry, rx, ry);
             if (PE) a.addpe(-(o.L * s_L) * inv_r3);
             const float bgm = -sm * cross_sv * inv_r3;
             a.bgz += bgm;
@@ -299,6 +327,9 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             const float gmm = pf.yuk_g * ym * o.m * sm * e;
             const float fd = gmm * (inv_r2 + mu * inv_r) * (ABERR ? inv_r * aberr : inv_r);
             a.addf(rx * fd, ry * fd);
+            if (F & F_JERK)
+                a.addj(fd, -(3.f * inv_r2 + 3.f * mu * inv_r + mu * mu) * rdv * gmm * inv_r2 *
+                               (ABERR ? inv_r * aberr : inv_r), vrx, vry, rx, ry);
             if (PE) a.addpe(-gmm * inv_r);
             if (pf.one_pn) {
                 const float nx = rx * inv_r, ny = ry * inv_r;
@@ -314,11 +345,11 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             }
         }
     }
-    if (F & F_GRAV) {                                               // forces.js:429-441
+    if (do_grav) {                                                  // forces.js:429-441
         const float cross_rv = rx * (svy - o.vy) - ry * (svx - o.vx);
         const float dw = o.ang_vel - cross_rv * inv_r2;
         float coupling = sm;
-        if (F & F_COUL) coupling += o.q * sq * o.inv_m;
+        if (do_coul) coupling += o.q * sq * o.inv_m;
         const float inv_r6 = inv_r2 * inv_r2 * inv_r2;
         a.tidal -= 64.f * coupling * coupling * o.tidal_ri5 * inv_r6 * dw;
     }

@@ -90,6 +90,7 @@ struct ForceArgs {
     const float* radius;
     const Node* nodes;
     float *fx, *fy, *pe, *bz, *dbzdx, *dbzdy, *bgz, *dbgzdx, *dbgzdy, *frame_drag, *tidal, *f1x, *f1y;
+    float *jx = nullptr, *jy = nullptr;     // analytic jerk (radiation on)
     const float* mass_for_cyc;  // for cyclotron scan in the epilogue
     const float* charge;
     Ctrl* ctrl;
@@ -339,7 +340,31 @@ struct StepArgs {
     uint32_t rel, mag, gm, ext_bz_on, grav, spin_orbit, one_pn, expansion;
     float hubble, cx, cy;
     int n_upper;
+    // Landau-Lifshitz radiation reaction (radiation and Coulomb on): the jerk
+    // of the last force pass in, each particle's radiated energy and momentum
+    // out (three FP64 columns of `rad_stride`)
+    uint32_t rad = 0;
+    const float *jx = nullptr, *jy = nullptr;
+    double* rad_e = nullptr;
+    int64_t rad_stride = 0;
 };
+// FP64 column sums: out[c] (+)= sum of column c over the live particles;
+// gate_active: nothing is added when the controller has no substep left.
+void launch_sum_cols(hipStream_t s, const double* cols, int ncols, int64_t stride, int n_upper, const Ctrl* ctrl,
+                     double* partial, double* out, bool accumulate, bool gate_active = false);
+// Quadrupole radiation once per update (integrator.js:1567-1740).
+struct QuadArgs {
+    const float *x, *y, *vx, *vy, *fx, *fy, *jx, *jy, *mass, *charge;
+    float *wx, *wy;
+    double* scr;                // nine FP64 columns of `stride`
+    int64_t stride;
+    double* partial;            // block sums
+    double* sums;               // [0..2] COM sums, [3..11] quadrupole sums
+    double* rad_tot;            // [0] radiated energy (accumulated)
+    const Ctrl* ctrl;
+    int gw, em;
+};
+void launch_quad_radiation(hipStream_t s, const QuadArgs& q, double dt, int n_upper);
 void launch_dt_scan(hipStream_t s, const StepArgs& a);
 // Batch of systems: per-slot system id and liveness, per-system controllers.
 struct BatchStepArgs {

@@ -70,8 +70,9 @@ __device__ __forceinline__ void load_obs(const ForceArgs& a, int i, Obs& o) {
     o.inv_m = o.m > kEpsMass ? 1.f / o.m : 0.f;
 }
 
-template <bool KAHAN>
+template <uint32_t F, bool KAHAN>
 __device__ __forceinline__ void store_acc(const ForceArgs& a, int i, const Obs& o, const Acc<KAHAN>& acc) {
+    if (F & F_JERK) { a.jx[i] = acc.jx; a.jy[i] = acc.jy; }
     // _applyExternalFields (integrator.js:563-592) folded into the store
     a.fx[i] = acc.fx.s + o.m * a.ext_gx + o.q * a.ext_ex;
     a.fy[i] = acc.fy.s + o.m * a.ext_gy + o.q * a.ext_ey;
@@ -234,7 +235,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
         if (far_n > a.far_cap) a.ctrl->overflow = 3;
     }
     wave_add_counters(a.ctrl, c_pair, c_node, c_visit);
-    if (valid) store_acc<KAHAN>(a, i, o, acc);
+    if (valid) store_acc<F, KAHAN>(a, i, o, acc);
 }
 
 __device__ __forceinline__ int64_t walk_group() { return (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; }
@@ -592,7 +593,7 @@ __device__ __forceinline__ void direct_body(const ForceArgs& a) {
         const int64_t nn = n;
         atomicAdd((unsigned long long*)&a.ctrl->cnt_pair, (unsigned long long)(nn * (nn - 1)));
     }
-    if (valid) store_acc<KAHAN>(a, i, o, acc);
+    if (valid) store_acc<F, KAHAN>(a, i, o, acc);
 }
 
 template <uint32_t F, bool KAHAN>
@@ -637,6 +638,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_dead_forces(ForceArgs a) {
     a.bgz[i] += acc.bgz; a.dbgzdx[i] += acc.dbgzdx; a.dbgzdy[i] += acc.dbgzdy;
     a.frame_drag[i] += acc.frame_drag; a.tidal[i] += acc.tidal;
     a.f1x[i] += acc.f1x; a.f1y[i] += acc.f1y;
+    if (F & F_JERK) { a.jx[i] += acc.jx; a.jy[i] += acc.jy; }
 }
 
 #define GEON_FEAT_SWITCH(KERNEL, feat, kahan, grid, stream, args)                                     \
@@ -662,13 +664,26 @@ __global__ __launch_bounds__(kWalkThreads) void k_dead_forces(ForceArgs a) {
         }                                                                                             \
     } while (0)
 
+// As GEON_FEAT_SWITCH, with the radiation variant (all forces runtime-checked
+// plus the jerk) for the kernels that run with radiation on.
+#define GEON_FEAT_SWITCH_R(KERNEL, feat, kahan, grid, stream, args)                                   \
+    do {                                                                                              \
+        if ((feat) & F_JERK) {                                                                        \
+            const dim3 g_(grid), b_(kWalkThreads);                                                    \
+            if (kahan) hipLaunchKernelGGL((KERNEL<15, true>), g_, b_, 0, stream, args);                \
+            else hipLaunchKernelGGL((KERNEL<15, false>), g_, b_, 0, stream, args);                     \
+        } else {                                                                                      \
+            GEON_FEAT_SWITCH(KERNEL, feat, kahan, grid, stream, args);                                \
+        }                                                                                             \
+    } while (0)
+
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    if (a.hist.on) GEON_FEAT_SWITCH(k_walk_delay, feat, kahan, grid, s, a);
+    if (a.hist.on) GEON_FEAT_SWITCH_R(k_walk_delay, feat, kahan, grid, s, a);
     else if (a.far_mode == kFarMultipole) GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
     else if (a.far_mode == kFarList) GEON_FEAT_SWITCH(k_walk_list, feat, kahan, grid, s, a);
-    else GEON_FEAT_SWITCH(k_walk, feat, kahan, grid, s, a);
+    else GEON_FEAT_SWITCH_R(k_walk, feat, kahan, grid, s, a);
 }
 
 #define GEON_FEAT_SWITCH_P(KERNEL, feat, kahan, grid, threads, stream, ...)                              \
@@ -696,14 +711,14 @@ void launch_walk_persistent(hipStream_t s, const ForceArgs& a, uint32_t feat, bo
 void launch_direct(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    if (a.hist.on) GEON_FEAT_SWITCH(k_direct_delay, feat, kahan, grid, s, a);
-    else GEON_FEAT_SWITCH(k_direct, feat, kahan, grid, s, a);
+    if (a.hist.on) GEON_FEAT_SWITCH_R(k_direct_delay, feat, kahan, grid, s, a);
+    else GEON_FEAT_SWITCH_R(k_direct, feat, kahan, grid, s, a);
 }
 
 void launch_dead_forces(hipStream_t s, const ForceArgs& a, uint32_t feat) {
     if (a.n_upper <= 0 || a.hist.n_dead <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    GEON_FEAT_SWITCH(k_dead_forces, feat, false, grid, s, a);
+    GEON_FEAT_SWITCH_R(k_dead_forces, feat, false, grid, s, a);
 }
 
 }  // namespace geon

@@ -176,6 +176,46 @@ __device__ __forceinline__ double kick_drift_one(const StepArgs& a, int i, float
                 }
             }
         }
+        if (a.rad) {            // Landau-Lifshitz radiation reaction (integrator.js:983-1086), FP64
+            double de = 0.0, dpx = 0.0, dpy = 0.0;
+            const double wm2 = (double)wx * wx + (double)wy * wy;
+            if (fabsf(q) >= 1e-12f && wm2 >= 1e-12) {
+                const double md = m, inv_md = 1.0 / md;
+                const double gamma = a.rel ? sqrt(1.0 + wm2) : 1.0;
+                const double tau = 2.0 / 3.0 * ((double)q * q) * inv_md;
+                double frx = tau * a.jx[i], fry = tau * a.jy[i];
+                if (a.rel && gamma > 1.0) {
+                    const double ig3 = 1.0 / (gamma * gamma * gamma);
+                    frx *= ig3; fry *= ig3;
+                    const double ig = 1.0 / gamma;
+                    const double ux = wx * ig, uy = wy * ig;
+                    const double fsq = (double)fx * fx + (double)fy * fy;
+                    const double vdf = ux * fx + uy * fy;
+                    const double t23 = -tau * gamma * (fsq - vdf * vdf) * inv_md;
+                    frx += t23 * ux; fry += t23 * uy;
+                }
+                const double ke_b = wm2 / (gamma + 1.0) * md;
+                double nwx = wx + frx * dt * inv_md, nwy = wy + fry * dt * inv_md;
+                if (nwx != nwx || nwy != nwy) { nwx = 0.0; nwy = 0.0; }
+                const double wm2a = nwx * nwx + nwy * nwy;
+                const double ke_a = wm2a / (sqrt(1.0 + wm2a) + 1.0) * md;
+                de = ke_b - ke_a > 0.0 ? ke_b - ke_a : 0.0;
+                if (de > 0.0) {
+                    const double ax = fx * inv_md, ay = fy * inv_md;
+                    const double am2 = ax * ax + ay * ay;
+                    if (am2 > 1e-12) {
+                        const double iam = 1.0 / sqrt(am2);
+                        dpx = -de * ax * iam;
+                        dpy = -de * ay * iam;
+                    }
+                }
+                wx = (float)nwx;
+                wy = (float)nwy;
+            }
+            a.rad_e[i] = de;
+            a.rad_e[a.rad_stride + i] = dpx;
+            a.rad_e[2 * a.rad_stride + i] = dpy;
+        }
         // drift
         const float inv_g = a.rel ? 1.f / sqrtf(1.f + wx * wx + wy * wy) : 1.f;
         vx = wx * inv_g;

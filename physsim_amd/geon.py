@@ -163,7 +163,7 @@ class HipPhysics:
 
     def forces(self, n_slots):
         names = ("fx", "fy", "bz", "dbzdx", "dbzdy", "bgz", "dbgzdx", "dbgzdy", "frame_drag_torque",
-                 "tidal_torque", "pe", "f1pn_x", "f1pn_y")
+                 "tidal_torque", "pe", "f1pn_x", "f1pn_y", "jerk_x", "jerk_y")
         out = {k: np.zeros(n_slots) for k in names}
         v = A.GeonForceView(*[_dptr(out[k]) for k in names])
         self._check(self.lib.geon_get_forces(self.h, C.byref(v), int(n_slots)))
