@@ -151,6 +151,126 @@ def run_batch(n_systems, steps):
                         "dt 1e-3 per update, despawn"}
 
 
+def _cpu_update_rate(prm, st, dt, warm, threads):
+    """The FP64 oracle on this host: particle-steps/s of one update after
+    `warm` untimed ones (same parameters, smaller N)."""
+    from oracle import bind
+    bind.set_threads(threads)
+    o = bind.Oracle(bind.default_params(**prm), len(st["x"]))
+    o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+    if "anti" in st:
+        o.set_antimatter(st["anti"])
+    for _ in range(warm):
+        o.update(dt)
+    n0 = o.count()
+    t0 = time.perf_counter()
+    steps = o.update(dt)
+    return {"value": steps * n0 / (time.perf_counter() - t0), "unit": "particle-steps/s", "cores": threads,
+            "kind": "port", "n": len(st["x"])}
+
+
+def run_legs(names, with_cpu, threads):
+    """The rows beyond the headline, each timed on the GPU with the FP64 oracle
+    beside it on a smaller N: signal-delayed leaf forces, collision mode merge,
+    radiation reaction.  Rates are particle-steps/s over whole updates."""
+    from physsim_amd import HipPhysics, default_params, _abi as A
+    out = {}
+    g0 = np.random.default_rng(4242)
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+
+    def state(n, W, speed, mass=(0.5, 3.0), spin=0.3, anti=0.0):
+        return dict(x=f32(g0.uniform(0, W, n)), y=f32(g0.uniform(0, W, n)), wx=f32(g0.normal(0, speed, n)),
+                    wy=f32(g0.normal(0, speed, n)), mass=f32(g0.uniform(*mass, n)),
+                    charge=g0.integers(-10, 11, n) * 0.1, angw=f32(g0.uniform(-spin, spin, n)),
+                    anti=(g0.random(n) < anti).astype(np.int32))
+
+    def gpu_rate(prm, st, dt, warm, steps, cap=None, timers=True):
+        g = HipPhysics(default_params(**prm), max_particles=cap or len(st["x"]))
+        g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"],
+                 antimatter=st.get("anti"))
+        for _ in range(warm):
+            g.update(dt)
+        g.timing(timers)
+        ps, t0 = 0, time.perf_counter()
+        for _ in range(steps):
+            n0 = g.getParticleCount()
+            ps += g.update(dt) * n0
+        wall = time.perf_counter() - t0
+        walk_ms, walk_n = g.kernel_timing(TIMER_WALK)
+        r = {"particle_steps_per_s": ps / wall, "ms_per_update": wall / steps * 1e3,
+             "walk_ms_avg": walk_ms / max(walk_n, 1), "particles": g.getParticleCount()}
+        g.destroy()
+        return r
+
+    if "delay" in names:
+        # light-cone sources for every leaf pair (signal-delay.js:16-221): config-2 forces, relativity
+        n, W, dt = 1 << 16, 256.0, 1.0 / 128
+        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.MAGNETIC | A.RELATIVITY | A.BARNES_HUT, domain_w=W,
+                   domain_h=W, boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0,
+                   signal_delay=A.DELAY_ON, history_stride=1)
+        st = state(n, W, 0.3)
+        r = gpu_rate(prm, st, dt, 8, 4)
+        r_off = gpu_rate(dict(prm, signal_delay=A.DELAY_OFF), st, dt, 8, 4)
+        r.update({"workload": "N=%d uniform %gx%g open box, gravity+Coulomb+magnetic, relativistic, BH theta 0.5, "
+                              "signal delay on, a history sample per update, 8 updates of history first" % (n, W, W),
+                  "current_time_sources": r_off, "slowdown_vs_current_time": r_off["particle_steps_per_s"] /
+                  r["particle_steps_per_s"]})
+        if with_cpu:
+            cs = state(16384, W * 0.5, 0.3)
+            r["cpu_baseline"] = _cpu_update_rate(prm, cs, dt, 8, threads)
+        out["signal_delay"] = r
+        log("leg delay: %.3e particle-steps/s" % r["particle_steps_per_s"])
+    if "merge" in names:
+        # collision mode merge (collisions.js:24-158): about one body in ten overlaps another at the start
+        n, W, dt = 1 << 20, 16384.0, 1.0 / 64
+        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, domain_w=W, domain_h=W,
+                   boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0, collision_mode=A.COL_MERGE,
+                   signal_delay=A.DELAY_OFF)
+        st = state(n, W, 0.5, anti=0.2)
+        g = HipPhysics(default_params(**prm), max_particles=2 * n)
+        g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"], antimatter=st["anti"])
+        g.compute_forces()
+        t0 = time.perf_counter()
+        g.update(dt)
+        first_ms = (time.perf_counter() - t0) * 1e3
+        events = len(g.collision_events())
+        left = g.getParticleCount()
+        g.destroy()
+        steady = gpu_rate(prm, st, dt, 2, 4, cap=2 * n)
+        passing = gpu_rate(dict(prm, collision_mode=A.COL_PASS), st, dt, 2, 4, cap=2 * n)
+        r = {"workload": "N=%d uniform %gx%g open box, masses U[0.5, 3] (radii 0.8-1.4), 20%% antimatter, "
+                         "gravity+Coulomb, BH theta 0.5, dt 1/64" % (n, W, W),
+             "first_update_ms": first_ms, "first_update_events": events, "particles_after_first": left,
+             "steady": steady, "pass_mode": passing,
+             "merge_pass_ms_per_update": steady["ms_per_update"] - passing["ms_per_update"],
+             "particle_steps_per_s": steady["particle_steps_per_s"]}
+        if with_cpu:
+            cs = state(65536, W * 0.25, 0.5, anti=0.2)
+            r["cpu_baseline"] = _cpu_update_rate(prm, cs, dt, 0, threads)
+        out["merge"] = r
+        log("leg merge: first update %.1f ms (%d events), steady %.3e particle-steps/s"
+            % (first_ms, events, steady["particle_steps_per_s"]))
+    if "radiation" in names:
+        # Landau-Lifshitz reaction every substep, quadrupole radiation every update
+        n, W, dt = 1 << 20, 1024.0, 1.0 / 128
+        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.RELATIVITY | A.RADIATION | A.BARNES_HUT, domain_w=W,
+                   domain_h=W, boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0, signal_delay=A.DELAY_OFF)
+        st = state(n, W, 0.3)
+        r = gpu_rate(prm, st, dt, 1, 4)
+        base = gpu_rate(dict(prm, toggles0=prm["toggles0"] & ~A.RADIATION), st, dt, 1, 4)
+        r.update({"workload": "N=%d uniform %gx%g open box, gravity+Coulomb, relativistic, BH theta 0.5, radiation "
+                              "reaction (jerk, Landau-Lifshitz, quadrupole)" % (n, W, W),
+                  "without_radiation": base,
+                  "overhead": base["particle_steps_per_s"] / r["particle_steps_per_s"] - 1.0})
+        if with_cpu:
+            cs = state(65536, W * 0.25, 0.3)
+            r["cpu_baseline"] = _cpu_update_rate(prm, cs, dt, 0, threads)
+        out["radiation"] = r
+        log("leg radiation: %.3e particle-steps/s (%.1f%% over the same run without)"
+            % (r["particle_steps_per_s"], 100 * r["overhead"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,12 +294,14 @@ def main():
     ap.add_argument("--drift-n", type=int, default=1 << 20)
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="batched-ensemble leg (config 4): systems; 0 skips it")
+    ap.add_argument("--legs", default="delay,merge,radiation",
+                    help="rows beyond the headline to time ('' for none)")
     ap.add_argument("--methods", default="plain,learned,multipole",
                     help="comma list of force methods to run (the learned one is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the Barnes-Hut path here is single-GPU "
-              "(see DESIGN.md for how it would shard); got --gpus %d" % args.gpus, file=sys.stderr)
+        print("bench.py runs on one GPU only: the Barnes-Hut path here is single-GPU; got --gpus %d"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from physsim_amd import HipPhysics, default_params, workloads as W, _abi as A
@@ -298,6 +420,8 @@ def main():
     batch = run_batch(args.batch_systems, args.steps) if args.batch_systems > 0 else None
     drift = run_drift(args.drift_n, args.drift_steps, L["theta"], model) if args.drift_steps > 0 else None
 
+    legs = run_legs([x for x in args.legs.split(",") if x], not args.no_cpu, threads) if args.legs else None
+
     cpu = None
     if not args.no_cpu:
         # ---- CPU baseline: the reference algorithm in FP64 on this host ----
@@ -345,6 +469,7 @@ def main():
         "roofline_walk": walk_roof,
         "batch": batch,
         "energy_drift": drift,
+        "legs": legs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

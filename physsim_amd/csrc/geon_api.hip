@@ -119,7 +119,8 @@ struct GeonContext {
     int32_t* mg_scan_tmp = nullptr;
     MergeRes *mg_res = nullptr, *mg_spawns = nullptr;
     double* mg_events = nullptr;
-    int32_t *mg_deg = nullptr, *mg_rank = nullptr, *mg_off = nullptr, *mg_s5 = nullptr;
+    int32_t *mg_deg = nullptr, *mg_rank = nullptr, *mg_off = nullptr, *mg_s5 = nullptr, *mg_mark = nullptr;
+    uint64_t* mg_chunk5 = nullptr;
     double* mg_work = nullptr;          // 4 x particles: mass, base mass, radius, death mass
     int32_t* mg_cnt = nullptr;          // 8 ints: totals, deaths, same-leaf flag
     Ctrl* mg_ctrl = nullptr;            // n_alive = pairs of the current fold
@@ -249,7 +250,7 @@ uint32_t feature_mask(const GeonParams& p) {
     if (t & GEON_GRAVITY) f |= F_GRAV;
     if (t & GEON_COULOMB) f |= F_COUL;
     if (t & (GEON_MAGNETIC | GEON_GRAVITOMAG | GEON_YUKAWA)) f |= F_EXTRA;
-    if (t & GEON_RADIATION) f = F_GRAV | F_COUL | F_EXTRA | F_JERK;   // forces runtime-checked, plus the jerk
+    if (t & GEON_RADIATION) f = F_GRAV | F_COUL | F_JERK | (f & F_EXTRA);   // gravity, Coulomb runtime-checked; the jerk
     return f;
 }
 
@@ -991,9 +992,11 @@ void merge_free_pairs(GeonContext* h) {
 
 void merge_release(GeonContext* h) {
     merge_free_pairs(h);
-    hipFree(h->mg_deg); hipFree(h->mg_rank); hipFree(h->mg_off); hipFree(h->mg_s5); hipFree(h->mg_work);
+    hipFree(h->mg_deg); hipFree(h->mg_rank); hipFree(h->mg_off); hipFree(h->mg_s5); hipFree(h->mg_mark);
+    hipFree(h->mg_chunk5); hipFree(h->mg_work);
+    h->mg_chunk5 = nullptr;
     hipFree(h->mg_cnt); hipFree(h->mg_ctrl);
-    h->mg_deg = h->mg_rank = h->mg_off = h->mg_s5 = h->mg_cnt = nullptr;
+    h->mg_deg = h->mg_rank = h->mg_off = h->mg_s5 = h->mg_mark = h->mg_cnt = nullptr;
     h->mg_work = nullptr;
     h->mg_ctrl = nullptr;
 }
@@ -1023,7 +1026,7 @@ int merge_reserve_pairs(GeonContext* h, int64_t np) {
     if ((rc = dmalloc(h, &h->mg_p1, cap)) || (rc = dmalloc(h, &h->mg_p2, cap)) || (rc = dmalloc(h, &h->mg_leaf, cap)) ||
         (rc = dmalloc(h, &sb.keys, cap)) || (rc = dmalloc(h, &sb.keys_alt, cap)) || (rc = dmalloc(h, &sb.perm, cap)) ||
         (rc = dmalloc(h, &sb.perm_alt, cap)) || (rc = dmalloc(h, &sb.hist, 256 * nblk)) ||
-        (rc = dmalloc(h, &sb.digit_total, 256)) || (rc = dmalloc(h, &h->mg_fl, 6 * cap)) ||
+        (rc = dmalloc(h, &sb.digit_total, 256)) || (rc = dmalloc(h, &h->mg_fl, 9 * cap)) ||
         (rc = dmalloc(h, &h->mg_scan_tmp, nblk + 16)) || (rc = dmalloc(h, &h->mg_res, cap)) ||
         (rc = dmalloc(h, &h->mg_spawns, cap)) || (rc = dmalloc(h, &h->mg_events, 6 * cap)))
         return rc;
@@ -1076,9 +1079,12 @@ int merge_step(GeonContext* h, bool* changed) {
     HIP_OK(hipStreamSynchronize(h->stream));
     const bool need_rank = cnt[5] != 0;
     if (need_rank) {
-        if (!h->mg_s5 && (rc = dmalloc(h, &h->mg_s5, 5 * (size_t)(1 + 4 * h->tb.cap_divided)))) return rc;
-        merge_leaf_ranks(h->stream, h->tb, I(h, C_SLOT), 1 + 4 * n_div, h->mg_s5, h->mg_p1, h->mg_leaf, h->mg_ctrl,
-                         h->mg_rank, np);
+        const size_t max_nodes = 1 + 4 * (size_t)h->tb.cap_divided;
+        if (!h->mg_s5 && ((rc = dmalloc(h, &h->mg_s5, 5 * max_nodes)) || (rc = dmalloc(h, &h->mg_mark, max_nodes)) ||
+                          (rc = dmalloc(h, &h->mg_chunk5, 5 * ((size_t)h->cap / 256 + 1)))))
+            return rc;
+        merge_leaf_ranks(h->stream, h->tb, I(h, C_SLOT), 1 + 4 * n_div, h->mg_s5, h->mg_mark, h->mg_chunk5, n, h->ctrl,
+                         h->mg_p1, h->mg_leaf, h->mg_ctrl, h->mg_rank, np);
     }
     merge_pair_keys(h->stream, h->mg_p1, h->mg_p2, h->mg_leaf, I(h, C_SLOT), need_rank ? h->mg_rank : nullptr,
                     h->mg_ctrl, h->ctrl, h->mg_sort.keys, h->mg_sort.perm, np);
@@ -1095,6 +1101,7 @@ int merge_step(GeonContext* h, bool* changed) {
     const size_t pc = (size_t)h->mg_cap_pairs;
     m.f_spawn = h->mg_fl; m.f_mev = h->mg_fl + pc; m.f_aev = h->mg_fl + 2 * pc;
     m.o_spawn = h->mg_fl + 3 * pc; m.o_mev = h->mg_fl + 4 * pc; m.o_aev = h->mg_fl + 5 * pc;
+    m.f_fold = h->mg_fl + 6 * pc; m.o_fold = h->mg_fl + 7 * pc; m.fold = h->mg_fl + 8 * pc;
     m.cnt = h->mg_cnt; m.scan_tmp = h->mg_scan_tmp; m.events = h->mg_events;
     m.mc = h->mg_ctrl; m.ctrl = h->ctrl;
     m.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;

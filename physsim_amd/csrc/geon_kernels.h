@@ -196,7 +196,8 @@ struct MergeArgs {
     double *wm, *wbm, *wr, *wdm;    // working mass, base mass, radius, death mass (FP64)
     MergeRes *res, *spawns;
     int32_t *f_spawn, *f_mev, *f_aev, *o_spawn, *o_mev, *o_aev;
-    int32_t* cnt;                   // totals: spawns, merge events, annihilation events, deaths
+    int32_t *f_fold, *o_fold, *fold;    // the pairs of the ordered fold, compacted in fold order
+    int32_t* cnt;                   // totals: [0] spawns, [1] merge events, [2] annihilation events, [3] deaths, [6] fold pairs
     int32_t* scan_tmp;
     double* events;                 // 6 doubles each: type, x, y, energy, px, py
     Ctrl* mc;                       // n_alive = number of pairs
@@ -216,7 +217,8 @@ struct SpawnArgs {
 };
 void merge_pair_same_leaf(hipStream_t s, const int32_t* p1, const int32_t* leaf, const Ctrl* mc, int32_t* need, int np);
 void merge_leaf_ranks(hipStream_t s, const TreeBufs& tb, const int32_t* slot, int n_nodes, int32_t* s5,
-                      const int32_t* p1, const int32_t* leaf, const Ctrl* mc, int32_t* rank, int np);
+                      int32_t* mark, uint64_t* chunk5, int n_upper, const Ctrl* ctrl, const int32_t* p1,
+                      const int32_t* leaf, const Ctrl* mc, int32_t* rank, int np);
 void merge_pair_keys(hipStream_t s, const int32_t* p1, const int32_t* p2, const int32_t* leaf, const int32_t* slot,
                      const int32_t* rank, const Ctrl* mc, const Ctrl* ctrl, uint64_t* keys, uint32_t* perm, int np);
 void merge_resolve(hipStream_t s, const MergeArgs& m, int np, int n);

@@ -664,14 +664,19 @@ __global__ __launch_bounds__(kWalkThreads) void k_dead_forces(ForceArgs a) {
         }                                                                                             \
     } while (0)
 
-// As GEON_FEAT_SWITCH, with the radiation variant (all forces runtime-checked
-// plus the jerk) for the kernels that run with radiation on.
+// As GEON_FEAT_SWITCH, with the radiation variants (gravity and Coulomb
+// runtime-checked plus the jerk: 11 without, 15 with the extra forces) for the
+// kernels that run with radiation on.
 #define GEON_FEAT_SWITCH_R(KERNEL, feat, kahan, grid, stream, args)                                   \
     do {                                                                                              \
-        if ((feat) & F_JERK) {                                                                        \
+        if (((feat) & F_JERK) && ((feat) & F_EXTRA)) {                                                \
             const dim3 g_(grid), b_(kWalkThreads);                                                    \
             if (kahan) hipLaunchKernelGGL((KERNEL<15, true>), g_, b_, 0, stream, args);                \
             else hipLaunchKernelGGL((KERNEL<15, false>), g_, b_, 0, stream, args);                     \
+        } else if ((feat) & F_JERK) {                                                                 \
+            const dim3 g_(grid), b_(kWalkThreads);                                                    \
+            if (kahan) hipLaunchKernelGGL((KERNEL<11, true>), g_, b_, 0, stream, args);                \
+            else hipLaunchKernelGGL((KERNEL<11, false>), g_, b_, 0, stream, args);                     \
         } else {                                                                                      \
             GEON_FEAT_SWITCH(KERNEL, feat, kahan, grid, stream, args);                                \
         }                                                                                             \

@@ -54,28 +54,17 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
     return v;
 }
 
-// The five smallest slots of every divided node's points, as sorted indices in
-// ascending slot order: one wave per node.
-__global__ __launch_bounds__(256) void k_s5(const Node* __restrict__ nodes, const int32_t* __restrict__ first,
-                                            const int32_t* __restrict__ npts, const int32_t* __restrict__ slot,
-                                            int n_nodes, int32_t* __restrict__ s5) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-    if (r >= n_nodes) return;
-    if (nodes[r].count >= 0) return;            // a leaf
-    const int a = first[r], e = a + npts[r];
-    uint64_t best[5];
+__device__ __forceinline__ void insert5(uint64_t* best, uint64_t v) {
 #pragma unroll
-    for (int q = 0; q < 5; q++) best[q] = ~0ull;
-    for (int j = a + lane; j < e; j += kWave) {
-        uint64_t v = ((uint64_t)(uint32_t)slot[j] << 32) | (uint32_t)j;
-#pragma unroll
-        for (int q = 0; q < 5; q++) {           // insertion into the lane's sorted five
-            const uint64_t lo = v < best[q] ? v : best[q];
-            v = v < best[q] ? best[q] : v;
-            best[q] = lo;
-        }
+    for (int q = 0; q < 5; q++) {               // insertion into a sorted five
+        const uint64_t lo = v < best[q] ? v : best[q];
+        v = v < best[q] ? best[q] : v;
+        best[q] = lo;
     }
+}
+
+// Pops the wave's five smallest keys from the lanes' sorted fives, ascending.
+__device__ __forceinline__ void wave_pop5(uint64_t* best, uint64_t* out) {
     for (int q = 0; q < 5; q++) {
         const uint64_t m = wave_min_u64(best[0]);
         if (best[0] == m) {                     // keys are unique: one lane pops
@@ -83,7 +72,92 @@ __global__ __launch_bounds__(256) void k_s5(const Node* __restrict__ nodes, cons
             for (int t = 0; t < 4; t++) best[t] = best[t + 1];
             best[4] = ~0ull;
         }
-        if (lane == 0) s5[(size_t)r * 5 + q] = m == ~0ull ? -1 : (int32_t)(uint32_t)m;
+        out[q] = m;
+    }
+}
+
+constexpr int kChunk = 256;                      // sorted points per chunk of k_chunk5
+
+__device__ __forceinline__ uint64_t slot_key(const int32_t* slot, int j) {
+    return ((uint64_t)(uint32_t)slot[j] << 32) | (uint32_t)j;
+}
+
+// chunk5[c] = the five smallest (slot, sorted index) keys of sorted points
+// [c * kChunk, (c + 1) * kChunk), ascending, ~0 past the end: one wave per chunk.
+__global__ __launch_bounds__(256) void k_chunk5(const int32_t* __restrict__ slot, const Ctrl* __restrict__ ctrl,
+                                                int n_chunks, uint64_t* __restrict__ chunk5) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (c >= n_chunks) return;
+    const int n = ctrl->n_alive;
+    uint64_t best[5] = {~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+#pragma unroll
+    for (int t = 0; t < kChunk / kWave; t++) {
+        const int j = c * kChunk + t * kWave + lane;
+        if (j < n) insert5(best, slot_key(slot, j));
+    }
+    uint64_t out[5];
+    wave_pop5(best, out);
+    if (lane == 0)
+        for (int q = 0; q < 5; q++) chunk5[(size_t)c * 5 + q] = out[q];
+}
+
+// The five smallest slots of the marked divided nodes' points, as sorted
+// indices in ascending slot order: one wave per node, over the fives of the
+// whole chunks inside the node's range and the points of the partial ends.
+__global__ __launch_bounds__(256) void k_s5(const Node* __restrict__ nodes, const int32_t* __restrict__ first,
+                                            const int32_t* __restrict__ npts, const int32_t* __restrict__ slot,
+                                            const uint64_t* __restrict__ chunk5, int n_nodes,
+                                            const int32_t* __restrict__ mark, int32_t* __restrict__ s5) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (r >= n_nodes) return;
+    if (!mark[r] || nodes[r].count >= 0) return;   // only the divided nodes on a ranked leaf's path
+    const int a = first[r], e = a + npts[r];
+    const int cf = (a + kChunk - 1) / kChunk, cl = e / kChunk;     // whole chunks [cf, cl)
+    uint64_t best[5] = {~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+    if (cf < cl) {
+        for (int j = a + lane; j < cf * kChunk; j += kWave) insert5(best, slot_key(slot, j));
+        for (int j = cl * kChunk + lane; j < e; j += kWave) insert5(best, slot_key(slot, j));
+        const int64_t nk = (int64_t)(cl - cf) * 5;
+        for (int64_t u = lane; u < nk; u += kWave) insert5(best, chunk5[(size_t)cf * 5 + u]);
+    } else {
+        for (int j = a + lane; j < e; j += kWave) insert5(best, slot_key(slot, j));
+    }
+    uint64_t out[5];
+    wave_pop5(best, out);
+    if (lane == 0)
+        for (int q = 0; q < 5; q++) s5[(size_t)r * 5 + q] = out[q] == ~0ull ? -1 : (int32_t)(uint32_t)out[q];
+}
+
+// Child of divided node `node` whose point range holds sorted index `target`, or -1.
+__device__ __forceinline__ int child_holding(const Node* nodes, const int32_t* first, const int32_t* npts, int node,
+                                             int target) {
+    const int cb = nodes[node].child;
+    for (int q = 0; q < 4; q++) {
+        const int a = first[cb + q];
+        if (target >= a && target < a + npts[cb + q]) return cb + q;
+    }
+    return -1;
+}
+
+// mark[node] = 1 for every divided node on the path from the root to a leaf
+// that k_leaf_rank will order (the same run heads).
+__global__ void k_mark_paths(const int32_t* __restrict__ p1, const int32_t* __restrict__ leaf,
+                             const Ctrl* __restrict__ mc, const Node* __restrict__ nodes,
+                             const int32_t* __restrict__ first, const int32_t* __restrict__ npts,
+                             int32_t* __restrict__ mark) {
+    const int np = mc->n_alive;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const bool with_next = k + 1 < np && p1[k] == p1[k + 1] && leaf[k] == leaf[k + 1];
+    const bool with_prev = k > 0 && p1[k] == p1[k - 1] && leaf[k] == leaf[k - 1];
+    if (!with_next || with_prev) return;
+    int node = 0;
+    for (int depth = 0; depth <= 64 && nodes[node].count < 0; depth++) {
+        mark[node] = 1;
+        node = child_holding(nodes, first, npts, node, leaf[k]);
+        if (node < 0) return;
     }
 }
 
@@ -108,13 +182,7 @@ __global__ void k_leaf_rank(const int32_t* __restrict__ p1, const int32_t* __res
     nf = 0;
     for (int q = 0; q < 5; q++) { const int v = s5[q]; if (v >= 0) f[nf++] = v; }
     for (int depth = 0; depth < 64; depth++) {
-        // child holding the target leaf
-        const int cb = nodes[node].child;
-        int c = -1;
-        for (int q = 0; q < 4; q++) {
-            const int a = first[cb + q], e = a + npts[cb + q];
-            if (target >= a && target < e) { c = cb + q; break; }
-        }
+        const int c = child_holding(nodes, first, npts, node, target);
         if (c < 0) return;
         const int ca = first[c], ce = ca + npts[c];
         // H(C): the node's first five arrivals that fall in C, newest first
@@ -290,17 +358,32 @@ __global__ void k_resolve_isolated(MergeArgs m) {
     resolve_pair(m, t, i, j, 2.0 * m.wr[i]);
 }
 
+// f_fold[t] = 1 for the pairs the fold takes in order (a particle in another pair too).
+__global__ void k_fold_flags(MergeArgs m) {
+    const int np = m.mc->n_alive;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    const int k = (int)m.order[t];
+    m.f_fold[t] = !(m.deg[m.p1[k]] == 1 && m.deg[m.p2[k]] == 1);
+}
+
+__global__ void k_fold_list(MergeArgs m) {
+    const int np = m.mc->n_alive;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < np && m.f_fold[t]) m.fold[m.o_fold[t]] = t;
+}
+
 // The remaining pairs in the reference's order, by one thread.
 __global__ void k_resolve_fold(MergeArgs m) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const int np = m.mc->n_alive;
+    const int nf = m.cnt[6];
     int prev = -1;
     bool live = false;
     double rw = 0.0;
-    for (int t = 0; t < np; t++) {
+    for (int u = 0; u < nf; u++) {
+        const int t = m.fold[u];
         const int k = (int)m.order[t];
         const int i = m.p1[k], j = m.p2[k];
-        if (m.deg[i] == 1 && m.deg[j] == 1) continue;
         if (i != prev) {                        // p1's turn begins (collisions.js:35-39)
             prev = i;
             live = m.wm[i] != 0.0;
@@ -431,12 +514,24 @@ void merge_pair_same_leaf(hipStream_t s, const int32_t* p1, const int32_t* leaf,
     if (np > 1) hipLaunchKernelGGL(k_pair_same_leaf, dim3(blocks_of(np)), dim3(256), 0, s, p1, leaf, mc, need);
 }
 
+__global__ void k_zero_i32(int32_t* __restrict__ a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
 void merge_leaf_ranks(hipStream_t s, const TreeBufs& tb, const int32_t* slot, int n_nodes, int32_t* s5,
-                      const int32_t* p1, const int32_t* leaf, const Ctrl* mc, int32_t* rank, int np) {
+                      int32_t* mark, uint64_t* chunk5, int n_upper, const Ctrl* ctrl, const int32_t* p1,
+                      const int32_t* leaf, const Ctrl* mc, int32_t* rank, int np) {
     if (n_nodes <= 0 || np <= 0) return;
+    const int n_chunks = (n_upper + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_chunk5, dim3((unsigned)(((int64_t)n_chunks * kWave + 255) / 256)), dim3(256), 0, s, slot,
+                       ctrl, n_chunks, chunk5);
+    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_of(n_nodes)), dim3(256), 0, s, mark, n_nodes);
+    hipLaunchKernelGGL(k_mark_paths, dim3(blocks_of(np)), dim3(256), 0, s, p1, leaf, mc, tb.nodes, tb.node_first,
+                       tb.node_npts, mark);
     const int64_t threads = (int64_t)n_nodes * kWave;
     hipLaunchKernelGGL(k_s5, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tb.nodes, tb.node_first,
-                       tb.node_npts, slot, n_nodes, s5);
+                       tb.node_npts, slot, chunk5, n_nodes, mark, s5);
     hipLaunchKernelGGL(k_leaf_rank, dim3(blocks_of(np)), dim3(256), 0, s, p1, leaf, mc, tb.nodes, tb.node_first,
                        tb.node_npts, slot, s5, rank);
 }
@@ -454,6 +549,9 @@ void merge_resolve(hipStream_t s, const MergeArgs& m, int np, int n) {
     hipLaunchKernelGGL(k_merge_init, dim3(blocks_of(np)), dim3(256), 0, s, m);
     hipLaunchKernelGGL(k_merge_degree, dim3(blocks_of(np)), dim3(256), 0, s, m);
     hipLaunchKernelGGL(k_resolve_isolated, dim3(blocks_of(np)), dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_fold_flags, dim3(blocks_of(np)), dim3(256), 0, s, m);
+    exclusive_scan_i32(s, m.f_fold, m.o_fold, np, &m.mc->n_alive, m.cnt + 6, m.scan_tmp);
+    hipLaunchKernelGGL(k_fold_list, dim3(blocks_of(np)), dim3(256), 0, s, m);
     hipLaunchKernelGGL(k_resolve_fold, dim3(1), dim3(64), 0, s, m);
     hipLaunchKernelGGL(k_merge_flags, dim3(blocks_of(np)), dim3(256), 0, s, m);
     exclusive_scan_i32(s, m.f_spawn, m.o_spawn, np, &m.mc->n_alive, m.cnt + 0, m.scan_tmp);
