@@ -458,6 +458,42 @@ def test_far_samples_reconstruct_direct_sum(oracle_lib):
     assert f["pe_total"] + pe_fix == pytest.approx(pe_exact, rel=1e-12)
 
 
+def test_far_features_of_a_known_cluster(oracle_lib):
+    # One divided node holds exactly the six cluster points (a sub-cluster of
+    # five forces further division, so its moments come through the
+    # parallel-axis shifts of its children); the observer accepts it.  Its 10
+    # features equal the moments of those points taken directly in numpy, in
+    # the frame u = d/|d| (d = COM - observer), v = (-u_y, u_x).
+    px = np.array([402.0, 403.5, 401.2, 402.7, 403.1, 411.0])
+    py = np.array([301.0, 300.2, 302.4, 303.3, 301.9, 309.5])
+    pm = np.array([1.0, 2.5, 0.7, 1.8, 1.2, 3.0])
+    pq = np.array([0.3, -0.5, 0.2, 0.0, -0.1, 0.4])
+    x = np.concatenate([[100.0], px]); y = np.concatenate([[120.0], py])
+    m = np.concatenate([[1.0], pm]); q = np.concatenate([[0.1], pq])
+    prm = oracle_lib.default_params(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, boundary_mode=A.BOUND_DESPAWN,
+                                    domain_w=512.0, domain_h=512.0, theta=0.5, softening=2.0)
+    o = oracle_lib.Oracle(prm, 7)
+    o.upload(x, y, np.zeros(7), np.zeros(7), m, q, np.zeros(7))
+    raw = o.far_samples(np.array([0], dtype=np.int32))
+    S, _ = o.far_training_samples(np.array([0], dtype=np.int32))
+    assert len(raw) == 1 and raw[0, 5] == pytest.approx(pm.sum(), rel=1e-15)
+    dx, dy, r2, s = raw[0, 1], raw[0, 2], raw[0, 3], raw[0, 4]
+    M, A_ = pm.sum(), np.abs(pq).sum()
+    cx, cy = (pm * px).sum() / M, (pm * py).sum() / M
+    assert (dx, dy) == pytest.approx((cx - 100.0, cy - 120.0), rel=1e-12)
+    u = np.array([dx, dy]) / np.hypot(dx, dy)
+    v = np.array([-u[1], u[0]])
+    Xu = (px - cx) * u[0] + (py - cy) * u[1]
+    Xv = (px - cx) * v[0] + (py - cy) * v[1]
+    want = np.array([
+        s / np.sqrt(r2), 4.0 / r2,
+        (pm * Xu * Xu).sum() / (M * s * s), (pm * Xu * Xv).sum() / (M * s * s), (pm * Xv * Xv).sum() / (M * s * s),
+        (pq * Xu).sum() / (A_ * s), (pq * Xv).sum() / (A_ * s),
+        (pq * Xu * Xu).sum() / (A_ * s * s), (pq * Xu * Xv).sum() / (A_ * s * s), (pq * Xv * Xv).sum() / (A_ * s * s)])
+    assert np.abs(want[2:]).min() > 1e-6            # every moment component is present
+    np.testing.assert_allclose(S[0, :10], want, rtol=2e-6, atol=1e-12)      # float32 samples
+
+
 def _random_model(rng):
     model = np.zeros(A.FAR_MODEL_FLOATS, dtype=np.float32)
     model[:A.FAR_PARAMS] = rng.normal(0, 0.3, A.FAR_PARAMS)
