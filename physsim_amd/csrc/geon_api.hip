@@ -115,6 +115,9 @@ struct GeonContext {
     int64_t mg_cap_pairs = 0;
     int32_t *mg_p1 = nullptr, *mg_p2 = nullptr, *mg_leaf = nullptr;
     TreeBufs mg_sort = {};              // keys, perm and radix scratch of the pair sort
+    TreeBufs mg_csort = {};             // the same for the fold pairs grouped by component
+    int32_t *mg_uf = nullptr, *mg_uf_fail = nullptr;
+    Ctrl* mg_cctrl = nullptr;           // n_alive = fold pairs (the component sort's count)
     int32_t *mg_fl = nullptr;           // 6 x pairs: spawn / merge-event / annihilation flags and offsets
     int32_t* mg_scan_tmp = nullptr;
     MergeRes *mg_res = nullptr, *mg_spawns = nullptr;
@@ -981,6 +984,9 @@ void merge_free_pairs(GeonContext* h) {
     hipFree(h->mg_p1); hipFree(h->mg_p2); hipFree(h->mg_leaf);
     hipFree(h->mg_sort.keys); hipFree(h->mg_sort.keys_alt); hipFree(h->mg_sort.perm); hipFree(h->mg_sort.perm_alt);
     hipFree(h->mg_sort.hist); hipFree(h->mg_sort.digit_total);
+    hipFree(h->mg_csort.keys); hipFree(h->mg_csort.keys_alt); hipFree(h->mg_csort.perm);
+    hipFree(h->mg_csort.perm_alt); hipFree(h->mg_csort.hist); hipFree(h->mg_csort.digit_total);
+    h->mg_csort = {};
     hipFree(h->mg_fl); hipFree(h->mg_scan_tmp); hipFree(h->mg_res); hipFree(h->mg_spawns); hipFree(h->mg_events);
     h->mg_p1 = h->mg_p2 = h->mg_leaf = nullptr;
     h->mg_sort = {};
@@ -995,7 +1001,9 @@ void merge_release(GeonContext* h) {
     hipFree(h->mg_deg); hipFree(h->mg_rank); hipFree(h->mg_off); hipFree(h->mg_s5); hipFree(h->mg_mark);
     hipFree(h->mg_chunk5); hipFree(h->mg_work);
     h->mg_chunk5 = nullptr;
-    hipFree(h->mg_cnt); hipFree(h->mg_ctrl);
+    hipFree(h->mg_cnt); hipFree(h->mg_ctrl); hipFree(h->mg_uf); hipFree(h->mg_uf_fail); hipFree(h->mg_cctrl);
+    h->mg_uf = h->mg_uf_fail = nullptr;
+    h->mg_cctrl = nullptr;
     h->mg_deg = h->mg_rank = h->mg_off = h->mg_s5 = h->mg_mark = h->mg_cnt = nullptr;
     h->mg_work = nullptr;
     h->mg_ctrl = nullptr;
@@ -1008,9 +1016,11 @@ int merge_alloc(GeonContext* h) {
     int rc;
     if ((rc = dmalloc(h, &h->mg_deg, cap)) || (rc = dmalloc(h, &h->mg_rank, cap)) ||
         (rc = dmalloc(h, &h->mg_off, cap)) || (rc = dmalloc(h, &h->mg_work, 4 * cap)) ||
-        (rc = dmalloc(h, &h->mg_cnt, 8)) || (rc = dmalloc(h, &h->mg_ctrl, 1)))
+        (rc = dmalloc(h, &h->mg_cnt, 8)) || (rc = dmalloc(h, &h->mg_ctrl, 1)) || (rc = dmalloc(h, &h->mg_uf, cap)) ||
+        (rc = dmalloc(h, &h->mg_uf_fail, 1)) || (rc = dmalloc(h, &h->mg_cctrl, 1)))
         return rc;
     HIP_OK(hipMemset(h->mg_ctrl, 0, sizeof(Ctrl)));
+    HIP_OK(hipMemset(h->mg_cctrl, 0, sizeof(Ctrl)));
     return GEON_OK;
 }
 
@@ -1029,6 +1039,11 @@ int merge_reserve_pairs(GeonContext* h, int64_t np) {
         (rc = dmalloc(h, &sb.digit_total, 256)) || (rc = dmalloc(h, &h->mg_fl, 9 * cap)) ||
         (rc = dmalloc(h, &h->mg_scan_tmp, nblk + 16)) || (rc = dmalloc(h, &h->mg_res, cap)) ||
         (rc = dmalloc(h, &h->mg_spawns, cap)) || (rc = dmalloc(h, &h->mg_events, 6 * cap)))
+        return rc;
+    TreeBufs& cb = h->mg_csort;
+    if ((rc = dmalloc(h, &cb.keys, cap)) || (rc = dmalloc(h, &cb.keys_alt, cap)) || (rc = dmalloc(h, &cb.perm, cap)) ||
+        (rc = dmalloc(h, &cb.perm_alt, cap)) || (rc = dmalloc(h, &cb.hist, 256 * nblk)) ||
+        (rc = dmalloc(h, &cb.digit_total, 256)))
         return rc;
     h->mg_cap_pairs = (int64_t)cap;
     return GEON_OK;
@@ -1106,7 +1121,8 @@ int merge_step(GeonContext* h, bool* changed) {
     m.mc = h->mg_ctrl; m.ctrl = h->ctrl;
     m.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
     m.dom_w = h->prm.domain_w; m.dom_h = h->prm.domain_h;
-    merge_resolve(h->stream, m, np, n);
+    m.uf = h->mg_uf; m.uf_fail = h->mg_uf_fail;
+    HIP_OK(merge_resolve(h->stream, m, h->mg_csort, h->mg_cctrl, np, n));
     merge_apply(h->stream, m, F(h, C_MASS), F(h, C_BASEMASS), F(h, C_RADIUS), h->alive, n);
     HIP_OK(hipMemcpyAsync(cnt, h->mg_cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));

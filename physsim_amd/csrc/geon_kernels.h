@@ -197,6 +197,8 @@ struct MergeArgs {
     MergeRes *res, *spawns;
     int32_t *f_spawn, *f_mev, *f_aev, *o_spawn, *o_mev, *o_aev;
     int32_t *f_fold, *o_fold, *fold;    // the pairs of the ordered fold, compacted in fold order
+    int32_t* uf;                        // union-find parents over the particles (the fold's components)
+    int32_t* uf_fail;                   // set if a union gave up: the one-thread fold runs instead
     int32_t* cnt;                   // totals: [0] spawns, [1] merge events, [2] annihilation events, [3] deaths, [6] fold pairs
     int32_t* scan_tmp;
     double* events;                 // 6 doubles each: type, x, y, energy, px, py
@@ -221,7 +223,9 @@ void merge_leaf_ranks(hipStream_t s, const TreeBufs& tb, const int32_t* slot, in
                       const int32_t* leaf, const Ctrl* mc, int32_t* rank, int np);
 void merge_pair_keys(hipStream_t s, const int32_t* p1, const int32_t* p2, const int32_t* leaf, const int32_t* slot,
                      const int32_t* rank, const Ctrl* mc, const Ctrl* ctrl, uint64_t* keys, uint32_t* perm, int np);
-void merge_resolve(hipStream_t s, const MergeArgs& m, int np, int n);
+// Fold pairs above which the fold runs by connected components.
+constexpr int kFoldOneThread = 64;      // one thread folds ~0.5 us per pair; the component path costs ~70 us
+hipError_t merge_resolve(hipStream_t s, const MergeArgs& m, TreeBufs& csort, Ctrl* cctrl, int np, int n);
 void merge_apply(hipStream_t s, const MergeArgs& m, float* mass, float* base_mass, float* radius, int32_t* alive,
                  int n);
 void merge_spawn(hipStream_t s, const SpawnArgs& a);

@@ -374,8 +374,9 @@ __global__ void k_fold_list(MergeArgs m) {
 }
 
 // The remaining pairs in the reference's order, by one thread.
-__global__ void k_resolve_fold(MergeArgs m) {
+__global__ void k_resolve_fold(MergeArgs m, int only_on_fail) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (only_on_fail && !*m.uf_fail) return;
     const int nf = m.cnt[6];
     int prev = -1;
     bool live = false;
@@ -392,6 +393,93 @@ __global__ void k_resolve_fold(MergeArgs m) {
         if (!live) { MergeRes r; r.type = 0; r.mev = 0; m.res[t] = r; continue; }
         resolve_pair(m, t, i, j, rw);
     }
+}
+
+// ---- the fold by connected components ---------------------------------------
+// Pairs that share no particle, directly or through a chain of pairs, do not
+// see each other's effects, and a particle's pairs are contiguous in the fold
+// order and all in its component.  So folding every component by its own
+// thread, in fold order, gives the one-thread fold's results exactly.
+// Components come from a lock-free union-find over the particles (a root is
+// hooked under a smaller root by compare-and-swap), then a stable radix sort
+// of the fold pairs by root.
+
+__device__ __forceinline__ int uf_load(const int32_t* p, int i) {
+    return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of v; parents only ever point to smaller indices, so the walk ends.
+__device__ int uf_find(int32_t* parent, int v) {
+    int p = uf_load(parent, v);
+    while (p != v) {
+        const int g = uf_load(parent, p);
+        if (g != p) __hip_atomic_store(parent + v, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // halving
+        v = p;
+        p = g;
+    }
+    return v;
+}
+
+__global__ void k_uf_init(int32_t* __restrict__ parent, const Ctrl* __restrict__ ctrl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ctrl->n_alive) parent[i] = i;
+}
+
+constexpr int kUnionTries = 1 << 16;
+
+__global__ void k_uf_union(MergeArgs m) {
+    const int nf = m.cnt[6];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nf) return;
+    const int k = (int)m.order[m.fold[u]];
+    const int a = m.p1[k], b = m.p2[k];
+    for (int tries = 0; tries < kUnionTries; tries++) {
+        const int ra = uf_find(m.uf, a), rb = uf_find(m.uf, b);
+        if (ra == rb) return;
+        const int hi = ra > rb ? ra : rb, lo = ra ^ rb ^ hi;
+        if (atomicCAS(m.uf + hi, hi, lo) == hi) return;
+    }
+    atomicExch(m.uf_fail, 1);                   // not reached in practice: the host folds on one thread
+}
+
+// Sort keys: the root of each fold pair's first particle, in fold order.
+__global__ void k_uf_keys(MergeArgs m, uint64_t* __restrict__ keys, uint32_t* __restrict__ perm) {
+    const int nf = m.cnt[6];
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nf) return;
+    const int k = (int)m.order[m.fold[u]];
+    keys[u] = (uint64_t)(uint32_t)uf_find(m.uf, m.p1[k]);
+    perm[u] = (uint32_t)u;
+}
+
+// One thread per component: the component's pairs in fold order.
+__global__ void k_fold_components(MergeArgs m, const uint64_t* __restrict__ keys,
+                                  const uint32_t* __restrict__ perm) {
+    const int nf = m.cnt[6];
+    const int v0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v0 >= nf || *m.uf_fail) return;
+    if (v0 > 0 && keys[v0] == keys[v0 - 1]) return;
+    int prev = -1;
+    bool live = false;
+    double rw = 0.0;
+    for (int v = v0; v < nf && keys[v] == keys[v0]; v++) {
+        const int t = m.fold[perm[v]];
+        const int k = (int)m.order[t];
+        const int i = m.p1[k], j = m.p2[k];
+        if (i != prev) {                        // p1's turn begins (collisions.js:35-39)
+            prev = i;
+            live = m.wm[i] != 0.0;
+            rw = 2.0 * m.wr[i];
+        }
+        if (!live) { MergeRes r; r.type = 0; r.mev = 0; m.res[t] = r; continue; }
+        resolve_pair(m, t, i, j, rw);
+    }
+}
+
+// The sort's count (the fold pairs) and a cleared failure flag.
+__global__ void k_set_count(Ctrl* __restrict__ c, const int32_t* __restrict__ n, int32_t* __restrict__ fail) {
+    c->n_alive = *n;
+    *fail = 0;
 }
 
 __global__ void k_merge_flags(MergeArgs m) {
@@ -543,8 +631,8 @@ void merge_pair_keys(hipStream_t s, const int32_t* p1, const int32_t* p2, const 
                            perm);
 }
 
-void merge_resolve(hipStream_t s, const MergeArgs& m, int np, int n) {
-    if (np <= 0) return;
+hipError_t merge_resolve(hipStream_t s, const MergeArgs& m, TreeBufs& csort, Ctrl* cctrl, int np, int n) {
+    if (np <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_of(n)), dim3(256), 0, s, m.deg, 0, m.ctrl);
     hipLaunchKernelGGL(k_merge_init, dim3(blocks_of(np)), dim3(256), 0, s, m);
     hipLaunchKernelGGL(k_merge_degree, dim3(blocks_of(np)), dim3(256), 0, s, m);
@@ -552,12 +640,27 @@ void merge_resolve(hipStream_t s, const MergeArgs& m, int np, int n) {
     hipLaunchKernelGGL(k_fold_flags, dim3(blocks_of(np)), dim3(256), 0, s, m);
     exclusive_scan_i32(s, m.f_fold, m.o_fold, np, &m.mc->n_alive, m.cnt + 6, m.scan_tmp);
     hipLaunchKernelGGL(k_fold_list, dim3(blocks_of(np)), dim3(256), 0, s, m);
-    hipLaunchKernelGGL(k_resolve_fold, dim3(1), dim3(64), 0, s, m);
+    int32_t nf = 0;
+    hipError_t e = hipMemcpyAsync(&nf, m.cnt + 6, sizeof(nf), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    if (nf <= kFoldOneThread) {
+        hipLaunchKernelGGL(k_resolve_fold, dim3(1), dim3(64), 0, s, m, 0);
+    } else {
+        hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, s, cctrl, m.cnt + 6, m.uf_fail);
+        hipLaunchKernelGGL(k_uf_init, dim3(blocks_of(n)), dim3(256), 0, s, m.uf, m.ctrl);
+        hipLaunchKernelGGL(k_uf_union, dim3(blocks_of(nf)), dim3(256), 0, s, m);
+        hipLaunchKernelGGL(k_uf_keys, dim3(blocks_of(nf)), dim3(256), 0, s, m, csort.keys, csort.perm);
+        radix_sort(s, csort, nf, 32, cctrl);
+        hipLaunchKernelGGL(k_fold_components, dim3(blocks_of(nf)), dim3(256), 0, s, m, csort.keys, csort.perm);
+        hipLaunchKernelGGL(k_resolve_fold, dim3(1), dim3(64), 0, s, m, 1);
+    }
     hipLaunchKernelGGL(k_merge_flags, dim3(blocks_of(np)), dim3(256), 0, s, m);
     exclusive_scan_i32(s, m.f_spawn, m.o_spawn, np, &m.mc->n_alive, m.cnt + 0, m.scan_tmp);
     exclusive_scan_i32(s, m.f_mev, m.o_mev, np, &m.mc->n_alive, m.cnt + 1, m.scan_tmp);
     exclusive_scan_i32(s, m.f_aev, m.o_aev, np, &m.mc->n_alive, m.cnt + 2, m.scan_tmp);
     hipLaunchKernelGGL(k_merge_emit, dim3(blocks_of(np)), dim3(256), 0, s, m);
+    return hipGetLastError();
 }
 
 void merge_apply(hipStream_t s, const MergeArgs& m, float* mass, float* base_mass, float* radius, int32_t* alive,

@@ -111,8 +111,12 @@ def test_clustered_fold_matches_oracle(seed, oracle_lib):
     leaf_of = dict(zip(lid.tolist(), key.tolist()))
     same_leaf = sum(1 for k in range(len(a) - 1) if a[k] == a[k + 1] and leaf_of[b[k]] == leaf_of[b[k + 1]])
     multi = sum(1 for i in set(a.tolist()) if (a == i).sum() > 1)
-    print("%d pairs, %d particles with several candidates, %d same-leaf neighbours" % (len(a), multi, same_leaf))
+    deg = np.bincount(np.concatenate([a, b]))
+    folded = int(np.sum((deg[a] > 1) | (deg[b] > 1)))    # pairs sharing a particle: the ordered fold
+    print("%d pairs, %d particles with several candidates, %d same-leaf neighbours, %d folded pairs"
+          % (len(a), multi, same_leaf, folded))
     assert multi > 50 and same_leaf > 10
+    assert folded > 64             # above this the fold runs by connected components (kFoldOneThread)
     g.update(DT)
     o.update(DT)
     _compare(g, o)
