@@ -286,7 +286,7 @@ struct FarInfer {
     halfx8 w3[4][kWave];
     halfx8 b2h[2][kWave];               // layer-2 bias as an MFMA operand (times a column of ones)
     float b3[8];
-    float mean[kFarIn], istd[kFarIn], oscale[8];
+    float istd[kFarIn], nmi[kFarIn], oscale[8];   // x = f istd + nmi, nmi = -mean istd
 };
 
 constexpr int kMlpWaves = 4;                   // waves per block, one 64-particle group each
@@ -387,8 +387,8 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         copy(reinterpret_cast<uint32_t*>(wfr.w3), reinterpret_cast<const uint32_t*>(g.w3h), sizeof(g.w3h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.b2h), reinterpret_cast<const uint32_t*>(g.b2h), sizeof(g.b2h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.b3), reinterpret_cast<const uint32_t*>(g.b3), kFarOut);
-        copy(reinterpret_cast<uint32_t*>(wfr.mean), reinterpret_cast<const uint32_t*>(g.mean), kFarIn);
         copy(reinterpret_cast<uint32_t*>(wfr.istd), reinterpret_cast<const uint32_t*>(g.istd), kFarIn);
+        if (threadIdx.x < kFarIn) wfr.nmi[threadIdx.x] = -g.mean[threadIdx.x] * g.istd[threadIdx.x];
         copy(reinterpret_cast<uint32_t*>(wfr.oscale), reinterpret_cast<const uint32_t*>(g.oscale), kFarOut);
     }
     __syncthreads();
@@ -418,6 +418,8 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         // 1. features of row l
         const bool valid = base + l < cnt;
         float f[kFarIn];
+#pragma unroll
+        for (int k = 0; k < kFarIn; k++) f[k] = 0.f;
         FarRow rw = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         int lane_of = 0;
         if (valid) {
@@ -433,8 +435,15 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
             rw.sp = a.grav ? s4.z * sp0 : 0.f;
             rw.cp = a.coul ? s4.w * cp0 : 0.f;
         }
+        // Standardised inputs without branches: a row past the list holds
+        // finite values whose outputs are discarded (MFMA output column n
+        // depends on input column n only).
         float x[16];
-        std_inputs(fr, f, valid, x);
+#pragma unroll
+        for (int k = 0; k < kFarIn; k++) x[k] = fmaf(f[k], fr.istd[k], fr.nmi[k]);
+        x[kFarIn] = valid ? 1.f : 0.f;
+#pragma unroll
+        for (int k = kFarIn + 1; k < 16; k++) x[k] = 0.f;
         *reinterpret_cast<bf16x8*>(xs + l * 16) = pack8(x);
         *reinterpret_cast<bf16x8*>(xs + l * 16 + 8) = pack8(x + 8);
         rows[l] = rw;
