@@ -47,11 +47,12 @@ constexpr double kChargeQuantum = 0.1;   // charges are integer multiples (main.
 
 // Moments of a divided node's points about its COM (not in the reference: the
 // inputs of the far-field correction).  Mass quadrupole q**, charge dipole d*,
-// charge quadrupole c**, and the node's total |charge| aq.
+// charge quadrupole c**, the node's total |charge| aq, and the feature scales
+// 1/(M s^2), 1/(A s), 1/(A s^2) (0 without mass / charge; s = node width).
 struct __align__(16) NodeMom {
     float qxx, qxy, qyy, aq;
     float dx, dy, cxx, cxy;
-    float cyy, pad0, pad1, pad2;
+    float cyy, ims2, ias, ias2;
 };
 
 struct __align__(64) Node {

@@ -128,6 +128,36 @@ __device__ __forceinline__ void pair_features(T dx, T dy, T eps2, T s, T M, cons
     cp0 = A * sr * ir;
 }
 
+// The same features for the inference pass, in FP32 with the node's scales
+// 1/(M s^2), 1/(A s), 1/(A s^2) precomputed at the tree build.
+__device__ __forceinline__ void pair_features_fast(float dx, float dy, float eps2, float s, float M,
+                                                   const NodeMom& c, float* f, float& ux, float& uy, float& sg0,
+                                                   float& sc0, float& sp0, float& cp0) {
+    const float rho2 = dx * dx + dy * dy;
+    const float r2 = rho2 + eps2;
+    const float ir = rsq_t(r2);
+    const float ir2 = ir * ir;
+    const float irho = rsq_t(rho2);
+    ux = dx * irho; uy = dy * irho;
+    const float sr = s * ir;
+    const float uu = ux * ux, vv = uy * uy, uv = ux * uy;
+    f[0] = sr;
+    f[1] = eps2 * ir2;
+    f[2] = (uu * c.qxx + 2.f * uv * c.qxy + vv * c.qyy) * c.ims2;
+    f[3] = (-uv * c.qxx + (uu - vv) * c.qxy + uv * c.qyy) * c.ims2;
+    f[4] = (vv * c.qxx - 2.f * uv * c.qxy + uu * c.qyy) * c.ims2;
+    f[5] = (ux * c.dx + uy * c.dy) * c.ias;
+    f[6] = (-uy * c.dx + ux * c.dy) * c.ias;
+    f[7] = (uu * c.cxx + 2.f * uv * c.cxy + vv * c.cyy) * c.ias2;
+    f[8] = (-uv * c.cxx + (uu - vv) * c.cxy + uv * c.cyy) * c.ias2;
+    f[9] = (vv * c.cxx - 2.f * uv * c.cxy + uu * c.cyy) * c.ias2;
+    const float A = c.aq;
+    sg0 = M * sr * sr * ir2;
+    sc0 = A * sr * ir2;
+    sp0 = M * sr * sr * ir;
+    cp0 = A * sr * ir;
+}
+
 // ---------------------------------------------------------------------------
 // Model fragments: the network's weights arranged per lane for the MFMAs
 // ---------------------------------------------------------------------------
@@ -138,6 +168,7 @@ struct FarFrags {
     halfx8 w2h[2][4][kWave];        // inference: layers 2 and 3 in FP16 (as w2, w3)
     halfx8 w3h[4][kWave];
     halfx8 b2h[2][kWave];           // inference: A[m][0], A[m][1] = b2 split into FP16 hi + lo, K 2..15 zero
+    halfx8 b3h[kWave];              // inference: the same for b3 (rows m < kFarOut)
     bf16x8 w3t[2][kWave];           // backward: A = W3 (hidden x outputs; K = 8h + j <-> output 4h + j)
     bf16x8 w2t[2][4][kWave];        // backward: A = W2 (in x out), K (out) permuted
     float b2[kFarHid];
@@ -185,6 +216,16 @@ __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict_
             v[1] = b - (float)hi;
         }
         fr->b2h[mb][l] = pack8h(v);
+    }
+    {
+        float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (h == 0 && c < kFarOut) {
+            const float b = P[kFarOffB3 + c];
+            const _Float16 hi = (_Float16)b;
+            v[0] = (float)hi;
+            v[1] = b - (float)hi;
+        }
+        fr->b3h[l] = pack8h(v);
     }
     if (l < kFarOut) fr->b3[l] = P[kFarOffB3 + l];
     if (l < kFarIn) {
@@ -285,7 +326,7 @@ struct FarInfer {
     halfx8 w2[2][4][kWave];             // FP16: the packed-FP16 activations feed them
     halfx8 w3[4][kWave];
     halfx8 b2h[2][kWave];               // layer-2 bias as an MFMA operand (times a column of ones)
-    float b3[8];
+    halfx8 b3h[kWave];                  // layer-3 bias, the same way
     float istd[kFarIn], nmi[kFarIn], oscale[8];   // x = f istd + nmi, nmi = -mean istd
 };
 
@@ -338,11 +379,16 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const __bf16* xs, i
     }
 #pragma unroll
     for (int mb = 0; mb < 2; mb++) {
+        // b2 in every column (hi + lo in FP32) as the C operand of both chains
+        const floatx16 bias = mfma32(fr.b2h[mb][l], ones, zero);
         floatx16 acc[2];
-        acc[0] = mfma32(fr.b2h[mb][l], ones, zero);             // b2 in every column, hi + lo in FP32
-        acc[1] = acc[0];
+        {
+            const halfx8 w = fr.w2[mb][0][l];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+            for (int nb = 0; nb < 2; nb++) acc[nb] = mfma32(w, a1[0][nb], bias);
+        }
+#pragma unroll
+        for (int t = 1; t < 4; t++) {
             const halfx8 w = fr.w2[mb][t][l];
 #pragma unroll
             for (int nb = 0; nb < 2; nb++) acc[nb] = mfma32(w, a1[t][nb], acc[nb]);
@@ -356,14 +402,15 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const __bf16* xs, i
             a2[2 * mb + 1][nb] = act8h(v + 8);
         }
     }
-    floatx16 b3 = {};
+    // b3 in every column: accumulator r < 4 of lane half h holds output 4h + r
+    const floatx16 b3 = mfma32(fr.b3h[l], ones, zero);
+    {
+        const halfx8 w = fr.w3[0][l];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-        if (out_of(h, r) < kFarOut) b3[r] = fr.b3[out_of(h, r)];
-    o[0] = b3;
-    o[1] = b3;
+        for (int nb = 0; nb < 2; nb++) o[nb] = mfma32(w, a2[0][nb], b3);
+    }
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
+    for (int t = 1; t < 4; t++) {
         const halfx8 w = fr.w3[t][l];
 #pragma unroll
         for (int nb = 0; nb < 2; nb++) o[nb] = mfma32(w, a2[t][nb], o[nb]);
@@ -386,7 +433,7 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         copy(reinterpret_cast<uint32_t*>(wfr.w2), reinterpret_cast<const uint32_t*>(g.w2h), sizeof(g.w2h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.w3), reinterpret_cast<const uint32_t*>(g.w3h), sizeof(g.w3h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.b2h), reinterpret_cast<const uint32_t*>(g.b2h), sizeof(g.b2h) / 4);
-        copy(reinterpret_cast<uint32_t*>(wfr.b3), reinterpret_cast<const uint32_t*>(g.b3), kFarOut);
+        copy(reinterpret_cast<uint32_t*>(wfr.b3h), reinterpret_cast<const uint32_t*>(g.b3h), sizeof(g.b3h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.istd), reinterpret_cast<const uint32_t*>(g.istd), kFarIn);
         if (threadIdx.x < kFarIn) wfr.nmi[threadIdx.x] = -g.mean[threadIdx.x] * g.istd[threadIdx.x];
         copy(reinterpret_cast<uint32_t*>(wfr.oscale), reinterpret_cast<const uint32_t*>(g.oscale), kFarOut);
@@ -407,14 +454,13 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
     const int cnt = a.far_count[g];
     const uint32_t* list = a.far_list + g * a.far_cap;
     const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
-    // software pipeline: the gathers of tile k + 1 are in flight while tile k
-    // computes (their latency, not the vector ALU, bounds this kernel)
-    FarIn cur = {}, nxt = {};
+    // software pipeline: once tile k's features are in LDS, the gathers of
+    // tile k + 1 go into the same registers and stay in flight through tile
+    // k's network pass
+    FarIn cur = {};
     if (l < cnt) far_gather(a, g, list[l], cur);
     uint32_t e_next = kWave + l < cnt ? list[kWave + l] : 0u;     // tile 1's entries
     for (int base = 0; base < cnt; base += kWave) {
-        if (base + kWave + l < cnt) far_gather(a, g, e_next, nxt);
-        e_next = base + 2 * kWave + l < cnt ? list[base + 2 * kWave + l] : 0u;
         // 1. features of row l
         const bool valid = base + l < cnt;
         float f[kFarIn];
@@ -428,8 +474,8 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
             float dx = cur.c4.x - s4.x, dy = cur.c4.y - s4.y;
             if (a.periodic) { dx = torus_wrap(dx, a.dom_w, half_w); dy = torus_wrap(dy, a.dom_h, half_h); }
             float sg0, sc0, sp0, cp0;
-            pair_features<float>(dx, dy, a.soft2, 2.f * cur.half, cur.c4.z, cur.mo, f, rw.ux, rw.uy, sg0, sc0, sp0,
-                                 cp0);
+            pair_features_fast(dx, dy, a.soft2, 2.f * cur.half, cur.c4.z, cur.mo, f, rw.ux, rw.uy, sg0, sc0, sp0,
+                               cp0);
             rw.sg = a.grav ? s4.z * sg0 : 0.f;
             rw.sc = a.coul ? s4.w * sc0 : 0.f;
             rw.sp = a.grav ? s4.z * sp0 : 0.f;
@@ -448,6 +494,8 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         *reinterpret_cast<bf16x8*>(xs + l * 16 + 8) = pack8(x + 8);
         rows[l] = rw;
         rlane[l] = lane_of;
+        if (base + kWave + l < cnt) far_gather(a, g, e_next, cur);
+        e_next = base + 2 * kWave + l < cnt ? list[base + 2 * kWave + l] : 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -477,7 +525,6 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        cur = nxt;
     }
     const int i = (int)(g * kWave) + l;
     if (i < n) {

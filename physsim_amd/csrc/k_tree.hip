@@ -657,7 +657,7 @@ __device__ NodeMom node_moments(const NodeBuild& b, int a, int e, double m, doub
     nm.cxx = (float)(S[5] - 2.0 * cX * S[3] + qtot * cX * cX);
     nm.cxy = (float)(S[6] - cX * S[4] - cY * S[3] + qtot * cX * cY);
     nm.cyy = (float)(S[7] - 2.0 * cY * S[4] + qtot * cY * cY);
-    nm.pad0 = nm.pad1 = nm.pad2 = 0.f;
+    nm.ims2 = nm.ias = nm.ias2 = 0.f;       // set by write_node, which knows the width
     return nm;
 }
 
@@ -715,7 +715,14 @@ __device__ void write_node(const NodeBuild& b, int slot, int level, uint64_t pat
     b.path[slot] = path;
     b.npts[slot] = cnt;
     b.first[slot] = a;
-    if (b.mom != nullptr && divided) b.mom[slot] = node_moments(b, a, e, m, comx, comy, qq * kChargeQuantum);
+    if (b.mom != nullptr && divided) {
+        NodeMom nm = node_moments(b, a, e, m, comx, comy, qq * kChargeQuantum);
+        const double w = 2.0 * (double)nd.half, M = nd.mass, A = nm.aq;
+        nm.ims2 = M > 0 ? (float)(1.0 / (M * w * w)) : 0.f;
+        nm.ias = A > 0 ? (float)(1.0 / (A * w)) : 0.f;
+        nm.ias2 = A > 0 ? (float)(1.0 / (A * w * w)) : 0.f;
+        b.mom[slot] = nm;
+    }
 }
 
 __device__ __forceinline__ int divided_rank(const NodeBuild& b, int a, int level) {
