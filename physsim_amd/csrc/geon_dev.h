@@ -223,7 +223,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
     if (ABERR) {
         const float n_dot_v = -(rx * svx + ry * svy) * inv_r;
         const float den = fmaxf(1.f - n_dot_v, 0.01f);
-        aberr = fminf(1.f / (den * den * den), 100.f);
+        aberr = fminf(__builtin_amdgcn_rcpf(den * den * den), 100.f);
     }
     const float inv_r3a = ABERR ? inv_r3 * aberr : inv_r3;
     const bool do_grav = (F & F_JERK) ? pf.grav_on != 0u : (F & F_GRAV) != 0u;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         const float inv_r5a = ABERR ? inv_r5 * aberr : inv_r5;
         const float cross_sv = svx * ry - svy * rx;
         if (pf.one_pn && (pf.gm || pf.mag)) {                       // forces.js:249-301
-            const float r = 1.f / inv_r;
+            const float r = r2 * inv_r;             // sqrt(r2) without a division
             const float nx = rx * inv_r, ny = ry * inv_r;
             const float v1v2 = o.vx * svx + o.vy * svy;
             if (pf.gm) {
@@ -322,7 +322,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         }
         if (pf.yuk && raw2 < pf.yuk_cut2) {                         // forces.js:389-427
             const float mu = pf.higgs ? pf.yuk_mu * sqrtf(o.higgs * s_higgs) : pf.yuk_mu;
-            const float r = 1.f / inv_r;
+            const float r = r2 * inv_r;             // sqrt(r2) without a division
             const float e = __expf(-mu * r);
             const float ym = pf.ax_mod ? sqrtf(o.yuk * s_yuk) : 1.f;
             const float gmm = pf.yuk_g * ym * o.m * sm * e;
@@ -364,7 +364,7 @@ __device__ __forceinline__ void accum_1pn(const Obs& o, float& f1x, float& f1y, 
     const float r2 = rx * rx + ry * ry + pf.soft2;
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float inv_r2 = inv_r * inv_r;
-    const float r = 1.f / inv_r;
+    const float r = r2 * inv_r;                 // sqrt(r2) without a division
     const float nx = rx * inv_r, ny = ry * inv_r;
     if (pf.gm) {
         const float v1sq = o.vx * o.vx + o.vy * o.vy, v2sq = svx * svx + svy * svy;
