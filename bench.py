@@ -44,7 +44,7 @@ BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 6)
 # HBM bytes of one k_far_mlp launch at the default workload (N=2^20, theta 0.28), from the
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (profiles/k_far_mlp_open1M_theta0.28.md)
-MLP_TRAFFIC_DEFAULT = 3150.7e6
+MLP_TRAFFIC_DEFAULT = 3115.7e6
 TIMER_WALK, TIMER_TREE, TIMER_SORT, TIMER_KICK, TIMER_MLP = 0, 1, 2, 3, 4
 
 
