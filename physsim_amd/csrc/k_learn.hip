@@ -417,7 +417,7 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const __bf16* xs, i
     }
 }
 
-__global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_far_mlp(FarArgs a) {
+__global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_far_mlp(FarArgs a) {
     __shared__ FarInfer wfr;
     __shared__ __attribute__((aligned(16))) __bf16 xs_all[kMlpWaves][kWave * 16];
     __shared__ FarRow rows_all[kMlpWaves][kWave];
