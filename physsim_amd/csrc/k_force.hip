@@ -242,9 +242,9 @@ __device__ __forceinline__ int64_t walk_group() { return (int64_t)blockIdx.x * k
 
 template <uint32_t F, bool KAHAN>
 // At least five waves per SIMD: the walks are latency-bound.  At 102 VGPRs the
-// heaviest instantiations spill at most 24 bytes per lane, outside the pair
-// loop; the multipole walk then runs 10% faster at N = 2^20.  The list walk
-// spills more and stays at four.
+// heaviest instantiations spill up to 24 bytes per lane and reload a few dwords
+// inside the walk loop; the multipole walk still runs 10% faster at N = 2^20.
+// The list walk was slower that way and stays at four.
 __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_walk(ForceArgs a) { walk_body<F, KAHAN, 0>(a, walk_group()); }
 template <uint32_t F, bool KAHAN>
 __global__ __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_walk_multipole(ForceArgs a) {
