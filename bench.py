@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--theta-multipole", default="0.35,0.32,0.3,0.28,0.25")
     ap.add_argument("--theta-train", type=float, default=0.3)
     ap.add_argument("--err-target", type=float, default=1e-4)
-    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--train-steps", type=int, default=80000)
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--err-sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
