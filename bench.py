@@ -42,9 +42,9 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 # network flops per (particle, node) pair: 2 (11 x 64 + 64 x 64 + 64 x 6)
 MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 6)
-# HBM bytes of one k_far_mlp launch at the default workload (N=2^20, theta 0.28), from the
-# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (profiles/k_far_mlp_open1M_theta0.28.md)
-MLP_TRAFFIC_DEFAULT = 3115.7e6
+# HBM bytes of one k_far_mlp launch at the default workload (N=2^20, theta 0.30), from the
+# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (profiles/k_far_mlp_open1M_theta0.30.md)
+MLP_TRAFFIC_DEFAULT = 3003.0e6
 TIMER_WALK, TIMER_TREE, TIMER_SORT, TIMER_KICK, TIMER_MLP = 0, 1, 2, 3, 4
 
 
@@ -400,12 +400,12 @@ def main():
     L, P = res["learned"], res.get("plain")
     dominant_mlp = L["phases_ms_avg"]["mlp"] >= L["phases_ms_avg"]["walk"]
     if dominant_mlp:
-        default = n == 1 << 20 and abs(L["theta"] - 0.28) < 1e-9
+        default = n == 1 << 20 and abs(L["theta"] - 0.30) < 1e-9
         roof = {"bound": "mfma", "kernel": "k_far_mlp", "achieved": L["mlp_tflops"], "peak": BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": L["mlp_tflops"] / BF16_PEAK_TFLOPS,
                 "traffic": MLP_TRAFFIC_DEFAULT if default else None,
                 "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch (copy-calibrated), "
-                                  "profiles/k_far_mlp_open1M_theta0.28.md" if default else None,
+                                  "profiles/k_far_mlp_open1M_theta0.30.md" if default else None,
                 "note": "the vector ALU (features, hard-swish, output rotation) bounds this kernel, not MFMA",
                 "pairs_per_launch": L["node_interactions_per_particle"] * n, "flops_per_pair": MLP_FLOPS_PER_PAIR,
                 "ms_avg": L["phases_ms_avg"]["mlp"]}
