@@ -1059,8 +1059,10 @@ int merge_step(GeonContext* h, bool* changed) {
     int rc;
     if ((rc = merge_alloc(h))) return rc;
     if ((rc = sort_and_gather(h))) return rc;
+    // the tree the force pass would build at these positions (it walks this one
+    // when the pass changes nothing)
     build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
-               I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+               I(h, C_QQ), n, key_params(h->prm), h->ctrl, want_moments(h));
     launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
                            F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), nullptr, nullptr, nullptr, h->src4,
                            h->src8a, h->src8b, n, h->ctrl, /*update_moments=*/0);
@@ -1458,9 +1460,11 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
                 // After a change the force pass walks a fresh tree; the reference
                 // walks its step-5 tree, where removed bodies still sit (mass 0)
                 // and spawned ones are missing, for this one substep.
+                // Without a change the merge pass's tree, built at these
+                // positions, is the one the force pass would build.
                 bool changed = false;
                 if ((rc = merge_step(h, &changed))) return rc;
-                if (changed) reuse = false;
+                reuse = !changed;
             }
             if ((rc = force_pass(h, reuse))) return rc;
             sa = step_args(h);
