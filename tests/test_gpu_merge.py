@@ -218,6 +218,28 @@ def test_gravitational_collapse_with_merges(oracle_lib):
     assert total > 20
 
 
+def test_merges_with_multipole_walk(oracle_lib):
+    # Barnes-Hut with the analytic multipole terms: the merge pass builds its
+    # tree with the far-field moments, and an update whose merge pass changes
+    # nothing walks that same tree in its force pass
+    # a sparse box: the initial overlaps merge in the first update, then
+    # updates go by with no change at all
+    st = _state(400, 1, 256.0)
+    prm = _params(domain_w=256.0, domain_h=256.0, toggles0=A.GRAVITY | A.BARNES_HUT, far_field=A.FAR_MULTIPOLE,
+                  theta=0.5)
+    g, o = _pair(st, prm, oracle_lib)
+    total, quiet = 0, 0
+    for k in range(5):
+        assert g.update(DT) == o.update(DT), k
+        new = len(o.collision_events())           # the events of this update
+        total += new
+        quiet += new == 0
+        _compare_events(g, o)
+        _compare(g, o, tol=2e-4, spin_tol=2e-2)
+    print("events over 5 updates: %d (%d updates without any), %d particles left" % (total, quiet, o.count()))
+    assert total > 5 and quiet >= 2
+
+
 @pytest.mark.parametrize("bh", [False, True])
 def test_merges_under_one_pn(bh, oracle_lib):
     # the 1PN force stored for the next kick follows its particle through the
