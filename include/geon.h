@@ -43,7 +43,7 @@ extern "C" {
 #define GEON_ONE_PN         (1u << 4)
 #define GEON_RELATIVITY     (1u << 5)
 #define GEON_SPIN_ORBIT     (1u << 6)
-#define GEON_RADIATION      (1u << 7)   /* not run by this backend: rejected */
+#define GEON_RADIATION      (1u << 7)   /* radiation reaction; rejected together with GEON_YUKAWA */
 #define GEON_BLACK_HOLE     (1u << 8)
 #define GEON_DISINTEGRATION (1u << 9)   /* not run by this backend: rejected */
 #define GEON_EXPANSION      (1u << 10)
@@ -224,9 +224,12 @@ int  geon_patch_mass(GeonHandle h, int32_t slot, double mass, double base_mass);
 
 /* Bulk state transfer (the SoA the reference's serialize/deserialize read and
  * write, gpu-physics.js:3749-3836, 3886).  Upload replaces the whole system:
- * slots 0..n-1.  Download fills slots 0..n_slots-1 and writes n_slots. */
+ * slots 0..n-1, every given array n entries long.  Download fills slots
+ * 0..n_slots-1 of arrays `cap` entries long and writes n_slots; when
+ * cap < n_slots it writes only *n_slots and returns GEON_EINVAL, so a caller
+ * can size its arrays with a first call of cap 0. */
 int  geon_upload_soa(GeonHandle h, const GeonSoAView* v, int32_t n);
-int  geon_download_soa(GeonHandle h, GeonSoAView* v, int32_t* n_slots);
+int  geon_download_soa(GeonHandle h, GeonSoAView* v, int32_t cap, int32_t* n_slots);
 
 /* One Physics.update(particles, dt, ...) call (integrator.js:726-1825):
  * adaptive substeps (<= max_substeps), boundary handling once at the end. */
@@ -371,7 +374,7 @@ int  geon_batch_create(const GeonParams* params, int32_t n_systems, const int32_
 void geon_batch_destroy(GeonBatchHandle h);
 /* One system's particles, as geon_upload_soa. */
 int  geon_batch_upload(GeonBatchHandle h, int32_t system, const GeonSoAView* v, int32_t n);
-int  geon_batch_download(GeonBatchHandle h, int32_t system, GeonSoAView* v, int32_t* n_slots);
+int  geon_batch_download(GeonBatchHandle h, int32_t system, GeonSoAView* v, int32_t cap, int32_t* n_slots);
 /* One Physics.update(dt) of every system; substeps (n_systems entries, may be
  * NULL) receives each system's substep count. */
 int  geon_batch_update(GeonBatchHandle h, double dt, int32_t* substeps);

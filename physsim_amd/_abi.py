@@ -112,7 +112,7 @@ PROTOTYPES = {
     "geon_remove_particle": (C.c_int, [H, C.c_int32]),
     "geon_patch_mass": (C.c_int, [H, C.c_int32, C.c_double, C.c_double]),
     "geon_upload_soa": (C.c_int, [H, C.POINTER(GeonSoAView), C.c_int32]),
-    "geon_download_soa": (C.c_int, [H, C.POINTER(GeonSoAView), C.POINTER(C.c_int32)]),
+    "geon_download_soa": (C.c_int, [H, C.POINTER(GeonSoAView), C.c_int32, C.POINTER(C.c_int32)]),
     "geon_update": (C.c_int, [H, C.c_double, C.POINTER(C.c_int32)]),
     "geon_compute_forces": (C.c_int, [H]),
     "geon_get_forces": (C.c_int, [H, C.POINTER(GeonForceView), C.c_int32]),
@@ -143,7 +143,7 @@ PROTOTYPES = {
     "geon_batch_create": (C.c_int, [C.POINTER(GeonParams), C.c_int32, C.POINTER(C.c_int32), C.POINTER(H)]),
     "geon_batch_destroy": (None, [H]),
     "geon_batch_upload": (C.c_int, [H, C.c_int32, C.POINTER(GeonSoAView), C.c_int32]),
-    "geon_batch_download": (C.c_int, [H, C.c_int32, C.POINTER(GeonSoAView), C.POINTER(C.c_int32)]),
+    "geon_batch_download": (C.c_int, [H, C.c_int32, C.POINTER(GeonSoAView), C.c_int32, C.POINTER(C.c_int32)]),
     "geon_batch_update": (C.c_int, [H, C.c_double, C.POINTER(C.c_int32)]),
     "geon_batch_compute_forces": (C.c_int, [H]),
     "geon_batch_get_forces": (C.c_int, [H, C.c_int32, C.POINTER(GeonForceView), C.c_int32]),

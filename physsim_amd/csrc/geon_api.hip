@@ -1125,18 +1125,24 @@ int merge_step(GeonContext* h, bool* changed) {
     m.dom_w = h->prm.domain_w; m.dom_h = h->prm.domain_h;
     m.uf = h->mg_uf; m.uf_fail = h->mg_uf_fail;
     HIP_OK(merge_resolve(h->stream, m, h->mg_csort, h->mg_cctrl, np, n));
+    // The spawn count is known once the fold has run: a pass whose merged
+    // products would not fit the slot space fails before it changes any
+    // particle (merge_apply below is the first write to the particle columns).
+    HIP_OK(hipMemcpyAsync(cnt, h->mg_cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    const int n_spawn = cnt[0], n_ev = cnt[1] + cnt[2];
+    if (n_spawn > 0 && h->next_slot + n_spawn > h->cap)
+        return set_err(h, GEON_EFULL, "slot space exhausted: merged particles need new slots (capacity bounds all slots ever used)");
     merge_apply(h->stream, m, F(h, C_MASS), F(h, C_BASEMASS), F(h, C_RADIUS), h->alive, n);
     HIP_OK(hipMemcpyAsync(cnt, h->mg_cnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
-    const int n_spawn = cnt[0], n_ev = cnt[1] + cnt[2], deaths = cnt[3];
+    const int deaths = cnt[3];
     if (n_ev > 0) {
         const size_t old = h->events_host.size();
         h->events_host.resize(old + 6 * (size_t)n_ev);
         HIP_OK(hipMemcpyAsync(h->events_host.data() + old, h->mg_events, sizeof(double) * 6 * n_ev,
                               hipMemcpyDeviceToHost, h->stream));
     }
-    if (n_spawn > 0 && h->next_slot + n_spawn > h->cap)
-        return set_err(h, GEON_EFULL, "slot space exhausted: merged particles need new slots (capacity bounds all slots ever used)");
     if (deaths > 0 && (rc = compact(h, t_now))) return rc;
     if (n_spawn > 0) {
         SpawnArgs sa;
@@ -1385,9 +1391,11 @@ int geon_patch_mass(GeonHandle h, int32_t slot, double mass, double base_mass) {
     return GEON_OK;
 }
 
-int geon_download_soa(GeonHandle h, GeonSoAView* v, int32_t* n_slots) {
+int geon_download_soa(GeonHandle h, GeonSoAView* v, int32_t cap, int32_t* n_slots) {
     if (!h || !v) return GEON_EINVAL;
     const int n = h->n_host, ns = h->next_slot;
+    if (n_slots) *n_slots = ns;
+    if (cap < ns) return set_err(h, GEON_EINVAL, "download arrays hold fewer entries than the slots in use");
     std::vector<std::vector<uint32_t>> col(N_STATE, std::vector<uint32_t>((size_t)n));
     for (int c = 0; c < N_STATE; c++)
         if (n) HIP_OK(hipMemcpyAsync(col[c].data(), h->st[c], 4 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
@@ -1438,7 +1446,15 @@ int geon_update(GeonHandle h, double dt, int32_t* substeps_taken) {
     StepArgs sa = step_args(h);
     const double soft = (h->prm.toggles0 & GEON_BLACK_HOLE) ? h->prm.bh_softening : h->prm.softening;
     int steps = 0;
-    if (n > 0) {
+    const bool active = dt > 1e-12;     // the substep loop runs while dtRemain > EPSILON (integrator.js:810)
+    if (active && n == 0) {
+        // No particles: dtSafe = dtRemain gives one substep of the whole dt, so
+        // simTime advances and the fields still step (integrator.js:830-836,
+        // 1294, 1320-1334).
+        launch_ctrl(h->stream, h->ctrl, soft, 1);
+        if (fields_on(h)) fields_step(h, 0.f);
+        steps = 1;
+    } else if (active) {
         launch_dt_scan(h->stream, sa);
         for (int k = 0; k < h->prm.max_substeps; k++) {
             launch_ctrl(h->stream, h->ctrl, soft, k == 0);

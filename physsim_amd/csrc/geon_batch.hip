@@ -517,9 +517,11 @@ int geon_batch_upload(GeonBatchHandle h, int32_t system, const GeonSoAView* v, i
     return GEON_OK;
 }
 
-int geon_batch_download(GeonBatchHandle h, int32_t system, GeonSoAView* v, int32_t* n_slots) {
+int geon_batch_download(GeonBatchHandle h, int32_t system, GeonSoAView* v, int32_t out_cap, int32_t* n_slots) {
     if (!h || !v || system < 0 || system >= h->S) return GEON_EINVAL;
     const int cap = h->cap[system], off = h->soff[system], ns = h->n_up[system];
+    if (n_slots) *n_slots = ns;
+    if (out_cap < ns) { h->err = "download arrays hold fewer entries than the system's slots"; return GEON_EINVAL; }
     std::vector<std::vector<uint32_t>> col(NB_STATE, std::vector<uint32_t>((size_t)cap));
     for (int c = 0; c < NB_STATE; c++)
         HIP_OK(hipMemcpyAsync(col[c].data(), h->st[c] + off, 4 * (size_t)cap, hipMemcpyDeviceToHost, h->stream));

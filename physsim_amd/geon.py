@@ -53,6 +53,13 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
 
+def _check_lengths(n, arrays, names):
+    """Every given column of an upload holds exactly n entries (the C ABI reads n from each)."""
+    for a, name in zip(arrays, names):
+        if a is not None and len(a) != n:
+            raise ValueError("upload column %s has %d entries, x has %d" % (name, len(a), n))
+
+
 class HipPhysics:
     """Third physics backend beside the reference's CPUPhysics and GPUPhysics."""
 
@@ -121,10 +128,11 @@ class HipPhysics:
         self._check(self.lib.geon_patch_mass(self.h, int(idx), float(mass), float(base_mass)))
 
     def upload(self, x, y, wx, wy, mass, charge, angw=None, base_mass=None, antimatter=None):
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() if a is not None else None
                 for a in (x, y, wx, wy, mass, base_mass, charge, angw)]
         n = len(arrs[0])
-        anti = np.ascontiguousarray(antimatter, dtype=np.int32) if antimatter is not None else None
+        anti = np.ascontiguousarray(antimatter, dtype=np.int32).ravel() if antimatter is not None else None
+        _check_lengths(n, arrs + [anti], ("x", "y", "wx", "wy", "mass", "base_mass", "charge", "angw", "antimatter"))
         v = A.GeonSoAView(*[_dptr(a) for a in arrs], None,
                           anti.ctypes.data_as(C.POINTER(C.c_int32)) if anti is not None else None)
         self._check(self.lib.geon_upload_soa(self.h, C.byref(v), n))
@@ -139,7 +147,7 @@ class HipPhysics:
         ip = C.POINTER(C.c_int32)
         v = A.GeonSoAView(*[_dptr(out[k]) for k in ("x", "y", "wx", "wy", "mass", "base_mass", "charge", "angw")],
                           alive.ctypes.data_as(ip), anti.ctypes.data_as(ip))
-        self._check(self.lib.geon_download_soa(self.h, C.byref(v), C.byref(ns)))
+        self._check(self.lib.geon_download_soa(self.h, C.byref(v), cap, C.byref(ns)))
         n = ns.value
         res = {k: a[:n] for k, a in out.items()}
         res["alive"] = alive[:n].astype(bool)
@@ -383,8 +391,9 @@ class HipBatch:
         return rc
 
     def upload(self, system, x, y, wx, wy, mass, charge, angw=None):
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() if a is not None else None
                 for a in (x, y, wx, wy, mass, None, charge, angw)]
+        _check_lengths(len(arrs[0]), arrs, ("x", "y", "wx", "wy", "mass", "base_mass", "charge", "angw"))
         v = A.GeonSoAView(*[_dptr(a) for a in arrs], None)
         self._check(self.lib.geon_batch_upload(self.h, int(system), C.byref(v), len(arrs[0])))
         self._keep = arrs
@@ -396,7 +405,7 @@ class HipBatch:
         v = A.GeonSoAView(_dptr(out["x"]), _dptr(out["y"]), _dptr(out["wx"]), _dptr(out["wy"]), _dptr(out["mass"]),
                           None, _dptr(out["charge"]), _dptr(out["angw"]), alive.ctypes.data_as(C.POINTER(C.c_int32)))
         ns = C.c_int32()
-        self._check(self.lib.geon_batch_download(self.h, int(system), C.byref(v), C.byref(ns)))
+        self._check(self.lib.geon_batch_download(self.h, int(system), C.byref(v), cap, C.byref(ns)))
         n = ns.value
         res = {k: a[:n] for k, a in out.items()}
         res["alive"] = alive[:n].astype(bool)

@@ -74,7 +74,7 @@ export default class HipPhysics {
 
     uploadSoA(arrays) { addon.uploadSoA(this._h, arrays); }
 
-    downloadSoA() { return addon.downloadSoA(this._h, this._max); }
+    downloadSoA() { return addon.downloadSoA(this._h); }
 
     computeForces() { addon.computeForces(this._h); }
 

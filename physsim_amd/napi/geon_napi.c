@@ -190,55 +190,64 @@ static napi_value js_patch_mass(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
-static double* f64_of(napi_env env, napi_value obj, const char* key, size_t* len) {
+/* A named typed-array column of `obj`: 0 absent (or undefined), 1 present with
+ * element type `want` (data and length filled), -1 present but of another kind. */
+static int column_of(napi_env env, napi_value obj, const char* key, napi_typedarray_type want, void** data,
+                     size_t* len) {
     bool has = false;
-    if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return NULL;
+    *data = NULL;
+    *len = 0;
+    if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return 0;
     napi_value v;
     napi_get_named_property(env, obj, key, &v);
+    napi_valuetype vt;
+    napi_typeof(env, v, &vt);
+    if (vt == napi_undefined) return 0;
     bool is_ta = false;
     napi_is_typedarray(env, v, &is_ta);
-    if (!is_ta) return NULL;
+    if (!is_ta) return -1;
     napi_typedarray_type type;
-    void* data;
     napi_value ab;
     size_t off;
-    napi_get_typedarray_info(env, v, &type, len, &data, &ab, &off);
-    return type == napi_float64_array ? (double*)data : NULL;
+    napi_get_typedarray_info(env, v, &type, len, data, &ab, &off);
+    return type == want ? 1 : -1;
 }
 
-static int32_t* i32_of(napi_env env, napi_value obj, const char* key, size_t* len) {
-    bool has = false;
-    if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return NULL;
-    napi_value v;
-    napi_get_named_property(env, obj, key, &v);
-    bool is_ta = false;
-    napi_is_typedarray(env, v, &is_ta);
-    if (!is_ta) return NULL;
-    napi_typedarray_type type;
-    void* data;
-    napi_value ab;
-    size_t off;
-    napi_get_typedarray_info(env, v, &type, len, &data, &ab, &off);
-    return type == napi_int32_array ? (int32_t*)data : NULL;
-}
-
-/* uploadSoA(h, {x, y, wx, wy, mass, charge, angw?, baseMass?} Float64Arrays, antimatter? Int32Array) */
+/* uploadSoA(h, {x, y, wx, wy, mass, charge, angw?, baseMass?} Float64Arrays, antimatter? Int32Array).
+ * Every column given must be a typed array of x's length: the C ABI reads n
+ * entries from each. */
 static napi_value js_upload(napi_env env, napi_callback_info info) {
     napi_value argv[2];
     if (!args_n(env, info, 2, argv)) return NULL;
     GeonHandle h = get_handle(env, argv[0]);
-    size_t n = 0, k;
     GeonSoAView v;
     memset(&v, 0, sizeof(v));
-    v.x = f64_of(env, argv[1], "x", &n);
-    v.y = f64_of(env, argv[1], "y", &k);
-    v.wx = f64_of(env, argv[1], "wx", &k);
-    v.wy = f64_of(env, argv[1], "wy", &k);
-    v.mass = f64_of(env, argv[1], "mass", &k);
-    v.charge = f64_of(env, argv[1], "charge", &k);
-    v.angw = f64_of(env, argv[1], "angw", &k);
-    v.base_mass = f64_of(env, argv[1], "baseMass", &k);
-    v.antimatter = i32_of(env, argv[1], "antimatter", &k);
+    const char* names[] = {"x", "y", "wx", "wy", "mass", "charge", "angw", "baseMass"};
+    double** ptrs[] = {&v.x, &v.y, &v.wx, &v.wy, &v.mass, &v.charge, &v.angw, &v.base_mass};
+    size_t n = 0;
+    char msg[160];
+    for (int i = 0; i < 9; i++) {
+        const int required = i < 6;
+        const char* key = i < 8 ? names[i] : "antimatter";
+        void* data;
+        size_t len;
+        const int r = column_of(env, argv[1], key, i < 8 ? napi_float64_array : napi_int32_array, &data, &len);
+        if (r < 0 || (r == 0 && required)) {
+            snprintf(msg, sizeof(msg), "uploadSoA: column '%s' must be %s%s", key,
+                     i < 8 ? "a Float64Array" : "an Int32Array", required ? "" : " when given");
+            napi_throw_type_error(env, NULL, msg);
+            return NULL;
+        }
+        if (i == 0) n = len;
+        if (r == 1 && len != n) {
+            snprintf(msg, sizeof(msg), "uploadSoA: column '%s' has %zu entries, x has %zu", key, len, n);
+            napi_throw_range_error(env, NULL, msg);
+            return NULL;
+        }
+        if (i < 8) *ptrs[i] = (double*)data;
+        else v.antimatter = (int32_t*)data;
+    }
+    if (n > 0x7fffffff) { napi_throw_range_error(env, NULL, "uploadSoA: too many particles"); return NULL; }
     int rc = geon_upload_soa(h, &v, (int32_t)n);
     if (rc) return throw_geon(env, h, rc);
     return NULL;
@@ -251,17 +260,19 @@ static napi_value new_f64(napi_env env, size_t n, double** data) {
     return ta;
 }
 
-/* downloadSoA(h, cap) -> {x, y, wx, wy, mass, baseMass, charge, angw, alive: Int32Array} */
+/* downloadSoA(h) -> {x, y, wx, wy, mass, baseMass, charge, angw, alive, antimatter, nSlots}: arrays of
+ * one entry per slot in use. */
 static napi_value js_download(napi_env env, napi_callback_info info) {
-    napi_value argv[2];
-    if (!args_n(env, info, 2, argv)) return NULL;
+    napi_value argv[1];
+    if (!args_n(env, info, 1, argv)) return NULL;
     GeonHandle h = get_handle(env, argv[0]);
-    int32_t cap;
-    CHECK(napi_get_value_int32(env, argv[1], &cap));
-    napi_value out;
-    napi_create_object(env, &out);
     GeonSoAView v;
     memset(&v, 0, sizeof(v));
+    int32_t cap = 0;
+    int rc = geon_download_soa(h, &v, 0, &cap);     /* sizes the arrays: writes only the slot count */
+    if (rc && rc != GEON_EINVAL) return throw_geon(env, h, rc);
+    napi_value out;
+    napi_create_object(env, &out);
     const char* names[] = {"x", "y", "wx", "wy", "mass", "baseMass", "charge", "angw"};
     double** ptrs[] = {&v.x, &v.y, &v.wx, &v.wy, &v.mass, &v.base_mass, &v.charge, &v.angw};
     for (int i = 0; i < 8; i++) napi_set_named_property(env, out, names[i], new_f64(env, (size_t)cap, ptrs[i]));
@@ -273,7 +284,7 @@ static napi_value js_download(napi_env env, napi_callback_info info) {
     napi_create_typedarray(env, napi_int32_array, (size_t)cap, ab, 0, &ta);
     napi_set_named_property(env, out, "antimatter", ta);
     int32_t ns = 0;
-    int rc = geon_download_soa(h, &v, &ns);
+    rc = geon_download_soa(h, &v, cap, &ns);
     if (rc) return throw_geon(env, h, rc);
     napi_set_named_property(env, out, "nSlots", i32(env, ns));
     return out;

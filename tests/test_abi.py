@@ -66,3 +66,14 @@ def test_create_without_gpu_fails_loudly(lib_built):
     from physsim_amd import HipPhysics, GeonError
     with pytest.raises(GeonError):
         HipPhysics(max_particles=16)
+
+
+def test_upload_rejects_ragged_columns():
+    """Every upload column must hold as many entries as x: the C ABI reads n
+    entries from each, so a shorter column would be read past its end."""
+    from physsim_amd.geon import _check_lengths
+    import numpy as np
+    names = ("x", "y", "mass")
+    _check_lengths(4, [np.zeros(4), np.zeros(4), None], names)
+    with pytest.raises(ValueError, match="column y has 3 entries"):
+        _check_lengths(4, [np.zeros(4), np.zeros(3), np.zeros(4)], names)

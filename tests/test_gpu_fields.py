@@ -123,3 +123,28 @@ def test_klein_gordon_mode_on_gpu():
     t = dt * steps
     want = A0 * np.exp(-gam * t / 2) * (np.cos(wd * t) + gam / (2 * wd) * np.sin(wd * t))
     assert amp == pytest.approx(want, rel=2e-3)
+
+
+def test_empty_system_still_steps_fields(oracle_lib):
+    """With no particles an update is one substep of the whole dt
+    (integrator.js:810-836: dtSafe = dtRemain), so simTime advances and the
+    Higgs field keeps evolving (1294, 1320-1328)."""
+    from physsim_amd import HipPhysics, default_params
+    G = 32
+    prm = dict(toggles0=A.GRAVITY | A.HIGGS, field_grid=G, domain_w=64.0, domain_h=64.0, boundary_mode=A.BOUND_LOOP)
+    r = np.random.default_rng(5)
+    h0 = W.f32(1.0 + W.gaussian_field(G, r, 0.02))
+    print("case: empty system, higgs G=%d" % G, flush=True)
+    g = HipPhysics(default_params(**prm), max_particles=16)
+    g.set_field(0, h0)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 16)
+    o.field_set(0, h0)
+    dt = 1.0 / 128
+    for _ in range(4):
+        assert g.update(dt) == o.update(dt) == 1
+    assert g.stats()["simTime"] == pytest.approx(4 * dt, rel=1e-12)
+    fd_g, fd_o = g.get_field(0, "field_dot"), o.field_get(0, "field_dot")
+    assert np.abs(fd_o).max() > 1e-4              # the field moved (it starts at rest)
+    assert _rel(fd_g, fd_o) < 5e-3
+    move_g, move_o = g.get_field(0, "field") - h0, o.field_get(0, "field") - h0
+    assert _rel(move_g, move_o) < 3e-2            # a change of ~1e-5 on values ~1 in FP32

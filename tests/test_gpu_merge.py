@@ -255,3 +255,42 @@ def test_merges_under_one_pn(bh, oracle_lib):
     _compare(g, o, tol=2e-4, spin_tol=2e-2)
     print("1PN bh=%d: %d particles left of 1000" % (bh, o.count()))
     assert o.count() < 990
+
+
+def test_merge_slot_space_exhausted_leaves_state_unchanged(oracle_lib):
+    """A merge pass whose products would need more slots than the capacity
+    fails with GEON_EFULL before it changes any particle: every body is still
+    alive with its mass, and the same call succeeds once the state is loaded
+    into a handle with room."""
+    from physsim_amd import HipPhysics, default_params, GeonError
+    st = _state(2048, 71, 128.0, clustered=True, mass=(1.0, 4.0))
+    prm = _params()
+    n = len(st["x"])
+    print("case: merge EFULL n=%d" % n, flush=True)
+    g = HipPhysics(default_params(**prm), max_particles=n)      # no spare slot
+    g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"], antimatter=st["anti"])
+    with pytest.raises(GeonError) as ei:
+        g.update(DT)
+    assert ei.value.code == A.EFULL
+    d = g.download()
+    assert d["alive"].all() and g.getParticleCount() == n
+    np.testing.assert_array_equal(d["mass"], f32(st["mass"]))
+    g.destroy()
+    g, o = _pair(st, prm, oracle_lib)
+    assert g.update(DT) == o.update(DT)
+    assert g.getParticleCount() < n
+    _compare(g, o)
+
+
+def test_merge_zero_dt_changes_nothing(oracle_lib):
+    """update(0) takes no substep (integrator.js:810: dtRemain > EPSILON), so
+    touching bodies neither merge nor annihilate."""
+    st = _state(2048, 72, 128.0, clustered=True, anti_frac=0.2)
+    g, o = _pair(st, _params(), oracle_lib)
+    n = len(st["x"])
+    assert g.update(0.0) == 0
+    assert g.getParticleCount() == n
+    d = g.download()
+    np.testing.assert_array_equal(d["x"], f32(st["x"]))
+    np.testing.assert_array_equal(d["mass"], f32(st["mass"]))
+    assert o.update(0.0) == 0 and o.count() == n

@@ -76,3 +76,20 @@ def test_node_harness_matches_python_path(tmp_path):
     assert np.array_equal(d["x"][alive], np.array([p["x"] for p in js["particles"]]))
     assert np.array_equal(d["wy"][alive], np.array([p["wy"] for p in js["particles"]]))
     assert js["stats"]["aliveCount"] == len(alive)
+
+
+def test_addon_upload_rejects_bad_columns():
+    """uploadSoA checks its columns before it reaches the library: a column
+    shorter than x, a plain Array, or a missing required column throws."""
+    _ensure_addon()
+    js = ("const g=require(%r); const f=n=>new Float64Array(n);"
+          "const cols=(n,m)=>({x:f(n),y:f(m),wx:f(n),wy:f(n),mass:f(n),charge:f(n)});"
+          "const tryit=(c)=>{try{g.uploadSoA(null,c);console.log('ok')}catch(e){console.log(e.name+': '+e.message)}};"
+          "tryit(cols(4,3)); const c=cols(4,4); c.x=[1,2,3,4]; tryit(c);"
+          "const d=cols(4,4); delete d.charge; tryit(d);"
+          "const e=cols(4,4); e.antimatter=new Int32Array(2); tryit(e);" % ADDON)
+    out = subprocess.run([NODE, "-e", js], capture_output=True, text=True, check=True).stdout.splitlines()
+    assert out[0].startswith("RangeError") and "'y' has 3 entries" in out[0]
+    assert out[1].startswith("TypeError") and "'x'" in out[1]
+    assert out[2].startswith("TypeError") and "'charge'" in out[2]
+    assert out[3].startswith("RangeError") and "'antimatter'" in out[3]
