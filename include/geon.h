@@ -296,6 +296,16 @@ int  geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse
  * (a, b) and the total to *n_pairs (a, b may be NULL to count only). */
 int  geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs);
 
+/* The same pair list with no device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its staging and sort
+ * buffers are one fixed device array per GPU (about 33 MB, held until the
+ * process ends), so a state with more than 2^20 overlapping pairs fails with
+ * GEON_EOVERFLOW and writes nothing; calls on one GPU that overlap in time
+ * (from several threads) share that array and race.  The pairs are sorted on
+ * the device; output as geon_collision_pairs. */
+int  geon_collision_pairs_noalloc(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs);
+
 /* Collision events of the last geon_update in collision mode merge
  * (collisions.js:62-92, integrator.js:1388-1415): merges that lost kinetic
  * energy (the field excitations' source), then annihilations (the photon

@@ -127,6 +127,8 @@ PROTOTYPES = {
     "geon_fft2d": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32]),
     "geon_collision_pairs": (C.c_int, [H, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64,
                                         C.POINTER(C.c_int64)]),
+    "geon_collision_pairs_noalloc": (C.c_int, [H, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64,
+                                        C.POINTER(C.c_int64)]),
     "geon_collision_events": (C.c_int, [H, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64)]),
     "geon_counters": (C.c_int, [H, C.POINTER(GeonCounters)]),
     "geon_timing_enable": (C.c_int, [H, C.c_int32]),

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include "../../include/geon.h"
 #include "geon_kernels.h"
@@ -49,6 +50,7 @@ thread_local std::string g_create_error;
 
 struct GeonContext {
     GeonParams prm;
+    unsigned char* na_scratch = nullptr;    // this device's copy of `scratch` (allocation-free pair list)
     int cap = 0;
     int n_host = 0;             // live particles (host mirror, exact between calls)
     int next_slot = 0;
@@ -751,6 +753,67 @@ int compact(GeonContext* h, double t_death) {
 }
 
 
+// ---- Allocation-free collision pair list ----------------------------------
+// Device scratch of geon_collision_pairs_noalloc: the pair staging, the packed
+// sort keys with their permutation, and the radix sort's histograms, for at
+// most kNoAllocPairs pairs (about 33 MB).  Every GPU the library runs on holds
+// one copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (na_scratch).  Calls that
+// overlap on one GPU share it and race.
+constexpr int64_t kNoAllocPairs = int64_t(1) << 20;
+constexpr size_t kNaCtrl = 0;
+constexpr size_t kNaPa = 256;
+constexpr size_t kNaPb = kNaPa + 4 * kNoAllocPairs;
+constexpr size_t kNaKeys = kNaPb + 4 * kNoAllocPairs;
+constexpr size_t kNaKeysAlt = kNaKeys + 8 * kNoAllocPairs;
+constexpr size_t kNaPerm = kNaKeysAlt + 8 * kNoAllocPairs;
+constexpr size_t kNaPermAlt = kNaPerm + 4 * kNoAllocPairs;
+constexpr size_t kNaHist = kNaPermAlt + 4 * kNoAllocPairs;
+constexpr size_t kNaDigit = kNaHist + 4 * 256 * ((kNoAllocPairs + kSortTileKeys - 1) / kSortTileKeys);
+constexpr size_t kNaBytes = kNaDigit + 4 * 256;
+static_assert(sizeof(Ctrl) <= kNaPa, "controller record fits its scratch slot");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mutex;
+unsigned char* g_na_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+int noalloc_init(GeonContext* h) {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_err(h, GEON_EDEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(h, GEON_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_na_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    h->na_scratch = g_na_ptr[dev];
+    return GEON_OK;
+}
+
+// (a, b) slot pairs -> keys a << bits | b, so one radix sort orders them by (a, b).
+__global__ void k_pack_pairs(const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, uint64_t* __restrict__ keys,
+                             uint32_t* __restrict__ perm, int bits, const Ctrl* __restrict__ c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c->n_alive) return;
+    keys[i] = ((uint64_t)(uint32_t)pa[i] << bits) | (uint32_t)pb[i];
+    perm[i] = (uint32_t)i;
+}
+
+__global__ void k_split_pairs(const uint64_t* __restrict__ keys, int32_t* __restrict__ pa, int32_t* __restrict__ pb,
+                              int bits, const Ctrl* __restrict__ c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c->n_alive) return;
+    const uint64_t k = keys[i];
+    pa[i] = (int32_t)(k >> bits);
+    pb[i] = (int32_t)(k & ((uint64_t(1) << bits) - 1));
+}
+
 int allocate(GeonContext* h) {
     const size_t cap = (size_t)h->cap;
     int rc;
@@ -1184,6 +1247,42 @@ int merge_step(GeonContext* h, bool* changed) {
     return GEON_OK;
 }
 
+// The counting half of the collision pair list (collisions.js:35-60,
+// quadtree.js:372-404): the tree at the current positions, each particle's
+// pair count and their exclusive scan.  *pa is then ready for the writing pass
+// but for its output arrays; *np is the total.
+int pairs_count(GeonContext* h, PairArgs* pa_out, int64_t* np) {
+    *np = 0;
+    if (h->prm.boundary_mode == GEON_BOUND_LOOP && h->prm.topology != GEON_TORUS)
+        return set_err(h, GEON_EUNSUPPORTED, "the pair list runs on open and torus domains");
+    const int n = h->n_host;
+    if (n == 0) return GEON_OK;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
+    int rc = sort_and_gather(h);
+    if (rc) return rc;
+    build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
+               I(h, C_QQ), n, key_params(h->prm), h->ctrl);
+    h->tree_valid = true;
+    launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
+                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), nullptr, nullptr, nullptr, h->src4,
+                           h->src8a, h->src8b, n, h->ctrl, /*update_moments=*/0);
+    PairArgs& pa = *pa_out;
+    pa.x = F(h, C_X); pa.y = F(h, C_Y); pa.mass = F(h, C_MASS); pa.radius = F(h, C_RADIUS);
+    pa.slot = I(h, C_SLOT); pa.src4 = h->src4; pa.nodes = h->tb.nodes; pa.max_nodes = 1 + 4 * h->tb.cap_divided;
+    pa.count = h->tb.owned; pa.offset = h->tb.rank_start; pa.pa = nullptr; pa.pb = nullptr;
+    pa.pl = nullptr; pa.index_out = 0; pa.slack = 1.f;
+    pa.dom_w = (float)h->prm.domain_w; pa.dom_h = (float)h->prm.domain_h;
+    pa.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
+    pa.ctrl = h->ctrl; pa.n_upper = n;
+    launch_pairs(h->stream, pa, false);
+    int32_t* total = &h->ctrl->n_divided;     // scratch: rebuilt by the next tree build
+    exclusive_scan_i32(h->stream, h->tb.owned, h->tb.rank_start, n, &h->ctrl->n_alive, total, h->tb.scan_tmp);
+    if ((rc = read_ctrl(h))) return rc;
+    *np = h->ctrl_host->n_divided;
+    h->tree_valid = false;                    // n_divided no longer describes the tree
+    return GEON_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1224,6 +1323,7 @@ int geon_create(const GeonParams* params, int32_t max_particles, GeonHandle* out
         return GEON_EDEVICE;
     }
     int rc = allocate(h);
+    if (!rc) rc = noalloc_init(h);
     if (rc) { g_create_error = h->err; release(h); delete h; return rc; }
     reset_fields(h);
     if (hipStreamSynchronize(h->stream) != hipSuccess) {
@@ -1774,34 +1874,11 @@ int geon_fft2d(GeonHandle h, double* re, double* im, int32_t n, int32_t inverse)
 
 int geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs) {
     if (!h || (cap > 0 && (!a || !b))) return GEON_EINVAL;
-    if (h->prm.boundary_mode == GEON_BOUND_LOOP && h->prm.topology != GEON_TORUS)
-        return set_err(h, GEON_EUNSUPPORTED, "the pair list runs on open and torus domains");
-    const int n = h->n_host;
     if (n_pairs) *n_pairs = 0;
-    if (n == 0) return GEON_OK;
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
-    int rc = sort_and_gather(h);
-    if (rc) return rc;
-    build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
-               I(h, C_QQ), n, key_params(h->prm), h->ctrl);
-    h->tree_valid = true;
-    launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
-                           F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), nullptr, nullptr, nullptr, h->src4,
-                           h->src8a, h->src8b, n, h->ctrl, /*update_moments=*/0);
     PairArgs pa;
-    pa.x = F(h, C_X); pa.y = F(h, C_Y); pa.mass = F(h, C_MASS); pa.radius = F(h, C_RADIUS);
-    pa.slot = I(h, C_SLOT); pa.src4 = h->src4; pa.nodes = h->tb.nodes; pa.max_nodes = 1 + 4 * h->tb.cap_divided;
-    pa.count = h->tb.owned; pa.offset = h->tb.rank_start; pa.pa = nullptr; pa.pb = nullptr;
-    pa.pl = nullptr; pa.index_out = 0; pa.slack = 1.f;
-    pa.dom_w = (float)h->prm.domain_w; pa.dom_h = (float)h->prm.domain_h;
-    pa.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
-    pa.ctrl = h->ctrl; pa.n_upper = n;
-    launch_pairs(h->stream, pa, false);
-    int32_t* total = &h->ctrl->n_divided;     // scratch: rebuilt by the next tree build
-    exclusive_scan_i32(h->stream, h->tb.owned, h->tb.rank_start, n, &h->ctrl->n_alive, total, h->tb.scan_tmp);
-    if ((rc = read_ctrl(h))) return rc;
-    const int64_t np = h->ctrl_host->n_divided;
-    h->tree_valid = false;                    // n_divided no longer describes the tree
+    int64_t np = 0;
+    int rc = pairs_count(h, &pa, &np);
+    if (rc) return rc;
     if (n_pairs) *n_pairs = np;
     if (np == 0 || cap <= 0) return GEON_OK;
     int32_t *da = nullptr, *db = nullptr;
@@ -1820,6 +1897,47 @@ int geon_collision_pairs(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int6
     std::sort(keys.begin(), keys.end());
     const int64_t m = std::min(cap, np);
     for (int64_t k = 0; k < m; k++) { a[k] = (int32_t)(keys[k] >> 32); b[k] = (int32_t)(keys[k] & 0xffffffff); }
+    return GEON_OK;
+}
+
+int geon_collision_pairs_noalloc(GeonHandle h, int32_t* a, int32_t* b, int64_t cap, int64_t* n_pairs) {
+    if (!h || (cap > 0 && (!a || !b))) return GEON_EINVAL;
+    if (!h->na_scratch) return set_err(h, GEON_ESTATE, "device scratch not set up");
+    PairArgs pa;
+    int64_t np = 0;
+    int rc = pairs_count(h, &pa, &np);
+    if (rc) return rc;
+    if (np > kNoAllocPairs)
+        return set_err(h, GEON_EOVERFLOW, "allocation-free pair list: " + std::to_string(np) +
+                                              " pairs exceed its fixed scratch of " + std::to_string(kNoAllocPairs));
+    if (np == 0 || cap <= 0) {
+        if (n_pairs) *n_pairs = np;
+        return GEON_OK;
+    }
+    unsigned char* const sc = h->na_scratch;
+    Ctrl* const nc = reinterpret_cast<Ctrl*>(sc + kNaCtrl);
+    pa.pa = reinterpret_cast<int32_t*>(sc + kNaPa);
+    pa.pb = reinterpret_cast<int32_t*>(sc + kNaPb);
+    launch_pairs(h->stream, pa, true);
+    hipLaunchKernelGGL(k_set_n, dim3(1), dim3(1), 0, h->stream, nc, (int)np);
+    TreeBufs sb = {};
+    sb.keys = reinterpret_cast<uint64_t*>(sc + kNaKeys);
+    sb.keys_alt = reinterpret_cast<uint64_t*>(sc + kNaKeysAlt);
+    sb.perm = reinterpret_cast<uint32_t*>(sc + kNaPerm);
+    sb.perm_alt = reinterpret_cast<uint32_t*>(sc + kNaPermAlt);
+    sb.hist = reinterpret_cast<uint32_t*>(sc + kNaHist);
+    sb.digit_total = reinterpret_cast<uint32_t*>(sc + kNaDigit);
+    int bits = 1;                                   // slot numbers are < next_slot
+    while (bits < 31 && (int64_t(1) << bits) < (int64_t)h->next_slot) bits++;
+    const dim3 grid((unsigned)((np + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_pack_pairs, grid, block, 0, h->stream, pa.pa, pa.pb, sb.keys, sb.perm, bits, nc);
+    radix_sort(h->stream, sb, (int)np, 2 * bits, nc);
+    hipLaunchKernelGGL(k_split_pairs, grid, block, 0, h->stream, sb.keys, pa.pa, pa.pb, bits, nc);
+    const int64_t m = std::min(cap, np);
+    HIP_OK(hipMemcpyAsync(a, pa.pa, 4 * (size_t)m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(b, pa.pb, 4 * (size_t)m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (n_pairs) *n_pairs = np;
     return GEON_OK;
 }
 

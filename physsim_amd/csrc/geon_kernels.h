@@ -59,6 +59,7 @@ void launch_keys(hipStream_t s, const float* x, const float* y, int n, KeyParams
                  uint64_t* keys, uint32_t* perm, Ctrl* ctrl);
 // Sorts keys/perm in place (ping-pong internally); n is a host upper bound,
 // the exact count is ctrl->n_alive.
+constexpr int kSortTileKeys = 2048;   // keys per radix-sort block: hist holds 256 words per block
 void radix_sort(hipStream_t s, TreeBufs& tb, int n_upper, int key_bits, const Ctrl* ctrl);
 void launch_gather(hipStream_t s, const Columns& cols, const uint32_t* perm, int n_upper, const Ctrl* ctrl);
 void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, const float* m,

@@ -29,6 +29,7 @@ namespace geon {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;   // 2048 keys per block
+static_assert(kSortTile == kSortTileKeys, "radix sort tile");
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 

@@ -256,6 +256,21 @@ class HipPhysics:
                                                   C.byref(n)))
         return a[:n.value], b[:n.value]
 
+    def collision_pairs_noalloc(self, cap=None):
+        """The same list through the entry point that allocates no device memory
+        (at most 2^20 pairs); cap limits the pairs returned."""
+        n = C.c_int64()
+        if cap is None:
+            self._check(self.lib.geon_collision_pairs_noalloc(self.h, None, None, 0, C.byref(n)))
+            cap = n.value
+        a = np.zeros(max(cap, 1), dtype=np.int32)
+        b = np.zeros(max(cap, 1), dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
+        self._check(self.lib.geon_collision_pairs_noalloc(self.h, a.ctypes.data_as(ip), b.ctypes.data_as(ip), int(cap),
+                                                          C.byref(n)))
+        m = min(cap, n.value)
+        return a[:m], b[:m], n.value
+
     def collision_events(self):
         """Events of the last update in merge mode: rows {type (1 merge, 2
         annihilation), x, y, energy, px, py}, merges first (collisions.js:62-92)."""
