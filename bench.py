@@ -68,6 +68,32 @@ def flops_per_interaction(toggles):
     return c
 
 
+def host_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    limited by OMP_NUM_THREADS where that is set (the host share of one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cores_available": avail,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "threads_used": host_threads()}
+
+
 def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
@@ -289,6 +315,7 @@ def main():
     ap.add_argument("--train-seeds", type=int, default=16)
     ap.add_argument("--err-sample", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
+    ap.add_argument("--cpu-sample-1core", type=int, default=1 << 14)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="energy-drift leg: updates of a virialised Plummer sphere with the learned method; 0 skips")
@@ -307,7 +334,7 @@ def main():
 
     from physsim_amd import HipPhysics, default_params, workloads as W, _abi as A
     from oracle import bind
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     bind.set_threads(threads)
     n = args.n
     st, prm, dt = W.config2(n, open_box=True)
@@ -426,22 +453,32 @@ def main():
     cpu = None
     if not args.no_cpu:
         # ---- CPU baseline: the reference algorithm in FP64 on this host ----
-        cs, cprm, _ = W.config2(args.cpu_sample, open_box=True)
         theta_c = P["theta"] if P else 0.04
-        cprm["theta"] = theta_c
-        oc = bind.Oracle(bind.default_params(**cprm), args.cpu_sample)
-        oc.upload(cs["x"], cs["y"], cs["wx"], cs["wy"], cs["mass"], cs["charge"], cs["angw"])
-        tc = time.perf_counter()
-        oc.compute_forces()                    # bootstrap force pass (part of the first update)
-        tb = time.perf_counter()
-        steps_c = oc.update(dt)
-        tc2 = time.perf_counter()
-        cpu = {"value": steps_c * oc.count() / (tc2 - tb), "unit": "particle-steps/s", "cores": threads,
-               "kind": "port",
+
+        def cpu_rate(nc, nthreads):
+            cs, cprm, _ = W.config2(nc, open_box=True)
+            cprm["theta"] = theta_c
+            bind.set_threads(nthreads)
+            oc = bind.Oracle(bind.default_params(**cprm), nc)
+            oc.upload(cs["x"], cs["y"], cs["wx"], cs["wy"], cs["mass"], cs["charge"], cs["angw"])
+            tc = time.perf_counter()
+            oc.compute_forces()                # bootstrap force pass (part of the first update)
+            tb = time.perf_counter()
+            steps_c = oc.update(dt)
+            tc2 = time.perf_counter()
+            return steps_c * oc.count() / (tc2 - tb), steps_c, tc2 - tb, tb - tc
+
+        rate, steps_c, t_upd, t_boot = cpu_rate(args.cpu_sample, threads)
+        rate1, steps1, t_upd1, _ = cpu_rate(args.cpu_sample_1core, 1)
+        bind.set_threads(threads)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": "FP64 oracle (integrator.js/forces.js/quadtree.js restated in C, OpenMP over particles), "
                          "plain monopole BH at theta %.2f, same dynamics at N=%d (same density), one update of %d "
                          "substeps, %.1f s; bootstrap force pass %.1f s untimed"
-                         % (theta_c, args.cpu_sample, steps_c, tc2 - tb, tb - tc)}
+                         % (theta_c, args.cpu_sample, steps_c, t_upd, t_boot),
+               "host": host_info(),
+               "one_core": {"value": rate1, "unit": "particle-steps/s", "cores": 1, "n": args.cpu_sample_1core,
+                            "seconds": round(t_upd1, 2), "substeps": steps1}}
 
     line = {
         "metric": "particle-steps/sec at N=1M, plain vs learned-correction BH at force err <=1e-4",
@@ -454,7 +491,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (forces, Kahan), bf16 MFMA with f32 accumulation (network)",
+        "dtype": "f32 (forces, Kahan-compensated sums); network: bf16 layer 1, fp16 layers 2-3, f32 accumulation",
         "data": "synthetic (Philox seed 2000: uniform open box, lognormal masses, 0.1-quantized charges)",
         "config": {"workload": "open 1024x1024 box, N=%d uniform, gravity+Coulomb+Yukawa(mu=1)+magnetic dipole, "
                                "relativistic, spins U[-1,1], softening 8, dt 1/128 per update; learned BH at "

@@ -354,7 +354,9 @@ int  geon_far_get_model(GeonHandle h, float* out, int32_t n);
 int  geon_far_set_model(GeonHandle h, const float* in, int32_t n);
 /* Kernel timing: average device duration (ms) over the launches of `kernel`
  * since the last reset, measured with HIP events on the handle's stream.
- * kernel: 0 = force walk / direct sum, 1 = tree build, 2 = sort, 3 = integrator. */
+ * kernel: 0 = force walk / direct sum, 1 = tree build, 2 = sort, 3 = integrator,
+ * 4 = far-field network pass, 5 = field deposits, 6 = field KDK steps (with
+ * their self-gravity FFTs), 7 = field -> particle kernels. */
 int  geon_timing_enable(GeonHandle h, int32_t on);
 int  geon_kernel_timing(GeonHandle h, int32_t kernel, double* total_ms, int64_t* launches);
 

@@ -125,6 +125,7 @@ struct OrcSys {
     OField higgsF, axionF;
     double* farModel;              /* learned far-field model (FAR_MODEL doubles) or NULL */
     int delay;                     /* signal delay in effect */
+    int delay1pn;                  /* compute1PN takes leaf sources from the light cone */
     int histStride;                /* _histStride (integrator.js:111) */
     P* dead;                       /* sim.deadParticles (main.js:103) */
     int ndead, deadcap;
@@ -1207,9 +1208,10 @@ static void accum1PN(const OrcSys* s, P* p, double sx, double sy, double svx, do
     }
 }
 
-/* compute1PN (forces.js:578-610) and its tree walk (511-571).  With signal
- * delay on, non-ghost leaf sources (and every direct-sum source) act from
- * their past light cone (542-546, 600-602); otherwise at the current time. */
+/* compute1PN (forces.js:578-610) and its tree walk (511-571).  Non-ghost leaf
+ * sources (and every direct-sum source) act from their past light cone
+ * (542-546, 600-602), in the reference whether or not relativity is on;
+ * with signal delay OFF they act at the current time. */
 static void compute1PN(OrcSys* s) {
     const int n = s->n;
     for (int i = 0; i < n; i++) s->p[i].f1x = s->p[i].f1y = 0;
@@ -1239,7 +1241,7 @@ static void compute1PN(OrcSys* s) {
                             const P* o = &s->tp[q->points[base + k]];
                             if (o->original == pi) continue;
                             const P* real = &s->p[o->original];
-                            if (s->delay && !o->isGhost) {
+                            if (s->delay1pn && !o->isGhost) {
                                 Delayed d;
                                 if (!delayedState(s, real, p->x, p->y, &d)) continue;
                                 accum1PN(s, p, d.x, d.y, d.vx, d.vy, o->mass, o->charge, real->yukMod, real->higgsMod);
@@ -1265,7 +1267,7 @@ static void compute1PN(OrcSys* s) {
             for (int j = 0; j < n; j++) {
                 if (i == j) continue;
                 const P* o = &s->p[j];
-                if (s->delay) {
+                if (s->delay1pn) {
                     Delayed d;
                     if (!delayedState(s, o, p->x, p->y, &d)) continue;
                     accum1PN(s, p, d.x, d.y, d.vx, d.vy, o->mass, o->charge, o->yukMod, o->higgsMod);
@@ -1298,6 +1300,10 @@ static void syncToggles(OrcSys* s) {                                      /* int
     s->yukCutoffSq = s->yuk ? (6 / muMin) * (6 / muMin) : INFINITY;       /* forces.js:82-83 */
     s->periodic = s->prm.boundary_mode == GEON_BOUND_LOOP;
     s->delay = s->prm.signal_delay == GEON_DELAY_ON || (s->prm.signal_delay == GEON_DELAY_FOLLOW && s->rel);
+    /* compute1PN takes every non-ghost leaf from the light cone whether or not
+     * relativity is on (forces.js:542-546, 600-602); without relativity no
+     * history is recorded (integrator.js:1549), so those leaves add nothing. */
+    s->delay1pn = s->delay || s->prm.signal_delay == GEON_DELAY_FOLLOW;
 }
 
 OrcSys* orc_create(const GeonParams* prm, int32_t capacity) {
@@ -1840,6 +1846,13 @@ int orc_stats(OrcSys* s, GeonStats* out) {
     if (s->axion) {
         out->axion_field_energy = of_field_energy(&s->axionF, W, H);
         out->pfi_energy += of_pfi_energy(s, &s->axionF);
+    }
+    for (int w = 0; w < 2; w++) {                                         /* energy.js:176-184 */
+        if (!(w == 0 ? s->higgs : s->axion)) continue;
+        double mx, my;
+        of_momentum(w == 0 ? &s->higgsF : &s->axionF, W, H, &mx, &my);
+        out->field_px += mx;
+        out->field_py += my;
     }
     out->pe += out->pfi_energy;                                           /* energy.js:186 */
     out->sim_time = s->simTime;

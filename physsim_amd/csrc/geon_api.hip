@@ -43,7 +43,7 @@ struct FieldState {
     int green_topo = -1;        // 0 torus / open, else the glide topology the grid was built for
 };
 
-constexpr int kTimers = 5;
+constexpr int kTimers = 8;   // walk, tree, sort, integrator, network, field deposit, field KDK, field -> particle
 thread_local std::string g_create_error;
 
 }  // namespace
@@ -505,15 +505,15 @@ void fields_step(GeonContext* h, float dt) {
     if (t & GEON_HIGGS) {
         if (grav) ensure_green(h, 0);
         FieldArgs f = field_args(h, 0);
-        field_deposit(h->stream, f, pv, n, rel, coul, yuk);
-        field_kdk(h->stream, f, (t & GEON_AXION) ? h->fld[1].field : nullptr, grav, dt);
-        field_modulate(h->stream, f, pv, F(h, C_HIGGS), dt, n);
+        { Timer tm(h, 5); field_deposit(h->stream, f, pv, n, rel, coul, yuk); }
+        { Timer tm(h, 6); field_kdk(h->stream, f, (t & GEON_AXION) ? h->fld[1].field : nullptr, grav, dt); }
+        { Timer tm(h, 7); field_modulate(h->stream, f, pv, F(h, C_HIGGS), dt, n); }
     }
     if (t & GEON_AXION) {
         if (grav) ensure_green(h, 1);
         FieldArgs f = field_args(h, 1);
-        field_deposit(h->stream, f, pv, n, rel, coul, yuk);
-        field_kdk(h->stream, f, (t & GEON_HIGGS) ? h->fld[0].field : nullptr, grav, dt);
+        { Timer tm(h, 5); field_deposit(h->stream, f, pv, n, rel, coul, yuk); }
+        { Timer tm(h, 6); field_kdk(h->stream, f, (t & GEON_HIGGS) ? h->fld[0].field : nullptr, grav, dt); }
     }
 }
 
@@ -613,9 +613,11 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
     }
     const uint32_t tg = h->prm.toggles0;
     const bool axion = !!(tg & GEON_AXION), higgs = !!(tg & GEON_HIGGS);
-    if (axion)      // _syncAxionField (integrator.js:595-605)
+    if (axion) {    // _syncAxionField (integrator.js:595-605)
+        Timer tm(h, 7);
         field_axmod(h->stream, field_args(h, 1), particle_view(h), F(h, C_AX), F(h, C_YUK), !!(tg & GEON_COULOMB),
                     !!(tg & GEON_YUKAWA), n);
+    }
     launch_prepare_sources(h->stream, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_CHARGE), F(h, C_VX), F(h, C_VY),
                            F(h, C_ANGVEL), F(h, C_RADIUS), F(h, C_MU), F(h, C_L), axion ? F(h, C_AX) : nullptr,
                            axion ? F(h, C_YUK) : nullptr, higgs ? F(h, C_HIGGS) : nullptr, h->src4, h->src8a,
@@ -633,10 +635,12 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
         launch_far_mlp(h->stream, far_args(h), n);
     }
     if (fa.hist.on && h->n_dead > 0) launch_dead_forces(h->stream, fa, feature_mask(h->prm));
-    if (higgs || axion)   // field forces after the pair forces (integrator.js:1494-1507)
+    if (higgs || axion) {   // field forces after the pair forces (integrator.js:1494-1507)
+        Timer tm(h, 7);
         field_forces(h->stream, field_args(h, 0), field_args(h, 1), higgs, axion, !!(tg & GEON_GRAVITY),
                      !!(tg & GEON_COULOMB), !!(tg & GEON_YUKAWA), particle_view(h), h->fc[K_FX], h->fc[K_FY],
                      h->fc[K_PEG], h->fc[K_PFI], n);
+    }
     HIP_OK(hipGetLastError());
     return GEON_OK;
 }
@@ -662,7 +666,12 @@ int one_pn_step(GeonContext* h) {
                            (tg & GEON_AXION) ? F(h, C_AX) : nullptr, (tg & GEON_AXION) ? F(h, C_YUK) : nullptr,
                            (tg & GEON_HIGGS) ? F(h, C_HIGGS) : nullptr, h->src4, h->src8a, h->src8b, n, h->ctrl,
                            /*update_moments=*/0);
-    launch_walk1pn(h->stream, force_args(h), bh);
+    ForceArgs fa = force_args(h);
+    // compute1PN takes every leaf from its past light cone, with or without
+    // relativity (forces.js:542-546, 600-602); in the reference's mode without
+    // relativity no history is ever recorded, so leaves add nothing.
+    fa.pn_leaves = !(h->prm.signal_delay == GEON_DELAY_FOLLOW && !delay_on(h->prm));
+    launch_walk1pn(h->stream, fa, bh);
     launch_vv_kick(h->stream, step_args(h));
     HIP_OK(hipGetLastError());
     return GEON_OK;

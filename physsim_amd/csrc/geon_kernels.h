@@ -115,6 +115,9 @@ struct ForceArgs {
     const int32_t* broot;
     // signal delay of leaf sources (hist.on), and the retired sources
     HistArgs hist;
+    // 1PN recompute without history: 0 takes no leaf source at all (the
+    // reference takes them only from a light cone that nothing has recorded)
+    int pn_leaves = 1;
 };
 constexpr int kFarMultipole = 1, kFarList = 2;
 constexpr int kFarLaneShift = 26;

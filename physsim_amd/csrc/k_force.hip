@@ -320,6 +320,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
         const Node nd = a.nodes[node];
         if (nd.mass < kEpsMass) continue;
         if (nd.count > 0) {
+            if (!DELAY && !a.pn_leaves) continue;
             for (int j = nd.child; j < nd.child + nd.count; j++) {
                 const float4 s4 = a.src4[j];
                 const float4 s8a = a.src8a[j];
@@ -386,7 +387,8 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct1pn(ForceArgs a) {
     const PhysFlags pf = a.pf;
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
     const double now = DELAY ? a.ctrl->sim_time : 0.0;
-    for (int base = 0; base < n; base += kWalkThreads) {
+    const int n_src = (DELAY || a.pn_leaves) ? n : 0;     // uniform: every thread skips the tiles alike
+    for (int base = 0; base < n_src; base += kWalkThreads) {
         const int j = base + threadIdx.x;
         __syncthreads();
         if (j < n) {

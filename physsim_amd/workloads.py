@@ -98,11 +98,16 @@ def gaussian_field(G, g, rms, index=-2.0):
 
 def config3(n=1 << 20, replicate=0, grid=128, side=1024.0, total_mass=16.0):
     """Projected Plummer sphere (scale radius 20, truncated at 0.45 W) with
-    virialised Gaussian velocities, gravity, Higgs and axion fields on a
-    grid^2 PQS grid with FFT self-gravity; initial fields Gaussian random with
+    virialised Gaussian velocities; gravity with the 1PN Einstein-Infeld-Hoffmann
+    terms (which the reference evaluates only with gravitomagnetism on,
+    forces.js:242-248, so that is on too), Higgs and axion fields on a grid^2
+    PQS grid with FFT self-gravity; initial fields Gaussian random with
     spectral index -2 and rms 0.01 about the vacuum (Higgs 1, axion 0).
     Masses: lognormal(0, 0.5) scaled to total_mass, so the virial speed stays
-    well below c."""
+    well below c.  Relativity is off, so with the reference's coupling of
+    signal delay to relativity the 1PN velocity-Verlet pass takes no leaf
+    sources (they would come from a light cone nothing has recorded,
+    forces.js:542-546): only accepted nodes."""
     g = rng(3, replicate)
     a = 20.0
     rmax = 0.45 * side
@@ -118,8 +123,9 @@ def config3(n=1 << 20, replicate=0, grid=128, side=1024.0, total_mass=16.0):
     wx = g.normal(0, sigma, n)
     wy = g.normal(0, sigma, n)
     st = dict(x=f32(x), y=f32(y), wx=f32(wx), wy=f32(wy), mass=f32(m), charge=np.zeros(n), angw=np.zeros(n))
-    prm = dict(toggles0=A.GRAVITY | A.HIGGS | A.AXION | A.BARNES_HUT, boundary_mode=A.BOUND_DESPAWN,
-               domain_w=float(side), domain_h=float(side), softening=8.0, theta=0.5, field_grid=grid)
+    prm = dict(toggles0=A.GRAVITY | A.GRAVITOMAG | A.ONE_PN | A.HIGGS | A.AXION | A.BARNES_HUT,
+               boundary_mode=A.BOUND_DESPAWN, domain_w=float(side), domain_h=float(side), softening=8.0, theta=0.5,
+               field_grid=grid)
     fields = dict(higgs=f32(1.0 + gaussian_field(grid, g, 0.01)), axion=f32(gaussian_field(grid, g, 0.01)))
     return st, prm, 1.0 / 128, fields
 

@@ -1,0 +1,136 @@
+"""Writes tests/golden/reference_*.npz: what the reference's own CPU physics
+computes on seeded inputs, for the parts of the oracle no closed form pins
+(the composite Physics.update sequence, signal delay with its history
+warm-up, HiggsField / AxionField.update with self-gravity, the periodic
+ghost walk).
+
+It runs the reference's JavaScript (integrator.js, higgs-field.js,
+axion-field.js, energy.js and their imports) under Node 12 in the build
+container, through reference_goldens.mjs, and keeps only data: each case's
+inputs and the reference's outputs.  The reference never travels; the
+fixtures are what tests/test_oracle_golden.py reads.
+
+    python tests/golden/make_reference_goldens.py [/root/reference]
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from physsim_amd import workloads as W  # noqa: E402
+
+TOGGLE_NAMES = ("gravityEnabled", "coulombEnabled", "magneticEnabled", "gravitomagEnabled", "onePNEnabled",
+                "relativityEnabled", "spinOrbitEnabled", "radiationEnabled", "blackHoleEnabled",
+                "disintegrationEnabled", "expansionEnabled", "yukawaEnabled", "higgsEnabled", "axionEnabled",
+                "barnesHutEnabled", "bosonInterEnabled")
+DESPAWN, LOOP, TORUS, PASS = 0, 2, 0, 0
+
+
+def _toggles(*names):
+    return {k: (k in names) for k in TOGGLE_NAMES}
+
+
+def _spinning_disk(n, seed, speed_noise=0.1, spin=0.5, radius=10.0):
+    g = np.random.Generator(np.random.Philox(9000 + seed))
+    st = W.disk(n, radius, 32.0, 32.0, 8.0, g, rel_noise=speed_noise)
+    st["angw"] = W.spin_to_angw(g.uniform(-spin, spin, n), st["mass"])
+    return st
+
+
+def _uniform_box(n, seed, side):
+    g = np.random.Generator(np.random.Philox(9100 + seed))
+    return dict(x=g.uniform(0, side, n), y=g.uniform(0, side, n), wx=g.normal(0, 0.2, n), wy=g.normal(0, 0.2, n),
+                mass=g.lognormal(0, 0.5, n), charge=g.integers(-10, 11, n) * 0.1, angw=np.zeros(n))
+
+
+def cases():
+    out = []
+
+    def add(name, st, toggles, updates, dt, boundary=DESPAWN, side=64.0, fields=None):
+        c = dict(name=name, toggles=toggles, updates=updates, dt=dt, collisionMode=PASS, boundaryMode=boundary,
+                 topology=TORUS, W=side, H=side,
+                 particles={k: np.asarray(st[k], np.float64).tolist() for k in
+                            ("x", "y", "wx", "wy", "mass", "charge", "angw")})
+        if fields:
+            c["fields"] = {k: np.asarray(v, np.float64).ravel().tolist() for k, v in fields.items()}
+        out.append(c)
+
+    # config-1 disk with the reference's constants (softening 8, theta 0.5): gravity + Coulomb, Barnes-Hut
+    add("disk_gc_bh", _spinning_disk(256, 1, 0.05, 0.0), _toggles("gravityEnabled", "coulombEnabled",
+                                                                   "barnesHutEnabled"), 100, 0.01)
+    # every pairForce term of the non-relativistic set, exact pairs: dipoles, Bz/Bgz and gradients, Darwin,
+    # EIH, Bazanski, scalar Breit (1PN + Yukawa), tidal torque, Boris, spin-orbit kicks, 1PN velocity Verlet
+    em = ("gravityEnabled", "coulombEnabled", "magneticEnabled", "gravitomagEnabled", "onePNEnabled",
+          "spinOrbitEnabled", "yukawaEnabled")
+    add("em_gm_pn_direct", _spinning_disk(96, 2), _toggles(*em), 10, 1.0 / 128)
+    # the same set through the tree walk (aggregate sources: momentum / mass, sAngVel 0, mods 1)
+    add("em_gm_pn_bh", _spinning_disk(256, 3), _toggles(*em, "barnesHutEnabled"), 5, 1.0 / 128)
+    # relativity: signal-delayed leaf sources after the history warm-up (a sample every 64 updates),
+    # aberration, frame-drag torque, relativistic spin and Boris
+    rel = ("gravityEnabled", "coulombEnabled", "magneticEnabled", "gravitomagEnabled", "onePNEnabled",
+           "spinOrbitEnabled", "relativityEnabled")
+    add("rel_delay_direct", _spinning_disk(48, 4, 0.1, 0.5, 6.0), _toggles(*rel), 200, 1.0 / 128)
+    # scalar fields with self-gravity and the Higgs-axion portal, Dirichlet (despawn) and periodic borders
+    g = np.random.Generator(np.random.Philox(9200))
+    fields = dict(higgs=1.0 + W.gaussian_field(64, g, 0.01), axion=W.gaussian_field(64, g, 0.01))
+    fz = ("gravityEnabled", "coulombEnabled", "higgsEnabled", "axionEnabled")
+    add("fields_despawn", _spinning_disk(64, 5, 0.1, 0.0), _toggles(*fz), 5, 1.0 / 128, fields=fields)
+    add("fields_torus", _uniform_box(64, 6, 64.0), _toggles(*fz), 5, 1.0 / 128, boundary=LOOP, fields=fields)
+    # periodic Barnes-Hut with the reference's ghost images (integrator.js:201-258)
+    add("torus_ghosts_bh", _uniform_box(256, 7, 64.0), _toggles("gravityEnabled", "coulombEnabled",
+                                                                "barnesHutEnabled"), 20, 1.0 / 128, boundary=LOOP)
+    return out
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    cs = cases()
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "in.json")
+        with open(inp, "w") as f:
+            json.dump({"cases": cs}, f)
+        env = dict(os.environ, GEON_REF_SRC=os.path.join(ref, "src"))
+        r = subprocess.run(["node", "--experimental-loader", "./ref_loader.mjs", "reference_goldens.mjs", inp],
+                           cwd=HERE, env=env, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise SystemExit(r.stderr)
+        res = json.loads(r.stdout)
+    for c in cs:
+        o = res[c["name"]]
+        arrs = {"in_" + k: np.asarray(v, np.float64) for k, v in c["particles"].items()}
+        for k, v in c.get("fields", {}).items():
+            arrs["in_field_" + k] = np.asarray(v, np.float64)
+        meta = {k: c[k] for k in ("name", "toggles", "updates", "dt", "collisionMode", "boundaryMode", "topology",
+                                  "W", "H")}
+        arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), np.uint8)
+        arrs["substeps"] = np.asarray(o["substeps"], np.int32)
+        for stage in ("bootstrap", "final"):
+            for k, v in o[stage]["forces"].items():
+                arrs["%s_%s" % (stage, k)] = np.asarray(v, np.float64)
+            arrs[stage + "_pe"] = np.float64(o[stage]["pe"])
+        for k, v in o["final"]["state"].items():
+            arrs["final_" + k] = np.asarray(v, np.float64)
+        arrs["final_sim_time"] = np.float64(o["final"]["sim_time"])
+        en = o["final"]["energies"]
+        arrs["energy_keys"] = np.frombuffer(json.dumps(sorted(en)).encode(), np.uint8)
+        arrs["energy_values"] = np.asarray([en[k] for k in sorted(en)], np.float64)
+        for f in ("higgs", "axion"):
+            if f in o["final"]:
+                arrs["final_%s_field" % f] = np.asarray(o["final"][f]["field"], np.float64)
+                arrs["final_%s_field_dot" % f] = np.asarray(o["final"][f]["field_dot"], np.float64)
+        path = os.path.join(HERE, "reference_%s.npz" % c["name"])
+        np.savez_compressed(path, **arrs)
+        print("%s: %d particles, %d updates, substeps %s -> %s (%d bytes)"
+              % (c["name"], len(c["particles"]["x"]), c["updates"], o["substeps"][:5], os.path.basename(path),
+                 os.path.getsize(path)))
+
+
+if __name__ == "__main__":
+    main()

@@ -571,3 +571,97 @@ def test_far_learned_correction_wiring(oracle_lib):
     np.testing.assert_allclose(f1["fy"] - f0["fy"], sy, rtol=0, atol=tol)
     assert np.abs(sx).max() > 1e-3 * np.abs(f0["fx"]).max()
     assert f1["pe_total"] - f0["pe_total"] == pytest.approx(0.5 * pe.sum(), rel=1e-5)
+
+
+def _pe_of_separation(oracle_lib, toggles, d, **kw):
+    return two_body(oracle_lib, toggles, d=d, **kw).forces()["pe_total"]
+
+
+@pytest.mark.parametrize("toggle, sign", [(A.MAGNETIC, -1.0), (A.GRAVITOMAG, 1.0)])
+def test_spin_dipole_forces_closed_form_and_conservative(toggle, sign, oracle_lib):
+    """Two spinning bodies at rest (forces.js:366-398): a uniformly charged
+    spinning sphere carries the moment q w R^2 / 5 and the solid sphere the
+    angular momentum 2/5 m w R^2, R = m^(1/3) (particle.js:91).  Parallel
+    z-dipoles side by side repel as 3 mu1 mu2 / r^4; the gravitomagnetic
+    analogue has the opposite sign (co-rotating masses attract).  The force
+    must also be minus the gradient of the pair's potential energy (a
+    central difference of the oracle's own PE in the separation)."""
+    m, q, w = (2.0, 3.0), (0.6, -0.9), (0.3, -0.45)
+    d, soft = 4.0, 1e-4
+    o = two_body(oracle_lib, toggle, m=m, q=q, d=d, soft=soft, angw=w)
+    f = o.forces()
+    R2 = [mi ** (2.0 / 3.0) for mi in m]
+    if toggle == A.MAGNETIC:
+        mom = [q[i] * w[i] * R2[i] / 5 for i in range(2)]
+    else:
+        mom = [0.4 * m[i] * w[i] * R2[i] for i in range(2)]
+    # force on body 0 along +x (towards body 1): repulsion is negative
+    expect = sign * 3 * mom[0] * mom[1] / d ** 4
+    assert f["fx"][0] == pytest.approx(expect, rel=1e-8)
+    assert f["fx"][1] == pytest.approx(-expect, rel=1e-8)
+    h = 1e-4
+    up = _pe_of_separation(oracle_lib, toggle, d + h, m=m, q=q, soft=soft, angw=w)
+    dn = _pe_of_separation(oracle_lib, toggle, d - h, m=m, q=q, soft=soft, angw=w)
+    # F_0x = -dU/dx_0 = +dU/dd with d = x_1 - x_0
+    assert f["fx"][0] == pytest.approx((up - dn) / (2 * h), rel=1e-6)
+
+
+def test_dipole_fields_closed_form(oracle_lib):
+    """Equatorial field of a z-dipole: Bz = -mu / r^3 at the observer; its
+    gravitomagnetic analogue from a spin L is Bgz = -2 L / r^3 (forces.js:356-359,
+    393-396)."""
+    m, q, w = (2.0, 3.0), (0.0, -0.9), (0.0, -0.45)
+    d, soft = 4.0, 1e-4
+    o = two_body(oracle_lib, A.MAGNETIC | A.GRAVITOMAG, m=m, q=q, d=d, soft=soft, angw=w)
+    f = o.forces()
+    R2 = m[1] ** (2.0 / 3.0)
+    mu, L = q[1] * w[1] * R2 / 5, 0.4 * m[1] * w[1] * R2
+    assert f["bz"][0] == pytest.approx(-mu / d ** 3, rel=1e-8)
+    assert f["bgz"][0] == pytest.approx(-2 * L / d ** 3, rel=1e-8)
+
+
+def test_field_gradients_are_derivatives_of_the_fields(oracle_lib):
+    """dBz/dx, dBz/dy, dBgz/dx, dBgz/dy (the spin-orbit drivers,
+    forces.js:350-359, 388-396) equal central differences of Bz and Bgz in the
+    observer's position, for a source that both moves and spins."""
+    m, q, w = (1.5, 2.5), (0.4, -0.7), (0.0, 0.35)
+    vel = ((0.0, 0.0), (0.12, -0.2))
+
+    def fields(dx, dy):
+        prm = oracle_lib.default_params(toggles0=A.MAGNETIC | A.GRAVITOMAG, softening=0.5, domain_w=64.0,
+                                        domain_h=64.0, signal_delay=A.DELAY_OFF)
+        o = oracle_lib.Oracle(prm, 2)
+        o.upload(np.array([30.0 + dx, 33.5]), np.array([32.0 + dy, 34.25]), np.array([vel[0][0], vel[1][0]]),
+                 np.array([vel[0][1], vel[1][1]]), np.array(m), np.array(q), np.array(w))
+        o.compute_forces()
+        return o.forces()
+
+    f0 = fields(0.0, 0.0)
+    h = 1e-4
+    for key, grad in (("bz", "dbz"), ("bgz", "dbgz")):
+        gx = (fields(h, 0.0)[key][0] - fields(-h, 0.0)[key][0]) / (2 * h)
+        gy = (fields(0.0, h)[key][0] - fields(0.0, -h)[key][0]) / (2 * h)
+        assert f0[grad + "dx"][0] == pytest.approx(gx, rel=1e-6), key
+        assert f0[grad + "dy"][0] == pytest.approx(gy, rel=1e-6), key
+
+
+def test_torques_vanish_at_corotation(oracle_lib):
+    """The frame-drag torque 2 L_s (w_s - w_p) / r^3 (forces.js:398-400) aligns
+    spins: it vanishes for equal spin rates and drives each body towards the
+    other's rate.  (It is weighted by the source's own spin, so unlike a
+    pair exchange it does not conserve the total spin.)  The tidal torque
+    (forces.js:427-441) vanishes when a body spins at the orbital rate of the
+    pair and opposes the excess."""
+    m, d = (2.0, 2.0), 5.0
+    o = two_body(oracle_lib, A.GRAVITOMAG, m=m, q=(0, 0), d=d, soft=0.5, angw=(0.2, 0.2))
+    assert o.forces()["frame_drag"] == pytest.approx([0.0, 0.0], abs=1e-15)
+    o = two_body(oracle_lib, A.GRAVITOMAG, m=m, q=(0, 0), d=d, soft=0.5, angw=(0.1, 0.3))
+    t = o.forces()["frame_drag"]
+    assert t[0] > 0 and t[1] < 0
+    # body 1 orbits body 0 at rate v / r: body 0 spinning at that rate feels no tidal torque
+    v, soft = 0.2, 0.5
+    wo = d * v / (d * d + soft * soft)
+    o = two_body(oracle_lib, A.GRAVITY, m=m, q=(0, 0), d=d, soft=soft, w=((0, 0), (0, v)), angw=(wo, 0.0))
+    assert o.forces()["tidal"][0] == pytest.approx(0.0, abs=1e-15)
+    o = two_body(oracle_lib, A.GRAVITY, m=m, q=(0, 0), d=d, soft=soft, w=((0, 0), (0, v)), angw=(wo + 0.1, 0.0))
+    assert o.forces()["tidal"][0] < 0
