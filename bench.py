@@ -46,6 +46,8 @@ MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 6)
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (profiles/k_far_mlp_open1M_theta0.30.md)
 MLP_TRAFFIC_DEFAULT = 3003.0e6
 TIMER_WALK, TIMER_TREE, TIMER_SORT, TIMER_KICK, TIMER_MLP = 0, 1, 2, 3, 4
+TIMER_DEPOSIT, TIMER_KDK, TIMER_FIELD_PART = 5, 6, 7
+HBM_PEAK_GBS = 8000.0
 
 
 def flops_per_interaction(toggles):
@@ -195,11 +197,12 @@ def _cpu_update_rate(prm, st, dt, warm, threads):
             "kind": "port", "n": len(st["x"])}
 
 
-def run_legs(names, with_cpu, threads):
+def run_legs(names, with_cpu, threads, model=None):
     """The rows beyond the headline, each timed on the GPU with the FP64 oracle
     beside it on a smaller N: signal-delayed leaf forces, collision mode merge,
     radiation reaction.  Rates are particle-steps/s over whole updates."""
-    from physsim_amd import HipPhysics, default_params, _abi as A
+    from physsim_amd import HipPhysics, default_params, workloads as W, _abi as A
+    from oracle import bind
     out = {}
     f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
 
@@ -277,6 +280,100 @@ def run_legs(names, with_cpu, threads):
         out["merge"] = r
         log("leg merge: first update %.1f ms (%d events), steady %.3e particle-steps/s"
             % (first_ms, events, steady["particle_steps_per_s"]))
+    if "config3" in names:
+        # configuration 3: gravity + 1PN EIH (with gravitomagnetism) + Higgs + axion on the 128^2 grid
+        n = 1 << 20
+        st, prm, dt, fields = W.config3(n, grid=128, side=1024.0, total_mass=16.0)
+        g = HipPhysics(default_params(**prm), max_particles=n)
+        g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+        g.set_field(0, fields["higgs"])
+        g.set_field(1, fields["axion"])
+        g.update(dt)
+        steps = 8
+        g.timing(True)
+        ps, t0 = 0, time.perf_counter()
+        for _ in range(steps):
+            n0 = g.getParticleCount()
+            ps += g.update(dt) * n0
+        wall = time.perf_counter() - t0
+        ph = {}
+        for key, tid in (("walk", TIMER_WALK), ("tree", TIMER_TREE), ("sort_gather", TIMER_SORT),
+                         ("kick_drift", TIMER_KICK), ("field_deposit", TIMER_DEPOSIT), ("field_kdk", TIMER_KDK),
+                         ("field_to_particle", TIMER_FIELD_PART)):
+            ms, cnt = g.kernel_timing(tid)
+            ph[key] = {"ms_avg": ms / cnt if cnt else 0.0, "launches": cnt}
+        g.timing(False)
+        G = 128
+        # algorithmic bytes (per deposit: positions + one value per particle, the grid written once;
+        # per KDK: two FFT self-gravity convolutions of 2 x G^2 x 8 B per transform pair and five
+        # stencil sweeps of ~40 B per cell)
+        dep_bytes = n * 12 + G * G * 4
+        kdk_bytes = 2 * 2 * 2 * G * G * 8 + 5 * G * G * 40
+        for key, b in (("field_deposit", dep_bytes), ("field_kdk", kdk_bytes)):
+            t = ph[key]["ms_avg"] * 1e-3
+            ph[key]["algorithmic_bytes"] = b
+            ph[key]["achieved_gbs"] = b / t / 1e9 if t > 0 else None
+            ph[key]["frac_of_hbm"] = b / t / 1e9 / HBM_PEAK_GBS if t > 0 else None
+        r = {"particle_steps_per_s": ps / wall, "ms_per_update": wall / steps * 1e3, "particles": g.getParticleCount(),
+             "updates": steps, "phases": ph,
+             "workload": "N=%d projected Plummer sphere (scale radius 20), gravity + gravitomagnetism + 1PN EIH "
+                         "(velocity-Verlet pass), Higgs + axion on a 128^2 PQS grid with FFT self-gravity, "
+                         "Gaussian random initial fields (index -2, rms 0.01), BH theta 0.5, dt 1/128" % n,
+             "note": "the 128^2 field work is latency-bound: one grid is 64 KiB, so a field kernel moves a few "
+                     "hundred KiB and its time is launch and dependency latency, not HBM bandwidth; the deposit "
+                     "is bound by its 16 int64 atomics per particle per field"}
+        g.destroy()
+        if with_cpu:
+            cs, cprm, cdt, cf = W.config3(16384, grid=128, side=256.0, total_mass=1.0)
+            bind.set_threads(threads)
+            o = bind.Oracle(bind.default_params(**cprm), 16384)
+            o.upload(cs["x"], cs["y"], cs["wx"], cs["wy"], cs["mass"], cs["charge"], cs["angw"])
+            o.field_set(0, cf["higgs"])
+            o.field_set(1, cf["axion"])
+            o.update(cdt)
+            t0 = time.perf_counter()
+            sc = o.update(cdt)
+            r["cpu_baseline"] = {"value": sc * o.count() / (time.perf_counter() - t0), "unit": "particle-steps/s",
+                                 "cores": threads, "kind": "port", "n": 16384}
+        out["config3"] = r
+        log("leg config3: %.3e particle-steps/s" % r["particle_steps_per_s"])
+    if "torus" in names:
+        # configuration 5 on the torus: the learned correction in windowed periodic mode (nodes straddling
+        # the observer's min-image window are opened, so Barnes-Hut converges to the min-image direct sum)
+        n = 1 << 20
+        st, prm, dt = W.config2(n)
+        prm = dict(prm, periodic_mode=A.PERIODIC_WINDOWED, far_list_cap=1 << 17)
+        g = HipPhysics(default_params(**prm), max_particles=n)
+        g.far_set_model(model)
+        g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+        rs = np.sort(np.random.Generator(np.random.Philox(5001)).choice(n, 4096, replace=False)).astype(np.int32)
+        o = bind.Oracle(bind.default_params(**prm), n)
+        o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+        fx, fy = o.direct_forces(rs)
+        del o
+        r = {"workload": "N=%d uniform %.0fx%.0f torus, windowed min-image Barnes-Hut, config-2 forces "
+                         "(relativistic, spins), dt 1/128; force error against the FP64 min-image direct sum "
+                         "on 4096 particles" % (n, prm["domain_w"], prm["domain_h"]), "methods": {}}
+        for name, theta, far in (("plain", 0.3, A.FAR_NONE), ("learned", 0.3, A.FAR_LEARNED),
+                                 ("multipole", 0.3, A.FAR_MULTIPOLE)):
+            p = A.GeonParams.from_buffer_copy(g.params)
+            p.theta, p.far_field = theta, far
+            g.set_params(p)
+            g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+            g.compute_forces()
+            f = g.forces(n)
+            err = float(np.sqrt(np.sum((f["fx"][rs] - fx) ** 2 + (f["fy"][rs] - fy) ** 2) / np.sum(fx ** 2 + fy ** 2)))
+            g.update(dt)
+            ps, t0 = 0, time.perf_counter()
+            for _ in range(4):
+                n0 = g.getParticleCount()
+                ps += g.update(dt) * n0
+            wall = time.perf_counter() - t0
+            r["methods"][name] = {"theta": theta, "force_rel_l2": err, "particle_steps_per_s": ps / wall,
+                                  "ms_per_update": wall / 4 * 1e3}
+            log("leg torus %s theta %.2f: error %.3e, %.3e particle-steps/s" % (name, theta, err, ps / wall))
+        g.destroy()
+        out["torus_windowed"] = r
     if "radiation" in names:
         # Landau-Lifshitz reaction every substep, quadrupole radiation every update
         n, W, dt = 1 << 20, 1024.0, 1.0 / 128
@@ -307,7 +404,7 @@ def main():
     # opening-angle ladders, coarsest first: each method runs at the coarsest
     # theta whose force error (measured below) is <= --err-target
     ap.add_argument("--theta-plain", default="0.05,0.045,0.04,0.035,0.03")
-    ap.add_argument("--theta-learned", default="0.3,0.28,0.26,0.24,0.22")
+    ap.add_argument("--theta-learned", default="0.3,0.29,0.28,0.26,0.24,0.22")
     ap.add_argument("--theta-multipole", default="0.35,0.32,0.3,0.28,0.25")
     ap.add_argument("--theta-train", type=float, default=0.3)
     ap.add_argument("--err-target", type=float, default=1e-4)
@@ -322,7 +419,7 @@ def main():
     ap.add_argument("--drift-n", type=int, default=1 << 20)
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="batched-ensemble leg (config 4): systems; 0 skips it")
-    ap.add_argument("--legs", default="delay,merge,radiation",
+    ap.add_argument("--legs", default="config3,torus,delay,merge,radiation",
                     help="rows beyond the headline to time ('' for none)")
     ap.add_argument("--methods", default="plain,learned,multipole",
                     help="comma list of force methods to run (the learned one is always run)")
@@ -448,7 +545,7 @@ def main():
     batch = run_batch(args.batch_systems, args.steps) if args.batch_systems > 0 else None
     drift = run_drift(args.drift_n, args.drift_steps, L["theta"], model) if args.drift_steps > 0 else None
 
-    legs = run_legs([x for x in args.legs.split(",") if x], not args.no_cpu, threads) if args.legs else None
+    legs = run_legs([x for x in args.legs.split(",") if x], not args.no_cpu, threads, model) if args.legs else None
 
     cpu = None
     if not args.no_cpu:

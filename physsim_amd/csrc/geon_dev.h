@@ -235,21 +235,25 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         rdv = rx * vrx + ry * vry;
         inv_r5j = ABERR ? inv_r3 * inv_r2 * aberr : inv_r3 * inv_r2;
     }
+    // The pair's central terms add into one radial coefficient fr (force =
+    // r fr), its velocity-dependent terms into (fnx, fny) and its potential
+    // into pe; the pair then enters the compensated sums once.
+    float fr = 0.f, fnx = 0.f, fny = 0.f, pe = 0.f;
     if (do_grav) {
         const float k = o.m * sm;
         const float fd = k * inv_r3a;
-        a.addf(rx * fd, ry * fd);
+        fr += fd;
         if (F & F_JERK) a.addj(fd, -3.f * k * rdv * inv_r5j, vrx, vry, rx, ry);
-        if (PE) a.addpe(-k * inv_r);
+        if (PE) pe -= k * inv_r;
     }
     float ax_pair = 1.f;
     if ((F & F_EXTRA) && pf.ax_mod) ax_pair = sqrtf(o.ax * s_ax);
     if (do_coul) {
         const float k = -(o.q * sq) * ax_pair;
         const float fd = k * inv_r3a;
-        a.addf(rx * fd, ry * fd);
+        fr += fd;
         if (F & F_JERK) a.addj(fd, -3.f * k * rdv * inv_r5j, vrx, vry, rx, ry);
-        if (PE) a.addpe(o.q * sq * inv_r * ax_pair);
+        if (PE) pe += o.q * sq * inv_r * ax_pair;
     }
     if (F & F_EXTRA) {
         const float inv_r5 = inv_r3 * inv_r2;
@@ -269,35 +273,35 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float base = o.m * sm * inv_r3;
                 const float fx = base * (rx * radial + (o.vx * c1 + svx * c2) * r);
                 const float fy = base * (ry * radial + (o.vy * c1 + svy * c2) * r);
-                a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
+                fnx += fx; fny += fy; a.f1x += fx; a.f1y += fy;
                 if (F & F_JERK) {               // position-only EIH term m1 m2 (5 m1 + 4 m2) r / r^4
                     const float ke = o.m * sm * (5.f * o.m + 4.f * sm);
                     const float r4 = inv_r2 * inv_r2;
                     a.addj(ke * r4, -4.f * ke * rdv * r4 * inv_r2, vrx, vry, rx, ry);
                 }
-                if (PE) a.addpe(-o.m * sm * inv_r * (1.5f * (v1sq + v2sq) - 3.5f * v1v2 - 0.5f * nv1 * nv2 + o.m * inv_r + sm * inv_r));
+                if (PE) pe -= o.m * sm * inv_r * (1.5f * (v1sq + v2sq) - 3.5f * v1v2 - 0.5f * nv1 * nv2 + o.m * inv_r + sm * inv_r);
             }
             if (pf.mag) {
                 const float v2n = svx * nx + svy * ny, v1n = o.vx * nx + o.vy * ny;
                 const float coeff = 0.5f * o.q * sq * inv_r2;
                 const float fx = coeff * (o.vx * v2n - 3.f * nx * v1n * v2n);
                 const float fy = coeff * (o.vy * v2n - 3.f * ny * v1n * v2n);
-                a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
-                if (PE) a.addpe(-0.5f * o.q * sq * inv_r * (v1v2 + v1n * v2n));
+                fnx += fx; fny += fy; a.f1x += fx; a.f1y += fy;
+                if (PE) pe -= 0.5f * o.q * sq * inv_r * (v1v2 + v1n * v2n);
             }
         }
         if (pf.one_pn && pf.gm && pf.mag) {                         // forces.js:303-320
             const float cc = o.q * sq * (o.m + sm) - (o.q * o.q * sm + sq * sq * o.m);
             const float fd = cc * inv_r2 * inv_r2;
-            a.addf(rx * fd, ry * fd); a.f1x += rx * fd; a.f1y += ry * fd;
+            fr += fd; a.f1x += rx * fd; a.f1y += ry * fd;
             if (F & F_JERK) a.addj(fd, -4.f * cc * rdv * inv_r2 * inv_r2 * inv_r2, vrx, vry, rx, ry);
-            if (PE) a.addpe(0.5f * cc * inv_r2);
+            if (PE) pe += 0.5f * cc * inv_r2;
         }
         if (pf.mag) {                                               // forces.js:322-353
             const float fd = -3.f * (o.mu * s_mu) * inv_r5a * ax_pair;
-            a.addf(rx * fd, ry * fd);
+            fr += fd;
             if (F & F_JERK) a.addj(fd, 15.f * (o.mu * s_mu) * rdv * inv_r5a * inv_r2 * ax_pair, vrx, vry, rx, ry);
-            if (PE) a.addpe((o.mu * s_mu) * inv_r3 * ax_pair);
+            if (PE) pe += (o.mu * s_mu) * inv_r3 * ax_pair;
             const float bzm = sq * cross_sv * inv_r3 * ax_pair;
             a.bz += bzm;
             a.dbzdx += 3.f * bzm * rx * inv_r2 + sq * svy * inv_r3 * ax_pair;
@@ -308,9 +312,9 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         }
         if (pf.gm) {                                                // forces.js:355-387
             const float fd = 3.f * (o.L * s_L) * inv_r5a;
-            a.addf(rx * fd, ry * fd);
+            fr += fd;
             if (F & F_JERK) a.addj(fd, -15.f * (o.L * s_L) * rdv * inv_r5a * inv_r2, vrx, vry, rx, ry);
-            if (PE) a.addpe(-(o.L * s_L) * inv_r3);
+            if (PE) pe -= (o.L * s_L) * inv_r3;
             const float bgm = -sm * cross_sv * inv_r3;
             a.bgz += bgm;
             a.dbgzdx += 3.f * bgm * rx * inv_r2 - sm * svy * inv_r3;
@@ -327,11 +331,11 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             const float ym = pf.ax_mod ? sqrtf(o.yuk * s_yuk) : 1.f;
             const float gmm = pf.yuk_g * ym * o.m * sm * e;
             const float fd = gmm * (inv_r2 + mu * inv_r) * (ABERR ? inv_r * aberr : inv_r);
-            a.addf(rx * fd, ry * fd);
+            fr += fd;
             if (F & F_JERK)
                 a.addj(fd, -(3.f * inv_r2 + 3.f * mu * inv_r + mu * mu) * rdv * gmm * inv_r2 *
                                (ABERR ? inv_r * aberr : inv_r), vrx, vry, rx, ry);
-            if (PE) a.addpe(-gmm * inv_r);
+            if (PE) pe -= gmm * inv_r;
             if (pf.one_pn) {
                 const float nx = rx * inv_r, ny = ry * inv_r;
                 const float nv1 = nx * o.vx + ny * o.vy, nv2 = nx * svx + ny * svy;
@@ -341,11 +345,13 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float radial = -(alpha * v1v2 + (alpha * alpha + alpha + 1.f) * nv1 * nv2);
                 const float fx = beta * (radial * nx + alpha * (nv2 * o.vx + nv1 * svx));
                 const float fy = beta * (radial * ny + alpha * (nv2 * o.vy + nv1 * svy));
-                a.addf(fx, fy); a.f1x += fx; a.f1y += fy;
-                if (PE) a.addpe(0.5f * gmm * inv_r * (v1v2 + nv1 * nv2 * alpha));
+                fnx += fx; fny += fy; a.f1x += fx; a.f1y += fy;
+                if (PE) pe += 0.5f * gmm * inv_r * (v1v2 + nv1 * nv2 * alpha);
             }
         }
     }
+    a.addf(rx * fr + fnx, ry * fr + fny);
+    if (PE) a.addpe(pe);
     if (do_grav) {                                                  // forces.js:429-441
         const float cross_rv = rx * (svy - o.vy) - ry * (svx - o.vx);
         const float dw = o.ang_vel - cross_rv * inv_r2;

@@ -83,32 +83,19 @@ __device__ __forceinline__ float pqs_gather(const float* arr, int G, int bc, int
     return v;
 }
 
-__device__ __forceinline__ void pqs_scatter(unsigned long long* out, int G, int bc, int topo, int ix, int iy, const float* wx,
-                                            const float* wy, float value) {
-    for (int jy = 0; jy < 4; jy++)
-        for (int jx = 0; jx < 4; jx++) {
-            const int idx = nb_index(G, ix + jx - 1, iy + jy - 1, bc, topo);
-            if (idx < 0) continue;
-            const double v = (double)value * (double)wx[jx] * (double)wy[jy];
-            atomicAdd(out + idx, (unsigned long long)(long long)llrint(v * kFix));
-        }
-}
-
-// Higgs: source g*baseMass and thermal KE (higgs-field.js:117-143);
-// axion: g*q^2 (Coulomb) + g*m (Yukawa) (axion-field.js:196-210).
-__global__ void k_field_deposit(FieldArgs f, const float* __restrict__ x, const float* __restrict__ y,
-                                const float* __restrict__ m, const float* __restrict__ base_mass,
-                                const float* __restrict__ q, const float* __restrict__ wx,
-                                const float* __restrict__ wy, const float* __restrict__ vx,
-                                const float* __restrict__ vy, int higgs, int rel, int coul, int yuk,
-                                const Ctrl* __restrict__ ctrl) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ctrl->n_alive) return;
-    int ix, iy;
-    float pw[4], qw[4];
+// One particle's deposit values: Higgs source g*baseMass and thermal KE
+// (higgs-field.js:117-143); axion g*q^2 (Coulomb) + g*m (Yukawa)
+// (axion-field.js:196-210).  False when the particle deposits nothing.
+__device__ __forceinline__ bool deposit_values(int i, const float* __restrict__ m, const float* __restrict__ base_mass,
+                                               const float* __restrict__ q, const float* __restrict__ wx,
+                                               const float* __restrict__ wy, const float* __restrict__ vx,
+                                               const float* __restrict__ vy, int higgs, int rel, int coul, int yuk,
+                                               float& s0, float& s1) {
+    s0 = 0.f;
+    s1 = 0.f;
     if (higgs) {
         const float bm = base_mass[i];
-        if (bm < 1e-12f) return;
+        if (bm < 1e-12f) return false;
         float ke;
         if (rel) {
             const float w2 = wx[i] * wx[i] + wy[i] * wy[i];
@@ -116,16 +103,110 @@ __global__ void k_field_deposit(FieldArgs f, const float* __restrict__ x, const 
         } else {
             ke = 0.5f * m[i] * (vx[i] * vx[i] + vy[i] * vy[i]);
         }
+        s0 = kHiggsCoupling * bm;
+        if (ke >= 1e-12f) s1 = ke;
+        return true;
+    }
+    if (coul) { const float q2 = q[i] * q[i]; if (q2 > 1e-12f) s0 += kAxionCoupling * q2; }
+    if (yuk && m[i] > 1e-12f) s0 += kAxionCoupling * m[i];
+    return s0 != 0.f;
+}
+
+__device__ __forceinline__ unsigned long long fixed_point(float value, float wx, float wy) {
+    const double v = (double)value * (double)wx * (double)wy;
+    return (unsigned long long)(long long)llrint(v * kFix);
+}
+
+// The deposit of kDepItems consecutive particles per thread, kDepThreads per
+// block.  Particles are in Morton order, so a block's particles cover a small
+// patch of the grid: the block sums its stencils in an LDS tile over the
+// patch (with 64-bit LDS atomics) and then adds each touched tile cell to the
+// grid with one global atomic.  Every stencil value is rounded to 2^-32 on
+// its own and the sums are of integers, so the grid is bit-identical to
+// depositing each stencil value straight into the grid.  A block whose patch
+// exceeds the tile deposits straight into the grid.
+constexpr int kDepThreads = 256;
+constexpr int kDepItems = 4;
+constexpr int kDepTile = 1024;        // tile cells per quantity (8 KB)
+
+__global__ __launch_bounds__(kDepThreads) void k_field_deposit(
+        FieldArgs f, const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ m,
+        const float* __restrict__ base_mass, const float* __restrict__ q, const float* __restrict__ wx,
+        const float* __restrict__ wy, const float* __restrict__ vx, const float* __restrict__ vy, int higgs, int rel,
+        int coul, int yuk, const Ctrl* __restrict__ ctrl) {
+    __shared__ unsigned long long tile[2][kDepTile];
+    __shared__ int s_box[4];
+    const int n = ctrl->n_alive;
+    const int base = blockIdx.x * kDepThreads * kDepItems;
+    if (threadIdx.x == 0) { s_box[0] = INT_MAX; s_box[1] = INT_MAX; s_box[2] = INT_MIN; s_box[3] = INT_MIN; }
+    __syncthreads();
+    int ixs[kDepItems], iys[kDepItems];
+    float vals0[kDepItems], vals1[kDepItems];
+    bool use[kDepItems];
+    int lx = INT_MAX, ly = INT_MAX, hx = INT_MIN, hy = INT_MIN;
+    for (int k = 0; k < kDepItems; k++) {
+        const int i = base + k * kDepThreads + threadIdx.x;
+        use[k] = i < n && deposit_values(i, m, base_mass, q, wx, wy, vx, vy, higgs, rel, coul, yuk, vals0[k],
+                                         vals1[k]);
+        ixs[k] = iys[k] = 0;
+        if (use[k]) {
+            const float gx = x[i] * f.icw - 0.5f, gy = y[i] * f.ich - 0.5f;   // pqs_weights' cell
+            ixs[k] = (int)floorf(gx);
+            iys[k] = (int)floorf(gy);
+            lx = min(lx, ixs[k]); hx = max(hx, ixs[k]);
+            ly = min(ly, iys[k]); hy = max(hy, iys[k]);
+        }
+    }
+    if (lx != INT_MAX) {
+        atomicMin(&s_box[0], lx); atomicMin(&s_box[1], ly);
+        atomicMax(&s_box[2], hx); atomicMax(&s_box[3], hy);
+    }
+    __syncthreads();
+    if (s_box[0] == INT_MAX) return;                  // nothing to deposit in this block (uniform)
+    // stencil cells run from ix - 1 to ix + 2
+    const long long tw_l = (long long)s_box[2] - s_box[0] + 4, th_l = (long long)s_box[3] - s_box[1] + 4;
+    const bool tiled = tw_l * th_l <= kDepTile;
+    const int tw = tiled ? (int)tw_l : 0;
+    const int x0 = s_box[0] - 1, y0 = s_box[1] - 1;
+    const int nq = higgs ? 2 : 1;
+    if (tiled) {
+        for (int c = threadIdx.x; c < nq * kDepTile; c += kDepThreads) tile[c / kDepTile][c % kDepTile] = 0ull;
+        __syncthreads();
+    }
+    for (int k = 0; k < kDepItems; k++) {
+        if (!use[k]) continue;
+        const int i = base + k * kDepThreads + threadIdx.x;
+        int ix, iy;
+        float pw[4], qw[4];
         pqs_weights(x[i], y[i], f.icw, f.ich, ix, iy, pw, qw);
-        pqs_scatter(f.dep_src, f.G, f.bc, f.topo, ix, iy, pw, qw, kHiggsCoupling * bm);
-        if (ke >= 1e-12f) pqs_scatter(f.dep_th, f.G, f.bc, f.topo, ix, iy, pw, qw, ke);
-    } else {
-        float s = 0.f;
-        if (coul) { const float q2 = q[i] * q[i]; if (q2 > 1e-12f) s += kAxionCoupling * q2; }
-        if (yuk && m[i] > 1e-12f) s += kAxionCoupling * m[i];
-        if (s == 0.f) return;
-        pqs_weights(x[i], y[i], f.icw, f.ich, ix, iy, pw, qw);
-        pqs_scatter(f.dep_src, f.G, f.bc, f.topo, ix, iy, pw, qw, s);
+        // the tile covers the cells the first pass found; a particle whose
+        // cell came out otherwise here goes straight to the grid
+        const bool in_tile = tiled && ix == ixs[k] && iy == iys[k];
+        for (int jy = 0; jy < 4; jy++)
+            for (int jx = 0; jx < 4; jx++) {
+                const unsigned long long a0 = fixed_point(vals0[k], pw[jx], qw[jy]);
+                const unsigned long long a1 = vals1[k] != 0.f ? fixed_point(vals1[k], pw[jx], qw[jy]) : 0ull;
+                if (in_tile) {
+                    const int t = (iy + jy - 1 - y0) * tw + (ix + jx - 1 - x0);
+                    atomicAdd(&tile[0][t], a0);
+                    if (a1) atomicAdd(&tile[1][t], a1);
+                } else {
+                    const int idx = nb_index(f.G, ix + jx - 1, iy + jy - 1, f.bc, f.topo);
+                    if (idx < 0) continue;
+                    atomicAdd(f.dep_src + idx, a0);
+                    if (a1) atomicAdd(f.dep_th + idx, a1);
+                }
+            }
+    }
+    if (!tiled) return;                               // uniform over the block
+    __syncthreads();
+    for (int t = threadIdx.x; t < (int)(tw_l * th_l); t += kDepThreads) {
+        const unsigned long long a0 = tile[0][t], a1 = nq == 2 ? tile[1][t] : 0ull;
+        if (a0 == 0ull && a1 == 0ull) continue;
+        const int idx = nb_index(f.G, x0 + t % tw, y0 + t / tw, f.bc, f.topo);
+        if (idx < 0) continue;                        // Dirichlet border: the value leaves the grid
+        if (a0) atomicAdd(f.dep_src + idx, a0);
+        if (a1) atomicAdd(f.dep_th + idx, a1);
     }
 }
 
@@ -563,8 +644,9 @@ void field_fft(hipStream_t s, const FieldArgs& f, int inverse) {
 
 void field_deposit(hipStream_t s, const FieldArgs& f, const ParticleView& p, int n_upper, int rel, int coul, int yuk) {
     if (n_upper > 0)
-        hipLaunchKernelGGL(k_field_deposit, dim3(grid_blocks(n_upper)), dim3(256), 0, s, f, p.x, p.y, p.m, p.base_mass,
-                           p.q, p.wx, p.wy, p.vx, p.vy, f.higgs, rel, coul, yuk, p.ctrl);
+        hipLaunchKernelGGL(k_field_deposit, dim3((n_upper + kDepThreads * kDepItems - 1) / (kDepThreads * kDepItems)),
+                           dim3(kDepThreads), 0, s, f, p.x, p.y, p.m, p.base_mass, p.q, p.wx, p.wy, p.vx, p.vy,
+                           f.higgs, rel, coul, yuk, p.ctrl);
     hipLaunchKernelGGL(k_field_finalize, dim3(grid_blocks(f.G * f.G)), dim3(256), 0, s, f);
 }
 
