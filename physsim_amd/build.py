@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgeon_hip.so")
 SOURCES = ["geon_api.hip", "geon_batch.hip", "k_tree.hip", "k_force.hip", "k_step.hip", "k_field.hip", "k_learn.hip",
-           "k_delay.hip", "k_merge.hip", "k_rad.hip"]
+           "k_delay.hip", "k_merge.hip", "k_rad.hip", "k_ghost.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -48,8 +48,21 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+// The tree points of the periodic-image mode (reals and their images).
+struct GhostState {
+    int64_t cap = 0;            // tree points held: factor x particles
+    int factor = 0;
+    GhostCols col = {}, alt = {};
+    int32_t *cnt = nullptr, *off = nullptr;
+    TreeBufs tb = {};
+    float4 *src4 = nullptr, *src8a = nullptr, *src8b = nullptr;
+    Ctrl* ctrl = nullptr;
+};
+
 struct GeonContext {
     GeonParams prm;
+    GhostState gh;
+    bool tree_ghost = false;    // the exported tree is the image tree
     unsigned char* na_scratch = nullptr;    // this device's copy of `scratch` (allocation-free pair list)
     int cap = 0;
     int n_host = 0;             // live particles (host mirror, exact between calls)
@@ -203,9 +216,17 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     }
     if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
     if (p->topology < GEON_TORUS || p->topology > GEON_RP2) { *why = "unknown topology"; return 1; }
-    if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode == GEON_PERIODIC_GHOSTS) { *why = "ghost-image periodic mode is not run (min-image and windowed are)"; return 1; }
+    if (p->periodic_mode < GEON_PERIODIC_MIN_IMAGE || p->periodic_mode > GEON_PERIODIC_WINDOWED) { *why = "unknown periodic_mode"; return 1; }
+    if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode == GEON_PERIODIC_GHOSTS && (t & GEON_BARNES_HUT)) {
+        // the reference's periodic walk over image particles (integrator.js:201-258)
+        if (p->far_field != GEON_FAR_NONE) { *why = "far-field corrections run in min-image or windowed periodic mode"; return 1; }
+        if (t & (GEON_ONE_PN | GEON_RADIATION)) { *why = "1PN and radiation run in min-image or windowed periodic mode"; return 1; }
+        if (p->collision_mode == GEON_COL_MERGE) { *why = "merge collisions run in min-image or windowed periodic mode"; return 1; }
+        const bool dl = p->signal_delay == GEON_DELAY_ON || (p->signal_delay == GEON_DELAY_FOLLOW && (t & GEON_RELATIVITY));
+        if (dl) { *why = "signal delay runs in min-image or windowed periodic mode"; return 1; }
+    }
     if (p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) {
-        if (p->periodic_mode != GEON_PERIODIC_MIN_IMAGE) { *why = "Klein and RP2 run min-image periodic mode"; return 1; }
+        if (p->periodic_mode == GEON_PERIODIC_WINDOWED) { *why = "Klein and RP2 run min-image or image-particle periodic mode"; return 1; }
         if (p->far_field != GEON_FAR_NONE) { *why = "far-field corrections run on open or torus domains"; return 1; }
     }
     if (!(p->domain_w > 0) || !(p->domain_h > 0)) { *why = "domain must be positive"; return 1; }
@@ -581,6 +602,55 @@ int sort_and_gather(GeonContext* h) {
     return GEON_OK;
 }
 
+int ghost_alloc(GeonContext* h, int factor);
+
+bool ghosts_on(const GeonParams& p) {
+    return p.boundary_mode == GEON_BOUND_LOOP && p.periodic_mode == GEON_PERIODIC_GHOSTS &&
+           (p.toggles0 & GEON_BARNES_HUT);
+}
+
+// The reference's periodic tree (integrator.js:201-258): the particles sorted
+// as usual, their images within max(W, H) * theta of an edge generated, and
+// every tree point (particle or image) sorted and built over.
+int ghost_tree(GeonContext* h) {
+    const int n = h->n_host;
+    const GeonParams& p = h->prm;
+    // a particle has images on both sides of an axis only when the margin
+    // exceeds half that side: at most (1 + sides_x)(1 + sides_y) tree points
+    const double margin = std::max(p.domain_w, p.domain_h) * p.theta;     // integrator.js:204
+    const int sx = margin > 0.5 * p.domain_w ? 2 : 1, sy = margin > 0.5 * p.domain_h ? 2 : 1;
+    const int factor = (1 + sx) * (1 + sy);
+    int rc;
+    if ((rc = ghost_alloc(h, factor))) return rc;
+    if ((rc = sort_and_gather(h))) return rc;
+    GhostState& g = h->gh;
+    GhostArgs ga;
+    ga.x = F(h, C_X); ga.y = F(h, C_Y); ga.m = F(h, C_MASS); ga.q = F(h, C_CHARGE);
+    ga.wx = F(h, C_WX); ga.wy = F(h, C_WY); ga.vx = F(h, C_VX); ga.vy = F(h, C_VY);
+    ga.ang_vel = F(h, C_ANGVEL); ga.radius = F(h, C_RADIUS); ga.mu = F(h, C_MU); ga.L = F(h, C_L);
+    ga.qq = I(h, C_QQ); ga.ctrl = h->ctrl;
+    ga.dom_w = (float)p.domain_w; ga.dom_h = (float)p.domain_h;
+    ga.margin = (float)margin;
+    ga.topo = p.topology;
+    ga.cap_points = (int)g.cap;
+    ga.out = g.col;
+    const int n4 = factor * n;
+    const KeyParams kp = key_params(p);
+    Timer t(h, 1);
+    ghost_tree_points(h->stream, ga, n, g.cnt, g.off, h->tb.scan_tmp, g.ctrl);
+    auto fcol = [&](int c) { return reinterpret_cast<float*>(g.col.c[c]); };
+    launch_keys(h->stream, fcol(G_X), fcol(G_Y), n4, kp, g.tb.keys, g.tb.perm, g.ctrl);
+    radix_sort(h->stream, g.tb, n4, 2 * kKeyLevels + 1, g.ctrl);
+    Columns cols;
+    cols.ncols = kGhostCols;
+    for (int c = 0; c < kGhostCols; c++) { cols.src[c] = g.col.c[c]; cols.dst[c] = g.alt.c[c]; }
+    launch_gather(h->stream, cols, g.tb.perm, n4, g.ctrl);
+    std::swap(g.col, g.alt);
+    build_tree(h->stream, g.tb, fcol(G_X), fcol(G_Y), fcol(G_M), fcol(G_WX), fcol(G_WY), fcol(G_MU), fcol(G_L),
+               reinterpret_cast<int32_t*>(g.col.c[G_QQ]), n4, kp, g.ctrl, false);
+    return GEON_OK;
+}
+
 // computeAllForces + external fields at the current state (forces.js:61-154,
 // integrator.js:563-592), preceded by the tree rebuild (integrator.js:1340-1348).
 // reuse_tree: the tree built for this substep's 1PN pass stands (integrator.js:1340-1348
@@ -599,7 +669,14 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
     }
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
     if (n == 0) return GEON_OK;
-    if (bh) {
+    const bool ghosts = bh && ghosts_on(h->prm);
+    if (ghosts) {
+        int rc = ghost_tree(h);
+        if (rc) return rc;
+        h->tree_valid = true;
+        h->tree_ghost = true;
+    } else if (bh) {
+        h->tree_ghost = false;
         if (!reuse_tree) {
             int rc = sort_and_gather(h);
             if (rc) return rc;
@@ -625,11 +702,24 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
     ForceArgs fa = force_args(h);
     if (bh && far == GEON_FAR_MULTIPOLE) fa.far_mode = kFarMultipole;
     if (bh && far == GEON_FAR_LEARNED) fa.far_mode = kFarList;
+    if (ghosts) {
+        // sources: the tree points; observers: the particles
+        GhostState& g = h->gh;
+        ghost_sources(h->stream, g.col, axion ? F(h, C_AX) : nullptr, axion ? F(h, C_YUK) : nullptr,
+                      higgs ? F(h, C_HIGGS) : nullptr, g.src4, g.src8a, g.src8b, g.factor * n, g.ctrl);
+        fa.obs4 = h->src4; fa.obs8a = h->src8a; fa.obs8b = h->src8b;
+        fa.src4 = g.src4; fa.src8a = g.src8a; fa.src8b = g.src8b;
+        fa.src_orig = reinterpret_cast<const int32_t*>(g.col.c[G_ORIG]);
+        fa.nodes = g.tb.nodes;
+        fa.max_nodes = 1 + 4 * g.tb.cap_divided;
+        fa.node_mom = nullptr;
+    }
     {
         Timer t(h, 0);
         if (bh) launch_walk(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
         else launch_direct(h->stream, fa, feature_mask(h->prm), h->prm.kahan != 0);
     }
+    if (ghosts) ghost_finish(h->stream, h->gh.ctrl, h->ctrl);
     if (bh && far == GEON_FAR_LEARNED && !collect) {
         Timer t(h, 4);
         launch_far_mlp(h->stream, far_args(h), n);
@@ -823,28 +913,9 @@ __global__ void k_split_pairs(const uint64_t* __restrict__ keys, int32_t* __rest
     pb[i] = (int32_t)(k & ((uint64_t(1) << bits) - 1));
 }
 
-int allocate(GeonContext* h) {
-    const size_t cap = (size_t)h->cap;
+// Buffers of one sort + tree build over up to `cap` points.
+int tree_alloc(GeonContext* h, TreeBufs& tb, size_t cap) {
     int rc;
-    for (int c = 0; c < N_STATE; c++) {
-        if ((rc = dmalloc(h, &h->st[c], cap))) return rc;
-        if ((rc = dmalloc(h, &h->st_alt[c], cap))) return rc;
-    }
-    for (int c = 0; c < N_FORCE; c++) {
-        if ((rc = dmalloc(h, &h->fc[c], cap))) return rc;
-        if ((rc = dmalloc(h, &h->fc_alt[c], cap))) return rc;
-    }
-    if ((rc = dmalloc(h, &h->alive, cap))) return rc;
-    if ((rc = dmalloc(h, &h->alive_scan, cap))) return rc;
-    if ((rc = dmalloc(h, &h->compact_perm, cap))) return rc;
-    if ((rc = dmalloc(h, &h->src4, cap))) return rc;
-    if ((rc = dmalloc(h, &h->src8a, cap))) return rc;
-    if ((rc = dmalloc(h, &h->src8b, cap))) return rc;
-    if ((rc = dmalloc(h, &h->ctrl, 1))) return rc;
-    if ((rc = dmalloc(h, &h->hist_fe, cap))) return rc;
-    if ((rc = dmalloc(h, &h->hist_cd, cap))) return rc;
-    if ((rc = dmalloc(h, &h->dead, cap))) return rc;
-    TreeBufs& tb = h->tb;
     tb.cap_particles = (int64_t)cap;
     tb.cap_divided = (int64_t)cap + 1024;
     const size_t nodes = 1 + 4 * (size_t)tb.cap_divided;
@@ -871,6 +942,73 @@ int allocate(GeonContext* h) {
     if ((rc = dmalloc(h, &tb.pre_mom, 9 * (cap + 1)))) return rc;
     if ((rc = dmalloc(h, &tb.scan_tmp, nblk + 1024))) return rc;
     if ((rc = dmalloc(h, &tb.scan_tmp64, 9 * (nblk + 1024)))) return rc;
+    return GEON_OK;
+}
+
+void tree_free(TreeBufs& tb) {
+    hipFree(tb.keys); hipFree(tb.keys_alt); hipFree(tb.perm); hipFree(tb.perm_alt); hipFree(tb.hist);
+    hipFree(tb.digit_total);
+    hipFree(tb.owned); hipFree(tb.rank_start); hipFree(tb.divnode); hipFree(tb.pre); hipFree(tb.nodes);
+    hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64); hipFree(tb.node_qq64);
+    hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
+    hipFree(tb.node_first); hipFree(tb.node_mom); hipFree(tb.pre_mom);
+    tb = TreeBufs{};
+}
+
+void ghost_release(GeonContext* h);
+
+// Periodic images in the tree (GEON_PERIODIC_GHOSTS): the tree points (a
+// particle and its images: `factor` per particle at most), their sort and
+// tree, allocated at the first such pass.
+int ghost_alloc(GeonContext* h, int factor) {
+    GhostState& g = h->gh;
+    if (g.cap && g.factor >= factor) return GEON_OK;
+    ghost_release(h);
+    const size_t cap4 = (size_t)factor * (size_t)h->cap;
+    int rc;
+    for (int c = 0; c < kGhostCols; c++)
+        if ((rc = dmalloc(h, &g.col.c[c], cap4)) || (rc = dmalloc(h, &g.alt.c[c], cap4))) return rc;
+    if ((rc = dmalloc(h, &g.cnt, (size_t)h->cap)) || (rc = dmalloc(h, &g.off, (size_t)h->cap)) ||
+        (rc = dmalloc(h, &g.src4, cap4)) || (rc = dmalloc(h, &g.src8a, cap4)) || (rc = dmalloc(h, &g.src8b, cap4)) ||
+        (rc = dmalloc(h, &g.ctrl, 1)))
+        return rc;
+    if ((rc = tree_alloc(h, g.tb, cap4))) return rc;
+    g.cap = (int64_t)cap4;
+    g.factor = factor;
+    return GEON_OK;
+}
+
+void ghost_release(GeonContext* h) {
+    GhostState& g = h->gh;
+    if (!g.cap) return;
+    for (int c = 0; c < kGhostCols; c++) { hipFree(g.col.c[c]); hipFree(g.alt.c[c]); }
+    hipFree(g.cnt); hipFree(g.off); hipFree(g.src4); hipFree(g.src8a); hipFree(g.src8b); hipFree(g.ctrl);
+    tree_free(g.tb);
+    g = GhostState{};
+}
+
+int allocate(GeonContext* h) {
+    const size_t cap = (size_t)h->cap;
+    int rc;
+    for (int c = 0; c < N_STATE; c++) {
+        if ((rc = dmalloc(h, &h->st[c], cap))) return rc;
+        if ((rc = dmalloc(h, &h->st_alt[c], cap))) return rc;
+    }
+    for (int c = 0; c < N_FORCE; c++) {
+        if ((rc = dmalloc(h, &h->fc[c], cap))) return rc;
+        if ((rc = dmalloc(h, &h->fc_alt[c], cap))) return rc;
+    }
+    if ((rc = dmalloc(h, &h->alive, cap))) return rc;
+    if ((rc = dmalloc(h, &h->alive_scan, cap))) return rc;
+    if ((rc = dmalloc(h, &h->compact_perm, cap))) return rc;
+    if ((rc = dmalloc(h, &h->src4, cap))) return rc;
+    if ((rc = dmalloc(h, &h->src8a, cap))) return rc;
+    if ((rc = dmalloc(h, &h->src8b, cap))) return rc;
+    if ((rc = dmalloc(h, &h->ctrl, 1))) return rc;
+    if ((rc = dmalloc(h, &h->hist_fe, cap))) return rc;
+    if ((rc = dmalloc(h, &h->hist_cd, cap))) return rc;
+    if ((rc = dmalloc(h, &h->dead, cap))) return rc;
+    if ((rc = tree_alloc(h, h->tb, cap))) return rc;
     if ((rc = dmalloc(h, &h->stat_partial, std::max<size_t>(16 * 1024, 3 * ((cap + 255) / 256) + 64)))) return rc;
     if ((rc = dmalloc(h, &h->stat_out, 32))) return rc;
     h->field_g = h->prm.field_grid > 0 ? h->prm.field_grid : 128;
@@ -912,12 +1050,8 @@ void release(GeonContext* h) {
     hipFree(h->hist4); hipFree(h->hist_aw); hipFree(h->hist_fe); hipFree(h->hist_cd); hipFree(h->ev_time);
     hipFree(h->dead);
     TreeBufs& tb = h->tb;
-    hipFree(tb.keys); hipFree(tb.keys_alt); hipFree(tb.perm); hipFree(tb.perm_alt); hipFree(tb.hist);
-    hipFree(tb.digit_total);
-    hipFree(tb.owned); hipFree(tb.rank_start); hipFree(tb.divnode); hipFree(tb.pre); hipFree(tb.nodes);
-    hipFree(tb.node_mass64); hipFree(tb.node_comx64); hipFree(tb.node_comy64); hipFree(tb.node_qq64);
-    hipFree(tb.node_path); hipFree(tb.node_npts); hipFree(tb.scan_tmp); hipFree(tb.scan_tmp64);
-    hipFree(tb.node_first); hipFree(tb.node_mom); hipFree(tb.pre_mom);
+    tree_free(tb);
+    ghost_release(h);
     merge_release(h);
     rad_release(h);
     far_release(h);
@@ -1289,6 +1423,7 @@ int pairs_count(GeonContext* h, PairArgs* pa_out, int64_t* np) {
     if ((rc = read_ctrl(h))) return rc;
     *np = h->ctrl_host->n_divided;
     h->tree_valid = false;                    // n_divided no longer describes the tree
+    h->tree_ghost = false;
     return GEON_OK;
 }
 
@@ -1777,11 +1912,19 @@ int geon_build_tree(GeonHandle h) {
     if (!h) return GEON_EINVAL;
     if (h->n_host == 0) return GEON_OK;
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
-    int rc = sort_and_gather(h);
-    if (rc) return rc;
+    int rc;
+    if (ghosts_on(h->prm)) {                    // the tree over particles and their images
+        if ((rc = ghost_tree(h))) return rc;
+        ghost_finish(h->stream, h->gh.ctrl, h->ctrl);
+        h->tree_valid = true;
+        h->tree_ghost = true;
+        return read_ctrl(h);
+    }
+    if ((rc = sort_and_gather(h))) return rc;
     build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU), F(h, C_L),
                I(h, C_QQ), h->n_host, key_params(h->prm), h->ctrl, want_moments(h));
     h->tree_valid = true;
+    h->tree_ghost = false;
     return read_ctrl(h);
 }
 
@@ -1795,17 +1938,18 @@ int geon_get_tree(GeonHandle h, GeonTreeNode* out, int64_t cap, int64_t* n_nodes
     if (!out) return GEON_OK;
     const int64_t m = std::min(cap, nn);
     if (m <= 0) return GEON_OK;
+    const TreeBufs& tb = h->tree_ghost ? h->gh.tb : h->tb;
     std::vector<Node> nodes((size_t)m);
     std::vector<double> mass((size_t)m), cx((size_t)m), cy((size_t)m), qq((size_t)m);
     std::vector<uint64_t> path((size_t)m);
     std::vector<int32_t> npts((size_t)m);
-    HIP_OK(hipMemcpyAsync(nodes.data(), h->tb.nodes, sizeof(Node) * m, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipMemcpyAsync(mass.data(), h->tb.node_mass64, 8 * m, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipMemcpyAsync(cx.data(), h->tb.node_comx64, 8 * m, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipMemcpyAsync(cy.data(), h->tb.node_comy64, 8 * m, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipMemcpyAsync(qq.data(), h->tb.node_qq64, 8 * m, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipMemcpyAsync(path.data(), h->tb.node_path, 8 * m, hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipMemcpyAsync(npts.data(), h->tb.node_npts, 4 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(nodes.data(), tb.nodes, sizeof(Node) * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(mass.data(), tb.node_mass64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(cx.data(), tb.node_comx64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(cy.data(), tb.node_comy64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(qq.data(), tb.node_qq64, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(path.data(), tb.node_path, 8 * m, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(npts.data(), tb.node_npts, 4 * m, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     for (int64_t i = 0; i < m; i++) {
         out[i].path = path[i];

@@ -118,7 +118,35 @@ struct ForceArgs {
     // 1PN recompute without history: 0 takes no leaf source at all (the
     // reference takes them only from a light cone that nothing has recorded)
     int pn_leaves = 1;
+    // observers when they are not the sources (periodic images in the tree):
+    // null means src4/src8a/src8b; src_orig[j] is the particle source j images
+    const float4 *obs4 = nullptr, *obs8a = nullptr, *obs8b = nullptr;
+    const int32_t* src_orig = nullptr;
 };
+
+// ---- periodic images in the tree (k_ghost.hip) ----
+enum GhostCol { G_X, G_Y, G_M, G_Q, G_WX, G_WY, G_VX, G_VY, G_AV, G_RAD, G_MU, G_L, G_QQ, G_ORIG, kGhostCols };
+struct GhostCols {
+    uint32_t* c[kGhostCols];
+};
+struct GhostArgs {
+    const float *x, *y, *m, *q, *wx, *wy, *vx, *vy, *ang_vel, *radius, *mu, *L;
+    const int32_t* qq;
+    const Ctrl* ctrl;           // the particles' (n_alive)
+    float dom_w, dom_h, margin;
+    int topo;
+    int cap_points;             // tree points the output columns hold
+    GhostCols out;
+};
+// Tree points of the particles (n of ctrl) and their images, unsorted; the
+// count goes to gctrl->n_alive.
+void ghost_tree_points(hipStream_t s, const GhostArgs& a, int n_upper, int32_t* cnt, int32_t* off, int32_t* scan_tmp,
+                       Ctrl* gctrl);
+// Walk sources of the sorted tree points; the modulations (axMod, yukMod,
+// higgsMod) are those of the particle a point images, current at the walk.
+void ghost_sources(hipStream_t s, const GhostCols& col, const float* ax, const float* yuk, const float* higgs,
+                   float4* src4, float4* src8a, float4* src8b, int n_upper, const Ctrl* gctrl);
+void ghost_finish(hipStream_t s, const Ctrl* gctrl, Ctrl* ctrl);
 constexpr int kFarMultipole = 1, kFarList = 2;
 constexpr int kFarLaneShift = 26;
 constexpr uint32_t kFarNodeMask = (1u << kFarLaneShift) - 1;

@@ -58,9 +58,9 @@ void launch_prepare_sources(hipStream_t s, const float* x, const float* y, const
 }
 
 __device__ __forceinline__ void load_obs(const ForceArgs& a, int i, Obs& o) {
-    const float4 s4 = a.src4[i];
-    const float4 s8a = a.src8a[i];
-    const float4 s8b = a.src8b[i];
+    const float4 s4 = (a.obs4 ? a.obs4 : a.src4)[i];
+    const float4 s8a = (a.obs8a ? a.obs8a : a.src8a)[i];
+    const float4 s8b = (a.obs8b ? a.obs8b : a.src8b)[i];
     o.x = s4.x; o.y = s4.y; o.m = s4.z; o.q = s4.w;
     o.vx = s8a.x; o.vy = s8a.y; o.ang_vel = s8a.z; o.mu = s8a.w;
     o.L = s8b.x; o.ax = s8b.y; o.yuk = s8b.z; o.higgs = s8b.w;
@@ -175,7 +175,8 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
                                                    s8b.w);
                         c_pair++;
                     }
-                } else if (mine && j != i) {
+                } else if (mine && (a.src_orig ? a.src_orig[j] : j) != i) {
+                    // with periodic images in the tree: not the observer nor one of its images
                     float rx, ry;
                     min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
                     pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x,
