@@ -32,7 +32,7 @@ enum BForce {
     BF_FX, BF_FY, BF_PE, BF_BZ, BF_DBZDX, BF_DBZDY, BF_BGZ, BF_DBGZDX, BF_DBGZDY, BF_FDRAG, BF_TIDAL, BF_F1X, BF_F1Y,
     NB_FORCE
 };
-constexpr int kBatchKeyBits = 2 * kKeyLevels + 1 + 9 + 1;   // Morton + outside bit + system id (< 2^10)
+constexpr int kBatchKeyBits = 2 * kBatchKeyLevels + 1 + 9 + 1;   // Morton + outside bit + system id (< 2^10)
 constexpr int kMaxSystems = 1 << 10;
 constexpr int kChunk = 4;           // substep bodies per captured graph (even: the state ping-pong returns)
 
@@ -150,8 +150,8 @@ KeyParams keyp(const GeonParams& p) {
     const double W = p.domain_w, H = p.domain_h;
     k.xmin = W / 2 - 2 * W;
     k.ymin = H / 2 - 2 * H;
-    k.scale_x = (double)(1u << kKeyLevels) / (4 * W);
-    k.scale_y = (double)(1u << kKeyLevels) / (4 * H);
+    k.scale_x = (double)(1u << kBatchKeyLevels) / (4 * W);
+    k.scale_y = (double)(1u << kBatchKeyLevels) / (4 * H);
     k.root_hw = 2 * W;
     return k;
 }

@@ -15,9 +15,10 @@
 namespace geon {
 
 constexpr int kWave = 64;
-constexpr int kKeyLevels = 24;          // quadtree levels resolved by the Morton key
+constexpr int kKeyLevels = 31;          // quadtree levels resolved by the Morton key of one system
+constexpr int kBatchKeyLevels = 24;     // of a batch, whose keys also carry the system id
 constexpr int kLeafCap = 4;             // QUADTREE_CAPACITY (config.js:59)
-constexpr int kWalkStack = 96;          // per-wave node stack: 3 * levels + 4 < 96
+constexpr int kWalkStack = 100;         // per-wave node stack: 3 * levels + 4 <= 97 entries
 constexpr float kEpsMass = 1e-12f;      // EPSILON (config.js:30), node-mass skip
 
 // Feature bits baked into kernel instantiations.

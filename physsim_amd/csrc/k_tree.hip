@@ -539,11 +539,11 @@ static void prefix_columns(hipStream_t s, const IN& in, double* bsum, double* pr
 // ---------------------------------------------------------------------------
 // Common prefix of two keys in tree levels; -1 when they differ above the
 // Morton bits (the outside-the-root-box bit, or a batch's system id).
-__device__ __forceinline__ int cpl_levels(uint64_t a, uint64_t b) {
+__device__ __forceinline__ int cpl_levels(uint64_t a, uint64_t b, int L) {
     const uint64_t d = a ^ b;
-    if (d == 0) return kKeyLevels;
-    if (d >> (2 * kKeyLevels)) return -1;
-    return (__clzll((long long)d) - (64 - 2 * kKeyLevels)) >> 1;
+    if (d == 0) return L;
+    if (d >> (2 * L)) return -1;
+    return (__clzll((long long)d) - (64 - 2 * L)) >> 1;
 }
 
 __global__ void k_count_tree(const uint64_t* __restrict__ keys, Ctrl* __restrict__ ctrl) {
@@ -562,10 +562,10 @@ __global__ void k_tree_levels(const uint64_t* __restrict__ keys, int32_t* __rest
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t k = keys[i];
-    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k);
+    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k, kKeyLevels);
     int D = -1;
     if (i + kLeafCap < n) {
-        const int cp = cpl_levels(k, keys[i + kLeafCap]);
+        const int cp = cpl_levels(k, keys[i + kLeafCap], kKeyLevels);
         if (cp == kKeyLevels && c < kKeyLevels) atomicAdd((unsigned long long*)&ctrl->depth_overflow, 1ull);
         D = min(cp, kKeyLevels - 1);
     }
@@ -578,9 +578,9 @@ __global__ void k_tree_scatter(const uint64_t* __restrict__ keys, const int32_t*
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t k = keys[i];
-    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k);
+    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k, kKeyLevels);
     int D = -1;
-    if (i + kLeafCap < n) D = min(cpl_levels(k, keys[i + kLeafCap]), kKeyLevels - 1);
+    if (i + kLeafCap < n) D = min(cpl_levels(k, keys[i + kLeafCap], kKeyLevels), kKeyLevels - 1);
     int r = rank_start[i];
     for (int d = c + 1; d <= D; d++, r++) {
         if (r < cap_div) divnode[r] = make_int2(i, d);
@@ -609,17 +609,19 @@ struct NodeBuild {
     double aspect_x, aspect_y;  // box extent per unit of `size` along x and y (1 and H/W)
     int64_t cap_div;
     bool batch;                 // roots are written per system by k_batch_roots, not at slot 0
+    int levels;                 // levels the keys resolve
 };
 
-__device__ __forceinline__ uint64_t prefix_at(uint64_t key, int level) {
-    return key >> (2 * (kKeyLevels - level));      // level 0 keeps the bits above the Morton code
+__device__ __forceinline__ uint64_t prefix_at(uint64_t key, int level, int L) {
+    return key >> (2 * (L - level));      // level 0 keeps the bits above the Morton code
 }
 
 // First index j in [lo, hi) with prefix_at(keys[j], level) >= target (keys sorted).
-__device__ __forceinline__ int lower_bound_prefix(const uint64_t* keys, int lo, int hi, int level, uint64_t target) {
+__device__ __forceinline__ int lower_bound_prefix(const uint64_t* keys, int lo, int hi, int level, uint64_t target,
+                                                  int L) {
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (prefix_at(keys[mid], level) < target) lo = mid + 1; else hi = mid;
+        if (prefix_at(keys[mid], level, L) < target) lo = mid + 1; else hi = mid;
     }
     return lo;
 }
@@ -727,7 +729,7 @@ __device__ void write_node(const NodeBuild& b, int slot, int level, uint64_t pat
 }
 
 __device__ __forceinline__ int divided_rank(const NodeBuild& b, int a, int level) {
-    const int ca = a == 0 ? -1 : cpl_levels(b.keys[a - 1], b.keys[a]);
+    const int ca = a == 0 ? -1 : cpl_levels(b.keys[a - 1], b.keys[a], b.levels);
     return b.rank_start[a] + (level - ca - 1);
 }
 
@@ -744,22 +746,23 @@ __global__ void k_tree_nodes(NodeBuild b, Ctrl* __restrict__ ctrl) {
     for (int r = gid; r < ndiv && r < b.cap_div; r += gridDim.x * blockDim.x) {
         const int2 dn = b.divnode[r];
         const int lo = dn.x, level = dn.y;
-        const uint64_t P = prefix_at(b.keys[lo], level);
+        const int L = b.levels;
+        const uint64_t P = prefix_at(b.keys[lo], level, L);
         // end of this node's range: galloping search for the first key with a larger prefix
         int step = 1, hi = lo + 1;
-        while (hi < n && prefix_at(b.keys[hi], level) == P) { hi = lo + step; step <<= 1; if (hi > n) hi = n; }
+        while (hi < n && prefix_at(b.keys[hi], level, L) == P) { hi = lo + step; step <<= 1; if (hi > n) hi = n; }
         int lo_b = lo, hi_b = hi;
         // now prefix(keys[hi_b]) != P or hi_b == n; binary search the boundary in [lo_b, hi_b)
-        hi = lower_bound_prefix(b.keys, lo_b, hi_b, level, P + 1);
+        hi = lower_bound_prefix(b.keys, lo_b, hi_b, level, P + 1, L);
         const int cl = level + 1;
         int bound[5];
         bound[0] = lo;
         bound[4] = hi;
-        for (int q = 1; q < 4; q++) bound[q] = lower_bound_prefix(b.keys, lo, hi, cl, (P << 2) | (uint64_t)q);
+        for (int q = 1; q < 4; q++) bound[q] = lower_bound_prefix(b.keys, lo, hi, cl, (P << 2) | (uint64_t)q, L);
         for (int q = 0; q < 4; q++) {
             const int a = bound[q], e = bound[q + 1];
             const int cnt = e - a;
-            bool div = cnt > kLeafCap && cl <= kKeyLevels - 1;
+            bool div = cnt > kLeafCap && cl <= L - 1;
             int child_base = 0;
             if (div) {
                 const int rc = divided_rank(b, a, cl);
@@ -812,6 +815,7 @@ void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, con
     b.aspect_x = 1.0;
     b.aspect_y = (kp.scale_x / kp.scale_y);
     b.cap_div = tb.cap_divided;
+    b.levels = kKeyLevels;
     int blocks = (int)((tb.cap_divided + 255) / 256);
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
@@ -826,7 +830,7 @@ void build_tree(hipStream_t s, TreeBufs& tb, const float* x, const float* y, con
 // restart at every slice, and the tree of one system never depends on the
 // others (a system's results are the same alone or in any batch).
 // ---------------------------------------------------------------------------
-constexpr int kBatchSysShift = 2 * kKeyLevels + 1;    // system id above the outside bit
+constexpr int kBatchSysShift = 2 * kBatchKeyLevels + 1;    // system id above the outside bit
 static_assert(kScanTile == kBatchAlign, "batch slices must align to the prefix-scan tile");
 
 __global__ void k_batch_keys(const float* __restrict__ x, const float* __restrict__ y,
@@ -835,13 +839,13 @@ __global__ void k_batch_keys(const float* __restrict__ x, const float* __restric
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const double px = (double)x[i], py = (double)y[i];
-    const double span_x = (double)(1u << kKeyLevels) / kp.scale_x;
-    const double span_y = (double)(1u << kKeyLevels) / kp.scale_y;
+    const double span_x = (double)(1u << kBatchKeyLevels) / kp.scale_x;
+    const double span_y = (double)(1u << kBatchKeyLevels) / kp.scale_y;
     const bool inside = valid[i] && px >= kp.xmin && px <= kp.xmin + span_x && py >= kp.ymin && py <= kp.ymin + span_y;
-    uint64_t key = 1ull << (2 * kKeyLevels);
+    uint64_t key = 1ull << (2 * kBatchKeyLevels);
     if (inside) {
-        const uint32_t cx = cell_index(px, kp.xmin, kp.scale_x, kKeyLevels);
-        const uint32_t cy = cell_index(py, kp.ymin, kp.scale_y, kKeyLevels);
+        const uint32_t cx = cell_index(px, kp.xmin, kp.scale_x, kBatchKeyLevels);
+        const uint32_t cy = cell_index(py, kp.ymin, kp.scale_y, kBatchKeyLevels);
         key = spread2(cx) | (spread2(cy) << 1);
     }
     keys[i] = key | ((uint64_t)sys[i] << kBatchSysShift);
@@ -853,7 +857,7 @@ __global__ void k_batch_count(const uint64_t* __restrict__ keys, const int32_t* 
                               int32_t* __restrict__ n_in, int S, const Ctrl* __restrict__ ctrl) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S || ctrl->n_tree == 0) return;       // no system active: nothing to build
-    const uint64_t out_bit = 1ull << (2 * kKeyLevels);
+    const uint64_t out_bit = 1ull << (2 * kBatchKeyLevels);
     int lo = soff[s], hi = soff[s + 1];
     const int a = lo;
     while (lo < hi) {
@@ -870,13 +874,13 @@ __global__ void k_batch_levels(const uint64_t* __restrict__ keys, const int32_t*
     if (i >= total) return;
     const uint64_t k = keys[i];
     seg_first[i] = soff[k >> kBatchSysShift];
-    if (k & (1ull << (2 * kKeyLevels))) { owned[i] = 0; return; }
-    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k);
+    if (k & (1ull << (2 * kBatchKeyLevels))) { owned[i] = 0; return; }
+    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k, kBatchKeyLevels);
     int D = -1;
     if (i + kLeafCap < total) {
-        const int cp = cpl_levels(k, keys[i + kLeafCap]);
-        if (cp == kKeyLevels && c < kKeyLevels) atomicAdd((unsigned long long*)&ctrl->depth_overflow, 1ull);
-        D = min(cp, kKeyLevels - 1);
+        const int cp = cpl_levels(k, keys[i + kLeafCap], kBatchKeyLevels);
+        if (cp == kBatchKeyLevels && c < kBatchKeyLevels) atomicAdd((unsigned long long*)&ctrl->depth_overflow, 1ull);
+        D = min(cp, kBatchKeyLevels - 1);
     }
     owned[i] = max(0, D - c);
 }
@@ -887,10 +891,10 @@ __global__ void k_batch_scatter(const uint64_t* __restrict__ keys, const int32_t
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const uint64_t k = keys[i];
-    if (k & (1ull << (2 * kKeyLevels))) return;
-    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k);
+    if (k & (1ull << (2 * kBatchKeyLevels))) return;
+    const int c = i == 0 ? -1 : cpl_levels(keys[i - 1], k, kBatchKeyLevels);
     int D = -1;
-    if (i + kLeafCap < total) D = min(cpl_levels(k, keys[i + kLeafCap]), kKeyLevels - 1);
+    if (i + kLeafCap < total) D = min(cpl_levels(k, keys[i + kLeafCap], kBatchKeyLevels), kBatchKeyLevels - 1);
     int r = rank_start[i];
     for (int d = c + 1; d <= D; d++, r++) {
         if (r < cap_div) divnode[r] = make_int2(i, d);
@@ -942,6 +946,7 @@ void build_tree_batch(hipStream_t s, TreeBufs& tb, const BatchTreeArgs& ba, cons
     b.first = tb.node_first;
     b.seg_first = ba.seg_first;
     b.batch = true;
+    b.levels = kBatchKeyLevels;
     b.mom = nullptr;
     b.pre_mom = nullptr;
     b.root_size = 2.0 * kp.root_hw;

@@ -278,3 +278,39 @@ def test_collision_pairs_bit_exact(periodic, oracle_lib):
     oa, ob = o.collision_pairs()
     assert len(oa) > 100
     assert np.array_equal(ga, oa) and np.array_equal(gb, ob)
+
+
+def test_tree_resolves_close_points_like_the_oracle(oracle_lib):
+    """Points a few FP32 ulps apart share a cell of 24 levels; the reference
+    keeps subdividing until each leaf holds at most 4 (quadtree.js:203-231:
+    the depth counter restarts for every re-inserted point, so the split goes
+    as deep as the points need).  The keys resolve 31 levels, which separates
+    FP32 points this close at these coordinates: the tree is the oracle's, bit
+    for bit, with nothing left in an over-full finest cell."""
+    st, prm, _ = W.config1(300)
+    g0 = np.random.default_rng(17)
+    xs, ys = [st["x"]], [st["y"]]
+    for c in range(5):
+        x0 = np.float32(g0.uniform(20, 44))
+        y0 = np.float32(g0.uniform(20, 44))
+        xs.append(np.array([np.nextafter(x0, np.float32(100), dtype=np.float32) if k % 2 else x0 for k in range(6)],
+                           np.float32).astype(np.float64) + np.arange(6) // 2 * np.spacing(np.float32(x0)))
+        ys.append(np.full(6, y0, np.float64) + np.arange(6) % 3 * np.spacing(np.float32(y0)))
+    n = 300 + 30
+    ext = dict(x=np.concatenate(xs), y=np.concatenate(ys))
+    for k in ("wx", "wy", "mass", "charge", "angw"):
+        ext[k] = np.concatenate([st[k], np.resize(st[k], 30)])
+    assert len(np.unique(np.stack([ext["x"], ext["y"]], 1), axis=0)) == n      # distinct, some by one ulp
+    g, o = _pair(ext, prm, oracle_lib)
+    g.build_tree()
+    tg, to = _canon(g.tree()), _canon(o.tree())
+    print("levels: gpu %d oracle %d" % (tg["level"].max(), to["level"].max()))
+    assert to["level"].max() > 24
+    assert g.counters()["depth_overflow"] == 0
+    assert len(tg) == len(to)
+    for f in ("level", "path", "divided", "count", "charge_quanta"):
+        assert np.array_equal(tg[f], to[f]), f
+    g.compute_forces()
+    o.compute_forces()
+    fg, fo = g.forces(n), o.forces()
+    assert _rel_l2(fg["fx"], fg["fy"], fo["fx"], fo["fy"]) < 2e-5
