@@ -217,8 +217,14 @@ int  geon_get_params(GeonHandle h, GeonParams* out);
 /* addParticle({x,y,vx,vy,mass,charge,angw,antimatter}) -> slot | GEON_EFULL
  * (gpu-physics.js:1857-1926).  Charge is quantized to multiples of 0.1 (1874). */
 int32_t geon_add_particle(GeonHandle h, const GeonParticleInit* p);
-/* removeParticle(idx) (gpu-physics.js:1936-1946). */
-int  geon_remove_particle(GeonHandle h, int32_t slot);
+/* removeParticle(idx, deathMass, deathAngVel) (gpu-physics.js:1936-1946).
+ * With signal delay on, the particle stays a retarded source: death_mass is
+ * its mass as one (the reference's _deathMass) and death_ang_vel its angular
+ * velocity in the final history sample; 0 keeps the particle's own value, as
+ * the reference's callers pass `p._deathMass || p.mass` (input.js:183).
+ * Negative or non-finite mass, or |w| r >= 1, is GEON_EINVAL.  Without signal
+ * delay the particle is simply removed (integrator.js:154). */
+int  geon_remove_particle(GeonHandle h, int32_t slot, double death_mass, double death_ang_vel);
 /* patchMass(idx, mass, baseMass) (gpu-physics.js:1955-1963). */
 int  geon_patch_mass(GeonHandle h, int32_t slot, double mass, double base_mass);
 
@@ -257,6 +263,40 @@ int  geon_particle_count(GeonHandle h, int32_t* alive);
 
 /* requestStats + readStats (gpu-physics.js:4146-4281), in FP64. */
 int  geon_get_stats(GeonHandle h, GeonStats* out);
+
+/* The selected particle of readStats (gpu-physics.js:4238-4276): its state
+ * and its force by type.  requestStats(selectedGpuIdx) (4146-4151) names it:
+ * geon_select(h, slot), -1 for none.  The force passes from then on record the
+ * selected particle's forces by type, as the reference keeps them for every
+ * particle (forces.js:36-46) and rebuilds them for display after the substeps
+ * (integrator.js:1755-1796); forces_valid is 0 until one has run.  Only that
+ * particle's are formed: the pass repeats its walk (or direct sum) with the
+ * terms summed by type, and the field, external and radiation-reaction
+ * kernels write its shares.  GEON_SF_FAR_FIELD has no reference counterpart:
+ * the learned far-field correction (GEON_FAR_LEARNED), 0 otherwise. */
+#define GEON_SF_GRAVITY     0   /* forceGravity (+ the fields' gravity, scalar-field.js:717) */
+#define GEON_SF_COULOMB     1
+#define GEON_SF_MAGNETIC    2   /* dipole force + q v x Bz */
+#define GEON_SF_GRAVITOMAG  3   /* GM dipole force + 4 m v x Bgz */
+#define GEON_SF_ONE_PN      4   /* EIH, Darwin, Bazanski, scalar Breit (the last 1PN evaluation) */
+#define GEON_SF_SPIN_CURV   5   /* Stern-Gerlach and Mathisson-Papapetrou */
+#define GEON_SF_RADIATION   6   /* Landau-Lifshitz force of the last substep */
+#define GEON_SF_YUKAWA      7
+#define GEON_SF_EXTERNAL    8   /* uniform gravity and electric field */
+#define GEON_SF_HIGGS       9
+#define GEON_SF_AXION      10
+#define GEON_SF_FAR_FIELD  11
+#define GEON_SF_COUNT      12
+typedef struct GeonSelected {
+    int32_t slot;               /* the selected slot; -1: none selected, or that particle is gone */
+    int32_t forces_valid;
+    int32_t antimatter;
+    int32_t pad;
+    double x, y, wx, wy, mass, charge, angw, base_mass, radius, vx, vy, ang_vel, mag_moment, ang_momentum;
+    double force[GEON_SF_COUNT][2];
+} GeonSelected;
+int  geon_select(GeonHandle h, int32_t slot);
+int  geon_get_selected(GeonHandle h, GeonSelected* out);
 
 /* Build the quadtree at the current positions and export it (quadtree.js:412-419). */
 int  geon_build_tree(GeonHandle h);

@@ -76,6 +76,17 @@ class GeonStats(C.Structure):
         (n, C.c_double) for n in ("radiated_energy", "radiated_px", "radiated_py")]
 
 
+SF_NAMES = ("forceGravity", "forceCoulomb", "forceMagnetic", "forceGravitomag", "force1PN", "forceSpinCurv",
+            "forceRadiation", "forceYukawa", "forceExternal", "forceHiggs", "forceAxion", "forceFarField")
+
+
+class GeonSelected(C.Structure):
+    _fields_ = [("slot", C.c_int32), ("forces_valid", C.c_int32), ("antimatter", C.c_int32), ("pad", C.c_int32)] + \
+               [(k, C.c_double) for k in ("x", "y", "wx", "wy", "mass", "charge", "angw", "base_mass", "radius",
+                                           "vx", "vy", "ang_vel", "mag_moment", "ang_momentum")] + \
+               [("force", (C.c_double * 2) * len(SF_NAMES))]
+
+
 class GeonHistoryView(C.Structure):
     _fields_ = [("x", _DP), ("y", _DP), ("vx", _DP), ("vy", _DP), ("angw", _DP), ("t", _DP)]
 
@@ -109,7 +120,7 @@ PROTOTYPES = {
     "geon_set_params": (C.c_int, [H, C.POINTER(GeonParams)]),
     "geon_get_params": (C.c_int, [H, C.POINTER(GeonParams)]),
     "geon_add_particle": (C.c_int32, [H, C.POINTER(GeonParticleInit)]),
-    "geon_remove_particle": (C.c_int, [H, C.c_int32]),
+    "geon_remove_particle": (C.c_int, [H, C.c_int32, C.c_double, C.c_double]),
     "geon_patch_mass": (C.c_int, [H, C.c_int32, C.c_double, C.c_double]),
     "geon_upload_soa": (C.c_int, [H, C.POINTER(GeonSoAView), C.c_int32]),
     "geon_download_soa": (C.c_int, [H, C.POINTER(GeonSoAView), C.c_int32, C.POINTER(C.c_int32)]),
@@ -117,6 +128,8 @@ PROTOTYPES = {
     "geon_compute_forces": (C.c_int, [H]),
     "geon_get_forces": (C.c_int, [H, C.POINTER(GeonForceView), C.c_int32]),
     "geon_get_stats": (C.c_int, [H, C.POINTER(GeonStats)]),
+    "geon_select": (C.c_int, [H, C.c_int32]),
+    "geon_get_selected": (C.c_int, [H, C.POINTER(GeonSelected)]),
     "geon_particle_count": (C.c_int, [H, C.POINTER(C.c_int32)]),
     "geon_get_history": (C.c_int, [H, C.c_int32, C.POINTER(GeonHistoryView), C.POINTER(C.c_int32)]),
     "geon_build_tree": (C.c_int, [H]),

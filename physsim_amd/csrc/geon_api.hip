@@ -146,6 +146,11 @@ struct GeonContext {
     // radiation: per-particle FP64 scratch (9 columns), block sums, quadrupole
     // sums and the radiated energy / momentum since the last upload
     double *rad_scr = nullptr, *rad_partial = nullptr, *rad_sums = nullptr, *rad_tot = nullptr;
+    // the selected particle (readStats' `selected`): its slot or -1, its index
+    // on the device, and its force record followed by its state and force row
+    int32_t sel_slot = -1;
+    int32_t* sel_idx = nullptr;
+    float* sel_rec = nullptr;
 };
 
 #define HIP_OK(expr)                                                                         \
@@ -320,6 +325,7 @@ StepArgs step_args(GeonContext* h) {
     a.jx = h->fc[K_JX]; a.jy = h->fc[K_JY];
     a.rad_e = h->rad_scr;
     a.rad_stride = h->cap;
+    if (h->sel_slot >= 0) { a.sel = h->sel_idx; a.sel_out = h->sel_rec; }
     return a;
 }
 
@@ -362,7 +368,13 @@ ForceArgs force_args(GeonContext* h) {
     a.hist.n_dead = h->n_dead;
     // no leaf source has two samples before the rings exist (every leaf skipped)
     a.hist.on = delay_on(h->prm) ? 1 : 0;
+    if (h->sel_slot >= 0) { a.sel = h->sel_idx; a.sel_out = h->sel_rec; }
     return a;
+}
+
+// The selected particle's current index (after every reordering).
+void sel_locate(GeonContext* h) {
+    if (h->sel_slot >= 0) launch_sel_find(h->stream, I(h, C_SLOT), h->sel_slot, h->sel_idx, h->n_host, h->ctrl);
 }
 
 HistWrite hist_write(GeonContext* h) {
@@ -599,6 +611,7 @@ int sort_and_gather(GeonContext* h) {
     }
     for (int c = 0; c < N_STATE; c++) std::swap(h->st[c], h->st_alt[c]);
     if (one_pn) { std::swap(h->fc[K_F1X], h->fc_alt[K_F1X]); std::swap(h->fc[K_F1Y], h->fc_alt[K_F1Y]); }
+    sel_locate(h);
     return GEON_OK;
 }
 
@@ -725,11 +738,12 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
         launch_far_mlp(h->stream, far_args(h), n);
     }
     if (fa.hist.on && h->n_dead > 0) launch_dead_forces(h->stream, fa, feature_mask(h->prm));
+    if (fa.sel_out) launch_sel_forces(h->stream, fa, bh, bh && far == GEON_FAR_LEARNED && !collect);
     if (higgs || axion) {   // field forces after the pair forces (integrator.js:1494-1507)
         Timer tm(h, 7);
         field_forces(h->stream, field_args(h, 0), field_args(h, 1), higgs, axion, !!(tg & GEON_GRAVITY),
                      !!(tg & GEON_COULOMB), !!(tg & GEON_YUKAWA), particle_view(h), h->fc[K_FX], h->fc[K_FY],
-                     h->fc[K_PEG], h->fc[K_PFI], n);
+                     h->fc[K_PEG], h->fc[K_PFI], n, fa.sel, fa.sel_out);
     }
     HIP_OK(hipGetLastError());
     return GEON_OK;
@@ -848,6 +862,7 @@ int compact(GeonContext* h, double t_death) {
     for (int c = 0; c < N_FORCE; c++) std::swap(h->fc[c], h->fc_alt[c]);
     h->n_host = survivors;
     h->tree_valid = false;
+    sel_locate(h);
     return GEON_OK;
 }
 
@@ -1049,6 +1064,7 @@ void release(GeonContext* h) {
     hipFree(h->src4); hipFree(h->src8a); hipFree(h->src8b); hipFree(h->ctrl);
     hipFree(h->hist4); hipFree(h->hist_aw); hipFree(h->hist_fe); hipFree(h->hist_cd); hipFree(h->ev_time);
     hipFree(h->dead);
+    hipFree(h->sel_idx); hipFree(h->sel_rec);
     TreeBufs& tb = h->tb;
     tree_free(tb);
     ghost_release(h);
@@ -1489,6 +1505,7 @@ void geon_destroy(GeonHandle h) {
 int geon_reset(GeonHandle h) {
     if (!h) return GEON_EINVAL;
     h->n_host = 0; h->next_slot = 0; h->forces_init = false; h->tree_valid = false;
+    h->sel_slot = -1;
     h->sim_time = 0; h->substeps_total = 0;
     hist_clear(h);
     if (h->rad_tot) HIP_OK(hipMemsetAsync(h->rad_tot, 0, 4 * sizeof(double), h->stream));
@@ -1562,6 +1579,7 @@ int geon_upload_soa(GeonHandle h, const GeonSoAView* v, int32_t n) {
     h->forces_init = false;
     h->tree_valid = false;
     h->sim_time = 0;
+    h->sel_slot = -1;           // a new system: slots name other particles
     return GEON_OK;
 }
 
@@ -1608,17 +1626,43 @@ static int find_slot(GeonHandle h, int32_t slot, int* idx) {
     return set_err(h, GEON_EINVAL, "no live particle in that slot");
 }
 
-int geon_remove_particle(GeonHandle h, int32_t slot) {
+// The death metadata follows the callers' `p._deathMass || p.mass` and
+// `p._deathAngVel || p.angVel` (input.js:183, 208-213): zero keeps the
+// particle's own value.  A retired particle acts through its dead record's
+// mass (forces.js:145-147) and through the spin of its final history sample
+// (_retireParticle, integrator.js:160-166), whose angw gives back the death
+// angular velocity under forces.js:141-143's angVel = angw / sqrt(1 + angw^2 r^2).
+int geon_remove_particle(GeonHandle h, int32_t slot, double death_mass, double death_ang_vel) {
     if (!h) return GEON_EINVAL;
+    if (!(death_mass >= 0) || !std::isfinite(death_mass) || !std::isfinite(death_ang_vel))
+        return set_err(h, GEON_EINVAL, "death mass must be finite and >= 0, death angular velocity finite");
     int idx;
     int rc = find_slot(h, slot, &idx);
     if (rc) return rc;
+    float radius = 0.f;
+    HIP_OK(hipMemcpyAsync(&radius, F(h, C_RADIUS) + idx, 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    const double r2 = (double)radius * radius, s = death_ang_vel * death_ang_vel * r2;
+    if (death_ang_vel != 0 && !(s < 1))
+        return set_err(h, GEON_EINVAL, "death angular velocity puts the surface at or above c");
     std::vector<int32_t> alive((size_t)h->n_host, 1);
     alive[idx] = 0;
     HIP_OK(hipMemcpyAsync(h->alive, alive.data(), 4 * alive.size(), hipMemcpyHostToDevice, h->stream));
-    rc = compact(h, h->sim_time);
+    const int dead0 = h->n_dead;
+    const int64_t ev = h->n_events;
+    if ((rc = compact(h, h->sim_time))) return rc;
+    if (h->n_dead > dead0) {             // signal delay on: the particle left a dead record
+        DeadRec& r = h->dead_host[(size_t)dead0];
+        if (death_mass > 0) r.mass = (float)death_mass;
+        HIP_OK(hipMemcpyAsync(h->dead + dead0, &r, sizeof(DeadRec), hipMemcpyHostToDevice, h->stream));
+        if (death_ang_vel != 0) {
+            const float aw = (float)(death_ang_vel / sqrt(1.0 - s));
+            HIP_OK(hipMemcpyAsync(h->hist_aw + (size_t)slot * kHistN + (size_t)(ev & (kHistN - 1)), &aw, 4,
+                                  hipMemcpyHostToDevice, h->stream));
+        }
+    }
     HIP_OK(hipStreamSynchronize(h->stream));
-    return rc;
+    return GEON_OK;
 }
 
 int geon_patch_mass(GeonHandle h, int32_t slot, double mass, double base_mass) {
@@ -1789,6 +1833,110 @@ int geon_get_forces(GeonHandle h, GeonForceView* v, int32_t n_slots) {
         const int s = slot[i];
         if (s < 0 || s >= n_slots) continue;
         for (int c = 0; c < N_FORCE; c++) if (outs[c]) outs[c][s] = fc[c][i];
+    }
+    return GEON_OK;
+}
+
+namespace {
+// One particle's state and force columns at the selected index, after the record.
+struct SelRow {
+    const uint32_t* c[N_STATE + N_FORCE];
+};
+__global__ void k_sel_row(SelRow r, const int32_t* __restrict__ idx, uint32_t* __restrict__ out) {
+    const int t = threadIdx.x;
+    const int i = *idx;
+    if (t < N_STATE + N_FORCE) out[t] = i >= 0 ? r.c[t][i] : 0u;
+    if (t == 0) out[N_STATE + N_FORCE] = (uint32_t)i;
+}
+constexpr int kSelRecWords = kSelFloats + N_STATE + N_FORCE + 1;
+}  // namespace
+
+// requestStats(selectedGpuIdx) (gpu-physics.js:4146-4151): the particle whose
+// per-type forces the following force passes record (-1: none).
+int geon_select(GeonHandle h, int32_t slot) {
+    if (!h) return GEON_EINVAL;
+    if (slot < -1 || slot >= h->next_slot) return set_err(h, GEON_EINVAL, "slot out of range");
+    if (slot == h->sel_slot) return GEON_OK;
+    h->sel_slot = -1;
+    if (slot < 0) return GEON_OK;
+    int rc;
+    if (!h->sel_idx && (rc = dmalloc(h, &h->sel_idx, 1))) return rc;
+    if (!h->sel_rec && (rc = dmalloc(h, &h->sel_rec, (size_t)kSelRecWords))) return rc;
+    HIP_OK(hipMemsetAsync(h->sel_rec, 0, sizeof(float) * kSelRecWords, h->stream));
+    h->sel_slot = slot;         // the kernels see the selection only once its buffers exist
+    sel_locate(h);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return GEON_OK;
+}
+
+// readStats().selected (gpu-physics.js:4238-4276).  The force types follow the
+// reference's per-particle vectors (forces.js:36-46) and its display pass after
+// the substeps (integrator.js:1755-1796): the Lorentz and gravitomagnetic
+// forces of the particle's velocity in the last pass's Bz and Bgz join the
+// magnetic and gravitomagnetic dipole forces, and the Stern-Gerlach /
+// Mathisson-Papapetrou forces form forceSpinCurv.
+int geon_get_selected(GeonHandle h, GeonSelected* out) {
+    if (!h || !out) return GEON_EINVAL;
+    memset(out, 0, sizeof(*out));
+    out->slot = -1;
+    if (h->sel_slot < 0 || h->n_host == 0) return GEON_OK;
+    SelRow r;
+    for (int c = 0; c < N_STATE; c++) r.c[c] = h->st[c];
+    for (int c = 0; c < N_FORCE; c++) r.c[N_STATE + c] = reinterpret_cast<const uint32_t*>(h->fc[c]);
+    uint32_t* row = reinterpret_cast<uint32_t*>(h->sel_rec) + kSelFloats;
+    hipLaunchKernelGGL(k_sel_row, dim3(1), dim3(64), 0, h->stream, r, h->sel_idx, row);
+    std::vector<uint32_t> w((size_t)kSelRecWords);
+    HIP_OK(hipMemcpyAsync(w.data(), h->sel_rec, sizeof(uint32_t) * kSelRecWords, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if ((int32_t)w[kSelRecWords - 1] < 0) return GEON_OK;         // the particle is gone
+    auto rec = [&](int k) { float f; memcpy(&f, &w[k], 4); return (double)f; };
+    auto st = [&](int c) { return rec(kSelFloats + c); };
+    auto fc = [&](int c) { return rec(kSelFloats + N_STATE + c); };
+    out->slot = h->sel_slot;
+    out->forces_valid = rec(SEL_VALID) != 0.0;
+    out->antimatter = (int32_t)(w[kSelFloats + C_FLAGS] & 1u);
+    out->x = st(C_X); out->y = st(C_Y); out->wx = st(C_WX); out->wy = st(C_WY);
+    out->mass = st(C_MASS); out->charge = st(C_CHARGE); out->angw = st(C_ANGW); out->base_mass = st(C_BASEMASS);
+    out->radius = st(C_RADIUS); out->vx = st(C_VX); out->vy = st(C_VY); out->ang_vel = st(C_ANGVEL);
+    out->mag_moment = st(C_MU); out->ang_momentum = st(C_L);
+    if (!out->forces_valid) return GEON_OK;
+    const uint32_t t = h->prm.toggles0;
+    const bool mag = !!(t & GEON_MAGNETIC), gm = !!(t & GEON_GRAVITOMAG);
+    double (*f)[2] = out->force;
+    auto split = [&](int type, int gf) { f[gf][0] += rec(SEL_SPLIT + 2 * type); f[gf][1] += rec(SEL_SPLIT + 2 * type + 1); };
+    auto add = [&](int k, int gf) { f[gf][0] += rec(k); f[gf][1] += rec(k + 1); };
+    split(S_GRAV, GEON_SF_GRAVITY);
+    add(SEL_FIELD_GRAV, GEON_SF_GRAVITY);                      // scalar-field.js:717-718
+    split(S_COUL, GEON_SF_COULOMB);
+    split(S_MAG, GEON_SF_MAGNETIC);
+    split(S_GM, GEON_SF_GRAVITOMAG);
+    split(S_YUK, GEON_SF_YUKAWA);
+    if (t & GEON_ONE_PN) { f[GEON_SF_ONE_PN][0] = fc(K_F1X); f[GEON_SF_ONE_PN][1] = fc(K_F1Y); }   // forces.js:581
+    add(SEL_EXT, GEON_SF_EXTERNAL);
+    add(SEL_HIGGS, GEON_SF_HIGGS);
+    add(SEL_AXION, GEON_SF_AXION);
+    if ((t & GEON_RADIATION) && (t & GEON_COULOMB)) add(SEL_RAD, GEON_SF_RADIATION);
+    add(SEL_FAR, GEON_SF_FAR_FIELD);
+    const double vx = out->vx, vy = out->vy, q = out->charge, m = out->mass;
+    if (mag || h->prm.ext_bz != 0) {                            // integrator.js:1760-1763
+        const double bz = fc(K_BZ);
+        f[GEON_SF_MAGNETIC][0] += q * vy * bz;
+        f[GEON_SF_MAGNETIC][1] -= q * vx * bz;
+    }
+    if (gm) {                                                   // 1764-1767
+        const double bgz = fc(K_BGZ);
+        f[GEON_SF_GRAVITOMAG][0] += 4.0 * m * vy * bgz;
+        f[GEON_SF_GRAVITOMAG][1] -= 4.0 * m * vx * bgz;
+    }
+    if ((t & GEON_SPIN_ORBIT) && fabs(out->ang_vel) >= 1e-12) {  // 1768-1786 (EPSILON, config.js:30)
+        if (mag && fabs(q) >= 1e-12) {
+            f[GEON_SF_SPIN_CURV][0] += out->mag_moment * fc(K_DBZDX);
+            f[GEON_SF_SPIN_CURV][1] += out->mag_moment * fc(K_DBZDY);
+        }
+        if (gm) {
+            f[GEON_SF_SPIN_CURV][0] -= out->ang_momentum * fc(K_DBGZDX);
+            f[GEON_SF_SPIN_CURV][1] -= out->ang_momentum * fc(K_DBGZDY);
+        }
     }
     return GEON_OK;
 }
@@ -1992,6 +2140,7 @@ int geon_get_field(GeonHandle h, int32_t which, int32_t what, double* out, int32
     FieldState& s = h->fld[which];
     const float* src[] = {s.field, s.field_dot, s.grad_x, s.grad_y, s.sg_phi, s.sg_gx, s.sg_gy, s.source, s.thermal};
     if (what < 0 || what > 8) return set_err(h, GEON_EINVAL, "bad field quantity");
+    if (!src[what]) return set_err(h, GEON_ESTATE, "that field quantity is not held (field off)");
     std::vector<float> buf((size_t)g2);
     HIP_OK(hipMemcpyAsync(buf.data(), src[what], 4 * (size_t)g2, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));

@@ -110,6 +110,15 @@ struct Acc {
     }
 };
 
+// Per-type force of one observer, kept only for a selected particle: the
+// reference's per-particle forceGravity, forceCoulomb, forceMagnetic,
+// forceGravitomag, force1PN and forceYukawa (forces.js:36-46, 210-427).
+enum SplitType { S_GRAV, S_COUL, S_MAG, S_GM, S_PN, S_YUK, kSplitTypes };
+struct Split {
+    float f[kSplitTypes][2];
+    __device__ __forceinline__ void add(int t, float x, float y) { f[t][0] += x; f[t][1] += y; }
+};
+
 __device__ __forceinline__ float torus_wrap(float d, float full, float half) {
     // topology.js:9-12
     return d > half ? d - full : (d < -half ? d + full : d);
@@ -173,9 +182,9 @@ __device__ __forceinline__ void min_image(const PhysFlags& pf, float ox, float o
 // node's mass quadrupole, charge dipole and charge quadrupole about its COM.
 // The matching potential terms (f = r2^-1/2, grad f = -d h) go to the PE:
 //   gravity  m_i (tr(Q) h / 2 + (d.Q.d) h'),  Coulomb  -q_i ((D.d) h + tr(C) h / 2 + (d.C.d) h').
-template <uint32_t F, bool KAHAN>
+template <uint32_t F, bool KAHAN, bool SPLIT = false>
 __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const PhysFlags& pf, float dx, float dy,
-                                              const NodeMom& c) {
+                                              const NodeMom& c, Split* sp = nullptr) {
     const float r2 = dx * dx + dy * dy + pf.soft2;
     const float ir2 = 1.f / r2;
     const float h = __builtin_amdgcn_rsqf(r2) * ir2;
@@ -187,6 +196,7 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
         fx += o.m * (2.f * qx * h1 + tr * dx * h1 + 2.f * dx * dqd * h2);
         fy += o.m * (2.f * qy * h1 + tr * dy * h1 + 2.f * dy * dqd * h2);
         pe += o.m * (0.5f * tr * h + dqd * h1);
+        if constexpr (SPLIT) sp->add(S_GRAV, fx, fy);
     }
     if (F & F_COUL) {
         float qi = o.q;
@@ -197,6 +207,9 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
         fx -= qi * (c.dx * h + 2.f * dx * dd * h1 + 2.f * qx * h1 + tr * dx * h1 + 2.f * dx * dqd * h2);
         fy -= qi * (c.dy * h + 2.f * dy * dd * h1 + 2.f * qy * h1 + tr * dy * h1 + 2.f * dy * dqd * h2);
         pe -= qi * (dd * h + 0.5f * tr * h + dqd * h1);
+        if constexpr (SPLIT)
+            sp->add(S_COUL, -qi * (c.dx * h + 2.f * dx * dd * h1 + 2.f * qx * h1 + tr * dx * h1 + 2.f * dx * dqd * h2),
+                    -qi * (c.dy * h + 2.f * dy * dd * h1 + 2.f * qy * h1 + tr * dy * h1 + 2.f * dy * dqd * h2));
     }
     a.addf(fx, fy);
     a.addpe(pe);
@@ -208,13 +221,14 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
 // displacement source - observer.  ABERR: the source is signal-delayed, and the
 // Lienard-Wiechert factor min((max(1 - n.v_s, 0.01))^-3, 100) scales its
 // position-dependent forces (forces.js:182-189); PE and fields are unscaled.
-// PE = false: no potential energy (dead sources, forces.js:130).
-template <uint32_t F, bool KAHAN, bool ABERR = false, bool PE = true>
+// PE = false: no potential energy (dead sources, forces.js:130).  SPLIT: also
+// sum the terms by type into *sp (the selected observer only).
+template <uint32_t F, bool KAHAN, bool ABERR = false, bool PE = true, bool SPLIT = false>
 __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const PhysFlags& pf,
                                            float rx, float ry, float sm, float sq,
                                            float svx, float svy, float s_ang_vel,
                                            float s_mu, float s_L, float s_ax, float s_yuk,
-                                           float s_higgs) {
+                                           float s_higgs, Split* sp = nullptr) {
     const float raw2 = rx * rx + ry * ry;
     const float r2 = raw2 + pf.soft2;
     const float inv_r = __builtin_amdgcn_rsqf(r2);   // 1/sqrt(r^2 + eps^2)
@@ -244,6 +258,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         const float k = o.m * sm;
         const float fd = k * inv_r3a;
         fr += fd;
+        if constexpr (SPLIT) sp->add(S_GRAV, rx * fd, ry * fd);
         if (F & F_JERK) a.addj(fd, -3.f * k * rdv * inv_r5j, vrx, vry, rx, ry);
         if (PE) pe -= k * inv_r;
     }
@@ -253,6 +268,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         const float k = -(o.q * sq) * ax_pair;
         const float fd = k * inv_r3a;
         fr += fd;
+        if constexpr (SPLIT) sp->add(S_COUL, rx * fd, ry * fd);
         if (F & F_JERK) a.addj(fd, -3.f * k * rdv * inv_r5j, vrx, vry, rx, ry);
         if (PE) pe += o.q * sq * inv_r * ax_pair;
     }
@@ -275,6 +291,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float fx = base * (rx * radial + (o.vx * c1 + svx * c2) * r);
                 const float fy = base * (ry * radial + (o.vy * c1 + svy * c2) * r);
                 fnx += fx; fny += fy; a.f1x += fx; a.f1y += fy;
+                if constexpr (SPLIT) sp->add(S_PN, fx, fy);
                 if (F & F_JERK) {               // position-only EIH term m1 m2 (5 m1 + 4 m2) r / r^4
                     const float ke = o.m * sm * (5.f * o.m + 4.f * sm);
                     const float r4 = inv_r2 * inv_r2;
@@ -288,6 +305,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float fx = coeff * (o.vx * v2n - 3.f * nx * v1n * v2n);
                 const float fy = coeff * (o.vy * v2n - 3.f * ny * v1n * v2n);
                 fnx += fx; fny += fy; a.f1x += fx; a.f1y += fy;
+                if constexpr (SPLIT) sp->add(S_PN, fx, fy);
                 if (PE) pe -= 0.5f * o.q * sq * inv_r * (v1v2 + v1n * v2n);
             }
         }
@@ -295,12 +313,14 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             const float cc = o.q * sq * (o.m + sm) - (o.q * o.q * sm + sq * sq * o.m);
             const float fd = cc * inv_r2 * inv_r2;
             fr += fd; a.f1x += rx * fd; a.f1y += ry * fd;
+            if constexpr (SPLIT) sp->add(S_PN, rx * fd, ry * fd);
             if (F & F_JERK) a.addj(fd, -4.f * cc * rdv * inv_r2 * inv_r2 * inv_r2, vrx, vry, rx, ry);
             if (PE) pe += 0.5f * cc * inv_r2;
         }
         if (pf.mag) {                                               // forces.js:322-353
             const float fd = -3.f * (o.mu * s_mu) * inv_r5a * ax_pair;
             fr += fd;
+            if constexpr (SPLIT) sp->add(S_MAG, rx * fd, ry * fd);
             if (F & F_JERK) a.addj(fd, 15.f * (o.mu * s_mu) * rdv * inv_r5a * inv_r2 * ax_pair, vrx, vry, rx, ry);
             if (PE) pe += (o.mu * s_mu) * inv_r3 * ax_pair;
             const float bzm = sq * cross_sv * inv_r3 * ax_pair;
@@ -314,6 +334,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         if (pf.gm) {                                                // forces.js:355-387
             const float fd = 3.f * (o.L * s_L) * inv_r5a;
             fr += fd;
+            if constexpr (SPLIT) sp->add(S_GM, rx * fd, ry * fd);
             if (F & F_JERK) a.addj(fd, -15.f * (o.L * s_L) * rdv * inv_r5a * inv_r2, vrx, vry, rx, ry);
             if (PE) pe -= (o.L * s_L) * inv_r3;
             const float bgm = -sm * cross_sv * inv_r3;
@@ -333,6 +354,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             const float gmm = pf.yuk_g * ym * o.m * sm * e;
             const float fd = gmm * (inv_r2 + mu * inv_r) * (ABERR ? inv_r * aberr : inv_r);
             fr += fd;
+            if constexpr (SPLIT) sp->add(S_YUK, rx * fd, ry * fd);
             if (F & F_JERK)
                 a.addj(fd, -(3.f * inv_r2 + 3.f * mu * inv_r + mu * mu) * rdv * gmm * inv_r2 *
                                (ABERR ? inv_r * aberr : inv_r), vrx, vry, rx, ry);
@@ -347,6 +369,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 const float fx = beta * (radial * nx + alpha * (nv2 * o.vx + nv1 * svx));
                 const float fy = beta * (radial * ny + alpha * (nv2 * o.vy + nv1 * svy));
                 fnx += fx; fny += fy; a.f1x += fx; a.f1y += fy;
+                if constexpr (SPLIT) sp->add(S_PN, fx, fy);
                 if (PE) pe += 0.5f * gmm * inv_r * (v1v2 + nv1 * nv2 * alpha);
             }
         }

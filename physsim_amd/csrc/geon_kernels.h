@@ -122,7 +122,26 @@ struct ForceArgs {
     // null means src4/src8a/src8b; src_orig[j] is the particle source j images
     const float4 *obs4 = nullptr, *obs8a = nullptr, *obs8b = nullptr;
     const int32_t* src_orig = nullptr;
+    // the selected particle (readStats' `selected`): its index, or -1, and its
+    // per-type force record (kSelFloats floats, layout SelField)
+    const int32_t* sel = nullptr;
+    float* sel_out = nullptr;
 };
+
+// Per-type force record of the selected particle: the pair terms by type, the
+// external fields, the learned far-field correction, the Higgs and axion
+// gradient forces, the fields' gravity and the last radiation-reaction force.
+enum SelField {
+    SEL_SPLIT = 0, SEL_EXT = 2 * kSplitTypes, SEL_FAR = SEL_EXT + 2, SEL_HIGGS = SEL_FAR + 2,
+    SEL_AXION = SEL_HIGGS + 2, SEL_FIELD_GRAV = SEL_AXION + 2, SEL_RAD = SEL_FIELD_GRAV + 2,
+    SEL_VALID = SEL_RAD + 2,    // 1 once a force pass has filled the record
+    kSelFloats = SEL_VALID + 1
+};
+void launch_sel_find(hipStream_t s, const int32_t* slot, int32_t target, int32_t* idx, int n_upper, const Ctrl* ctrl);
+// After the pair and dead-source forces of a pass: the selected particle's
+// pair forces again, by type (bh: the pass walked the tree; residual: the
+// remainder of its pair force is the learned far-field correction).
+void launch_sel_forces(hipStream_t s, const ForceArgs& a, bool bh, bool residual);
 
 // ---- periodic images in the tree (k_ghost.hip) ----
 enum GhostCol { G_X, G_Y, G_M, G_Q, G_WX, G_WY, G_VX, G_VY, G_AV, G_RAD, G_MU, G_L, G_QQ, G_ORIG, kGhostCols };
@@ -365,7 +384,8 @@ void field_modulate(hipStream_t s, const FieldArgs& f, const ParticleView& p, fl
 void field_axmod(hipStream_t s, const FieldArgs& f, const ParticleView& p, float* ax, float* yuk, int coul, int yukawa,
                  int n_upper);
 void field_forces(hipStream_t s, const FieldArgs& h, const FieldArgs& a, int use_h, int use_a, int grav, int coul,
-                  int yukawa, const ParticleView& p, float* fx, float* fy, float* pe_grav, float* pfi, int n_upper);
+                  int yukawa, const ParticleView& p, float* fx, float* fy, float* pe_grav, float* pfi, int n_upper,
+                  const int32_t* sel = nullptr, float* sel_out = nullptr);
 void field_energy(hipStream_t s, const FieldArgs& f, const float* partner, double* out);
 
 // ---- integrator + reductions (k_step.hip) ----
@@ -385,6 +405,9 @@ struct StepArgs {
     const float *jx = nullptr, *jy = nullptr;
     double* rad_e = nullptr;
     int64_t rad_stride = 0;
+    // the selected particle's index and force record (its radiation reaction)
+    const int32_t* sel = nullptr;
+    float* sel_out = nullptr;
 };
 // FP64 column sums: out[c] (+)= sum of column c over the live particles;
 // gate_active: nothing is added when the controller has no substep left.

@@ -530,10 +530,15 @@ __global__ void k_field_forces(FieldArgs h, FieldArgs a, int use_h, int use_a, i
                                const float* __restrict__ m, const float* __restrict__ base_mass,
                                const float* __restrict__ q, float* __restrict__ fx, float* __restrict__ fy,
                                float* __restrict__ pe_grav, float* __restrict__ pfi,
-                               const Ctrl* __restrict__ ctrl) {
+                               const Ctrl* __restrict__ ctrl, const int32_t* __restrict__ sel,
+                               float* __restrict__ sel_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ctrl->n_alive) return;
     float Fx = 0.f, Fy = 0.f, peg = 0.f, e_pfi = 0.f;
+    // by type for the selected particle: Higgs gradient, axion gradient, field
+    // gravity (higgs-field.js:324, axion-field.js:350, scalar-field.js:717)
+    float* so = (sel_out && i == *sel) ? sel_out : nullptr;
+    if (so) for (int k = SEL_HIGGS; k < SEL_FIELD_GRAV + 2; k++) so[k] = 0.f;
     const float mi = m[i], bm = base_mass[i], qi = q[i];
     if (use_h) {
         int ix, iy;
@@ -547,12 +552,16 @@ __global__ void k_field_forces(FieldArgs h, FieldArgs a, int use_h, int use_a, i
                 const float sgn = val >= 0.f ? 1.f : -1.f;
                 Fx += kHiggsCoupling * bm * sgn * gx;
                 Fy += kHiggsCoupling * bm * sgn * gy;
+                if (so) { so[SEL_HIGGS] = kHiggsCoupling * bm * sgn * gx; so[SEL_HIGGS + 1] = kHiggsCoupling * bm * sgn * gy; }
             }
             e_pfi -= bm * (fabsf(val) - 1.f);
         }
         if (grav && *h.has_energy && mi >= 1e-12f) {
-            Fx += -mi * pqs_gather(h.sg_gx, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.icw;
-            Fy += -mi * pqs_gather(h.sg_gy, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.ich;
+            const float gfx = -mi * pqs_gather(h.sg_gx, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.icw;
+            const float gfy = -mi * pqs_gather(h.sg_gy, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f) * h.ich;
+            Fx += gfx;
+            Fy += gfy;
+            if (so) { so[SEL_FIELD_GRAV] += gfx; so[SEL_FIELD_GRAV + 1] += gfy; }
             peg += mi * pqs_gather(h.sg_phi, h.G, h.bc, h.topo, ix, iy, pw, qw, 0.f);
         }
     }
@@ -569,14 +578,18 @@ __global__ void k_field_forces(FieldArgs h, FieldArgs a, int use_h, int use_a, i
             if (gx == gx && gy == gy) {
                 Fx += kAxionCoupling * coupling * gx;
                 Fy += kAxionCoupling * coupling * gy;
+                if (so) { so[SEL_AXION] = kAxionCoupling * coupling * gx; so[SEL_AXION + 1] = kAxionCoupling * coupling * gy; }
             }
         }
         const float av = pqs_gather(a.field, a.G, a.bc, a.topo, ix, iy, pw, qw, a.vac);
         if (coul) e_pfi -= kAxionCoupling * qi * qi * av;
         if (yukawa && mi > 1e-12f) e_pfi -= kAxionCoupling * mi * av;
         if (grav && *a.has_energy && mi >= 1e-12f) {
-            Fx += -mi * pqs_gather(a.sg_gx, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.icw;
-            Fy += -mi * pqs_gather(a.sg_gy, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.ich;
+            const float gfx = -mi * pqs_gather(a.sg_gx, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.icw;
+            const float gfy = -mi * pqs_gather(a.sg_gy, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f) * a.ich;
+            Fx += gfx;
+            Fy += gfy;
+            if (so) { so[SEL_FIELD_GRAV] += gfx; so[SEL_FIELD_GRAV + 1] += gfy; }
             peg += mi * pqs_gather(a.sg_phi, a.G, a.bc, a.topo, ix, iy, pw, qw, 0.f);
         }
     }
@@ -685,10 +698,11 @@ void field_axmod(hipStream_t s, const FieldArgs& f, const ParticleView& p, float
 }
 
 void field_forces(hipStream_t s, const FieldArgs& h, const FieldArgs& a, int use_h, int use_a, int grav, int coul,
-                  int yukawa, const ParticleView& p, float* fx, float* fy, float* pe_grav, float* pfi, int n_upper) {
+                  int yukawa, const ParticleView& p, float* fx, float* fy, float* pe_grav, float* pfi, int n_upper,
+                  const int32_t* sel, float* sel_out) {
     if (n_upper <= 0) return;
     hipLaunchKernelGGL(k_field_forces, dim3(grid_blocks(n_upper)), dim3(256), 0, s, h, a, use_h, use_a, grav, coul,
-                       yukawa, p.x, p.y, p.m, p.base_mass, p.q, fx, fy, pe_grav, pfi, p.ctrl);
+                       yukawa, p.x, p.y, p.m, p.base_mass, p.q, fx, fy, pe_grav, pfi, p.ctrl, sel, sel_out);
 }
 
 void field_energy(hipStream_t s, const FieldArgs& f, const float* partner, double* out) {

@@ -99,9 +99,11 @@ __device__ __forceinline__ void wave_add_counters(Ctrl* ctrl, int64_t pairs, int
 
 // FAR: 0 monopole nodes (the reference), kFarMultipole, kFarList (compile-time,
 // so the reference walk carries no far-field registers).  DELAY: leaf sources
-// act from their past light cone (forces.js:651-672).
-template <uint32_t F, bool KAHAN, int FAR, int STACK = kWalkStack, bool DELAY = false>
-__device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
+// act from their past light cone (forces.js:651-672).  SEL: only the lane of
+// the selected particle (*a.sel) walks, summing its forces per type into *split
+// and storing nothing else.
+template <uint32_t F, bool KAHAN, int FAR, int STACK = kWalkStack, bool DELAY = false, bool SEL = false>
+__device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Split* split = nullptr) {
     __shared__ int32_t s_node[kWavesPerBlock][STACK];
     __shared__ uint64_t s_mask[kWavesPerBlock][STACK];
     const int n = a.ctrl->n_alive;
@@ -109,6 +111,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
     const int wv = threadIdx.x / kWave;
     const int i = (int)(group * kWave) + lane;
     bool valid = i < n;
+    if constexpr (SEL) valid = valid && i == *a.sel;
     int my_sys = -1;
     if (a.bsys != nullptr && valid) {
         // batch: live particles of systems still inside their update
@@ -170,17 +173,17 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
                         const float rad = a.radius[j];
                         const float r2 = rad * rad;
                         const float av = retarded_ang_vel(rt.aw, r2);
-                        pair_force<F, KAHAN, true>(o, acc, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, av,
-                                                   0.2f * s4.w * av * r2, 0.4f * s4.z * av * r2, s8b.y, s8b.z,
-                                                   s8b.w);
+                        pair_force<F, KAHAN, true, true, SEL>(o, acc, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, av,
+                                                              0.2f * s4.w * av * r2, 0.4f * s4.z * av * r2, s8b.y,
+                                                              s8b.z, s8b.w, split);
                         c_pair++;
                     }
                 } else if (mine && (a.src_orig ? a.src_orig[j] : j) != i) {
                     // with periodic images in the tree: not the observer nor one of its images
                     float rx, ry;
                     min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
-                    pair_force<F, KAHAN>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w, s8b.x,
-                                         s8b.y, s8b.z, s8b.w);
+                    pair_force<F, KAHAN, false, true, SEL>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z,
+                                                           s8a.w, s8b.x, s8b.y, s8b.z, s8b.w, split);
                     c_pair++;
                 }
             }
@@ -197,10 +200,10 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
             }
             if (accept) {
                 // aggregate: sAngVel = 0, mods = 1 (forces.js:681-685)
-                pair_force<F, KAHAN>(o, acc, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 0.f, nd.mu, nd.L,
-                                     1.f, 1.f, 1.f);
+                pair_force<F, KAHAN, false, true, SEL>(o, acc, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 0.f,
+                                                       nd.mu, nd.L, 1.f, 1.f, 1.f, split);
                 c_node++;
-                if (FAR == kFarMultipole) far_multipole<F, KAHAN>(o, acc, pf, dx, dy, a.node_mom[node]);
+                if (FAR == kFarMultipole) far_multipole<F, KAHAN, SEL>(o, acc, pf, dx, dy, a.node_mom[node], split);
             }
             if (FAR == kFarList) {
                 // the accepting lanes append (node, lane) in lane order: one coalesced run
@@ -230,6 +233,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group) {
             }
         }
     }
+    if constexpr (SEL) return;                     // the force pass has its counters and outputs
     if (overflow && lane == 0) a.ctrl->overflow = 2;
     if (FAR == kFarList && lane == 0) {
         a.far_count[group] = (int32_t)(far_n < a.far_cap ? far_n : a.far_cap);
@@ -648,6 +652,104 @@ __global__ __launch_bounds__(kWalkThreads) void k_dead_forces(ForceArgs a) {
     if (F & F_JERK) { a.jx[i] += acc.jx; a.jy[i] += acc.jy; }
 }
 
+// ---- The selected particle ------------------------------------------------
+// readStats' `selected` (gpu-physics.js:4238-4276) shows one particle's force by
+// type, which the reference keeps for every particle (forces.js:36-46).  Here
+// only the selected particle's are formed: after the pair forces of a pass, one
+// wave repeats that particle's walk (or direct sum) and its dead sources with
+// every term runtime-gated and summed by type.
+
+__global__ void k_sel_clear(int32_t* idx) { *idx = -1; }
+
+__global__ void k_sel_find(const int32_t* __restrict__ slot, int32_t target, int32_t* __restrict__ idx,
+                           const Ctrl* __restrict__ ctrl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ctrl->n_alive && slot[i] == target) *idx = i;
+}
+
+void launch_sel_find(hipStream_t s, const int32_t* slot, int32_t target, int32_t* idx, int n_upper, const Ctrl* ctrl) {
+    hipLaunchKernelGGL(k_sel_clear, dim3(1), dim3(1), 0, s, idx);
+    if (n_upper > 0)
+        hipLaunchKernelGGL(k_sel_find, dim3((n_upper + 255) / 256), dim3(256), 0, s, slot, target, idx, ctrl);
+}
+
+constexpr uint32_t kSelF = F_GRAV | F_COUL | F_EXTRA | F_JERK;   // every term, each gated by the flags
+
+// One source j of the direct sum on the selected observer (direct_body's body).
+template <bool DELAY>
+__device__ __forceinline__ void sel_direct_source(const ForceArgs& a, const Obs& o, Acc<false>& acc, int j,
+                                                  double now, Split* sp) {
+    const PhysFlags& pf = a.pf;
+    const float4 s4 = a.src4[j];
+    const float4 s8a = a.src8a[j];
+    const float4 s8b = a.src8b[j];
+    if (DELAY) {
+        Retarded rt;
+        if (!delayed_state(a.hist, pf, a.hist.slot[j], o.x, o.y, now, rt)) return;
+        const float rad = a.radius[j];
+        const float r2 = rad * rad;
+        const float av = retarded_ang_vel(rt.aw, r2);
+        pair_force<kSelF, false, true, true, true>(o, acc, pf, rt.dx, rt.dy, s4.z, s4.w, rt.vx, rt.vy, av,
+                                                   0.2f * s4.w * av * r2, 0.4f * s4.z * av * r2, s8b.y, s8b.z, s8b.w,
+                                                   sp);
+    } else {
+        float rx, ry;
+        min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
+        pair_force<kSelF, false, false, true, true>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z, s8a.w,
+                                                    s8b.x, s8b.y, s8b.z, s8b.w, sp);
+    }
+}
+
+template <int FAR, bool DELAY, bool DIRECT>
+__global__ __launch_bounds__(kWave) void k_sel_forces(ForceArgs a, int residual) {
+    const int n = a.ctrl->n_alive;
+    const int si = *a.sel;
+    if (si < 0 || si >= n) return;
+    const int lane = threadIdx.x;
+    Split sp;
+    for (int t = 0; t < kSplitTypes; t++) sp.f[t][0] = sp.f[t][1] = 0.f;
+    Obs o;
+    load_obs(a, si, o);
+    Acc<false> acc;
+    acc.init();
+    const double now = a.ctrl->sim_time;
+    if (DIRECT) {
+        for (int j = lane; j < n; j += kWave)
+            if (j != si) sel_direct_source<DELAY>(a, o, acc, j, now, &sp);
+    } else {
+        walk_body<kSelF, false, FAR, kWalkStack, DELAY, true>(a, si / kWave, &sp);
+    }
+    // retired sources (k_dead_forces), shared over the lanes
+    for (int d = lane; d < a.hist.n_dead; d += kWave) {
+        const DeadRec r = a.hist.dead[d];
+        Retarded rt;
+        if (!delayed_state(a.hist, a.pf, r.slot, o.x, o.y, now, rt)) continue;
+        const float av = retarded_ang_vel(rt.aw, r.r2);
+        pair_force<kSelF, false, true, false, true>(o, acc, a.pf, rt.dx, rt.dy, r.mass, r.charge, rt.vx, rt.vy, av,
+                                              0.2f * r.charge * av * r.r2, 0.4f * r.mass * av * r.r2,
+                                              r.ax != 0.f ? r.ax : 1.f, r.yuk != 0.f ? r.yuk : 1.f,
+                                              r.higgs != 0.f ? r.higgs : 1.f, &sp);
+    }
+    float sum[2] = {0.f, 0.f};
+    for (int t = 0; t < kSplitTypes; t++)
+        for (int c = 0; c < 2; c++) {
+            float v = sp.f[t][c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            sp.f[t][c] = v;
+            sum[c] += v;
+        }
+    if (lane != 0) return;
+    float* out = a.sel_out;
+    for (int t = 0; t < kSplitTypes; t++) { out[SEL_SPLIT + 2 * t] = sp.f[t][0]; out[SEL_SPLIT + 2 * t + 1] = sp.f[t][1]; }
+    const float ex = o.m * a.ext_gx + o.q * a.ext_ex, ey = o.m * a.ext_gy + o.q * a.ext_ey;
+    out[SEL_EXT] = ex;
+    out[SEL_EXT + 1] = ey;
+    // the learned correction: what the pass added beyond the terms above
+    out[SEL_FAR] = residual ? a.fx[si] - ex - sum[0] : 0.f;
+    out[SEL_FAR + 1] = residual ? a.fy[si] - ey - sum[1] : 0.f;
+    out[SEL_VALID] = 1.f;
+}
+
 #define GEON_FEAT_SWITCH(KERNEL, feat, kahan, grid, stream, args)                                     \
     do {                                                                                              \
         const dim3 g_(grid), b_(kWalkThreads);                                                        \
@@ -731,6 +833,22 @@ void launch_dead_forces(hipStream_t s, const ForceArgs& a, uint32_t feat) {
     if (a.n_upper <= 0 || a.hist.n_dead <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
     GEON_FEAT_SWITCH_R(k_dead_forces, feat, false, grid, s, a);
+}
+
+void launch_sel_forces(hipStream_t s, const ForceArgs& a, bool bh, bool residual) {
+    if (a.n_upper <= 0 || !a.sel || !a.sel_out) return;
+    const dim3 g(1), b(kWave);
+    const int r = residual ? 1 : 0;
+    if (!bh) {
+        if (a.hist.on) hipLaunchKernelGGL((k_sel_forces<0, true, true>), g, b, 0, s, a, r);
+        else hipLaunchKernelGGL((k_sel_forces<0, false, true>), g, b, 0, s, a, r);
+    } else if (a.hist.on) {
+        hipLaunchKernelGGL((k_sel_forces<0, true, false>), g, b, 0, s, a, r);
+    } else if (a.far_mode == kFarMultipole) {
+        hipLaunchKernelGGL((k_sel_forces<kFarMultipole, false, false>), g, b, 0, s, a, r);
+    } else {
+        hipLaunchKernelGGL((k_sel_forces<0, false, false>), g, b, 0, s, a, r);
+    }
 }
 
 }  // namespace geon

@@ -178,12 +178,14 @@ __device__ __forceinline__ double kick_drift_one(const StepArgs& a, int i, float
         }
         if (a.rad) {            // Landau-Lifshitz radiation reaction (integrator.js:983-1086), FP64
             double de = 0.0, dpx = 0.0, dpy = 0.0;
+            double frx = 0.0, fry = 0.0;        // the force shown for the selected particle (1031-1032)
             const double wm2 = (double)wx * wx + (double)wy * wy;
             if (fabsf(q) >= 1e-12f && wm2 >= 1e-12) {
                 const double md = m, inv_md = 1.0 / md;
                 const double gamma = a.rel ? sqrt(1.0 + wm2) : 1.0;
                 const double tau = 2.0 / 3.0 * ((double)q * q) * inv_md;
-                double frx = tau * a.jx[i], fry = tau * a.jy[i];
+                frx = tau * a.jx[i];
+                fry = tau * a.jy[i];
                 if (a.rel && gamma > 1.0) {
                     const double ig3 = 1.0 / (gamma * gamma * gamma);
                     frx *= ig3; fry *= ig3;
@@ -215,6 +217,7 @@ __device__ __forceinline__ double kick_drift_one(const StepArgs& a, int i, float
             a.rad_e[i] = de;
             a.rad_e[a.rad_stride + i] = dpx;
             a.rad_e[2 * a.rad_stride + i] = dpy;
+            if (a.sel_out && i == *a.sel) { a.sel_out[SEL_RAD] = (float)frx; a.sel_out[SEL_RAD + 1] = (float)fry; }
         }
         // drift
         const float inv_g = a.rel ? 1.f / sqrtf(1.f + wx * wx + wy * wy) : 1.f;

@@ -73,6 +73,7 @@ class HipPhysics:
         self.h = h
         self.max_particles = int(max_particles)
         self._stats = None
+        self._selected = None
 
     # ---- lifecycle ----------------------------------------------------
     def destroy(self):                                      # gpu-physics.js:4291
@@ -121,8 +122,8 @@ class HipPhysics:
             return -1
         return self._check(rc)
 
-    def removeParticle(self, idx):                          # gpu-physics.js:1936
-        self._check(self.lib.geon_remove_particle(self.h, int(idx)))
+    def removeParticle(self, idx, deathMass=0.0, deathAngVel=0.0):   # gpu-physics.js:1936
+        self._check(self.lib.geon_remove_particle(self.h, int(idx), float(deathMass), float(deathAngVel)))
 
     def patchMass(self, idx, mass, base_mass):              # gpu-physics.js:1955
         self._check(self.lib.geon_patch_mass(self.h, int(idx), float(mass), float(base_mass)))
@@ -179,9 +180,31 @@ class HipPhysics:
 
     # ---- stats -----------------------------------------------------------
     def requestStats(self, sel_idx=-1):                     # gpu-physics.js:4146
+        """The stats now; sel_idx selects the particle whose forces by type
+        the force passes record from then on (-1: none; None keeps the
+        current selection)."""
+        if sel_idx is not None:
+            self._check(self.lib.geon_select(self.h, int(sel_idx)))
         st = A.GeonStats()
         self._check(self.lib.geon_get_stats(self.h, C.byref(st)))
         self._stats = st
+        self._selected = self.selected()
+
+    def selected(self):                                     # readStats().selected, gpu-physics.js:4238-4276
+        s = A.GeonSelected()
+        self._check(self.lib.geon_get_selected(self.h, C.byref(s)))
+        if s.slot < 0:
+            return None
+        out = {"slot": s.slot, "forcesValid": bool(s.forces_valid), "antimatter": bool(s.antimatter),
+               "flags": int(s.antimatter)}
+        for k, name in (("x", "posX"), ("y", "posY"), ("wx", "velWX"), ("wy", "velWY"), ("mass", "mass"),
+                        ("charge", "charge"), ("angw", "angW"), ("base_mass", "baseMass"), ("radius", "radius"),
+                        ("vx", "velX"), ("vy", "velY"), ("ang_vel", "angVel"), ("mag_moment", "magMoment"),
+                        ("ang_momentum", "angMomentum")):
+            out[name] = getattr(s, k)
+        for t, name in enumerate(A.SF_NAMES):
+            out[name] = {"x": s.force[t][0], "y": s.force[t][1]}
+        return out
 
     def readStats(self):                                    # gpu-physics.js:4215-4281
         st = self._stats
@@ -196,10 +219,11 @@ class HipPhysics:
             "higgsFieldEnergy": st.higgs_field_energy, "axionFieldEnergy": st.axion_field_energy,
             "pfiEnergy": st.pfi_energy, "simTime": st.sim_time, "deadCount": st.dead_count,
             "totalRadiated": st.radiated_energy, "totalRadiatedPx": st.radiated_px, "totalRadiatedPy": st.radiated_py,
+            "selected": self._selected,
         }
 
     def stats(self):
-        self.requestStats()
+        self.requestStats(None)
         return self.readStats()
 
     def history(self, slot):

@@ -68,7 +68,8 @@ export default class HipPhysics {
         return addon.addParticle(this._h, { x, y, vx, vy, mass, charge, angw, antimatter });
     }
 
-    removeParticle(idx) { addon.removeParticle(this._h, idx); }
+    // deathMass / deathAngVel: 0 keeps the particle's own values (input.js:183)
+    removeParticle(idx, deathMass = 0, deathAngVel = 0) { addon.removeParticle(this._h, idx, deathMass, deathAngVel); }
 
     patchMass(idx, mass, baseMass) { addon.patchMass(this._h, idx, mass, baseMass); }
 
@@ -115,9 +116,38 @@ export default class HipPhysics {
 
     getParticleCount() { return addon.particleCount(this._h); }
 
-    requestStats(selIdx = -1) { this._stats = addon.getStats(this._h); }
+    /** requestStats(selectedGpuIdx) (gpu-physics.js:4146): the stats now, and
+     *  the selected particle, whose forces by type the force passes record
+     *  from its selection on (`selected.forcesValid`). */
+    requestStats(selIdx = -1) {
+        addon.select(this._h, selIdx);
+        this._stats = addon.getStats(this._h);
+        this._stats.selected = addon.getSelected(this._h);
+    }
 
     readStats() { return this._stats; }
+
+    /** getFieldBuffers(which) (gpu-physics.js:3307): the field ('higgs' or
+     *  'axion') as host arrays of fieldGrid^2 values, row-major. */
+    getFieldBuffers(which) {
+        const w = which === 'higgs' ? 0 : 1;
+        if (!(this._params.toggles0 & (w === 0 ? TOGGLE_BITS.higgsEnabled : TOGGLE_BITS.axionEnabled))) return null;
+        const q = (k) => addon.getField(this._h, w, k);
+        return { field: q(0), fieldDot: q(1), gradX: q(2), gradY: q(3), source: q(7) };
+    }
+
+    /** readbackFieldData() (gpu-physics.js:3844): Float32Arrays of the fields
+     *  and their time derivatives at the device grid, null for a field that is off. */
+    async readbackFieldData() {
+        const out = { higgsField: null, higgsFieldDot: null, axionField: null, axionFieldDot: null };
+        for (const name of ['higgs', 'axion']) {
+            const b = this.getFieldBuffers(name);
+            if (!b) continue;
+            out[name + 'Field'] = Float32Array.from(b.field);
+            out[name + 'FieldDot'] = Float32Array.from(b.fieldDot);
+        }
+        return out;
+    }
 
     /** Events of the last update in collision mode merge (collisions.js:62-92):
      *  [{type: 'merge'|'annihilation', x, y, energy, px, py}], merges first.

@@ -166,12 +166,15 @@ static napi_value js_add_particle(napi_env env, napi_callback_info info) {
 }
 
 static napi_value js_remove_particle(napi_env env, napi_callback_info info) {
-    napi_value argv[2];
-    if (!args_n(env, info, 2, argv)) return NULL;
+    napi_value argv[4];
+    if (!args_n(env, info, 4, argv)) return NULL;
     GeonHandle h = get_handle(env, argv[0]);
     int32_t slot;
+    double death_mass, death_ang_vel;
     CHECK(napi_get_value_int32(env, argv[1], &slot));
-    int rc = geon_remove_particle(h, slot);
+    CHECK(napi_get_value_double(env, argv[2], &death_mass));
+    CHECK(napi_get_value_double(env, argv[3], &death_ang_vel));
+    int rc = geon_remove_particle(h, slot, death_mass, death_ang_vel);
     if (rc) return throw_geon(env, h, rc);
     return NULL;
 }
@@ -393,6 +396,75 @@ static napi_value js_get_stats(napi_env env, napi_callback_info info) {
     return o;
 }
 
+/* select(h, slot): requestStats(selectedGpuIdx)'s selection (gpu-physics.js:4146); -1 for none */
+static napi_value js_select(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!args_n(env, info, 2, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    int32_t slot;
+    CHECK(napi_get_value_int32(env, argv[1], &slot));
+    int rc = geon_select(h, slot);
+    if (rc) return throw_geon(env, h, rc);
+    return NULL;
+}
+
+static napi_value vec2(napi_env env, const double* f) {
+    napi_value o;
+    napi_create_object(env, &o);
+    napi_set_named_property(env, o, "x", num(env, f[0]));
+    napi_set_named_property(env, o, "y", num(env, f[1]));
+    return o;
+}
+
+/* getSelected(h) -> readStats().selected (gpu-physics.js:4238-4276), or null */
+static napi_value js_get_selected(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    if (!args_n(env, info, 1, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    GeonSelected s;
+    int rc = geon_get_selected(h, &s);
+    if (rc) return throw_geon(env, h, rc);
+    napi_value o;
+    if (s.slot < 0) { napi_get_null(env, &o); return o; }
+    napi_create_object(env, &o);
+    const char* sn[] = {"posX", "posY", "velWX", "velWY", "mass", "charge", "angW", "baseMass", "radius", "velX",
+                        "velY", "angVel", "magMoment", "angMomentum"};
+    const double sv[] = {s.x, s.y, s.wx, s.wy, s.mass, s.charge, s.angw, s.base_mass, s.radius, s.vx, s.vy,
+                         s.ang_vel, s.mag_moment, s.ang_momentum};
+    for (int i = 0; i < 14; i++) napi_set_named_property(env, o, sn[i], num(env, sv[i]));
+    napi_value b;
+    napi_get_boolean(env, s.antimatter != 0, &b);
+    napi_set_named_property(env, o, "antimatter", b);
+    napi_set_named_property(env, o, "flags", i32(env, s.antimatter ? 1 : 0));
+    napi_set_named_property(env, o, "slot", i32(env, s.slot));
+    napi_get_boolean(env, s.forces_valid != 0, &b);
+    napi_set_named_property(env, o, "forcesValid", b);
+    const char* fn[GEON_SF_COUNT] = {"forceGravity", "forceCoulomb", "forceMagnetic", "forceGravitomag", "force1PN",
+                                     "forceSpinCurv", "forceRadiation", "forceYukawa", "forceExternal",
+                                     "forceHiggs", "forceAxion", "forceFarField"};
+    for (int k = 0; k < GEON_SF_COUNT; k++) napi_set_named_property(env, o, fn[k], vec2(env, s.force[k]));
+    return o;
+}
+
+/* getField(h, which, what) -> Float64Array(fieldGrid^2), row-major (geon_get_field) */
+static napi_value js_get_field(napi_env env, napi_callback_info info) {
+    napi_value argv[3];
+    if (!args_n(env, info, 3, argv)) return NULL;
+    GeonHandle h = get_handle(env, argv[0]);
+    int32_t which, what;
+    CHECK(napi_get_value_int32(env, argv[1], &which));
+    CHECK(napi_get_value_int32(env, argv[2], &what));
+    GeonParams p;
+    int rc = geon_get_params(h, &p);
+    if (rc) return throw_geon(env, h, rc);
+    const size_t n = (size_t)p.field_grid * (size_t)p.field_grid;
+    double* data;
+    napi_value ta = new_f64(env, n, &data);
+    rc = geon_get_field(h, which, what, data, (int32_t)n);
+    if (rc) return throw_geon(env, h, rc);
+    return ta;
+}
+
 static napi_value js_last_error(napi_env env, napi_callback_info info) {
     napi_value argv[1];
     size_t argc = 1;
@@ -502,6 +574,9 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("computeForces", js_compute_forces);
     EXPORT("getForces", js_get_forces);
     EXPORT("getStats", js_get_stats);
+    EXPORT("select", js_select);
+    EXPORT("getSelected", js_get_selected);
+    EXPORT("getField", js_get_field);
     EXPORT("particleCount", js_particle_count);
     EXPORT("lastError", js_last_error);
     EXPORT("farCollect", js_far_collect);

@@ -93,3 +93,48 @@ def test_addon_upload_rejects_bad_columns():
     assert out[1].startswith("TypeError") and "'x'" in out[1]
     assert out[2].startswith("TypeError") and "'charge'" in out[2]
     assert out[3].startswith("RangeError") and "'antimatter'" in out[3]
+
+
+@pytest.mark.gpu
+def test_node_selection_and_field_readback():
+    """requestStats(selIdx)/readStats().selected, removeParticle's death
+    metadata, getFieldBuffers and readbackFieldData through the addon and the
+    HipPhysics module (gpu-physics.js:1936, 3307, 3844, 4146-4276)."""
+    _ensure_addon()
+    module = os.path.join(ROOT, "physsim_amd", "js", "hip-physics.mjs")
+    js = """
+import HipPhysics, { TOGGLE_BITS } from %r;
+async function main() {
+  const t = TOGGLE_BITS.gravityEnabled | TOGGLE_BITS.higgsEnabled;
+  const g = await new HipPhysics({ toggles0: t, fieldGrid: 32, domainW: 64, domainH: 64 }, 8).init();
+  for (const [x, m] of [[20, 1], [26, 2], [40, 1]]) g.addParticle({ x, y: 32, mass: m });
+  g.requestStats(1);
+  for (let k = 0; k < 16; k++) g.update(0.25);
+  g.requestStats(1);
+  const sel = g.readStats().selected;
+  const d = g.downloadSoA();
+  const f = await g.readbackFieldData();
+  const b = g.getFieldBuffers('higgs');
+  g.removeParticle(2, 3.0, 0.0);
+  g.requestStats(-1);
+  const st = g.readStats();
+  console.log(JSON.stringify({ slot: sel.slot, valid: sel.forcesValid, posX: sel.posX, x1: d.x[1],
+    grav: sel.forceGravity, higgs: sel.forceHiggs, keys: Object.keys(sel),
+    hLen: f.higgsField.length, hType: f.higgsField.constructor.name, axion: f.axionField,
+    hMin: Math.min(...f.higgsField), hMax: Math.max(...f.higgsField), bLen: b.field.length,
+    same: b.field.every((v, i) => Math.fround(v) === f.higgsField[i]), dead: st.deadCount, selNull: st.selected === null }));
+}
+main().catch(e => { console.error(e); process.exit(1); });
+""" % module
+    print("case: node selection and field readback", flush=True)
+    r = subprocess.run([NODE, "--input-type=module", "-e", js], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o = json.loads(r.stdout)
+    assert o["slot"] == 1 and o["valid"] and o["posX"] == o["x1"]
+    assert o["grav"]["x"] < 0 and np.isfinite(o["higgs"]["x"])   # pulled toward the nearer neighbour at -x
+    for k in ("posX", "velWX", "mass", "angVel", "magMoment", "forceGravity", "forceSpinCurv", "forceAxion"):
+        assert k in o["keys"]
+    assert o["hLen"] == 32 * 32 and o["hType"] == "Float32Array" and o["axion"] is None
+    assert 0.5 < o["hMin"] <= o["hMax"] < 1.5                    # about the vacuum value 1
+    assert o["bLen"] == 32 * 32 and o["same"]
+    assert o["dead"] == 0 and o["selNull"]                       # no signal delay: nothing stays behind
