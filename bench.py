@@ -157,7 +157,11 @@ def run_batch(n_systems, steps):
     """Batched ensemble (config 4): n_systems config-1 disks, N log-uniform in
     [256, 16384], gravity + Coulomb, theta 0.5, dt 1e-3 per update, stepped
     together (captured substep graphs, persistent walk with autotuned tile and
-    stack).  systems-steps/s = systems x updates / wall time."""
+    stack).  systems-steps/s = systems x updates / wall time.  The reported
+    value replays graphs of 2 substep bodies, as many back to back as the last
+    update needed; graphs of 4 bodies and the whole 32-substep loop as one
+    graph per update (finished systems predicated off inside it) are timed
+    beside it."""
     from physsim_amd import HipBatch, default_params, workloads as W
     states, prm, dt = W.config4(n_systems)
     sizes = np.array([len(s["x"]) for s in states])
@@ -165,16 +169,25 @@ def run_batch(n_systems, steps):
     for k, st in enumerate(states):
         b.upload(k, st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
     b.compute_forces()
-    tune = b.autotune()
-    b.update(dt)                                  # warmup (captures the graphs)
-    t0 = time.perf_counter()
-    subs = np.array([b.update(dt) for _ in range(steps)])
-    wall = time.perf_counter() - t0
-    log("batch: %d systems, %.3e systems-steps/s" % (n_systems, n_systems * steps / wall))
+    b.autotune()
+    timed = {}
+    for bodies in (32, 4, 2):
+        tune = b.tune(graph_bodies=bodies)
+        b.update(dt)                              # warmup (captures this loop shape's graphs)
+        t0 = time.perf_counter()
+        subs = np.array([b.update(dt) for _ in range(steps)])
+        wall = time.perf_counter() - t0
+        timed[bodies] = (wall, subs, tune)
+        log("batch: %d systems, %d bodies per graph: %.3e systems-steps/s"
+            % (n_systems, bodies, n_systems * steps / wall))
+    wall, subs, tune = timed[2]
+    w4, w32 = timed[4][0], timed[32][0]
     return {"metric": "systems-steps/sec", "value": n_systems * steps / wall, "systems": n_systems,
             "particles": int(sizes.sum()), "steps": steps, "ms_per_step": wall / steps * 1e3,
             "particle_steps_per_s": float((subs * sizes[None, :]).sum() / wall),
             "substeps_mean": float(subs.mean()), "substeps_max": int(subs.max()), "tune": tune,
+            "graphs_of_4_bodies": {"value": n_systems * steps / w4, "ms_per_step": w4 / steps * 1e3},
+            "one_graph_of_32_bodies": {"value": n_systems * steps / w32, "ms_per_step": w32 / steps * 1e3},
             "workload": "config-1 disks, N = floor(256 * 64^U), gravity + Coulomb, softening 0.01, theta 0.5, "
                         "dt 1e-3 per update, despawn"}
 
@@ -484,6 +497,24 @@ def main():
         res[name] = r
         methods[name] = (r["theta"], far)
 
+    def end_state_error(gg, nn, sample):
+        """Relative L2 force error of the sampled live particles against the
+        oracle's FP64 direct sum, at the current state."""
+        gg.compute_forces()
+        f = gg.forces(nn)
+        d = gg.download()
+        live = np.nonzero(d["alive"])[0]
+        o = bind.Oracle(bind.default_params(**prm), len(live))
+        o.upload(d["x"][live], d["y"][live], d["wx"][live], d["wy"][live], d["mass"][live], d["charge"][live],
+                 d["angw"][live])
+        pos = np.full(len(d["alive"]), -1, dtype=np.int64)
+        pos[live] = np.arange(len(live))
+        keep = sample[pos[sample] >= 0]
+        ex, ey = o.direct_forces(pos[keep].astype(np.int32))
+        del o
+        dx, dy = f["fx"][keep] - ex, f["fy"][keep] - ey
+        return float(np.sqrt(np.sum(dx ** 2 + dy ** 2) / np.sum(ex ** 2 + ey ** 2)))
+
     # ---- timed updates, each method from the same initial state ----
     c_f = flops_per_interaction(prm["toggles0"])
     for name, (theta, far) in methods.items():
@@ -521,6 +552,9 @@ def main():
         if name == "learned" and ph["mlp"] > 0:
             r["mlp_tflops"] = cnt["node_interactions"] * MLP_FLOPS_PER_PAIR / (ph["mlp"] * 1e-3) / 1e12
         log("%s: %.3e particle-steps/s" % (name, r["particle_steps_per_s"]))
+        # the same error measure at the state the timed updates ended in
+        r["force_rel_l2_end"] = end_state_error(g, n, rs)
+        log("%s: force error %.3e at the end of the timed updates" % (name, r["force_rel_l2_end"]))
 
     L, P = res["learned"], res.get("plain")
     dominant_mlp = L["phases_ms_avg"]["mlp"] >= L["phases_ms_avg"]["walk"]

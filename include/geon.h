@@ -418,7 +418,12 @@ typedef struct GeonBatchTune {
     int32_t walk_threads;       /* persistent walk block size: 64, 128 or 256 */
     int32_t waves_per_cu;       /* persistent walk grid: resident waves per compute unit */
     int32_t stack;              /* walk node-stack entries per wave: 64 or 96 */
-    int32_t use_graph;          /* 1: replay the substep loop from captured graphs of 4 bodies */
+    int32_t use_graph;          /* 1: replay the substep loop from captured graphs */
+    int32_t graph_bodies;       /* substep bodies per captured graph (even, 2..32; 0 = 2).  Systems that
+                                   finished their update sit predicated off inside a graph.  The graphs
+                                   the last update's longest system needed go out back to back; then
+                                   the host checks for a system left between graphs.  graph_bodies >=
+                                   max_substeps runs each update as one graph launch */
     double  walk_ms;            /* geon_batch_autotune: the chosen configuration's walk time */
 } GeonBatchTune;
 int  geon_batch_create(const GeonParams* params, int32_t n_systems, const int32_t* max_particles,

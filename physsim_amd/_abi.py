@@ -105,7 +105,7 @@ class GeonTreeNode(C.Structure):
 
 class GeonBatchTune(C.Structure):
     _fields_ = [("walk_threads", C.c_int32), ("waves_per_cu", C.c_int32), ("stack", C.c_int32),
-                ("use_graph", C.c_int32), ("walk_ms", C.c_double)]
+                ("use_graph", C.c_int32), ("graph_bodies", C.c_int32), ("walk_ms", C.c_double)]
 
 
 H = C.c_void_p

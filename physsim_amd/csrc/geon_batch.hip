@@ -8,10 +8,12 @@
 // under each system's dt, one sort by (system, Morton key), one tree build
 // whose prefix sums restart at every slice, and a persistent walk whose waves
 // take 64-particle groups from a queue (a wave never crosses a slice).  The
-// substep bodies are captured into hipGraphs of four and replayed until no
-// system can take another substep; systems that finished their update are
-// predicated off inside the kernels (sort and tree skip entirely once none is
-// active).  A system's results do not depend on the other systems in the batch.
+// substep bodies are captured into hipGraphs (two bodies each by default) and
+// replayed until no system can take another substep: as many graphs as the
+// last update's longest system needed go out back to back, then the host
+// checks between graphs.  Systems that finished their update are predicated
+// off inside the kernels (sort and tree skip entirely once none is active).
+// A system's results do not depend on the other systems in the batch.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -34,7 +36,7 @@ enum BForce {
 };
 constexpr int kBatchKeyBits = 2 * kBatchKeyLevels + 1 + 9 + 1;   // Morton + outside bit + system id (< 2^10)
 constexpr int kMaxSystems = 1 << 10;
-constexpr int kChunk = 4;           // substep bodies per captured graph (even: the state ping-pong returns)
+constexpr int kChunk = 2;           // default substep bodies per captured graph (even: the state ping-pong returns)
 
 thread_local std::string g_batch_create_error;
 
@@ -57,13 +59,14 @@ struct GeonBatch_ {
     Ctrl* actrl = nullptr;          // counts gated on any system being active (sort, tree)
     int32_t *d_soff = nullptr, *n_in = nullptr, *seg_first = nullptr, *root_slot = nullptr, *queue = nullptr;
     double* stats = nullptr;        // 10 per system
-    // the substep loop as captured chunks of kChunk bodies: the first chunk
+    // the substep loop as captured chunks of `bodies` substeps: the first chunk
     // (controller's first-substep rule) and the one replayed after it
     hipGraphExec_t graph = nullptr, graph_next = nullptr;
     uint32_t* graph_st0 = nullptr;  // the state buffer the graphs start from (ping-pong parity)
     int32_t* pending = nullptr;     // device flag: a system still inside its update
     int32_t* pending_host = nullptr;
     GeonBatchTune tune = {};
+    int last_max_steps = 0;         // the most substeps any system took in the last update
     int cus = 0;
     bool forces_init = false;
     std::string err;
@@ -321,10 +324,17 @@ void drop_graph(GeonBatch_* h) {
     h->graph_next = nullptr;
 }
 
+// Substep bodies per captured graph: even, at most the substep cap rounded up to even.
+int graph_bodies(const GeonBatch_* h) {
+    const int cap = (h->prm.max_substeps + 1) / 2 * 2;
+    const int b = h->tune.graph_bodies > 0 ? h->tune.graph_bodies : kChunk;
+    return std::max(2, std::min(b, cap));
+}
+
 int capture_chunk(GeonBatch_* h, int first_body, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < kChunk; k++) substep(h, first_body + k);
+    for (int k = 0; k < graph_bodies(h); k++) substep(h, first_body + k);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (e != hipSuccess) return set_err(h, GEON_EDEVICE, std::string("substep graph capture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -336,8 +346,9 @@ int capture_chunk(GeonBatch_* h, int first_body, hipGraphExec_t* out) {
 int capture(GeonBatch_* h) {
     drop_graph(h);
     int rc;
+    const int b = graph_bodies(h);
     if ((rc = capture_chunk(h, 0, &h->graph))) return rc;
-    if ((rc = capture_chunk(h, kChunk, &h->graph_next))) { drop_graph(h); return rc; }
+    if (b < h->prm.max_substeps && (rc = capture_chunk(h, b, &h->graph_next))) { drop_graph(h); return rc; }
     h->graph_st0 = h->st[0];
     return GEON_OK;
 }
@@ -447,6 +458,7 @@ int geon_batch_create(const GeonParams* params, int32_t n_systems, const int32_t
     h->tune.waves_per_cu = 16;
     h->tune.stack = 96;
     h->tune.use_graph = 1;
+    h->tune.graph_bodies = kChunk;
     h->tune.walk_ms = 0;
     *out = h;
     return GEON_OK;
@@ -567,12 +579,15 @@ int geon_batch_update(GeonBatchHandle h, double dt, int32_t* substeps) {
     }
     launch_batch_begin(h->stream, h->bctrl, h->S, dt, h->prm.max_substeps);
     launch_batch_dt_scan(h->stream, step_args(h), bstep(h));
-    // The substep loop in chunks of kChunk bodies (systems that finished are
-    // predicated off inside the kernels); after each chunk the host stops once
-    // no system can take another substep.
-    const int chunks = (h->prm.max_substeps + kChunk - 1) / kChunk;
+    // The substep loop in chunks of graph_bodies bodies (systems that finished
+    // are predicated off inside the kernels).  The chunks that the last
+    // update's longest system needed go out back to back; after that the host
+    // checks, after each chunk, whether any system can take another substep.
+    const int B = graph_bodies(h);
+    const int chunks = (h->prm.max_substeps + B - 1) / B;
+    const int ahead = std::max(1, (h->last_max_steps + B - 1) / B);
     for (int c = 0; c < chunks; c++) {
-        if (c > 0) {
+        if (c >= ahead) {
             bool any = false;
             if ((rc = pending(h, &any))) return rc;
             if (!any) break;
@@ -585,12 +600,14 @@ int geon_batch_update(GeonBatchHandle h, double dt, int32_t* substeps) {
             }
             HIP_OK(hipGraphLaunch(c == 0 ? h->graph : h->graph_next, h->stream));
         } else {
-            for (int k = 0; k < kChunk; k++) substep(h, c * kChunk + k);
+            for (int k = 0; k < B; k++) substep(h, c * B + k);
         }
     }
     launch_batch_boundary(h->stream, step_args(h), bstep(h), h->prm.boundary_mode, h->prm.domain_w, h->prm.domain_h);
     HIP_OK(hipGetLastError());
     if ((rc = read_ctrls(h))) return rc;
+    h->last_max_steps = 0;
+    for (int s = 0; s < h->S; s++) h->last_max_steps = std::max(h->last_max_steps, h->bctrl_host[s].total_steps);
     if (substeps)
         for (int s = 0; s < h->S; s++) substeps[s] = h->bctrl_host[s].total_steps;
     return GEON_OK;
@@ -643,8 +660,10 @@ int geon_batch_get_tune(GeonBatchHandle h, GeonBatchTune* out) {
 int geon_batch_set_tune(GeonBatchHandle h, const GeonBatchTune* t) {
     if (!h || !t) return GEON_EINVAL;
     if ((t->walk_threads != 64 && t->walk_threads != 128 && t->walk_threads != 256) || t->waves_per_cu < 1 ||
-        t->waves_per_cu > 64 || (t->stack != 64 && t->stack != 96))
-        return set_err(h, GEON_EINVAL, "tune: walk_threads 64/128/256, waves_per_cu 1..64, stack 64/96");
+        t->waves_per_cu > 64 || (t->stack != 64 && t->stack != 96) || t->graph_bodies < 0 || t->graph_bodies > 32 ||
+        (t->graph_bodies & 1))
+        return set_err(h, GEON_EINVAL,
+                       "tune: walk_threads 64/128/256, waves_per_cu 1..64, stack 64/96, graph_bodies even 0..32");
     h->tune = *t;
     drop_graph(h);
     return GEON_OK;
@@ -688,6 +707,7 @@ int geon_batch_autotune(GeonBatchHandle h, GeonBatchTune* out) {
             }
     if (best.walk_ms >= 1e30) { h->tune = keep; return set_err(h, GEON_EOVERFLOW, "no walk configuration ran"); }
     best.use_graph = keep.use_graph;
+    best.graph_bodies = keep.graph_bodies;
     h->tune = best;
     drop_graph(h);
     HIP_OK(hipStreamSynchronize(h->stream));

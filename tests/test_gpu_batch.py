@@ -84,9 +84,13 @@ def test_batch_update_matches_single_backend():
         assert eb["total_mass"] == pytest.approx(eg["totalMass"], rel=1e-12)
 
 
-def test_batch_graph_equals_eager():
+@pytest.mark.parametrize("bodies", [4, 32, 2])
+def test_batch_graph_equals_eager(bodies):
+    """Captured substep graphs of 2, 4 or all 32 bodies (one graph per update,
+    the finished systems predicated off inside it) step exactly as the eager loop."""
     states, prm, dt = W.config4(5, replicate=9, n_max=2048)
     g1 = _batch(states, prm)
+    g1.tune(graph_bodies=bodies)
     g2 = _batch(states, prm)
     g2.tune(use_graph=0)
     for _ in range(3):
@@ -95,6 +99,8 @@ def test_batch_graph_equals_eager():
         a, b = g1.download(k), g2.download(k)
         for f in ("x", "y", "wx", "wy"):
             assert np.array_equal(a[f], b[f]), (k, f)
+    with pytest.raises(Exception):
+        g1.tune(graph_bodies=3)
 
 
 def test_batch_autotune_keeps_results():
