@@ -107,6 +107,8 @@ typedef struct GeonParams {
     int32_t  far_list_cap;      /* LEARNED: listed (particle, node) pairs per 64 particles; 0 = 65536 */
     int32_t  signal_delay;      /* GEON_DELAY_*: FOLLOW couples it to GEON_RELATIVITY (forces.js:85) */
     int32_t  history_stride;    /* HISTORY_STRIDE = 64 updates per history sample (config.js:122); 0 = 64 */
+    double   bounce_friction;   /* bounceFriction = 0.4 (integrator.js:72): tangential friction of the
+                                   Hertz wall contacts in boundary mode bounce */
 } GeonParams;
 
 /* Signal delay (signal-delay.js:16-221): every leaf source acts from the point

@@ -109,6 +109,7 @@ def default_params(**kw):
     p.ext_gravity_angle = np.pi / 2
     p.field_grid = 128
     p.kahan = 1
+    p.bounce_friction = 0.4
     for k, v in kw.items():
         setattr(p, k, v)
     return p

@@ -82,7 +82,7 @@ typedef struct {
     double axMod, yukMod, higgsMod;
     double fx, fy;
     double Bz, dBzdx, dBzdy, Bgz, dBgzdx, dBgzdy;
-    double frameDrag, tidal;
+    double frameDrag, tidal, contactTorque;
     double f1x, f1y, f1ox, f1oy;
     double jx, jy;                 /* analytic jerk dF/dt (radiation on, forces.js:198-402) */
     double radAccum, quadAccum, emQuadAccum;   /* _radAccum, _quadAccum, _emQuadAccum */
@@ -1088,9 +1088,65 @@ static void resetForces(OrcSys* s) {                                      /* for
         p->f1x = p->f1y = 0;
         p->Bz = p->dBzdx = p->dBzdy = 0;
         p->Bgz = p->dBgzdx = p->dBgzdy = 0;
-        p->frameDrag = p->tidal = 0;
+        p->frameDrag = p->tidal = p->contactTorque = 0;
         p->jx = p->jy = 0;
         p->pe = 0;
+    }
+}
+
+/* _applyBoundaryForces (integrator.js:606-667): Hertz repulsion delta^1.5 of
+ * each wall the body overlaps, with a tangential friction force clamp(10 vt)
+ * times -friction Fn and its contact torque r Ft; domain [0, W] x [0, H]. */
+static void applyBoundaryForces(OrcSys* s) {
+    const double friction = s->prm.bounce_friction;
+    const double left = 0, top = 0, right = s->prm.domain_w, bottom = s->prm.domain_h;
+    for (int i = 0; i < s->n; i++) {
+        P* p = &s->p[i];
+        const double r = p->radius;
+        double delta = r - (p->x - left);
+        if (delta > 0) {
+            const double Fn = delta * sqrt(delta);
+            p->fx += Fn;
+            if (friction > 0) {
+                const double vt = p->vy + p->angVel * r;
+                const double Ft = -friction * Fn * fmax(-1.0, fmin(1.0, vt * 10));
+                p->fy += Ft;
+                p->contactTorque += r * Ft;
+            }
+        }
+        delta = r - (right - p->x);
+        if (delta > 0) {
+            const double Fn = delta * sqrt(delta);
+            p->fx -= Fn;
+            if (friction > 0) {
+                const double vt = -p->vy - p->angVel * r;
+                const double Ft = -friction * Fn * fmax(-1.0, fmin(1.0, vt * 10));
+                p->fy -= Ft;
+                p->contactTorque -= r * Ft;
+            }
+        }
+        delta = r - (p->y - top);
+        if (delta > 0) {
+            const double Fn = delta * sqrt(delta);
+            p->fy += Fn;
+            if (friction > 0) {
+                const double vt = -p->vx + p->angVel * r;
+                const double Ft = -friction * Fn * fmax(-1.0, fmin(1.0, vt * 10));
+                p->fx -= Ft;
+                p->contactTorque += r * Ft;
+            }
+        }
+        delta = r - (bottom - p->y);
+        if (delta > 0) {
+            const double Fn = delta * sqrt(delta);
+            p->fy -= Fn;
+            if (friction > 0) {
+                const double vt = p->vx - p->angVel * r;
+                const double Ft = -friction * Fn * fmax(-1.0, fmin(1.0, vt * 10));
+                p->fx += Ft;
+                p->contactTorque -= r * Ft;
+            }
+        }
     }
 }
 
@@ -1422,6 +1478,7 @@ static void forcePass(OrcSys* s, int reuseTree) {
         if (s->higgs) of_grav_forces(s, &s->higgsF);
         if (s->axion) of_grav_forces(s, &s->axionF);
     }
+    if (s->prm.boundary_mode == GEON_BOUND_BOUNCE) applyBoundaryForces(s);   /* integrator.js:780, 1546 */
 }
 
 int orc_compute_forces(OrcSys* s) {
@@ -1572,6 +1629,7 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
                     double torque = 0;
                     if (hasGM && relOn) torque += p->frameDrag;
                     if (hasGrav) torque += p->tidal;
+                    torque += p->contactTorque;
                     if (torque != 0) {
                         const double I = INERTIA_K * p->mass * p->radiusSq;
                         if (I >= EPS_) {

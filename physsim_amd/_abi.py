@@ -42,7 +42,7 @@ class GeonParams(C.Structure):
         ("hubble", C.c_double), ("ext_gravity", C.c_double), ("ext_gravity_angle", C.c_double),
         ("ext_electric", C.c_double), ("ext_electric_angle", C.c_double), ("ext_bz", C.c_double),
         ("field_grid", C.c_int32), ("kahan", C.c_int32), ("far_field", C.c_int32), ("far_list_cap", C.c_int32),
-        ("signal_delay", C.c_int32), ("history_stride", C.c_int32),
+        ("signal_delay", C.c_int32), ("history_stride", C.c_int32), ("bounce_friction", C.c_double),
     ]
 
 

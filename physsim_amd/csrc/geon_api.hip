@@ -29,7 +29,7 @@ enum StateCol {
 // Force columns: results of the last force pass, kept between updates.
 enum ForceCol {
     K_FX, K_FY, K_PE, K_BZ, K_DBZDX, K_DBZDY, K_BGZ, K_DBGZDX, K_DBGZDY, K_FDRAG, K_TIDAL, K_F1X, K_F1Y,
-    K_PEG, K_PFI, K_JX, K_JY, N_FORCE
+    K_PEG, K_PFI, K_JX, K_JY, K_CONTACT, N_FORCE
 };
 
 // Device state of one scalar field (scalar-field.js:13-64).
@@ -219,7 +219,6 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     if (p->collision_mode == GEON_COL_MERGE && p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) {
         *why = "merge collisions run on open and torus domains"; return 1;
     }
-    if (p->boundary_mode == GEON_BOUND_BOUNCE) { *why = "bounce boundary (Hertz wall forces) is not run"; return 1; }
     if (p->topology < GEON_TORUS || p->topology > GEON_RP2) { *why = "unknown topology"; return 1; }
     if (p->periodic_mode < GEON_PERIODIC_MIN_IMAGE || p->periodic_mode > GEON_PERIODIC_WINDOWED) { *why = "unknown periodic_mode"; return 1; }
     if (p->boundary_mode == GEON_BOUND_LOOP && p->periodic_mode == GEON_PERIODIC_GHOSTS && (t & GEON_BARNES_HUT)) {
@@ -326,6 +325,7 @@ StepArgs step_args(GeonContext* h) {
     a.rad_e = h->rad_scr;
     a.rad_stride = h->cap;
     if (h->sel_slot >= 0) { a.sel = h->sel_idx; a.sel_out = h->sel_rec; }
+    if (h->prm.boundary_mode == GEON_BOUND_BOUNCE) a.contact = h->fc[K_CONTACT];
     return a;
 }
 
@@ -744,6 +744,14 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
         field_forces(h->stream, field_args(h, 0), field_args(h, 1), higgs, axion, !!(tg & GEON_GRAVITY),
                      !!(tg & GEON_COULOMB), !!(tg & GEON_YUKAWA), particle_view(h), h->fc[K_FX], h->fc[K_FY],
                      h->fc[K_PEG], h->fc[K_PFI], n, fa.sel, fa.sel_out);
+    }
+    if (h->prm.boundary_mode == GEON_BOUND_BOUNCE) {    // after every other force (integrator.js:780, 1546)
+        WallArgs w;
+        w.x = F(h, C_X); w.y = F(h, C_Y); w.vx = F(h, C_VX); w.vy = F(h, C_VY); w.ang_vel = F(h, C_ANGVEL);
+        w.radius = F(h, C_RADIUS); w.fx = h->fc[K_FX]; w.fy = h->fc[K_FY]; w.contact = h->fc[K_CONTACT];
+        w.sel = fa.sel; w.sel_out = fa.sel_out; w.ctrl = h->ctrl;
+        w.w = (float)h->prm.domain_w; w.h = (float)h->prm.domain_h; w.friction = (float)h->prm.bounce_friction;
+        launch_wall_forces(h->stream, w, n);
     }
     HIP_OK(hipGetLastError());
     return GEON_OK;
@@ -1466,6 +1474,7 @@ void geon_default_params(GeonParams* p) {
     p->ext_gravity_angle = M_PI * 0.5;
     p->field_grid = 128;
     p->kahan = 1;
+    p->bounce_friction = 0.4;
 }
 
 int geon_create(const GeonParams* params, int32_t max_particles, GeonHandle* out) {
@@ -1828,7 +1837,7 @@ int geon_get_forces(GeonHandle h, GeonForceView* v, int32_t n_slots) {
     HIP_OK(hipStreamSynchronize(h->stream));
     double* outs[N_FORCE] = {v->fx, v->fy, v->pe, v->bz, v->dbzdx, v->dbzdy, v->bgz, v->dbgzdx, v->dbgzdy,
                              v->frame_drag_torque, v->tidal_torque, v->f1pn_x, v->f1pn_y, nullptr, nullptr,
-                             v->jerk_x, v->jerk_y};
+                             v->jerk_x, v->jerk_y, nullptr};
     for (int i = 0; i < n; i++) {
         const int s = slot[i];
         if (s < 0 || s >= n_slots) continue;

@@ -408,7 +408,20 @@ struct StepArgs {
     // the selected particle's index and force record (its radiation reaction)
     const int32_t* sel = nullptr;
     float* sel_out = nullptr;
+    // boundary mode bounce: each particle's wall contact torque of the last force pass
+    const float* contact = nullptr;
 };
+// _applyBoundaryForces (integrator.js:606-667): Hertz wall repulsion with
+// friction on [0, w] x [0, h], added to the force; the contact torque written.
+struct WallArgs {
+    const float *x, *y, *vx, *vy, *ang_vel, *radius;
+    float *fx, *fy, *contact;
+    const int32_t* sel;
+    float* sel_out;
+    const Ctrl* ctrl;
+    float w, h, friction;
+};
+void launch_wall_forces(hipStream_t s, const WallArgs& a, int n_upper);
 // FP64 column sums: out[c] (+)= sum of column c over the live particles;
 // gate_active: nothing is added when the controller has no substep left.
 void launch_sum_cols(hipStream_t s, const double* cols, int ncols, int64_t stride, int n_upper, const Ctrl* ctrl,

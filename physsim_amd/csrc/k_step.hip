@@ -166,6 +166,7 @@ __device__ __forceinline__ double kick_drift_one(const StepArgs& a, int i, float
             float torque = 0.f;
             if (a.gm && a.rel) torque += a.frame_drag[i];
             if (a.grav) torque += a.tidal[i];
+            if (a.contact) torque += a.contact[i];       // integrator.js:970
             if (torque != 0.f) {
                 const float I = 0.4f * m * r * r;
                 if (I >= 1e-12f) {
@@ -336,6 +337,48 @@ __global__ void k_boundary(StepArgs a, int mode, int topo, float w, float h) {
     a.x[i] = x; a.y[i] = y;
     a.alive[i] = alive;
     if (!alive) atomicAdd(&a.ctrl->deaths, 1);
+}
+
+__global__ void k_wall_forces(WallArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ctrl->n_alive) return;
+    const float r = a.radius[i], x = a.x[i], y = a.y[i], vx = a.vx[i], vy = a.vy[i], w = a.ang_vel[i];
+    const float mu = a.friction;
+    float fx = a.fx[i], fy = a.fy[i], tq = 0.f, ex = 0.f, ey = 0.f;
+    auto tangential = [&](float Fn, float vt) { return -mu * Fn * fminf(1.f, fmaxf(-1.f, vt * 10.f)); };
+    float d = r - x;                                  // left wall
+    if (d > 0.f) {
+        const float Fn = d * sqrtf(d);
+        fx += Fn; ex += Fn;
+        if (mu > 0.f) { const float Ft = tangential(Fn, vy + w * r); fy += Ft; tq += r * Ft; }
+    }
+    d = r - (a.w - x);                                // right wall
+    if (d > 0.f) {
+        const float Fn = d * sqrtf(d);
+        fx -= Fn; ex -= Fn;
+        if (mu > 0.f) { const float Ft = tangential(Fn, -vy - w * r); fy -= Ft; tq -= r * Ft; }
+    }
+    d = r - y;                                        // top wall
+    if (d > 0.f) {
+        const float Fn = d * sqrtf(d);
+        fy += Fn; ey += Fn;
+        if (mu > 0.f) { const float Ft = tangential(Fn, -vx + w * r); fx -= Ft; tq += r * Ft; }
+    }
+    d = r - (a.h - y);                                // bottom wall
+    if (d > 0.f) {
+        const float Fn = d * sqrtf(d);
+        fy -= Fn; ey -= Fn;
+        if (mu > 0.f) { const float Ft = tangential(Fn, vx - w * r); fx += Ft; tq -= r * Ft; }
+    }
+    a.fx[i] = fx; a.fy[i] = fy;
+    a.contact[i] = tq;
+    // forceExternal takes the normal forces only (integrator.js:619-658)
+    if (a.sel_out && i == *a.sel) { a.sel_out[SEL_EXT] += ex; a.sel_out[SEL_EXT + 1] += ey; }
+}
+
+void launch_wall_forces(hipStream_t s, const WallArgs& a, int n_upper) {
+    if (n_upper <= 0) return;
+    hipLaunchKernelGGL(k_wall_forces, dim3((n_upper + 255) / 256), dim3(256), 0, s, a);
 }
 
 void launch_boundary(hipStream_t s, const StepArgs& a, int mode, int topo, double w, double h) {

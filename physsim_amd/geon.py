@@ -369,7 +369,8 @@ class HipPhysics:
     def serialize(self, sim=None):
         """GPUPhysics.serialize (gpu-physics.js:3749-3836): the v1 state of the
         live particles in slot order, toggles, parameters and modes.  `sim`
-        may carry the app's speedScale, friction and camera."""
+        may carry the app's speedScale and camera (settings.friction is the
+        bounce friction parameter)."""
         d = self.download()
         particles = [{"x": float(d["x"][i]), "y": float(d["y"][i]), "wx": float(d["wx"][i]),
                       "wy": float(d["wy"][i]), "mass": float(d["mass"][i]), "baseMass": float(d["base_mass"][i]),

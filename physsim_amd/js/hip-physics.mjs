@@ -164,7 +164,7 @@ export default class HipPhysics {
 
     /** serialize(sim): the v1 state (gpu-physics.js:3749-3836): live particles
      *  in slot order, toggles, parameters and modes; `sim` may carry the app's
-     *  speedScale, friction and camera. */
+     *  speedScale and camera (settings.friction is the bounce friction). */
     async serialize(sim = {}) {
         const d = this.downloadSoA();
         const particles = [];

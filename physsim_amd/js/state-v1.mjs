@@ -37,10 +37,11 @@ export const PARAM_KEYS = [
 // Reference defaults (src/config.js) of the fields a state carries.
 const DEFAULTS = { toggles0: 16387, collisionMode: 0, boundaryMode: 0, topology: 0, yukawaMu: 0.15,
                    axionMass: 0.05, hubbleParam: 0.001, higgsMass: 0.5, extGravity: 0, extGravityAngle: Math.PI / 2,
-                   extElectric: 0, extElectricAngle: 0, extBz: 0 };
+                   extElectric: 0, extElectricAngle: 0, extBz: 0, bounceFriction: 0.4 };
 
 /** A v1 state object from addon params and the live particles; `sim` may
- *  carry the app's speedScale, friction and camera. */
+ *  carry the app's speedScale and camera.  settings.friction is the
+ *  physics' bounceFriction (save-load.js:103). */
 export function paramsToState(params, particles, sim = {}) {
     const p = { ...DEFAULTS, ...params };
     const state = {
@@ -52,7 +53,7 @@ export function paramsToState(params, particles, sim = {}) {
             boundary: BOUND_NAMES[p.boundaryMode],
             topology: TOPO_NAMES[p.topology],
             speed: sim.speedScale != null ? sim.speedScale : 16,
-            friction: sim.friction != null ? sim.friction : 0.4,
+            friction: p.bounceFriction != null ? p.bounceFriction : 0.4,
         },
     };
     for (const k of TOGGLE_KEYS) state.toggles[k] = !!(p.toggles0 & TOGGLE_BITS[k]);
@@ -83,6 +84,7 @@ export function applyState(params, state) {
         p.collisionMode = colFromString(col);
         p.boundaryMode = boundFromString(st.boundary);
         p.topology = topoFromString(st.topology);
+        if (st.friction != null) p.bounceFriction = st.friction;          // save-load.js:223
     }
     return p;
 }

@@ -85,6 +85,7 @@ static void read_params(napi_env env, napi_value obj, GeonParams* p) {
     p->far_list_cap = (int32_t)get_num(env, obj, "farListCap", p->far_list_cap);
     p->signal_delay = (int32_t)get_num(env, obj, "signalDelay", p->signal_delay);
     p->history_stride = (int32_t)get_num(env, obj, "historyStride", p->history_stride);
+    p->bounce_friction = get_num(env, obj, "bounceFriction", p->bounce_friction);
 }
 
 static napi_value args_n(napi_env env, napi_callback_info info, size_t want, napi_value* argv) {

@@ -59,7 +59,8 @@ PARAM_KEYS = [
 def params_to_state(params, particles, sim=None):
     """A v1 state object from GeonParams and the live particles (a list of
     dicts with the particle keys).  `sim` may carry the app's UI settings
-    (`speedScale`, `friction`, `camera`), which the format also stores."""
+    (`speedScale`, `camera`), which the format also stores; settings.friction
+    is the physics' bounceFriction (save-load.js:103)."""
     sim = sim or {}
     state = {
         "version": 1,
@@ -70,7 +71,7 @@ def params_to_state(params, particles, sim=None):
             "boundary": BOUND_NAMES[params.boundary_mode],
             "topology": TOPO_NAMES[params.topology],
             "speed": sim.get("speedScale", 16),
-            "friction": sim.get("friction", 0.4),
+            "friction": params.bounce_friction,
         },
     }
     if "camera" in sim:
@@ -106,6 +107,8 @@ def apply_state(params, state):
         p.collision_mode = col_from_string(col)
         p.boundary_mode = bound_from_string(st.get("boundary"))
         p.topology = topo_from_string(st.get("topology"))
+        if st.get("friction") is not None:
+            p.bounce_friction = float(st["friction"])                  # save-load.js:223
     return p
 
 

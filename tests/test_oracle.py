@@ -665,3 +665,58 @@ def test_torques_vanish_at_corotation(oracle_lib):
     assert o.forces()["tidal"][0] == pytest.approx(0.0, abs=1e-15)
     o = two_body(oracle_lib, A.GRAVITY, m=m, q=(0, 0), d=d, soft=soft, w=((0, 0), (0, v)), angw=(wo + 0.1, 0.0))
     assert o.forces()["tidal"][0] < 0
+
+
+def _wall_body(oracle_lib, x, y, wx, wy, angw=0.0, friction=0.4, toggles=A.GRAVITY):
+    prm = dict(toggles0=toggles, boundary_mode=A.BOUND_BOUNCE, domain_w=64.0, domain_h=64.0, softening=1.0,
+               bounce_friction=friction)
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1)
+    o.upload(np.array([x]), np.array([y]), np.array([wx]), np.array([wy]), np.array([8.0]), np.array([0.0]),
+             np.array([angw]))
+    return o
+
+
+def test_bounce_wall_hertz_force_friction_and_torque(oracle_lib):
+    """_applyBoundaryForces (integrator.js:606-667) on a lone body of mass 8
+    (radius 2): the normal force is delta^1.5 (Hertz), the friction force
+    -mu Fn clamp(10 vt) along the wall, its torque r Ft turns the spin by
+    torque dt / (0.4 m r^2) in the kick (integrator.js:964-981)."""
+    # 0.5 into the left wall, sliding at +0.3 along it: clamp(3) = 1
+    o = _wall_body(oracle_lib, 1.5, 32.0, 0.0, 0.3)
+    o.compute_forces()
+    f = o.forces()
+    Fn = 0.5 ** 1.5
+    assert f["fx"][0] == pytest.approx(Fn, rel=1e-14)
+    assert f["fy"][0] == pytest.approx(-0.4 * Fn, rel=1e-14)
+    # slow slide: the clamp is linear, vt = vy + angVel r = 0.05 + 0.01 * 2
+    o = _wall_body(oracle_lib, 1.5, 32.0, 0.0, 0.05, angw=0.01)
+    o.compute_forces()
+    assert o.forces()["fy"][0] == pytest.approx(-0.4 * Fn * 10 * 0.07, rel=1e-12)
+    # the right wall mirrors it (vt = -vy - angVel r), the top and bottom walls likewise
+    o = _wall_body(oracle_lib, 62.5, 32.0, 0.0, 0.3)
+    o.compute_forces()
+    f = o.forces()
+    assert (f["fx"][0], f["fy"][0]) == (pytest.approx(-Fn, rel=1e-14), pytest.approx(-0.4 * Fn, rel=1e-14))
+    o = _wall_body(oracle_lib, 32.0, 1.0, 0.3, 0.0)      # 1 into the top wall, moving +x: vt = -vx
+    o.compute_forces()
+    f = o.forces()
+    assert (f["fx"][0], f["fy"][0]) == (pytest.approx(-0.4, rel=1e-14), pytest.approx(1.0, rel=1e-14))
+    # no friction: normal force only
+    o = _wall_body(oracle_lib, 32.0, 63.0, 0.3, 0.0, friction=0.0)
+    o.compute_forces()
+    f = o.forces()
+    assert (f["fx"][0], f["fy"][0]) == (0.0, pytest.approx(-1.0, rel=1e-14))
+    # the contact torque spins the body in the first kick: torque r Ft, I = 0.4 m r^2
+    o = _wall_body(oracle_lib, 1.5, 32.0, 0.0, 0.3)
+    dt = 1e-3
+    assert o.update(dt) == 1
+    assert o.download()["angw"][0] == pytest.approx(2.0 * (-0.4 * Fn) * dt / (0.4 * 8 * 4), rel=1e-12)
+
+
+def test_bounce_boundary_clamps_once_per_update(oracle_lib):
+    """The safety clamp of step 8 (integrator.js:1812-1818) puts a body that
+    left the box back on its edge; inside, nothing moves it."""
+    o = _wall_body(oracle_lib, -3.0, 70.0, -0.1, 0.1, friction=0.0)
+    o.update(1e-3)
+    d = o.download()
+    assert (d["x"][0], d["y"][0]) == (0.0, 64.0)
