@@ -2,7 +2,7 @@
 computes on seeded inputs, for the parts of the oracle no closed form pins
 (the composite Physics.update sequence, signal delay with its history
 warm-up, HiggsField / AxionField.update with self-gravity, the periodic
-ghost walk).
+ghost walk, the bounce walls).
 
 It runs the reference's JavaScript (integrator.js, higgs-field.js,
 axion-field.js, energy.js and their imports) under Node 12 in the build
@@ -30,7 +30,7 @@ TOGGLE_NAMES = ("gravityEnabled", "coulombEnabled", "magneticEnabled", "gravitom
                 "relativityEnabled", "spinOrbitEnabled", "radiationEnabled", "blackHoleEnabled",
                 "disintegrationEnabled", "expansionEnabled", "yukawaEnabled", "higgsEnabled", "axionEnabled",
                 "barnesHutEnabled", "bosonInterEnabled")
-DESPAWN, LOOP, TORUS, PASS = 0, 2, 0, 0
+DESPAWN, BOUNCE, LOOP, TORUS, PASS = 0, 1, 2, 0, 0
 
 
 def _toggles(*names):
@@ -48,6 +48,20 @@ def _uniform_box(n, seed, side):
     g = np.random.Generator(np.random.Philox(9100 + seed))
     return dict(x=g.uniform(0, side, n), y=g.uniform(0, side, n), wx=g.normal(0, 0.2, n), wy=g.normal(0, 0.2, n),
                 mass=g.lognormal(0, 0.5, n), charge=g.integers(-10, 11, n) * 0.1, angw=np.zeros(n))
+
+
+def _pressed_box(n, seed, side):
+    """A box whose first third of bodies overlap a wall by up to half their radius, moving along it."""
+    g = np.random.Generator(np.random.Philox(9300 + seed))
+    st = _uniform_box(n, seed, side)
+    r = np.cbrt(st["mass"])
+    k = n // 3
+    wall = g.integers(0, 4, k)
+    depth = r[:k] * g.uniform(0.1, 0.5, k)
+    st["x"][:k] = np.where(wall == 0, depth, np.where(wall == 1, side - depth, st["x"][:k]))
+    st["y"][:k] = np.where(wall == 2, depth, np.where(wall == 3, side - depth, st["y"][:k]))
+    st["angw"] = g.uniform(-0.2, 0.2, n)
+    return st
 
 
 def cases():
@@ -86,6 +100,9 @@ def cases():
     # periodic Barnes-Hut with the reference's ghost images (integrator.js:201-258)
     add("torus_ghosts_bh", _uniform_box(256, 7, 64.0), _toggles("gravityEnabled", "coulombEnabled",
                                                                 "barnesHutEnabled"), 20, 1.0 / 128, boundary=LOOP)
+    # bounce walls: Hertz repulsion with friction, contact torque, the clamp (integrator.js:606-667, 1812-1818)
+    add("bounce_walls", _pressed_box(96, 8, 32.0), _toggles("gravityEnabled", "coulombEnabled"), 40, 1.0 / 64,
+        boundary=BOUNCE, side=32.0)
     return out
 
 

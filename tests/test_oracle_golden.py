@@ -8,8 +8,9 @@ tests/golden/make_reference_goldens.py.  They pin the parts of the oracle no
 closed form fixes: the composite substep sequence (kick order, H8 a_max
 reuse, PE halving, the 1PN velocity Verlet), every pairForce term in
 combination (forces.js:163-442), signal-delayed leaves after the history
-warm-up, the field KDK with self-gravity and portal terms, and the periodic
-walk over ghost images.
+warm-up, the field KDK with self-gravity and portal terms, the periodic walk
+over ghost images, and the bounce walls (Hertz force, friction, contact torque,
+clamp).
 
 Each case runs the oracle on the fixture's inputs with the reference's
 constants (softening 8, theta 0.5, capacity 4, 64^2 field grid): the forces of
