@@ -37,8 +37,13 @@ function forcesOf(ps) {
         dbzdx: f(p => p.dBzdx), dbzdy: f(p => p.dBzdy), dbgzdx: f(p => p.dBgzdx), dbgzdy: f(p => p.dBgzdy),
         frame_drag: f(p => p._frameDragTorque), tidal: f(p => p._tidalTorque),
         f1pn_x: f(p => p.force1PN.x), f1pn_y: f(p => p.force1PN.y),
+        // the per-type vectors readStats' selected shows (after the display pass, integrator.js:1751-1796)
+        ...Object.fromEntries(DISPLAY.flatMap(k => [[`${k}_x`, f(p => p[k].x)], [`${k}_y`, f(p => p[k].y)]])),
     };
 }
+
+const DISPLAY = ['forceGravity', 'forceCoulomb', 'forceMagnetic', 'forceGravitomag', 'force1PN', 'forceSpinCurv',
+                 'forceRadiation', 'forceYukawa', 'forceExternal', 'forceHiggs', 'forceAxion'];
 
 function stateOf(ps) {
     const f = k => ps.map(p => k(p));

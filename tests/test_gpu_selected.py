@@ -228,3 +228,49 @@ def test_remove_particle_without_signal_delay_ignores_metadata():
     g.removeParticle(1, deathMass=5.0, deathAngVel=0.2)
     assert g.stats()["deadCount"] == 0
     assert list(g.download()["alive"]) == [True, False, True]
+
+
+# ---- against the reference's own per-type vectors ---------------------------
+import glob as _glob
+import json as _json
+import os as _os
+
+_FIX = sorted(_glob.glob(_os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "golden", "reference_*.npz")))
+_TYPES = ("forceGravity", "forceCoulomb", "forceMagnetic", "forceGravitomag", "force1PN", "forceSpinCurv",
+          "forceRadiation", "forceYukawa", "forceExternal", "forceHiggs", "forceAxion")
+
+
+@pytest.mark.parametrize("path", _FIX, ids=[_os.path.basename(p)[10:-4] for p in _FIX])
+def test_selected_matches_reference_display_forces(path):
+    """The reference's per-particle force vectors after its bootstrap pass and
+    display pass (Physics.update(0), integrator.js:750-782, 1751-1796), run
+    under Node into tests/golden: the GPU's selected particle shows the same
+    vector of every type, within FP32 rounding of the pass."""
+    from physsim_amd import HipPhysics, default_params
+    z = np.load(path)
+    meta = _json.loads(bytes(z["meta"]).decode())
+    t = 0
+    for name, bit in A.TOGGLE_BITS.items():
+        if meta["toggles"].get(name):
+            t |= bit
+    prm = dict(toggles0=t, boundary_mode=meta["boundaryMode"], topology=meta["topology"],
+               collision_mode=meta["collisionMode"], domain_w=meta["W"], domain_h=meta["H"], field_grid=64,
+               periodic_mode=A.PERIODIC_GHOSTS)
+    n = len(z["in_x"])
+    ftot = np.hypot(z["bootstrap_fx"], z["bootstrap_fy"])
+    print("case: %s n=%d toggles=0x%x" % (meta["name"], n, t), flush=True)
+    for k in (0, n // 2, n - 1):
+        g = HipPhysics(default_params(**prm), max_particles=n)        # the bootstrap pass of a fresh system
+        g.upload(z["in_x"], z["in_y"], z["in_wx"], z["in_wy"], z["in_mass"], z["in_charge"], z["in_angw"])
+        for w, f in ((0, "in_field_higgs"), (1, "in_field_axion")):
+            if f in z:
+                g.set_field(w, z[f].reshape(64, 64))
+        g.requestStats(k)
+        g.compute_forces()
+        s = g.selected()
+        for name in _TYPES:
+            got = _v(s, name)
+            want = np.array([z["bootstrap_%s_x" % name][k], z["bootstrap_%s_y" % name][k]])
+            err = np.linalg.norm(got - want)
+            assert err <= 3e-5 * np.linalg.norm(want) + 3e-6 * ftot[k] + 1e-12, (meta["name"], k, name, got, want)
+        g.destroy()
