@@ -219,10 +219,10 @@ def run_legs(names, with_cpu, threads, model=None):
     out = {}
     f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
 
-    def state(seed, n, W, speed, mass=(0.5, 3.0), spin=0.3, anti=0.0):
+    def state(seed, n, side, speed, mass=(0.5, 3.0), spin=0.3, anti=0.0):
         # each leg draws from its own seed: a leg's workload does not depend on which legs ran before it
         g0 = np.random.default_rng(seed)
-        return dict(x=f32(g0.uniform(0, W, n)), y=f32(g0.uniform(0, W, n)), wx=f32(g0.normal(0, speed, n)),
+        return dict(x=f32(g0.uniform(0, side, n)), y=f32(g0.uniform(0, side, n)), wx=f32(g0.normal(0, speed, n)),
                     wy=f32(g0.normal(0, speed, n)), mass=f32(g0.uniform(*mass, n)),
                     charge=g0.integers(-10, 11, n) * 0.1, angw=f32(g0.uniform(-spin, spin, n)),
                     anti=(g0.random(n) < anti).astype(np.int32))
@@ -247,29 +247,29 @@ def run_legs(names, with_cpu, threads, model=None):
 
     if "delay" in names:
         # light-cone sources for every leaf pair (signal-delay.js:16-221): config-2 forces, relativity
-        n, W, dt = 1 << 16, 256.0, 1.0 / 128
-        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.MAGNETIC | A.RELATIVITY | A.BARNES_HUT, domain_w=W,
-                   domain_h=W, boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0,
+        n, side, dt = 1 << 16, 256.0, 1.0 / 128
+        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.MAGNETIC | A.RELATIVITY | A.BARNES_HUT, domain_w=side,
+                   domain_h=side, boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0,
                    signal_delay=A.DELAY_ON, history_stride=1)
-        st = state(4242, n, W, 0.3)
+        st = state(4242, n, side, 0.3)
         r = gpu_rate(prm, st, dt, 8, 4)
         r_off = gpu_rate(dict(prm, signal_delay=A.DELAY_OFF), st, dt, 8, 4)
         r.update({"workload": "N=%d uniform %gx%g open box, gravity+Coulomb+magnetic, relativistic, BH theta 0.5, "
-                              "signal delay on, a history sample per update, 8 updates of history first" % (n, W, W),
+                              "signal delay on, a history sample per update, 8 updates of history first" % (n, side, side),
                   "current_time_sources": r_off, "slowdown_vs_current_time": r_off["particle_steps_per_s"] /
                   r["particle_steps_per_s"]})
         if with_cpu:
-            cs = state(4243, 16384, W * 0.5, 0.3)
+            cs = state(4243, 16384, side * 0.5, 0.3)
             r["cpu_baseline"] = _cpu_update_rate(prm, cs, dt, 8, threads)
         out["signal_delay"] = r
         log("leg delay: %.3e particle-steps/s" % r["particle_steps_per_s"])
     if "merge" in names:
         # collision mode merge (collisions.js:24-158): about one body in ten overlaps another at the start
-        n, W, dt = 1 << 20, 16384.0, 1.0 / 64
-        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, domain_w=W, domain_h=W,
+        n, side, dt = 1 << 20, 16384.0, 1.0 / 64
+        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, domain_w=side, domain_h=side,
                    boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0, collision_mode=A.COL_MERGE,
                    signal_delay=A.DELAY_OFF)
-        st = state(4244, n, W, 0.5, anti=0.2)
+        st = state(4244, n, side, 0.5, anti=0.2)
         g = HipPhysics(default_params(**prm), max_particles=2 * n)
         g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"], antimatter=st["anti"])
         g.compute_forces()
@@ -282,13 +282,13 @@ def run_legs(names, with_cpu, threads, model=None):
         steady = gpu_rate(prm, st, dt, 2, 4, cap=2 * n)
         passing = gpu_rate(dict(prm, collision_mode=A.COL_PASS), st, dt, 2, 4, cap=2 * n)
         r = {"workload": "N=%d uniform %gx%g open box, masses U[0.5, 3] (radii 0.8-1.4), 20%% antimatter, "
-                         "gravity+Coulomb, BH theta 0.5, dt 1/64" % (n, W, W),
+                         "gravity+Coulomb, BH theta 0.5, dt 1/64" % (n, side, side),
              "first_update_ms": first_ms, "first_update_events": events, "particles_after_first": left,
              "steady": steady, "pass_mode": passing,
              "merge_pass_ms_per_update": steady["ms_per_update"] - passing["ms_per_update"],
              "particle_steps_per_s": steady["particle_steps_per_s"]}
         if with_cpu:
-            cs = state(4245, 65536, W * 0.25, 0.5, anti=0.2)
+            cs = state(4245, 65536, side * 0.25, 0.5, anti=0.2)
             r["cpu_baseline"] = _cpu_update_rate(prm, cs, dt, 0, threads)
         out["merge"] = r
         log("leg merge: first update %.1f ms (%d events), steady %.3e particle-steps/s"
@@ -389,18 +389,18 @@ def run_legs(names, with_cpu, threads, model=None):
         out["torus_windowed"] = r
     if "radiation" in names:
         # Landau-Lifshitz reaction every substep, quadrupole radiation every update
-        n, W, dt = 1 << 20, 1024.0, 1.0 / 128
-        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.RELATIVITY | A.RADIATION | A.BARNES_HUT, domain_w=W,
-                   domain_h=W, boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0, signal_delay=A.DELAY_OFF)
-        st = state(4246, n, W, 0.3)
+        n, side, dt = 1 << 20, 1024.0, 1.0 / 128
+        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.RELATIVITY | A.RADIATION | A.BARNES_HUT, domain_w=side,
+                   domain_h=side, boundary_mode=A.BOUND_DESPAWN, theta=0.5, softening=8.0, signal_delay=A.DELAY_OFF)
+        st = state(4246, n, side, 0.3)
         r = gpu_rate(prm, st, dt, 1, 4)
         base = gpu_rate(dict(prm, toggles0=prm["toggles0"] & ~A.RADIATION), st, dt, 1, 4)
         r.update({"workload": "N=%d uniform %gx%g open box, gravity+Coulomb, relativistic, BH theta 0.5, radiation "
-                              "reaction (jerk, Landau-Lifshitz, quadrupole)" % (n, W, W),
+                              "reaction (jerk, Landau-Lifshitz, quadrupole)" % (n, side, side),
                   "without_radiation": base,
                   "overhead": base["particle_steps_per_s"] / r["particle_steps_per_s"] - 1.0})
         if with_cpu:
-            cs = state(4247, 65536, W * 0.25, 0.3)
+            cs = state(4247, 65536, side * 0.25, 0.3)
             r["cpu_baseline"] = _cpu_update_rate(prm, cs, dt, 0, threads)
         out["radiation"] = r
         log("leg radiation: %.3e particle-steps/s (%.1f%% over the same run without)"
