@@ -334,7 +334,8 @@ def run_legs(names, with_cpu, threads, model=None):
                          "Gaussian random initial fields (index -2, rms 0.01), BH theta 0.5, dt 1/128" % n,
              "note": "the 128^2 field work is latency-bound: one grid is 64 KiB, so a field kernel moves a few "
                      "hundred KiB and its time is launch and dependency latency, not HBM bandwidth; the deposit "
-                     "is bound by its 16 int64 atomics per particle per field"}
+                     "reads 12 B per particle and sums each block's stencils in an LDS tile, so its time is the "
+                     "stencil arithmetic and one global atomic per touched cell"}
         g.destroy()
         if with_cpu:
             cs, cprm, cdt, cf = W.config3(16384, grid=128, side=256.0, total_mass=1.0)
