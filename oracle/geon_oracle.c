@@ -1453,6 +1453,69 @@ int orc_download(OrcSys* s, double* x, double* y, double* wx, double* wy, double
  * the tree is built (forces.js:71-76).  reuseTree: the tree compute1PN built at
  * these positions stands, its aggregates from before the 1PN kick, refreshed
  * only when Higgs changed the masses (integrator.js:1341-1344). */
+/* _repelPair (integrator.js:687-724): Hertz contact delta^1.5 along the line of
+ * centres, p2 taken where the tree holds it (an image for a ghost), and the
+ * friction of the tangential velocity with its contact torques. */
+static void repelPair(P* p1, double p2x, double p2y, P* p2, double friction) {
+    const double dx = p2x - p1->x, dy = p2y - p1->y;
+    const double distSq = dx * dx + dy * dy;
+    const double minDist = p1->radius + p2->radius;
+    if (distSq >= minDist * minDist) return;
+    const double dist = sqrt(distSq);
+    const double delta = minDist - dist;
+    if (delta <= 0) return;
+    const double Fn = delta * sqrt(delta);
+    const double safeDist = dist > EPS_ ? dist : EPS_;
+    const double nx = dx / safeDist, ny = dy / safeDist;
+    p1->fx -= nx * Fn; p1->fy -= ny * Fn;
+    p2->fx += nx * Fn; p2->fy += ny * Fn;
+    if (friction > 0) {
+        const double tx = -ny, ty = nx;
+        const double v1t = p1->vx * tx + p1->vy * ty + p1->angVel * p1->radius;
+        const double v2t = p2->vx * tx + p2->vy * ty - p2->angVel * p2->radius;
+        const double vRel = v1t - v2t;
+        const double Ft = -friction * Fn * fmax(-1.0, fmin(1.0, vRel * 10));
+        p1->fx += tx * Ft; p1->fy += ty * Ft;
+        p2->fx -= tx * Ft; p2->fy -= ty * Ft;
+        p1->contactTorque += p1->radius * Ft;
+        p2->contactTorque -= p2->radius * Ft;
+    }
+}
+
+/* _applyRepulsion (integrator.js:669-685): every body queries the substep's tree
+ * (ghost images included when periodic) with a box of half-size 2 r and repels
+ * each candidate of larger id, in the order the query meets them. */
+static void applyRepulsion(OrcSys* s) {
+    const Pool* q = &s->pool;
+    const double friction = s->prm.bounce_friction;
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(q->count + 8));
+    for (int ci = 0; ci < s->n; ci++) {
+        P* p1 = &s->p[ci];
+        const double rx = p1->x, ry = p1->y, rw = p1->radius * 2, rh = p1->radius * 2;
+        int top = 0;
+        stack[top++] = s->root;
+        while (top > 0) {
+            const int ni = stack[--top];
+            if (rx - rw > q->bx[ni] + q->bw[ni] || rx + rw < q->bx[ni] - q->bw[ni] ||
+                ry - rh > q->by[ni] + q->bh[ni] || ry + rh < q->by[ni] - q->bh[ni]) continue;
+            if (!q->divided[ni]) {
+                const size_t base = (size_t)ni * QT_CAP;
+                for (int k = 0; k < q->pointCount[ni]; k++) {
+                    const P* p2 = &s->tp[q->points[base + k]];
+                    if (!(p2->x >= rx - rw && p2->x <= rx + rw && p2->y >= ry - rh && p2->y <= ry + rh)) continue;
+                    P* real2 = &s->p[p2->original];
+                    if (p2->original == ci || p1->id >= real2->id) continue;
+                    repelPair(p1, p2->x, p2->y, real2, friction);
+                }
+            } else {
+                stack[top++] = q->nw[ni]; stack[top++] = q->ne[ni];
+                stack[top++] = q->sw[ni]; stack[top++] = q->se[ni];
+            }
+        }
+    }
+    free(stack);
+}
+
 static void forcePass(OrcSys* s, int reuseTree) {
     if (s->barnesHut) {
         if (!reuseTree) buildTree(s);
@@ -1477,6 +1540,12 @@ static void forcePass(OrcSys* s, int reuseTree) {
     if (s->grav) {
         if (s->higgs) of_grav_forces(s, &s->higgsF);
         if (s->axion) of_grav_forces(s, &s->axionF);
+    }
+    if (s->prm.collision_mode == GEON_COL_BOUNCE) {                      /* integrator.js:779, 1545 */
+        /* the substep's tree, which the reference always builds, with its ghosts when periodic */
+        const int ghostTree = s->periodic && s->prm.periodic_mode == GEON_PERIODIC_GHOSTS;
+        if (!s->barnesHut || (s->periodic && !ghostTree)) buildTreeG(s, s->periodic);
+        applyRepulsion(s);
     }
     if (s->prm.boundary_mode == GEON_BOUND_BOUNCE) applyBoundaryForces(s);   /* integrator.js:780, 1546 */
 }
@@ -1602,7 +1671,7 @@ int orc_update(OrcSys* s, double dt, int32_t* substeps) {
         }
         prevMaxAccelSq = nextMaxAccelSq;
         const int needSpinOrbit = s->spinOrbit && (hasMagnetic || hasGM);     /* integrator.js:930-981 */
-        const int needTorques = (hasGM && relOn) || hasGrav;
+        const int needTorques = (hasGM && relOn) || hasGrav || s->prm.collision_mode == GEON_COL_BOUNCE;
         if (needSpinOrbit || needTorques) {
             for (int i = 0; i < n; i++) {
                 P* p = &s->p[i];

@@ -30,7 +30,7 @@ TOGGLE_NAMES = ("gravityEnabled", "coulombEnabled", "magneticEnabled", "gravitom
                 "relativityEnabled", "spinOrbitEnabled", "radiationEnabled", "blackHoleEnabled",
                 "disintegrationEnabled", "expansionEnabled", "yukawaEnabled", "higgsEnabled", "axionEnabled",
                 "barnesHutEnabled", "bosonInterEnabled")
-DESPAWN, BOUNCE, LOOP, TORUS, PASS = 0, 1, 2, 0, 0
+DESPAWN, BOUNCE, LOOP, TORUS, PASS, COL_BOUNCE = 0, 1, 2, 0, 0, 2
 
 
 def _toggles(*names):
@@ -64,11 +64,18 @@ def _pressed_box(n, seed, side):
     return st
 
 
+def _crowd(n, seed, side):
+    """Bodies of radius about 1 packed so that most touch a neighbour, spinning and moving."""
+    g = np.random.Generator(np.random.Philox(9400 + seed))
+    return dict(x=g.uniform(0, side, n), y=g.uniform(0, side, n), wx=g.normal(0, 0.2, n), wy=g.normal(0, 0.2, n),
+                mass=g.lognormal(0, 0.3, n), charge=g.integers(-3, 4, n) * 0.1, angw=g.uniform(-0.3, 0.3, n))
+
+
 def cases():
     out = []
 
-    def add(name, st, toggles, updates, dt, boundary=DESPAWN, side=64.0, fields=None):
-        c = dict(name=name, toggles=toggles, updates=updates, dt=dt, collisionMode=PASS, boundaryMode=boundary,
+    def add(name, st, toggles, updates, dt, boundary=DESPAWN, side=64.0, fields=None, collision=PASS):
+        c = dict(name=name, toggles=toggles, updates=updates, dt=dt, collisionMode=collision, boundaryMode=boundary,
                  topology=TORUS, W=side, H=side,
                  particles={k: np.asarray(st[k], np.float64).tolist() for k in
                             ("x", "y", "wx", "wy", "mass", "charge", "angw")})
@@ -103,6 +110,12 @@ def cases():
     # bounce walls: Hertz repulsion with friction, contact torque, the clamp (integrator.js:606-667, 1812-1818)
     add("bounce_walls", _pressed_box(96, 8, 32.0), _toggles("gravityEnabled", "coulombEnabled"), 40, 1.0 / 64,
         boundary=BOUNCE, side=32.0)
+    # bounce collisions: Hertz contact with friction between bodies (integrator.js:669-724), in a crowded box with
+    # bounce walls, and on a torus where contacts cross the edges through the ghost images
+    add("bounce_collisions", _crowd(160, 9, 28.0), _toggles("gravityEnabled", "coulombEnabled"), 30, 1.0 / 64,
+        boundary=BOUNCE, side=28.0, collision=COL_BOUNCE)
+    add("bounce_collisions_torus", _crowd(160, 10, 28.0), _toggles("gravityEnabled", "barnesHutEnabled"), 30,
+        1.0 / 64, boundary=LOOP, side=28.0, collision=COL_BOUNCE)
     return out
 
 
