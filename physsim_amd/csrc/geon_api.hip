@@ -151,6 +151,11 @@ struct GeonContext {
     int32_t sel_slot = -1;
     int32_t* sel_idx = nullptr;
     float* sel_rec = nullptr;
+    // collision mode bounce: the overlap pairs (at most kBouncePairs per body)
+    // and each body's list of the pairs it is the higher-slot member of
+    int32_t *bc_pa = nullptr, *bc_pb = nullptr, *bc_cnt1 = nullptr, *bc_off1 = nullptr;
+    int32_t *bc_cnt2 = nullptr, *bc_off2 = nullptr, *bc_fill2 = nullptr, *bc_list2 = nullptr, *bc_tot = nullptr;
+    int64_t bc_cap = 0;
 };
 
 #define HIP_OK(expr)                                                                         \
@@ -213,11 +218,9 @@ uint32_t validate(const GeonParams* p, std::string* why) {
     if ((t & (GEON_HIGGS | GEON_AXION)) && (G < 8 || G > 128 || (G & (G - 1)))) {
         *why = "field_grid must be a power of two in [8, 128]"; return 1;
     }
-    if (p->collision_mode != GEON_COL_PASS && p->collision_mode != GEON_COL_MERGE) {
-        *why = "collision mode 'bounce' (Hertz contact forces) is not run"; return 1;
-    }
-    if (p->collision_mode == GEON_COL_MERGE && p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) {
-        *why = "merge collisions run on open and torus domains"; return 1;
+    if (p->collision_mode < GEON_COL_PASS || p->collision_mode > GEON_COL_BOUNCE) { *why = "unknown collision mode"; return 1; }
+    if (p->collision_mode != GEON_COL_PASS && p->boundary_mode == GEON_BOUND_LOOP && p->topology != GEON_TORUS) {
+        *why = "merge and bounce collisions run on open and torus domains"; return 1;
     }
     if (p->topology < GEON_TORUS || p->topology > GEON_RP2) { *why = "unknown topology"; return 1; }
     if (p->periodic_mode < GEON_PERIODIC_MIN_IMAGE || p->periodic_mode > GEON_PERIODIC_WINDOWED) { *why = "unknown periodic_mode"; return 1; }
@@ -325,7 +328,8 @@ StepArgs step_args(GeonContext* h) {
     a.rad_e = h->rad_scr;
     a.rad_stride = h->cap;
     if (h->sel_slot >= 0) { a.sel = h->sel_idx; a.sel_out = h->sel_rec; }
-    if (h->prm.boundary_mode == GEON_BOUND_BOUNCE) a.contact = h->fc[K_CONTACT];
+    a.col_bounce = h->prm.collision_mode == GEON_COL_BOUNCE;
+    if (h->prm.boundary_mode == GEON_BOUND_BOUNCE || a.col_bounce) a.contact = h->fc[K_CONTACT];
     return a;
 }
 
@@ -664,6 +668,54 @@ int ghost_tree(GeonContext* h) {
     return GEON_OK;
 }
 
+constexpr int kBouncePairs = 8;     // overlap pairs per body a bounce pass holds
+
+// _applyRepulsion (integrator.js:669-724) over the pair list of the tree the
+// pass built at these positions: the pairs counted, scanned and written
+// (lower slot first), then every body's contacts summed in a fixed order.
+int repulsion(GeonContext* h, const ForceArgs& fa) {
+    const int n = h->n_host;
+    if (n == 0) return GEON_OK;
+    int rc;
+    if (!h->bc_pa) {
+        h->bc_cap = (int64_t)kBouncePairs * h->cap;
+        if ((rc = dmalloc(h, &h->bc_pa, (size_t)h->bc_cap)) || (rc = dmalloc(h, &h->bc_pb, (size_t)h->bc_cap)) ||
+            (rc = dmalloc(h, &h->bc_list2, (size_t)h->bc_cap)) || (rc = dmalloc(h, &h->bc_cnt1, (size_t)h->cap)) ||
+            (rc = dmalloc(h, &h->bc_off1, (size_t)h->cap)) || (rc = dmalloc(h, &h->bc_cnt2, (size_t)h->cap)) ||
+            (rc = dmalloc(h, &h->bc_off2, (size_t)h->cap)) || (rc = dmalloc(h, &h->bc_fill2, (size_t)h->cap)) ||
+            (rc = dmalloc(h, &h->bc_tot, 2)))
+            return rc;
+    }
+    PairArgs pa;
+    pa.x = F(h, C_X); pa.y = F(h, C_Y); pa.mass = F(h, C_MASS); pa.radius = F(h, C_RADIUS);
+    pa.slot = I(h, C_SLOT); pa.src4 = h->src4; pa.nodes = h->tb.nodes; pa.max_nodes = 1 + 4 * h->tb.cap_divided;
+    pa.count = h->bc_cnt1; pa.offset = h->bc_off1; pa.pa = nullptr; pa.pb = nullptr; pa.pl = nullptr;
+    pa.index_out = 1;
+    pa.slack = 1.f;
+    pa.dom_w = (float)h->prm.domain_w; pa.dom_h = (float)h->prm.domain_h;
+    pa.periodic = h->prm.boundary_mode == GEON_BOUND_LOOP;
+    pa.ctrl = h->ctrl; pa.n_upper = n;
+    pa.cap_pairs = h->bc_cap;
+    launch_pairs(h->stream, pa, false);
+    exclusive_scan_i32(h->stream, h->bc_cnt1, h->bc_off1, n, &h->ctrl->n_alive, h->bc_tot, h->tb.scan_tmp);
+    pa.pa = h->bc_pa; pa.pb = h->bc_pb;
+    launch_pairs(h->stream, pa, true);
+    RepelArgs r;
+    r.x = F(h, C_X); r.y = F(h, C_Y); r.vx = F(h, C_VX); r.vy = F(h, C_VY); r.ang_vel = F(h, C_ANGVEL);
+    r.radius = F(h, C_RADIUS);
+    r.pa = h->bc_pa; r.pb = h->bc_pb; r.cnt1 = h->bc_cnt1; r.off1 = h->bc_off1;
+    r.cnt2 = h->bc_cnt2; r.off2 = h->bc_off2; r.fill2 = h->bc_fill2; r.list2 = h->bc_list2;
+    r.total2 = h->bc_tot + 1; r.n_pairs = h->bc_tot;
+    r.fx = h->fc[K_FX]; r.fy = h->fc[K_FY]; r.contact = h->fc[K_CONTACT];
+    r.sel = fa.sel; r.sel_out = fa.sel_out; r.ctrl = h->ctrl;
+    r.dom_w = (float)h->prm.domain_w; r.dom_h = (float)h->prm.domain_h; r.friction = (float)h->prm.bounce_friction;
+    r.periodic = pa.periodic;
+    r.cap_pairs = h->bc_cap;
+    r.scan_tmp = h->tb.scan_tmp;
+    launch_repulsion(h->stream, r, n);
+    return GEON_OK;
+}
+
 // computeAllForces + external fields at the current state (forces.js:61-154,
 // integrator.js:563-592), preceded by the tree rebuild (integrator.js:1340-1348).
 // reuse_tree: the tree built for this substep's 1PN pass stands (integrator.js:1340-1348
@@ -683,11 +735,25 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, h->stream, h->ctrl);
     if (n == 0) return GEON_OK;
     const bool ghosts = bh && ghosts_on(h->prm);
+    const bool col_bounce = h->prm.collision_mode == GEON_COL_BOUNCE;
     if (ghosts) {
         int rc = ghost_tree(h);
         if (rc) return rc;
         h->tree_valid = true;
         h->tree_ghost = true;
+        if (col_bounce) {           // the contact query runs over the particles' own tree
+            Timer t(h, 1);
+            build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
+                       F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl, false);
+        }
+    } else if (!bh && col_bounce) {     // the reference builds its tree every substep (integrator.js:1340-1348)
+        int rc = sort_and_gather(h);
+        if (rc) return rc;
+        Timer t(h, 1);
+        build_tree(h->stream, h->tb, F(h, C_X), F(h, C_Y), F(h, C_MASS), F(h, C_WX), F(h, C_WY), F(h, C_MU),
+                   F(h, C_L), I(h, C_QQ), n, key_params(h->prm), h->ctrl, false);
+        h->tree_valid = true;
+        h->tree_ghost = false;
     } else if (bh) {
         h->tree_ghost = false;
         if (!reuse_tree) {
@@ -745,8 +811,13 @@ int force_pass(GeonContext* h, bool reuse_tree = false, bool collect = false) {
                      !!(tg & GEON_COULOMB), !!(tg & GEON_YUKAWA), particle_view(h), h->fc[K_FX], h->fc[K_FY],
                      h->fc[K_PEG], h->fc[K_PFI], n, fa.sel, fa.sel_out);
     }
+    if (col_bounce) {           // _applyRepulsion (integrator.js:779, 1545)
+        int rc = repulsion(h, fa);
+        if (rc) return rc;
+    }
     if (h->prm.boundary_mode == GEON_BOUND_BOUNCE) {    // after every other force (integrator.js:780, 1546)
         WallArgs w;
+        w.accumulate = col_bounce;
         w.x = F(h, C_X); w.y = F(h, C_Y); w.vx = F(h, C_VX); w.vy = F(h, C_VY); w.ang_vel = F(h, C_ANGVEL);
         w.radius = F(h, C_RADIUS); w.fx = h->fc[K_FX]; w.fy = h->fc[K_FY]; w.contact = h->fc[K_CONTACT];
         w.sel = fa.sel; w.sel_out = fa.sel_out; w.ctrl = h->ctrl;
@@ -793,6 +864,7 @@ int check_overflow(GeonContext* h) {
     switch (h->ctrl_host->overflow) {
         case 0: return GEON_OK;
         case 3: return set_err(h, GEON_EOVERFLOW, "far-field pair list full: raise far_list_cap");
+        case 4: return set_err(h, GEON_EOVERFLOW, "bounce collisions: more overlap pairs than 8 per body");
         default: return set_err(h, GEON_EOVERFLOW, "tree node pool or walk stack overflow");
     }
 }
@@ -1073,6 +1145,9 @@ void release(GeonContext* h) {
     hipFree(h->hist4); hipFree(h->hist_aw); hipFree(h->hist_fe); hipFree(h->hist_cd); hipFree(h->ev_time);
     hipFree(h->dead);
     hipFree(h->sel_idx); hipFree(h->sel_rec);
+    for (int32_t* p : {h->bc_pa, h->bc_pb, h->bc_cnt1, h->bc_off1, h->bc_cnt2, h->bc_off2, h->bc_fill2, h->bc_list2,
+                       h->bc_tot})
+        hipFree(p);
     TreeBufs& tb = h->tb;
     tree_free(tb);
     ghost_release(h);

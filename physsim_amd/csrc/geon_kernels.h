@@ -228,8 +228,33 @@ struct PairArgs {
     int periodic;
     Ctrl* ctrl;
     int n_upper;
+    int64_t cap_pairs = INT64_MAX;  // write pass: pairs the output holds (beyond: overflow 4, not written)
 };
 void launch_pairs(hipStream_t s, const PairArgs& a, bool write);
+
+// ---- collision mode bounce (k_step.hip) ----
+// _applyRepulsion / _repelPair (integrator.js:669-724): the overlap pairs of
+// the pair list (pa lower slot, pb higher, grouped by pa), each body's sum of
+// its contacts in a fixed order: as the higher-slot member in pair order,
+// then as the lower in list order.  Hertz force and friction into the force,
+// contact torque into `contact`, normal forces into the selected particle's
+// external share.
+struct RepelArgs {
+    const float *x, *y, *vx, *vy, *ang_vel, *radius;
+    const int32_t *pa, *pb, *cnt1, *off1;
+    int32_t *cnt2, *off2, *fill2, *list2;
+    int32_t* total2;                // device: scratch for the scan's total
+    const int32_t* n_pairs;         // device: pairs in the list
+    float *fx, *fy, *contact;
+    const int32_t* sel;
+    float* sel_out;
+    const Ctrl* ctrl;
+    float dom_w, dom_h, friction;
+    int periodic;
+    int64_t cap_pairs;
+    int32_t* scan_tmp;
+};
+void launch_repulsion(hipStream_t s, const RepelArgs& a, int n_upper);
 
 // ---- collision mode merge (k_merge.hip) ----
 // Outcome of one pair of the fold; spawn records use the same layout.
@@ -396,6 +421,7 @@ struct StepArgs {
     int32_t* alive;
     Ctrl* ctrl;
     uint32_t rel, mag, gm, ext_bz_on, grav, spin_orbit, one_pn, expansion;
+    uint32_t col_bounce = 0;        // collision mode bounce: the torque step always runs (integrator.js:932)
     float hubble, cx, cy;
     int n_upper;
     // Landau-Lifshitz radiation reaction (radiation and Coulomb on): the jerk
@@ -420,6 +446,7 @@ struct WallArgs {
     float* sel_out;
     const Ctrl* ctrl;
     float w, h, friction;
+    int accumulate = 0;             // 1: add to the contact torque the body contacts left
 };
 void launch_wall_forces(hipStream_t s, const WallArgs& a, int n_upper);
 // FP64 column sums: out[c] (+)= sum of column c over the live particles;

@@ -501,9 +501,13 @@ __global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
                     const float md = r1 + a.radius[j] * a.slack;
                     if (dx * dx + dy * dy < md * md) {
                         if (WRITE) {
-                            a.pa[out] = a.index_out ? i : slot1;
-                            a.pb[out] = a.index_out ? j : slot2;
-                            if (a.pl) a.pl[out] = nd.child;
+                            if (out < a.cap_pairs) {
+                                a.pa[out] = a.index_out ? i : slot1;
+                                a.pb[out] = a.index_out ? j : slot2;
+                                if (a.pl) a.pl[out] = nd.child;
+                            } else {
+                                a.ctrl->overflow = 4;
+                            }
                             out++;
                         }
                         found++;

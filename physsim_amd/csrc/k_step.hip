@@ -140,7 +140,7 @@ __device__ __forceinline__ double kick_drift_one(const StepArgs& a, int i, float
         float ang_vel = a.ang_vel[i];
         const float r = a.radius[i];
         const bool need_so = a.spin_orbit && (a.mag || a.gm);
-        const bool need_tq = (a.gm && a.rel) || a.grav;
+        const bool need_tq = (a.gm && a.rel) || a.grav || a.col_bounce;
         if (need_so && fabsf(ang_vel) >= 1e-12f) {
             const float ixo = a.L[i];       // angMomentum cached by the last force pass
             if (fabsf(ixo) >= 1e-12f) {
@@ -371,7 +371,7 @@ __global__ void k_wall_forces(WallArgs a) {
         if (mu > 0.f) { const float Ft = tangential(Fn, vx - w * r); fx += Ft; tq -= r * Ft; }
     }
     a.fx[i] = fx; a.fy[i] = fy;
-    a.contact[i] = tq;
+    a.contact[i] = (a.accumulate ? a.contact[i] : 0.f) + tq;
     // forceExternal takes the normal forces only (integrator.js:619-658)
     if (a.sel_out && i == *a.sel) { a.sel_out[SEL_EXT] += ex; a.sel_out[SEL_EXT + 1] += ey; }
 }
@@ -379,6 +379,103 @@ __global__ void k_wall_forces(WallArgs a) {
 void launch_wall_forces(hipStream_t s, const WallArgs& a, int n_upper) {
     if (n_upper <= 0) return;
     hipLaunchKernelGGL(k_wall_forces, dim3((n_upper + 255) / 256), dim3(256), 0, s, a);
+}
+
+// One contact of the pair (p1, p2): Hertz force delta^1.5 along the unit
+// vector n from p1 to p2 and the friction force Ft along t = (-n_y, n_x);
+// false when they do not overlap.  Both members evaluate it with the same
+// operands, so both see the same numbers.
+__device__ __forceinline__ bool repel_pair(const RepelArgs& a, int i1, int i2, float& nFx, float& nFy, float& Ft,
+                                           float& tx, float& ty) {
+    float dx = a.x[i2] - a.x[i1], dy = a.y[i2] - a.y[i1];
+    if (a.periodic) { dx = torus_wrap(dx, a.dom_w, 0.5f * a.dom_w); dy = torus_wrap(dy, a.dom_h, 0.5f * a.dom_h); }
+    const float r1 = a.radius[i1], r2 = a.radius[i2];
+    const float dist_sq = dx * dx + dy * dy;
+    const float min_dist = r1 + r2;
+    if (dist_sq >= min_dist * min_dist) return false;
+    const float dist = sqrtf(dist_sq);
+    const float delta = min_dist - dist;
+    if (delta <= 0.f) return false;
+    const float Fn = delta * sqrtf(delta);
+    const float safe = dist > 1e-12f ? dist : 1e-12f;
+    const float nx = dx / safe, ny = dy / safe;
+    nFx = nx * Fn; nFy = ny * Fn;
+    tx = -ny; ty = nx;
+    Ft = 0.f;
+    if (a.friction > 0.f) {
+        const float v1t = a.vx[i1] * tx + a.vy[i1] * ty + a.ang_vel[i1] * r1;
+        const float v2t = a.vx[i2] * tx + a.vy[i2] * ty - a.ang_vel[i2] * r2;
+        Ft = -a.friction * Fn * fminf(1.f, fmaxf(-1.f, (v1t - v2t) * 10.f));
+    }
+    return true;
+}
+
+__global__ void k_repel_zero(int32_t* __restrict__ cnt2, int32_t* __restrict__ fill2, const Ctrl* __restrict__ ctrl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ctrl->n_alive) { cnt2[i] = 0; fill2[i] = 0; }
+}
+
+// grid-stride over the pairs the list holds
+__global__ void k_repel_count2(RepelArgs a) {
+    const int64_t np = min((int64_t)*a.n_pairs, a.cap_pairs);
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < np; k += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&a.cnt2[a.pb[k]], 1);
+}
+
+__global__ void k_repel_fill2(RepelArgs a) {
+    const int64_t np = min((int64_t)*a.n_pairs, a.cap_pairs);
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < np; k += (int64_t)gridDim.x * blockDim.x) {
+        const int b = a.pb[k];
+        a.list2[a.off2[b] + atomicAdd(&a.fill2[b], 1)] = (int32_t)k;
+    }
+}
+
+__global__ void k_repulsion(RepelArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ctrl->n_alive) return;
+    float fx = a.fx[i], fy = a.fy[i], tq = 0.f, ex = 0.f, ey = 0.f;
+    const float ri = a.radius[i];
+    // as the higher-slot member: this body's entries in pair order (the
+    // atomic fill left them in any order; a short insertion sort fixes it)
+    int32_t* l2 = a.list2 + a.off2[i];
+    const int c2 = a.cnt2[i];
+    for (int u = 1; u < c2; u++) {
+        const int32_t v = l2[u];
+        int w = u - 1;
+        while (w >= 0 && l2[w] > v) { l2[w + 1] = l2[w]; w--; }
+        l2[w + 1] = v;
+    }
+    float nFx, nFy, Ft, tx, ty;
+    for (int u = 0; u < c2; u++) {
+        if (!repel_pair(a, a.pa[l2[u]], i, nFx, nFy, Ft, tx, ty)) continue;
+        fx += nFx; fy += nFy; ex += nFx; ey += nFy;
+        fx -= tx * Ft; fy -= ty * Ft;
+        tq -= ri * Ft;
+    }
+    // as the lower-slot member: its own pairs in list order
+    const int64_t np = min((int64_t)*a.n_pairs, a.cap_pairs);
+    const int o1 = a.off1[i], c1 = a.cnt1[i];
+    for (int u = 0; u < c1 && o1 + u < np; u++) {
+        if (!repel_pair(a, i, a.pb[o1 + u], nFx, nFy, Ft, tx, ty)) continue;
+        fx -= nFx; fy -= nFy; ex -= nFx; ey -= nFy;
+        fx += tx * Ft; fy += ty * Ft;
+        tq += ri * Ft;
+    }
+    a.fx[i] = fx; a.fy[i] = fy;
+    a.contact[i] = tq;
+    if (a.sel_out && i == *a.sel) { a.sel_out[SEL_EXT] += ex; a.sel_out[SEL_EXT + 1] += ey; }
+}
+
+void launch_repulsion(hipStream_t s, const RepelArgs& a, int n_upper) {
+    if (n_upper <= 0) return;
+    const dim3 grid((n_upper + 255) / 256), block(256);
+    const int64_t pb = (a.cap_pairs + 255) / 256;
+    const dim3 pgrid((unsigned)(pb < 4096 ? pb : 4096));
+    hipLaunchKernelGGL(k_repel_zero, grid, block, 0, s, a.cnt2, a.fill2, a.ctrl);
+    hipLaunchKernelGGL(k_repel_count2, pgrid, block, 0, s, a);
+    exclusive_scan_i32(s, a.cnt2, a.off2, n_upper, &a.ctrl->n_alive, a.total2, a.scan_tmp);
+    hipLaunchKernelGGL(k_repel_fill2, pgrid, block, 0, s, a);
+    hipLaunchKernelGGL(k_repulsion, grid, block, 0, s, a);
 }
 
 void launch_boundary(hipStream_t s, const StepArgs& a, int mode, int topo, double w, double h) {

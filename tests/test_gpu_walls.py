@@ -1,7 +1,9 @@
-"""Boundary mode bounce on the GPU (through the C ABI): the Hertz wall forces
-with friction and their contact torque (integrator.js:606-667, 964-981) and
-the once-per-update clamp (integrator.js:1812-1818), against the FP64 oracle
-(itself pinned by closed forms in test_oracle.py)."""
+"""Boundary mode bounce and collision mode bounce on the GPU (through the C
+ABI): the Hertz wall forces with friction and their contact torque
+(integrator.js:606-667, 964-981), the once-per-update clamp
+(integrator.js:1812-1818), and the Hertz contacts between bodies
+(integrator.js:669-724), against closed forms, the reference's own forces and
+the FP64 oracle (pinned to the reference by the fixtures in tests/golden)."""
 import numpy as np
 import pytest
 
@@ -106,3 +108,68 @@ def test_selected_external_share_holds_the_wall_force():
     assert s["forceExternal"]["x"] == pytest.approx(Fn, rel=1e-6) and s["forceExternal"]["y"] == 0
     f = g.forces(1)
     assert f["fx"][0] == pytest.approx(Fn, rel=1e-6) and f["fy"][0] == pytest.approx(-0.4 * Fn, rel=1e-6)
+
+
+# ---- collision mode bounce: Hertz contacts between bodies ------------------
+
+def test_contact_pair_closed_form():
+    """Two bodies of mass 8 (radius 2) 1.5 apart overlap by 2.5: each feels
+    2.5^1.5 along the line of centres, away from the other, and the friction
+    of their tangential velocities (integrator.js:687-724)."""
+    from physsim_amd import HipPhysics, default_params
+    prm = dict(toggles0=A.GRAVITY, collision_mode=A.COL_BOUNCE, domain_w=64.0, domain_h=64.0, softening=1.0,
+               bounce_friction=0.0)
+    g = HipPhysics(default_params(**prm), max_particles=2)
+    g.addParticle(30.0, 32.0, 0.0, 0.0, 8.0, 0.0)
+    g.addParticle(31.5, 32.0, 0.0, 0.0, 8.0, 0.0)
+    g.compute_forces()
+    f = g.forces(2)
+    grav = 64.0 * 1.5 / (1.5 ** 2 + 1.0) ** 1.5        # the gravity pair force, softening 1
+    Fn = 2.5 ** 1.5
+    assert f["fx"][0] == pytest.approx(grav - Fn, rel=1e-6) and f["fx"][1] == pytest.approx(Fn - grav, rel=1e-6)
+    assert f["fy"][0] == 0 and f["fy"][1] == 0
+    # friction: body 2 slides up along the contact (t = (0, 1) seen from body 1), vRel = -0.3
+    prm["bounce_friction"] = 0.4
+    g = HipPhysics(default_params(**prm), max_particles=2)
+    g.addParticle(30.0, 32.0, 0.0, 0.0, 8.0, 0.0)
+    g.addParticle(31.5, 32.0, 0.0, 0.3, 8.0, 0.0)
+    g.compute_forces()
+    f = g.forces(2)
+    Ft = -0.4 * Fn * -1.0                               # clamp(10 * -0.3) = -1
+    assert f["fy"][0] == pytest.approx(Ft, rel=1e-6) and f["fy"][1] == pytest.approx(-Ft, rel=1e-6)
+
+
+@pytest.mark.parametrize("case", ["bounce_collisions", "bounce_collisions_torus"])
+def test_contacts_match_reference_bootstrap_and_oracle_trajectory(case, oracle_lib):
+    """The reference's own bootstrap forces of a crowded box (contacts dominate
+    them) and, over 10 updates, the oracle, which the same fixture pins to the
+    reference to FP64 rounding."""
+    import json
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_%s.npz" % case))
+    meta = json.loads(bytes(z["meta"]).decode())
+    t = 0
+    for name, bit in A.TOGGLE_BITS.items():
+        if meta["toggles"].get(name):
+            t |= bit
+    # the reference's periodic walk counts the ghost images (A1): the image-particle mode
+    prm = dict(toggles0=t, boundary_mode=meta["boundaryMode"], topology=meta["topology"],
+               collision_mode=meta["collisionMode"], domain_w=meta["W"], domain_h=meta["H"],
+               periodic_mode=A.PERIODIC_GHOSTS)
+    st = {k: z["in_" + k] for k in ("x", "y", "wx", "wy", "mass", "charge", "angw")}
+    g, o = _pair(st, prm, oracle_lib)
+    g.compute_forces()
+    n = len(st["x"])
+    f = g.forces(n)
+    fref = np.hypot(z["bootstrap_fx"], z["bootstrap_fy"])
+    err = np.hypot(f["fx"] - z["bootstrap_fx"], f["fy"] - z["bootstrap_fy"])
+    print("%s: bootstrap max |dF| %.3e of max |F| %.3e" % (case, err.max(), fref.max()))
+    # FP32: the overlap delta = r1 + r2 - |d| loses digits to cancellation (positions ~30, ulp 2e-6)
+    assert err.max() <= 3e-5 * fref.max()
+    for step in range(10):
+        assert g.update(meta["dt"]) == o.update(meta["dt"]), step
+    dg, do = g.download(), o.download()
+    ids = do["id"]
+    np.testing.assert_allclose(dg["x"][ids], do["x"], rtol=0, atol=1e-4)
+    np.testing.assert_allclose(dg["y"][ids], do["y"], rtol=0, atol=1e-4)
+    np.testing.assert_allclose(dg["angw"][ids], do["angw"], rtol=0, atol=1e-4)
