@@ -293,6 +293,27 @@ def run_legs(names, with_cpu, threads, model=None):
         out["merge"] = r
         log("leg merge: first update %.1f ms (%d events), steady %.3e particle-steps/s"
             % (first_ms, events, steady["particle_steps_per_s"]))
+    if "bounce" in names:
+        # collision mode bounce (integrator.js:669-724) and bounce walls (606-667): bodies packed at about half
+        # the box's area, so most touch a neighbour
+        n, side, dt = 1 << 20, 2896.0, 1.0 / 64
+        prm = dict(toggles0=A.GRAVITY | A.COULOMB | A.BARNES_HUT, domain_w=side, domain_h=side,
+                   boundary_mode=A.BOUND_BOUNCE, theta=0.5, softening=8.0, collision_mode=A.COL_BOUNCE,
+                   signal_delay=A.DELAY_OFF)
+        st = state(4248, n, side, 0.3)
+        contacts = gpu_rate(prm, st, dt, 2, 4)
+        passing = gpu_rate(dict(prm, collision_mode=A.COL_PASS), st, dt, 2, 4)
+        r = {"workload": "N=%d uniform %gx%g box with bounce walls, masses U[0.5, 3] (radii 0.8-1.4, about half the "
+                         "area covered), gravity+Coulomb, BH theta 0.5, dt 1/64, collision mode bounce"
+                         % (n, side, side),
+             "particle_steps_per_s": contacts["particle_steps_per_s"], "contacts": contacts, "pass_mode": passing,
+             "contact_ms_per_update": contacts["ms_per_update"] - passing["ms_per_update"]}
+        if with_cpu:
+            cs = state(4249, 65536, side * 0.25, 0.3)
+            r["cpu_baseline"] = _cpu_update_rate(prm, cs, dt, 0, threads)
+        out["bounce"] = r
+        log("leg bounce: %.3e particle-steps/s (pass mode %.3e)"
+            % (contacts["particle_steps_per_s"], passing["particle_steps_per_s"]))
     if "config3" in names:
         # configuration 3: gravity + 1PN EIH (with gravitomagnetism) + Higgs + axion on the 128^2 grid
         n = 1 << 20
@@ -433,7 +454,7 @@ def main():
     ap.add_argument("--drift-n", type=int, default=1 << 20)
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="batched-ensemble leg (config 4): systems; 0 skips it")
-    ap.add_argument("--legs", default="config3,torus,delay,merge,radiation",
+    ap.add_argument("--legs", default="config3,torus,delay,merge,bounce,radiation",
                     help="rows beyond the headline to time ('' for none)")
     ap.add_argument("--methods", default="plain,learned,multipole",
                     help="comma list of force methods to run (the learned one is always run)")
