@@ -104,7 +104,8 @@ typedef struct GeonParams {
     int32_t  field_grid;        /* scalar-field grid side: 64 CPU, 128 GPU (config.js:84,125) */
     int32_t  kahan;             /* 1: compensated FP32 force accumulation */
     int32_t  far_field;         /* GEON_FAR_*: correction added to accepted nodes */
-    int32_t  far_list_cap;      /* LEARNED: listed (particle, node) pairs per 64 particles; 0 = 65536 */
+    int32_t  far_list_cap;      /* LEARNED: listed (particle, node) pairs per 64 particles, one queue of
+                                   far_list_cap / 64 per particle (rounded up to 4); 0 = 65536 */
     int32_t  signal_delay;      /* GEON_DELAY_*: FOLLOW couples it to GEON_RELATIVITY (forces.js:85) */
     int32_t  history_stride;    /* HISTORY_STRIDE = 64 updates per history sample (config.js:122); 0 = 64 */
     double   bounce_friction;   /* bounceFriction = 0.4 (integrator.js:72): tangential friction of the

@@ -428,15 +428,15 @@ bool want_moments(GeonContext* h) {
 
 int far_list_alloc(GeonContext* h) {
     const int64_t groups = ((int64_t)h->cap + kWave - 1) / kWave;
-    const int64_t cap = h->prm.far_list_cap > 0 ? h->prm.far_list_cap : 65536;
+    // whole 1 KB blocks of the lane queues: a multiple of 4 entries per lane
+    const int64_t blk = 4 * kWave;
+    const int64_t cap = ((h->prm.far_list_cap > 0 ? (int64_t)h->prm.far_list_cap : 65536) + blk - 1) / blk * blk;
     if (h->far_list && h->far_cap == cap && h->far_groups == groups) return GEON_OK;
-    if (1 + 4 * h->tb.cap_divided > (int64_t)kFarNodeMask)
-        return set_err(h, GEON_EUNSUPPORTED, "too many tree nodes for the far-list encoding");
     hipFree(h->far_list); hipFree(h->far_count);
     h->far_list = nullptr; h->far_count = nullptr;
     int rc;
     if ((rc = dmalloc(h, &h->far_list, (size_t)(groups * cap)))) return rc;
-    if ((rc = dmalloc(h, &h->far_count, (size_t)groups))) return rc;
+    if ((rc = dmalloc(h, &h->far_count, (size_t)(groups * kWave)))) return rc;
     h->far_cap = cap; h->far_groups = groups;
     return GEON_OK;
 }
@@ -2393,11 +2393,15 @@ int geon_far_collect(GeonHandle h, int32_t group_stride, int64_t* n_samples) {
         if ((rc = read_ctrl(h))) return rc;
         if ((rc = check_overflow(h))) return rc;
         const int64_t groups = (n + kWave - 1) / kWave;
-        std::vector<int32_t> cnt((size_t)groups);
-        HIP_OK(hipMemcpy(cnt.data(), h->far_count, 4 * (size_t)groups, hipMemcpyDeviceToHost));
+        std::vector<int32_t> cnt((size_t)(groups * kWave));
+        HIP_OK(hipMemcpy(cnt.data(), h->far_count, 4 * (size_t)(groups * kWave), hipMemcpyDeviceToHost));
         std::vector<int64_t> sel, off;
         int64_t rows = 0;
-        for (int64_t g = 0; g < groups; g += group_stride) { sel.push_back(g); off.push_back(rows); rows += cnt[g]; }
+        for (int64_t g = 0; g < groups; g += group_stride) {
+            sel.push_back(g);
+            off.push_back(rows);
+            for (int l = 0; l < kWave; l++) rows += cnt[(size_t)(g * kWave + l)];
+        }
         const int64_t ns = (int64_t)sel.size();
         if (2 * ns > h->far_sel_cap) {
             hipFree(h->far_sel);
@@ -2431,7 +2435,7 @@ int geon_far_collect(GeonHandle h, int32_t group_stride, int64_t* n_samples) {
         a.nodes = h->tb.nodes; a.node_mom = h->tb.node_mom;
         a.mass64 = h->tb.node_mass64; a.comx64 = h->tb.node_comx64; a.comy64 = h->tb.node_comy64;
         a.qq64 = h->tb.node_qq64; a.first = h->tb.node_first; a.npts = h->tb.node_npts;
-        a.far_list = h->far_list; a.far_cap = h->far_cap;
+        a.far_list = h->far_list; a.far_count = h->far_count; a.far_cap = h->far_cap;
         a.groups = h->far_sel; a.row_off = h->far_sel + ns; a.n_groups = (int)ns; a.n_rows = rows;
         a.f2ref = h->far_f2;
         a.out = h->far_samples + (size_t)h->far_n_samples * kFarSample;

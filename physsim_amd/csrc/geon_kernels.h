@@ -104,8 +104,8 @@ struct ForceArgs {
     // terms from node_mom, 2 also list (node, lane) per 64-particle group
     int far_mode;
     const NodeMom* node_mom;
-    uint32_t* far_list;         // far_cap entries per group: node | lane << 26
-    int32_t* far_count;         // entries per group
+    uint32_t* far_list;         // far_cap entries per group in lane queues (far_slot)
+    int32_t* far_count;         // entries per lane queue (kWave per group)
     int64_t far_cap;
     // batch of systems (null for one system): per slot system id and liveness,
     // per system controller (its `active` flag) and root node slot
@@ -167,8 +167,14 @@ void ghost_sources(hipStream_t s, const GhostCols& col, const float* ax, const f
                    float4* src4, float4* src8a, float4* src8b, int n_upper, const Ctrl* gctrl);
 void ghost_finish(hipStream_t s, const Ctrl* gctrl, Ctrl* ctrl);
 constexpr int kFarMultipole = 1, kFarList = 2;
-constexpr int kFarLaneShift = 26;
-constexpr uint32_t kFarNodeMask = (1u << kFarLaneShift) - 1;
+// Far pair lists (learned far field): a 64-particle group owns far_cap entries
+// (a multiple of 4 * kWave), one queue of far_cap / kWave accepted node
+// indices per lane.  Entry t of lane l sits at (t / 4) * 4 kWave + 4 l + t % 4
+// in the group's slice, so the t-th entries of the group's lanes lie in one
+// 1 KB block and each lane's next three in the same 16 bytes.
+__host__ __device__ __forceinline__ int64_t far_slot(int64_t group, int64_t far_cap, int lane, int t) {
+    return group * far_cap + (int64_t)(t >> 2) * (4 * kWave) + 4 * lane + (t & 3);
+}
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan);
 // Persistent walk: `grid` blocks of `threads` pull 64-particle groups from *queue
 // (reset to 0 before the launch) until all n_groups are done.
@@ -344,6 +350,7 @@ struct FarSampleArgs {
     const double *mass64, *comx64, *comy64, *qq64;
     const int32_t *first, *npts;
     const uint32_t* far_list;
+    const int32_t* far_count;   // entries per lane queue
     int64_t far_cap;
     const int64_t* groups;      // selected groups
     const int64_t* row_off;     // first sample row of each selected group

@@ -130,7 +130,8 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
     const float half_w = 0.5f * pf.dom_w, half_h = 0.5f * pf.dom_h;
     const double now = DELAY ? a.ctrl->sim_time : 0.0;
     int64_t c_pair = 0, c_node = 0, c_visit = 0;
-    int64_t far_n = 0;                             // far list of this group (FAR == kFarList)
+    int far_n = 0;                                 // entries of this lane's far queue (FAR == kFarList)
+    const int far_lane_cap = (int)(a.far_cap / kWave);
     int sp = 0;
     if (a.bsys == nullptr) {
         if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
@@ -205,15 +206,10 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
                 c_node++;
                 if (FAR == kFarMultipole) far_multipole<F, KAHAN, SEL>(o, acc, pf, dx, dy, a.node_mom[node], split);
             }
-            if (FAR == kFarList) {
-                // the accepting lanes append (node, lane) in lane order: one coalesced run
-                const uint64_t took = __ballot(accept);
-                if (took) {
-                    const int64_t pos = far_n + __popcll(took & (my_bit - 1));
-                    if (accept && pos < a.far_cap)
-                        a.far_list[group * a.far_cap + pos] = (uint32_t)node | ((uint32_t)lane << kFarLaneShift);
-                    far_n += __popcll(took);
-                }
+            if (FAR == kFarList && accept) {
+                // each accepting lane appends the node to its own queue
+                if (far_n < far_lane_cap) a.far_list[far_slot(group, a.far_cap, lane, far_n)] = (uint32_t)node;
+                far_n++;
             }
             const uint64_t open = __ballot(mine && !accept);
             if (open) {
@@ -235,9 +231,9 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
     }
     if constexpr (SEL) return;                     // the force pass has its counters and outputs
     if (overflow && lane == 0) a.ctrl->overflow = 2;
-    if (FAR == kFarList && lane == 0) {
-        a.far_count[group] = (int32_t)(far_n < a.far_cap ? far_n : a.far_cap);
-        if (far_n > a.far_cap) a.ctrl->overflow = 3;
+    if (FAR == kFarList) {
+        a.far_count[group * kWave + lane] = far_n < far_lane_cap ? far_n : far_lane_cap;
+        if (__ballot(far_n > far_lane_cap) != 0 && lane == 0) a.ctrl->overflow = 3;
     }
     wave_add_counters(a.ctrl, c_pair, c_node, c_visit);
     if (valid) store_acc<F, KAHAN>(a, i, o, acc);

@@ -170,7 +170,6 @@ struct FarFrags {
     halfx8 w2h[2][4][kWave];        // inference: layers 2 and 3 in FP16 (as w2, w3)
     halfx8 w3h[4][kWave];
     halfx8 b2h[2][kWave];           // inference: A[m][0], A[m][1] = b2 split into FP16 hi + lo, K 2..15 zero
-    halfx8 b3h[kWave];              // inference: the same for b3 (rows m < kFarOut)
     bf16x8 w3t[2][kWave];           // backward: A = W3 (hidden x outputs; K = 8h + j <-> output 4h + j)
     bf16x8 w2t[2][4][kWave];        // backward: A = W2 (in x out), K (out) permuted
     float b2[kFarHid];
@@ -218,16 +217,6 @@ __global__ void k_far_pack(const float* __restrict__ P, const float* __restrict_
             v[1] = b - (float)hi;
         }
         fr->b2h[mb][l] = pack8h(v);
-    }
-    {
-        float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (h == 0 && c < kFarOut) {
-            const float b = P[kFarOffB3 + c];
-            const _Float16 hi = (_Float16)b;
-            v[0] = (float)hi;
-            v[1] = b - (float)hi;
-        }
-        fr->b3h[l] = pack8h(v);
     }
     if (l < kFarOut) fr->b3[l] = P[kFarOffB3 + l];
     if (l < kFarIn) {
@@ -317,10 +306,6 @@ __device__ __forceinline__ void mlp_forward(const FarFrags& fr, const __bf16* xs
 // ---------------------------------------------------------------------------
 // Inference: add the predicted residual of every listed pair to the forces
 // ---------------------------------------------------------------------------
-struct FarRow {
-    float ux, uy, sg, sc, sp, cp;
-};
-
 // The inference part of FarFrags, staged once per block in LDS so that no wave
 // holds the weights in registers across its tiles.
 struct FarInfer {
@@ -328,103 +313,164 @@ struct FarInfer {
     halfx8 w2[2][4][kWave];             // FP16: the packed-FP16 activations feed them
     halfx8 w3[4][kWave];
     halfx8 b2h[2][kWave];               // layer-2 bias as an MFMA operand (times a column of ones)
-    halfx8 b3h[kWave];                  // layer-3 bias, the same way
-    float istd[kFarIn], nmi[kFarIn], oscale[8];   // x = f istd + nmi, nmi = -mean istd
 };
 
 constexpr int kMlpWaves = 4;                   // waves per block, one 64-particle group each
 
-// The gathered inputs of one listed pair: the observer's {x, y, m, q}, the
-// node's {COM x, y, mass, charge} and half width, its moments.
-struct FarIn {
-    float4 s4, c4;
+// The gathered node of one listed pair: {COM x, y, mass, charge}, half width
+// and moments.
+struct FarNode {
+    float4 c4;
     float half;
     NodeMom mo;
-    int lane;
 };
 
-__device__ __forceinline__ void far_gather(const FarArgs& a, int64_t g, uint32_t e, FarIn& in) {
-    const int node = (int)(e & kFarNodeMask);
-    in.lane = (int)(e >> kFarLaneShift);
-    in.s4 = a.src4[(int)(g * kWave) + in.lane];
-    in.c4 = *reinterpret_cast<const float4*>(&a.nodes[node].comx);
-    in.half = a.nodes[node].half;
-    in.mo = a.node_mom[node];
+__device__ __forceinline__ void far_gather(const FarArgs& a, uint32_t node, FarNode& nd) {
+    nd.c4 = *reinterpret_cast<const float4*>(&a.nodes[node].comx);
+    nd.half = a.nodes[node].half;
+    nd.mo = a.node_mom[node];
 }
 
-// Layers 1-3 for the 64 rows of the tile in xs, both 32-row halves per weight
-// fragment read from LDS (the LDS is busy most of this kernel's time):
-// lane (h, c) gets outputs 4h + r of row 32 nb + c in accumulator r of o[nb]
-// (r < 4, 4h + r < kFarOut).
-__device__ __forceinline__ void mlp_tile(const FarInfer& fr, const __bf16* xs, int l, floatx16* o) {
-    const int h = l >> 5, c = l & 31;
-    bf16x8 bx[2];
-#pragma unroll
-    for (int nb = 0; nb < 2; nb++) bx[nb] = *reinterpret_cast<const bf16x8*>(xs + (32 * nb + c) * 16 + 8 * h);
+// __builtin_amdgcn_permlane32_swap(x, y) returns {x', y'}: lanes 32-63 of x
+// trade places with lanes 0-31 of y (v_permlane32_swap).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Layers 1-3 for NB 32-row column blocks whose layer-1 B operands are bx[nb]
+// (lane (h, c) holds inputs 8h..8h+7 of row 32 nb + c), all blocks per weight
+// fragment read from LDS: lane (h, c) gets outputs 4h + r of row 32 nb + c,
+// without the layer-3 bias, in accumulator r of o[nb] (r < 4, 4h + r <
+// kFarOut).
+//
+// A wave issues independent vector instructions while its own MFMAs execute,
+// but an activation waits for the product it reads.  The network is
+// therefore laid out as stages (fenced with sched_barrier) in which the
+// products of one step issue next to the activations of the step before:
+//   S0-S3   layer-1 products (mb, nb) in turn, each with the activation of
+//           the previous one;
+//   S4, S5  layer-2 products of M-block 0 (bias, K-steps 0-1 need only the
+//           layer-1 M-block 0), with the last layer-1 activation;
+//   S6, S7  layer-2 products of M-block 1, with the activations of M-block 0;
+//   S8, S9  layer-3 products (K-steps 0-1 read M-block 0), with the
+//           activations of M-block 1.
+// The layer-2 bias enters as the C operand of the chains (one product per
+// M-block); the six layer-3 biases are cheaper to add to the outputs.
+// Weight fragments are read from LDS one stage before their products.
+#define GEON_STAGE() __builtin_amdgcn_sched_barrier(0)
+template <int NB>
+__device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, int l, floatx16* o) {
+    static_assert(NB % 2 == 0, "the M-block activations are split over two stages");
+    const int h = l >> 5;
     const floatx16 zero = {};
     // B operand of the bias product: B[0][n] = B[1][n] = 1, the rest 0
     halfx8 ones = {};
     if (h == 0) { ones[0] = (_Float16)1.f; ones[1] = (_Float16)1.f; }
-    halfx8 a1[4][2], a2[4][2];
+    auto act16 = [](const floatx16& z, halfx8& lo, halfx8& hi) {
+        float v[16];
 #pragma unroll
-    for (int mb = 0; mb < 2; mb++) {
-        const bf16x8 w = fr.w1[mb][l];
+        for (int r = 0; r < 16; r++) v[r] = z[r];
+        lo = act8h(v);
+        hi = act8h(v + 8);
+    };
+    halfx8 a1[4][NB], a2[4][NB];
+    floatx16 z[2], acc0[NB], acc1[NB];
+    // S0 .. S(2NB-1): layer-1 product k = NB mb + nb, activation of k - 1
+    const bf16x8 w1a = fr.w1[0][l], w1b = fr.w1[1][l];
+    halfx8 wb0 = fr.b2h[0][l], w0 = fr.w2[0][0][l], w1 = fr.w2[0][1][l];
 #pragma unroll
-        for (int nb = 0; nb < 2; nb++) {
-            const floatx16 z1 = mfma32(w, bx[nb], zero);
-            float v[16];
-#pragma unroll
-            for (int r = 0; r < 16; r++) v[r] = z1[r];
-            a1[2 * mb][nb] = act8h(v);
-            a1[2 * mb + 1][nb] = act8h(v + 8);
+    for (int k = 0; k < 2 * NB; k++) {
+        z[k & 1] = mfma32(k < NB ? w1a : w1b, bx[k % NB], zero);
+        if (k > 0) {
+            const int p = k - 1;
+            act16(z[p & 1], a1[2 * (p / NB)][p % NB], a1[2 * (p / NB) + 1][p % NB]);
         }
+        GEON_STAGE();
     }
-#pragma unroll
-    for (int mb = 0; mb < 2; mb++) {
-        // b2 in every column (hi + lo in FP32) as the C operand of both chains
-        const floatx16 bias = mfma32(fr.b2h[mb][l], ones, zero);
-        floatx16 acc[2];
-        {
-            const halfx8 w = fr.w2[mb][0][l];
-#pragma unroll
-            for (int nb = 0; nb < 2; nb++) acc[nb] = mfma32(w, a1[0][nb], bias);
-        }
-#pragma unroll
-        for (int t = 1; t < 4; t++) {
-            const halfx8 w = fr.w2[mb][t][l];
-#pragma unroll
-            for (int nb = 0; nb < 2; nb++) acc[nb] = mfma32(w, a1[t][nb], acc[nb]);
-        }
-#pragma unroll
-        for (int nb = 0; nb < 2; nb++) {
-            float v[16];
-#pragma unroll
-            for (int r = 0; r < 16; r++) v[r] = acc[nb][r];
-            a2[2 * mb][nb] = act8h(v);
-            a2[2 * mb + 1][nb] = act8h(v + 8);
-        }
-    }
-    // b3 in every column: accumulator r < 4 of lane half h holds output 4h + r
-    const floatx16 b3 = mfma32(fr.b3h[l], ones, zero);
+    // S4: layer-2 M-block 0, bias and K-steps 0-1; the last layer-1 activation
+    halfx8 w2 = fr.w2[0][2][l], w3 = fr.w2[0][3][l];
     {
-        const halfx8 w = fr.w3[0][l];
+        const floatx16 bias = mfma32(wb0, ones, zero);
 #pragma unroll
-        for (int nb = 0; nb < 2; nb++) o[nb] = mfma32(w, a2[0][nb], b3);
+        for (int nb = 0; nb < NB; nb++) acc0[nb] = mfma32(w0, a1[0][nb], bias);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc0[nb] = mfma32(w1, a1[1][nb], acc0[nb]);
+        const int p = 2 * NB - 1;
+        act16(z[p & 1], a1[2][NB - 1], a1[3][NB - 1]);
     }
+    GEON_STAGE();
+    // S5: layer-2 M-block 0, K-steps 2-3
+    halfx8 wb1 = fr.b2h[1][l];
+    w0 = fr.w2[1][0][l];
+    w1 = fr.w2[1][1][l];
 #pragma unroll
-    for (int t = 1; t < 4; t++) {
-        const halfx8 w = fr.w3[t][l];
+    for (int nb = 0; nb < NB; nb++) acc0[nb] = mfma32(w2, a1[2][nb], acc0[nb]);
 #pragma unroll
-        for (int nb = 0; nb < 2; nb++) o[nb] = mfma32(w, a2[t][nb], o[nb]);
+    for (int nb = 0; nb < NB; nb++) acc0[nb] = mfma32(w3, a1[3][nb], acc0[nb]);
+    GEON_STAGE();
+    // S6: layer-2 M-block 1, bias and K-steps 0-1; half of M-block 0's activations
+    w2 = fr.w2[1][2][l];
+    w3 = fr.w2[1][3][l];
+    {
+        const floatx16 bias = mfma32(wb1, ones, zero);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc1[nb] = mfma32(w0, a1[0][nb], bias);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc1[nb] = mfma32(w1, a1[1][nb], acc1[nb]);
+#pragma unroll
+        for (int nb = 0; nb < NB / 2; nb++) act16(acc0[nb], a2[0][nb], a2[1][nb]);
     }
+    GEON_STAGE();
+    // S7: layer-2 M-block 1, K-steps 2-3; the rest of M-block 0's activations
+    w0 = fr.w3[0][l];
+    w1 = fr.w3[1][l];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc1[nb] = mfma32(w2, a1[2][nb], acc1[nb]);
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc1[nb] = mfma32(w3, a1[3][nb], acc1[nb]);
+#pragma unroll
+    for (int nb = NB / 2; nb < NB; nb++) act16(acc0[nb], a2[0][nb], a2[1][nb]);
+    GEON_STAGE();
+    // S8: layer 3, K-steps 0-1 (M-block 0); M-block 1's activations.  The
+    //     caller adds b3 once the outputs sit in their rows' lanes.
+    w2 = fr.w3[2][l];
+    w3 = fr.w3[3][l];
+    {
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) o[nb] = mfma32(w0, a2[0][nb], zero);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) o[nb] = mfma32(w1, a2[1][nb], o[nb]);
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) act16(acc1[nb], a2[2][nb], a2[3][nb]);
+    }
+    GEON_STAGE();
+    // S9: layer 3, K-steps 2-3
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) o[nb] = mfma32(w2, a2[2][nb], o[nb]);
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) o[nb] = mfma32(w3, a2[3][nb], o[nb]);
 }
+#undef GEON_STAGE
 
-__global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_far_mlp(FarArgs a) {
+// Tiles per iteration of k_far_mlp and waves per SIMD.  One tile with the
+// staged network fits 160 registers (three waves per SIMD); two tiles per
+// iteration (four MFMA chains per layer) need more than 256 and spill.
+constexpr int kFarTiles = 1;
+constexpr int kFarMlpOccupancy = 3;
+
+// One wave per 64-particle group, lane l for particle 64 g + l: tile t holds
+// every lane's t-th listed node (far_slot), row l of the tile is lane l's
+// pair, so the predicted residual of a row lands in the lane that owns its
+// observer and is summed there, in list order, without a cross-lane scatter.
+// The inputs and outputs change lane halves with v_permlane32_swap:
+//   inputs: lane (h, c) needs inputs 8h..8h+7 of rows c and 32 + c, i.e. its
+//     own half h and the half h of its partner lane l ^ 32;
+//   outputs: row l's force outputs (0..3) are in lane (0, l % 32), its
+//     potential outputs (4, 5) in lane (1, l % 32), both of tile half l / 32.
+// kFarTiles tiles go through the network together; a lane adds its pairs in
+// list order either way, so the sums do not depend on kFarTiles.
+__global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(kFarMlpOccupancy, kFarMlpOccupancy)))
+void k_far_mlp(FarArgs a) {
+    constexpr int NT = kFarTiles;
     __shared__ FarInfer wfr;
-    __shared__ __attribute__((aligned(16))) __bf16 xs_all[kMlpWaves][kWave * 16];
-    __shared__ FarRow rows_all[kMlpWaves][kWave];
-    __shared__ int rlane_all[kMlpWaves][kWave];
-    __shared__ float sacc_all[kMlpWaves][kWave][3];     // fx, fy, PE of each observer lane
     {
         // stage the weights: every field is a whole number of dwords
         const FarFrags& g = *a.frags;
@@ -435,104 +481,121 @@ __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_
         copy(reinterpret_cast<uint32_t*>(wfr.w2), reinterpret_cast<const uint32_t*>(g.w2h), sizeof(g.w2h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.w3), reinterpret_cast<const uint32_t*>(g.w3h), sizeof(g.w3h) / 4);
         copy(reinterpret_cast<uint32_t*>(wfr.b2h), reinterpret_cast<const uint32_t*>(g.b2h), sizeof(g.b2h) / 4);
-        copy(reinterpret_cast<uint32_t*>(wfr.b3h), reinterpret_cast<const uint32_t*>(g.b3h), sizeof(g.b3h) / 4);
-        copy(reinterpret_cast<uint32_t*>(wfr.istd), reinterpret_cast<const uint32_t*>(g.istd), kFarIn);
-        if (threadIdx.x < kFarIn) wfr.nmi[threadIdx.x] = -g.mean[threadIdx.x] * g.istd[threadIdx.x];
-        copy(reinterpret_cast<uint32_t*>(wfr.oscale), reinterpret_cast<const uint32_t*>(g.oscale), kFarOut);
     }
     __syncthreads();
+    // wave-uniform constants, read once into scalar registers (the inputs are
+    // standardised as (f - mean) istd, one scalar operand per instruction)
+    float mean[kFarIn], istd[kFarIn], osc[kFarOut], b3[kFarOut];
+#pragma unroll
+    for (int k = 0; k < kFarIn; k++) {
+        mean[k] = a.frags->mean[k];
+        istd[k] = a.frags->istd[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kFarOut; k++) {
+        osc[k] = a.frags->oscale[k];
+        b3[k] = a.frags->b3[k];
+    }
     const int l = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
-    const int h = l >> 5, c = l & 31;
     const int64_t g = (int64_t)blockIdx.x * kMlpWaves + wv;
     const int n = a.ctrl->n_alive;
     if (g * kWave >= n) return;                  // no barrier below: a whole wave leaves
-    __bf16* xs = xs_all[wv];
-    FarRow* rows = rows_all[wv];
-    int* rlane = rlane_all[wv];
-    float (*sacc)[3] = sacc_all[wv];
     const FarInfer& fr = wfr;
-    sacc[l][0] = 0.f; sacc[l][1] = 0.f; sacc[l][2] = 0.f;
-    const int cnt = a.far_count[g];
-    const uint32_t* list = a.far_list + g * a.far_cap;
-    const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
-    // software pipeline: once tile k's features are in LDS, the gathers of
-    // tile k + 1 go into the same registers and stay in flight through tile
-    // k's network pass
-    FarIn cur = {};
-    if (l < cnt) far_gather(a, g, list[l], cur);
-    uint32_t e_next = kWave + l < cnt ? list[kWave + l] : 0u;     // tile 1's entries
-    for (int base = 0; base < cnt; base += kWave) {
-        // 1. features of row l
-        const bool valid = base + l < cnt;
-        float f[kFarIn];
-#pragma unroll
-        for (int k = 0; k < kFarIn; k++) f[k] = 0.f;
-        FarRow rw = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        int lane_of = 0;
-        if (valid) {
-            lane_of = cur.lane;
-            const float4 s4 = cur.s4;
-            float dx = cur.c4.x - s4.x, dy = cur.c4.y - s4.y;
-            if (a.periodic) { dx = torus_wrap(dx, a.dom_w, half_w); dy = torus_wrap(dy, a.dom_h, half_h); }
-            float sg0, sc0, sp0, cp0;
-            pair_features_fast(dx, dy, a.soft2, 2.f * cur.half, cur.c4.z, cur.mo, f, rw.ux, rw.uy, sg0, sc0, sp0,
-                               cp0);
-            rw.sg = a.grav ? s4.z * sg0 : 0.f;
-            rw.sc = a.coul ? s4.w * sc0 : 0.f;
-            rw.sp = a.grav ? s4.z * sp0 : 0.f;
-            rw.cp = a.coul ? s4.w * cp0 : 0.f;
-        }
-        // Standardised inputs without branches: a row past the list holds
-        // finite values whose outputs are discarded (MFMA output column n
-        // depends on input column n only).
-        float x[16];
-#pragma unroll
-        for (int k = 0; k < kFarIn; k++) x[k] = fmaf(f[k], fr.istd[k], fr.nmi[k]);
-        x[kFarIn] = valid ? 1.f : 0.f;
-#pragma unroll
-        for (int k = kFarIn + 1; k < 16; k++) x[k] = 0.f;
-        *reinterpret_cast<bf16x8*>(xs + l * 16) = pack8(x);
-        *reinterpret_cast<bf16x8*>(xs + l * 16 + 8) = pack8(x + 8);
-        rows[l] = rw;
-        rlane[l] = lane_of;
-        if (base + kWave + l < cnt) far_gather(a, g, e_next, cur);
-        e_next = base + 2 * kWave + l < cnt ? list[base + 2 * kWave + l] : 0u;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // 2. the network, then 3. rotate, scale and accumulate per observer
-        //    lane: the force from outputs 0..3 (lanes h = 0), the potential
-        //    from outputs 4, 5 (h = 1)
-        floatx16 o[2];
-        mlp_tile(fr, xs, l, o);
-#pragma unroll
-        for (int nb = 0; nb < 2; nb++) {
-            const int row = 32 * nb + c;
-            if (base + row < cnt) {
-                const FarRow q = rows[row];
-                if (h == 0) {
-                    const float gu = o[nb][0] * fr.oscale[0], gv = o[nb][1] * fr.oscale[1];
-                    const float cu = o[nb][2] * fr.oscale[2], cv = o[nb][3] * fr.oscale[3];
-                    const float fx = q.sg * (q.ux * gu - q.uy * gv) + q.sc * (q.ux * cu - q.uy * cv);
-                    const float fy = q.sg * (q.uy * gu + q.ux * gv) + q.sc * (q.uy * cu + q.ux * cv);
-                    atomicAdd(&sacc[rlane[row]][0], fx);
-                    atomicAdd(&sacc[rlane[row]][1], fy);
-                } else {
-                    const float pe = q.sp * o[nb][0] * fr.oscale[4] + q.cp * o[nb][1] * fr.oscale[5];
-                    atomicAdd(&sacc[rlane[row]][2], pe);
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
     const int i = (int)(g * kWave) + l;
+    const int cnt = i < n ? a.far_count[g * kWave + l] : 0;
+    int tiles = cnt;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tiles = max(tiles, __shfl_xor(tiles, off, kWave));
+    const float4 s4 = i < n ? a.src4[i] : make_float4(0.f, 0.f, 0.f, 0.f);   // observer x, y, m, q
+    const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
+    float sx = 0.f, sy = 0.f, sp = 0.f;
+    // software pipeline: the nodes of the next NT tiles are gathered into the
+    // same registers once this iteration's features are formed and stay in
+    // flight through its network pass; the list entries are read an iteration
+    // before that
+    FarNode cur[NT];
+    uint32_t e_next[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        cur[j] = {};
+        if (j < cnt) far_gather(a, a.far_list[far_slot(g, a.far_cap, l, j)], cur[j]);
+        e_next[j] = NT + j < cnt ? a.far_list[far_slot(g, a.far_cap, l, NT + j)] : 0u;
+    }
+    for (int t = 0; t < tiles; t += NT) {
+        // The loop stores nothing to LDS: without this compiler barrier the
+        // weight fragments (about 90 registers) would be hoisted out of it.
+        asm volatile("" ::: "memory");
+        bf16x8 bx[2 * NT];
+        float ux[NT], uy[NT], sg[NT], sc[NT], spg[NT], spc[NT];
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            // 1. features of this lane's pair in tile t + j
+            const int tt = t + j;
+            const bool valid = tt < cnt;
+            float f[kFarIn];
+#pragma unroll
+            for (int k = 0; k < kFarIn; k++) f[k] = 0.f;
+            ux[j] = uy[j] = sg[j] = sc[j] = spg[j] = spc[j] = 0.f;
+            if (valid) {
+                const FarNode& nd = cur[j];
+                float dx = nd.c4.x - s4.x, dy = nd.c4.y - s4.y;
+                if (a.periodic) { dx = torus_wrap(dx, a.dom_w, half_w); dy = torus_wrap(dy, a.dom_h, half_h); }
+                float sg0, sc0, sp0, cp0;
+                pair_features_fast(dx, dy, a.soft2, 2.f * nd.half, nd.c4.z, nd.mo, f, ux[j], uy[j], sg0, sc0, sp0,
+                                   cp0);
+                sg[j] = a.grav ? s4.z * sg0 : 0.f;
+                sc[j] = a.coul ? s4.w * sc0 : 0.f;
+                spg[j] = a.grav ? s4.z * sp0 : 0.f;
+                spc[j] = a.coul ? s4.w * cp0 : 0.f;
+            }
+            // Standardised inputs without branches: a row past the lane's
+            // list holds finite values whose outputs are discarded (MFMA
+            // output column n depends on input column n only).
+            float x[16];
+#pragma unroll
+            for (int k = 0; k < kFarIn; k++) x[k] = (f[k] - mean[k]) * istd[k];
+            x[kFarIn] = valid ? 1.f : 0.f;
+#pragma unroll
+            for (int k = kFarIn + 1; k < 16; k++) x[k] = 0.f;
+            if (tt + NT < cnt) far_gather(a, e_next[j], cur[j]);
+            e_next[j] = tt + 2 * NT < cnt ? a.far_list[far_slot(g, a.far_cap, l, tt + 2 * NT)] : 0u;
+            // 2. layer-1 operands: the rows' inputs 0-7 and 8-15 trade lane halves
+            u32x4 u0 = __builtin_bit_cast(u32x4, pack8(x)), u1 = __builtin_bit_cast(u32x4, pack8(x + 8));
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const auto r = __builtin_amdgcn_permlane32_swap(u0[k], u1[k], false, false);
+                u0[k] = r[0];
+                u1[k] = r[1];
+            }
+            bx[2 * j] = __builtin_bit_cast(bf16x8, u0);
+            bx[2 * j + 1] = __builtin_bit_cast(bf16x8, u1);
+        }
+        floatx16 o[2 * NT];
+        mlp_tile<2 * NT>(fr, bx, l, o);
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            // 3. this lane's row: force outputs from half 0, potential outputs
+            //    from half 1 of its own tile half, then rotate, scale and add
+            float q[4], p[2];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const auto w = __builtin_amdgcn_permlane32_swap(__float_as_uint(o[2 * j][r]),
+                                                                __float_as_uint(o[2 * j + 1][r]), false, false);
+                q[r] = __uint_as_float(w[0]) + b3[r];
+                if (r < 2) p[r] = __uint_as_float(w[1]) + b3[4 + r];
+            }
+            const float gu = q[0] * osc[0], gv = q[1] * osc[1];
+            const float cu = q[2] * osc[2], cv = q[3] * osc[3];
+            sx += sg[j] * (ux[j] * gu - uy[j] * gv) + sc[j] * (ux[j] * cu - uy[j] * cv);
+            sy += sg[j] * (uy[j] * gu + ux[j] * gv) + sc[j] * (uy[j] * cu + ux[j] * cv);
+            sp += spg[j] * p[0] * osc[4] + spc[j] * p[1] * osc[5];
+        }
+    }
     if (i < n) {
-        a.fx[i] += sacc[l][0];
-        a.fy[i] += sacc[l][1];
-        a.pe[i] += sacc[l][2];
+        a.fx[i] += sx;
+        a.fy[i] += sy;
+        a.pe[i] += sp;
     }
 }
 
@@ -554,9 +617,19 @@ __global__ __launch_bounds__(256) void k_far_samples(FarSampleArgs a) {
         if (a.row_off[mid] <= row) lo = mid; else hi = mid - 1;
     }
     const int64_t g = a.groups[lo];
-    const uint32_t e = a.far_list[g * a.far_cap + (row - a.row_off[lo])];
-    const int node = (int)(e & kFarNodeMask);
-    const int i = (int)(g * kWave) + (int)(e >> kFarLaneShift);
+    // the group's rows run through its lane queues in lane order
+    const int r = (int)(row - a.row_off[lo]);
+    const int c = a.far_count[g * kWave + l];
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int v = __shfl_up(incl, off, kWave);
+        if (l >= off) incl += v;
+    }
+    const int lane = (int)__builtin_ctzll(__ballot(r < incl));
+    const int k = r - __shfl(incl - c, lane, kWave);
+    const int node = (int)a.far_list[far_slot(g, a.far_cap, lane, k)];
+    const int i = (int)(g * kWave) + lane;
     const double xi = a.x[i], yi = a.y[i];
     const double cx = a.comx64[node], cy = a.comy64[node];
     const double M = a.mass64[node], Q = a.qq64[node] * kChargeQuantum;

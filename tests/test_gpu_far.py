@@ -95,6 +95,28 @@ def test_far_samples_match_oracle(oracle_lib):
     np.testing.assert_allclose(a[:, 16:], b[:, 16:], rtol=1e-4, atol=1e-9 * b[:, 16:].max())
 
 
+def test_far_list_overflow_is_reported():
+    # Each lane's queue holds far_list_cap / 64 nodes (whole 4-entry blocks):
+    # a cap of 256 leaves 4 per lane, far fewer than theta 0.5 accepts, and the
+    # call must say so rather than drop pairs.  The default cap then runs, and
+    # its samples match a second collection of the same state.
+    from physsim_amd.geon import GeonError
+    st, prm = W.plummer_charged(2048, 3)
+    g = _gpu(st, dict(prm, theta=0.5, far_list_cap=256))
+    with pytest.raises(GeonError) as ei:
+        g.far_collect(1)
+    assert ei.value.code == A.EOVERFLOW and "far_list_cap" in str(ei.value)
+    _set(g, far_list_cap=0)
+    g.far_clear()
+    n1 = g.far_collect(1)
+    S1, o1 = g.far_samples(observers=True)
+    g.far_clear()
+    assert g.far_collect(1) == n1 > 0
+    S2, o2 = g.far_samples(observers=True)
+    np.testing.assert_array_equal(o1, o2)
+    np.testing.assert_array_equal(S1, S2)
+
+
 def _wscale(S):
     w = S[:, 16:].ravel()
     w = w[w > 0]
