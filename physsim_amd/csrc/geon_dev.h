@@ -27,6 +27,11 @@ enum : uint32_t {
     F_COUL = 2u,        // Coulomb
     F_EXTRA = 4u,       // any of magnetic / GM / 1PN / Yukawa / axion / Higgs (runtime-checked)
     F_JERK = 8u,        // radiation on: the analytic jerk too; gravity and Coulomb runtime-checked
+    // Fixed instantiations for the common sets: the walk and its pair terms
+    // compile without the runtime branches (and the register copies they cost).
+    F_OPEN = 16u,       // open box: no minimum image, no window test
+    F_FIXED = 32u,      // the extra forces are the bits below (no axion, no Higgs), not runtime toggles
+    F_MAG = 64u, F_GM = 128u, F_YUK = 256u, F_1PN = 512u,
 };
 
 // Runtime toggles of the extra forces (uniform per launch).
@@ -42,6 +47,19 @@ struct PhysFlags {
     uint32_t topology;      // periodic: 0 torus, 1 Klein bottle, 2 RP^2 (config.js:22-26)
     uint32_t grav_on = 0, coul_on = 0;  // gravity / Coulomb, read by the F_JERK variants
 };
+
+// The extra-force toggles as a pair_force<F> instantiation sees them: compile-time
+// constants under F_FIXED, the launch's uniform PhysFlags otherwise.
+template <uint32_t F>
+struct Feat {
+    static constexpr bool fixed = (F & F_FIXED) != 0u;
+    bool mag, gm, one_pn, yuk, ax_mod, higgs;
+    __device__ __forceinline__ explicit Feat(const PhysFlags& pf)
+        : mag(fixed ? (F & F_MAG) != 0u : pf.mag != 0u), gm(fixed ? (F & F_GM) != 0u : pf.gm != 0u),
+          one_pn(fixed ? (F & F_1PN) != 0u : pf.one_pn != 0u), yuk(fixed ? (F & F_YUK) != 0u : pf.yuk != 0u),
+          ax_mod(fixed ? false : pf.ax_mod != 0u), higgs(fixed ? false : pf.higgs != 0u) {}
+};
+
 
 // Tree node record: 16 dwords.
 constexpr double kChargeQuantum = 0.1;   // charges are integer multiples (main.js:504-539)
@@ -172,6 +190,13 @@ __device__ __forceinline__ void min_image(const PhysFlags& pf, float ox, float o
                                           float& dy) {
     min_image_t<float>(pf.periodic, pf.topology, pf.dom_w, pf.dom_h, ox, oy, sx, sy, dx, dy);
 }
+// As min_image, with the open box (F_OPEN) known at compile time.
+template <uint32_t F>
+__device__ __forceinline__ void min_image_f(const PhysFlags& pf, float ox, float oy, float sx, float sy, float& dx,
+                                            float& dy) {
+    if constexpr ((F & F_OPEN) != 0u) { dx = sx - ox; dy = sy - oy; }
+    else min_image(pf, ox, oy, sx, sy, dx, dy);
+}
 
 // Far-field terms of an accepted node beyond its monopole (not in the
 // reference): with the softened kernel g(d) = d h(r2), h = r2^-3/2,
@@ -200,7 +225,7 @@ __device__ __forceinline__ void far_multipole(const Obs& o, Acc<KAHAN>& a, const
     }
     if (F & F_COUL) {
         float qi = o.q;
-        if ((F & F_EXTRA) && pf.ax_mod) qi *= sqrtf(o.ax);   // node sources carry axMod = 1
+        if ((F & F_EXTRA) && Feat<F>(pf).ax_mod) qi *= sqrtf(o.ax);   // node sources carry axMod = 1
         const float dd = c.dx * dx + c.dy * dy;
         const float qx = c.cxx * dx + c.cxy * dy, qy = c.cxy * dx + c.cyy * dy;
         const float dqd = dx * qx + dy * qy, tr = c.cxx + c.cyy;
@@ -229,6 +254,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                                            float svx, float svy, float s_ang_vel,
                                            float s_mu, float s_L, float s_ax, float s_yuk,
                                            float s_higgs, Split* sp = nullptr) {
+    const Feat<F> ft(pf);
     const float raw2 = rx * rx + ry * ry;
     const float r2 = raw2 + pf.soft2;
     const float inv_r = __builtin_amdgcn_rsqf(r2);   // 1/sqrt(r^2 + eps^2)
@@ -263,7 +289,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         if (PE) pe -= k * inv_r;
     }
     float ax_pair = 1.f;
-    if ((F & F_EXTRA) && pf.ax_mod) ax_pair = sqrtf(o.ax * s_ax);
+    if ((F & F_EXTRA) && ft.ax_mod) ax_pair = sqrtf(o.ax * s_ax);
     if (do_coul) {
         const float k = -(o.q * sq) * ax_pair;
         const float fd = k * inv_r3a;
@@ -276,11 +302,11 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         const float inv_r5 = inv_r3 * inv_r2;
         const float inv_r5a = ABERR ? inv_r5 * aberr : inv_r5;
         const float cross_sv = svx * ry - svy * rx;
-        if (pf.one_pn && (pf.gm || pf.mag)) {                       // forces.js:249-301
+        if (ft.one_pn && (ft.gm || ft.mag)) {                       // forces.js:249-301
             const float r = r2 * inv_r;             // sqrt(r2) without a division
             const float nx = rx * inv_r, ny = ry * inv_r;
             const float v1v2 = o.vx * svx + o.vy * svy;
-            if (pf.gm) {
+            if (ft.gm) {
                 const float v1sq = o.vx * o.vx + o.vy * o.vy;
                 const float v2sq = svx * svx + svy * svy;
                 const float nv1 = nx * o.vx + ny * o.vy;
@@ -299,7 +325,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 }
                 if (PE) pe -= o.m * sm * inv_r * (1.5f * (v1sq + v2sq) - 3.5f * v1v2 - 0.5f * nv1 * nv2 + o.m * inv_r + sm * inv_r);
             }
-            if (pf.mag) {
+            if (ft.mag) {
                 const float v2n = svx * nx + svy * ny, v1n = o.vx * nx + o.vy * ny;
                 const float coeff = 0.5f * o.q * sq * inv_r2;
                 const float fx = coeff * (o.vx * v2n - 3.f * nx * v1n * v2n);
@@ -309,7 +335,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 if (PE) pe -= 0.5f * o.q * sq * inv_r * (v1v2 + v1n * v2n);
             }
         }
-        if (pf.one_pn && pf.gm && pf.mag) {                         // forces.js:303-320
+        if (ft.one_pn && ft.gm && ft.mag) {                         // forces.js:303-320
             const float cc = o.q * sq * (o.m + sm) - (o.q * o.q * sm + sq * sq * o.m);
             const float fd = cc * inv_r2 * inv_r2;
             fr += fd; a.f1x += rx * fd; a.f1y += ry * fd;
@@ -317,7 +343,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             if (F & F_JERK) a.addj(fd, -4.f * cc * rdv * inv_r2 * inv_r2 * inv_r2, vrx, vry, rx, ry);
             if (PE) pe += 0.5f * cc * inv_r2;
         }
-        if (pf.mag) {                                               // forces.js:322-353
+        if (ft.mag) {                                               // forces.js:322-353
             const float fd = -3.f * (o.mu * s_mu) * inv_r5a * ax_pair;
             fr += fd;
             if constexpr (SPLIT) sp->add(S_MAG, rx * fd, ry * fd);
@@ -331,7 +357,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             a.dbzdx -= 3.f * s_mu * rx * inv_r5 * ax_pair;
             a.dbzdy -= 3.f * s_mu * ry * inv_r5 * ax_pair;
         }
-        if (pf.gm) {                                                // forces.js:355-387
+        if (ft.gm) {                                                // forces.js:355-387
             const float fd = 3.f * (o.L * s_L) * inv_r5a;
             fr += fd;
             if constexpr (SPLIT) sp->add(S_GM, rx * fd, ry * fd);
@@ -346,11 +372,11 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
             a.dbgzdy -= 6.f * s_L * ry * inv_r5;
             a.frame_drag += 2.f * s_L * (s_ang_vel - o.ang_vel) * inv_r3;
         }
-        if (pf.yuk && raw2 < pf.yuk_cut2) {                         // forces.js:389-427
-            const float mu = pf.higgs ? pf.yuk_mu * sqrtf(o.higgs * s_higgs) : pf.yuk_mu;
+        if (ft.yuk && raw2 < pf.yuk_cut2) {                         // forces.js:389-427
+            const float mu = ft.higgs ? pf.yuk_mu * sqrtf(o.higgs * s_higgs) : pf.yuk_mu;
             const float r = r2 * inv_r;             // sqrt(r2) without a division
             const float e = __expf(-mu * r);
-            const float ym = pf.ax_mod ? sqrtf(o.yuk * s_yuk) : 1.f;
+            const float ym = ft.ax_mod ? sqrtf(o.yuk * s_yuk) : 1.f;
             const float gmm = pf.yuk_g * ym * o.m * sm * e;
             const float fd = gmm * (inv_r2 + mu * inv_r) * (ABERR ? inv_r * aberr : inv_r);
             fr += fd;
@@ -359,7 +385,7 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
                 a.addj(fd, -(3.f * inv_r2 + 3.f * mu * inv_r + mu * mu) * rdv * gmm * inv_r2 *
                                (ABERR ? inv_r * aberr : inv_r), vrx, vry, rx, ry);
             if (PE) pe -= gmm * inv_r;
-            if (pf.one_pn) {
+            if (ft.one_pn) {
                 const float nx = rx * inv_r, ny = ry * inv_r;
                 const float nv1 = nx * o.vx + ny * o.vy, nv2 = nx * svx + ny * svy;
                 const float v1v2 = o.vx * svx + o.vy * svy;

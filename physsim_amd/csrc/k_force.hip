@@ -182,7 +182,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
                 } else if (mine && (a.src_orig ? a.src_orig[j] : j) != i) {
                     // with periodic images in the tree: not the observer nor one of its images
                     float rx, ry;
-                    min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
+                    min_image_f<F>(pf, o.x, o.y, s4.x, s4.y, rx, ry);
                     pair_force<F, KAHAN, false, true, SEL>(o, acc, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8a.z,
                                                            s8a.w, s8b.x, s8b.y, s8b.z, s8b.w, split);
                     c_pair++;
@@ -191,10 +191,10 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
         } else if (nd.count < 0) {
             const float rawx = nd.comx - o.x, rawy = nd.comy - o.y;
             float dx, dy;
-            min_image(pf, o.x, o.y, nd.comx, nd.comy, dx, dy);
+            min_image_f<F>(pf, o.x, o.y, nd.comx, nd.comy, dx, dy);
             const float d2 = dx * dx + dy * dy;
             bool accept = mine && (nd.size2 < pf.theta2 * d2);   // forces.js:680
-            if (pf.window) {
+            if (!(F & F_OPEN) && pf.window) {
                 // the node's whole box must map through the COM's image shift into the window
                 const float cx = nd.bx - o.x + (dx - rawx), cy = nd.by - o.y + (dy - rawy);
                 accept = accept && (fabsf(cx) + nd.half <= half_w) && (fabsf(cy) + nd.half * (pf.dom_h / pf.dom_w) <= half_h);
@@ -791,12 +791,53 @@ __global__ __launch_bounds__(kWave) void k_sel_forces(ForceArgs a, int residual)
         }                                                                                             \
     } while (0)
 
+// The fixed instantiation that runs this launch's feature set in an open box,
+// or 0 when none does and the runtime-checked kernel runs:
+//   kFixG     gravity;
+//   kFixGC    gravity + Coulomb;
+//   kFixGCMY  gravity + Coulomb + magnetic + Yukawa, no axion modulation or
+//             Higgs-scaled Yukawa range (the N = 65536 configuration's set);
+//   kFixGP    gravity + gravitomagnetism + 1PN, no Yukawa or axion-modulated
+//             charges (the Higgs field only scales the Yukawa range, so it may
+//             be on): the plain walk only.
+constexpr uint32_t kFixG = F_GRAV | F_OPEN;
+constexpr uint32_t kFixGC = F_GRAV | F_COUL | F_OPEN;
+constexpr uint32_t kFixGCMY = F_GRAV | F_COUL | F_EXTRA | F_OPEN | F_FIXED | F_MAG | F_YUK;
+constexpr uint32_t kFixGP = F_GRAV | F_EXTRA | F_OPEN | F_FIXED | F_GM | F_1PN;
+static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
+    if (!kahan || pf.periodic || (feat & F_JERK)) return 0;
+    switch (feat & 7u) {
+        case F_GRAV: return kFixG;
+        case F_GRAV | F_COUL: return kFixGC;
+        case F_GRAV | F_COUL | F_EXTRA:
+            return pf.mag && pf.yuk && !pf.gm && !pf.one_pn && !pf.ax_mod && !pf.higgs ? kFixGCMY : 0;
+        case F_GRAV | F_EXTRA:
+            return pf.gm && pf.one_pn && !pf.mag && !pf.yuk && !pf.ax_mod ? kFixGP : 0;
+        default: return 0;
+    }
+}
+#define GEON_FIXED_LAUNCH(KERNEL, fix, grid, stream, args)                                                  \
+    do {                                                                                                \
+        const dim3 g_(grid), b_(kWalkThreads);                                                          \
+        if ((fix) == kFixG) hipLaunchKernelGGL((KERNEL<kFixG, true>), g_, b_, 0, stream, args);           \
+        else if ((fix) == kFixGC) hipLaunchKernelGGL((KERNEL<kFixGC, true>), g_, b_, 0, stream, args);    \
+        else hipLaunchKernelGGL((KERNEL<kFixGCMY, true>), g_, b_, 0, stream, args);                        \
+    } while (0)
+
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
+    const uint32_t fix = a.hist.on ? 0u : fixed_feat(a.pf, feat, kahan);
     if (a.hist.on) GEON_FEAT_SWITCH_R(k_walk_delay, feat, kahan, grid, s, a);
-    else if (a.far_mode == kFarMultipole) GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
-    else if (a.far_mode == kFarList) GEON_FEAT_SWITCH(k_walk_list, feat, kahan, grid, s, a);
+    else if (a.far_mode == kFarMultipole) {
+        if (fix && fix != kFixGP) GEON_FIXED_LAUNCH(k_walk_multipole, fix, grid, s, a);
+        else GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
+    } else if (a.far_mode == kFarList) {
+        if (fix && fix != kFixGP) GEON_FIXED_LAUNCH(k_walk_list, fix, grid, s, a);
+        else GEON_FEAT_SWITCH(k_walk_list, feat, kahan, grid, s, a);
+    } else if (fix == kFixGP) {
+        hipLaunchKernelGGL((k_walk<kFixGP, true>), dim3(grid), dim3(kWalkThreads), 0, s, a);
+    } else if (fix) GEON_FIXED_LAUNCH(k_walk, fix, grid, s, a);
     else GEON_FEAT_SWITCH_R(k_walk, feat, kahan, grid, s, a);
 }
 
@@ -818,7 +859,13 @@ void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
 void launch_walk_persistent(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan, int* queue,
                             int64_t n_groups, int grid, int threads, int stack) {
     if (n_groups <= 0) return;
-    if (stack <= 64) GEON_FEAT_SWITCH_P(k_walk_persistent64, feat, kahan, grid, threads, s, a, queue, n_groups);
+    const uint32_t fix = fixed_feat(a.pf, feat, kahan);
+    const dim3 g_(grid), b_(threads);
+    if (fix == kFixGC && stack <= 64)
+        hipLaunchKernelGGL((k_walk_persistent64<kFixGC, true>), g_, b_, 0, s, a, queue, n_groups);
+    else if (fix == kFixGC)
+        hipLaunchKernelGGL((k_walk_persistent96<kFixGC, true>), g_, b_, 0, s, a, queue, n_groups);
+    else if (stack <= 64) GEON_FEAT_SWITCH_P(k_walk_persistent64, feat, kahan, grid, threads, s, a, queue, n_groups);
     else GEON_FEAT_SWITCH_P(k_walk_persistent96, feat, kahan, grid, threads, s, a, queue, n_groups);
 }
 
