@@ -130,36 +130,6 @@ __device__ __forceinline__ void pair_features(T dx, T dy, T eps2, T s, T M, cons
     cp0 = A * sr * ir;
 }
 
-// The same features for the inference pass, in FP32 with the node's scales
-// 1/(M s^2), 1/(A s), 1/(A s^2) precomputed at the tree build.
-__device__ __forceinline__ void pair_features_fast(float dx, float dy, float eps2, float s, float M,
-                                                   const NodeMom& c, float* f, float& ux, float& uy, float& sg0,
-                                                   float& sc0, float& sp0, float& cp0) {
-    const float rho2 = dx * dx + dy * dy;
-    const float r2 = rho2 + eps2;
-    const float ir = rsq_t(r2);
-    const float ir2 = ir * ir;
-    const float irho = rsq_t(rho2);
-    ux = dx * irho; uy = dy * irho;
-    const float sr = s * ir;
-    const float uu = ux * ux, vv = uy * uy, uv = ux * uy;
-    f[0] = sr;
-    f[1] = eps2 * ir2;
-    f[2] = (uu * c.qxx + 2.f * uv * c.qxy + vv * c.qyy) * c.ims2;
-    f[3] = (-uv * c.qxx + (uu - vv) * c.qxy + uv * c.qyy) * c.ims2;
-    f[4] = (vv * c.qxx - 2.f * uv * c.qxy + uu * c.qyy) * c.ims2;
-    f[5] = (ux * c.dx + uy * c.dy) * c.ias;
-    f[6] = (-uy * c.dx + ux * c.dy) * c.ias;
-    f[7] = (uu * c.cxx + 2.f * uv * c.cxy + vv * c.cyy) * c.ias2;
-    f[8] = (-uv * c.cxx + (uu - vv) * c.cxy + uv * c.cyy) * c.ias2;
-    f[9] = (vv * c.cxx - 2.f * uv * c.cxy + uu * c.cyy) * c.ias2;
-    const float A = c.aq;
-    sg0 = M * sr * sr * ir2;
-    sc0 = A * sr * ir2;
-    sp0 = M * sr * sr * ir;
-    cp0 = A * sr * ir;
-}
-
 // ---------------------------------------------------------------------------
 // Model fragments: the network's weights arranged per lane for the MFMAs
 // ---------------------------------------------------------------------------
@@ -356,8 +326,8 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // M-block); the six layer-3 biases are cheaper to add to the outputs.
 // Weight fragments are read from LDS one stage before their products.
 #define GEON_STAGE() __builtin_amdgcn_sched_barrier(0)
-template <int NB>
-__device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, int l, floatx16* o) {
+template <int NB, typename Hook>
+__device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, int l, floatx16* o, Hook&& hook) {
     static_assert(NB % 2 == 0, "the M-block activations are split over two stages");
     const int h = l >> 5;
     const floatx16 zero = {};
@@ -383,6 +353,7 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, i
             const int p = k - 1;
             act16(z[p & 1], a1[2 * (p / NB)][p % NB], a1[2 * (p / NB) + 1][p % NB]);
         }
+        hook(k);
         GEON_STAGE();
     }
     // S4: layer-2 M-block 0, bias and K-steps 0-1; the last layer-1 activation
@@ -396,6 +367,7 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, i
         const int p = 2 * NB - 1;
         act16(z[p & 1], a1[2][NB - 1], a1[3][NB - 1]);
     }
+    hook(4);
     GEON_STAGE();
     // S5: layer-2 M-block 0, K-steps 2-3
     halfx8 wb1 = fr.b2h[1][l];
@@ -405,6 +377,7 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, i
     for (int nb = 0; nb < NB; nb++) acc0[nb] = mfma32(w2, a1[2][nb], acc0[nb]);
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) acc0[nb] = mfma32(w3, a1[3][nb], acc0[nb]);
+    hook(5);
     GEON_STAGE();
     // S6: layer-2 M-block 1, bias and K-steps 0-1; half of M-block 0's activations
     w2 = fr.w2[1][2][l];
@@ -418,6 +391,7 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, i
 #pragma unroll
         for (int nb = 0; nb < NB / 2; nb++) act16(acc0[nb], a2[0][nb], a2[1][nb]);
     }
+    hook(6);
     GEON_STAGE();
     // S7: layer-2 M-block 1, K-steps 2-3; the rest of M-block 0's activations
     w0 = fr.w3[0][l];
@@ -428,6 +402,7 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, i
     for (int nb = 0; nb < NB; nb++) acc1[nb] = mfma32(w3, a1[3][nb], acc1[nb]);
 #pragma unroll
     for (int nb = NB / 2; nb < NB; nb++) act16(acc0[nb], a2[0][nb], a2[1][nb]);
+    hook(7);
     GEON_STAGE();
     // S8: layer 3, K-steps 0-1 (M-block 0); M-block 1's activations.  The
     //     caller adds b3 once the outputs sit in their rows' lanes.
@@ -441,35 +416,107 @@ __device__ __forceinline__ void mlp_tile(const FarInfer& fr, const bf16x8* bx, i
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) act16(acc1[nb], a2[2][nb], a2[3][nb]);
     }
+    hook(8);
     GEON_STAGE();
     // S9: layer 3, K-steps 2-3
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) o[nb] = mfma32(w2, a2[2][nb], o[nb]);
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) o[nb] = mfma32(w3, a2[3][nb], o[nb]);
+    hook(9);
 }
 #undef GEON_STAGE
 
-// Tiles per iteration of k_far_mlp and waves per SIMD.  One tile with the
-// staged network fits 160 registers (three waves per SIMD); two tiles per
-// iteration (four MFMA chains per layer) need more than 256 and spill.
-constexpr int kFarTiles = 1;
+// Waves per SIMD of k_far_mlp: the staged network fits 170 registers.
 constexpr int kFarMlpOccupancy = 3;
+
+// The features of one listed pair, formed in parts so that the next tile's
+// pair can be worked on inside this tile's network stages (pair_features in
+// FP32, with the node's scales 1/(M s^2), 1/(A s), 1/(A s^2) precomputed at
+// the tree build).  A row past the lane's list
+// takes the offset (1, 0) and whatever node record the registers hold, all
+// finite, and zero scales: its outputs add nothing.
+struct PairFeat {
+    float f[kFarIn];
+    float ux, uy, uu, vv, uv;
+    float sg, sc, spg, spc;              // force and potential scales, times m_i and q_i (0 past the list)
+};
+
+__device__ __forceinline__ void feat_geometry(const FarArgs& a, const FarNode& nd, const float4& s4, bool valid,
+                                              PairFeat& P) {
+    float dx = valid ? nd.c4.x - s4.x : 1.f, dy = valid ? nd.c4.y - s4.y : 0.f;
+    if (a.periodic) {
+        dx = torus_wrap(dx, a.dom_w, 0.5f * a.dom_w);
+        dy = torus_wrap(dy, a.dom_h, 0.5f * a.dom_h);
+    }
+    const float rho2 = dx * dx + dy * dy;
+    const float r2 = rho2 + a.soft2;
+    const float ir = rsq_t(r2);
+    const float ir2 = ir * ir;
+    const float irho = rsq_t(rho2);
+    P.ux = dx * irho; P.uy = dy * irho;
+    const float sr = 2.f * nd.half * ir;
+    P.uu = P.ux * P.ux; P.vv = P.uy * P.uy; P.uv = P.ux * P.uy;
+    P.f[0] = sr;
+    P.f[1] = a.soft2 * ir2;
+    const float M = nd.c4.z, A = nd.mo.aq;
+    const float mi = valid && a.grav ? s4.z : 0.f, qi = valid && a.coul ? s4.w : 0.f;
+    P.sg = mi * (M * sr * sr * ir2);
+    P.sc = qi * (A * sr * ir2);
+    P.spg = mi * (M * sr * sr * ir);
+    P.spc = qi * (A * sr * ir);
+    if (!valid) P.ux = P.uy = 0.f;       // after the products above: the moments below stay finite
+}
+
+__device__ __forceinline__ void feat_moments(const NodeMom& c, PairFeat& P) {
+    const float uu = P.uu, vv = P.vv, uv = P.uv, ux = P.ux, uy = P.uy;
+    P.f[2] = (uu * c.qxx + 2.f * uv * c.qxy + vv * c.qyy) * c.ims2;
+    P.f[3] = (-uv * c.qxx + (uu - vv) * c.qxy + uv * c.qyy) * c.ims2;
+    P.f[4] = (vv * c.qxx - 2.f * uv * c.qxy + uu * c.qyy) * c.ims2;
+    P.f[5] = (ux * c.dx + uy * c.dy) * c.ias;
+    P.f[6] = (-uy * c.dx + ux * c.dy) * c.ias;
+    P.f[7] = (uu * c.cxx + 2.f * uv * c.cxy + vv * c.cyy) * c.ias2;
+    P.f[8] = (-uv * c.cxx + (uu - vv) * c.cxy + uv * c.cyy) * c.ias2;
+    P.f[9] = (vv * c.cxx - 2.f * uv * c.cxy + uu * c.cyy) * c.ias2;
+}
+
+// Standardised inputs (f - mean) istd, the bias input, then the layer-1 B
+// operands: lane (h, c) needs inputs 8h..8h+7 of rows c and 32 + c, i.e. its
+// own half h and the half h of its partner lane l ^ 32.
+__device__ __forceinline__ void feat_operands(const PairFeat& P, bool valid, const float* mean, const float* istd,
+                                              bf16x8* bx) {
+    float x[16];
+#pragma unroll
+    for (int k = 0; k < kFarIn; k++) x[k] = (P.f[k] - mean[k]) * istd[k];
+    x[kFarIn] = valid ? 1.f : 0.f;
+#pragma unroll
+    for (int k = kFarIn + 1; k < 16; k++) x[k] = 0.f;
+    u32x4 u0 = __builtin_bit_cast(u32x4, pack8(x)), u1 = __builtin_bit_cast(u32x4, pack8(x + 8));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const auto r = __builtin_amdgcn_permlane32_swap(u0[k], u1[k], false, false);
+        u0[k] = r[0];
+        u1[k] = r[1];
+    }
+    bx[0] = __builtin_bit_cast(bf16x8, u0);
+    bx[1] = __builtin_bit_cast(bf16x8, u1);
+}
 
 // One wave per 64-particle group, lane l for particle 64 g + l: tile t holds
 // every lane's t-th listed node (far_slot), row l of the tile is lane l's
 // pair, so the predicted residual of a row lands in the lane that owns its
 // observer and is summed there, in list order, without a cross-lane scatter.
-// The inputs and outputs change lane halves with v_permlane32_swap:
-//   inputs: lane (h, c) needs inputs 8h..8h+7 of rows c and 32 + c, i.e. its
-//     own half h and the half h of its partner lane l ^ 32;
-//   outputs: row l's force outputs (0..3) are in lane (0, l % 32), its
-//     potential outputs (4, 5) in lane (1, l % 32), both of tile half l / 32.
-// kFarTiles tiles go through the network together; a lane adds its pairs in
-// list order either way, so the sums do not depend on kFarTiles.
+// The outputs change lane halves with v_permlane32_swap: row l's force
+// outputs (0..3) are in lane (0, l % 32), its potential outputs (4, 5) in
+// lane (1, l % 32), both of tile half l / 32.
+//
+// Software pipeline over the tiles: tile t + 1's features are formed inside
+// tile t's network stages that have room for vector work (geometry in S4,
+// moments and operands in S5), tile t + 2's node is gathered in S5 once
+// tile t + 1's features no longer need the registers, and its list entry is
+// read a tile earlier still.
 __global__ __launch_bounds__(kMlpWaves * kWave) __attribute__((amdgpu_waves_per_eu(kFarMlpOccupancy, kFarMlpOccupancy)))
 void k_far_mlp(FarArgs a) {
-    constexpr int NT = kFarTiles;
     __shared__ FarInfer wfr;
     {
         // stage the weights: every field is a whole number of dwords
@@ -508,89 +555,53 @@ void k_far_mlp(FarArgs a) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) tiles = max(tiles, __shfl_xor(tiles, off, kWave));
     const float4 s4 = i < n ? a.src4[i] : make_float4(0.f, 0.f, 0.f, 0.f);   // observer x, y, m, q
-    const float half_w = 0.5f * a.dom_w, half_h = 0.5f * a.dom_h;
+    auto entry = [&](int t) { return t < cnt ? a.far_list[far_slot(g, a.far_cap, l, t)] : 0u; };
     float sx = 0.f, sy = 0.f, sp = 0.f;
-    // software pipeline: the nodes of the next NT tiles are gathered into the
-    // same registers once this iteration's features are formed and stay in
-    // flight through its network pass; the list entries are read an iteration
-    // before that
-    FarNode cur[NT];
-    uint32_t e_next[NT];
-#pragma unroll
-    for (int j = 0; j < NT; j++) {
-        cur[j] = {};
-        if (j < cnt) far_gather(a, a.far_list[far_slot(g, a.far_cap, l, j)], cur[j]);
-        e_next[j] = NT + j < cnt ? a.far_list[far_slot(g, a.far_cap, l, NT + j)] : 0u;
-    }
-    for (int t = 0; t < tiles; t += NT) {
+    // prologue: tile 0's operands, tile 1's node, tile 2's list entry
+    FarNode cur = {};
+    PairFeat P;
+    bf16x8 bx[2];
+    if (0 < cnt) far_gather(a, entry(0), cur);
+    uint32_t e_next = entry(1);
+    feat_geometry(a, cur, s4, 0 < cnt, P);
+    feat_moments(cur.mo, P);
+    feat_operands(P, 0 < cnt, mean, istd, bx);
+    if (1 < cnt) far_gather(a, e_next, cur);
+    e_next = entry(2);
+    for (int t = 0; t < tiles; t++) {
         // The loop stores nothing to LDS: without this compiler barrier the
         // weight fragments (about 90 registers) would be hoisted out of it.
         asm volatile("" ::: "memory");
-        bf16x8 bx[2 * NT];
-        float ux[NT], uy[NT], sg[NT], sc[NT], spg[NT], spc[NT];
-#pragma unroll
-        for (int j = 0; j < NT; j++) {
-            // 1. features of this lane's pair in tile t + j
-            const int tt = t + j;
-            const bool valid = tt < cnt;
-            float f[kFarIn];
-#pragma unroll
-            for (int k = 0; k < kFarIn; k++) f[k] = 0.f;
-            ux[j] = uy[j] = sg[j] = sc[j] = spg[j] = spc[j] = 0.f;
-            if (valid) {
-                const FarNode& nd = cur[j];
-                float dx = nd.c4.x - s4.x, dy = nd.c4.y - s4.y;
-                if (a.periodic) { dx = torus_wrap(dx, a.dom_w, half_w); dy = torus_wrap(dy, a.dom_h, half_h); }
-                float sg0, sc0, sp0, cp0;
-                pair_features_fast(dx, dy, a.soft2, 2.f * nd.half, nd.c4.z, nd.mo, f, ux[j], uy[j], sg0, sc0, sp0,
-                                   cp0);
-                sg[j] = a.grav ? s4.z * sg0 : 0.f;
-                sc[j] = a.coul ? s4.w * sc0 : 0.f;
-                spg[j] = a.grav ? s4.z * sp0 : 0.f;
-                spc[j] = a.coul ? s4.w * cp0 : 0.f;
+        const float ux = P.ux, uy = P.uy, sg = P.sg, sc = P.sc, spg = P.spg, spc = P.spc;
+        const bool vn = t + 1 < cnt;
+        bf16x8 bn[2];
+        floatx16 o[2];
+        mlp_tile<2>(fr, bx, l, o, [&](int stage) {
+            if (stage == 4) feat_geometry(a, cur, s4, vn, P);
+            if (stage == 5) {
+                feat_moments(cur.mo, P);
+                feat_operands(P, vn, mean, istd, bn);
+                if (t + 2 < cnt) far_gather(a, e_next, cur);
+                e_next = entry(t + 3);
             }
-            // Standardised inputs without branches: a row past the lane's
-            // list holds finite values whose outputs are discarded (MFMA
-            // output column n depends on input column n only).
-            float x[16];
+        });
+        // this lane's row: force outputs from half 0, potential outputs from
+        // half 1 of its own tile half, then rotate, scale and add
+        float q[4], p[2];
 #pragma unroll
-            for (int k = 0; k < kFarIn; k++) x[k] = (f[k] - mean[k]) * istd[k];
-            x[kFarIn] = valid ? 1.f : 0.f;
-#pragma unroll
-            for (int k = kFarIn + 1; k < 16; k++) x[k] = 0.f;
-            if (tt + NT < cnt) far_gather(a, e_next[j], cur[j]);
-            e_next[j] = tt + 2 * NT < cnt ? a.far_list[far_slot(g, a.far_cap, l, tt + 2 * NT)] : 0u;
-            // 2. layer-1 operands: the rows' inputs 0-7 and 8-15 trade lane halves
-            u32x4 u0 = __builtin_bit_cast(u32x4, pack8(x)), u1 = __builtin_bit_cast(u32x4, pack8(x + 8));
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const auto r = __builtin_amdgcn_permlane32_swap(u0[k], u1[k], false, false);
-                u0[k] = r[0];
-                u1[k] = r[1];
-            }
-            bx[2 * j] = __builtin_bit_cast(bf16x8, u0);
-            bx[2 * j + 1] = __builtin_bit_cast(bf16x8, u1);
+        for (int r = 0; r < 4; r++) {
+            const auto w = __builtin_amdgcn_permlane32_swap(__float_as_uint(o[0][r]), __float_as_uint(o[1][r]), false,
+                                                            false);
+            q[r] = __uint_as_float(w[0]) + b3[r];
+            if (r < 2) p[r] = __uint_as_float(w[1]) + b3[4 + r];
         }
-        floatx16 o[2 * NT];
-        mlp_tile<2 * NT>(fr, bx, l, o);
-#pragma unroll
-        for (int j = 0; j < NT; j++) {
-            // 3. this lane's row: force outputs from half 0, potential outputs
-            //    from half 1 of its own tile half, then rotate, scale and add
-            float q[4], p[2];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const auto w = __builtin_amdgcn_permlane32_swap(__float_as_uint(o[2 * j][r]),
-                                                                __float_as_uint(o[2 * j + 1][r]), false, false);
-                q[r] = __uint_as_float(w[0]) + b3[r];
-                if (r < 2) p[r] = __uint_as_float(w[1]) + b3[4 + r];
-            }
-            const float gu = q[0] * osc[0], gv = q[1] * osc[1];
-            const float cu = q[2] * osc[2], cv = q[3] * osc[3];
-            sx += sg[j] * (ux[j] * gu - uy[j] * gv) + sc[j] * (ux[j] * cu - uy[j] * cv);
-            sy += sg[j] * (uy[j] * gu + ux[j] * gv) + sc[j] * (uy[j] * cu + ux[j] * cv);
-            sp += spg[j] * p[0] * osc[4] + spc[j] * p[1] * osc[5];
-        }
+        const float gu = q[0] * osc[0], gv = q[1] * osc[1];
+        const float cu = q[2] * osc[2], cv = q[3] * osc[3];
+        sx += sg * (ux * gu - uy * gv) + sc * (ux * cu - uy * cv);
+        sy += sg * (uy * gu + ux * gv) + sc * (uy * cu + ux * cv);
+        sp += spg * p[0] * osc[4] + spc * p[1] * osc[5];
+        bx[0] = bn[0];
+        bx[1] = bn[1];
     }
     if (i < n) {
         a.fx[i] += sx;
