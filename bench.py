@@ -39,12 +39,14 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 FP32_PEAK_TFLOPS = 157.3
+# without packed-FP32 instructions (the walks are built without them): one FMA per lane per cycle
+FP32_NONPACKED_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 # network flops per (particle, node) pair: 2 (11 x 64 + 64 x 64 + 64 x 6)
 MLP_FLOPS_PER_PAIR = 2 * (11 * 64 + 64 * 64 + 64 * 6)
 # HBM bytes of one k_far_mlp launch at the default workload (N=2^20, theta 0.30), from the
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (profiles/k_far_mlp_open1M_theta0.30.md)
-MLP_TRAFFIC_DEFAULT = 3003.0e6
+MLP_TRAFFIC_DEFAULT = 2696.0e6
 TIMER_WALK, TIMER_TREE, TIMER_SORT, TIMER_KICK, TIMER_MLP = 0, 1, 2, 3, 4
 TIMER_DEPOSIT, TIMER_KDK, TIMER_FIELD_PART = 5, 6, 7
 HBM_PEAK_GBS = 8000.0
@@ -599,16 +601,24 @@ def main():
                 "traffic": MLP_TRAFFIC_DEFAULT if default else None,
                 "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch (copy-calibrated), "
                                   "profiles/k_far_mlp_open1M_theta0.30.md" if default else None,
-                "note": "the vector ALU (features, hard-swish, output rotation) bounds this kernel, not MFMA",
+                "note": "vector work (the hard-swish of 128 hidden values per lane, features, output rotation) and the MFMAs "
+                        "barely overlap on a SIMD; their sum, not MFMA throughput, bounds this kernel",
                 "pairs_per_launch": L["node_interactions_per_particle"] * n, "flops_per_pair": MLP_FLOPS_PER_PAIR,
                 "ms_avg": L["phases_ms_avg"]["mlp"]}
     else:
         roof = {"bound": "valu", "kernel": "k_walk", "achieved": L["walk_tflops"], "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": L["walk_tflops"] / FP32_PEAK_TFLOPS, "traffic": None,
                 "ms_avg": L["phases_ms_avg"]["walk"]}
-    walk_roof = {"bound": "valu", "kernel": "k_walk (learned run)", "achieved": L["walk_tflops"],
+    walk_roof = {"bound": "valu", "kernel": "k_walk_list (learned run)", "achieved": L["walk_tflops"],
                  "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": L["walk_tflops"] / FP32_PEAK_TFLOPS,
+                 "frac_nonpacked": L["walk_tflops"] / FP32_NONPACKED_TFLOPS,
                  "flops_per_interaction": c_f, "ms_avg": L["phases_ms_avg"]["walk"]}
+    walk_roof_plain = None
+    if P is not None and "walk_tflops" in P:
+        walk_roof_plain = {"bound": "valu", "kernel": "k_walk (plain run)", "achieved": P["walk_tflops"],
+                           "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": P["walk_tflops"] / FP32_PEAK_TFLOPS,
+                           "frac_nonpacked": P["walk_tflops"] / FP32_NONPACKED_TFLOPS,
+                           "flops_per_interaction": c_f, "ms_avg": P["phases_ms_avg"]["walk"]}
 
     batch = run_batch(args.batch_systems, args.steps) if args.batch_systems > 0 else None
     drift = run_drift(args.drift_n, args.drift_steps, L["theta"], model) if args.drift_steps > 0 else None
@@ -670,6 +680,7 @@ def main():
         "training": train_info,
         "roofline": roof,
         "roofline_walk": walk_roof,
+        "roofline_walk_plain": walk_roof_plain,
         "batch": batch,
         "energy_drift": drift,
         "legs": legs,

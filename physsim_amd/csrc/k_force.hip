@@ -797,12 +797,15 @@ __global__ __launch_bounds__(kWave) void k_sel_forces(ForceArgs a, int residual)
 //   kFixGC    gravity + Coulomb;
 //   kFixGCMY  gravity + Coulomb + magnetic + Yukawa, no axion modulation or
 //             Higgs-scaled Yukawa range (the N = 65536 configuration's set);
+//   kFixGCM   gravity + Coulomb + magnetic, no axion modulation;
 //   kFixGP    gravity + gravitomagnetism + 1PN, no Yukawa or axion-modulated
-//             charges (the Higgs field only scales the Yukawa range, so it may
-//             be on): the plain walk only.
+//             charges: the plain walk only.
+// (The Higgs field only scales the Yukawa range, so without Yukawa it may be
+// on.)  The signal-delay walk has the gravity + Coulomb (+ magnetic) ones.
 constexpr uint32_t kFixG = F_GRAV | F_OPEN;
 constexpr uint32_t kFixGC = F_GRAV | F_COUL | F_OPEN;
 constexpr uint32_t kFixGCMY = F_GRAV | F_COUL | F_EXTRA | F_OPEN | F_FIXED | F_MAG | F_YUK;
+constexpr uint32_t kFixGCM = F_GRAV | F_COUL | F_EXTRA | F_OPEN | F_FIXED | F_MAG;
 constexpr uint32_t kFixGP = F_GRAV | F_EXTRA | F_OPEN | F_FIXED | F_GM | F_1PN;
 static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
     if (!kahan || pf.periodic || (feat & F_JERK)) return 0;
@@ -810,7 +813,9 @@ static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
         case F_GRAV: return kFixG;
         case F_GRAV | F_COUL: return kFixGC;
         case F_GRAV | F_COUL | F_EXTRA:
-            return pf.mag && pf.yuk && !pf.gm && !pf.one_pn && !pf.ax_mod && !pf.higgs ? kFixGCMY : 0;
+            if (!pf.mag || pf.gm || pf.one_pn || pf.ax_mod) return 0;
+            if (!pf.yuk) return kFixGCM;
+            return pf.higgs ? 0 : kFixGCMY;
         case F_GRAV | F_EXTRA:
             return pf.gm && pf.one_pn && !pf.mag && !pf.yuk && !pf.ax_mod ? kFixGP : 0;
         default: return 0;
@@ -821,15 +826,20 @@ static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
         const dim3 g_(grid), b_(kWalkThreads);                                                          \
         if ((fix) == kFixG) hipLaunchKernelGGL((KERNEL<kFixG, true>), g_, b_, 0, stream, args);           \
         else if ((fix) == kFixGC) hipLaunchKernelGGL((KERNEL<kFixGC, true>), g_, b_, 0, stream, args);    \
+        else if ((fix) == kFixGCM) hipLaunchKernelGGL((KERNEL<kFixGCM, true>), g_, b_, 0, stream, args);  \
         else hipLaunchKernelGGL((KERNEL<kFixGCMY, true>), g_, b_, 0, stream, args);                        \
     } while (0)
 
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
-    const uint32_t fix = a.hist.on ? 0u : fixed_feat(a.pf, feat, kahan);
-    if (a.hist.on) GEON_FEAT_SWITCH_R(k_walk_delay, feat, kahan, grid, s, a);
-    else if (a.far_mode == kFarMultipole) {
+    const uint32_t fix = fixed_feat(a.pf, feat, kahan);
+    if (a.hist.on) {
+        const dim3 g_(grid), b_(kWalkThreads);
+        if (fix == kFixGC) hipLaunchKernelGGL((k_walk_delay<kFixGC, true>), g_, b_, 0, s, a);
+        else if (fix == kFixGCM) hipLaunchKernelGGL((k_walk_delay<kFixGCM, true>), g_, b_, 0, s, a);
+        else GEON_FEAT_SWITCH_R(k_walk_delay, feat, kahan, grid, s, a);
+    } else if (a.far_mode == kFarMultipole) {
         if (fix && fix != kFixGP) GEON_FIXED_LAUNCH(k_walk_multipole, fix, grid, s, a);
         else GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
     } else if (a.far_mode == kFarList) {

@@ -109,6 +109,35 @@ def test_direct_sum_matches_oracle(oracle_lib):
     assert g.counters()["pair_interactions"] == 2048 * 2047
 
 
+@pytest.mark.parametrize("toggles", [
+    A.GRAVITY | A.BARNES_HUT,                                               # compiled: gravity
+    A.GRAVITY | A.COULOMB | A.BARNES_HUT,                                   # compiled: gravity + Coulomb
+    A.GRAVITY | A.COULOMB | A.YUKAWA | A.MAGNETIC | A.RELATIVITY | A.BARNES_HUT,   # compiled: the config-2 set
+    A.GRAVITY | A.GRAVITOMAG | A.ONE_PN | A.BARNES_HUT,                     # compiled: gravitomagnetism + 1PN
+    A.GRAVITY | A.COULOMB | A.MAGNETIC | A.AXION | A.BARNES_HUT,            # runtime-checked (axion-modulated charges)
+])
+def test_feature_set_walks_match_oracle(toggles, oracle_lib):
+    """Open-box walks for each feature set compiled with its toggles as
+    constants, and one that falls back to the runtime-checked walk: forces,
+    fields, PE and interaction counts against the oracle."""
+    st, prm, _ = W.config2(4096, open_box=True)
+    prm.update(toggles0=toggles, signal_delay=A.DELAY_OFF, theta=0.5)
+    g, o = _pair(st, prm, oracle_lib)
+    g.compute_forces()
+    o.compute_forces()
+    fg, fo = g.forces(len(st["x"])), o.forces()
+    assert _rel_l2(fg["fx"], fg["fy"], fo["fx"], fo["fy"]) < 2e-5
+    fields = (("bz", "bz"), ("dbzdx", "dbzdx"), ("dbzdy", "dbzdy")) if toggles & A.MAGNETIC else ()
+    fields += (("bgz", "bgz"),) if toggles & A.GRAVITOMAG else ()
+    for k_g, k_o in fields:
+        a, b = fg[k_g], fo[k_o]
+        assert np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)) < 1e-4, k_g
+    assert 0.5 * fg["pe"].sum() == pytest.approx(fo["pe_total"], rel=1e-5)
+    cg, co = g.counters(), o.counters()
+    assert abs(cg["pair_interactions"] - co["pair"]) <= 1e-3 * co["pair"]
+    assert abs(cg["node_interactions"] - co["node"]) <= 1e-3 * max(co["node"], 1)
+
+
 @pytest.mark.parametrize("mode", [A.PERIODIC_MIN_IMAGE, A.PERIODIC_WINDOWED])
 def test_config2_forces_fields_periodic(mode, oracle_lib):
     """Gravity + Coulomb + Yukawa + magnetic, relativistic, torus (min-image / windowed)."""
