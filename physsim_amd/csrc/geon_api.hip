@@ -283,6 +283,8 @@ uint32_t feature_mask(const GeonParams& p) {
     if (t & GEON_GRAVITY) f |= F_GRAV;
     if (t & GEON_COULOMB) f |= F_COUL;
     if (t & (GEON_MAGNETIC | GEON_GRAVITOMAG | GEON_YUKAWA)) f |= F_EXTRA;
+    // the axion field modulates Coulomb pairs even without the other extra forces (forces.js:79, 223)
+    if ((t & GEON_COULOMB) && (t & GEON_AXION)) f |= F_EXTRA;
     if (t & GEON_RADIATION) f = F_GRAV | F_COUL | F_JERK | (f & F_EXTRA);   // gravity, Coulomb runtime-checked; the jerk
     return f;
 }

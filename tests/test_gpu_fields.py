@@ -183,3 +183,40 @@ def test_deposit_tiled_and_direct_paths(order, oracle_lib):
     e = _rel(g.get_field(0, "thermal"), o.field_get(0, "thermal"))
     print("thermal: %.3e" % e)
     assert e < 3e-5
+
+
+def test_axion_modulates_coulomb_without_magnetic(oracle_lib):
+    """Gravity + Coulomb + axion with no magnetic, gravitomagnetic or Yukawa
+    term: the Coulomb pairs still take sqrt(axMod_i axMod_j) (forces.js:79,
+    223).  One update on both sides (the fields set the modulations), then the
+    force pass; the axion-free pass must differ by far more than the tolerance."""
+    from physsim_amd import HipPhysics, default_params
+    G, n = 64, 4096
+    st, prm, dt, fields = W.config3(n, grid=G, side=256.0, total_mass=4.0)
+    r = np.random.default_rng(11)
+    st["charge"] = np.round(r.uniform(-1.0, 1.0, n) * 10.0) / 10.0
+    prm["toggles0"] = A.GRAVITY | A.COULOMB | A.AXION | A.BARNES_HUT
+    axion = W.f32(W.gaussian_field(G, r, 0.3))
+
+    def pair(p):
+        print("case: axion-modulated Coulomb n=%d G=%d toggles=0x%x" % (n, G, p["toggles0"]), flush=True)
+        g = HipPhysics(default_params(**p), max_particles=n)
+        g.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+        o = oracle_lib.Oracle(oracle_lib.default_params(**p), n)
+        o.upload(st["x"], st["y"], st["wx"], st["wy"], st["mass"], st["charge"], st["angw"])
+        if p["toggles0"] & A.AXION:
+            g.set_field(1, axion)
+            o.field_set(1, axion)
+        assert g.update(dt) == o.update(dt)
+        g.compute_forces()
+        o.compute_forces()
+        ids = o.download()["id"]
+        fg, fo = g.forces(n), o.forces()
+        return fg["fx"][ids], fg["fy"][ids], fo["fx"], fo["fy"]
+
+    gx, gy, ox, oy = pair(prm)
+    err = float(np.sqrt(np.sum((gx - ox) ** 2 + (gy - oy) ** 2) / np.sum(ox ** 2 + oy ** 2)))
+    assert err < 1e-4, err
+    _, _, px, py = pair(dict(prm, toggles0=prm["toggles0"] & ~A.AXION))
+    effect = float(np.sqrt(np.sum((px - ox) ** 2 + (py - oy) ** 2) / np.sum(ox ** 2 + oy ** 2)))
+    assert effect > 100 * err, (effect, err)
