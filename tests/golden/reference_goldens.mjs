@@ -91,6 +91,19 @@ async function runCase(c, mods) {
     ph._syncAxionField = (...a) => { calls++; return sync(...a); };
     const args = [c.collisionMode, c.boundaryMode, c.topology, c.W, c.H];
     ph.update(ps, 0, ...args);                         // the bootstrap force pass only (dtRemain = 0)
+    if (c.timeOnly) {
+        // wall time of the updates alone, particle-steps counted as live particles per substep
+        let steps = 0;
+        const t0 = process.hrtime.bigint();
+        for (let k = 0; k < c.updates; k++) {
+            calls = 0;
+            const n0 = ps.length;
+            ph.update(ps, c.dt, ...args);
+            steps += calls * n0;
+        }
+        const sec = Number(process.hrtime.bigint() - t0) * 1e-9;
+        return { seconds: sec, particle_steps: steps, n: ps.length };
+    }
     const out = { bootstrap: { forces: forcesOf(ps), pe: ph.potentialEnergy } };
     const substeps = [];
     for (let k = 0; k < c.updates; k++) {
