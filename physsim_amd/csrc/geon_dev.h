@@ -32,6 +32,8 @@ enum : uint32_t {
     F_OPEN = 16u,       // open box: no minimum image, no window test
     F_FIXED = 32u,      // the extra forces are the bits below (no axion, no Higgs), not runtime toggles
     F_MAG = 64u, F_GM = 128u, F_YUK = 256u, F_1PN = 512u,
+    F_TORUS = 1024u,    // periodic torus (topology 0): the minimum image without the topology branch
+    F_WINDOW = 2048u,   // with F_TORUS: the windowed walk (nodes accepted only inside the min-image window)
 };
 
 // Runtime toggles of the extra forces (uniform per launch).
@@ -190,12 +192,28 @@ __device__ __forceinline__ void min_image(const PhysFlags& pf, float ox, float o
                                           float& dy) {
     min_image_t<float>(pf.periodic, pf.topology, pf.dom_w, pf.dom_h, ox, oy, sx, sy, dx, dy);
 }
-// As min_image, with the open box (F_OPEN) known at compile time.
+// As min_image, with the open box (F_OPEN) or the torus (F_TORUS) known at
+// compile time.
 template <uint32_t F>
 __device__ __forceinline__ void min_image_f(const PhysFlags& pf, float ox, float oy, float sx, float sy, float& dx,
                                             float& dy) {
-    if constexpr ((F & F_OPEN) != 0u) { dx = sx - ox; dy = sy - oy; }
-    else min_image(pf, ox, oy, sx, sy, dx, dy);
+    if constexpr ((F & F_OPEN) != 0u) {
+        dx = sx - ox;
+        dy = sy - oy;
+    } else if constexpr ((F & F_TORUS) != 0u) {
+        dx = wrap_t(sx - ox, pf.dom_w, 0.5f * pf.dom_w);
+        dy = wrap_t(sy - oy, pf.dom_h, 0.5f * pf.dom_h);
+    } else {
+        min_image(pf, ox, oy, sx, sy, dx, dy);
+    }
+}
+// Whether the walk tests the min-image window: compile-time for the open box
+// and the torus instantiations, the launch's flag otherwise.
+template <uint32_t F>
+__device__ __forceinline__ bool window_f(const PhysFlags& pf) {
+    if constexpr ((F & F_OPEN) != 0u) return false;
+    else if constexpr ((F & F_TORUS) != 0u) return (F & F_WINDOW) != 0u;
+    else return pf.window != 0u;
 }
 
 // Far-field terms of an accepted node beyond its monopole (not in the

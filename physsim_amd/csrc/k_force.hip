@@ -194,7 +194,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
             min_image_f<F>(pf, o.x, o.y, nd.comx, nd.comy, dx, dy);
             const float d2 = dx * dx + dy * dy;
             bool accept = mine && (nd.size2 < pf.theta2 * d2);   // forces.js:680
-            if (!(F & F_OPEN) && pf.window) {
+            if (window_f<F>(pf)) {
                 // the node's whole box must map through the COM's image shift into the window
                 const float cx = nd.bx - o.x + (dx - rawx), cy = nd.by - o.y + (dy - rawy);
                 accept = accept && (fabsf(cx) + nd.half <= half_w) && (fabsf(cy) + nd.half * (pf.dom_h / pf.dom_w) <= half_h);
@@ -791,8 +791,8 @@ __global__ __launch_bounds__(kWave) void k_sel_forces(ForceArgs a, int residual)
         }                                                                                             \
     } while (0)
 
-// The fixed instantiation that runs this launch's feature set in an open box,
-// or 0 when none does and the runtime-checked kernel runs:
+// The fixed instantiation that runs this launch's feature set, or 0 when none
+// does and the runtime-checked kernel runs.  In an open box:
 //   kFixG     gravity;
 //   kFixGC    gravity + Coulomb;
 //   kFixGCMY  gravity + Coulomb + magnetic + Yukawa, no axion modulation or
@@ -801,14 +801,19 @@ __global__ __launch_bounds__(kWave) void k_sel_forces(ForceArgs a, int residual)
 //   kFixGP    gravity + gravitomagnetism + 1PN, no Yukawa or axion-modulated
 //             charges: the plain walk only.
 // (The Higgs field only scales the Yukawa range, so without Yukawa it may be
-// on.)  The signal-delay walk has the gravity + Coulomb (+ magnetic) ones.
+// on.)  On a torus (topology 0), minimum image or windowed: the gravity +
+// Coulomb and configuration-2 sets.  The signal-delay walk has the open-box
+// gravity + Coulomb (+ magnetic) ones.
 constexpr uint32_t kFixG = F_GRAV | F_OPEN;
 constexpr uint32_t kFixGC = F_GRAV | F_COUL | F_OPEN;
 constexpr uint32_t kFixGCMY = F_GRAV | F_COUL | F_EXTRA | F_OPEN | F_FIXED | F_MAG | F_YUK;
 constexpr uint32_t kFixGCM = F_GRAV | F_COUL | F_EXTRA | F_OPEN | F_FIXED | F_MAG;
 constexpr uint32_t kFixGP = F_GRAV | F_EXTRA | F_OPEN | F_FIXED | F_GM | F_1PN;
-static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
-    if (!kahan || pf.periodic || (feat & F_JERK)) return 0;
+constexpr uint32_t kFixGCT = (kFixGC & ~F_OPEN) | F_TORUS;
+constexpr uint32_t kFixGCW = kFixGCT | F_WINDOW;
+constexpr uint32_t kFixGCMYT = (kFixGCMY & ~F_OPEN) | F_TORUS;
+constexpr uint32_t kFixGCMYW = kFixGCMYT | F_WINDOW;
+static uint32_t fixed_set(const PhysFlags& pf, uint32_t feat) {
     switch (feat & 7u) {
         case F_GRAV: return kFixG;
         case F_GRAV | F_COUL: return kFixGC;
@@ -821,13 +826,26 @@ static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
         default: return 0;
     }
 }
+static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
+    if (!kahan || (feat & F_JERK)) return 0;
+    const uint32_t set = fixed_set(pf, feat);
+    if (!pf.periodic) return set;
+    if (pf.topology != 0 || (set != kFixGC && set != kFixGCMY)) return 0;
+    return (set & ~F_OPEN) | F_TORUS | (pf.window ? F_WINDOW : 0u);
+}
 #define GEON_FIXED_LAUNCH(KERNEL, fix, grid, stream, args)                                                  \
     do {                                                                                                \
         const dim3 g_(grid), b_(kWalkThreads);                                                          \
-        if ((fix) == kFixG) hipLaunchKernelGGL((KERNEL<kFixG, true>), g_, b_, 0, stream, args);           \
-        else if ((fix) == kFixGC) hipLaunchKernelGGL((KERNEL<kFixGC, true>), g_, b_, 0, stream, args);    \
-        else if ((fix) == kFixGCM) hipLaunchKernelGGL((KERNEL<kFixGCM, true>), g_, b_, 0, stream, args);  \
-        else hipLaunchKernelGGL((KERNEL<kFixGCMY, true>), g_, b_, 0, stream, args);                        \
+        switch (fix) {                                                                                  \
+            case kFixG: hipLaunchKernelGGL((KERNEL<kFixG, true>), g_, b_, 0, stream, args); break;        \
+            case kFixGC: hipLaunchKernelGGL((KERNEL<kFixGC, true>), g_, b_, 0, stream, args); break;      \
+            case kFixGCM: hipLaunchKernelGGL((KERNEL<kFixGCM, true>), g_, b_, 0, stream, args); break;    \
+            case kFixGCMY: hipLaunchKernelGGL((KERNEL<kFixGCMY, true>), g_, b_, 0, stream, args); break;  \
+            case kFixGCT: hipLaunchKernelGGL((KERNEL<kFixGCT, true>), g_, b_, 0, stream, args); break;    \
+            case kFixGCW: hipLaunchKernelGGL((KERNEL<kFixGCW, true>), g_, b_, 0, stream, args); break;    \
+            case kFixGCMYT: hipLaunchKernelGGL((KERNEL<kFixGCMYT, true>), g_, b_, 0, stream, args); break; \
+            default: hipLaunchKernelGGL((KERNEL<kFixGCMYW, true>), g_, b_, 0, stream, args); break;       \
+        }                                                                                               \
     } while (0)
 
 void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {

@@ -139,6 +139,26 @@ def test_feature_set_walks_match_oracle(toggles, oracle_lib):
 
 
 @pytest.mark.parametrize("mode", [A.PERIODIC_MIN_IMAGE, A.PERIODIC_WINDOWED])
+@pytest.mark.parametrize("toggles", [A.GRAVITY | A.COULOMB | A.BARNES_HUT, A.GRAVITY | A.BARNES_HUT])
+def test_torus_walks_match_oracle(toggles, mode, oracle_lib):
+    """Torus walks: gravity + Coulomb compiled with the torus minimum image
+    (and the window) as constants, gravity alone on the runtime-checked walk;
+    forces, PE and interaction counts against the oracle (configuration-2
+    state in its 256^2 torus, theta 0.5)."""
+    st, prm, _ = W.config2(4096)
+    prm.update(toggles0=toggles, periodic_mode=mode, signal_delay=A.DELAY_OFF)
+    g, o = _pair(st, prm, oracle_lib)
+    g.compute_forces()
+    o.compute_forces()
+    fg, fo = g.forces(len(st["x"])), o.forces()
+    assert _rel_l2(fg["fx"], fg["fy"], fo["fx"], fo["fy"]) < 1e-4
+    assert 0.5 * fg["pe"].sum() == pytest.approx(fo["pe_total"], rel=1e-5)
+    cg, co = g.counters(), o.counters()
+    assert abs(cg["pair_interactions"] - co["pair"]) <= 1e-3 * co["pair"]
+    assert abs(cg["node_interactions"] - co["node"]) <= 1e-3 * max(co["node"], 1)
+
+
+@pytest.mark.parametrize("mode", [A.PERIODIC_MIN_IMAGE, A.PERIODIC_WINDOWED])
 def test_config2_forces_fields_periodic(mode, oracle_lib):
     """Gravity + Coulomb + Yukawa + magnetic, relativistic, torus (min-image / windowed)."""
     st, prm, _ = W.config2(4096)
