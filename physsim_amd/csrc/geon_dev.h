@@ -285,8 +285,10 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
         aberr = fminf(__builtin_amdgcn_rcpf(den * den * den), 100.f);
     }
     const float inv_r3a = ABERR ? inv_r3 * aberr : inv_r3;
-    const bool do_grav = (F & F_JERK) ? pf.grav_on != 0u : (F & F_GRAV) != 0u;
-    const bool do_coul = (F & F_JERK) ? pf.coul_on != 0u : (F & F_COUL) != 0u;
+    // the jerk instantiations check gravity and Coulomb at run time, unless fixed
+    constexpr bool kRuntimeGC = (F & F_JERK) != 0u && (F & F_FIXED) == 0u;
+    const bool do_grav = kRuntimeGC ? pf.grav_on != 0u : (F & F_GRAV) != 0u;
+    const bool do_coul = kRuntimeGC ? pf.coul_on != 0u : (F & F_COUL) != 0u;
     float vrx = 0.f, vry = 0.f, rdv = 0.f, inv_r5j = 0.f;
     if (F & F_JERK) {                                               // forces.js:198-203
         vrx = svx - o.vx;

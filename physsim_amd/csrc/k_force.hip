@@ -809,6 +809,7 @@ constexpr uint32_t kFixGC = F_GRAV | F_COUL | F_OPEN;
 constexpr uint32_t kFixGCMY = F_GRAV | F_COUL | F_EXTRA | F_OPEN | F_FIXED | F_MAG | F_YUK;
 constexpr uint32_t kFixGCM = F_GRAV | F_COUL | F_EXTRA | F_OPEN | F_FIXED | F_MAG;
 constexpr uint32_t kFixGP = F_GRAV | F_EXTRA | F_OPEN | F_FIXED | F_GM | F_1PN;
+constexpr uint32_t kFixGCJ = F_GRAV | F_COUL | F_JERK | F_OPEN | F_FIXED;   // radiation: gravity + Coulomb + jerk
 constexpr uint32_t kFixGCT = (kFixGC & ~F_OPEN) | F_TORUS;
 constexpr uint32_t kFixGCW = kFixGCT | F_WINDOW;
 constexpr uint32_t kFixGCMYT = (kFixGCMY & ~F_OPEN) | F_TORUS;
@@ -827,7 +828,9 @@ static uint32_t fixed_set(const PhysFlags& pf, uint32_t feat) {
     }
 }
 static uint32_t fixed_feat(const PhysFlags& pf, uint32_t feat, bool kahan) {
-    if (!kahan || (feat & F_JERK)) return 0;
+    if (!kahan) return 0;
+    if (feat & F_JERK)   // the radiation walks: gravity and Coulomb both on, no extra force, open box
+        return feat == (F_GRAV | F_COUL | F_JERK) && pf.grav_on && pf.coul_on && !pf.periodic ? kFixGCJ : 0;
     const uint32_t set = fixed_set(pf, feat);
     if (!pf.periodic) return set;
     if (pf.topology != 0 || (set != kFixGC && set != kFixGCMY)) return 0;
@@ -858,13 +861,15 @@ void launch_walk(hipStream_t s, const ForceArgs& a, uint32_t feat, bool kahan) {
         else if (fix == kFixGCM) hipLaunchKernelGGL((k_walk_delay<kFixGCM, true>), g_, b_, 0, s, a);
         else GEON_FEAT_SWITCH_R(k_walk_delay, feat, kahan, grid, s, a);
     } else if (a.far_mode == kFarMultipole) {
-        if (fix && fix != kFixGP) GEON_FIXED_LAUNCH(k_walk_multipole, fix, grid, s, a);
+        if (fix && fix != kFixGP && fix != kFixGCJ) GEON_FIXED_LAUNCH(k_walk_multipole, fix, grid, s, a);
         else GEON_FEAT_SWITCH(k_walk_multipole, feat, kahan, grid, s, a);
     } else if (a.far_mode == kFarList) {
-        if (fix && fix != kFixGP) GEON_FIXED_LAUNCH(k_walk_list, fix, grid, s, a);
+        if (fix && fix != kFixGP && fix != kFixGCJ) GEON_FIXED_LAUNCH(k_walk_list, fix, grid, s, a);
         else GEON_FEAT_SWITCH(k_walk_list, feat, kahan, grid, s, a);
     } else if (fix == kFixGP) {
         hipLaunchKernelGGL((k_walk<kFixGP, true>), dim3(grid), dim3(kWalkThreads), 0, s, a);
+    } else if (fix == kFixGCJ) {
+        hipLaunchKernelGGL((k_walk<kFixGCJ, true>), dim3(grid), dim3(kWalkThreads), 0, s, a);
     } else if (fix) GEON_FIXED_LAUNCH(k_walk, fix, grid, s, a);
     else GEON_FEAT_SWITCH_R(k_walk, feat, kahan, grid, s, a);
 }
