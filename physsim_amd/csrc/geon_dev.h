@@ -434,15 +434,17 @@ __device__ __forceinline__ void pair_force(const Obs& o, Acc<KAHAN>& a, const Ph
 
 // _accum1PN (forces.js:445-506): the 1PN force of one source for the
 // velocity-Verlet correction (EIH, Darwin, Bazanski, scalar Breit).
+template <uint32_t F = 0u>
 __device__ __forceinline__ void accum_1pn(const Obs& o, float& f1x, float& f1y, const PhysFlags& pf, float rx,
                                           float ry, float sm, float sq, float svx, float svy, float s_yuk,
                                           float s_higgs) {
+    const Feat<F> ft(pf);
     const float r2 = rx * rx + ry * ry + pf.soft2;
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float inv_r2 = inv_r * inv_r;
     const float r = r2 * inv_r;                 // sqrt(r2) without a division
     const float nx = rx * inv_r, ny = ry * inv_r;
-    if (pf.gm) {
+    if (ft.gm) {
         const float v1sq = o.vx * o.vx + o.vy * o.vy, v2sq = svx * svx + svy * svy;
         const float nv1 = nx * o.vx + ny * o.vy, nv2 = nx * svx + ny * svy;
         const float radial = -v1sq - 2.f * v2sq + 1.5f * nv2 * nv2 + 5.f * o.m * inv_r + 4.f * sm * inv_r;
@@ -451,20 +453,20 @@ __device__ __forceinline__ void accum_1pn(const Obs& o, float& f1x, float& f1y, 
         f1x += base * (rx * radial + (o.vx * c1 + svx * c2) * r);
         f1y += base * (ry * radial + (o.vy * c1 + svy * c2) * r);
     }
-    if (pf.mag) {
+    if (ft.mag) {
         const float v2n = svx * nx + svy * ny, v1n = o.vx * nx + o.vy * ny;
         const float coeff = 0.5f * o.q * sq * inv_r2;
         f1x += coeff * (o.vx * v2n - 3.f * nx * v1n * v2n);
         f1y += coeff * (o.vy * v2n - 3.f * ny * v1n * v2n);
     }
-    if (pf.gm && pf.mag) {
+    if (ft.gm && ft.mag) {
         const float cc = o.q * sq * (o.m + sm) - (o.q * o.q * sm + sq * sq * o.m);
         const float fd = cc * inv_r2 * inv_r2;
         f1x += rx * fd;
         f1y += ry * fd;
     }
-    if (pf.yuk) {
-        const float mu = pf.higgs ? pf.yuk_mu * sqrtf(o.higgs * s_higgs) : pf.yuk_mu;
+    if (ft.yuk) {
+        const float mu = ft.higgs ? pf.yuk_mu * sqrtf(o.higgs * s_higgs) : pf.yuk_mu;
         const float e = __expf(-mu * r);
         const float nv1 = nx * o.vx + ny * o.vy, nv2 = nx * svx + ny * svy;
         const float v1v2 = o.vx * svx + o.vy * svy;

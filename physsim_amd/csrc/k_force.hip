@@ -287,7 +287,8 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk_persistent96(ForceArgs a,
 // 1PN recompute for the velocity-Verlet correction: the same wavefront walk
 // with _accum1PN at leaves and accepted nodes (forces.js:511-571); leaf sources
 // from their past light cone when DELAY (542-546), else at the current time.
-template <bool DELAY>
+// F: 0 for the runtime-checked walk, or a fixed open-box set (its 1PN toggles as constants).
+template <bool DELAY, uint32_t F = 0u>
 __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
     __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
     __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
@@ -330,24 +331,24 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk1pn(ForceArgs a) {
                     Retarded rt;
                     if (mine && j != i && delayed_state(a.hist, pf, a.hist.slot[j], o.x, o.y, now, rt)) {
                         const float rx = rt.dx, ry = rt.dy;
-                        accum_1pn(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, s8b.z, s8b.w);
+                        accum_1pn<F>(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, rt.vx, rt.vy, s8b.z, s8b.w);
                     }
                 } else if (mine && j != i) {
                     float rx, ry;
-                    min_image(pf, o.x, o.y, s4.x, s4.y, rx, ry);
-                    accum_1pn(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8b.z, s8b.w);
+                    min_image_f<F>(pf, o.x, o.y, s4.x, s4.y, rx, ry);
+                    accum_1pn<F>(o, f1x, f1y, pf, rx, ry, s4.z, s4.w, s8a.x, s8a.y, s8b.z, s8b.w);
                 }
             }
         } else if (nd.count < 0) {
             const float rawx = nd.comx - o.x, rawy = nd.comy - o.y;
             float dx, dy;
-            min_image(pf, o.x, o.y, nd.comx, nd.comy, dx, dy);
+            min_image_f<F>(pf, o.x, o.y, nd.comx, nd.comy, dx, dy);
             bool accept = mine && (nd.size2 < pf.theta2 * (dx * dx + dy * dy));
-            if (pf.window) {
+            if (window_f<F>(pf)) {
                 const float cx = nd.bx - o.x + (dx - rawx), cy = nd.by - o.y + (dy - rawy);
                 accept = accept && (fabsf(cx) + nd.half <= half_w) && (fabsf(cy) + nd.half * (pf.dom_h / pf.dom_w) <= half_h);
             }
-            if (accept) accum_1pn(o, f1x, f1y, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 1.f, 1.f);
+            if (accept) accum_1pn<F>(o, f1x, f1y, pf, dx, dy, nd.mass, nd.charge, nd.avx, nd.avy, 1.f, 1.f);
             const uint64_t open = __ballot(mine && !accept);
             if (open) {
                 if (sp + 4 <= kWalkStack) {
@@ -419,6 +420,13 @@ __global__ __launch_bounds__(kWalkThreads) void k_direct1pn(ForceArgs a) {
 void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree) {
     if (a.n_upper <= 0) return;
     const int grid = (a.n_upper + kWalkThreads - 1) / kWalkThreads;
+    // gravitomagnetism + 1PN in an open box, no magnetic, Yukawa or axion terms (config 3's set)
+    const PhysFlags& pf = a.pf;
+    if (tree && !a.hist.on && !pf.periodic && pf.gm && pf.one_pn && !pf.mag && !pf.yuk && !pf.ax_mod) {
+        constexpr uint32_t kFix1PN = F_GRAV | F_EXTRA | F_OPEN | F_FIXED | F_GM | F_1PN;
+        hipLaunchKernelGGL((k_walk1pn<false, kFix1PN>), dim3(grid), dim3(kWalkThreads), 0, s, a);
+        return;
+    }
     if (a.hist.on) {
         if (tree) hipLaunchKernelGGL(k_walk1pn<true>, dim3(grid), dim3(kWalkThreads), 0, s, a);
         else hipLaunchKernelGGL(k_direct1pn<true>, dim3(grid), dim3(kWalkThreads), 0, s, a);
