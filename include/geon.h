@@ -33,7 +33,7 @@ extern "C" {
 #define GEON_ENOMEM      -4   /* device allocation failed */
 #define GEON_ESTATE      -5   /* call not valid in the current state */
 #define GEON_EUNSUPPORTED -6  /* toggle/mode combination this backend does not run */
-#define GEON_EOVERFLOW   -7   /* tree node pool or stack capacity exceeded */
+#define GEON_EOVERFLOW   -7   /* a capacity exceeded: node pool, walk stack, far-field pair queue, pair lists */
 
 /* ---- toggles0 bits: same values as src/gpu/gpu-constants.js:196-213 ---- */
 #define GEON_GRAVITY        (1u << 0)
