@@ -241,7 +241,7 @@ struct FftLds {
 __device__ void fft_pass(FftLds& s, int N, int logN, bool along_x, bool inverse) {
     const int t = threadIdx.x;
     for (int e = t; e < N * N; e += kFftThreads) {
-        const int line = e / N, k = e % N;
+        const int line = e >> logN, k = e & (N - 1);
         const int j = (int)(__brev((unsigned)k) >> (32 - logN));
         if (k < j) {
             const int a = along_x ? line * kPitch + k : k * kPitch + line;
@@ -252,11 +252,12 @@ __device__ void fft_pass(FftLds& s, int N, int logN, bool along_x, bool inverse)
         }
     }
     __syncthreads();
-    for (int len = 2; len <= N; len <<= 1) {
-        const int half = len >> 1, stride = N / len;
+    // N is a power of two: line, group and position by shifts and masks
+    for (int lg = 1; lg <= logN; lg++) {
+        const int len = 1 << lg, half = len >> 1, stride = N >> lg;
         for (int e = t; e < N * (N / 2); e += kFftThreads) {
-            const int line = e / (N / 2), k = e % (N / 2);
-            const int grp = k / half, jj = k % half;
+            const int line = e >> (logN - 1), k = e & (N / 2 - 1);
+            const int grp = k >> (lg - 1), jj = k & (half - 1);
             const int ia = grp * len + jj, ib = ia + half;
             const int a = along_x ? line * kPitch + ia : ia * kPitch + line;
             const int b = along_x ? line * kPitch + ib : ib * kPitch + line;
@@ -273,7 +274,7 @@ __device__ void fft_pass(FftLds& s, int N, int logN, bool along_x, bool inverse)
     if (inverse) {
         const float inv = 1.f / (float)N;          // fft.js:63-69 normalizes each 1-D inverse
         for (int e = t; e < N * N; e += kFftThreads) {
-            const int a = (e / N) * kPitch + (e % N);
+            const int a = (e >> logN) * kPitch + (e & (N - 1));
             s.re[a] *= inv; s.im[a] *= inv;
         }
         __syncthreads();
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
         const bool glide = f.bc == 2 && f.topo != 0;
         const double W = (double)f.cw * N, H = (double)f.ch * N;
         for (int e = t; e < N * N; e += kFftThreads) {
-            const int iy = e / N, ix = e % N;
+            const int iy = e >> logN, ix = e & (N - 1);
             double ddx, ddy;
             if (glide) {
                 // Klein / RP^2: the reference takes minImage of the grid offset (scalar-field.js:538-548)
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
         __syncthreads();
         fft2d_lds(s, N, logN, false);
         for (int e = t; e < N * N; e += kFftThreads) {
-            const int a = (e / N) * kPitch + (e % N);
+            const int a = (e >> logN) * kPitch + (e & (N - 1));
             f.green[e] = make_float2(s.re[a], s.im[a]);
         }
         return;
@@ -332,13 +333,13 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
         // plain 2-D FFT of the complex grid in f.green, in place (2 forward, 3 inverse)
         for (int e = t; e < N * N; e += kFftThreads) {
             const float2 v = f.green[e];
-            s.re[(e / N) * kPitch + (e % N)] = v.x;
-            s.im[(e / N) * kPitch + (e % N)] = v.y;
+            s.re[(e >> logN) * kPitch + (e & (N - 1))] = v.x;
+            s.im[(e >> logN) * kPitch + (e & (N - 1))] = v.y;
         }
         __syncthreads();
         fft2d_lds(s, N, logN, mode == 3);
         for (int e = t; e < N * N; e += kFftThreads) {
-            const int a = (e / N) * kPitch + (e % N);
+            const int a = (e >> logN) * kPitch + (e & (N - 1));
             f.green[e] = make_float2(s.re[a], s.im[a]);
         }
         return;
@@ -356,8 +357,8 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
         }
         const float rho = 0.5f * fd * fd + 0.5f * (gx * gx * icw2 + gy * gy * ich2) + pot;
         if (rho >= 1e-12f) s.has_energy = 1;
-        s.re[(e / N) * kPitch + (e % N)] = rho * area;
-        s.im[(e / N) * kPitch + (e % N)] = 0.f;
+        s.re[(e >> logN) * kPitch + (e & (N - 1))] = rho * area;
+        s.im[(e >> logN) * kPitch + (e & (N - 1))] = 0.f;
     }
     __syncthreads();
     if (!s.has_energy) {
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
     }
     fft2d_lds(s, N, logN, false);
     for (int e = t; e < N * N; e += kFftThreads) {
-        const int a = (e / N) * kPitch + (e % N);
+        const int a = (e >> logN) * kPitch + (e & (N - 1));
         const float2 g = f.green[e];
         const float ar = s.re[a], ai = s.im[a];
         s.re[a] = ar * g.x - ai * g.y;
@@ -377,10 +378,10 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
     fft2d_lds(s, N, logN, true);
     // Phi = Re, central-difference gradient (outside = 0) (scalar-field.js:567-596)
     for (int e = t; e < N * N; e += kFftThreads) {
-        const int iy = e / N, ix = e % N;
+        const int iy = e >> logN, ix = e & (N - 1);
         const int iL = nb_index(N, ix - 1, iy, f.bc, f.topo), iR = nb_index(N, ix + 1, iy, f.bc, f.topo);
         const int iT = nb_index(N, ix, iy - 1, f.bc, f.topo), iB = nb_index(N, ix, iy + 1, f.bc, f.topo);
-        auto P = [&](int idx) { return idx >= 0 ? s.re[(idx / N) * kPitch + (idx % N)] : 0.f; };
+        auto P = [&](int idx) { return idx >= 0 ? s.re[(idx >> logN) * kPitch + (idx & (N - 1))] : 0.f; };
         f.sg_gx[e] = (P(iR) - P(iL)) * 0.5f;
         f.sg_gy[e] = (P(iB) - P(iT)) * 0.5f;
         f.sg_phi[e] = s.re[iy * kPitch + ix];
