@@ -38,6 +38,8 @@ struct FieldState {
     float *source = nullptr, *thermal = nullptr, *sg_phi = nullptr, *sg_gx = nullptr, *sg_gy = nullptr;
     unsigned long long *dep_src = nullptr, *dep_th = nullptr;
     float2* green = nullptr;
+    float2* sg_c = nullptr;
+    int* sg_rows = nullptr;
     int* has_energy = nullptr;
     double green_w = -1, green_h = -1, green_soft2 = -1;
     int green_topo = -1;        // 0 torus / open, else the glide topology the grid was built for
@@ -508,6 +510,7 @@ FieldArgs field_args(GeonContext* h, int which) {
     f.grad_x = s.grad_x; f.grad_y = s.grad_y; f.source = s.source; f.thermal = s.thermal;
     f.sg_phi = s.sg_phi; f.sg_gx = s.sg_gx; f.sg_gy = s.sg_gy;
     f.dep_src = s.dep_src; f.dep_th = s.dep_th; f.green = s.green; f.has_energy = s.has_energy;
+    f.sg_c = s.sg_c; f.sg_rows = s.sg_rows;
     f.ctrl = h->ctrl;
     return f;
 }
@@ -1126,6 +1129,8 @@ int allocate(GeonContext* h) {
         if ((rc = dmalloc(h, &s.dep_th, g2))) return rc;
         if ((rc = dmalloc(h, &s.green, g2))) return rc;
         if ((rc = dmalloc(h, &s.has_energy, 1))) return rc;
+        if ((rc = dmalloc(h, &s.sg_c, g2))) return rc;
+        if ((rc = dmalloc(h, &s.sg_rows, (size_t)h->field_g))) return rc;
         HIP_OK(hipMemset(s.dep_src, 0, 8 * g2));
         HIP_OK(hipMemset(s.dep_th, 0, 8 * g2));
     }
@@ -1162,7 +1167,7 @@ void release(GeonContext* h) {
         float* fa[] = {s.field, s.field_dot, s.field_dot_alt, s.grad_x, s.grad_y, s.source, s.thermal,
                        s.sg_phi, s.sg_gx, s.sg_gy};
         for (float* p : fa) hipFree(p);
-        hipFree(s.dep_src); hipFree(s.dep_th); hipFree(s.green); hipFree(s.has_energy);
+        hipFree(s.dep_src); hipFree(s.dep_th); hipFree(s.green); hipFree(s.has_energy); hipFree(s.sg_c); hipFree(s.sg_rows);
     }
     if (h->ctrl_host) hipHostFree(h->ctrl_host);
     if (h->ctrl_ev) hipEventDestroy(h->ctrl_ev);

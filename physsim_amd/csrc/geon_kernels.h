@@ -396,6 +396,8 @@ struct FieldArgs {
     float *source, *thermal, *sg_phi, *sg_gx, *sg_gy;
     unsigned long long *dep_src, *dep_th;   // 2^32 fixed-point deposit accumulators
     float2* green;              // G^ (FFT of the softened Green's function)
+    float2* sg_c;               // self-gravity: the complex grid between its row and column passes
+    int* sg_rows;               // self-gravity: per row, whether any cell holds energy
     int* has_energy;
     const Ctrl* ctrl;           // non-null: time-stepping kernels take dt_sub / active from it
 };

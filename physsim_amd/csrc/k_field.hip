@@ -232,7 +232,6 @@ struct FftLds {
     float re[kFftMax * kPitch];
     float im[kFftMax * kPitch];
     float tw_re[kFftMax / 2], tw_im[kFftMax / 2];
-    int has_energy;
 };
 
 // In-place radix-2 DIT over all rows (along_x) or all columns of the LDS grid.
@@ -295,15 +294,16 @@ __device__ void fft_twiddles(FftLds& s, int N) {
     }
 }
 
-// mode 0: self-gravity of the field; mode 1: build G^ = FFT(-1/sqrt(r^2+eps^2))
-// (scalar-field.js:524-560; the min-image and wrapped-index distance grids of
-// the reference coincide, so one grid serves both boundary modes).
+// mode 1: build G^ = FFT(-1/sqrt(r^2+eps^2)) (scalar-field.js:524-560; the
+// min-image and wrapped-index distance grids of the reference coincide, so one
+// grid serves both boundary modes); modes 2, 3: a plain 2-D FFT of f.green.
+// (Once per grid, or for tests: the per-substep self-gravity runs over many
+// workgroups below.)
 __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int mode, float soft2) {
     __shared__ FftLds s;
     const int N = f.G, logN = 31 - __clz(N);
     const int t = threadIdx.x;
     fft_twiddles(s, N);
-    if (t == 0) s.has_energy = 0;
     __syncthreads();
     if (mode == 1) {
         const bool glide = f.bc == 2 && f.topo != 0;
@@ -342,51 +342,156 @@ __global__ __launch_bounds__(kFftThreads) void k_field_selfgrav(FieldArgs f, int
             const int a = (e >> logN) * kPitch + (e & (N - 1));
             f.green[e] = make_float2(s.re[a], s.im[a]);
         }
-        return;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Self-gravity of a field (scalar-field.js:567-660): rho -> FFT -> x G^ ->
+// IFFT -> Phi, grad Phi, over one workgroup per grid line, so that the grid's
+// global reads and writes spread over as many CUs as it has lines:
+//   k_sg_rows_fwd  the energy density of a row and its forward FFT;
+//   k_sg_cols      a column's forward FFT, the product with G^, its inverse FFT;
+//   k_sg_rows_inv  a row's inverse FFT: Phi = Re;
+//   k_sg_grad      central-difference gradient (outside = 0).
+// The 2-D transform is separable, so running the inverse's columns before its
+// rows changes only the FP32 rounding order.  A grid without energy
+// (no cell >= 1e-12) gives Phi = 0 and has_energy = 0, as the reference.
+// ---------------------------------------------------------------------------
+struct LineLds {
+    float re[kFftMax], im[kFftMax];
+    float tw_re[kFftMax / 2], tw_im[kFftMax / 2];
+};
+
+// blockDim.x == N: thread t holds element t of the line.
+__device__ void line_twiddles(LineLds& s, int N) {
+    const int t = threadIdx.x;
+    if (t < N / 2) {
+        double sn, cs;
+        sincospi(-2.0 * (double)t / (double)N, &sn, &cs);
+        s.tw_re[t] = (float)cs;
+        s.tw_im[t] = (float)sn;
+    }
+}
+
+// In-place radix-2 DIT of the line in LDS, with fft_pass's butterflies.  The
+// caller has synchronised the line's writes.
+__device__ void line_fft(LineLds& s, int N, int logN, bool inverse) {
+    const int t = threadIdx.x;
+    {
+        const int j = (int)(__brev((unsigned)t) >> (32 - logN));
+        const float vr = s.re[j], vi = s.im[j];
+        __syncthreads();
+        s.re[t] = vr;
+        s.im[t] = vi;
+        __syncthreads();
+    }
+    for (int lg = 1; lg <= logN; lg++) {
+        const int len = 1 << lg, half = len >> 1, stride = N >> lg;
+        if (t < N / 2) {
+            const int grp = t >> (lg - 1), jj = t & (half - 1);
+            const int ia = grp * len + jj, ib = ia + half;
+            const float wr = s.tw_re[jj * stride];
+            const float wi = inverse ? -s.tw_im[jj * stride] : s.tw_im[jj * stride];
+            const float br = s.re[ib], bi = s.im[ib];
+            const float tr = wr * br - wi * bi, ti = wr * bi + wi * br;
+            const float ar = s.re[ia], ai = s.im[ia];
+            s.re[ib] = ar - tr; s.im[ib] = ai - ti;
+            s.re[ia] = ar + tr; s.im[ia] = ai + ti;
+        }
+        __syncthreads();
+    }
+    if (inverse) {
+        const float inv = 1.f / (float)N;          // fft.js:63-69 normalizes each 1-D inverse
+        s.re[t] *= inv;
+        s.im[t] *= inv;
+        __syncthreads();
+    }
+}
+
+// Whether any row of the grid holds energy: every block reduces the row flags
+// (blockDim.x >= N).
+__device__ __forceinline__ bool sg_has_energy(const FieldArgs& f) {
+    return __syncthreads_or(threadIdx.x < (unsigned)f.G ? f.sg_rows[threadIdx.x] : 0) != 0;
+}
+
+__global__ __launch_bounds__(kFftMax) void k_sg_rows_fwd(FieldArgs f) {
+    __shared__ LineLds s;
+    const int N = f.G, logN = 31 - __clz(N);
+    const int y = blockIdx.x, x = threadIdx.x, e = y * N + x;
+    line_twiddles(s, N);
     // energy density (scalar-field.js:486-516) times the cell area
     const float icw2 = f.icw * f.icw, ich2 = f.ich * f.ich, area = f.cw * f.ch;
-    for (int e = t; e < N * N; e += kFftThreads) {
-        const float fd = f.field_dot[e], gx = f.grad_x[e], gy = f.grad_y[e], v = f.field[e];
-        float pot;
-        if (f.higgs) {
-            const float v2 = v * v;
-            pot = f.mu2 * (-0.5f * v2 + 0.25f * v2 * v2) + 0.25f * f.mu2;
-        } else {
-            pot = 0.5f * f.m2 * v * v;
-        }
-        const float rho = 0.5f * fd * fd + 0.5f * (gx * gx * icw2 + gy * gy * ich2) + pot;
-        if (rho >= 1e-12f) s.has_energy = 1;
-        s.re[(e >> logN) * kPitch + (e & (N - 1))] = rho * area;
-        s.im[(e >> logN) * kPitch + (e & (N - 1))] = 0.f;
+    const float fd = f.field_dot[e], gx = f.grad_x[e], gy = f.grad_y[e], v = f.field[e];
+    float pot;
+    if (f.higgs) {
+        const float v2 = v * v;
+        pot = f.mu2 * (-0.5f * v2 + 0.25f * v2 * v2) + 0.25f * f.mu2;
+    } else {
+        pot = 0.5f * f.m2 * v * v;
     }
+    const float rho = 0.5f * fd * fd + 0.5f * (gx * gx * icw2 + gy * gy * ich2) + pot;
+    s.re[x] = rho * area;
+    s.im[x] = 0.f;
+    const int any = __syncthreads_or(rho >= 1e-12f);
+    if (x == 0) f.sg_rows[y] = any;
+    line_fft(s, N, logN, false);
+    f.sg_c[e] = make_float2(s.re[x], s.im[x]);
+}
+
+__global__ __launch_bounds__(kFftMax) void k_sg_cols(FieldArgs f) {
+    __shared__ LineLds s;
+    const int N = f.G, logN = 31 - __clz(N);
+    const int x = blockIdx.x, y = threadIdx.x, e = y * N + x;
+    if (!sg_has_energy(f)) return;                 // uniform over the block
+    line_twiddles(s, N);
+    const float2 v = f.sg_c[e];
+    s.re[y] = v.x;
+    s.im[y] = v.y;
     __syncthreads();
-    if (!s.has_energy) {
-        for (int e = t; e < N * N; e += kFftThreads) { f.sg_phi[e] = 0.f; f.sg_gx[e] = 0.f; f.sg_gy[e] = 0.f; }
-        if (t == 0) *f.has_energy = 0;
+    line_fft(s, N, logN, false);
+    const float2 g = f.green[e];
+    const float ar = s.re[y], ai = s.im[y];
+    s.re[y] = ar * g.x - ai * g.y;
+    s.im[y] = ar * g.y + ai * g.x;
+    __syncthreads();
+    line_fft(s, N, logN, true);
+    f.sg_c[e] = make_float2(s.re[y], s.im[y]);
+}
+
+__global__ __launch_bounds__(kFftMax) void k_sg_rows_inv(FieldArgs f) {
+    __shared__ LineLds s;
+    const int N = f.G, logN = 31 - __clz(N);
+    const int y = blockIdx.x, x = threadIdx.x, e = y * N + x;
+    if (!sg_has_energy(f)) return;                 // uniform over the block
+    line_twiddles(s, N);
+    const float2 v = f.sg_c[e];
+    s.re[x] = v.x;
+    s.im[x] = v.y;
+    __syncthreads();
+    line_fft(s, N, logN, true);
+    f.sg_phi[e] = s.re[x];
+}
+
+__global__ __launch_bounds__(256) void k_sg_grad(FieldArgs f) {
+    const int N = f.G, logN = 31 - __clz(N);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool has = sg_has_energy(f);             // before any thread leaves: it synchronises the block
+    if (e >= N * N) return;
+    if (!has) {
+        f.sg_phi[e] = 0.f;
+        f.sg_gx[e] = 0.f;
+        f.sg_gy[e] = 0.f;
+        if (e == 0) *f.has_energy = 0;
         return;
     }
-    fft2d_lds(s, N, logN, false);
-    for (int e = t; e < N * N; e += kFftThreads) {
-        const int a = (e >> logN) * kPitch + (e & (N - 1));
-        const float2 g = f.green[e];
-        const float ar = s.re[a], ai = s.im[a];
-        s.re[a] = ar * g.x - ai * g.y;
-        s.im[a] = ar * g.y + ai * g.x;
-    }
-    __syncthreads();
-    fft2d_lds(s, N, logN, true);
     // Phi = Re, central-difference gradient (outside = 0) (scalar-field.js:567-596)
-    for (int e = t; e < N * N; e += kFftThreads) {
-        const int iy = e >> logN, ix = e & (N - 1);
-        const int iL = nb_index(N, ix - 1, iy, f.bc, f.topo), iR = nb_index(N, ix + 1, iy, f.bc, f.topo);
-        const int iT = nb_index(N, ix, iy - 1, f.bc, f.topo), iB = nb_index(N, ix, iy + 1, f.bc, f.topo);
-        auto P = [&](int idx) { return idx >= 0 ? s.re[(idx >> logN) * kPitch + (idx & (N - 1))] : 0.f; };
-        f.sg_gx[e] = (P(iR) - P(iL)) * 0.5f;
-        f.sg_gy[e] = (P(iB) - P(iT)) * 0.5f;
-        f.sg_phi[e] = s.re[iy * kPitch + ix];
-    }
-    if (t == 0) *f.has_energy = 1;
+    const int iy = e >> logN, ix = e & (N - 1);
+    const int iL = nb_index(N, ix - 1, iy, f.bc, f.topo), iR = nb_index(N, ix + 1, iy, f.bc, f.topo);
+    const int iT = nb_index(N, ix, iy - 1, f.bc, f.topo), iB = nb_index(N, ix, iy + 1, f.bc, f.topo);
+    auto P = [&](int idx) { return idx >= 0 ? f.sg_phi[idx] : 0.f; };
+    f.sg_gx[e] = (P(iR) - P(iL)) * 0.5f;
+    f.sg_gy[e] = (P(iB) - P(iT)) * 0.5f;
+    if (e == 0) *f.has_energy = 1;
 }
 
 // Half-kick (higgs-field.js:148-218, axion-field.js:129-190): fd_out = fd_in + ddphi * halfDt.
@@ -649,7 +754,11 @@ void field_build_green(hipStream_t s, const FieldArgs& f, float soft2) {
 }
 
 void field_selfgrav(hipStream_t s, const FieldArgs& f) {
-    hipLaunchKernelGGL(k_field_selfgrav, dim3(1), dim3(kFftThreads), 0, s, f, 0, 0.f);
+    const int N = f.G;                          // a power of two in [8, kFftMax]
+    hipLaunchKernelGGL(k_sg_rows_fwd, dim3(N), dim3(N), 0, s, f);
+    hipLaunchKernelGGL(k_sg_cols, dim3(N), dim3(N), 0, s, f);
+    hipLaunchKernelGGL(k_sg_rows_inv, dim3(N), dim3(N), 0, s, f);
+    hipLaunchKernelGGL(k_sg_grad, dim3((N * N + 255) / 256), dim3(256), 0, s, f);
 }
 
 void field_fft(hipStream_t s, const FieldArgs& f, int inverse) {
