@@ -131,7 +131,11 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
     const double now = DELAY ? a.ctrl->sim_time : 0.0;
     int64_t c_pair = 0, c_node = 0, c_visit = 0;
     int far_n = 0;                                 // entries of this lane's far queue (FAR == kFarList)
-    const int far_lane_cap = (int)(a.far_cap / kWave);
+    const int far_lane_cap = (int)(a.far_cap / kWave);   // a multiple of 4
+    // Each lane's last queue entries wait in LDS until they fill one 16-byte
+    // chunk of far_slot's layout, written with one store: the lists' HBM
+    // writes fall to about their size, without registers held for them.
+    __shared__ uint4 s_far[(FAR == kFarList) ? kWavesPerBlock * kWave : 1];
     int sp = 0;
     if (a.bsys == nullptr) {
         if (lane == 0) { s_node[wv][0] = 0; s_mask[wv][0] = active; }
@@ -208,7 +212,11 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
             }
             if (FAR == kFarList && accept) {
                 // each accepting lane appends the node to its own queue
-                if (far_n < far_lane_cap) a.far_list[far_slot(group, a.far_cap, lane, far_n)] = (uint32_t)node;
+                uint32_t* q = reinterpret_cast<uint32_t*>(&s_far[wv * kWave + lane]);
+                q[far_n & 3] = (uint32_t)node;
+                if ((far_n & 3) == 3 && far_n < far_lane_cap)
+                    *reinterpret_cast<uint4*>(a.far_list + far_slot(group, a.far_cap, lane, far_n - 3)) =
+                        s_far[wv * kWave + lane];
                 far_n++;
             }
             const uint64_t open = __ballot(mine && !accept);
@@ -232,6 +240,9 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
     if constexpr (SEL) return;                     // the force pass has its counters and outputs
     if (overflow && lane == 0) a.ctrl->overflow = 2;
     if (FAR == kFarList) {
+        if ((far_n & 3) != 0 && far_n < far_lane_cap)   // the partial last chunk; entries past far_n are not read
+            *reinterpret_cast<uint4*>(a.far_list + far_slot(group, a.far_cap, lane, far_n & ~3)) =
+                s_far[wv * kWave + lane];
         a.far_count[group * kWave + lane] = far_n < far_lane_cap ? far_n : far_lane_cap;
         if (__ballot(far_n > far_lane_cap) != 0 && lane == 0) a.ctrl->overflow = 3;
     }
