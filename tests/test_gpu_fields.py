@@ -220,3 +220,27 @@ def test_axion_modulates_coulomb_without_magnetic(oracle_lib):
     _, _, px, py = pair(dict(prm, toggles0=prm["toggles0"] & ~A.AXION))
     effect = float(np.sqrt(np.sum((px - ox) ** 2 + (py - oy) ** 2) / np.sum(ox ** 2 + oy ** 2)))
     assert effect > 100 * err, (effect, err)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_selfgravity_of_vacuum_is_zero(which, oracle_lib):
+    """A field with energy, then the vacuum (Higgs 1, axion 0, at rest): the
+    second solve finds no cell holding energy and writes Phi = grad Phi = 0
+    over the first one's values (scalar-field.js:600-606)."""
+    G = 64
+    prm = dict(toggles0=A.GRAVITY | A.HIGGS | A.AXION, field_grid=G, domain_w=256.0, domain_h=256.0, softening=8.0,
+               boundary_mode=A.BOUND_LOOP, axion_mass=0.2)
+    r = np.random.default_rng(7 + which)
+    vac = 1.0 if which == 0 else 0.0
+    g = _gpu(prm)
+    g.set_field(which, W.f32(vac + W.gaussian_field(G, r, 0.05)), W.f32(0.02 * r.normal(size=(G, G))))
+    g.field_selfgravity(which)
+    assert np.abs(g.get_field(which, "sg_phi")).max() > 0
+    g.set_field(which, np.full((G, G), vac), np.zeros((G, G)))
+    g.field_selfgravity(which)
+    for what in ("sg_phi", "sg_gx", "sg_gy"):
+        assert np.all(g.get_field(which, what) == 0.0), what
+    o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1)
+    o.field_set(which, np.full((G, G), vac), np.zeros((G, G)))
+    o.field_selfgrav(which)
+    assert np.all(o.field_get(which, "sg_phi") == 0.0)
