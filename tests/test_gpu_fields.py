@@ -32,9 +32,9 @@ def test_lds_fft_matches_numpy(n):
     assert _rel(c + 1j * d, re + 1j * im) < 2e-6
 
 
+@pytest.mark.parametrize("G", [8, 32, 64, 128])
 @pytest.mark.parametrize("bc", [A.BOUND_LOOP, A.BOUND_DESPAWN])
-def test_selfgravity_matches_oracle(bc, oracle_lib):
-    G = 64
+def test_selfgravity_matches_oracle(bc, G, oracle_lib):
     prm = dict(toggles0=A.GRAVITY | A.AXION, field_grid=G, domain_w=256.0, domain_h=256.0, softening=8.0,
                boundary_mode=bc, axion_mass=0.2)
     r = np.random.default_rng(3)
@@ -46,8 +46,15 @@ def test_selfgravity_matches_oracle(bc, oracle_lib):
     o = oracle_lib.Oracle(oracle_lib.default_params(**prm), 1)
     o.field_set(1, a, ad)
     o.field_selfgrav(1)
-    for what in ("sg_phi", "sg_gx", "sg_gy"):
-        assert _rel(g.get_field(1, what), o.field_get(1, what)) < 2e-5, what
+    assert _rel(g.get_field(1, "sg_phi"), o.field_get(1, "sg_phi")) < 2e-5
+    # On the torus the potential is nearly flat across neighbouring cells, so
+    # its central differences lose about three digits to the FP32 rounding of
+    # Phi (which matches to about 1e-7): at 128^2 the gradients agree to about
+    # 1e-4 (7e-5 to 1e-4, with the grid in one workgroup's LDS and with the
+    # line-parallel transform alike).
+    tol = 3e-4 if bc == A.BOUND_LOOP and G >= 128 else 2e-5
+    for what in ("sg_gx", "sg_gy"):
+        assert _rel(g.get_field(1, what), o.field_get(1, what)) < tol, what
 
 
 def _config3_pair(oracle_lib, n=4096, G=64, toggles=None):
