@@ -103,7 +103,9 @@ __device__ __forceinline__ void wave_add_counters(Ctrl* ctrl, int64_t pairs, int
 // the selected particle (*a.sel) walks, summing its forces per type into *split
 // and storing nothing else.
 template <uint32_t F, bool KAHAN, int FAR, int STACK = kWalkStack, bool DELAY = false, bool SEL = false>
-__device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Split* split = nullptr) {
+__device__ __forceinline__ void walk_core(const ForceArgs& a, int64_t group, Split* split, const Node* nodes,
+                                          uint32_t* far_list, const float4* src4, const float4* src8a,
+                                          const float4* src8b) {
     __shared__ int32_t s_node[kWavesPerBlock][STACK];
     __shared__ uint64_t s_mask[kWavesPerBlock][STACK];
     const int n = a.ctrl->n_alive;
@@ -159,17 +161,17 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
         const uint64_t mask = s_mask[wv][sp];
         const bool mine = (mask & my_bit) != 0;
         if (node < 0 || node >= a.max_nodes) { overflow = true; continue; }
-        const Node nd = a.nodes[node];
+        const Node nd = nodes[node];
         c_visit += mine;
         if (nd.mass < kEpsMass) continue;          // forces.js:630
         if (nd.count > 0) {
             // leaf: every point except the particle itself (forces.js:643-676)
             const int first = nd.child, cnt = nd.count;
             for (int j = first; j < first + cnt; j++) {
-                const float4 s4 = a.src4[j];
-                const float4 s8a = a.src8a[j];
+                const float4 s4 = src4[j];
+                const float4 s8a = src8a[j];
                 float4 s8b = make_float4(0.f, 1.f, 1.f, 1.f);
-                if (F & F_EXTRA) s8b = a.src8b[j];
+                if (F & F_EXTRA) s8b = src8b[j];
                 if (DELAY) {
                     const int sj = a.hist.slot[j];
                     Retarded rt;
@@ -215,7 +217,7 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
                 uint32_t* q = reinterpret_cast<uint32_t*>(&s_far[wv * kWave + lane]);
                 q[far_n & 3] = (uint32_t)node;
                 if ((far_n & 3) == 3 && far_n < far_lane_cap)
-                    *reinterpret_cast<uint4*>(a.far_list + far_slot(group, a.far_cap, lane, far_n - 3)) =
+                    *reinterpret_cast<uint4*>(far_list + far_slot(group, a.far_cap, lane, far_n - 3)) =
                         s_far[wv * kWave + lane];
                 far_n++;
             }
@@ -241,13 +243,36 @@ __device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Spl
     if (overflow && lane == 0) a.ctrl->overflow = 2;
     if (FAR == kFarList) {
         if ((far_n & 3) != 0 && far_n < far_lane_cap)   // the partial last chunk; entries past far_n are not read
-            *reinterpret_cast<uint4*>(a.far_list + far_slot(group, a.far_cap, lane, far_n & ~3)) =
+            *reinterpret_cast<uint4*>(far_list + far_slot(group, a.far_cap, lane, far_n & ~3)) =
                 s_far[wv * kWave + lane];
         a.far_count[group * kWave + lane] = far_n < far_lane_cap ? far_n : far_lane_cap;
         if (__ballot(far_n > far_lane_cap) != 0 && lane == 0) a.ctrl->overflow = 3;
     }
     wave_add_counters(a.ctrl, c_pair, c_node, c_visit);
     if (valid) store_acc<F, KAHAN>(a, i, o, acc);
+}
+
+// The listing walk takes the tree and source pointers as restrict parameters:
+// its far-queue stores then provably miss the node and source records, so
+// those loads of a wave-uniform address stay scalar loads as in the other
+// walks.  With a possibly aliasing global store in the loop the compiler turns
+// them into vector loads (1.3e8 more vector memory instructions per pass at
+// N = 2^20, 9% of the walk's time).  The other walks store nothing in the loop
+// and keep plain pointers (restrict made the plain walk 1.5% slower).
+template <uint32_t F, bool KAHAN, int STACK, bool DELAY, bool SEL>
+__device__ __forceinline__ void walk_list_core(const ForceArgs& a, int64_t group, Split* split,
+                                               const Node* __restrict__ nodes, uint32_t* __restrict__ far_list,
+                                               const float4* __restrict__ src4, const float4* __restrict__ src8a,
+                                               const float4* __restrict__ src8b) {
+    walk_core<F, KAHAN, kFarList, STACK, DELAY, SEL>(a, group, split, nodes, far_list, src4, src8a, src8b);
+}
+
+template <uint32_t F, bool KAHAN, int FAR, int STACK = kWalkStack, bool DELAY = false, bool SEL = false>
+__device__ __forceinline__ void walk_body(const ForceArgs& a, int64_t group, Split* split = nullptr) {
+    if constexpr (FAR == kFarList)
+        walk_list_core<F, KAHAN, STACK, DELAY, SEL>(a, group, split, a.nodes, a.far_list, a.src4, a.src8a, a.src8b);
+    else
+        walk_core<F, KAHAN, FAR, STACK, DELAY, SEL>(a, group, split, a.nodes, a.far_list, a.src4, a.src8a, a.src8b);
 }
 
 __device__ __forceinline__ int64_t walk_group() { return (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; }
