@@ -299,25 +299,28 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk_delay(ForceArgs a) {
 // Persistent walk: every wave pulls 64-particle groups from a queue until none
 // are left, so uneven work (systems of a batch, dense regions) balances itself.
 // Node-stack entries per wave: 64 or 96 (LDS per block = 4 waves x 12 B x entries).
-template <uint32_t F, bool KAHAN>
-__global__ __launch_bounds__(kWalkThreads) void k_walk_persistent64(ForceArgs a, int* queue, int64_t n_groups) {
+// The tree and source pointers are restrict parameters, as in the listing walk:
+// the result stores and queue atomics between groups then provably miss them,
+// and the leaf-source loads of a wave-uniform address stay scalar loads.
+template <uint32_t F, bool KAHAN, int STACK>
+__device__ __forceinline__ void persistent_walks(const ForceArgs& a, int* queue, int64_t n_groups,
+                                                 const Node* __restrict__ nodes, const float4* __restrict__ src4,
+                                                 const float4* __restrict__ src8a, const float4* __restrict__ src8b) {
     for (;;) {
         int g = 0;
         if ((threadIdx.x & (kWave - 1)) == 0) g = atomicAdd(queue, 1);
         g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
         if (g >= n_groups) break;
-        walk_body<F, KAHAN, 0, 64>(a, g);
+        walk_core<F, KAHAN, 0, STACK>(a, g, nullptr, nodes, a.far_list, src4, src8a, src8b);
     }
 }
 template <uint32_t F, bool KAHAN>
+__global__ __launch_bounds__(kWalkThreads) void k_walk_persistent64(ForceArgs a, int* queue, int64_t n_groups) {
+    persistent_walks<F, KAHAN, 64>(a, queue, n_groups, a.nodes, a.src4, a.src8a, a.src8b);
+}
+template <uint32_t F, bool KAHAN>
 __global__ __launch_bounds__(kWalkThreads) void k_walk_persistent96(ForceArgs a, int* queue, int64_t n_groups) {
-    for (;;) {
-        int g = 0;
-        if ((threadIdx.x & (kWave - 1)) == 0) g = atomicAdd(queue, 1);
-        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
-        if (g >= n_groups) break;
-        walk_body<F, KAHAN, 0, 96>(a, g);
-    }
+    persistent_walks<F, KAHAN, 96>(a, queue, n_groups, a.nodes, a.src4, a.src8a, a.src8b);
 }
 
 // 1PN recompute for the velocity-Verlet correction: the same wavefront walk
