@@ -482,9 +482,13 @@ void launch_walk1pn(hipStream_t s, const ForceArgs& a, bool tree) {
 // lanes whose box still intersects the node.  On a torus the box is also
 // queried at its wrapped images (the reference finds those neighbours through
 // ghost particles, integrator.js:201-258).  WRITE = 0 counts, 1 writes at the
-// lane's offset.
+// lane's offset.  The tree, source and body arrays come in as restrict
+// parameters, so the pair and flag stores in the loop provably miss them and
+// the loads of a wave-uniform address stay scalar loads.
 template <int WRITE>
-__global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
+__device__ __forceinline__ void pairs_query(const PairArgs& a, const Node* __restrict__ nodes,
+                                            const float4* __restrict__ src4, const int32_t* __restrict__ slot,
+                                            const float* __restrict__ radius) {
     __shared__ int32_t s_node[kWavesPerBlock][kWalkStack];
     __shared__ uint64_t s_mask[kWavesPerBlock][kWalkStack];
     const int n = a.ctrl->n_alive;
@@ -499,8 +503,8 @@ __global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
     }
     const uint64_t my_bit = 1ull << lane;
     const float px = valid ? a.x[i] : 0.f, py = valid ? a.y[i] : 0.f;
-    const float r1 = valid ? a.radius[i] * a.slack : 0.f;
-    const int slot1 = valid ? a.slot[i] : 0;
+    const float r1 = valid ? radius[i] * a.slack : 0.f;
+    const int slot1 = valid ? slot[i] : 0;
     const float rw = 2.f * r1;
     int found = 0;
     int out = WRITE && valid ? a.offset[i] : 0;
@@ -527,21 +531,21 @@ __global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
             const int node = __builtin_amdgcn_readfirstlane(s_node[wv][sp]);
             const uint64_t mask = s_mask[wv][sp];
             if (node < 0 || node >= a.max_nodes) continue;
-            const Node nd = a.nodes[node];
+            const Node nd = nodes[node];
             const float bw = nd.half, bh = nd.half * (a.dom_h / a.dom_w);
             const bool mine = (mask & my_bit) != 0 &&
                               !(qx - rw > nd.bx + bw || qx + rw < nd.bx - bw || qy - rw > nd.by + bh ||
                                 qy + rw < nd.by - bh);
             if (nd.count >= 0) {
                 for (int j = nd.child; j < nd.child + nd.count; j++) {
-                    const float4 s4 = a.src4[j];
+                    const float4 s4 = src4[j];
                     if (!mine) continue;
                     if (!(s4.x >= qx - rw && s4.x <= qx + rw && s4.y >= qy - rw && s4.y <= qy + rw)) continue;
-                    const int slot2 = a.slot[j];
+                    const int slot2 = slot[j];
                     if (j == i || s4.z == 0.f || slot1 >= slot2) continue;
                     float dx = s4.x - px, dy = s4.y - py;
                     if (a.periodic) { dx = torus_wrap(dx, a.dom_w, 0.5f * a.dom_w); dy = torus_wrap(dy, a.dom_h, 0.5f * a.dom_h); }
-                    const float md = r1 + a.radius[j] * a.slack;
+                    const float md = r1 + radius[j] * a.slack;
                     if (dx * dx + dy * dy < md * md) {
                         if (WRITE) {
                             if (out < a.cap_pairs) {
@@ -574,6 +578,11 @@ __global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
         }
     }
     if (!WRITE && i < n) a.count[i] = valid ? found : 0;
+}
+
+template <int WRITE>
+__global__ __launch_bounds__(kWalkThreads) void k_pairs(PairArgs a) {
+    pairs_query<WRITE>(a, a.nodes, a.src4, a.slot, a.radius);
 }
 
 void launch_pairs(hipStream_t s, const PairArgs& a, bool write) {
